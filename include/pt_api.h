@@ -1,0 +1,339 @@
+/*
+ * pt_api.h -- C ABI of the MI355X (gfx950) PaintTween synthesis hot path.
+ *
+ * The entry points mirror the reference's TexSynth namespace
+ * (src/PaintTweenCUDA/texSynth_kernel.h:38-93) one to one, with three
+ * differences forced by a plain C boundary:
+ *   - an explicit context handle replaces the process-global texture
+ *     references and __constant__ parameters (texSynth_kernel.cu:64-96);
+ *   - host-side ImagePyramid<T> objects (imagePyramid.h) become device-resident
+ *     pt_pyramid handles that are built on the GPU from level 0;
+ *   - errors are returned as negative status codes with pt_last_error()
+ *     instead of checkCUDAError() -> exit(1) (cudaHostUtil.cpp:46-66).
+ * All calls on one context are synchronous with respect to the host, as every
+ * reference launcher ends in cudaThreadSynchronize().  A context is used by
+ * one host thread at a time; distinct contexts are independent.
+ *
+ * Structs are fixed-width PODs (no bool or enum members) so the layout is the
+ * same from C, C++, ctypes and cgo.
+ */
+#ifndef PT_API_H
+#define PT_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PT_OK 0
+#define PT_ERR_INVALID_ARG (-1)
+#define PT_ERR_UNSUPPORTED (-2)
+#define PT_ERR_HIP (-3)
+#define PT_ERR_OUT_OF_MEMORY (-4)
+#define PT_ERR_STATE (-5)
+#define PT_ERR_BUDGET (-6)
+
+/* ---- PassDirection (types.h:128-132) ---------------------------------- */
+#define PT_FORWARD 0
+#define PT_BACKWARD 1
+#define PT_BIDIRECTIONAL 2
+
+/* ---- SynthesisScheme (types.h:134-140) -------------------------------- */
+#define PT_S_FMBM 0
+#define PT_S_FMBM_RM 1
+
+/* ---- TsImageLayer (texSynth_interface.h:52-75), same values ----------- */
+#define PT_LAYER_EXEMPLAR_BASE 0
+#define PT_LAYER_EXEMPLAR_OUTPUT 1
+#define PT_LAYER_EXEMPLAR_ORIENTATION 2
+#define PT_LAYER_EXEMPLAR_DIST_TRANS 3
+#define PT_LAYER_EXEMPLAR_OBJECT_IDS 4
+#define PT_LAYER_INPUT_COLOR 5
+#define PT_LAYER_INPUT_ORIENTATION 6
+#define PT_LAYER_INPUT_DIST_TRANS 7
+#define PT_LAYER_INPUT_SCALE 8
+#define PT_LAYER_INPUT_ID 9
+#define PT_LAYER_INPUT_SURF_ID 10
+#define PT_LAYER_INPUT_VEL_B 11
+#define PT_LAYER_INPUT_VEL_F 12
+#define PT_LAYER_INPUT_VEL_B_PREVIOUS 13
+#define PT_LAYER_INPUT_VEL_F_PREVIOUS 14
+#define PT_LAYER_RANDOM 21
+
+/* ---- operation flags (texSynth_kernel.h:79-91), same bits ------------- */
+#define PT_DONT_COPY_OFFSETS 0x01
+#define PT_DONT_UPDATE_RESIDUAL 0x10
+#define PT_ADVECT_IGNORE_RANDOM 0x1
+#define PT_ADVECT_DO_NOT_CHECK_PREVIOUS 0x10
+#define PT_ADVECT_UPDATE_RESIDUAL 0x80
+
+/* ---- scatter random-number modes (no counterpart in the reference) ----
+ * PT_RNG_REFERENCE reproduces the reference scatter exactly: its randFloat()
+ * returns an unsigned int that truncates to 0 (texSynth_util.h:155-163), so
+ * every sample is the shift (+0.9*dist, 0).  PT_RNG_COUNTER draws r and theta
+ * from Philox4x32-10 keyed by (seed, frame*level), counter (x, y, call, i). */
+#define PT_RNG_REFERENCE 0
+#define PT_RNG_COUNTER 1
+
+/* ---- pyramid element kinds and downsample modes ----------------------- */
+#define PT_ELEM_F32 1   /* float          */
+#define PT_ELEM_F32X2 2 /* float2         */
+#define PT_ELEM_F32X3 3 /* float3 tensor  */
+#define PT_ELEM_F32X4 4 /* Color4 / float4 */
+/* DownsampleScaleMode (downsample.h:35-38), same values. */
+#define PT_DOWNSAMPLE_SCALE_WITH_LEVELS 0
+#define PT_DOWNSAMPLE_SCALE_CONSTANT 1
+
+/*
+ * pt_params -- TsParameters (texSynth_interface.h:79-152) as a fixed-width
+ * POD.  Field order and meaning follow the reference; bool fields are int32
+ * (0/1).  The last three fields are extensions with no reference
+ * counterpart.  Defaults (G/nlParameters.cpp:63-101) come from
+ * pt_params_default().
+ */
+typedef struct pt_params {
+    int32_t residual_window_size;      /* r: patch is (2r+1)^2 taps          */
+    float coherence_weight;
+    float dist_trans_weight;
+    float lum_shift_weight;
+    int32_t lum_shift_enable;
+    float input_analogy_weight;
+    float canvas_output_weight;
+    int32_t num_styles;
+    float offsets_histogram_slope[5];
+    float offsets_histogram_threshold[5];
+    float max_distortion;
+    float alpha_weight;
+    int32_t transparency_ok;
+    float style_obj_id_weight;
+    int32_t orientation_enabled;
+    int32_t colorspace;
+    float coherence_ratio_range;
+    float coherence_angle_range;
+    int32_t levels;
+    int32_t scatter_samples_count;
+    float advection_jitter;
+    float time_derivative_input_weight;
+    float time_derivative_output_weight;
+    float temporal_coherence_weight;
+    float hysteresis_weight;
+    int32_t interpolate_key_frame;
+    int32_t use_guide;
+    int32_t current_interpolation_index;
+    int32_t num_frames;
+    int32_t num_key_frames;
+    int32_t first_key_index;
+    int32_t last_key_index;
+    int32_t dist_trans_cull;
+    float residual_window_scale_u;
+    float residual_window_scale_v;
+    int32_t direction;                 /* PT_FORWARD / PT_BACKWARD / ...     */
+    /* extensions */
+    int32_t rng_mode;                  /* PT_RNG_REFERENCE / PT_RNG_COUNTER  */
+    uint32_t seed;                     /* Philox key word 0                  */
+    int32_t frame_number;              /* frame used to key the RNG          */
+} pt_params;
+
+/*
+ * pt_patchxf -- PatchXF (types.h:98-114), 36 bytes, same field order:
+ * the nearest-neighbour-field element.  Used only at the host boundary;
+ * on the device the field is stored as structure-of-arrays planes.
+ */
+typedef struct pt_patchxf {
+    float x, y, theta, hysteresis;
+    int32_t layer;
+    float scale_u, scale_v;
+    float lum_y, lum_a;
+} pt_patchxf;
+
+typedef struct pt_ctx pt_ctx;
+typedef struct pt_pyramid pt_pyramid;
+
+/* ---- parameters --------------------------------------------------------- */
+
+/* NLParameters::getTsDefaultParams (G/nlParameters.cpp:63-101) plus
+ * distTransCull=50 (synthesisProcessor.cpp:220), direction FORWARD,
+ * rng_mode PT_RNG_REFERENCE, seed 0, frame 0. */
+void pt_params_default(pt_params *out);
+
+/* ---- context -------------------------------------------------------------- */
+
+/* Create a context on HIP device `device`.  No reference counterpart (the
+ * reference uses process globals). */
+int pt_create(pt_ctx **out, int device);
+int pt_destroy(pt_ctx *ctx);
+/* Message of the last failing call on ctx ("" if none). */
+const char *pt_last_error(pt_ctx *ctx);
+/* Message of the last failure that had no context to record it on. */
+const char *pt_global_last_error(void);
+
+/* TexSynth::setup (texSynth_kernel.cu:109-117): Gaussian tap weights
+ * (convolution.cu:37-60) and parameters. */
+int pt_setup(pt_ctx *ctx, const pt_params *params);
+/* TexSynth::uploadParameters (texSynth_kernel.cu:119-123). */
+int pt_upload_parameters(pt_ctx *ctx, const pt_params *params);
+/* WorkingBuffers::initialize (workingBuffers.cpp:48-95) together with
+ * TexSynth::initialize_images (texSynth_kernel.cu:2111-2117): level-0-sized
+ * working buffers, zero-filled, and the two-layer image textures.
+ * style_w/style_h are the level-0 exemplar dimensions (they size the
+ * offsets-histogram buffer whose dimensions drive the patch wrap). */
+int pt_initialize(pt_ctx *ctx, int image_w, int image_h, int style_w, int style_h);
+/* TexSynth::initialize_exemplars (texSynth_kernel.cu:2119-2126): exemplar
+ * textures of one level (w, h are that level's dimensions). */
+int pt_initialize_exemplars(pt_ctx *ctx, int w, int h, int num_layers);
+/* WorkingBuffers::resetAdvectedBuffers (workingBuffers.cpp:113-117). */
+int pt_reset_advected_buffers(pt_ctx *ctx);
+/* TexSynth::flipAnimBuffers (texSynth_kernel.cu:1534-1538). */
+int pt_flip_anim_buffers(pt_ctx *ctx);
+
+/* ---- device image pyramids ------------------------------------------------ */
+
+/* ImagePyramid<T>::initialize (imagePyramid.cpp:81-106) with
+ * downsampleAverage (downsample.cpp:41-160) run on the GPU: level L+1 is the
+ * 2x2 box mean of level L, times 0.5 more for PT_DOWNSAMPLE_SCALE_WITH_LEVELS.
+ * `host_level0` is w*h elements of kind `elem`, tightly packed; NULL fills
+ * level 0 with `default_value` in every channel (the absent-input defaults of
+ * G/nlDataAccess.cpp:96-155).  The pyramid has min(floor(log2 w),
+ * floor(log2 h)) levels. */
+int pt_pyramid_create(pt_ctx *ctx, pt_pyramid **out, int elem, int w, int h,
+                      int scale_mode, const void *host_level0, float default_value);
+/* NLDataAccess::toOrientationPyramid (G/nlDataAccess.cpp:82-93): converts a
+ * PT_ELEM_F32X3 structure-tensor pyramid into a PT_ELEM_F32 angle pyramid
+ * (strucTensorToAngle, imageIO.h:136-139) on the GPU. */
+int pt_pyramid_orientation(pt_ctx *ctx, pt_pyramid **out, const pt_pyramid *tensor);
+/* A pyramid belongs to the context that built it: destroy it before that
+ * context. */
+int pt_pyramid_destroy(pt_pyramid *pyr);
+int pt_pyramid_levels(const pt_pyramid *pyr);
+int pt_pyramid_dims(const pt_pyramid *pyr, int level, int *w, int *h);
+/* Copies one level to the host (w*h elements, tightly packed). */
+int pt_pyramid_download(const pt_pyramid *pyr, int level, void *host);
+
+/* TexSynth::uploadImage_{float,float2,Color4} (texSynth_kernel.cu:2128-2253):
+ * binds `level` of `pyr` to image layer `layer`, device to device.  The
+ * element kind must match the layer (color/exemplar/random: F32X4, velocity
+ * and scale: F32X2, orientation and distance transform: F32). */
+int pt_upload_image(pt_ctx *ctx, int level, int layer, const pt_pyramid *pyr, int style_layer);
+
+/* SynthesisFrame::createRandomPyramid (synthesisFrame.cpp:66-150) on the GPU
+ * for one level, then uploaded as TS_LAYER_RANDOM.  glibc rand() seeded with
+ * srand(frame*level) is replaced by Philox4x32-10 keyed (seed, frame*level)
+ * with counter (x, y, draw, 0).  Needs the level's image color and
+ * orientation, and the exemplar output and orientation pyramids. */
+int pt_create_random_pyramid(pt_ctx *ctx, int level, int frame_number,
+                             const pt_pyramid *image_color, const pt_pyramid *image_orientation,
+                             const pt_pyramid *exemplar_output, const pt_pyramid *exemplar_orientation);
+
+/* ---- NNF operators (TexSynth::*) --------------------------------------- */
+
+/* TexSynth::randomizeOffsets (texSynth_kernel.cu:2100-2109). */
+int pt_randomize_offsets(pt_ctx *ctx, int level);
+/* TexSynth::propagate (texSynth_kernel.cu:1306-1357); key_index must be -1. */
+int pt_propagate(pt_ctx *ctx, int level, int jump, int key_index);
+/* TexSynth::scatter (texSynth_kernel.cu:1360-1401); key_index must be -1. */
+int pt_scatter(pt_ctx *ctx, int level, int key_index);
+/* TexSynth::upsample (texSynth_kernel.cu:1513-1532). */
+int pt_upsample(pt_ctx *ctx, int fine_level);
+/* TexSynth::syncBuffers (texSynth_kernel.cu:1240-1303). */
+int pt_sync_buffers(pt_ctx *ctx, int level, int flags);
+/* TexSynth::updateResidualCache (texSynth_kernel.cu:1221-1238). */
+int pt_update_residual_cache(pt_ctx *ctx, int level);
+/* TexSynth::advectOffsets (texSynth_kernel.cu:1695-1726). */
+int pt_advect_offsets(pt_ctx *ctx, int level, int time_step, int forwards, int flags);
+/* TexSynth::mergeRandomly (texSynth_kernel.cu:2003-2024). */
+int pt_merge_randomly(pt_ctx *ctx, int level, int forwards, float weight_advected, int force_valid);
+/* TexSynth::mergeUsingResidual (texSynth_kernel.cu:1978-2001). */
+int pt_merge_using_residual(pt_ctx *ctx, int level, int forwards);
+
+/* WorkingBuffers::getOutput (workingBuffers.cpp:125-142): level-sized copies
+ * of the base offsets, the canvas (RGBA) and the residual; any pointer may be
+ * NULL. */
+int pt_get_output(pt_ctx *ctx, int level, pt_patchxf *offsets, float *canvas_rgba, float *residual);
+/* WorkingBuffers::loadOffsets / loadResidual (workingBuffers.cpp:166-207):
+ * writes the level rectangle of the base offsets / base residual. */
+int pt_load_offsets(pt_ctx *ctx, int level, const pt_patchxf *offsets);
+int pt_load_residual(pt_ctx *ctx, int level, const float *residual);
+/* Reads an advected buffer set (advectedBuffers.h) for tests: which = 0 for
+ * F, 1 for B.  Any pointer may be NULL. */
+int pt_get_advected(pt_ctx *ctx, int level, int which, pt_patchxf *offsets,
+                    float *base_rgba, float *output_rgba, float *residual, int32_t *time_step);
+
+/* Patch-distance evaluations launched by this context so far (nominal count:
+ * every candidate tested, rejected ones included, plus every residual
+ * refresh). */
+int pt_pde_count(pt_ctx *ctx, uint64_t *out);
+
+/* ---- timing (no reference counterpart; the reference has no GPU timers) --
+ * pt_profile_enable(ctx, 1) clears the log and starts recording a pair of HIP
+ * events on the context's stream around every NNF-path launch;
+ * pt_profile_get() sums the launches of one kernel class ("propagate",
+ * "scatter", "update_residual", "canvas", "upsample", "advect", "random")
+ * since then: launch count, total GPU milliseconds, total pixels covered. */
+int pt_profile_enable(pt_ctx *ctx, int enable);
+int pt_profile_get(pt_ctx *ctx, const char *kernel, uint64_t *launches, double *total_ms,
+                   uint64_t *pixels);
+
+/* ---- one-shot schedules ------------------------------------------------- */
+
+/*
+ * pt_frame_guides -- one frame's level-0 guides, host memory, tightly packed
+ * row-major.  NULL entries take the reference's absent-input defaults
+ * (G/nlDataAccess.cpp:96-155): orientation tensor 0 (angle -pi/2),
+ * distance transform 1.0, scale (1,1), velocities (0,0).
+ */
+typedef struct pt_frame_guides {
+    const float *color_rgba;        /* required, w*h*4                     */
+    const float *orientation_xx_xy_yy; /* w*h*3 structure tensor or NULL   */
+    const float *dist_trans;        /* w*h or NULL                          */
+    const float *scale_uv;          /* w*h*2 or NULL                        */
+    const float *vel_b;             /* w*h*2 pixels/frame or NULL           */
+    const float *vel_f;             /* w*h*2 pixels/frame or NULL           */
+} pt_frame_guides;
+
+/* The exemplar ("style") pair with its guides.  NULL orientation or distance
+ * transform take the absent-style-element default, a zero image
+ * (G/nlDataAccess.cpp:335-339). */
+typedef struct pt_style_guides {
+    const float *base_rgba;         /* required, w*h*4 exemplar input      */
+    const float *output_rgba;       /* required, w*h*4 exemplar output     */
+    const float *orientation_xx_xy_yy; /* w*h*3 or NULL                     */
+    const float *dist_trans;        /* w*h or NULL                          */
+} pt_style_guides;
+
+/*
+ * Synthesis of a sequence of `num_frames` frames (1 = a still) with the
+ * reference FMBM / FMBM_RM schedule: SynthesisProcessor::renderAnimationFMBM
+ * (synthesisProcessor.cpp:846-878) over renderConsecutivePass (:703-753) and
+ * SynthesisFrame::synthesizeConsecutive (synthesisFrame.cpp:214-323) with
+ * propagateAndScatter (:483-508), at maxLevel 0.  Frames are processed in
+ * the reference order (levels coarse to fine; passes alternate direction).
+ * out_offsets / out_canvas / out_residual (each may be NULL) receive every
+ * frame's level-0 result of the final pass: num_frames*w*h elements.
+ * Each call starts from zeroed working buffers (a fresh WorkingBuffers), so
+ * repeated calls with the same inputs give identical results.
+ */
+int pt_synthesize(pt_ctx *ctx, const pt_params *params, int scheme, int iterations,
+                  int fixup_passes, int image_w, int image_h, int num_frames,
+                  const pt_frame_guides *frames, int style_w, int style_h,
+                  const pt_style_guides *style, pt_patchxf *out_offsets,
+                  float *out_canvas_rgba, float *out_residual);
+
+/*
+ * pt_synthesize with every guide pointer and output pointer in device memory
+ * of the context's GPU (no counterpart in the reference, whose inputs always
+ * arrive from host pyramids).  Used when the frames already live in HBM.
+ */
+int pt_synthesize_device(pt_ctx *ctx, const pt_params *params, int scheme, int iterations,
+                         int fixup_passes, int image_w, int image_h, int num_frames,
+                         const pt_frame_guides *frames, int style_w, int style_h,
+                         const pt_style_guides *style, pt_patchxf *out_offsets,
+                         float *out_canvas_rgba, float *out_residual);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_API_H */

@@ -1,0 +1,194 @@
+"""ctypes mirror of include/pt_api.h.
+
+The structs here must match the C definitions field for field; a CPU test
+checks their sizes against the values the library reports for itself.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpainttween.so")
+
+PT_OK = 0
+PT_FORWARD, PT_BACKWARD, PT_BIDIRECTIONAL = 0, 1, 2
+PT_S_FMBM, PT_S_FMBM_RM = 0, 1
+PT_RNG_REFERENCE, PT_RNG_COUNTER = 0, 1
+PT_DONT_COPY_OFFSETS = 0x01
+PT_DONT_UPDATE_RESIDUAL = 0x10
+PT_ADVECT_IGNORE_RANDOM = 0x1
+PT_ADVECT_DO_NOT_CHECK_PREVIOUS = 0x10
+PT_ADVECT_UPDATE_RESIDUAL = 0x80
+PT_ELEM_F32, PT_ELEM_F32X2, PT_ELEM_F32X3, PT_ELEM_F32X4 = 1, 2, 3, 4
+PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT = 0, 1
+
+LAYER = dict(
+    EXEMPLAR_BASE=0, EXEMPLAR_OUTPUT=1, EXEMPLAR_ORIENTATION=2, EXEMPLAR_DIST_TRANS=3,
+    INPUT_COLOR=5, INPUT_ORIENTATION=6, INPUT_DIST_TRANS=7, INPUT_SCALE=8,
+    INPUT_VEL_B=11, INPUT_VEL_F=12, INPUT_VEL_B_PREVIOUS=13, INPUT_VEL_F_PREVIOUS=14,
+    RANDOM=21,
+)
+
+
+class PtParams(C.Structure):
+    _fields_ = [
+        ("residual_window_size", C.c_int32),
+        ("coherence_weight", C.c_float),
+        ("dist_trans_weight", C.c_float),
+        ("lum_shift_weight", C.c_float),
+        ("lum_shift_enable", C.c_int32),
+        ("input_analogy_weight", C.c_float),
+        ("canvas_output_weight", C.c_float),
+        ("num_styles", C.c_int32),
+        ("offsets_histogram_slope", C.c_float * 5),
+        ("offsets_histogram_threshold", C.c_float * 5),
+        ("max_distortion", C.c_float),
+        ("alpha_weight", C.c_float),
+        ("transparency_ok", C.c_int32),
+        ("style_obj_id_weight", C.c_float),
+        ("orientation_enabled", C.c_int32),
+        ("colorspace", C.c_int32),
+        ("coherence_ratio_range", C.c_float),
+        ("coherence_angle_range", C.c_float),
+        ("levels", C.c_int32),
+        ("scatter_samples_count", C.c_int32),
+        ("advection_jitter", C.c_float),
+        ("time_derivative_input_weight", C.c_float),
+        ("time_derivative_output_weight", C.c_float),
+        ("temporal_coherence_weight", C.c_float),
+        ("hysteresis_weight", C.c_float),
+        ("interpolate_key_frame", C.c_int32),
+        ("use_guide", C.c_int32),
+        ("current_interpolation_index", C.c_int32),
+        ("num_frames", C.c_int32),
+        ("num_key_frames", C.c_int32),
+        ("first_key_index", C.c_int32),
+        ("last_key_index", C.c_int32),
+        ("dist_trans_cull", C.c_int32),
+        ("residual_window_scale_u", C.c_float),
+        ("residual_window_scale_v", C.c_float),
+        ("direction", C.c_int32),
+        ("rng_mode", C.c_int32),
+        ("seed", C.c_uint32),
+        ("frame_number", C.c_int32),
+    ]
+
+
+class PatchXF(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("theta", C.c_float), ("hysteresis", C.c_float),
+                ("layer", C.c_int32), ("scale_u", C.c_float), ("scale_v", C.c_float),
+                ("lum_y", C.c_float), ("lum_a", C.c_float)]
+
+
+PATCHXF_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("theta", "<f4"), ("hysteresis", "<f4"),
+                          ("layer", "<i4"), ("scale_u", "<f4"), ("scale_v", "<f4"),
+                          ("lum_y", "<f4"), ("lum_a", "<f4")])
+assert PATCHXF_DTYPE.itemsize == C.sizeof(PatchXF) == 36
+
+FP = C.POINTER(C.c_float)
+
+
+class FrameGuides(C.Structure):
+    _fields_ = [("color_rgba", FP), ("orientation_xx_xy_yy", FP), ("dist_trans", FP),
+                ("scale_uv", FP), ("vel_b", FP), ("vel_f", FP)]
+
+
+class StyleGuides(C.Structure):
+    _fields_ = [("base_rgba", FP), ("output_rgba", FP), ("orientation_xx_xy_yy", FP), ("dist_trans", FP)]
+
+
+def fptr(a):
+    """float* into a C-contiguous float32 array, or NULL for None."""
+    if a is None:
+        return FP()
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(FP)
+
+
+def vptr(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+VP = C.c_void_p
+I = C.c_int
+F = C.c_float
+
+# (name, restype, argtypes) with the prefix left off; shared by the HIP
+# library ("pt_") and the CPU oracle ("ptref_").
+SIGNATURES = [
+    ("setup", I, [VP, C.POINTER(PtParams)]),
+    ("upload_parameters", I, [VP, C.POINTER(PtParams)]),
+    ("initialize", I, [VP, I, I, I, I]),
+    ("initialize_exemplars", I, [VP, I, I, I]),
+    ("reset_advected_buffers", I, [VP]),
+    ("flip_anim_buffers", I, [VP]),
+    ("pyramid_create", I, [VP, C.POINTER(VP), I, I, I, I, VP, F]),
+    ("pyramid_orientation", I, [VP, C.POINTER(VP), VP]),
+    ("pyramid_destroy", I, [VP]),
+    ("pyramid_levels", I, [VP]),
+    ("pyramid_dims", I, [VP, I, C.POINTER(I), C.POINTER(I)]),
+    ("pyramid_download", I, [VP, I, VP]),
+    ("upload_image", I, [VP, I, I, VP, I]),
+    ("create_random_pyramid", I, [VP, I, I, VP, VP, VP, VP]),
+    ("randomize_offsets", I, [VP, I]),
+    ("propagate", I, [VP, I, I, I]),
+    ("scatter", I, [VP, I, I]),
+    ("upsample", I, [VP, I]),
+    ("sync_buffers", I, [VP, I, I]),
+    ("update_residual_cache", I, [VP, I]),
+    ("advect_offsets", I, [VP, I, I, I, I]),
+    ("merge_randomly", I, [VP, I, I, F, I]),
+    ("merge_using_residual", I, [VP, I, I]),
+    ("get_output", I, [VP, I, VP, VP, VP]),
+    ("load_offsets", I, [VP, I, VP]),
+    ("load_residual", I, [VP, I, VP]),
+    ("get_advected", I, [VP, I, I, VP, VP, VP, VP, VP]),
+    ("pde_count", I, [VP, C.POINTER(C.c_uint64)]),
+    ("synthesize", I, [VP, C.POINTER(PtParams), I, I, I, I, I, I, C.POINTER(FrameGuides), I, I,
+                       C.POINTER(StyleGuides), VP, VP, VP]),
+    ("last_error", C.c_char_p, [VP]),
+    ("destroy", I, [VP]),
+]
+
+
+def bind(lib, prefix):
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+    getattr(lib, prefix + "params_default").restype = None
+    getattr(lib, prefix + "params_default").argtypes = [C.POINTER(PtParams)]
+    return lib
+
+
+_LIB = None
+
+
+def load_library():
+    """Load the in-tree HIP library; fail loudly if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "painttween_amd/libpainttween.so is missing: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = C.CDLL(LIB_PATH)
+        bind(lib, "pt_")
+        lib.pt_create.restype = I
+        lib.pt_create.argtypes = [C.POINTER(VP), I]
+        lib.pt_global_last_error.restype = C.c_char_p
+        lib.pt_global_last_error.argtypes = []
+        lib.pt_synthesize_device.restype = I
+        lib.pt_synthesize_device.argtypes = [VP, C.POINTER(PtParams), I, I, I, I, I, I, C.POINTER(FrameGuides),
+                                             I, I, C.POINTER(StyleGuides), VP, VP, VP]
+        lib.pt_profile_enable.restype = I
+        lib.pt_profile_enable.argtypes = [VP, I]
+        lib.pt_profile_get.restype = I
+        lib.pt_profile_get.argtypes = [VP, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_uint64)]
+        _LIB = lib
+    return _LIB

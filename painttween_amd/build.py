@@ -1,0 +1,64 @@
+"""Build the gfx950 HIP library and the CPU oracle in place.
+
+``build()`` always recompiles:
+  painttween_amd/libpainttween.so  -- HIP kernels + host runtime (hipcc, gfx950)
+  oracle/libptoracle.so           -- CPU oracle (gcc, OpenMP), test infrastructure
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpainttween.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "libptoracle.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_SOURCES = ["pt_kernels.hip", "pt_runtime.hip"]
+
+# -ffp-contract=off: only the explicit fmaf calls fuse, so the device runs the
+# same IEEE operation sequence as the CPU build of the algorithm.
+# -packed-fp32-ops is disabled: packed FP32 forms are generated by nothing here.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+ORACLE_FLAGS = ["-std=c11", "-O2", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+                "-mfma", "-fPIC", "-shared", "-Wall", "-Wextra"]
+
+
+def _run(cmd):
+    proc = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return proc.stdout
+
+
+def build_hip():
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    tmp = ORACLE_LIB + ".tmp"
+    _run(["gcc"] + ORACLE_FLAGS + ["-o", tmp, os.path.join(ORACLE_DIR, "pt_oracle.c"), "-lm"])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def build():
+    build_oracle()
+    build_hip()
+
+
+if __name__ == "__main__":
+    build()
+    print("built", LIB, ORACLE_LIB)

@@ -1,0 +1,919 @@
+/*
+ * pt_kernels.hip -- CDNA4 (gfx950) kernels of the PaintTween synthesis hot
+ * path.  Each kernel cites the reference kernel it implements
+ * (src/PaintTweenCUDA/texSynth_kernel.cu unless noted).
+ *
+ * Numerics: compiled with -ffp-contract=off; every fused multiply-add is an
+ * explicit fmaf, and all other arithmetic is the same sequence of IEEE
+ * single-precision operations as the host CPU build of the algorithm, so the
+ * offset field, canvas and residual match it bit for bit.  Texture fetches
+ * are done in software: clamp addressing, point = floor(x + 0.5), bilinear
+ * with FP32 weights from the texel-space coordinate.
+ */
+#include "pt_internal.h"
+#include "pt_math.h"
+
+#define PT_BX 16
+#define PT_BY 16
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* ------------------------------------------------------------------ */
+/* software texture fetches                                            */
+/* ------------------------------------------------------------------ */
+struct Bilin {
+    int i00, i10, i01, i11;
+    float w00, w10, w01, w11;
+};
+
+__device__ __forceinline__ Bilin bilin(int pitch, int w, int h, float x, float y)
+{
+    Bilin q;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int x0 = (int)fx, y0 = (int)fy;
+    int xa = clampi(x0, 0, w - 1), xb = clampi(x0 + 1, 0, w - 1);
+    int ya = clampi(y0, 0, h - 1), yb = clampi(y0 + 1, 0, h - 1);
+    q.w00 = (1.0f - a) * (1.0f - b);
+    q.w10 = a * (1.0f - b);
+    q.w01 = (1.0f - a) * b;
+    q.w11 = a * b;
+    q.i00 = ya * pitch + xa;
+    q.i10 = ya * pitch + xb;
+    q.i01 = yb * pitch + xa;
+    q.i11 = yb * pitch + xb;
+    return q;
+}
+
+__device__ __forceinline__ float mix1(const Bilin &q, float t00, float t10, float t01, float t11)
+{
+    return fmaf(q.w11, t11, fmaf(q.w01, t01, fmaf(q.w10, t10, q.w00 * t00)));
+}
+
+__device__ __forceinline__ float4 mix4(const Bilin &q, float4 t00, float4 t10, float4 t01, float4 t11)
+{
+    float4 r;
+    r.x = mix1(q, t00.x, t10.x, t01.x, t11.x);
+    r.y = mix1(q, t00.y, t10.y, t01.y, t11.y);
+    r.z = mix1(q, t00.z, t10.z, t01.z, t11.z);
+    r.w = mix1(q, t00.w, t10.w, t01.w, t11.w);
+    return r;
+}
+
+__device__ __forceinline__ float4 tex4_lin(const float4 *t, int pitch, int w, int h, float x, float y)
+{
+    Bilin q = bilin(pitch, w, h, x, y);
+    return mix4(q, t[q.i00], t[q.i10], t[q.i01], t[q.i11]);
+}
+
+__device__ __forceinline__ float2 tex2_lin(const float2 *t, int pitch, int w, int h, float x, float y)
+{
+    Bilin q = bilin(pitch, w, h, x, y);
+    float2 t00 = t[q.i00], t10 = t[q.i10], t01 = t[q.i01], t11 = t[q.i11];
+    return make_float2(mix1(q, t00.x, t10.x, t01.x, t11.x), mix1(q, t00.y, t10.y, t01.y, t11.y));
+}
+
+__device__ __forceinline__ float tex1_lin(const float *t, int pitch, int w, int h, float x, float y)
+{
+    Bilin q = bilin(pitch, w, h, x, y);
+    return mix1(q, t[q.i00], t[q.i10], t[q.i01], t[q.i11]);
+}
+
+__device__ __forceinline__ int point_index(int w, int h, float x, float y)
+{
+    int xi = clampi((int)floorf(x + 0.5f), 0, w - 1);
+    int yi = clampi((int)floorf(y + 0.5f), 0, h - 1);
+    return yi * w + xi;
+}
+
+/* Exemplar base and output share one bilinear footprint: texel i holds
+ * {base, output} at ex[2i], ex[2i+1]. */
+__device__ __forceinline__ void tex_ex_pair(const PtArgs &A, float x, float y, float4 &b, float4 &o)
+{
+    Bilin q = bilin(A.ew, A.ew, A.eh, x, y);
+    const float4 *e = A.ex;
+    b = mix4(q, e[2 * q.i00], e[2 * q.i10], e[2 * q.i01], e[2 * q.i11]);
+    o = mix4(q, e[2 * q.i00 + 1], e[2 * q.i10 + 1], e[2 * q.i01 + 1], e[2 * q.i11 + 1]);
+}
+
+__device__ __forceinline__ float4 tex_ex_out(const PtArgs &A, float x, float y)
+{
+    Bilin q = bilin(A.ew, A.ew, A.eh, x, y);
+    const float4 *e = A.ex;
+    return mix4(q, e[2 * q.i00 + 1], e[2 * q.i10 + 1], e[2 * q.i01 + 1], e[2 * q.i11 + 1]);
+}
+
+__device__ __forceinline__ float tex_ex_out_alpha(const PtArgs &A, float x, float y)
+{
+    Bilin q = bilin(A.ew, A.ew, A.eh, x, y);
+    const float4 *e = A.ex;
+    return mix1(q, e[2 * q.i00 + 1].w, e[2 * q.i10 + 1].w, e[2 * q.i01 + 1].w, e[2 * q.i11 + 1].w);
+}
+
+__device__ __forceinline__ float2 scale_at(const PtArgs &A, float x, float y)
+{
+    return tex2_lin(A.scale, A.scw, A.scw, A.sch, x, y);
+}
+
+/* ------------------------------------------------------------------ */
+/* texSynth_util.h helpers                                             */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ bool in_bounds(int x, int y, int w, int h, int border)
+{
+    return x >= border && y >= border && x < w - border && y < h - border;
+}
+
+/* clampToExemplar, float version (texSynth_util.h:50-57) */
+__device__ __forceinline__ void clamp_ex(float &x, float &y, int w, int h)
+{
+    if (x < 0) x = 0;
+    if (y < 0) y = 0;
+    if (x >= (float)w) x = (float)(w - 1);
+    if (y >= (float)h) y = (float)(h - 1);
+}
+
+__device__ __forceinline__ float ssd_alpha(float4 a, float4 b, float alpha)
+{
+    float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+    float s = d0 * d0;
+    s = fmaf(d1, d1, s);
+    s = fmaf(d2, d2, s);
+    s = fmaf(alpha * d3, d3, s);
+    return s;
+}
+
+__device__ __forceinline__ float ssd4(float4 a, float4 b)
+{
+    float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+    float s = d0 * d0;
+    s = fmaf(d1, d1, s);
+    s = fmaf(d2, d2, s);
+    s = fmaf(d3, d3, s);
+    return s;
+}
+
+__device__ __forceinline__ float4 sub4(float4 a, float4 b)
+{
+    return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+}
+
+/* screenToExemplarBasis (texSynth_util.h:95-104) */
+__device__ __forceinline__ void basis(float angle, float su, float sv, float &b0x, float &b0y, float &b1x,
+                                      float &b1y)
+{
+    float s, c;
+    pt_sincosf(angle, &s, &c);
+    b1x = -s * sv;
+    b1y = c * sv;
+    b0x = c * su;
+    b0y = s * su;
+}
+
+/* calculateNewExemplarCoord (texSynth_util.h:106-118) */
+__device__ __forceinline__ void new_coord(float i, float j, float ex, float ey, float b0x, float b0y, float b1x,
+                                          float b1y, float &ox, float &oy)
+{
+    float cx = i * b0x + j * b1x;
+    float cy = i * b0y + j * b1y;
+    ox = ex + cx;
+    oy = ey + cy;
+}
+
+/* screenToExemplarAngle (texSynth_kernel.cu:425-436) */
+__device__ __forceinline__ float screen_to_ex_angle(const PtArgs &A, float x, float y, float ex, float ey)
+{
+    if (A.orient_en) {
+        float theta = A.ex_orient[point_index(A.ew, A.eh, ex, ey)] - A.orient[point_index(A.ow, A.oh, x, y)];
+        if ((double)theta > 1.57079632679489661923)
+            return (float)((double)theta - 3.14159265358979323846);
+        if ((double)theta < -1.57079632679489661923)
+            return (float)(3.14159265358979323846 + (double)theta);
+        return theta;
+    }
+    return 0.0f;
+}
+
+/* isImagePixelCulled (texSynth_kernel.cu:176-186), integer pixel */
+__device__ __forceinline__ bool image_culled(const PtArgs &A, int x, int y)
+{
+    return A.dt[A.cur][y * A.W + x] > ((float)A.cull - 0.0f);
+}
+
+/* isExemplarPixelCulled (texSynth_kernel.cu:200-205) */
+__device__ __forceinline__ bool exemplar_culled(const PtArgs &A, float x, float y)
+{
+    float d = tex1_lin(A.ex_dt, A.ew, A.ew, A.eh, x, y);
+    return (double)d > (double)A.cull * 1.25;
+}
+
+/* hysteresisTerm (texSynth_kernel.cu:506-535) */
+__device__ float hysteresis_term(const PtArgs &A, int x, int y)
+{
+    float maxD = A.max_dist / (float)(1 << A.level);
+    const float2 *v = (A.direction == PT_FORWARD) ? A.velb[A.cur] : A.velf[A.cur];
+    float2 d00 = tex2_lin(v, A.W, A.W, A.H, (float)x, (float)y);
+    float2 d10 = tex2_lin(v, A.W, A.W, A.H, (float)x + 1.f, (float)y);
+    float2 d01 = tex2_lin(v, A.W, A.W, A.H, (float)x, (float)y + 1.f);
+    float2 d11 = tex2_lin(v, A.W, A.W, A.H, (float)x + 1.f, (float)y + 1.f);
+    float maxX = fmaxf(d00.x, fmaxf(d01.x, fmaxf(d10.x, d11.x)));
+    float maxY = fmaxf(d00.y, fmaxf(d01.y, fmaxf(d10.y, d11.y)));
+    float minX = fminf(d00.x, fminf(d01.x, fminf(d10.x, d11.x)));
+    float minY = fminf(d00.y, fminf(d01.y, fminf(d10.y, d11.y)));
+    if ((maxX - minX) < maxD && (maxY - minY) < maxD)
+        return A.hyst_w;
+    return 0.f;
+}
+
+/* timeDerivativeTerms (texSynth_kernel.cu:438-469) */
+__device__ float time_terms(const PtArgs &A, const PtAdv &d, float4 prev, float pb0x, float pb0y, float pb1x,
+                            float pb1y, float4 base, float4 tb, float4 canvas, float4 to, int oo, int i, int j)
+{
+    float4 prev_base = d.base[oo];
+    float4 prev_out = d.out[oo];
+    float ptx, pty;
+    new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, ptx, pty);
+    float4 peb, peo;
+    tex_ex_pair(A, ptx, pty, peb, peo);
+    float r = ssd4(sub4(base, prev_base), sub4(tb, peb)) * A.tdi;
+    r = fmaf(ssd4(sub4(canvas, prev_out), sub4(to, peo)), A.tdo, r);
+    r = fmaf(ssd4(prev_out, to), A.tcw, r);
+    return r;
+}
+
+/*
+ * calculateResidual (texSynth_kernel.cu:537-659): one patch distance of
+ * candidate (t, tsc) at pixel (x, y).  Taps run i over x outer, j over y
+ * inner, accumulated sequentially per lane in that order.  TIME selects the
+ * variant with the temporal terms (an advected offset is valid at the
+ * centre); both variants perform the same operations when those terms are
+ * inactive.
+ */
+template <int RT, bool TIME>
+__device__ float calc_residual_impl(const PtArgs &A, int x, int y, float4 t, float2 tsc, float4 pf, float4 pb,
+                                    bool cpv, bool cnv)
+{
+    const int r = RT > 0 ? RT : A.r;
+    int o = y * A.W + x;
+    bool any_time_w = A.tdi > 0.f || A.tdo > 0.f || A.tcw > 0.f;
+    bool fwd_dir = A.direction == PT_FORWARD || A.direction == PT_BIDIRECTIONAL;
+    bool bwd_dir = A.direction == PT_BACKWARD || A.direction == PT_BIDIRECTIONAL;
+    float fb0x = 0, fb0y = 0, fb1x = 0, fb1y = 0, bb0x = 0, bb0y = 0, bb1x = 0, bb1y = 0;
+    float sum_w = 0.f, sum_r = 0.f;
+    if (TIME) {
+        if (cpv) {
+            float2 ps = A.adv[0].sc[o];
+            float ang = screen_to_ex_angle(A, (float)x, (float)y, pf.x, pf.y);
+            basis(ang, A.su * ps.x, A.sv * ps.y, fb0x, fb0y, fb1x, fb1y);
+        }
+        if (cnv) {
+            float2 ps = A.adv[1].sc[o];
+            float ang = screen_to_ex_angle(A, (float)x, (float)y, pb.x, pb.y);
+            basis(ang, A.su * ps.x, A.sv * ps.y, bb0x, bb0y, bb1x, bb1y);
+        }
+        if (cpv)
+            sum_r -= hysteresis_term(A, x, y);
+    }
+    float b0x, b0y, b1x, b1y;
+    basis(t.z, A.su * tsc.x, A.sv * tsc.y, b0x, b0y, b1x, b1y);
+    const float4 *color = A.color[A.cur];
+#pragma unroll 1
+    for (int i = -r; i <= r; i++) {
+        if (x + i < 0 || x + i >= A.lw)
+            continue;
+#pragma unroll
+        for (int j = -r; j <= r; j++) {
+            if (y + j < 0 || y + j >= A.lh)
+                continue;
+            int oo = (y + j) * A.W + (x + i);
+            float4 base = color[oo];
+            float4 canvas = A.canvas[oo];
+            float tx, ty;
+            new_coord((float)i, (float)j, t.x, t.y, b0x, b0y, b1x, b1y, tx, ty);
+            if (tx >= (float)A.hw) tx -= (float)A.hw;
+            else if (tx < 0) tx += (float)A.hw;
+            if (ty >= (float)A.hh) ty -= (float)A.hh;
+            else if (ty < 0) ty += (float)A.hh;
+            float4 tb, to;
+            tex_ex_pair(A, tx, ty, tb, to);
+            float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
+            if (TIME && any_time_w) {
+                if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
+                    res = res + time_terms(A, A.adv[0], pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo, i, j);
+                if (bwd_dir && cnv && A.adv[1].xyth[oo].w > 0)
+                    res = res + time_terms(A, A.adv[1], pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo, i, j);
+            }
+            if (i == 0 && j == 0) {
+                if (A.dt_w > 0.f) {
+                    float de = tex1_lin(A.ex_dt, A.ew, A.ew, A.eh, tx, ty);
+                    de /= tsc.y;
+                    float di = A.dt[A.cur][o];
+                    float diff = fabsf(de - di);
+                    diff = fminf(fmaxf(diff, 0.0f), 10.0f);
+                    res = res + diff * A.dt_w;
+                }
+            } else {
+                res = fmaf(ssd_alpha(canvas, to, A.alpha_w), A.canvas_w, res);
+            }
+            float wgt = A.gauss[i + r] * A.gauss[j + r];
+            sum_r = fmaf(res, wgt, sum_r);
+            sum_w += wgt;
+        }
+    }
+    if (sum_w < 0.000001f)
+        return 1e31f;
+    return sum_r / sum_w;
+}
+
+/* TK (a kernel-level switch) is false when the host knows no advected
+ * offset is valid (the advected buffers were reset and not written since);
+ * the per-pixel test then always takes the plain variant. */
+template <int RT, bool TK>
+__device__ __forceinline__ float calc_residual(const PtArgs &A, int x, int y, float4 t, float2 tsc)
+{
+    if (TK) {
+        int o = y * A.W + x;
+        float4 pf = A.adv[0].xyth[o], pb = A.adv[1].xyth[o];
+        bool cpv = pf.w > 0, cnv = pb.w > 0;
+        if (cpv || cnv)
+            return calc_residual_impl<RT, true>(A, x, y, t, tsc, pf, pb, cpv, cnv);
+    }
+    float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    return calc_residual_impl<RT, false>(A, x, y, t, tsc, z, z, false, false);
+}
+
+/* test_offset (texSynth_kernel.cu:695-721) */
+template <int RT, bool TK>
+__device__ __forceinline__ float test_offset(const PtArgs &A, int x, int y, float4 t, float2 tsc, float image_alpha)
+{
+    if (!exemplar_culled(A, t.x, t.y)) {
+        if (!A.transp) {
+            float ea = tex_ex_out_alpha(A, t.x, t.y);
+            if (ea < 0.99f && image_alpha > 0.99f)
+                return 1e31f;
+        }
+        return calc_residual<RT, TK>(A, x, y, t, tsc);
+    }
+    return 1e31f;
+}
+
+/* ------------------------------------------------------------------ */
+/* kernels                                                             */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ bool pixel_of(const PtArgs &A, int &x, int &y)
+{
+    x = blockIdx.x * PT_BX + threadIdx.x;
+    y = blockIdx.y * PT_BY + threadIdx.y;
+    return x < A.lw && y < A.lh;
+}
+
+/* updateResidualCache_kernel (texSynth_kernel.cu:661-693) */
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_update_residual(PtArgs A)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    int o = y * A.W + x;
+    if (image_culled(A, x, y)) {
+        A.res_base[o] = 0.0f;
+        return;
+    }
+    A.res_base[o] = calc_residual<RT, TK>(A, x, y, A.base.xyth[o], A.base.sc[o]);
+}
+
+/* offsetsToOutput (texSynth_kernel.cu:263-274) */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_canvas(PtArgs A)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    int o = y * A.W + x;
+    float4 n = A.base.xyth[o];
+    A.canvas[o] = tex_ex_out(A, n.x, n.y);
+}
+
+/* copyRandomKernel (texSynth_kernel.cu:2044-2059) */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    int o = y * A.W + x;
+    float4 r = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
+    A.work.xyth[o] = make_float4(r.x, r.y, r.z, 0.f);
+    A.work.sc[o] = make_float2(1.f, 1.f);
+}
+
+/* calculatePropagateOffset (texSynth_kernel.cu:956-982); pixelWrap
+ * (cudaImageBuffer.h:81-87) wraps modulo the level-0 dimensions. */
+__device__ __forceinline__ void propagate_offset(const PtArgs &A, int tx, int ty, float x, float y, float4 &c,
+                                                 float2 &csc)
+{
+    int wx = (int)((float)tx - (float)A.W * floorf((float)tx / (float)A.W));
+    int wy = (int)((float)ty - (float)A.H * floorf((float)ty / (float)A.H));
+    float4 t = A.base.xyth[wy * A.W + wx];
+    float2 sc = scale_at(A, (float)tx, (float)ty);
+    float b0x, b0y, b1x, b1y;
+    basis(t.z, A.su * sc.x, A.sv * sc.y, b0x, b0y, b1x, b1y);
+    float ex, ey;
+    new_coord(x - (float)tx, y - (float)ty, t.x, t.y, b0x, b0y, b1x, b1y, ex, ey);
+    clamp_ex(ex, ey, A.ew, A.eh);
+    float ang = screen_to_ex_angle(A, x, y, ex, ey);
+    csc = scale_at(A, x, y);
+    c = make_float4(ex, ey, ang, 0.f);
+}
+
+/* propagate_kernel (texSynth_kernel.cu:1112-1219) */
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    int o = y * A.W + x;
+    if (image_culled(A, x, y)) {
+        A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+        A.work.sc[o] = make_float2(1.f, 1.f);
+        return;
+    }
+    float best = A.res_base[o];
+    float ia = A.color[A.cur][o].w;
+    float4 ow = A.base.xyth[o];
+    float2 osc = A.base.sc[o];
+    for (int i = 0; i < 4; i++) {
+        int tx = x, ty = y;
+        if (i == 0) tx = x + jump;
+        else if (i == 1) tx = x - jump;
+        else if (i == 2) ty = y + jump;
+        else ty = y - jump;
+        float4 c;
+        float2 csc;
+        propagate_offset(A, tx, ty, (float)x, (float)y, c, csc);
+        float r = test_offset<RT, TK>(A, x, y, c, csc, ia);
+        if (r < best) {
+            ow = c;
+            osc = csc;
+            best = r;
+        }
+    }
+    A.work.xyth[o] = ow;
+    A.work.sc[o] = osc;
+}
+
+__device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
+
+/* scatter_kernel (texSynth_kernel.cu:865-952) with averageOffset (:841-862).
+ * As in the reference, a pixel is written only when a candidate beats its
+ * cached residual; others keep the working buffer's current content. */
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter(PtArgs A)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    int o = y * A.W + x;
+    if (image_culled(A, x, y)) return;
+    float4 off = A.base.xyth[o];
+    float2 offsc = A.base.sc[o];
+    float cached = A.res_base[o];
+    float best = (float)((double)cached + 10e-7);
+    float4 ow = off;
+    float2 osc = offsc;
+    float ia = A.color[A.cur][o].w;
+    float dist = (float)(A.ew > A.eh ? A.ew : A.eh);
+    const float closest = 10.f, factor = 0.9f;
+    float2 sc = scale_at(A, (float)x, (float)y);
+    for (int i = 0; (i < A.samples) && (dist * factor >= closest); i++) {
+        float r = 0.0f, theta = 0.0f;
+        if (A.rng_mode == PT_RNG_COUNTER) {
+            uint32_t ctr[4] = {(uint32_t)x, (uint32_t)y, A.call, (uint32_t)i}, key[2] = {A.key0, A.key1}, rnd[4];
+            pt_philox4x32_10(ctr, key, rnd);
+            r = u24(rnd[0]);
+            theta = (float)((double)u24(rnd[1]) * 0.63661977236758134308);
+        }
+        float dmax = dist, dmin = dist * factor;
+        float s, co;
+        pt_sincosf(theta, &s, &co);
+        float len = r * (dmax - dmin) + dmin;
+        float ex = ow.x + co * len;
+        float ey = ow.y + s * len;
+        clamp_ex(ex, ey, A.ew, A.eh);
+        float ang = screen_to_ex_angle(A, (float)x, (float)y, ex, ey);
+        float4 c = make_float4(ex, ey, ang, 0.f);
+        float res = test_offset<RT, TK>(A, x, y, c, sc, ia);
+        if (res < best) {
+            ow = c;
+            osc = sc;
+            best = res;
+        }
+        dist *= factor;
+    }
+    /* averageOffset */
+    float ax = 0.f, ay = 0.f;
+    int sum = 0;
+    for (int i = -1; i <= 1; i++)
+        for (int j = -1; j <= 1; j++) {
+            if (!in_bounds(x + i, y + j, A.lw, A.lh, 0) || (i == 0 && j == 0)) continue;
+            float4 n = A.base.xyth[(y + j) * A.W + (x + i)];
+            ax += n.x;
+            ay += n.y;
+            sum++;
+        }
+    ax /= (float)sum;
+    ay /= (float)sum;
+    float4 avg = make_float4(ax, ay, screen_to_ex_angle(A, (float)x, (float)y, ax, ay), 1.f);
+    float2 avgsc = make_float2(1.f, 1.f);
+    float res = test_offset<RT, TK>(A, x, y, avg, avgsc, ia);
+    if (res < best) {
+        ow = avg;
+        osc = avgsc;
+        best = res;
+    }
+    if (best < cached) {
+        A.work.xyth[o] = ow;
+        A.work.sc[o] = osc;
+    }
+}
+
+/* upsampleBilinearKernel (texSynth_kernel.cu:1453-1510) with
+ * offsetUpsampleNearest (:1403-1417). */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    if (!in_bounds(x, y, A.lw, A.lh, 1)) return;
+    float cxf = (float)((double)x - 0.5) / 2.0f;
+    float cyf = (float)((double)y - 0.5) / 2.0f;
+    int lcx = (int)floorf(cxf), lcy = (int)floorf(cyf);
+    float4 o00 = A.base.xyth[lcy * A.W + lcx];
+    float4 o10 = A.base.xyth[lcy * A.W + lcx + 1];
+    float4 o01 = A.base.xyth[(lcy + 1) * A.W + lcx];
+    float4 o11 = A.base.xyth[(lcy + 1) * A.W + lcx + 1];
+    float a = cxf - (float)lcx, b = cyf - (float)lcy;
+    float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
+    float ix = ((w00 * o00.x + w10 * o10.x) + w01 * o01.x) + w11 * o11.x;
+    float iy = ((w00 * o00.y + w10 * o10.y) + w01 * o01.y) + w11 * o11.y;
+    float ux = ix * 2.0f + 0.5f, uy = iy * 2.0f + 0.5f;
+    const float M = 4.0f * 4.0f;
+#define D2(q) (((ix - (q).x) * (ix - (q).x)) + ((iy - (q).y) * (iy - (q).y)))
+    if (D2(o00) > M || D2(o10) > M || D2(o01) > M || D2(o11) > M) {
+        int rcx = (int)roundf(cxf), rcy = (int)roundf(cyf);
+        float4 cx = A.base.xyth[rcy * A.W + rcx];
+        float b0x, b0y, b1x, b1y;
+        basis(cx.z, 1, 1, b0x, b0y, b1x, b1y);
+        float nx, ny;
+        new_coord(cxf - (float)rcx, cyf - (float)rcy, cx.x, cx.y, b0x, b0y, b1x, b1y, nx, ny);
+        clamp_ex(nx, ny, A.ew, A.eh);
+        ux = (float)(((double)nx * 2.0) + 0.5);
+        uy = (float)(((double)ny * 2.0) + 0.5);
+    }
+#undef D2
+    float rw = A.res_base[lcy * A.W + lcx];
+    float ang = screen_to_ex_angle(A, (float)x, (float)y, ux, uy);
+    float2 sc = scale_at(A, (float)x, (float)y);
+    int o = y * A.W + x;
+    A.work.xyth[o] = make_float4(ux, uy, ang, 0.f);
+    A.work.sc[o] = sc;
+    A.res_work[o] = rw;
+}
+
+/* calculateTranslatedPosition (texSynth_kernel.cu:1541-1575) */
+__device__ __forceinline__ bool translated_position(const PtArgs &A, int fwd, int x, int y, float &tx, float &ty)
+{
+    int prev = (A.cur + 1) % 2;
+    float2 forward, back;
+    if (fwd) {
+        back = tex2_lin(A.velb[A.cur], A.W, A.W, A.H, (float)x, (float)y);
+        tx = (float)x + back.x;
+        ty = (float)y + back.y;
+        forward = tex2_lin(A.velf[prev], A.W, A.W, A.H, tx, ty);
+    } else {
+        forward = tex2_lin(A.velf[A.cur], A.W, A.W, A.H, (float)x, (float)y);
+        tx = (float)x + forward.x;
+        ty = (float)y + forward.y;
+        back = tex2_lin(A.velb[prev], A.W, A.W, A.H, tx, ty);
+    }
+    const float valid_dist = 1.414 * 2;
+    forward.x = -forward.x;
+    forward.y = -forward.y;
+    float dist = sqrtf((forward.x - back.x) * (forward.x - back.x) + (forward.y - back.y) * (forward.y - back.y));
+    return in_bounds((int)ceilf(tx), (int)ceilf(ty), A.lw, A.lh, 0) &&
+           in_bounds((int)floorf(tx), (int)floorf(ty), A.lw, A.lh, 0) && (dist < valid_dist);
+}
+
+/* advectKernel (texSynth_kernel.cu:1577-1693), without the keyframe branch */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int time_step, int flags)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    int o = y * A.W + x;
+    const PtAdv &d = A.adv[fwd ? 0 : 1];
+    if (image_culled(A, x, y)) {
+        d.base[o] = make_float4(0, 0, 0, 0);
+        d.out[o] = make_float4(0, 0, 0, 0);
+        d.xyth[o] = make_float4(0, 0, 0, 0);
+        d.sc[o] = make_float2(1.f, 1.f);
+        return;
+    }
+    int prev = (A.cur + 1) % 2;
+    float4 rnd = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
+    float4 frac = make_float4(rnd.x - floorf(rnd.x), rnd.y - floorf(rnd.y), rnd.z - floorf(rnd.z),
+                              rnd.w - floorf(rnd.w));
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    float tx, ty;
+    if (!(flags & PT_ADVECT_DO_NOT_CHECK_PREVIOUS) && translated_position(A, fwd, x, y, tx, ty) &&
+        !(tex1_lin(A.dt[prev], A.W, A.W, A.H, tx, ty) > ((float)A.cull - 8.0f))) {
+        int rtx = (int)roundf(tx), rty = (int)roundf(ty);
+        float4 nr = A.base.xyth[rty * A.W + rtx];
+        float b0x, b0y, b1x, b1y;
+        basis(nr.z, 1, 1, b0x, b0y, b1x, b1y);
+        float ix, iy;
+        new_coord(tx - (float)rtx, ty - (float)rty, nr.x, nr.y, b0x, b0y, b1x, b1y, ix, iy);
+        clamp_ex(ix, iy, A.ew, A.eh);
+        float4 prev_color = tex4_lin(A.color[prev], A.W, A.W, A.H, tx, ty);
+        out = make_float4(ix, iy, nr.z, 1.f);
+        d.base[o] = prev_color;
+        d.out[o] = tex_ex_out(A, out.x, out.y);
+        d.ts[o] = time_step;
+    }
+    if (out.w == 0.f) {
+        out = make_float4(rnd.x, rnd.y, rnd.z, 0.f);
+        d.base[o] = make_float4(0, 0, 0, 0);
+        d.out[o] = make_float4(0, 0, 0, 0);
+        d.ts[o] = -1;
+    }
+    float jit = A.jitter;
+    out.x = out.x * (1 - jit) + frac.x * (jit);
+    out.y = out.y * (1 - jit) + frac.y * (jit);
+    out.z = screen_to_ex_angle(A, (float)x, (float)y, out.x, out.y);
+    d.xyth[o] = out;
+    d.sc[o] = make_float2(1.f, 1.f);
+}
+
+/* mergeRandomlyKernel (texSynth_kernel.cu:1891-1936); the ribbon textures
+ * are unbound without keyframes and read as zero. */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int fwd, float weight, int force)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    int o = y * A.W + x;
+    const PtAdv &d = A.adv[fwd ? 0 : 1];
+    float4 a = d.xyth[o];
+    float4 rnd = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
+    float r = fwd ? rnd.x - floorf(rnd.x) : rnd.y - floorf(rnd.y);
+    if ((a.w > 0 || force) && r < weight) {
+        A.work.xyth[o] = a;
+        A.work.sc[o] = d.sc[o];
+    } else {
+        A.work.xyth[o] = A.base.xyth[o];
+        A.work.sc[o] = A.base.sc[o];
+    }
+}
+
+/* mergeUsingResidualKernel (texSynth_kernel.cu:1856-1889) */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_residual(PtArgs A, int fwd)
+{
+    int x, y;
+    if (!pixel_of(A, x, y)) return;
+    int o = y * A.W + x;
+    const PtAdv &d = A.adv[fwd ? 0 : 1];
+    if (d.res[o] < A.res_base[o]) {
+        float4 a = d.xyth[o];
+        a.w = 1.f;
+        A.work.xyth[o] = a;
+        A.work.sc[o] = d.sc[o];
+    } else {
+        float4 u = A.base.xyth[o];
+        u.w = 0.f;
+        A.work.xyth[o] = u;
+        A.work.sc[o] = A.base.sc[o];
+    }
+}
+
+/* downsampleAverage (downsample.cpp:41-160): one level, `ch` floats per
+ * element. */
+__global__ void k_downsample(const float *src, int pw, float *dst, int lw, int lh, int ch, int scale_mode)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= lw || y >= lh) return;
+    for (int k = 0; k < ch; k++) {
+        float s = 0.f;
+        s += src[((y * 2) * pw + x * 2) * ch + k];
+        s += src[((y * 2) * pw + x * 2 + 1) * ch + k];
+        s += src[((y * 2 + 1) * pw + x * 2) * ch + k];
+        s += src[((y * 2 + 1) * pw + x * 2 + 1) * ch + k];
+        s *= 0.25f;
+        if (scale_mode == PT_DOWNSAMPLE_SCALE_WITH_LEVELS)
+            s *= 0.5f;
+        dst[(y * lw + x) * ch + k] = s;
+    }
+}
+
+__global__ void k_fill(float *dst, size_t n, float v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+/* strucTensorToAngle (imageIO.h:136-139) per element */
+__global__ void k_tensor_angle(const float *t, float *a, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = pt_tensor_to_angle(t[3 * i], t[3 * i + 1], t[3 * i + 2]);
+}
+
+/* Writes a w x h level into slot `slot` (0 base, 1 output) of the
+ * interleaved exemplar texture of width ew. */
+__global__ void k_interleave(const float4 *src, int w, int h, float4 *ex, int ew, int slot)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    ex[2 * (y * ew + x) + slot] = src[y * w + x];
+}
+
+/* createRandomPyramid (synthesisFrame.cpp:102-143) for one level, Philox
+ * draws: key (seed, frame*level), counter (x, y, trial, 0), fractional parts
+ * from counter (x, y, 50, 0). */
+__global__ void k_random_pyramid(const float4 *color, const float *iorient, int w, int h, const float4 *eout,
+                                 const float *eorient, int ew, int eh, int transp, int orient_en, uint32_t key0,
+                                 uint32_t key1, float4 *out)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    uint32_t key[2] = {key0, key1};
+    uint32_t ctr[4] = {(uint32_t)x, (uint32_t)y, 0u, 0u}, o[4];
+    int ox, oy, trial = 0;
+    float ia = color[y * w + x].w;
+    do {
+        ctr[2] = (uint32_t)trial;
+        pt_philox4x32_10(ctr, key, o);
+        ox = pt_rand31(o[0]) % (ew - 1);
+        oy = pt_rand31(o[1]) % (eh - 1);
+        trial++;
+    } while (!transp && ia > 0.99f && eout[oy * ew + ox].w < 0.99f && trial < 50);
+    float orientation = orient_en ? iorient[y * w + x] - eorient[oy * ew + ox] : 0.f;
+    ctr[2] = 50u;
+    pt_philox4x32_10(ctr, key, o);
+    float rx = pt_rand_unit(o[0]);
+    float ry = pt_rand_unit(o[1]);
+    out[y * w + x] = make_float4((float)ox + rx, (float)oy + ry, orientation, 0.0f);
+}
+
+/* PatchXF (36 B, types.h:98-114) <-> SoA planes for the host boundary. */
+struct PatchXF36 {
+    float x, y, theta, hysteresis;
+    int32_t layer;
+    float su, sv, ly, la;
+};
+
+__global__ void k_pack_nnf(PtNnf s, int W, int lw, int lh, PatchXF36 *d)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= lw || y >= lh) return;
+    float4 v = s.xyth[y * W + x];
+    float2 c = s.sc[y * W + x];
+    PatchXF36 p = {v.x, v.y, v.z, v.w, 0, c.x, c.y, 0.f, 0.f};
+    d[y * lw + x] = p;
+}
+
+__global__ void k_unpack_nnf(const PatchXF36 *s, int lw, int lh, PtNnf d, int W)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= lw || y >= lh) return;
+    PatchXF36 p = s[y * lw + x];
+    d.xyth[y * W + x] = make_float4(p.x, p.y, p.theta, p.hysteresis);
+    d.sc[y * W + x] = make_float2(p.su, p.sv);
+}
+
+/* ------------------------------------------------------------------ */
+/* launchers                                                           */
+/* ------------------------------------------------------------------ */
+static dim3 grid_for(int w, int h) { return dim3((w + PT_BX - 1) / PT_BX, (h + PT_BY - 1) / PT_BY); }
+static const dim3 kBlock(PT_BX, PT_BY);
+
+#define PT_DISPATCH_R(KERNEL, A, ...)                                                                     \
+    do {                                                                                                  \
+        dim3 g = grid_for((A).lw, (A).lh);                                                                \
+        if ((A).time_active) {                                                                            \
+            switch ((A).r) {                                                                              \
+            case 2: hipLaunchKernelGGL((KERNEL<2, true>), g, kBlock, 0, s, (A), ##__VA_ARGS__); break;    \
+            case 3: hipLaunchKernelGGL((KERNEL<3, true>), g, kBlock, 0, s, (A), ##__VA_ARGS__); break;    \
+            case 4: hipLaunchKernelGGL((KERNEL<4, true>), g, kBlock, 0, s, (A), ##__VA_ARGS__); break;    \
+            default: hipLaunchKernelGGL((KERNEL<0, true>), g, kBlock, 0, s, (A), ##__VA_ARGS__); break;   \
+            }                                                                                             \
+        } else {                                                                                          \
+            switch ((A).r) {                                                                              \
+            case 2: hipLaunchKernelGGL((KERNEL<2, false>), g, kBlock, 0, s, (A), ##__VA_ARGS__); break;   \
+            case 3: hipLaunchKernelGGL((KERNEL<3, false>), g, kBlock, 0, s, (A), ##__VA_ARGS__); break;   \
+            case 4: hipLaunchKernelGGL((KERNEL<4, false>), g, kBlock, 0, s, (A), ##__VA_ARGS__); break;   \
+            default: hipLaunchKernelGGL((KERNEL<0, false>), g, kBlock, 0, s, (A), ##__VA_ARGS__); break;  \
+            }                                                                                             \
+        }                                                                                                 \
+    } while (0)
+
+hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s)
+{
+    PT_DISPATCH_R(k_update_residual, a);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_canvas(const PtArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_canvas, grid_for(a.lw, a.lh), kBlock, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_copy_random(const PtArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_random, grid_for(a.lw, a.lh), kBlock, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_propagate(const PtArgs &a, int jump, hipStream_t s)
+{
+    PT_DISPATCH_R(k_propagate, a, jump);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_scatter(const PtArgs &a, hipStream_t s)
+{
+    PT_DISPATCH_R(k_scatter, a);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_upsample(const PtArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_upsample, grid_for(a.lw, a.lh), kBlock, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_advect(const PtArgs &a, int forwards, int time_step, int flags, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_advect, grid_for(a.lw, a.lh), kBlock, 0, s, a, forwards, time_step, flags);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_merge_randomly(const PtArgs &a, int forwards, float weight, int force, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_merge_randomly, grid_for(a.lw, a.lh), kBlock, 0, s, a, forwards, weight, force);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_merge_residual(const PtArgs &a, int forwards, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_merge_residual, grid_for(a.lw, a.lh), kBlock, 0, s, a, forwards);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_downsample(const float *src, int pw, float *dst, int lw, int lh, int ch, int scale_mode,
+                                hipStream_t s)
+{
+    dim3 b(16, 16), g((lw + 15) / 16, (lh + 15) / 16);
+    hipLaunchKernelGGL(k_downsample, g, b, 0, s, src, pw, dst, lw, lh, ch, scale_mode);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_fill(float *dst, size_t n, float v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, n, v);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_tensor_angle(const float *t, float *a, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tensor_angle, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, a, n);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_interleave(const float4 *src, int w, int h, float4 *ex, int ew, int slot, hipStream_t s)
+{
+    dim3 b(16, 16), g((w + 15) / 16, (h + 15) / 16);
+    hipLaunchKernelGGL(k_interleave, g, b, 0, s, src, w, h, ex, ew, slot);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_random_pyramid(const float4 *color, const float *iorient, int w, int h, const float4 *eout,
+                                    const float *eorient, int ew, int eh, int transp, int orient_en,
+                                    uint32_t key0, uint32_t key1, float4 *out, hipStream_t s)
+{
+    dim3 b(16, 16), g((w + 15) / 16, (h + 15) / 16);
+    hipLaunchKernelGGL(k_random_pyramid, g, b, 0, s, color, iorient, w, h, eout, eorient, ew, eh, transp,
+                       orient_en, key0, key1, out);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_pack_nnf(const PtNnf &src, int W, int lw, int lh, void *dst, hipStream_t s)
+{
+    dim3 b(16, 16), g((lw + 15) / 16, (lh + 15) / 16);
+    hipLaunchKernelGGL(k_pack_nnf, g, b, 0, s, src, W, lw, lh, (PatchXF36 *)dst);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_unpack_nnf(const void *src, int lw, int lh, PtNnf dst, int W, hipStream_t s)
+{
+    dim3 b(16, 16), g((lw + 15) / 16, (lh + 15) / 16);
+    hipLaunchKernelGGL(k_unpack_nnf, g, b, 0, s, (const PatchXF36 *)src, lw, lh, dst, W);
+    return hipGetLastError();
+}

@@ -1,0 +1,121 @@
+"""Synthetic inputs for the synthesis workloads (no datasets are available).
+
+SinNoise(seed, stream, W, H, C): per channel, four sinusoids with amplitude
+a ~ U(0.5, 1), frequencies fx, fy ~ +-U(0.5, 8) cycles per image and phase
+U(0, 2pi); v = 0.9 * (1/2 + 1/2 * sum(a sin(.)) / sum(a)) + 0.1 * U(0, 1),
+so every channel, alpha included, lies in [0, 1].
+
+Orientation guides are unit vectors at angle 2pi * SmoothNoise (a cubic
+upsampled 8x8 lattice of U(0,1)), delivered as the structure tensor
+(c^2, c s, s^2) the reference's orientation input expects
+(G/nlInputElements.cpp:33).  Distance transforms are the distance to 64
+uniform seed points divided by the maximum.
+"""
+import numpy as np
+from scipy import ndimage
+
+
+def _rng(seed, stream):
+    return np.random.default_rng([int(seed), int(stream), 0x9A1D])
+
+
+def sin_noise(seed, stream, w, h, c=4):
+    rng = _rng(seed, stream)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    out = np.empty((h, w, c), np.float32)
+    for k in range(c):
+        a = rng.uniform(0.5, 1.0, 4)
+        fx = rng.uniform(0.5, 8.0, 4) * rng.choice([-1.0, 1.0], 4)
+        fy = rng.uniform(0.5, 8.0, 4) * rng.choice([-1.0, 1.0], 4)
+        ph = rng.uniform(0.0, 2 * np.pi, 4)
+        s = np.zeros((h, w))
+        for i in range(4):
+            s += a[i] * np.sin(2 * np.pi * (fx[i] * xx / w + fy[i] * yy / h) + ph[i])
+        v = 0.9 * (0.5 + 0.5 * s / a.sum()) + 0.1 * rng.uniform(0.0, 1.0, (h, w))
+        out[..., k] = v
+    return out
+
+
+def smooth_noise(seed, stream, w, h, lattice=8):
+    rng = _rng(seed, stream)
+    g = rng.uniform(0.0, 1.0, (lattice, lattice))
+    z = ndimage.zoom(g, (h / lattice, w / lattice), order=3, mode="grid-wrap", grid_mode=True)
+    return z[:h, :w]
+
+
+def orientation_tensor(seed, stream, w, h):
+    th = 2 * np.pi * smooth_noise(seed, stream, w, h)
+    c, s = np.cos(th), np.sin(th)
+    return np.stack([c * c, c * s, s * s], axis=-1).astype(np.float32)
+
+
+def distance_transform(seed, stream, w, h, points=64):
+    rng = _rng(seed, stream)
+    px = rng.uniform(0, w, points)
+    py = rng.uniform(0, h, points)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    d = np.full((h, w), np.inf)
+    for i in range(points):
+        d = np.minimum(d, np.hypot(xx - px[i], yy - py[i]))
+    return (d / d.max()).astype(np.float32)
+
+
+def style_pair(w, h, seed, guides=False):
+    """Exemplar input/output pair (streams 0, 1) and optional guides."""
+    st = {"base": sin_noise(seed, 0, w, h), "output": sin_noise(seed, 1, w, h)}
+    if guides:
+        st["orientation"] = orientation_tensor(seed, 10, w, h)
+        st["dist_trans"] = distance_transform(seed, 11, w, h)
+    return st
+
+
+def still_frame(w, h, seed, guides=False):
+    """A target frame (stream 2) with optional orientation and DT guides."""
+    fr = {"color": sin_noise(seed, 2, w, h)}
+    if guides:
+        fr["orientation"] = orientation_tensor(seed, 20, w, h)
+        fr["dist_trans"] = distance_transform(seed, 21, w, h)
+    return fr
+
+
+def _motion(seed, t, w, h):
+    """M_t(x) = R(0.5 deg)(x - c) + c - x + (3, 1) + 2 * SmoothNoise2(t)."""
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+    a = np.deg2rad(0.5)
+    rx = np.cos(a) * (xx - cx) - np.sin(a) * (yy - cy) + cx - xx
+    ry = np.sin(a) * (xx - cx) + np.cos(a) * (yy - cy) + cy - yy
+    nx = 2.0 * (smooth_noise(seed, 100 + 2 * t, w, h) - 0.5)
+    ny = 2.0 * (smooth_noise(seed, 101 + 2 * t, w, h) - 0.5)
+    return np.stack([rx + 3.0 + nx, ry + 1.0 + ny], axis=-1)
+
+
+def animation(w, h, frames, seed, guides=True):
+    """Frames whose colour follows the motion field; vel_f[t] = M_t and
+    vel_b[t](x) = -M_{t-1}(x - M_{t-1}(x)) (zero for the first frame)."""
+    base = sin_noise(seed, 2, w + 64, h + 64)
+    out = []
+    shift = np.zeros(2)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    for t in range(frames):
+        m = _motion(seed, t, w, h)
+        sx, sy = shift
+        col = np.empty((h, w, 4), np.float32)
+        for k in range(4):
+            col[..., k] = ndimage.map_coordinates(base[..., k], [yy + 32 - sy, xx + 32 - sx], order=1, mode="wrap")
+        fr = {"color": col, "vel_f": m.astype(np.float32)}
+        if t > 0:
+            mp = _motion(seed, t - 1, w, h)
+            qx = xx - mp[..., 0]
+            qy = yy - mp[..., 1]
+            bx = -ndimage.map_coordinates(mp[..., 0], [qy, qx], order=1, mode="nearest")
+            by = -ndimage.map_coordinates(mp[..., 1], [qy, qx], order=1, mode="nearest")
+            fr["vel_b"] = np.stack([bx, by], axis=-1).astype(np.float32)
+        else:
+            fr["vel_b"] = np.zeros((h, w, 2), np.float32)
+        if guides:
+            fr["orientation"] = orientation_tensor(seed, 20 + t, w, h)
+            fr["dist_trans"] = distance_transform(seed, 21, w, h)
+        out.append(fr)
+        shift += (3.0, 1.0)
+    return out

@@ -1,0 +1,90 @@
+"""Pins of the deterministic math the oracle and the kernels share
+(painttween_amd/csrc/pt_math.h): published known-answer vectors and libm
+error bounds."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.oracle_ctx import load_oracle
+
+
+def philox(ctr, key):
+    lib = load_oracle()
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.ptref_philox(c, k, o)
+    return list(o)
+
+
+def test_philox_known_answers():
+    # Random123 known-answer vectors for philox4x32-10 (Salmon et al., SC'11).
+    assert philox([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert philox([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def sincos(x):
+    lib = load_oracle()
+    s, c = C.c_float(), C.c_float()
+    lib.ptref_sincosf(C.c_float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def test_sincos_within_two_ulp_of_libm():
+    xs = np.concatenate([np.linspace(-13.0, 13.0, 20001, dtype=np.float32),
+                         np.float32([0.0, -0.0, np.pi / 2, np.pi, -np.pi, 2 * np.pi, 1e-6, 0.63661975])])
+    worst = 0.0
+    for x in xs:
+        s, c = sincos(float(x))
+        xd = float(np.float32(x))
+        for got, ref in ((s, np.sin(xd)), (c, np.cos(xd))):
+            ulp = np.spacing(np.float32(max(abs(ref), 2 ** -24)))
+            worst = max(worst, abs(got - ref) / ulp)
+    assert worst <= 2.0, worst
+
+
+def test_sincos_exact_at_zero():
+    assert sincos(0.0) == (0.0, 1.0)
+
+
+def test_atan2_within_three_ulp():
+    lib = load_oracle()
+    rng = np.random.default_rng(0)
+    pts = np.concatenate([rng.uniform(-3, 3, (20000, 2)), rng.uniform(-1e-3, 1e-3, (2000, 2)),
+                          [[0, 1], [1, 0], [0, -1], [-1, 0], [1, 1], [-1, -1], [1e-30, 1]]]).astype(np.float32)
+    worst = 0.0
+    for y, x in pts:
+        got = lib.ptref_atan2f(float(y), float(x))
+        ref = np.arctan2(float(y), float(x))
+        ulp = np.spacing(np.float32(max(abs(ref), 2 ** -24)))
+        worst = max(worst, abs(got - ref) / ulp)
+    assert worst <= 3.0, worst
+    assert lib.ptref_atan2f(0.0, 0.0) == 0.0
+    assert abs(lib.ptref_atan2f(0.0, -1.0) - np.pi) < 3e-7
+
+
+@pytest.mark.parametrize("deg", [0, 10, 45, 89, 90, 91, 135, 179, 180, 200, 270, 359])
+def test_tensor_angle_closed_form(deg):
+    """A unit gradient u = (c, s) gives the tensor (c^2, cs, s^2), whose angle
+    is atan2(s, c) + pi/2 folded into [-pi/2, pi/2) (imageIO.h:136-139)."""
+    lib = load_oracle()
+    t = np.deg2rad(deg)
+    c, s = np.float32(np.cos(t)), np.float32(np.sin(t))
+    got = lib.ptref_tensor_to_angle(float(c * c), float(c * s), float(s * s))
+    ref = np.arctan2(2 * float(c * s), float(c * c) - float(s * s)) / 2 + np.pi / 2
+    if ref >= np.pi / 2:
+        ref -= np.pi
+    d = abs(got - ref)
+    d = min(d, abs(d - np.pi))  # the fold point is +-pi/2
+    assert d < 1e-6, (deg, got, ref)
+
+
+def test_zero_tensor_angle():
+    """An absent orientation input is a zero tensor: atan2(0, 0) = 0, so the
+    angle is float(pi/2) folded by pi, i.e. just above -pi/2."""
+    lib = load_oracle()
+    got = lib.ptref_tensor_to_angle(0.0, 0.0, 0.0)
+    assert got == np.float32(float(np.float32(np.pi / 2)) - np.pi)

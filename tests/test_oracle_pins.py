@@ -1,0 +1,382 @@
+"""Pins of the CPU oracle against what the mathematics fixes, so a plausible
+mistake in the restatement (a dropped term, a wrong sign or index, a
+transposed operand, a missing wrap) fails here.  The reference ships no tests,
+fixtures or golden data (see the oracle's header), so these checks, not the
+reference's outputs, anchor the oracle."""
+import numpy as np
+import pytest
+
+from painttween_amd import _ffi
+from painttween_amd import workloads as W
+from tests.harness import Session
+from tests.oracle_ctx import OracleContext
+
+E1, E2, E3, E4 = 1, 2, 3, 4
+
+
+# ---------------------------------------------------------------- weights
+def test_gaussian_weights_printed_values():
+    """allocateGaussianKernel(r, false) (convolution.cu:37-60):
+    k[i] = exp(-i^2 / (2 sigma^2)), sigma = r/3, unnormalised."""
+    with OracleContext() as o:
+        o.setup(o.default_params(residual_window_size=3))
+        k = o.gaussian_weights(3)
+        np.testing.assert_allclose(k, [0.0111090, 0.135335, 0.606531, 1, 0.606531, 0.135335, 0.0111090], rtol=1e-5)
+        np.testing.assert_allclose(k, np.exp(-np.arange(-3, 4) ** 2 / 2.0), rtol=1e-6)
+        o.setup(o.default_params(residual_window_size=2))
+        k = o.gaussian_weights(2)
+        np.testing.assert_allclose(k, [0.0111090, 0.324652, 1, 0.324652, 0.0111090], rtol=1e-5)
+        np.testing.assert_allclose(k, np.exp(-np.arange(-2, 3) ** 2 / (2 * (2 / 3) ** 2)), rtol=1e-6)
+
+
+# ---------------------------------------------------------------- pyramid
+def _box(prev):
+    h, w = prev.shape[0] // 2, prev.shape[1] // 2
+    p = prev[:2 * h, :2 * w].astype(np.float32)
+    s = np.float32(0) + p[0::2, 0::2]
+    s = s + p[0::2, 1::2]
+    s = s + p[1::2, 0::2]
+    s = s + p[1::2, 1::2]
+    return (s * np.float32(0.25)).astype(np.float32)
+
+
+def test_box_pyramid_closed_form_and_odd_sizes():
+    a = np.random.default_rng(1).uniform(0, 1, (37, 53, 4)).astype(np.float32)
+    with OracleContext() as o:
+        p = o.pyramid(E4, a)
+        assert p.levels == 5  # min(floor(log2 53), floor(log2 37))
+        prev = a
+        for l in range(1, p.levels):
+            exp = _box(prev)
+            assert p.dims(l) == (53 >> l, 37 >> l)
+            got = p.download(l)
+            assert got.tobytes() == exp.tobytes(), l
+            prev = exp
+
+
+def test_pyramid_constant_and_velocity_scaling():
+    with OracleContext() as o:
+        c = o.pyramid(E1, np.full((32, 32), 0.375, np.float32))
+        for l in range(c.levels):
+            assert np.all(c.download(l) == np.float32(0.375))
+        v = o.pyramid(E2, np.full((32, 32, 2), 3.0, np.float32), _ffi.PT_DOWNSAMPLE_SCALE_WITH_LEVELS)
+        for l in range(v.levels):
+            assert np.all(v.download(l) == np.float32(3.0 * 0.5 ** l)), l
+
+
+def test_absent_inputs_use_defaults():
+    with OracleContext() as o:
+        d = o.pyramid(E1, None, default=1.0, shape=(16, 8))
+        assert d.dims(0) == (16, 8) and np.all(d.download(0) == 1.0)
+        t = o.pyramid(E3, None, shape=(16, 8))
+        a = o.orientation_pyramid(t)
+        assert np.all(a.download(1) == np.float32(float(np.float32(np.pi / 2)) - np.pi))
+
+
+# ---------------------------------------------------------------- patch distance
+def _bilinear(tex, x, y):
+    h, w = tex.shape[:2]
+    fx, fy = np.floor(x), np.floor(y)
+    a, b = x - fx, y - fy
+    x0, y0 = int(fx), int(fy)
+    xa, xb = np.clip([x0, x0 + 1], 0, w - 1)
+    ya, yb = np.clip([y0, y0 + 1], 0, h - 1)
+    t = tex.astype(np.float64)
+    return ((1 - a) * (1 - b) * t[ya, xa] + a * (1 - b) * t[ya, xb] + (1 - a) * b * t[yb, xa] + a * b * t[yb, xb])
+
+
+def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm):
+    """Independent float64 statement of calculateResidual
+    (texSynth_kernel.cu:537-659) without temporal terms."""
+    r = prm.residual_window_size
+    lh, lw = target.shape[:2]
+    hh, hw = exb.shape[:2]
+    th = float(cand["theta"])
+    su = prm.residual_window_scale_u * float(cand["scale_u"])
+    sv = prm.residual_window_scale_v * float(cand["scale_v"])
+    b0 = (np.cos(th) * su, np.sin(th) * su)
+    b1 = (-np.sin(th) * sv, np.cos(th) * sv)
+    num = den = 0.0
+    for i in range(-r, r + 1):
+        for j in range(-r, r + 1):
+            if not (0 <= x + i < lw and 0 <= y + j < lh):
+                continue
+            tx = float(cand["x"]) + i * b0[0] + j * b1[0]
+            ty = float(cand["y"]) + i * b0[1] + j * b1[1]
+            if tx >= hw:
+                tx -= hw
+            elif tx < 0:
+                tx += hw
+            if ty >= hh:
+                ty -= hh
+            elif ty < 0:
+                ty += hh
+            tb, to = _bilinear(exb, tx, ty), _bilinear(exo, tx, ty)
+            d = target[y + j, x + i].astype(np.float64) - tb
+            res = (d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + prm.alpha_weight * d[3] ** 2) * prm.input_analogy_weight
+            if i == 0 and j == 0:
+                if prm.dist_trans_weight > 0:
+                    de = _bilinear(exdt[..., None], tx, ty)[0] / float(cand["scale_v"])
+                    res += min(max(abs(de - float(imgdt[y, x])), 0.0), 10.0) * prm.dist_trans_weight
+            else:
+                d = canvas[y + j, x + i].astype(np.float64) - to
+                res += (d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + prm.alpha_weight * d[3] ** 2) * prm.canvas_output_weight
+            wgt = float(g[i + r]) * float(g[j + r])
+            num += res * wgt
+            den += wgt
+    return num / den
+
+
+@pytest.mark.parametrize("r,level", [(2, 0), (3, 0), (3, 1)])
+def test_patch_distance_matches_float64_brute_force(r, level):
+    st = W.style_pair(24, 20, seed=41, guides=True)
+    fr = W.still_frame(22, 18, seed=42, guides=True)
+    rng = np.random.default_rng(43)
+    with OracleContext() as o:
+        prm = o.default_params(residual_window_size=r, levels=2, dist_trans_weight=0.7, alpha_weight=1.5,
+                               canvas_output_weight=0.8, input_analogy_weight=1.3, residual_window_scale_u=1.1,
+                               residual_window_scale_v=0.9)
+        s = Session(o, st, [fr], prm)
+        s.run([("style", level), ("frame", level, 0), ("randomize",)])
+        off, canvas, _ = s.state()
+        exb, exo = s.sb.download(level), s.so.download(level)
+        exdt = s.sdt.download(level)
+        target = s.frames[0]["color"].download(level)
+        imgdt = s.frames[0]["dt"].download(level)
+        g = o.gaussian_weights(r)
+        lh, lw = target.shape[:2]
+        pts = [(0, 0), (lw - 1, lh - 1), (0, lh - 1), (lw - 1, 0)] + \
+              [(int(a), int(b)) for a, b in zip(rng.integers(0, lw, 12), rng.integers(0, lh, 12))]
+        for x, y in pts:
+            cand = np.zeros((), _ffi.PATCHXF_DTYPE)
+            cand["x"] = rng.uniform(-3, exb.shape[1] + 3)
+            cand["y"] = rng.uniform(-3, exb.shape[0] + 3)
+            cand["theta"] = rng.uniform(-1.5, 1.5)
+            cand["scale_u"] = rng.uniform(0.8, 1.2)
+            cand["scale_v"] = rng.uniform(0.8, 1.2)
+            got = o.patch_distance(x, y, level, cand)
+            exp = _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm)
+            assert abs(got - exp) <= 3e-6 * max(exp, 1e-3), (x, y, got, exp)
+
+
+def _identity_nnf(w, h, dx=0, dy=0, ew=None, eh=None):
+    ew, eh = ew or w, eh or h
+    n = np.zeros((h, w), _ffi.PATCHXF_DTYPE)
+    yy, xx = np.mgrid[0:h, 0:w]
+    n["x"] = (xx + dx) % ew
+    n["y"] = (yy + dy) % eh
+    n["scale_u"] = 1
+    n["scale_v"] = 1
+    return n
+
+
+def test_identity_offsets_give_zero_energy_and_copy_exemplar():
+    """Target = exemplar input = exemplar output and the identity field: every
+    tap compares a texel with itself, so the residual is exactly 0 and the
+    canvas is the exemplar output exactly (offsetsToOutput, :263-274)."""
+    ex = W.sin_noise(51, 0, 32, 24)
+    with OracleContext() as o:
+        s = Session(o, {"base": ex, "output": ex}, [{"color": ex}],
+                    o.default_params(residual_window_size=3, levels=1, orientation_enabled=0))
+        s.run([("style", 0), ("frame", 0, 0)])
+        o.load_offsets(0, _identity_nnf(32, 24))
+        o.sync_buffers(0, _ffi.PT_DONT_COPY_OFFSETS)
+        _, canvas, res = s.state()
+        assert canvas.tobytes() == ex.tobytes()
+        assert np.all(res == 0.0)
+
+
+def test_periodic_exemplar_integer_shift_wraps_to_zero():
+    """A toroidally periodic exemplar, the target equal to its (dx, dy) shift,
+    and the field x -> x + (dx, dy) mod size: the patch taps wrap by the
+    exemplar size (texSynth_kernel.cu:590-599) and every difference vanishes."""
+    w, h, dx, dy = 32, 32, 7, 29
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    ex = np.stack([0.5 + 0.4 * np.sin(2 * np.pi * (k + 1) * xx / w) * np.cos(2 * np.pi * (k + 2) * yy / h)
+                   for k in range(4)], -1).astype(np.float32)
+    tgt = np.roll(np.roll(ex, -dy, 0), -dx, 1)
+    with OracleContext() as o:
+        s = Session(o, {"base": ex, "output": ex}, [{"color": tgt}],
+                    o.default_params(residual_window_size=3, levels=1, orientation_enabled=0))
+        s.run([("style", 0), ("frame", 0, 0)])
+        o.load_offsets(0, _identity_nnf(w, h, dx, dy))
+        o.sync_buffers(0, _ffi.PT_DONT_COPY_OFFSETS)
+        _, canvas, res = s.state()
+        assert np.all(res == 0.0), res.max()
+
+
+def test_bilinear_is_exact_on_affine_fields():
+    yy, xx = np.mgrid[0:24, 0:32].astype(np.float32)
+    ex = np.stack([xx * 0.25, yy * 0.5, xx * 0.125 + yy * 0.0625, np.full_like(xx, 1.0)], -1).astype(np.float32)
+    nnf = _identity_nnf(16, 16)
+    rng = np.random.default_rng(3)
+    nnf["x"] = rng.uniform(0, 30.5, (16, 16)).astype(np.float32)
+    nnf["y"] = rng.uniform(0, 22.5, (16, 16)).astype(np.float32)
+    with OracleContext() as o:
+        s = Session(o, {"base": ex, "output": ex}, [{"color": W.sin_noise(1, 2, 16, 16)}],
+                    o.default_params(residual_window_size=1, levels=1, orientation_enabled=0))
+        s.run([("style", 0), ("frame", 0, 0)])
+        o.load_offsets(0, nnf)
+        o.sync_buffers(0, _ffi.PT_DONT_COPY_OFFSETS | _ffi.PT_DONT_UPDATE_RESIDUAL)
+        _, canvas, _ = s.state()
+        np.testing.assert_allclose(canvas[..., 0], nnf["x"] * 0.25, atol=2e-6)
+        np.testing.assert_allclose(canvas[..., 1], nnf["y"] * 0.5, atol=2e-6)
+
+
+# ---------------------------------------------------------------- selection
+def _seq(levels, iters):
+    from tests.test_gpu_parity import _still_ops
+    return _still_ops(levels, iters)
+
+
+def test_selection_never_increases_energy_without_canvas_term():
+    """With the canvas term off a pixel's residual depends on its own offset
+    only, so the strict-min selection of propagate and scatter can never raise
+    any pixel's residual."""
+    st = W.style_pair(24, 24, seed=61)
+    fr = W.still_frame(20, 20, seed=62)
+    with OracleContext() as o:
+        s = Session(o, st, [fr], o.default_params(residual_window_size=2, levels=1, canvas_output_weight=0.0))
+        states = s.run(_seq(1, 3))
+    prev = None
+    for name, (_, _, res) in states:
+        if prev is not None and name in ("propagate", "scatter"):
+            assert np.all(res <= prev), name
+        prev = res
+    assert states[-1][1][2].mean() < 0.75 * states[4][1][2].mean()
+
+
+def test_result_independent_of_thread_count():
+    st = W.style_pair(40, 40, seed=71, guides=True)
+    fr = W.still_frame(36, 30, seed=72, guides=True)
+    outs = []
+    for n in (1, 3, 8):
+        with OracleContext(n) as o:
+            outs.append(o.synthesize(o.default_params(residual_window_size=2, levels=2, dist_trans_weight=0.5),
+                                     [fr], st, iterations=2))
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert a.tobytes() == b.tobytes()
+
+
+# ---------------------------------------------------------------- upsample
+def test_upsample_constant_field_and_border_ring():
+    """A constant coarse field (cx, cy, theta 0) gives 2c + 0.5 inside the fine
+    level; the one-pixel ring keeps the working buffer's content, which is
+    still zero here (texSynth_kernel.cu:1461, cudaImageBuffer.h:332-334)."""
+    with OracleContext() as o:
+        s = Session(o, W.style_pair(32, 32, seed=81), [W.still_frame(32, 24, seed=82)],
+                    o.default_params(residual_window_size=2, levels=2, orientation_enabled=0))
+        s.run([("style", 0), ("frame", 0, 0)])
+        coarse = _identity_nnf(16, 12)
+        coarse["x"] = 5.25
+        coarse["y"] = 7.5
+        o.load_offsets(1, coarse)
+        o.upsample(0)
+        off, _, _ = o.get_output(0, 32, 24)
+        inner = off[1:-1, 1:-1]
+        assert np.all(inner["x"] == np.float32(2 * 5.25 + 0.5)) and np.all(inner["y"] == np.float32(2 * 7.5 + 0.5))
+        ring = np.concatenate([off[0], off[-1], off[:, 0], off[:, -1]])
+        assert np.all(ring["x"] == 0) and np.all(ring["y"] == 0) and np.all(ring["scale_u"] == 0)
+
+
+# ---------------------------------------------------------------- advection
+def _advect_setup(o, vel_f_prev, vel_b_cur, w=24, h=20):
+    st = W.style_pair(32, 32, seed=91)
+    f0 = {"color": W.sin_noise(92, 2, w, h), "vel_f": vel_f_prev, "vel_b": np.zeros((h, w, 2), np.float32)}
+    f1 = {"color": W.sin_noise(93, 2, w, h), "vel_f": np.zeros((h, w, 2), np.float32), "vel_b": vel_b_cur}
+    s = Session(o, st, [f0, f1], o.default_params(residual_window_size=2, levels=1, orientation_enabled=0))
+    s.run([("style", 0), ("reset_adv",), ("frame", 0, 0)])
+    nnf = _identity_nnf(w, h)
+    rng = np.random.default_rng(94)
+    nnf["x"] = rng.uniform(0, 30, (h, w)).astype(np.float32)
+    nnf["y"] = rng.uniform(0, 30, (h, w)).astype(np.float32)
+    o.load_offsets(0, nnf)
+    o.sync_buffers(0, _ffi.PT_DONT_COPY_OFFSETS)
+    s.run([("flip",), ("frame", 0, 1), ("advect", 1, 0)])
+    return nnf, o.get_advected(0, 0, w, h)
+
+
+def test_advect_zero_velocity_keeps_offsets():
+    h, w = 20, 24
+    z = np.zeros((h, w, 2), np.float32)
+    with OracleContext() as o:
+        nnf, (adv, base, out, res, ts) = _advect_setup(o, z, z)
+    assert np.all(adv["hysteresis"] == 1) and np.all(ts == 1)
+    assert adv["x"].tobytes() == nnf["x"].tobytes() and adv["y"].tobytes() == nnf["y"].tobytes()
+
+
+def test_advect_integer_translation_shifts_offsets():
+    h, w = 20, 24
+    vf = np.zeros((h, w, 2), np.float32)
+    vf[..., 0], vf[..., 1] = 3, 1
+    vb = -vf
+    with OracleContext() as o:
+        nnf, (adv, base, out, res, ts) = _advect_setup(o, vf, vb)
+    inner = adv[1:, 3:]
+    assert np.all(inner["hysteresis"] == 1)
+    assert inner["x"].tobytes() == nnf["x"][:-1, :-3].tobytes()
+    assert inner["y"].tobytes() == nnf["y"][:-1, :-3].tobytes()
+    assert np.all(adv[0]["hysteresis"] == 0) and np.all(ts[0] == -1)
+
+
+def test_advect_inconsistent_motion_falls_back_to_random():
+    h, w = 20, 24
+    vb = np.zeros((h, w, 2), np.float32)
+    vb[..., 0], vb[..., 1] = -3, -1
+    with OracleContext() as o:
+        nnf, (adv, base, out, res, ts) = _advect_setup(o, np.zeros((h, w, 2), np.float32), vb)
+    assert np.all(adv["hysteresis"] == 0) and np.all(ts == -1)
+    assert np.all(base == 0) and np.all(out == 0)
+
+
+# ---------------------------------------------------------------- merges
+def test_merge_randomly_extremes_and_residual_merge():
+    h, w = 20, 24
+    z = np.zeros((h, w, 2), np.float32)
+    with OracleContext() as o:
+        nnf, (adv, *_r) = _advect_setup(o, z, z)
+        # after advection base == advected offsets; load a different field
+        other = _identity_nnf(w, h)
+        o.load_offsets(0, other)
+        o.merge_randomly(0, True, 0.0)
+        off, _, _ = o.get_output(0, w, h)
+        assert off["x"].tobytes() == other["x"].astype(np.float32).tobytes()
+        o.load_offsets(0, other)
+        o.merge_randomly(0, True, 1.0)
+        off, _, _ = o.get_output(0, w, h)
+        assert off["x"].tobytes() == adv["x"].tobytes()
+        # residual merge: advected residual 0 everywhere beats any positive residual
+        o.load_offsets(0, other)
+        o.load_residual(0, np.full((h, w), 5.0, np.float32))
+        o.merge_using_residual(0, True)
+        off, _, _ = o.get_output(0, w, h)
+        adv_res = o.get_advected(0, 0, w, h)[3]
+        pick_adv = adv_res < 5.0
+        assert np.all(off["hysteresis"][pick_adv] == 1)
+        assert off["x"][pick_adv].tobytes() == adv["x"][pick_adv].tobytes()
+
+
+# ---------------------------------------------------------------- random init
+def test_random_pyramid_ranges_and_alpha_rule():
+    """rand() % (W_ex - 1) never picks the last column/row; fractional parts
+    are rand()/RAND_MAX in [0, 1]; opaque pixels start on opaque exemplar
+    texels when any exist (synthesisFrame.cpp:102-127)."""
+    st = W.style_pair(40, 30, seed=101)
+    st["output"][..., 3] = 0.1
+    st["output"][10:20, 5:25, 3] = 1.0
+    fr = W.still_frame(32, 32, seed=102)
+    fr["color"][..., 3] = 1.0
+    with OracleContext() as o:
+        s = Session(o, st, [fr], o.default_params(residual_window_size=2, levels=1))
+        s.run([("style", 0), ("frame", 0, 0), ("randomize",)])
+        off, _, _ = s.state()
+    ox, oy = np.floor(off["x"]).astype(int), np.floor(off["y"]).astype(int)
+    fx, fy = off["x"] - np.floor(off["x"]), off["y"] - np.floor(off["y"])
+    assert ox.min() >= 0 and ox.max() <= 39 and oy.min() >= 0 and oy.max() <= 29
+    assert fx.min() >= 0 and fx.max() < 1 and fy.min() >= 0 and fy.max() < 1
+    # texel actually chosen is the integer draw (x + rx with rx <= 1)
+    draw_x = np.where(fx == 0, ox - 1, ox).clip(0)
+    draw_y = np.where(fy == 0, oy - 1, oy).clip(0)
+    opaque = (draw_x >= 5) & (draw_x < 25) & (draw_y >= 10) & (draw_y < 20)
+    assert opaque.mean() > 0.99
