@@ -159,6 +159,11 @@ typedef struct pt_pyramid pt_pyramid;
  * rng_mode PT_RNG_REFERENCE, seed 0, frame 0. */
 void pt_params_default(pt_params *out);
 
+/* sizeof of the ABI structs as compiled into the library, for bindings to
+ * check their layouts: 0 pt_params, 1 pt_patchxf, 2 pt_frame_guides,
+ * 3 pt_style_guides; 0 for an unknown id.  No reference counterpart. */
+size_t pt_struct_size(int which);
+
 /* ---- context -------------------------------------------------------------- */
 
 /* Create a context on HIP device `device`.  No reference counterpart (the
@@ -265,6 +270,14 @@ int pt_get_advected(pt_ctx *ctx, int level, int which, pt_patchxf *offsets,
  * every candidate tested, rejected ones included, plus every residual
  * refresh). */
 int pt_pde_count(pt_ctx *ctx, uint64_t *out);
+
+/* ---- execution options (no reference counterpart) ----------------------
+ * PT_TUNE_EARLY_EXIT (default 1): propagate and scatter reject a candidate
+ * as soon as a partial patch energy provably exceeds the best so far (only
+ * when every energy weight is non-negative and the hysteresis weight is 0);
+ * results are bit-identical either way.  0 forces the plain kernels. */
+#define PT_TUNE_EARLY_EXIT 1
+int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --
  * pt_profile_enable(ctx, 1) clears the log and starts recording a pair of HIP

@@ -21,6 +21,7 @@ PT_ADVECT_IGNORE_RANDOM = 0x1
 PT_ADVECT_DO_NOT_CHECK_PREVIOUS = 0x10
 PT_ADVECT_UPDATE_RESIDUAL = 0x80
 PT_ELEM_F32, PT_ELEM_F32X2, PT_ELEM_F32X3, PT_ELEM_F32X4 = 1, 2, 3, 4
+PT_TUNE_EARLY_EXIT = 1
 PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT = 0, 1
 
 LAYER = dict(
@@ -185,6 +186,8 @@ def load_library():
         lib.pt_synthesize_device.restype = I
         lib.pt_synthesize_device.argtypes = [VP, C.POINTER(PtParams), I, I, I, I, I, I, C.POINTER(FrameGuides),
                                              I, I, C.POINTER(StyleGuides), VP, VP, VP]
+        lib.pt_set_tuning.restype = I
+        lib.pt_set_tuning.argtypes = [VP, I, I]
         lib.pt_profile_enable.restype = I
         lib.pt_profile_enable.argtypes = [VP, I]
         lib.pt_profile_get.restype = I

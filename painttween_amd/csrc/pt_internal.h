@@ -50,6 +50,7 @@ struct PtArgs {
     int samples, rng_mode;
     uint32_t key0, key1, call;
     int time_active;   /* some advected offset may be valid */
+    int early_exit;    /* all energy terms non-negative: exact early rejection allowed */
 
     PtNnf base, work;
     float *res_base, *res_work;

@@ -86,16 +86,6 @@ __device__ __forceinline__ int point_index(int w, int h, float x, float y)
     return yi * w + xi;
 }
 
-/* Exemplar base and output share one bilinear footprint: texel i holds
- * {base, output} at ex[2i], ex[2i+1]. */
-__device__ __forceinline__ void tex_ex_pair(const PtArgs &A, float x, float y, float4 &b, float4 &o)
-{
-    Bilin q = bilin(A.ew, A.ew, A.eh, x, y);
-    const float4 *e = A.ex;
-    b = mix4(q, e[2 * q.i00], e[2 * q.i10], e[2 * q.i01], e[2 * q.i11]);
-    o = mix4(q, e[2 * q.i00 + 1], e[2 * q.i10 + 1], e[2 * q.i01 + 1], e[2 * q.i11 + 1]);
-}
-
 __device__ __forceinline__ float4 tex_ex_out(const PtArgs &A, float x, float y)
 {
     Bilin q = bilin(A.ew, A.ew, A.eh, x, y);
@@ -224,16 +214,66 @@ __device__ float hysteresis_term(const PtArgs &A, int x, int y)
     return 0.f;
 }
 
+/* ------------------------------------------------------------------ */
+/* patch distance                                                      */
+/* ------------------------------------------------------------------ */
+
+/* The workgroup's target colour and canvas, staged in LDS with an r-pixel
+ * halo: tile row pitch tw = PT_BX + 2r; the lane's pixel sits at
+ * (lx + r, ly + r). */
+struct Tile {
+    const float4 *col;
+    const float4 *can;
+    int tw, lx, ly;
+};
+
+typedef float pt_f4v __attribute__((ext_vector_type(4)));
+
+/* 16-byte load through a buffer resource at a 32-bit byte offset. */
+__device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, int off)
+{
+    pt_f4v v = __builtin_bit_cast(pt_f4v, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+/* Buffer resource over the interleaved exemplar texture (32 B per texel). */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ex_rsrc(const PtArgs &A)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)A.ex, (short)0, A.ew * A.eh * 32, 0x00020000);
+}
+
+/* Bilinear exemplar base and output at (x, y), clamp addressing; the same
+ * weights and fused order as tex4_lin. */
+__device__ __forceinline__ void ex_pair(__amdgpu_buffer_rsrc_t er, int ew, int eh, float x, float y, float4 &b,
+                                        float4 &o)
+{
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, c = y - fy;
+    int x0 = (int)fx, y0 = (int)fy;
+    int xa = clampi(x0, 0, ew - 1), xb = clampi(x0 + 1, 0, ew - 1);
+    int ya = clampi(y0, 0, eh - 1), yb = clampi(y0 + 1, 0, eh - 1);
+    Bilin q;
+    q.w00 = (1.0f - a) * (1.0f - c);
+    q.w10 = a * (1.0f - c);
+    q.w01 = (1.0f - a) * c;
+    q.w11 = a * c;
+    int ra = ya * ew, rb = yb * ew;
+    int o00 = (ra + xa) << 5, o10 = (ra + xb) << 5, o01 = (rb + xa) << 5, o11 = (rb + xb) << 5;
+    b = mix4(q, ld4(er, o00), ld4(er, o10), ld4(er, o01), ld4(er, o11));
+    o = mix4(q, ld4(er, o00 + 16), ld4(er, o10 + 16), ld4(er, o01 + 16), ld4(er, o11 + 16));
+}
+
 /* timeDerivativeTerms (texSynth_kernel.cu:438-469) */
-__device__ float time_terms(const PtArgs &A, const PtAdv &d, float4 prev, float pb0x, float pb0y, float pb1x,
-                            float pb1y, float4 base, float4 tb, float4 canvas, float4 to, int oo, int i, int j)
+__device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, const PtAdv &d, float4 prev, float pb0x,
+                              float pb0y, float pb1x, float pb1y, float4 base, float4 tb, float4 canvas, float4 to,
+                              int oo, int i, int j)
 {
     float4 prev_base = d.base[oo];
     float4 prev_out = d.out[oo];
     float ptx, pty;
     new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, ptx, pty);
     float4 peb, peo;
-    tex_ex_pair(A, ptx, pty, peb, peo);
+    ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo);
     float r = ssd4(sub4(base, prev_base), sub4(tb, peb)) * A.tdi;
     r = fmaf(ssd4(sub4(canvas, prev_out), sub4(to, peo)), A.tdo, r);
     r = fmaf(ssd4(prev_out, to), A.tcw, r);
@@ -243,16 +283,20 @@ __device__ float time_terms(const PtArgs &A, const PtAdv &d, float4 prev, float 
 /*
  * calculateResidual (texSynth_kernel.cu:537-659): one patch distance of
  * candidate (t, tsc) at pixel (x, y).  Taps run i over x outer, j over y
- * inner, accumulated sequentially per lane in that order.  TIME selects the
- * variant with the temporal terms (an advected offset is valid at the
- * centre); both variants perform the same operations when those terms are
- * inactive.
+ * inner and are accumulated sequentially per lane in that order.  TIME
+ * selects the variant with the temporal terms (an advected offset is valid at
+ * the centre); both variants perform the same operations when those terms are
+ * inactive.  Target and canvas come from the LDS tile; i*b0 is formed once
+ * per row, exactly the product the reference forms per tap.
  */
 template <int RT, bool TIME>
-__device__ float calc_residual_impl(const PtArgs &A, int x, int y, float4 t, float2 tsc, float4 pf, float4 pb,
-                                    bool cpv, bool cnv)
+__device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc, float4 pf,
+                                    float4 pb, bool cpv, bool cnv)
 {
     const int r = RT > 0 ? RT : A.r;
+    __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
+    const int ew = A.ew, eh = A.eh;
+    const float hw = (float)A.hw, hh = (float)A.hh;
     int o = y * A.W + x;
     bool any_time_w = A.tdi > 0.f || A.tdo > 0.f || A.tcw > 0.f;
     bool fwd_dir = A.direction == PT_FORWARD || A.direction == PT_BIDIRECTIONAL;
@@ -275,32 +319,34 @@ __device__ float calc_residual_impl(const PtArgs &A, int x, int y, float4 t, flo
     }
     float b0x, b0y, b1x, b1y;
     basis(t.z, A.su * tsc.x, A.sv * tsc.y, b0x, b0y, b1x, b1y);
-    const float4 *color = A.color[A.cur];
 #pragma unroll 1
     for (int i = -r; i <= r; i++) {
         if (x + i < 0 || x + i >= A.lw)
             continue;
+        const float ibx = (float)i * b0x, iby = (float)i * b0y;
+        const float4 *tcol = T.col + (T.ly + r) * T.tw + (T.lx + r + i);
+        const float4 *tcan = T.can + (T.ly + r) * T.tw + (T.lx + r + i);
 #pragma unroll
         for (int j = -r; j <= r; j++) {
             if (y + j < 0 || y + j >= A.lh)
                 continue;
-            int oo = (y + j) * A.W + (x + i);
-            float4 base = color[oo];
-            float4 canvas = A.canvas[oo];
-            float tx, ty;
-            new_coord((float)i, (float)j, t.x, t.y, b0x, b0y, b1x, b1y, tx, ty);
-            if (tx >= (float)A.hw) tx -= (float)A.hw;
-            else if (tx < 0) tx += (float)A.hw;
-            if (ty >= (float)A.hh) ty -= (float)A.hh;
-            else if (ty < 0) ty += (float)A.hh;
+            float4 base = tcol[j * T.tw];
+            float4 canvas = tcan[j * T.tw];
+            float tx = t.x + (ibx + (float)j * b1x);
+            float ty = t.y + (iby + (float)j * b1y);
+            if (tx >= hw) tx -= hw;
+            else if (tx < 0) tx += hw;
+            if (ty >= hh) ty -= hh;
+            else if (ty < 0) ty += hh;
             float4 tb, to;
-            tex_ex_pair(A, tx, ty, tb, to);
+            ex_pair(er, ew, eh, tx, ty, tb, to);
             float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
             if (TIME && any_time_w) {
+                int oo = (y + j) * A.W + (x + i);
                 if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
-                    res = res + time_terms(A, A.adv[0], pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo, i, j);
+                    res = res + time_terms_b(A, er, A.adv[0], pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo, i, j);
                 if (bwd_dir && cnv && A.adv[1].xyth[oo].w > 0)
-                    res = res + time_terms(A, A.adv[1], pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo, i, j);
+                    res = res + time_terms_b(A, er, A.adv[1], pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo, i, j);
             }
             if (i == 0 && j == 0) {
                 if (A.dt_w > 0.f) {
@@ -328,22 +374,23 @@ __device__ float calc_residual_impl(const PtArgs &A, int x, int y, float4 t, flo
  * offset is valid (the advected buffers were reset and not written since);
  * the per-pixel test then always takes the plain variant. */
 template <int RT, bool TK>
-__device__ __forceinline__ float calc_residual(const PtArgs &A, int x, int y, float4 t, float2 tsc)
+__device__ __forceinline__ float calc_residual(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc)
 {
     if (TK) {
         int o = y * A.W + x;
         float4 pf = A.adv[0].xyth[o], pb = A.adv[1].xyth[o];
         bool cpv = pf.w > 0, cnv = pb.w > 0;
         if (cpv || cnv)
-            return calc_residual_impl<RT, true>(A, x, y, t, tsc, pf, pb, cpv, cnv);
+            return calc_residual_impl<RT, true>(A, T, x, y, t, tsc, pf, pb, cpv, cnv);
     }
     float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    return calc_residual_impl<RT, false>(A, x, y, t, tsc, z, z, false, false);
+    return calc_residual_impl<RT, false>(A, T, x, y, t, tsc, z, z, false, false);
 }
 
 /* test_offset (texSynth_kernel.cu:695-721) */
 template <int RT, bool TK>
-__device__ __forceinline__ float test_offset(const PtArgs &A, int x, int y, float4 t, float2 tsc, float image_alpha)
+__device__ __forceinline__ float test_offset(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc,
+                                             float image_alpha)
 {
     if (!exemplar_culled(A, t.x, t.y)) {
         if (!A.transp) {
@@ -351,7 +398,7 @@ __device__ __forceinline__ float test_offset(const PtArgs &A, int x, int y, floa
             if (ea < 0.99f && image_alpha > 0.99f)
                 return 1e31f;
         }
-        return calc_residual<RT, TK>(A, x, y, t, tsc);
+        return calc_residual<RT, TK>(A, T, x, y, t, tsc);
     }
     return 1e31f;
 }
@@ -366,18 +413,52 @@ __device__ __forceinline__ bool pixel_of(const PtArgs &A, int &x, int &y)
     return x < A.lw && y < A.lh;
 }
 
+template <int RT> struct TileSize {
+    static constexpr int R = RT > 0 ? RT : PT_MAX_R;
+    static constexpr int N = (PT_BX + 2 * R) * (PT_BY + 2 * R);
+};
+
+/* Stages the block's target colour and canvas (with an r halo, coordinates
+ * clamped into the level; clamped entries are never read because taps
+ * outside the level are skipped) and returns the lane's view. */
+template <int RT>
+__device__ __forceinline__ Tile stage_tile(const PtArgs &A, float4 *scol, float4 *scan)
+{
+    const int r = RT > 0 ? RT : A.r;
+    const int tw = PT_BX + 2 * r, th = PT_BY + 2 * r;
+    const int x0 = blockIdx.x * PT_BX - r, y0 = blockIdx.y * PT_BY - r;
+    const float4 *color = A.color[A.cur];
+    for (int k = threadIdx.y * PT_BX + threadIdx.x; k < tw * th; k += PT_BX * PT_BY) {
+        int tx = clampi(x0 + k % tw, 0, A.lw - 1);
+        int ty = clampi(y0 + k / tw, 0, A.lh - 1);
+        scol[k] = color[ty * A.W + tx];
+        scan[k] = A.canvas[ty * A.W + tx];
+    }
+    __syncthreads();
+    Tile T;
+    T.col = scol;
+    T.can = scan;
+    T.tw = tw;
+    T.lx = threadIdx.x;
+    T.ly = threadIdx.y;
+    return T;
+}
+
 /* updateResidualCache_kernel (texSynth_kernel.cu:661-693) */
 template <int RT, bool TK>
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_update_residual(PtArgs A)
 {
+    __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    bool valid = pixel_of(A, x, y);
+    Tile T = stage_tile<RT>(A, scol, scan);
+    if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) {
         A.res_base[o] = 0.0f;
         return;
     }
-    A.res_base[o] = calc_residual<RT, TK>(A, x, y, A.base.xyth[o], A.base.sc[o]);
+    A.res_base[o] = calc_residual<RT, TK>(A, T, x, y, A.base.xyth[o], A.base.sc[o]);
 }
 
 /* offsetsToOutput (texSynth_kernel.cu:263-274) */
@@ -424,8 +505,11 @@ __device__ __forceinline__ void propagate_offset(const PtArgs &A, int tx, int ty
 template <int RT, bool TK>
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
 {
+    __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    bool valid = pixel_of(A, x, y);
+    Tile T = stage_tile<RT>(A, scol, scan);
+    if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) {
         A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -436,6 +520,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
     float ia = A.color[A.cur][o].w;
     float4 ow = A.base.xyth[o];
     float2 osc = A.base.sc[o];
+#pragma unroll 1
     for (int i = 0; i < 4; i++) {
         int tx = x, ty = y;
         if (i == 0) tx = x + jump;
@@ -445,7 +530,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
         float4 c;
         float2 csc;
         propagate_offset(A, tx, ty, (float)x, (float)y, c, csc);
-        float r = test_offset<RT, TK>(A, x, y, c, csc, ia);
+        float r = test_offset<RT, TK>(A, T, x, y, c, csc, ia);
         if (r < best) {
             ow = c;
             osc = csc;
@@ -464,8 +549,11 @@ __device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * (1.0
 template <int RT, bool TK>
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter(PtArgs A)
 {
+    __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    bool valid = pixel_of(A, x, y);
+    Tile T = stage_tile<RT>(A, scol, scan);
+    if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) return;
     float4 off = A.base.xyth[o];
@@ -495,7 +583,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter(PtArgs A)
         clamp_ex(ex, ey, A.ew, A.eh);
         float ang = screen_to_ex_angle(A, (float)x, (float)y, ex, ey);
         float4 c = make_float4(ex, ey, ang, 0.f);
-        float res = test_offset<RT, TK>(A, x, y, c, sc, ia);
+        float res = test_offset<RT, TK>(A, T, x, y, c, sc, ia);
         if (res < best) {
             ow = c;
             osc = sc;
@@ -518,11 +606,343 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter(PtArgs A)
     ay /= (float)sum;
     float4 avg = make_float4(ax, ay, screen_to_ex_angle(A, (float)x, (float)y, ax, ay), 1.f);
     float2 avgsc = make_float2(1.f, 1.f);
-    float res = test_offset<RT, TK>(A, x, y, avg, avgsc, ia);
+    float res = test_offset<RT, TK>(A, T, x, y, avg, avgsc, ia);
     if (res < best) {
         ow = avg;
         osc = avgsc;
         best = res;
+    }
+    if (best < cached) {
+        A.work.xyth[o] = ow;
+        A.work.sc[o] = osc;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* exact early rejection of candidates                                 */
+/* ------------------------------------------------------------------ */
+/*
+ * With every weight non-negative and no hysteresis term, each tap adds a
+ * non-negative amount to the patch energy, so any partial sum is a lower
+ * bound of the reference's in-order total.  A candidate is rejected as soon
+ * as a partial sum reaches T = best * sum_w * (1 + 1e-4); the margin exceeds
+ * the worst-case relative error of two FP32 recursive sums of (2r+1)^2 <= 225
+ * non-negative terms (about 1.6e-5 at r = 7) plus the rounding of T, so a
+ * rejected candidate's in-order value r = sum_r / sum_w is provably >= best
+ * and the reference's strict `r < best` test would have rejected it too.
+ * Survivors are re-evaluated in the reference order (phase 1), giving the
+ * bit-identical value the plain kernels compute.
+ *
+ * Phase 0 evaluates the three heaviest rows (i = 0, -1, +1: ~88% of the
+ * Gaussian weight) first; phase 1 runs rows -r..r in order with the same
+ * bound check after each row.  Lanes advance through their own candidates
+ * row by row, so a wave does not wait on its slowest candidate.
+ */
+struct Cand {
+    float4 t;   /* x, y, theta, hysteresis */
+    float2 sc;  /* scaleU, scaleV */
+    float b0x, b0y, b1x, b1y;
+};
+
+__device__ __forceinline__ float ee_threshold(float best, float sw) { return best * sw * 1.0001f; }
+
+/* Sum of the in-bounds tap weights of pixel (x, y), in the reference order. */
+__device__ __forceinline__ float tap_weight_sum(const PtArgs &A, int r, int x, int y)
+{
+    float sw = 0.f;
+    for (int i = -r; i <= r; i++) {
+        if (x + i < 0 || x + i >= A.lw) continue;
+        for (int j = -r; j <= r; j++) {
+            if (y + j < 0 || y + j >= A.lh) continue;
+            sw += A.gauss[i + r] * A.gauss[j + r];
+        }
+    }
+    return sw;
+}
+
+/* One row (fixed i) of calculateResidual's tap loop, accumulated into
+ * (acc, swa) with the reference's per-tap arithmetic. */
+template <int RT, bool TIME>
+__device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
+                                          int i, const Cand &c, float4 pf, float4 pb, bool cpv, bool cnv,
+                                          const float *tb4, float &acc, float &swa)
+{
+    const int r = RT > 0 ? RT : A.r;
+    const int ew = A.ew, eh = A.eh;
+    const float hw = (float)A.hw, hh = (float)A.hh;
+    const float gi = A.gauss[i + r];
+    const float ibx = (float)i * c.b0x, iby = (float)i * c.b0y;
+    const float4 *tcol = T.col + (T.ly + r) * T.tw + (T.lx + r + i);
+    const float4 *tcan = T.can + (T.ly + r) * T.tw + (T.lx + r + i);
+#pragma unroll
+    for (int j = -r; j <= r; j++) {
+        if (y + j < 0 || y + j >= A.lh)
+            continue;
+        float4 base = tcol[j * T.tw];
+        float4 canvas = tcan[j * T.tw];
+        float tx = c.t.x + (ibx + (float)j * c.b1x);
+        float ty = c.t.y + (iby + (float)j * c.b1y);
+        if (tx >= hw) tx -= hw;
+        else if (tx < 0) tx += hw;
+        if (ty >= hh) ty -= hh;
+        else if (ty < 0) ty += hh;
+        float4 tb, to;
+        ex_pair(er, ew, eh, tx, ty, tb, to);
+        float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
+        if (TIME) {
+            bool any_time_w = A.tdi > 0.f || A.tdo > 0.f || A.tcw > 0.f;
+            bool fwd_dir = A.direction == PT_FORWARD || A.direction == PT_BIDIRECTIONAL;
+            bool bwd_dir = A.direction == PT_BACKWARD || A.direction == PT_BIDIRECTIONAL;
+            if (any_time_w) {
+                int oo = (y + j) * A.W + (x + i);
+                if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
+                    res = res + time_terms_b(A, er, A.adv[0], pf, tb4[0], tb4[1], tb4[2], tb4[3], base, tb, canvas,
+                                             to, oo, i, j);
+                if (bwd_dir && cnv && A.adv[1].xyth[oo].w > 0)
+                    res = res + time_terms_b(A, er, A.adv[1], pb, tb4[4], tb4[5], tb4[6], tb4[7], base, tb, canvas,
+                                             to, oo, i, j);
+            }
+        }
+        if (i == 0 && j == 0) {
+            if (A.dt_w > 0.f) {
+                float de = tex1_lin(A.ex_dt, A.ew, A.ew, A.eh, tx, ty);
+                de /= c.sc.y;
+                float di = A.dt[A.cur][y * A.W + x];
+                float diff = fabsf(de - di);
+                diff = fminf(fmaxf(diff, 0.0f), 10.0f);
+                res = res + diff * A.dt_w;
+            }
+        } else {
+            res = fmaf(ssd_alpha(canvas, to, A.alpha_w), A.canvas_w, res);
+        }
+        float wgt = gi * A.gauss[j + r];
+        acc = fmaf(res, wgt, acc);
+        swa += wgt;
+    }
+}
+
+/* Temporal bases of the advected offsets at (x, y) (texSynth_kernel.cu:552-557). */
+template <bool TIME>
+__device__ __forceinline__ void time_setup(const PtArgs &A, int x, int y, float4 &pf, float4 &pb, bool &cpv,
+                                           bool &cnv, float *tb4)
+{
+    for (int k = 0; k < 8; k++) tb4[k] = 0.f;
+    cpv = cnv = false;
+    pf = pb = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!TIME) return;
+    int o = y * A.W + x;
+    pf = A.adv[0].xyth[o];
+    pb = A.adv[1].xyth[o];
+    cpv = pf.w > 0;
+    cnv = pb.w > 0;
+    if (cpv) {
+        float2 ps = A.adv[0].sc[o];
+        float ang = screen_to_ex_angle(A, (float)x, (float)y, pf.x, pf.y);
+        basis(ang, A.su * ps.x, A.sv * ps.y, tb4[0], tb4[1], tb4[2], tb4[3]);
+    }
+    if (cnv) {
+        float2 ps = A.adv[1].sc[o];
+        float ang = screen_to_ex_angle(A, (float)x, (float)y, pb.x, pb.y);
+        basis(ang, A.su * ps.x, A.sv * ps.y, tb4[4], tb4[5], tb4[6], tb4[7]);
+    }
+}
+
+/* Candidate acceptance test of test_offset (texSynth_kernel.cu:695-721)
+ * before the patch distance: false means the candidate scores 1e31. */
+__device__ __forceinline__ bool candidate_testable(const PtArgs &A, float4 t, float image_alpha)
+{
+    if (exemplar_culled(A, t.x, t.y)) return false;
+    if (!A.transp) {
+        float ea = tex_ex_out_alpha(A, t.x, t.y);
+        if (ea < 0.99f && image_alpha > 0.99f) return false;
+    }
+    return true;
+}
+
+/*
+ * Runs one lane's candidate stream through the two-phase evaluation.  NEXT
+ * produces candidate k (returns false when there are no more) and may read
+ * the current winner; ACCEPT records a winner.  `best` is updated in place.
+ */
+template <int RT, bool TIME, class Next, class Accept>
+__device__ __forceinline__ void ee_search(const PtArgs &A, const Tile &T, int x, int y, float &best, Next next,
+                                          Accept accept)
+{
+    const int r = RT > 0 ? RT : A.r;
+    __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
+    float4 pf, pb;
+    bool cpv, cnv;
+    float tb4[8];
+    time_setup<TIME>(A, x, y, pf, pb, cpv, cnv, tb4);
+    const float sw = tap_weight_sum(A, r, x, y);
+    float T0 = ee_threshold(best, sw);
+    Cand c;
+    int ph = 0, step = 0;
+    float acc = 0.f, swa = 0.f;
+    bool need = true;
+    while (true) {
+        if (need) {
+            if (!next(c)) break;
+            if (!(best > 0.f)) continue; /* nothing can score below a non-positive best */
+            basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
+            ph = 0;
+            step = 0;
+            acc = 0.f;
+            swa = 0.f;
+            need = false;
+        }
+        int i = ph == 0 ? (step == 0 ? 0 : (step == 1 ? -1 : 1)) : step - r;
+        if (x + i >= 0 && x + i < A.lw)
+            row_accum<RT, TIME>(A, T, er, x, y, i, c, pf, pb, cpv, cnv, tb4, acc, swa);
+        if (acc >= T0) {
+            need = true;
+        } else if (ph == 0 && step == 2) {
+            ph = 1;
+            step = 0;
+            acc = 0.f;
+            swa = 0.f;
+        } else if (ph == 1 && step == 2 * r) {
+            float rr = (swa < 0.000001f) ? 1e31f : acc / swa;
+            if (rr < best) {
+                best = rr;
+                T0 = ee_threshold(best, sw);
+                accept(c);
+            }
+            need = true;
+        } else {
+            step++;
+        }
+    }
+}
+
+/* propagate_kernel (texSynth_kernel.cu:1112-1219) with exact early rejection */
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate_ee(PtArgs A, int jump)
+{
+    __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
+    int x, y;
+    bool valid = pixel_of(A, x, y);
+    Tile T = stage_tile<RT>(A, scol, scan);
+    if (!valid) return;
+    int o = y * A.W + x;
+    if (image_culled(A, x, y)) {
+        A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+        A.work.sc[o] = make_float2(1.f, 1.f);
+        return;
+    }
+    float best = A.res_base[o];
+    float ia = A.color[A.cur][o].w;
+    float4 ow = A.base.xyth[o];
+    float2 osc = A.base.sc[o];
+    int kk = 0;
+    auto next_seq = [&](Cand &c) -> bool {
+        while (kk < 4) {
+            int k = kk++;
+            int tx = x, ty = y;
+            if (k == 0) tx = x + jump;
+            else if (k == 1) tx = x - jump;
+            else if (k == 2) ty = y + jump;
+            else ty = y - jump;
+            propagate_offset(A, tx, ty, (float)x, (float)y, c.t, c.sc);
+            if (candidate_testable(A, c.t, ia)) return true;
+        }
+        return false;
+    };
+    auto accept = [&](const Cand &c) {
+        ow = c.t;
+        osc = c.sc;
+    };
+    if (TK) {
+        float4 pf = A.adv[0].xyth[o], pb = A.adv[1].xyth[o];
+        if (pf.w > 0 || pb.w > 0)
+            ee_search<RT, true>(A, T, x, y, best, next_seq, accept);
+        else
+            ee_search<RT, false>(A, T, x, y, best, next_seq, accept);
+    } else {
+        ee_search<RT, false>(A, T, x, y, best, next_seq, accept);
+    }
+    A.work.xyth[o] = ow;
+    A.work.sc[o] = osc;
+}
+
+/* scatter_kernel (texSynth_kernel.cu:865-952) with exact early rejection.
+ * Sample k is generated only after samples < k are decided, from the winner
+ * so far, exactly as in the reference's sequential loop. */
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter_ee(PtArgs A)
+{
+    __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
+    int x, y;
+    bool valid = pixel_of(A, x, y);
+    Tile T = stage_tile<RT>(A, scol, scan);
+    if (!valid) return;
+    int o = y * A.W + x;
+    if (image_culled(A, x, y)) return;
+    float cached = A.res_base[o];
+    float best = (float)((double)cached + 10e-7);
+    float4 ow = A.base.xyth[o];
+    float2 osc = A.base.sc[o];
+    float ia = A.color[A.cur][o].w;
+    const float factor = 0.9f;
+    float dist = (float)(A.ew > A.eh ? A.ew : A.eh);
+    float2 sc = scale_at(A, (float)x, (float)y);
+    int i = 0;          /* next random sample */
+    bool avg_done = false;
+    auto next = [&](Cand &c) -> bool {
+        while (i < A.samples && dist * factor >= 10.f) {
+            float r = 0.0f, theta = 0.0f;
+            if (A.rng_mode == PT_RNG_COUNTER) {
+                uint32_t ctr[4] = {(uint32_t)x, (uint32_t)y, A.call, (uint32_t)i}, key[2] = {A.key0, A.key1}, rnd[4];
+                pt_philox4x32_10(ctr, key, rnd);
+                r = u24(rnd[0]);
+                theta = (float)((double)u24(rnd[1]) * 0.63661977236758134308);
+            }
+            float dmax = dist, dmin = dist * factor;
+            float s, co;
+            pt_sincosf(theta, &s, &co);
+            float len = r * (dmax - dmin) + dmin;
+            float ex = ow.x + co * len;
+            float ey = ow.y + s * len;
+            clamp_ex(ex, ey, A.ew, A.eh);
+            float ang = screen_to_ex_angle(A, (float)x, (float)y, ex, ey);
+            c.t = make_float4(ex, ey, ang, 0.f);
+            c.sc = sc;
+            dist *= factor;
+            i++;
+            if (candidate_testable(A, c.t, ia)) return true;
+        }
+        if (!avg_done) {
+            avg_done = true;
+            float ax = 0.f, ay = 0.f;
+            int sum = 0;
+            for (int a = -1; a <= 1; a++)
+                for (int b = -1; b <= 1; b++) {
+                    if (!in_bounds(x + a, y + b, A.lw, A.lh, 0) || (a == 0 && b == 0)) continue;
+                    float4 n = A.base.xyth[(y + b) * A.W + (x + a)];
+                    ax += n.x;
+                    ay += n.y;
+                    sum++;
+                }
+            ax /= (float)sum;
+            ay /= (float)sum;
+            c.t = make_float4(ax, ay, screen_to_ex_angle(A, (float)x, (float)y, ax, ay), 1.f);
+            c.sc = make_float2(1.f, 1.f);
+            if (candidate_testable(A, c.t, ia)) return true;
+        }
+        return false;
+    };
+    auto accept = [&](const Cand &c) {
+        ow = c.t;
+        osc = c.sc;
+    };
+    if (TK) {
+        float4 pf = A.adv[0].xyth[o], pb = A.adv[1].xyth[o];
+        if (pf.w > 0 || pb.w > 0)
+            ee_search<RT, true>(A, T, x, y, best, next, accept);
+        else
+            ee_search<RT, false>(A, T, x, y, best, next, accept);
+    } else {
+        ee_search<RT, false>(A, T, x, y, best, next, accept);
     }
     if (best < cached) {
         A.work.xyth[o] = ow;
@@ -831,13 +1251,15 @@ hipError_t pt_launch_copy_random(const PtArgs &a, hipStream_t s)
 
 hipError_t pt_launch_propagate(const PtArgs &a, int jump, hipStream_t s)
 {
-    PT_DISPATCH_R(k_propagate, a, jump);
+    if (a.early_exit) PT_DISPATCH_R(k_propagate_ee, a, jump);
+    else PT_DISPATCH_R(k_propagate, a, jump);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_scatter(const PtArgs &a, hipStream_t s)
 {
-    PT_DISPATCH_R(k_scatter, a);
+    if (a.early_exit) PT_DISPATCH_R(k_scatter_ee, a);
+    else PT_DISPATCH_R(k_scatter, a);
     return hipGetLastError();
 }
 

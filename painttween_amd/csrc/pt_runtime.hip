@@ -102,6 +102,7 @@ struct pt_ctx {
     float4 *canvas = nullptr;
     DevAdv adv[2];
     bool adv_written[2] = {false, false}; /* advected set k may hold valid offsets */
+    bool no_early_exit = false;            /* force the plain candidate kernels (tests) */
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -244,6 +245,17 @@ extern "C" void pt_params_default(pt_params *p)
 }
 
 extern "C" const char *pt_global_last_error(void) { return g_last_error; }
+
+extern "C" size_t pt_struct_size(int which)
+{
+    switch (which) {
+    case 0: return sizeof(pt_params);
+    case 1: return sizeof(pt_patchxf);
+    case 2: return sizeof(pt_frame_guides);
+    case 3: return sizeof(pt_style_guides);
+    default: return 0;
+    }
+}
 
 extern "C" const char *pt_last_error(pt_ctx *c) { return c ? c->err.c_str() : g_last_error; }
 
@@ -400,6 +412,18 @@ extern "C" int pt_profile_get(pt_ctx *c, const char *kernel, uint64_t *launches,
         *pixels += r.pixels;
     }
     return PT_OK;
+}
+
+extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
+{
+    if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
+    switch (key) {
+    case PT_TUNE_EARLY_EXIT:
+        c->no_early_exit = (value == 0);
+        return PT_OK;
+    default:
+        return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
+    }
 }
 
 extern "C" int pt_upload_parameters(pt_ctx *c, const pt_params *p)
@@ -829,6 +853,11 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.key0 = p.seed ^ 0x5CA77E12u;
     a.key1 = (uint32_t)(p.frame_number * level);
     a.time_active = (c->adv_written[0] || c->adv_written[1]) ? 1 : 0;
+    a.early_exit = (p.hysteresis_weight == 0.f && p.input_analogy_weight >= 0.f && p.canvas_output_weight >= 0.f &&
+                    p.alpha_weight >= 0.f && p.dist_trans_weight >= 0.f && p.time_derivative_input_weight >= 0.f &&
+                    p.time_derivative_output_weight >= 0.f && p.temporal_coherence_weight >= 0.f && !c->no_early_exit)
+                       ? 1
+                       : 0;
     a.base = c->base;
     a.work = c->work;
     a.res_base = c->res_base;

@@ -117,6 +117,10 @@ class Context:
     def default_params(self, **overrides):
         return default_params(self.lib, self.prefix, **overrides)
 
+    def set_tuning(self, key, value):
+        """Execution options of the HIP library (pt_set_tuning)."""
+        self._check(self.lib.pt_set_tuning(self.handle, key, value))
+
     # -- TexSynth:: -----------------------------------------------------
     def setup(self, params):
         self.call("setup", C.byref(params))

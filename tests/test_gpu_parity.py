@@ -21,8 +21,9 @@ def gpu():
     ctx.close()
 
 
-def _both(style, frames, overrides, ops):
+def _both(style, frames, overrides, ops, early_exit=1):
     g = Context.gpu(0)
+    g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, early_exit)
     o = OracleContext()
     try:
         sg = Session(g, style, frames, g.default_params(**overrides))
@@ -66,12 +67,14 @@ def _still_ops(levels, iterations=2):
     return ops
 
 
-@pytest.mark.parametrize("r,orient,dtw", [(2, False, 0.0), (3, True, 0.5), (1, True, 0.0), (4, False, 0.0)])
-def test_still_operator_sequence(r, orient, dtw):
+@pytest.mark.parametrize("early_exit", [1, 0])
+@pytest.mark.parametrize("r,orient,dtw", [(2, False, 0.0), (3, True, 0.5), (1, True, 0.0), (4, False, 0.0),
+                                          (5, True, 0.0)])
+def test_still_operator_sequence(r, orient, dtw, early_exit):
     st = W.style_pair(40, 36, seed=5, guides=orient or dtw > 0)
     fr = W.still_frame(44, 34, seed=6, guides=orient or dtw > 0)
     ov = dict(residual_window_size=r, levels=3, orientation_enabled=int(orient), dist_trans_weight=dtw)
-    ga, oa = _both(st, [fr], ov, _still_ops(3))
+    ga, oa = _both(st, [fr], ov, _still_ops(3), early_exit)
     assert_same_states(ga, oa)
 
 
@@ -113,13 +116,27 @@ def _anim_ops(levels, frames, scheme_merge="merge_random"):
     return ops
 
 
-@pytest.mark.parametrize("merge", ["merge_random", "merge_residual"])
-def test_animation_operator_sequence(merge):
+@pytest.mark.parametrize("merge,hyst", [("merge_random", 0.3), ("merge_residual", 0.3), ("merge_random", 0.0)])
+def test_animation_operator_sequence(merge, hyst):
     st = W.style_pair(48, 48, seed=12, guides=True)
     frames = W.animation(40, 32, 3, seed=13)
-    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=0.3)
+    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=hyst)
     ga, oa = _both(st, frames, ov, _anim_ops(2, 3, merge))
     assert_same_states(ga, oa)
+
+
+def test_early_exit_matches_plain_kernels_at_scale():
+    """The early-rejection kernels against the plain ones on a 256^2 still
+    with guides: offsets, canvas and residual identical."""
+    st = W.style_pair(256, 256, seed=51, guides=True)
+    fr = W.still_frame(256, 256, seed=52, guides=True)
+    outs = []
+    for ee in (1, 0):
+        with Context.gpu(0) as g:
+            g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, ee)
+            p = g.default_params(residual_window_size=3, levels=4, dist_trans_weight=0.5)
+            outs.append(g.synthesize(p, [fr], st, iterations=4))
+    _cmp_synth(outs[0], outs[1])
 
 
 def _cmp_synth(g_out, o_out):
