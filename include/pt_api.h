@@ -277,6 +277,10 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * when every energy weight is non-negative and the hysteresis weight is 0);
  * results are bit-identical either way.  0 forces the plain kernels. */
 #define PT_TUNE_EARLY_EXIT 1
+/* Early rejection is used on levels with at least this many pixels
+ * (default 512*512); smaller levels have too few waves to hide the longer
+ * per-lane dependency chain. */
+#define PT_TUNE_EARLY_EXIT_MIN_PIXELS 2
 int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --

@@ -71,6 +71,7 @@ struct PtArgs {
 /* Host-side launchers (pt_kernels.hip).  All run on `stream`. */
 hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s);
 hipError_t pt_launch_canvas(const PtArgs &a, hipStream_t s);
+hipError_t pt_launch_sync_canvas(const PtArgs &a, int dw, int dh, hipStream_t s);
 hipError_t pt_launch_copy_random(const PtArgs &a, hipStream_t s);
 hipError_t pt_launch_propagate(const PtArgs &a, int jump, hipStream_t s);
 hipError_t pt_launch_scatter(const PtArgs &a, hipStream_t s);

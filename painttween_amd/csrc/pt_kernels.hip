@@ -471,6 +471,29 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_canvas(PtArgs A)
     A.canvas[o] = tex_ex_out(A, n.x, n.y);
 }
 
+/* syncBuffers' copyWorkingToBase + offsetsToOutput in one pass
+ * (texSynth_kernel.cu:1244-1246,1274-1275): pixels of the copy rectangle
+ * (dw x dh) take working -> base; pixels of the level rectangle then get
+ * the canvas of their (new) base offset. */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_sync_canvas(PtArgs A, int dw, int dh)
+{
+    int x = blockIdx.x * PT_BX + threadIdx.x;
+    int y = blockIdx.y * PT_BY + threadIdx.y;
+    int o = y * A.W + x;
+    float4 n;
+    bool have = false;
+    if (x < dw && y < dh) {
+        n = A.work.xyth[o];
+        A.base.xyth[o] = n;
+        A.base.sc[o] = A.work.sc[o];
+        have = true;
+    }
+    if (x < A.lw && y < A.lh) {
+        if (!have) n = A.base.xyth[o];
+        A.canvas[o] = tex_ex_out(A, n.x, n.y);
+    }
+}
+
 /* copyRandomKernel (texSynth_kernel.cu:2044-2059) */
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A)
 {
@@ -1240,6 +1263,13 @@ hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s)
 hipError_t pt_launch_canvas(const PtArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(k_canvas, grid_for(a.lw, a.lh), kBlock, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_sync_canvas(const PtArgs &a, int dw, int dh, hipStream_t s)
+{
+    int gw = dw > a.lw ? dw : a.lw, gh = dh > a.lh ? dh : a.lh;
+    hipLaunchKernelGGL(k_sync_canvas, grid_for(gw, gh), kBlock, 0, s, a, dw, dh);
     return hipGetLastError();
 }
 

@@ -103,6 +103,9 @@ struct pt_ctx {
     DevAdv adv[2];
     bool adv_written[2] = {false, false}; /* advected set k may hold valid offsets */
     bool no_early_exit = false;            /* force the plain candidate kernels (tests) */
+    /* Early rejection lengthens each lane's dependent chain; it pays only on
+     * levels with enough waves to hide the latency. */
+    int64_t early_exit_min_pixels = 512 * 512;
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -420,6 +423,10 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
     switch (key) {
     case PT_TUNE_EARLY_EXIT:
         c->no_early_exit = (value == 0);
+        return PT_OK;
+    case PT_TUNE_EARLY_EXIT_MIN_PIXELS:
+        if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative pixel count");
+        c->early_exit_min_pixels = value;
         return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
@@ -855,7 +862,8 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.time_active = (c->adv_written[0] || c->adv_written[1]) ? 1 : 0;
     a.early_exit = (p.hysteresis_weight == 0.f && p.input_analogy_weight >= 0.f && p.canvas_output_weight >= 0.f &&
                     p.alpha_weight >= 0.f && p.dist_trans_weight >= 0.f && p.time_derivative_input_weight >= 0.f &&
-                    p.time_derivative_output_weight >= 0.f && p.temporal_coherence_weight >= 0.f && !c->no_early_exit)
+                    p.time_derivative_output_weight >= 0.f && p.temporal_coherence_weight >= 0.f && !c->no_early_exit &&
+                    (int64_t)a.lw * a.lh >= c->early_exit_min_pixels)
                        ? 1
                        : 0;
     a.base = c->base;
@@ -923,10 +931,15 @@ static int update_residual(pt_ctx *c, int level)
 /* TexSynth::syncBuffers (texSynth_kernel.cu:1240-1303) */
 static int sync_buffers(pt_ctx *c, int level, int flags)
 {
-    if (!(flags & PT_DONT_COPY_OFFSETS)) RC_CHECK(copy_working_to_base(c));
     PtArgs a = make_args(c, level);
     int pr = prof_begin(c, PK_CANVAS, (uint64_t)a.lw * a.lh);
-    HIP_CHECK(c, pt_launch_canvas(a, c->stream));
+    if (!(flags & PT_DONT_COPY_OFFSETS) && !c->off_diff.empty()) {
+        /* copy of the differing rectangle fused with the canvas pass */
+        HIP_CHECK(c, pt_launch_sync_canvas(a, c->off_diff.w, c->off_diff.h, c->stream));
+        c->off_diff = Rect();
+    } else {
+        HIP_CHECK(c, pt_launch_canvas(a, c->stream));
+    }
     prof_end(c, pr);
     if (!(flags & PT_DONT_UPDATE_RESIDUAL)) RC_CHECK(update_residual(c, level));
     return PT_OK;

@@ -24,6 +24,7 @@ def gpu():
 def _both(style, frames, overrides, ops, early_exit=1):
     g = Context.gpu(0)
     g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, early_exit)
+    g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)  # exercise the early-exit kernels on small levels
     o = OracleContext()
     try:
         sg = Session(g, style, frames, g.default_params(**overrides))
@@ -134,6 +135,7 @@ def test_early_exit_matches_plain_kernels_at_scale():
     for ee in (1, 0):
         with Context.gpu(0) as g:
             g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, ee)
+            g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)
             p = g.default_params(residual_window_size=3, levels=4, dist_trans_weight=0.5)
             outs.append(g.synthesize(p, [fr], st, iterations=4))
     _cmp_synth(outs[0], outs[1])
