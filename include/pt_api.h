@@ -281,6 +281,11 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * (default 512*512); smaller levels have too few waves to hide the longer
  * per-lane dependency chain. */
 #define PT_TUNE_EARLY_EXIT_MIN_PIXELS 2
+/* PT_TUNE_UNIT_PARALLEL (default 1): propagate and scatter spread every
+ * (pixel, candidate) unit of a workgroup's tile over its lanes through an LDS
+ * queue and resolve each pixel in candidate order afterwards; 0 evaluates
+ * each pixel's candidates sequentially on its own lane.  Same results. */
+#define PT_TUNE_UNIT_PARALLEL 3
 int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --

@@ -51,6 +51,7 @@ struct PtArgs {
     uint32_t key0, key1, call;
     int time_active;   /* some advected offset may be valid */
     int early_exit;    /* all energy terms non-negative: exact early rejection allowed */
+    int unit_parallel; /* candidate units spread over the lanes of a workgroup */
 
     PtNnf base, work;
     float *res_base, *res_work;

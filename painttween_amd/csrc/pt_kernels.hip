@@ -973,6 +973,456 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter_ee(PtArgs A)
     }
 }
 
+/* screenToExemplarAngle with the image orientation at the pixel already
+ * fetched (io = imgOrient(x, y)); the same operations. */
+__device__ __forceinline__ float ex_angle_io(const PtArgs &A, float io, float ex, float ey)
+{
+    if (A.orient_en) {
+        float theta = A.ex_orient[point_index(A.ew, A.eh, ex, ey)] - io;
+        if ((double)theta > 1.57079632679489661923)
+            return (float)((double)theta - 3.14159265358979323846);
+        if ((double)theta < -1.57079632679489661923)
+            return (float)(3.14159265358979323846 + (double)theta);
+        return theta;
+    }
+    return 0.0f;
+}
+
+__device__ __forceinline__ float image_orient_at(const PtArgs &A, int x, int y)
+{
+    return A.orient[point_index(A.ow, A.oh, (float)x, (float)y)];
+}
+
+/* Scatter sample k of pixel (x, y) placed from winner `ow`
+ * (texSynth_kernel.cu:909-925); dist is the loop's dist at iteration k. */
+__device__ __forceinline__ float4 scatter_sample(const PtArgs &A, int x, int y, int k, float dist, float4 ow, float io)
+{
+    float rr = 0.0f, theta = 0.0f;
+    if (A.rng_mode == PT_RNG_COUNTER) {
+        uint32_t ctr[4] = {(uint32_t)x, (uint32_t)y, A.call, (uint32_t)k}, key[2] = {A.key0, A.key1}, rnd[4];
+        pt_philox4x32_10(ctr, key, rnd);
+        rr = (float)(rnd[0] >> 8) * (1.0f / 16777216.0f);
+        theta = (float)((double)((float)(rnd[1] >> 8) * (1.0f / 16777216.0f)) * 0.63661977236758134308);
+    }
+    float dmax = dist, dmin = dist * 0.9f;
+    float sn, co;
+    pt_sincosf(theta, &sn, &co);
+    float len = rr * (dmax - dmin) + dmin;
+    float ex = ow.x + co * len;
+    float ey = ow.y + sn * len;
+    clamp_ex(ex, ey, A.ew, A.eh);
+    return make_float4(ex, ey, ex_angle_io(A, io, ex, ey), 0.f);
+}
+
+/* averageOffset (texSynth_kernel.cu:841-862) with its angle (:938-940). */
+__device__ __forceinline__ float4 average_candidate(const PtArgs &A, int x, int y, float io)
+{
+    float ax = 0.f, ay = 0.f;
+    int sum = 0;
+    for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++) {
+            if (!in_bounds(x + a, y + b, A.lw, A.lh, 0) || (a == 0 && b == 0)) continue;
+            float4 nb = A.base.xyth[(y + b) * A.W + (x + a)];
+            ax += nb.x;
+            ay += nb.y;
+            sum++;
+        }
+    ax /= (float)sum;
+    ay /= (float)sum;
+    return make_float4(ax, ay, ex_angle_io(A, io, ax, ay), 1.f);
+}
+
+/* ------------------------------------------------------------------ */
+/* unit-parallel candidate search                                      */
+/* ------------------------------------------------------------------ */
+/*
+ * A workgroup owns a TX x TY tile of pixels.  Every candidate of every pixel
+ * is a unit; lanes pull units from an LDS queue, evaluate them (with the
+ * exact early rejection of ee_search, or fully), and store each unit's value
+ * (+inf when it provably cannot beat the pixel's best at the time).  A
+ * per-pixel pass then applies the reference's sequential rule, candidate by
+ * candidate, `if (r < best) { winner = candidate; best = r; }`.
+ *
+ * Propagate candidates do not depend on earlier acceptances, so one round
+ * suffices.  Scatter sample k is placed relative to the winner so far, so
+ * samples are evaluated speculatively from the current winner; when sample k
+ * is accepted, samples after k are evaluated again from the new winner in a
+ * further round.  The averaged-neighbour candidate does not depend on the
+ * winner and is evaluated in the first round.  The outcome is the reference's
+ * sequential result, bit for bit.
+ */
+#define PT_UP_THREADS 256
+#define PT_UP_MAX_SAMPLES 32
+#define PT_INF __builtin_inff()
+
+template <int RT, int TX, int TY> struct UpTile {
+    static constexpr int R = RT > 0 ? RT : PT_MAX_R;
+    static constexpr int N = (TX + 2 * R) * (TY + 2 * R);
+};
+
+template <int RT, int TX, int TY>
+__device__ __forceinline__ void stage_tile_up(const PtArgs &A, int x0, int y0, float4 *scol, float4 *scan)
+{
+    const int r = RT > 0 ? RT : A.r;
+    const int tw = TX + 2 * r, th = TY + 2 * r;
+    const float4 *color = A.color[A.cur];
+    for (int k = threadIdx.x; k < tw * th; k += PT_UP_THREADS) {
+        int tx = clampi(x0 - r + k % tw, 0, A.lw - 1);
+        int ty = clampi(y0 - r + k / tw, 0, A.lh - 1);
+        scol[k] = color[ty * A.W + tx];
+        scan[k] = A.canvas[ty * A.W + tx];
+    }
+}
+
+/* Evaluates one unit for pixel (x, y) at tile position (lx, ly): the exact
+ * patch distance, or +inf when a partial sum proves it cannot beat `best`
+ * (EE).  Runs the row state machine inline so lanes stay independent. */
+template <int RT, bool EE, int TX>
+__device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
+                                         Cand c, float best, float sw)
+{
+    const int r = RT > 0 ? RT : A.r;
+    if (!(best > 0.f)) return PT_INF;
+    basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
+    float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float tb4[8];
+    float acc = 0.f, swa = 0.f;
+    if (EE) {
+        const float T0 = ee_threshold(best, sw);
+        /* phase 0: the three heaviest rows */
+#pragma unroll 1
+        for (int step = 0; step < 3; step++) {
+            int i = step == 0 ? 0 : (step == 1 ? -1 : 1);
+            if (i > r) break;
+            if (x + i >= 0 && x + i < A.lw)
+                row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+            if (acc >= T0) return PT_INF;
+        }
+        acc = 0.f;
+        swa = 0.f;
+#pragma unroll 1
+        for (int i = -r; i <= r; i++) {
+            if (x + i >= 0 && x + i < A.lw)
+                row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+            if (acc >= T0) return PT_INF;
+        }
+    } else {
+#pragma unroll 1
+        for (int i = -r; i <= r; i++) {
+            if (x + i >= 0 && x + i < A.lw)
+                row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+        }
+    }
+    return (swa < 0.000001f) ? 1e31f : acc / swa;
+}
+
+/*
+ * Row-granular state machine over a queue of units (EE variant): a lane
+ * pulls the next unit as soon as its current one is decided, so a wave's
+ * time follows the mean cost of its units rather than the slowest one.
+ * SETUP(u, c, x, y, p) prepares unit u (returns false when the unit needs no
+ * evaluation; it has then stored its value itself); DONE(u, value) stores a
+ * decided unit's value (+inf when rejected by the bound).
+ */
+template <int RT, int TX, class Setup, class Done>
+__device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_buffer_rsrc_t er, int first_u,
+                                            int U, int *s_next, const float *s_best, const float *s_sw, Setup setup,
+                                            Done done)
+{
+    const int r = RT > 0 ? RT : A.r;
+    float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float tb4[8];
+    Cand c;
+    int u = first_u, x = 0, y = 0, p = 0, ph = 0, step = 0;
+    float acc = 0.f, swa = 0.f, T0 = 0.f;
+    bool need = true;
+    while (true) {
+        if (need) {
+            bool found = false;
+            while (u < U) {
+                if (setup(u, c, x, y, p)) {
+                    found = true;
+                    break;
+                }
+                u = atomicAdd(s_next, 1);
+            }
+            if (!found) break;
+            float best = s_best[p];
+            T0 = ee_threshold(best, s_sw[p]);
+            T.lx = p % TX;
+            T.ly = p / TX;
+            basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
+            ph = 0;
+            step = 0;
+            acc = 0.f;
+            swa = 0.f;
+            need = false;
+        }
+        int i = ph == 0 ? (step == 0 ? 0 : (step == 1 ? -1 : 1)) : step - r;
+        if (x + i >= 0 && x + i < A.lw)
+            row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+        float val = 0.f;
+        bool decided = false;
+        if (acc >= T0) {
+            val = PT_INF;
+            decided = true;
+        } else if (ph == 0 && step == 2) {
+            ph = 1;
+            step = 0;
+            acc = 0.f;
+            swa = 0.f;
+        } else if (ph == 1 && step == 2 * r) {
+            val = (swa < 0.000001f) ? 1e31f : acc / swa;
+            decided = true;
+        } else {
+            step++;
+        }
+        if (decided) {
+            done(u, val);
+            need = true;
+            u = atomicAdd(s_next, 1);
+        }
+    }
+}
+
+/* propagate_kernel (texSynth_kernel.cu:1112-1219), unit-parallel. */
+template <int RT, bool EE, int TX, int TY>
+__global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int jump)
+{
+    constexpr int P = TX * TY, U = 4 * P;
+    __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
+    __shared__ float s_best[P], s_sw[P], s_ia[P];
+    __shared__ float s_r[U];
+    __shared__ int s_next;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int r = RT > 0 ? RT : A.r;
+    stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
+    for (int p = threadIdx.x; p < P; p += PT_UP_THREADS) {
+        int x = x0 + p % TX, y = y0 + p / TX;
+        bool live = x < A.lw && y < A.lh && !image_culled(A, x, y);
+        int o = y * A.W + x;
+        s_best[p] = live ? A.res_base[o] : -1.f; /* -1: nothing to search */
+        s_sw[p] = live ? tap_weight_sum(A, r, x, y) : 0.f;
+        s_ia[p] = live ? A.color[A.cur][o].w : 0.f;
+    }
+    if (threadIdx.x == 0) s_next = PT_UP_THREADS;
+    __syncthreads();
+    __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
+    Tile T;
+    T.col = scol;
+    T.can = scan;
+    T.tw = TX + 2 * r;
+    auto setup = [&](int u, Cand &c, int &x, int &y, int &p) -> bool {
+        p = u >> 2;
+        int k = u & 3;
+        x = x0 + p % TX;
+        y = y0 + p / TX;
+        if (!(s_best[p] > 0.f)) {
+            s_r[u] = PT_INF;
+            return false;
+        }
+        int tx = x, ty = y;
+        if (k == 0) tx = x + jump;
+        else if (k == 1) tx = x - jump;
+        else if (k == 2) ty = y + jump;
+        else ty = y - jump;
+        propagate_offset(A, tx, ty, (float)x, (float)y, c.t, c.sc);
+        if (!candidate_testable(A, c.t, s_ia[p])) {
+            s_r[u] = PT_INF;
+            return false;
+        }
+        return true;
+    };
+    if (EE) {
+        up_queue_ee<RT, TX>(A, T, er, threadIdx.x, U, &s_next, s_best, s_sw, setup,
+                            [&](int u, float v) { s_r[u] = v; });
+    } else {
+        for (int u = threadIdx.x; u < U;) {
+            Cand c;
+            int x, y, p;
+            if (setup(u, c, x, y, p)) {
+                T.lx = p % TX;
+                T.ly = p / TX;
+                s_r[u] = up_eval<RT, false, TX>(A, T, er, x, y, c, s_best[p], s_sw[p]);
+            }
+            u = atomicAdd(&s_next, 1);
+        }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < P; p += PT_UP_THREADS) {
+        int x = x0 + p % TX, y = y0 + p / TX;
+        if (x >= A.lw || y >= A.lh) continue;
+        int o = y * A.W + x;
+        if (image_culled(A, x, y)) {
+            A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+            A.work.sc[o] = make_float2(1.f, 1.f);
+            continue;
+        }
+        float best = A.res_base[o];
+        int win = -1;
+        for (int k = 0; k < 4; k++) {
+            float v = s_r[4 * p + k];
+            if (v < best) {
+                best = v;
+                win = k;
+            }
+        }
+        if (win < 0) {
+            A.work.xyth[o] = A.base.xyth[o];
+            A.work.sc[o] = A.base.sc[o];
+        } else {
+            int tx = x, ty = y;
+            if (win == 0) tx = x + jump;
+            else if (win == 1) tx = x - jump;
+            else if (win == 2) ty = y + jump;
+            else ty = y - jump;
+            float4 c;
+            float2 csc;
+            propagate_offset(A, tx, ty, (float)x, (float)y, c, csc);
+            A.work.xyth[o] = c;
+            A.work.sc[o] = csc;
+        }
+    }
+}
+
+/* scatter_kernel (texSynth_kernel.cu:865-952), unit-parallel in rounds.
+ * n = number of random samples (the reference's loop bound, identical for
+ * every pixel); unit k == n is the averaged-neighbour candidate. */
+template <int RT, bool EE, int TX, int TY>
+__global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
+{
+    constexpr int P = TX * TY;
+    constexpr int S = PT_UP_MAX_SAMPLES + 1;
+    __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
+    __shared__ float4 s_ow[P];
+    __shared__ float2 s_osc[P];
+    __shared__ float s_best[P], s_sw[P], s_ia[P], s_cached[P], s_iori[P];
+    __shared__ float2 s_sc[P];
+    __shared__ int s_start[P];
+    __shared__ float s_dist[S];
+    __shared__ float s_r[P * S];
+    __shared__ int s_next, s_more;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int r = RT > 0 ? RT : A.r;
+    const int SU = n + 1; /* units per pixel */
+    stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
+    if (threadIdx.x == 0) {
+        float d = (float)(A.ew > A.eh ? A.ew : A.eh);
+        for (int k = 0; k < n; k++) {
+            s_dist[k] = d;
+            d *= 0.9f;
+        }
+    }
+    for (int p = threadIdx.x; p < P; p += PT_UP_THREADS) {
+        int x = x0 + p % TX, y = y0 + p / TX;
+        bool live = x < A.lw && y < A.lh && !image_culled(A, x, y);
+        int o = y * A.W + x;
+        float cached = live ? A.res_base[o] : 0.f;
+        s_cached[p] = cached;
+        s_best[p] = live ? (float)((double)cached + 10e-7) : -1.f;
+        s_sw[p] = live ? tap_weight_sum(A, r, x, y) : 0.f;
+        s_ia[p] = live ? A.color[A.cur][o].w : 0.f;
+        s_ow[p] = live ? A.base.xyth[o] : make_float4(0.f, 0.f, 0.f, 0.f);
+        s_osc[p] = live ? A.base.sc[o] : make_float2(1.f, 1.f);
+        s_iori[p] = live ? image_orient_at(A, x, y) : 0.f;
+        s_sc[p] = live ? scale_at(A, (float)x, (float)y) : make_float2(1.f, 1.f);
+        s_start[p] = live ? 0 : SU;
+    }
+    __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
+    Tile T;
+    T.col = scol;
+    T.can = scan;
+    T.tw = TX + 2 * r;
+    const float2 one = make_float2(1.f, 1.f);
+    bool first = true;
+    while (true) {
+        if (threadIdx.x == 0) s_next = PT_UP_THREADS;
+        __syncthreads();
+        auto setup = [&](int u, Cand &c, int &x, int &y, int &p) -> bool {
+            p = u / SU;
+            int k = u % SU;
+            int st = s_start[p];
+            bool todo = (k < n) ? (k >= st && st < n) : (first && st <= n);
+            if (!todo) return false;
+            x = x0 + p % TX;
+            y = y0 + p / TX;
+            if (k < n) {
+                c.t = scatter_sample(A, x, y, k, s_dist[k], s_ow[p], s_iori[p]);
+                c.sc = s_sc[p];
+            } else {
+                c.t = average_candidate(A, x, y, s_iori[p]);
+                c.sc = one;
+            }
+            if (!(s_best[p] > 0.f) || !candidate_testable(A, c.t, s_ia[p])) {
+                s_r[p * S + k] = PT_INF;
+                return false;
+            }
+            return true;
+        };
+        auto done = [&](int u, float v) { s_r[(u / SU) * S + (u % SU)] = v; };
+        if (EE) {
+            up_queue_ee<RT, TX>(A, T, er, threadIdx.x, P * SU, &s_next, s_best, s_sw, setup, done);
+        } else {
+            for (int u = threadIdx.x; u < P * SU;) {
+                Cand c;
+                int x, y, p;
+                if (setup(u, c, x, y, p)) {
+                    T.lx = p % TX;
+                    T.ly = p / TX;
+                    done(u, up_eval<RT, false, TX>(A, T, er, x, y, c, s_best[p], s_sw[p]));
+                }
+                u = atomicAdd(&s_next, 1);
+            }
+        }
+        if (threadIdx.x == 0) s_more = 0;
+        __syncthreads();
+        /* resolution of the samples of this round, in order */
+        for (int p = threadIdx.x; p < P; p += PT_UP_THREADS) {
+            int st = s_start[p];
+            if (st >= n) continue;
+            int x = x0 + p % TX, y = y0 + p / TX;
+            float best = s_best[p];
+            int k = st;
+            for (; k < n; k++) {
+                float v = s_r[p * S + k];
+                if (v < best) break;
+            }
+            if (k < n) {
+                /* sample k wins: rebuild it from the winner it was placed from */
+                s_ow[p] = scatter_sample(A, x, y, k, s_dist[k], s_ow[p], s_iori[p]);
+                s_osc[p] = s_sc[p];
+                s_best[p] = s_r[p * S + k];
+                s_start[p] = k + 1;
+                if (k + 1 < n) s_more = 1;
+            } else {
+                s_start[p] = n;
+            }
+        }
+        first = false;
+        __syncthreads();
+        if (!s_more) break;
+    }
+    /* averaged-neighbour candidate, then the write-back rule */
+    for (int p = threadIdx.x; p < P; p += PT_UP_THREADS) {
+        int x = x0 + p % TX, y = y0 + p / TX;
+        if (x >= A.lw || y >= A.lh || image_culled(A, x, y)) continue;
+        int o = y * A.W + x;
+        float best = s_best[p];
+        float4 ow = s_ow[p];
+        float2 osc = s_osc[p];
+        float v = s_r[p * S + n];
+        if (v < best) {
+            best = v;
+            ow = average_candidate(A, x, y, s_iori[p]);
+            osc = one;
+        }
+        if (best < s_cached[p]) {
+            A.work.xyth[o] = ow;
+            A.work.sc[o] = osc;
+        }
+    }
+}
+
 /* upsampleBilinearKernel (texSynth_kernel.cu:1453-1510) with
  * offsetUpsampleNearest (:1403-1417). */
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A)
@@ -1279,17 +1729,48 @@ hipError_t pt_launch_copy_random(const PtArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
+#define PT_UP_LAUNCH(KERNEL, EE, TX, TY, A, ...)                                                         \
+    do {                                                                                                  \
+        dim3 g(((A).lw + (TX)-1) / (TX), ((A).lh + (TY)-1) / (TY));                                       \
+        dim3 b(PT_UP_THREADS);                                                                            \
+        switch ((A).r) {                                                                                  \
+        case 2: hipLaunchKernelGGL((KERNEL<2, EE, TX, TY>), g, b, 0, s, (A), ##__VA_ARGS__); break;       \
+        case 3: hipLaunchKernelGGL((KERNEL<3, EE, TX, TY>), g, b, 0, s, (A), ##__VA_ARGS__); break;       \
+        case 4: hipLaunchKernelGGL((KERNEL<4, EE, TX, TY>), g, b, 0, s, (A), ##__VA_ARGS__); break;       \
+        default: hipLaunchKernelGGL((KERNEL<0, EE, TX, TY>), g, b, 0, s, (A), ##__VA_ARGS__); break;      \
+        }                                                                                                 \
+    } while (0)
+
 hipError_t pt_launch_propagate(const PtArgs &a, int jump, hipStream_t s)
 {
-    if (a.early_exit) PT_DISPATCH_R(k_propagate_ee, a, jump);
-    else PT_DISPATCH_R(k_propagate, a, jump);
+    if (a.unit_parallel && !a.time_active) {
+        if (a.early_exit) PT_UP_LAUNCH(k_propagate_up, true, 16, 16, a, jump);
+        else PT_UP_LAUNCH(k_propagate_up, false, 8, 8, a, jump);
+    } else if (a.early_exit) {
+        PT_DISPATCH_R(k_propagate_ee, a, jump);
+    } else {
+        PT_DISPATCH_R(k_propagate, a, jump);
+    }
     return hipGetLastError();
 }
 
 hipError_t pt_launch_scatter(const PtArgs &a, hipStream_t s)
 {
-    if (a.early_exit) PT_DISPATCH_R(k_scatter_ee, a);
-    else PT_DISPATCH_R(k_scatter, a);
+    /* samples of the reference loop: identical for every pixel */
+    int n = 0;
+    float dist = (float)(a.ew > a.eh ? a.ew : a.eh);
+    for (int i = 0; (i < a.samples) && (dist * 0.9f >= 10.f); i++) {
+        n++;
+        dist *= 0.9f;
+    }
+    if (a.unit_parallel && !a.time_active && n <= PT_UP_MAX_SAMPLES) {
+        if (a.early_exit) PT_UP_LAUNCH(k_scatter_up, true, 16, 8, a, n);
+        else PT_UP_LAUNCH(k_scatter_up, false, 4, 4, a, n);
+    } else if (a.early_exit) {
+        PT_DISPATCH_R(k_scatter_ee, a);
+    } else {
+        PT_DISPATCH_R(k_scatter, a);
+    }
     return hipGetLastError();
 }
 

@@ -106,6 +106,7 @@ struct pt_ctx {
     /* Early rejection lengthens each lane's dependent chain; it pays only on
      * levels with enough waves to hide the latency. */
     int64_t early_exit_min_pixels = 512 * 512;
+    bool unit_parallel = true; /* candidate units over workgroup lanes */
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -423,6 +424,9 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
     switch (key) {
     case PT_TUNE_EARLY_EXIT:
         c->no_early_exit = (value == 0);
+        return PT_OK;
+    case PT_TUNE_UNIT_PARALLEL:
+        c->unit_parallel = value != 0;
         return PT_OK;
     case PT_TUNE_EARLY_EXIT_MIN_PIXELS:
         if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative pixel count");
@@ -866,6 +870,7 @@ static PtArgs make_args(const pt_ctx *c, int level)
                     (int64_t)a.lw * a.lh >= c->early_exit_min_pixels)
                        ? 1
                        : 0;
+    a.unit_parallel = c->unit_parallel ? 1 : 0;
     a.base = c->base;
     a.work = c->work;
     a.res_base = c->res_base;

@@ -21,9 +21,10 @@ def gpu():
     ctx.close()
 
 
-def _both(style, frames, overrides, ops, early_exit=1):
+def _both(style, frames, overrides, ops, early_exit=1, unit_parallel=1):
     g = Context.gpu(0)
     g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, early_exit)
+    g.set_tuning(_ffi.PT_TUNE_UNIT_PARALLEL, unit_parallel)
     g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)  # exercise the early-exit kernels on small levels
     o = OracleContext()
     try:
@@ -68,22 +69,23 @@ def _still_ops(levels, iterations=2):
     return ops
 
 
-@pytest.mark.parametrize("early_exit", [1, 0])
+@pytest.mark.parametrize("early_exit,unit_parallel", [(1, 1), (0, 1), (1, 0), (0, 0)])
 @pytest.mark.parametrize("r,orient,dtw", [(2, False, 0.0), (3, True, 0.5), (1, True, 0.0), (4, False, 0.0),
                                           (5, True, 0.0)])
-def test_still_operator_sequence(r, orient, dtw, early_exit):
+def test_still_operator_sequence(r, orient, dtw, early_exit, unit_parallel):
     st = W.style_pair(40, 36, seed=5, guides=orient or dtw > 0)
     fr = W.still_frame(44, 34, seed=6, guides=orient or dtw > 0)
     ov = dict(residual_window_size=r, levels=3, orientation_enabled=int(orient), dist_trans_weight=dtw)
-    ga, oa = _both(st, [fr], ov, _still_ops(3), early_exit)
+    ga, oa = _both(st, [fr], ov, _still_ops(3), early_exit, unit_parallel)
     assert_same_states(ga, oa)
 
 
-def test_counter_rng_scatter():
+@pytest.mark.parametrize("early_exit,unit_parallel", [(1, 1), (0, 1), (1, 0)])
+def test_counter_rng_scatter(early_exit, unit_parallel):
     st = W.style_pair(64, 64, seed=7)
     fr = W.still_frame(48, 40, seed=8)
     ov = dict(residual_window_size=2, levels=2, rng_mode=_ffi.PT_RNG_COUNTER, seed=1234, orientation_enabled=0)
-    ga, oa = _both(st, [fr], ov, _still_ops(2, iterations=3))
+    ga, oa = _both(st, [fr], ov, _still_ops(2, iterations=3), early_exit, unit_parallel)
     assert_same_states(ga, oa)
 
 
@@ -126,19 +128,21 @@ def test_animation_operator_sequence(merge, hyst):
     assert_same_states(ga, oa)
 
 
-def test_early_exit_matches_plain_kernels_at_scale():
-    """The early-rejection kernels against the plain ones on a 256^2 still
+def test_kernel_variants_agree_at_scale():
+    """Early rejection and unit-parallel search on and off, on a 256^2 still
     with guides: offsets, canvas and residual identical."""
     st = W.style_pair(256, 256, seed=51, guides=True)
     fr = W.still_frame(256, 256, seed=52, guides=True)
     outs = []
-    for ee in (1, 0):
+    for ee, up in ((1, 1), (0, 0), (1, 0), (0, 1)):
         with Context.gpu(0) as g:
             g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, ee)
+            g.set_tuning(_ffi.PT_TUNE_UNIT_PARALLEL, up)
             g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)
             p = g.default_params(residual_window_size=3, levels=4, dist_trans_weight=0.5)
             outs.append(g.synthesize(p, [fr], st, iterations=4))
-    _cmp_synth(outs[0], outs[1])
+    for other in outs[1:]:
+        _cmp_synth(outs[0], other)
 
 
 def _cmp_synth(g_out, o_out):
