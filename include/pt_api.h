@@ -161,7 +161,8 @@ void pt_params_default(pt_params *out);
 
 /* sizeof of the ABI structs as compiled into the library, for bindings to
  * check their layouts: 0 pt_params, 1 pt_patchxf, 2 pt_frame_guides,
- * 3 pt_style_guides; 0 for an unknown id.  No reference counterpart. */
+ * 3 pt_style_guides, 4 pt_job, 5 pt_batch_stats, 6 pt_sequence_desc,
+ * 7 pt_server_stats; 0 for an unknown id.  No reference counterpart. */
 size_t pt_struct_size(int which);
 
 /* ---- context -------------------------------------------------------------- */
@@ -353,6 +354,144 @@ int pt_synthesize_device(pt_ctx *ctx, const pt_params *params, int scheme, int i
                          const pt_frame_guides *frames, int style_w, int style_h,
                          const pt_style_guides *style, pt_patchxf *out_offsets,
                          float *out_canvas_rgba, float *out_residual);
+
+/* ---- streaming: one frame of a sequence at a time ------------------------ */
+
+/*
+ * Synthesizes the next frame of the sequence held by ctx (frame 0 after
+ * pt_create or pt_sequence_reset) and returns its level-0 result.  Frames are
+ * processed frame-major: every level of this frame, coarse to fine, before
+ * the next frame exists.  Per level this is the forward first pass of
+ * SynthesisFrame::synthesizeConsecutive (synthesisFrame.cpp:214-323) with the
+ * FMBM_RM merge (mergeRandomly, weight 0.5): frame 0 starts from random
+ * offsets / the upsampled coarser level; later frames advect the previous
+ * frame's offsets of the same level (advectOffsets) and merge them with the
+ * upsampled coarser result.  The previous frame's per-level offsets,
+ * residual and images stay resident on the device between calls.  The
+ * reference has no frame-major schedule (all of renderAnimationFMBM, FBMO,
+ * TCTF and IND loop levels outside frames, synthesisProcessor.cpp:807-1312);
+ * this one exists for callers that receive frames one at a time.  All
+ * pointers are device pointers; frame size, exemplar size and levels are
+ * fixed for a sequence.
+ */
+int pt_synthesize_next_frame(pt_ctx *ctx, const pt_params *params, int iterations, int image_w, int image_h,
+                             const pt_frame_guides *frame, int style_w, int style_h,
+                             const pt_style_guides *style, pt_patchxf *out_offsets, float *out_canvas_rgba,
+                             float *out_residual);
+/* The next pt_synthesize_next_frame call starts a new sequence (frame 0). */
+int pt_sequence_reset(pt_ctx *ctx);
+/* Index of the frame the next call will synthesize. */
+int pt_sequence_frame(pt_ctx *ctx, int *frame);
+
+/* ---- batched form: many independent jobs ------------------------------ */
+
+/* One job of a batch: the arguments of pt_synthesize_device (all pointers
+ * are device pointers of the batch's GPU; outputs may be NULL). */
+typedef struct pt_job {
+    pt_params params;
+    int32_t scheme, iterations, fixup_passes;
+    int32_t image_w, image_h, num_frames;
+    const pt_frame_guides *frames; /* num_frames entries */
+    int32_t style_w, style_h;
+    const pt_style_guides *style;
+    pt_patchxf *out_offsets;
+    float *out_canvas_rgba;
+    float *out_residual;
+} pt_job;
+
+typedef struct pt_batch pt_batch;
+
+/*
+ * Plans a batch: every job gets its own device state, its stream work is
+ * recorded once, and the same step of all jobs (propagate at one jump,
+ * scatter, residual refresh, sync/canvas) runs as ONE multi-job kernel
+ * launch over the concatenated tiles of every job, whatever their sizes,
+ * levels, patch radii and exemplars.  The launch sequence is captured in a
+ * HIP graph on the first run and replayed afterwards.  Each job's outputs
+ * are bit-identical to pt_synthesize_device on its own.  No reference
+ * counterpart (the reference synthesizes one working set per process).
+ */
+int pt_batch_create(pt_batch **out, int device, int num_jobs, const pt_job *jobs);
+int pt_batch_destroy(pt_batch *b);
+const char *pt_batch_last_error(pt_batch *b);
+/* Runs every job to completion (synchronous).  Repeated runs recompute the
+ * same outputs from the same inputs. */
+int pt_batch_run(pt_batch *b);
+/*
+ * Autotuner: for each operator with several kernel shapes (propagate,
+ * scatter) and each level-size class (< 128^2, < 512^2, >= 512^2 pixels),
+ * times every valid shape inside whole-batch runs with HIP events and keeps
+ * the fastest; drops the captured graph so the next run captures the tuned
+ * plan.  `rounds` whole-batch runs per shape (>= 1).
+ */
+int pt_batch_tune(pt_batch *b, int rounds);
+#define PT_BATCH_USE_GRAPH 1 /* default 1: capture/replay the plan as a HIP graph */
+#define PT_BATCH_XCD_REMAP 2 /* default 1: deal contiguous tile ranges to each XCD */
+#define PT_BATCH_SHAPE 3     /* value = op*256 + class*16 + shape (shape 15 = default policy) */
+int pt_batch_set(pt_batch *b, int key, int value);
+
+typedef struct pt_batch_stats {
+    int64_t jobs, ops, launches, multi_launches, ticks;
+    double last_run_ms;  /* host wall time of the last pt_batch_run */
+    double tune_ms;      /* total time spent in pt_batch_tune */
+    int32_t shape[2][3]; /* chosen shape per (propagate, scatter) x size class; -1 = default */
+} pt_batch_stats;
+int pt_batch_get_stats(pt_batch *b, pt_batch_stats *out);
+
+/* One-shot: plan, run and free a batch. */
+int pt_submit_batch(int device, int num_jobs, const pt_job *jobs);
+
+/* ---- serving form: streamed sequences with continuous batching ---------- */
+
+typedef struct pt_server pt_server;
+
+/* A sequence to serve: parameters, iterations per level and the frame and
+ * exemplar shapes; the exemplar guides (device pointers) must stay valid
+ * until the sequence is closed. */
+typedef struct pt_sequence_desc {
+    pt_params params;
+    int32_t iterations;
+    int32_t image_w, image_h;
+    int32_t style_w, style_h;
+    const pt_style_guides *style;
+} pt_sequence_desc;
+
+/*
+ * A request loop on one GPU.  Frames submitted for open sequences are
+ * admitted as in-flight slots free up (max_in_flight frames at once, at most
+ * one per sequence, as frames of a sequence depend on each other) and while
+ * the device memory of the resident sequences stays within mem_budget_bytes
+ * (a sequence becomes resident, with its offset field, previous frame and
+ * per-level state on the device, when its first frame is admitted, and stays
+ * so until closed).  Every engine tick advances every in-flight frame by one
+ * step, merging the same step of different frames into one multi-job launch,
+ * so frames that arrive while others run join the running batch.  Each
+ * frame's outputs are bit-identical to pt_synthesize_next_frame on a context
+ * that served the same sequence alone.
+ */
+int pt_server_create(pt_server **out, int device, uint64_t mem_budget_bytes, int max_in_flight);
+int pt_server_destroy(pt_server *s);
+const char *pt_server_last_error(pt_server *s);
+int pt_server_open(pt_server *s, const pt_sequence_desc *desc, int32_t *seq);
+/* Queues the next frame of `seq` (device guides and outputs); *ticket
+ * identifies it in pt_server_poll. */
+int pt_server_submit(pt_server *s, int32_t seq, const pt_frame_guides *frame, pt_patchxf *out_offsets,
+                     float *out_canvas_rgba, float *out_residual, uint64_t *ticket);
+/* Runs up to max_ticks engine ticks (0: only collect completions) and
+ * returns up to max_done completed tickets with their host-observed latency
+ * from submit to completion in ms (latency_ms may be NULL). */
+int pt_server_poll(pt_server *s, int max_ticks, uint64_t *done, double *latency_ms, int max_done, int *n_done);
+/* Runs ticks until every submitted frame has completed. */
+int pt_server_drain(pt_server *s);
+/* Releases a sequence's device state once its queued frames are done. */
+int pt_server_close(pt_server *s, int32_t seq);
+
+typedef struct pt_server_stats {
+    int64_t submitted, completed, ticks, launches, multi_launches;
+    int64_t resident_sequences, waiting_frames, in_flight;
+    uint64_t resident_bytes, budget_bytes;
+} pt_server_stats;
+int pt_server_get_stats(pt_server *s, pt_server_stats *out);
 
 #ifdef __cplusplus
 }

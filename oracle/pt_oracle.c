@@ -63,6 +63,16 @@ struct ptref_pyramid {
     float *data[32];
 };
 
+typedef struct {
+    ptref_pyramid *color, *orient, *dt, *scale, *velb, *velf;
+} frame_pyr;
+
+typedef struct {
+    pt_patchxf *off;
+    float *res;
+    float *canvas;
+} snap_t;
+
 struct ptref_ctx {
     char err[512];
     int nthreads;
@@ -91,6 +101,11 @@ struct ptref_ctx {
     int ew, eh;
     uint32_t scatter_calls;
     uint64_t pde;
+    /* sequence state of ptref_synthesize_next_frame */
+    int seq_frame, seq_levels;
+    snap_t *seq_snap[2];       /* [frame parity][level] */
+    frame_pyr seq_pyr[2];      /* [frame parity] */
+    ptref_pyramid *seq_style[4]; /* base, output, orientation angle, distance transform */
 };
 
 static int fail(ptref_ctx *c, int code, const char *fmt, ...)
@@ -495,9 +510,12 @@ static void free_buffers(ptref_ctx *c)
     c->off_base = c->off_work = NULL; c->res_base = c->res_work = NULL; c->canvas = NULL;
 }
 
+static void free_sequence(ptref_ctx *c);
+
 int ptref_destroy(ptref_ctx *c)
 {
     if (!c) return PT_OK;
+    free_sequence(c);
     free_buffers(c);
     free(c->img_orient); free(c->img_scale); free(c->random);
     free(c->ex_base); free(c->ex_out); free(c->ex_orient); free(c->ex_dt);
@@ -1336,15 +1354,6 @@ float ptref_patch_distance(ptref_ctx *c, int x, int y, int level, const pt_patch
 /* ------------------------------------------------------------------ */
 /* schedule                                                            */
 /* ------------------------------------------------------------------ */
-typedef struct {
-    ptref_pyramid *color, *orient, *dt, *scale, *velb, *velf;
-} frame_pyr;
-
-typedef struct {
-    pt_patchxf *off;
-    float *res;
-    float *canvas;
-} snap_t;
 
 static void snap_save(ptref_ctx *c, int level, snap_t *s)
 {
@@ -1519,4 +1528,146 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
     free(fp);
     ptref_pyramid_destroy(sb); ptref_pyramid_destroy(so); ptref_pyramid_destroy(sor); ptref_pyramid_destroy(sdt);
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* per-frame sequence schedule                                         */
+/* ------------------------------------------------------------------ */
+static void free_sequence(ptref_ctx *c)
+{
+    for (int k = 0; k < 2; k++) {
+        if (c->seq_snap[k]) {
+            for (int l = 0; l < c->seq_levels; l++) {
+                free(c->seq_snap[k][l].off); free(c->seq_snap[k][l].res); free(c->seq_snap[k][l].canvas);
+            }
+            free(c->seq_snap[k]);
+            c->seq_snap[k] = NULL;
+        }
+        free_frame_pyramids(&c->seq_pyr[k]);
+        memset(&c->seq_pyr[k], 0, sizeof(frame_pyr));
+    }
+    for (int k = 0; k < 4; k++) {
+        ptref_pyramid_destroy(c->seq_style[k]);
+        c->seq_style[k] = NULL;
+    }
+    c->seq_levels = 0;
+}
+
+int ptref_sequence_reset(ptref_ctx *c)
+{
+    if (!c) return PT_ERR_INVALID_ARG;
+    c->seq_frame = 0;
+    return PT_OK;
+}
+
+/*
+ * The frame-major schedule of pt_synthesize_next_frame: every level of one
+ * frame before the next frame.  Per level, the forward first pass of
+ * SynthesisFrame::synthesizeConsecutive (synthesisFrame.cpp:214-323) with the
+ * FMBM_RM merge, after restoring the previous frame's state of that level
+ * (its offsets and residual, and its images in the other texture layer).
+ */
+int ptref_synthesize_next_frame(ptref_ctx *c, const pt_params *params, int iterations, int W, int H,
+                                const pt_frame_guides *frame, int SW, int SH, const pt_style_guides *style,
+                                pt_patchxf *out_off, float *out_canvas, float *out_res)
+{
+    if (!c || !params || !frame || !style || iterations < 1) return fail(c, PT_ERR_INVALID_ARG, "bad frame arguments");
+    int levels = params->levels;
+    int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
+    int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
+    if (levels < 1 || levels > img_lv || levels > sty_lv) return fail(c, PT_ERR_INVALID_ARG, "levels");
+    int f = c->seq_frame, rc;
+    if (f == 0) {
+        rc = ptref_setup(c, params);
+        if (rc) return rc;
+        free_sequence(c);
+        free_buffers(c);
+        rc = ptref_initialize(c, W, H, SW, SH);
+        if (rc) return rc;
+        c->scatter_calls = 0;
+        for (int k = 0; k < 2; k++) c->seq_snap[k] = (snap_t *)calloc(levels, sizeof(snap_t));
+        c->seq_levels = levels;
+        ptref_pyramid *ten = NULL;
+        rc = ptref_pyramid_create(c, &c->seq_style[0], PT_ELEM_F32X4, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT,
+                                  style->base_rgba, 0.f);
+        if (!rc) rc = ptref_pyramid_create(c, &c->seq_style[1], PT_ELEM_F32X4, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT,
+                                           style->output_rgba, 0.f);
+        if (!rc) rc = ptref_pyramid_create(c, &ten, PT_ELEM_F32X3, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT,
+                                           style->orientation_xx_xy_yy, 0.f);
+        if (!rc) rc = ptref_pyramid_orientation(c, &c->seq_style[2], ten);
+        ptref_pyramid_destroy(ten);
+        if (!rc) rc = ptref_pyramid_create(c, &c->seq_style[3], PT_ELEM_F32, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT,
+                                           style->dist_trans, 0.f);
+        if (rc) return rc;
+    } else if (c->W != W || c->H != H || c->SW != SW || c->SH != SH || c->seq_levels != levels) {
+        return fail(c, PT_ERR_INVALID_ARG, "frame shape differs from the sequence");
+    }
+    frame_pyr *cur = &c->seq_pyr[f & 1], *prev = &c->seq_pyr[(f + 1) & 1];
+    free_frame_pyramids(cur);
+    memset(cur, 0, sizeof(frame_pyr));
+    rc = build_frame_pyramids(c, W, H, frame, cur);
+    if (!rc) rc = ptref_setup(c, params);
+    if (rc) return rc;
+    pt_params fp = *params;
+    fp.direction = PT_FORWARD;
+    fp.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+    fp.frame_number = f;
+    rc = ptref_upload_parameters(c, &fp);
+    snap_t *now = c->seq_snap[f & 1], *before = c->seq_snap[(f + 1) & 1];
+    ptref_pyramid **sp = c->seq_style;
+    for (int level = levels - 1; level >= 0 && !rc; level--) {
+        rc = ptref_initialize_exemplars(c, SW >> level, SH >> level, 1);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sp[0], 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, sp[1], 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sp[2], 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sp[3], 0);
+        if (!rc) rc = ptref_reset_advected_buffers(c);
+        if (!rc && f > 0) {
+            c->cur = (f + 1) % 2;
+            rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, prev->color, 0);
+            if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, prev->velb, 0);
+            if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, prev->velf, 0);
+            if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, prev->dt, 0);
+            if (!rc) rc = ptref_load_offsets(c, level, before[level].off);
+            if (!rc) rc = ptref_load_residual(c, level, before[level].res);
+        }
+        c->cur = f % 2;
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, cur->color, 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, cur->orient, 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, cur->velb, 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, cur->velf, 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, cur->dt, 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_SCALE, cur->scale, 0);
+        if (!rc) rc = ptref_create_random_pyramid(c, level, f, cur->color, cur->orient, sp[1], sp[2]);
+        if (rc) break;
+        if (f == 0) {
+            if (level == levels - 1) {
+                rc = ptref_randomize_offsets(c, level);
+            } else {
+                rc = ptref_load_offsets(c, level + 1, now[level + 1].off);
+                if (!rc) rc = ptref_upsample(c, level);
+                if (!rc) rc = ptref_sync_buffers(c, level, 0);
+            }
+        } else if (level == levels - 1) {
+            rc = ptref_advect_offsets(c, level, 1, 1, 0);
+            if (!rc) memcpy(c->off_work, c->off_base, sizeof(nnf) * c->W * c->H);
+        } else {
+            rc = ptref_advect_offsets(c, level, 1, 1, PT_ADVECT_IGNORE_RANDOM);
+            if (!rc) rc = ptref_load_offsets(c, level + 1, now[level + 1].off);
+            if (!rc) rc = ptref_load_residual(c, level + 1, now[level + 1].res);
+            if (!rc) rc = ptref_upsample(c, level);
+            if (!rc) rc = ptref_sync_buffers(c, level, 0);
+            if (!rc) rc = ptref_merge_randomly(c, level, 1, 0.5f, 0);
+        }
+        if (!rc) rc = ptref_sync_buffers(c, level, 0);
+        if (!rc) rc = propagate_and_scatter(c, level, iterations);
+        if (!rc) snap_save(c, level, &now[level]);
+    }
+    if (rc) return rc;
+    size_t n = (size_t)W * H;
+    if (out_off) memcpy(out_off, now[0].off, sizeof(pt_patchxf) * n);
+    if (out_canvas) memcpy(out_canvas, now[0].canvas, sizeof(float) * 4 * n);
+    if (out_res) memcpy(out_res, now[0].res, sizeof(float) * n);
+    c->seq_frame = f + 1;
+    return PT_OK;
 }

@@ -68,6 +68,14 @@ int ptref_synthesize(ptref_ctx *ctx, const pt_params *params, int scheme, int it
                      const pt_style_guides *style, pt_patchxf *out_offsets,
                      float *out_canvas_rgba, float *out_residual);
 
+/* The frame-major per-frame schedule of pt_synthesize_next_frame, with the
+ * sequence state kept in ctx; ptref_sequence_reset starts a new sequence. */
+int ptref_synthesize_next_frame(ptref_ctx *ctx, const pt_params *params, int iterations, int image_w,
+                                int image_h, const pt_frame_guides *frame, int style_w, int style_h,
+                                const pt_style_guides *style, pt_patchxf *out_offsets, float *out_canvas_rgba,
+                                float *out_residual);
+int ptref_sequence_reset(ptref_ctx *ctx);
+
 /* Single patch distance at (x, y) for a candidate, for unit tests. */
 float ptref_patch_distance(ptref_ctx *ctx, int x, int y, int level, const pt_patchxf *cand);
 /* Gaussian tap weights of the current setup (2r+1 floats). */

@@ -101,6 +101,42 @@ class StyleGuides(C.Structure):
     _fields_ = [("base_rgba", FP), ("output_rgba", FP), ("orientation_xx_xy_yy", FP), ("dist_trans", FP)]
 
 
+class PtJob(C.Structure):
+    _fields_ = [("params", PtParams), ("scheme", C.c_int32), ("iterations", C.c_int32),
+                ("fixup_passes", C.c_int32), ("image_w", C.c_int32), ("image_h", C.c_int32),
+                ("num_frames", C.c_int32), ("frames", C.POINTER(FrameGuides)), ("style_w", C.c_int32),
+                ("style_h", C.c_int32), ("style", C.POINTER(StyleGuides)), ("out_offsets", C.c_void_p),
+                ("out_canvas_rgba", C.c_void_p), ("out_residual", C.c_void_p)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("jobs", C.c_int64), ("ops", C.c_int64), ("launches", C.c_int64),
+                ("multi_launches", C.c_int64), ("ticks", C.c_int64), ("last_run_ms", C.c_double),
+                ("tune_ms", C.c_double), ("shape", (C.c_int32 * 3) * 2)]
+
+
+class SequenceDesc(C.Structure):
+    _fields_ = [("params", PtParams), ("iterations", C.c_int32), ("image_w", C.c_int32),
+                ("image_h", C.c_int32), ("style_w", C.c_int32), ("style_h", C.c_int32),
+                ("style", C.POINTER(StyleGuides))]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("ticks", C.c_int64),
+                ("launches", C.c_int64), ("multi_launches", C.c_int64), ("resident_sequences", C.c_int64),
+                ("waiting_frames", C.c_int64), ("in_flight", C.c_int64), ("resident_bytes", C.c_uint64),
+                ("budget_bytes", C.c_uint64)]
+
+
+# struct ids of pt_struct_size()
+STRUCTS = {0: PtParams, 1: PatchXF, 2: FrameGuides, 3: StyleGuides, 4: PtJob, 5: BatchStats,
+           6: SequenceDesc, 7: ServerStats}
+
+PT_BATCH_USE_GRAPH, PT_BATCH_XCD_REMAP, PT_BATCH_SHAPE = 1, 2, 3
+PT_OP_PROPAGATE, PT_OP_SCATTER = 0, 1
+SHAPE_NAMES = ["pixel", "pixel_ee", "unit_ee_a", "unit_a", "unit_ee_b", "unit_b"]
+
+
 def fptr(a):
     """float* into a C-contiguous float32 array, or NULL for None."""
     if a is None:
@@ -179,6 +215,14 @@ def load_library():
             raise RuntimeError(
                 "painttween_amd/libpainttween.so is missing: build it with "
                 "`python -c 'import __graft_entry__ as g; g.build()'`")
+        # One HIP runtime per process: when PyTorch is installed, let it load
+        # its HIP runtime first so that the library and torch's device
+        # allocations (used for device-resident inputs) share it, whichever
+        # of the two the caller touches first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         bind(lib, "pt_")
         lib.pt_create.restype = I
@@ -188,6 +232,35 @@ def load_library():
         lib.pt_synthesize_device.restype = I
         lib.pt_synthesize_device.argtypes = [VP, C.POINTER(PtParams), I, I, I, I, I, I, C.POINTER(FrameGuides),
                                              I, I, C.POINTER(StyleGuides), VP, VP, VP]
+        U64 = C.c_uint64
+        extra = [
+            ("pt_synthesize_next_frame", I, [VP, C.POINTER(PtParams), I, I, I, C.POINTER(FrameGuides), I, I,
+                                             C.POINTER(StyleGuides), VP, VP, VP]),
+            ("pt_sequence_reset", I, [VP]),
+            ("pt_sequence_frame", I, [VP, C.POINTER(I)]),
+            ("pt_struct_size", C.c_size_t, [I]),
+            ("pt_batch_create", I, [C.POINTER(VP), I, I, C.POINTER(PtJob)]),
+            ("pt_batch_destroy", I, [VP]),
+            ("pt_batch_last_error", C.c_char_p, [VP]),
+            ("pt_batch_run", I, [VP]),
+            ("pt_batch_tune", I, [VP, I]),
+            ("pt_batch_set", I, [VP, I, I]),
+            ("pt_batch_get_stats", I, [VP, C.POINTER(BatchStats)]),
+            ("pt_submit_batch", I, [I, I, C.POINTER(PtJob)]),
+            ("pt_server_create", I, [C.POINTER(VP), I, U64, I]),
+            ("pt_server_destroy", I, [VP]),
+            ("pt_server_last_error", C.c_char_p, [VP]),
+            ("pt_server_open", I, [VP, C.POINTER(SequenceDesc), C.POINTER(C.c_int32)]),
+            ("pt_server_submit", I, [VP, C.c_int32, C.POINTER(FrameGuides), VP, VP, VP, C.POINTER(U64)]),
+            ("pt_server_poll", I, [VP, I, C.POINTER(U64), C.POINTER(C.c_double), I, C.POINTER(I)]),
+            ("pt_server_drain", I, [VP]),
+            ("pt_server_close", I, [VP, C.c_int32]),
+            ("pt_server_get_stats", I, [VP, C.POINTER(ServerStats)]),
+        ]
+        for name, res, args in extra:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
         lib.pt_set_tuning.restype = I
         lib.pt_set_tuning.argtypes = [VP, I, I]
         lib.pt_profile_enable.restype = I

@@ -1,7 +1,7 @@
 """Build the gfx950 HIP library and the CPU oracle in place.
 
 ``build()`` always recompiles:
-  painttween_amd/libpainttween.so  -- HIP kernels + host runtime (hipcc, gfx950)
+  painttween_amd/libpainttween.so  -- HIP kernels + host runtime + multi-job executor (hipcc, gfx950)
   oracle/libptoracle.so           -- CPU oracle (gcc, OpenMP), test infrastructure
 """
 import os
@@ -16,7 +16,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "libptoracle.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-HIP_SOURCES = ["pt_kernels.hip", "pt_runtime.hip"]
+HIP_SOURCES = ["pt_kernels.hip", "pt_runtime.hip", "pt_engine.hip"]
 
 # -ffp-contract=off: only the explicit fmaf calls fuse, so the device runs the
 # same IEEE operation sequence as the CPU build of the algorithm.

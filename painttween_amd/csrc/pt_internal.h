@@ -50,8 +50,14 @@ struct PtArgs {
     int samples, rng_mode;
     uint32_t key0, key1, call;
     int time_active;   /* some advected offset may be valid */
-    int early_exit;    /* all energy terms non-negative: exact early rejection allowed */
-    int unit_parallel; /* candidate units spread over the lanes of a workgroup */
+    int ee_ok;         /* all energy terms non-negative: exact early rejection is exact */
+    int early_exit;    /* default policy: use early rejection on this launch */
+    int unit_parallel; /* default policy: candidate units over the lanes of a workgroup */
+    /* per-launch operands, so that one multi-job launch can serve jobs whose
+     * operands differ */
+    int jump;          /* propagate: jump distance                     */
+    int nscat;         /* scatter: random samples per pixel            */
+    int dw, dh;        /* sync: rectangle where working and base differ */
 
     PtNnf base, work;
     float *res_base, *res_work;
@@ -68,6 +74,46 @@ struct PtArgs {
     const float *ex_orient;
     const float *ex_dt;
 };
+
+/* Operators with several kernel shapes and a multi-job form. */
+enum PtOp { PT_OP_PROPAGATE = 0, PT_OP_SCATTER = 1, PT_OP_RESIDUAL = 2, PT_OP_SYNC_CANVAS = 3, PT_OP_COUNT = 4 };
+
+/* Kernel shapes of an operator.  Every valid shape gives bit-identical
+ * results; they differ only in how (pixel, candidate, tap) work maps onto
+ * lanes.  Propagate/scatter: PIXEL = one lane per pixel, candidates in
+ * sequence; PIXEL_EE = the same with exact early rejection; UP_* =
+ * candidate units spread over a 256-lane workgroup owning a pixel tile
+ * (EE_A: propagate 16x16 / scatter 16x8 with early rejection; A: 8x8 / 4x4;
+ * EE_B: 8x8 / 8x8 with early rejection; B: 16x16 / 8x8).  Residual and
+ * sync have the PIXEL shape only. */
+enum PtShape {
+    PT_SHAPE_PIXEL = 0,
+    PT_SHAPE_PIXEL_EE = 1,
+    PT_SHAPE_UP_EE_A = 2,
+    PT_SHAPE_UP_A = 3,
+    PT_SHAPE_UP_EE_B = 4,
+    PT_SHAPE_UP_B = 5,
+    PT_SHAPE_COUNT = 6
+};
+
+/* A multi-job launch: workgroup g serves job j with prefix[j] <= g <
+ * prefix[j+1], tile (g - prefix[j]) of that job in row-major order over
+ * tiles_x[j] tiles per row.  All arrays live in device memory. */
+struct PtJobTable {
+    const PtArgs *args;
+    const int *prefix;  /* n + 1 entries */
+    const int *tiles_x; /* n entries     */
+    int n;
+    int xcd_remap;      /* spread contiguous tile ranges over the 8 XCDs */
+};
+
+int pt_default_shape(int op, const PtArgs &a);
+int pt_shape_valid(int op, int shape, const PtArgs &a);
+/* Tiles of job `a` for (op, shape); *tiles_x receives tiles per row. */
+int pt_shape_tiles(int op, int shape, const PtArgs &a, int *tiles_x);
+/* Launch key: jobs may share one multi-job launch iff their keys match. */
+int pt_shape_key(int op, int shape, const PtArgs &a);
+hipError_t pt_launch_multi(int op, int shape, const PtArgs &first, const PtJobTable &t, int ntiles, hipStream_t s);
 
 /* Host-side launchers (pt_kernels.hip).  All run on `stream`. */
 hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s);

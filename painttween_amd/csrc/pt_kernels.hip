@@ -406,10 +406,10 @@ __device__ __forceinline__ float test_offset(const PtArgs &A, const Tile &T, int
 /* ------------------------------------------------------------------ */
 /* kernels                                                             */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ bool pixel_of(const PtArgs &A, int &x, int &y)
+__device__ __forceinline__ bool pixel_of(const PtArgs &A, int bx, int by, int &x, int &y)
 {
-    x = blockIdx.x * PT_BX + threadIdx.x;
-    y = blockIdx.y * PT_BY + threadIdx.y;
+    x = bx * PT_BX + threadIdx.x;
+    y = by * PT_BY + threadIdx.y;
     return x < A.lw && y < A.lh;
 }
 
@@ -422,11 +422,11 @@ template <int RT> struct TileSize {
  * clamped into the level; clamped entries are never read because taps
  * outside the level are skipped) and returns the lane's view. */
 template <int RT>
-__device__ __forceinline__ Tile stage_tile(const PtArgs &A, float4 *scol, float4 *scan)
+__device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, float4 *scol, float4 *scan)
 {
     const int r = RT > 0 ? RT : A.r;
     const int tw = PT_BX + 2 * r, th = PT_BY + 2 * r;
-    const int x0 = blockIdx.x * PT_BX - r, y0 = blockIdx.y * PT_BY - r;
+    const int x0 = bx * PT_BX - r, y0 = by * PT_BY - r;
     const float4 *color = A.color[A.cur];
     for (int k = threadIdx.y * PT_BX + threadIdx.x; k < tw * th; k += PT_BX * PT_BY) {
         int tx = clampi(x0 + k % tw, 0, A.lw - 1);
@@ -446,12 +446,12 @@ __device__ __forceinline__ Tile stage_tile(const PtArgs &A, float4 *scol, float4
 
 /* updateResidualCache_kernel (texSynth_kernel.cu:661-693) */
 template <int RT, bool TK>
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_update_residual(PtArgs A)
+__device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int by)
 {
     __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
     int x, y;
-    bool valid = pixel_of(A, x, y);
-    Tile T = stage_tile<RT>(A, scol, scan);
+    bool valid = pixel_of(A, bx, by, x, y);
+    Tile T = stage_tile<RT>(A, bx, by, scol, scan);
     if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) {
@@ -461,24 +461,35 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_update_residual(PtArgs A)
     A.res_base[o] = calc_residual<RT, TK>(A, T, x, y, A.base.xyth[o], A.base.sc[o]);
 }
 
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_update_residual(PtArgs A)
+{
+    b_update_residual<RT, TK>(A, blockIdx.x, blockIdx.y);
+}
+
 /* offsetsToOutput (texSynth_kernel.cu:263-274) */
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_canvas(PtArgs A)
+__device__ __forceinline__ void b_canvas(const PtArgs &A, int bx, int by)
 {
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    if (!pixel_of(A, bx, by, x, y)) return;
     int o = y * A.W + x;
     float4 n = A.base.xyth[o];
     A.canvas[o] = tex_ex_out(A, n.x, n.y);
+}
+
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_canvas(PtArgs A)
+{
+    b_canvas(A, blockIdx.x, blockIdx.y);
 }
 
 /* syncBuffers' copyWorkingToBase + offsetsToOutput in one pass
  * (texSynth_kernel.cu:1244-1246,1274-1275): pixels of the copy rectangle
  * (dw x dh) take working -> base; pixels of the level rectangle then get
  * the canvas of their (new) base offset. */
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_sync_canvas(PtArgs A, int dw, int dh)
+__device__ __forceinline__ void b_sync_canvas(const PtArgs &A, int bx, int by, int dw, int dh)
 {
-    int x = blockIdx.x * PT_BX + threadIdx.x;
-    int y = blockIdx.y * PT_BY + threadIdx.y;
+    int x = bx * PT_BX + threadIdx.x;
+    int y = by * PT_BY + threadIdx.y;
     int o = y * A.W + x;
     float4 n;
     bool have = false;
@@ -494,11 +505,16 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_sync_canvas(PtArgs A, int dw, 
     }
 }
 
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_sync_canvas(PtArgs A, int dw, int dh)
+{
+    b_sync_canvas(A, blockIdx.x, blockIdx.y, dw, dh);
+}
+
 /* copyRandomKernel (texSynth_kernel.cu:2044-2059) */
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A)
 {
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     float4 r = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
     A.work.xyth[o] = make_float4(r.x, r.y, r.z, 0.f);
@@ -526,12 +542,12 @@ __device__ __forceinline__ void propagate_offset(const PtArgs &A, int tx, int ty
 
 /* propagate_kernel (texSynth_kernel.cu:1112-1219) */
 template <int RT, bool TK>
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
+__device__ __forceinline__ void b_propagate(const PtArgs &A, int bx, int by, int jump)
 {
     __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
     int x, y;
-    bool valid = pixel_of(A, x, y);
-    Tile T = stage_tile<RT>(A, scol, scan);
+    bool valid = pixel_of(A, bx, by, x, y);
+    Tile T = stage_tile<RT>(A, bx, by, scol, scan);
     if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) {
@@ -564,18 +580,24 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
     A.work.sc[o] = osc;
 }
 
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
+{
+    b_propagate<RT, TK>(A, blockIdx.x, blockIdx.y, jump);
+}
+
 __device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
 
 /* scatter_kernel (texSynth_kernel.cu:865-952) with averageOffset (:841-862).
  * As in the reference, a pixel is written only when a candidate beats its
  * cached residual; others keep the working buffer's current content. */
 template <int RT, bool TK>
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter(PtArgs A)
+__device__ __forceinline__ void b_scatter(const PtArgs &A, int bx, int by)
 {
     __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
     int x, y;
-    bool valid = pixel_of(A, x, y);
-    Tile T = stage_tile<RT>(A, scol, scan);
+    bool valid = pixel_of(A, bx, by, x, y);
+    Tile T = stage_tile<RT>(A, bx, by, scol, scan);
     if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) return;
@@ -639,6 +661,12 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter(PtArgs A)
         A.work.xyth[o] = ow;
         A.work.sc[o] = osc;
     }
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter(PtArgs A)
+{
+    b_scatter<RT, TK>(A, blockIdx.x, blockIdx.y);
 }
 
 /* ------------------------------------------------------------------ */
@@ -840,12 +868,12 @@ __device__ __forceinline__ void ee_search(const PtArgs &A, const Tile &T, int x,
 
 /* propagate_kernel (texSynth_kernel.cu:1112-1219) with exact early rejection */
 template <int RT, bool TK>
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate_ee(PtArgs A, int jump)
+__device__ __forceinline__ void b_propagate_ee(const PtArgs &A, int bx, int by, int jump)
 {
     __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
     int x, y;
-    bool valid = pixel_of(A, x, y);
-    Tile T = stage_tile<RT>(A, scol, scan);
+    bool valid = pixel_of(A, bx, by, x, y);
+    Tile T = stage_tile<RT>(A, bx, by, scol, scan);
     if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) {
@@ -888,16 +916,22 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate_ee(PtArgs A, int jum
     A.work.sc[o] = osc;
 }
 
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate_ee(PtArgs A, int jump)
+{
+    b_propagate_ee<RT, TK>(A, blockIdx.x, blockIdx.y, jump);
+}
+
 /* scatter_kernel (texSynth_kernel.cu:865-952) with exact early rejection.
  * Sample k is generated only after samples < k are decided, from the winner
  * so far, exactly as in the reference's sequential loop. */
 template <int RT, bool TK>
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter_ee(PtArgs A)
+__device__ __forceinline__ void b_scatter_ee(const PtArgs &A, int bx, int by)
 {
     __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
     int x, y;
-    bool valid = pixel_of(A, x, y);
-    Tile T = stage_tile<RT>(A, scol, scan);
+    bool valid = pixel_of(A, bx, by, x, y);
+    Tile T = stage_tile<RT>(A, bx, by, scol, scan);
     if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) return;
@@ -971,6 +1005,12 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter_ee(PtArgs A)
         A.work.xyth[o] = ow;
         A.work.sc[o] = osc;
     }
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter_ee(PtArgs A)
+{
+    b_scatter_ee<RT, TK>(A, blockIdx.x, blockIdx.y);
 }
 
 /* screenToExemplarAngle with the image orientation at the pixel already
@@ -1187,14 +1227,14 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
 
 /* propagate_kernel (texSynth_kernel.cu:1112-1219), unit-parallel. */
 template <int RT, bool EE, int TX, int TY>
-__global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int jump)
+__device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, int jump)
 {
     constexpr int P = TX * TY, U = 4 * P;
     __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
     __shared__ float s_best[P], s_sw[P], s_ia[P];
     __shared__ float s_r[U];
     __shared__ int s_next;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
     for (int p = threadIdx.x; p < P; p += PT_UP_THREADS) {
@@ -1285,11 +1325,17 @@ __global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int ju
     }
 }
 
+template <int RT, bool EE, int TX, int TY>
+__global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int jump)
+{
+    b_propagate_up<RT, EE, TX, TY>(A, blockIdx.x, blockIdx.y, jump);
+}
+
 /* scatter_kernel (texSynth_kernel.cu:865-952), unit-parallel in rounds.
  * n = number of random samples (the reference's loop bound, identical for
  * every pixel); unit k == n is the averaged-neighbour candidate. */
 template <int RT, bool EE, int TX, int TY>
-__global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
+__device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, int n)
 {
     constexpr int P = TX * TY;
     constexpr int S = PT_UP_MAX_SAMPLES + 1;
@@ -1302,7 +1348,7 @@ __global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
     __shared__ float s_dist[S];
     __shared__ float s_r[P * S];
     __shared__ int s_next, s_more;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     const int SU = n + 1; /* units per pixel */
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
@@ -1423,12 +1469,18 @@ __global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
     }
 }
 
+template <int RT, bool EE, int TX, int TY>
+__global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
+{
+    b_scatter_up<RT, EE, TX, TY>(A, blockIdx.x, blockIdx.y, n);
+}
+
 /* upsampleBilinearKernel (texSynth_kernel.cu:1453-1510) with
  * offsetUpsampleNearest (:1403-1417). */
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A)
 {
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
     if (!in_bounds(x, y, A.lw, A.lh, 1)) return;
     float cxf = (float)((double)x - 0.5) / 2.0f;
     float cyf = (float)((double)y - 0.5) / 2.0f;
@@ -1493,7 +1545,7 @@ __device__ __forceinline__ bool translated_position(const PtArgs &A, int fwd, in
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int time_step, int flags)
 {
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     const PtAdv &d = A.adv[fwd ? 0 : 1];
     if (image_culled(A, x, y)) {
@@ -1543,7 +1595,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int fwd, float weight, int force)
 {
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     const PtAdv &d = A.adv[fwd ? 0 : 1];
     float4 a = d.xyth[o];
@@ -1562,7 +1614,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int f
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_residual(PtArgs A, int fwd)
 {
     int x, y;
-    if (!pixel_of(A, x, y)) return;
+    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     const PtAdv &d = A.adv[fwd ? 0 : 1];
     if (d.res[o] < A.res_base[o]) {
@@ -1679,6 +1731,117 @@ __global__ void k_unpack_nnf(const PatchXF36 *s, int lw, int lh, PtNnf d, int W)
 }
 
 /* ------------------------------------------------------------------ */
+/* multi-job kernels                                                   */
+/* ------------------------------------------------------------------ */
+/*
+ * One launch serves many jobs (frames of different sizes, levels, patch
+ * radii and exemplars): the grid is the concatenation of every job's tiles.
+ * A workgroup finds its job by binary search over the prefix sums of tile
+ * counts (uniform, scalar work), then runs the same body as the single-job
+ * kernel on that job's arguments, so results are bit-identical.
+ */
+struct MultiTile {
+    int j, bx, by;
+};
+
+/* Kernel parameters of a multi-job launch.  The argument table is read
+ * through the constant address space: its loads become scalar loads and the
+ * buffer pointers read from it are known to address global memory (global,
+ * not flat, memory instructions), as for pointers passed directly as kernel
+ * arguments. */
+typedef const __attribute__((address_space(4))) PtArgs ConstArgs;
+#define PT_MULTI_PARAMS                                                                                   \
+    ConstArgs *args, const int *__restrict__ prefix, const int *__restrict__ tiles_x, int njobs, int xcd
+#define PT_JOB(j) ((const PtArgs &)args[j])
+
+__device__ __forceinline__ MultiTile multi_tile(const int *__restrict__ prefix, const int *__restrict__ tiles_x,
+                                                int njobs, int xcd)
+{
+    const int n = prefix[njobs];
+    int g = blockIdx.x;
+    if (xcd) {
+        /* workgroups are dealt to the 8 XCDs round-robin: give XCD x the
+         * contiguous range of tiles starting at sum_{y<x} count_y so that
+         * neighbouring tiles share one L2 */
+        int q = n >> 3, rem = n & 7, x = g & 7, k = g >> 3;
+        g = x * q + (x < rem ? x : rem) + k;
+    }
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    int loc = g - prefix[lo];
+    int tx = tiles_x[lo];
+    MultiTile t;
+    t.j = lo;
+    t.bx = loc % tx;
+    t.by = loc / tx;
+    return t;
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void km_update_residual(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    b_update_residual<RT, TK>(PT_JOB(t.j), t.bx, t.by);
+}
+
+__global__ __launch_bounds__(PT_BX *PT_BY) void km_sync_canvas(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    const PtArgs &A = PT_JOB(t.j);
+    b_sync_canvas(A, t.bx, t.by, A.dw, A.dh);
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void km_propagate(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    const PtArgs &A = PT_JOB(t.j);
+    b_propagate<RT, TK>(A, t.bx, t.by, A.jump);
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void km_propagate_ee(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    const PtArgs &A = PT_JOB(t.j);
+    b_propagate_ee<RT, TK>(A, t.bx, t.by, A.jump);
+}
+
+template <int RT, bool EE, int TX, int TY>
+__global__ __launch_bounds__(PT_UP_THREADS) void km_propagate_up(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    const PtArgs &A = PT_JOB(t.j);
+    b_propagate_up<RT, EE, TX, TY>(A, t.bx, t.by, A.jump);
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void km_scatter(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    b_scatter<RT, TK>(PT_JOB(t.j), t.bx, t.by);
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_BX *PT_BY) void km_scatter_ee(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    b_scatter_ee<RT, TK>(PT_JOB(t.j), t.bx, t.by);
+}
+
+template <int RT, bool EE, int TX, int TY>
+__global__ __launch_bounds__(PT_UP_THREADS) void km_scatter_up(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    const PtArgs &A = PT_JOB(t.j);
+    b_scatter_up<RT, EE, TX, TY>(A, t.bx, t.by, A.nscat);
+}
+
+/* ------------------------------------------------------------------ */
 /* launchers                                                           */
 /* ------------------------------------------------------------------ */
 static dim3 grid_for(int w, int h) { return dim3((w + PT_BX - 1) / PT_BX, (h + PT_BY - 1) / PT_BY); }
@@ -1741,35 +1904,208 @@ hipError_t pt_launch_copy_random(const PtArgs &a, hipStream_t s)
         }                                                                                                 \
     } while (0)
 
-hipError_t pt_launch_propagate(const PtArgs &a, int jump, hipStream_t s)
+/* samples of the reference scatter loop (texSynth_kernel.cu:909): identical
+ * for every pixel of a launch */
+static int scatter_count(const PtArgs &a)
 {
-    if (a.unit_parallel && !a.time_active) {
-        if (a.early_exit) PT_UP_LAUNCH(k_propagate_up, true, 16, 16, a, jump);
-        else PT_UP_LAUNCH(k_propagate_up, false, 8, 8, a, jump);
-    } else if (a.early_exit) {
-        PT_DISPATCH_R(k_propagate_ee, a, jump);
-    } else {
-        PT_DISPATCH_R(k_propagate, a, jump);
-    }
-    return hipGetLastError();
-}
-
-hipError_t pt_launch_scatter(const PtArgs &a, hipStream_t s)
-{
-    /* samples of the reference loop: identical for every pixel */
     int n = 0;
     float dist = (float)(a.ew > a.eh ? a.ew : a.eh);
     for (int i = 0; (i < a.samples) && (dist * 0.9f >= 10.f); i++) {
         n++;
         dist *= 0.9f;
     }
-    if (a.unit_parallel && !a.time_active && n <= PT_UP_MAX_SAMPLES) {
-        if (a.early_exit) PT_UP_LAUNCH(k_scatter_up, true, 16, 8, a, n);
-        else PT_UP_LAUNCH(k_scatter_up, false, 4, 4, a, n);
-    } else if (a.early_exit) {
-        PT_DISPATCH_R(k_scatter_ee, a);
-    } else {
-        PT_DISPATCH_R(k_scatter, a);
+    return n;
+}
+
+/* pixel tile of (op, shape) */
+static void shape_tile(int op, int shape, int &tx, int &ty)
+{
+    tx = PT_BX;
+    ty = PT_BY;
+    if (op == PT_OP_PROPAGATE) {
+        if (shape == PT_SHAPE_UP_EE_A || shape == PT_SHAPE_UP_B) tx = ty = 16;
+        else if (shape == PT_SHAPE_UP_A || shape == PT_SHAPE_UP_EE_B) tx = ty = 8;
+    } else if (op == PT_OP_SCATTER) {
+        if (shape == PT_SHAPE_UP_EE_A) { tx = 16; ty = 8; }
+        else if (shape == PT_SHAPE_UP_A) { tx = 4; ty = 4; }
+        else if (shape == PT_SHAPE_UP_EE_B || shape == PT_SHAPE_UP_B) { tx = 8; ty = 8; }
+    }
+}
+
+static bool shape_is_up(int shape) { return shape >= PT_SHAPE_UP_EE_A; }
+static bool shape_is_ee(int shape)
+{
+    return shape == PT_SHAPE_PIXEL_EE || shape == PT_SHAPE_UP_EE_A || shape == PT_SHAPE_UP_EE_B;
+}
+
+int pt_shape_valid(int op, int shape, const PtArgs &a)
+{
+    if (shape < 0 || shape >= PT_SHAPE_COUNT) return 0;
+    if (op == PT_OP_RESIDUAL || op == PT_OP_SYNC_CANVAS) return shape == PT_SHAPE_PIXEL;
+    if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return 0;
+    if (shape_is_ee(shape) && !a.ee_ok) return 0;
+    if (shape_is_up(shape)) {
+        if (a.time_active) return 0; /* the unit kernels carry no temporal terms */
+        if (op == PT_OP_SCATTER && a.nscat > PT_UP_MAX_SAMPLES) return 0;
+    }
+    return 1;
+}
+
+int pt_default_shape(int op, const PtArgs &a)
+{
+    if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return PT_SHAPE_PIXEL;
+    if (a.unit_parallel && pt_shape_valid(op, PT_SHAPE_UP_A, a))
+        return a.early_exit ? PT_SHAPE_UP_EE_A : PT_SHAPE_UP_A;
+    return a.early_exit ? PT_SHAPE_PIXEL_EE : PT_SHAPE_PIXEL;
+}
+
+int pt_shape_tiles(int op, int shape, const PtArgs &a, int *tiles_x)
+{
+    int tx, ty;
+    shape_tile(op, shape, tx, ty);
+    int w = a.lw, h = a.lh;
+    if (op == PT_OP_SYNC_CANVAS) {
+        w = a.dw > w ? a.dw : w;
+        h = a.dh > h ? a.dh : h;
+    }
+    int nx = (w + tx - 1) / tx, ny = (h + ty - 1) / ty;
+    if (tiles_x) *tiles_x = nx;
+    return nx * ny;
+}
+
+static int rt_of(int r) { return (r == 2 || r == 3 || r == 4) ? r : 0; }
+
+int pt_shape_key(int op, int shape, const PtArgs &a)
+{
+    int tk = (op == PT_OP_SYNC_CANVAS) ? 0 : (a.time_active ? 1 : 0);
+    int rt = (op == PT_OP_SYNC_CANVAS) ? 0 : rt_of(a.r);
+    return ((op * PT_SHAPE_COUNT + shape) * 8 + rt) * 2 + tk;
+}
+
+/* One kernel instantiation per (op, shape, RT, TK); `L` launches it either
+ * as a single-job kernel or as a multi-job kernel. */
+#define PT_RT_SWITCH(r, F)                                                                                \
+    switch (rt_of(r)) {                                                                                   \
+    case 2: F(2); break;                                                                                  \
+    case 3: F(3); break;                                                                                  \
+    case 4: F(4); break;                                                                                  \
+    default: F(0); break;                                                                                 \
+    }
+
+hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
+{
+    PtArgs a = a0;
+    a.jump = jump;
+    int shape = pt_default_shape(PT_OP_PROPAGATE, a);
+    int tx, ty;
+    shape_tile(PT_OP_PROPAGATE, shape, tx, ty);
+    dim3 g((a.lw + tx - 1) / tx, (a.lh + ty - 1) / ty), bu(PT_UP_THREADS);
+    bool tk = a.time_active != 0;
+#define L_PIX(R) hipLaunchKernelGGL((tk ? k_propagate<R, true> : k_propagate<R, false>), g, kBlock, 0, s, a, jump)
+#define L_PEE(R) hipLaunchKernelGGL((tk ? k_propagate_ee<R, true> : k_propagate_ee<R, false>), g, kBlock, 0, s, a, jump)
+#define L_UEA(R) hipLaunchKernelGGL((k_propagate_up<R, true, 16, 16>), g, bu, 0, s, a, jump)
+#define L_UA(R) hipLaunchKernelGGL((k_propagate_up<R, false, 8, 8>), g, bu, 0, s, a, jump)
+    switch (shape) {
+    case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
+    case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
+    case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
+    default: PT_RT_SWITCH(a.r, L_UA); break;
+    }
+#undef L_PIX
+#undef L_PEE
+#undef L_UEA
+#undef L_UA
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_scatter(const PtArgs &a0, hipStream_t s)
+{
+    PtArgs a = a0;
+    a.nscat = scatter_count(a);
+    const int n = a.nscat;
+    int shape = pt_default_shape(PT_OP_SCATTER, a);
+    int tx, ty;
+    shape_tile(PT_OP_SCATTER, shape, tx, ty);
+    dim3 g((a.lw + tx - 1) / tx, (a.lh + ty - 1) / ty), bu(PT_UP_THREADS);
+    bool tk = a.time_active != 0;
+#define L_PIX(R) hipLaunchKernelGGL((tk ? k_scatter<R, true> : k_scatter<R, false>), g, kBlock, 0, s, a)
+#define L_PEE(R) hipLaunchKernelGGL((tk ? k_scatter_ee<R, true> : k_scatter_ee<R, false>), g, kBlock, 0, s, a)
+#define L_UEA(R) hipLaunchKernelGGL((k_scatter_up<R, true, 16, 8>), g, bu, 0, s, a, n)
+#define L_UA(R) hipLaunchKernelGGL((k_scatter_up<R, false, 4, 4>), g, bu, 0, s, a, n)
+    switch (shape) {
+    case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
+    case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
+    case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
+    default: PT_RT_SWITCH(a.r, L_UA); break;
+    }
+#undef L_PIX
+#undef L_PEE
+#undef L_UEA
+#undef L_UA
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable &t, int ntiles, hipStream_t s)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (!pt_shape_valid(op, shape, a)) return hipErrorInvalidValue;
+    dim3 g(ntiles), bu(PT_UP_THREADS);
+    bool tk = a.time_active != 0;
+    switch (op) {
+    case PT_OP_RESIDUAL: {
+#define L(R) hipLaunchKernelGGL((tk ? km_update_residual<R, true> : km_update_residual<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+        PT_RT_SWITCH(a.r, L);
+#undef L
+        break;
+    }
+    case PT_OP_SYNC_CANVAS: hipLaunchKernelGGL(km_sync_canvas, g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap); break;
+    case PT_OP_PROPAGATE: {
+#define L_PIX(R) hipLaunchKernelGGL((tk ? km_propagate<R, true> : km_propagate<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_PEE(R) hipLaunchKernelGGL((tk ? km_propagate_ee<R, true> : km_propagate_ee<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEA(R) hipLaunchKernelGGL((km_propagate_up<R, true, 16, 16>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UA(R) hipLaunchKernelGGL((km_propagate_up<R, false, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEB(R) hipLaunchKernelGGL((km_propagate_up<R, true, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UB(R) hipLaunchKernelGGL((km_propagate_up<R, false, 16, 16>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+        switch (shape) {
+        case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
+        case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
+        case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
+        case PT_SHAPE_UP_A: PT_RT_SWITCH(a.r, L_UA); break;
+        case PT_SHAPE_UP_EE_B: PT_RT_SWITCH(a.r, L_UEB); break;
+        default: PT_RT_SWITCH(a.r, L_UB); break;
+        }
+#undef L_PIX
+#undef L_PEE
+#undef L_UEA
+#undef L_UA
+#undef L_UEB
+#undef L_UB
+        break;
+    }
+    case PT_OP_SCATTER: {
+#define L_PIX(R) hipLaunchKernelGGL((tk ? km_scatter<R, true> : km_scatter<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_PEE(R) hipLaunchKernelGGL((tk ? km_scatter_ee<R, true> : km_scatter_ee<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEA(R) hipLaunchKernelGGL((km_scatter_up<R, true, 16, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UA(R) hipLaunchKernelGGL((km_scatter_up<R, false, 4, 4>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEB(R) hipLaunchKernelGGL((km_scatter_up<R, true, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UB(R) hipLaunchKernelGGL((km_scatter_up<R, false, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+        switch (shape) {
+        case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
+        case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
+        case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
+        case PT_SHAPE_UP_A: PT_RT_SWITCH(a.r, L_UA); break;
+        case PT_SHAPE_UP_EE_B: PT_RT_SWITCH(a.r, L_UEB); break;
+        default: PT_RT_SWITCH(a.r, L_UB); break;
+        }
+#undef L_PIX
+#undef L_PEE
+#undef L_UEA
+#undef L_UA
+#undef L_UEB
+#undef L_UB
+        break;
+    }
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

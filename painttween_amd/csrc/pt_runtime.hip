@@ -14,6 +14,7 @@
  * whole-buffer copy.
  */
 #include "pt_internal.h"
+#include "pt_plan.h"
 
 #include <math.h>
 #include <stdarg.h>
@@ -22,6 +23,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -134,6 +136,14 @@ struct pt_ctx {
     StylePyr style_pyr;
     pt_patchxf *pack_tmp = nullptr;
     size_t pack_cap = 0;
+    /* when set, stream work is appended here instead of issued (plans for
+     * the multi-job executor); host-side state advances as if it ran */
+    std::vector<RecOp> *rec = nullptr;
+    /* per-frame sequence state (pt_synthesize_next_frame) */
+    int seq_frame = 0;
+    int seq_levels = 0;
+    std::vector<Snapshot> seq_snap[2]; /* [frame parity][level] */
+    FramePyr seq_pyr[2];                /* [frame parity] */
     /* optional per-launch timing with HIP events on the context stream */
     bool prof_on = false;
     std::vector<hipEvent_t> prof_events;
@@ -142,6 +152,7 @@ struct pt_ctx {
 
 static void free_frame(FramePyr &f);
 static void free_style(StylePyr &sp);
+static void free_seq(pt_ctx *c);
 static void prof_clear(pt_ctx *c);
 
 static int set_err(pt_ctx *c, int code, const char *fmt, ...)
@@ -257,6 +268,10 @@ extern "C" size_t pt_struct_size(int which)
     case 1: return sizeof(pt_patchxf);
     case 2: return sizeof(pt_frame_guides);
     case 3: return sizeof(pt_style_guides);
+    case 4: return sizeof(pt_job);
+    case 5: return sizeof(pt_batch_stats);
+    case 6: return sizeof(pt_sequence_desc);
+    case 7: return sizeof(pt_server_stats);
     default: return 0;
     }
 }
@@ -330,6 +345,7 @@ extern "C" int pt_destroy(pt_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_snaps(c);
+    free_seq(c);
     free_working(c);
     for (auto &f : c->frame_pyrs) free_frame(f);
     free_style(c->style_pyr);
@@ -345,8 +361,46 @@ extern "C" int pt_destroy(pt_ctx *c)
 
 static int sync_stream(pt_ctx *c)
 {
+    if (c->rec) return PT_OK;
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return PT_OK;
+}
+
+/* Issues stream work now, or appends it to the plan being recorded. */
+static int issue(pt_ctx *c, const char *what, std::function<hipError_t(hipStream_t)> fn)
+{
+    if (c->rec) {
+        RecOp o;
+        o.op = -1;
+        o.fn = std::move(fn);
+        c->rec->push_back(std::move(o));
+        return PT_OK;
+    }
+    hipError_t e = fn(c->stream);
+    if (e != hipSuccess) return set_err(c, PT_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+    return PT_OK;
+}
+
+/* An operator launch with a multi-job form (PT_OP_*). */
+static int issue_op(pt_ctx *c, int op, const PtArgs &a)
+{
+    if (c->rec) {
+        RecOp o;
+        o.op = op;
+        o.a = a;
+        c->rec->push_back(std::move(o));
+        return PT_OK;
+    }
+    hipError_t e = pt_run_op(op, a, c->stream);
+    if (e != hipSuccess) return set_err(c, PT_ERR_HIP, "operator %d launch failed: %s", op, hipGetErrorString(e));
+    return PT_OK;
+}
+
+static int zero_fill(pt_ctx *c, void *p, size_t bytes)
+{
+    float *f = (float *)p;
+    size_t n = bytes / sizeof(float);
+    return issue(c, "zero fill", [f, n](hipStream_t s) { return pt_launch_fill(f, n, 0.f, s); });
 }
 
 static int prof_event(pt_ctx *c)
@@ -360,7 +414,7 @@ static int prof_event(pt_ctx *c)
 /* Records the start of a timed launch; returns the record index or -1. */
 static int prof_begin(pt_ctx *c, int kind, uint64_t pixels)
 {
-    if (!c->prof_on) return -1;
+    if (!c->prof_on || c->rec) return -1;
     ProfRec r;
     r.kind = kind;
     r.pixels = pixels;
@@ -521,6 +575,7 @@ extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
     if (c->ex && c->ew == w && c->eh == h) return PT_OK;
     size_t n = (size_t)w * h;
     if (n > c->ex_cap) {
+        if (c->rec) return set_err(c, PT_ERR_STATE, "exemplar buffers must be sized before recording");
         HIP_CHECK(c, hipStreamSynchronize(c->stream));
         dfree(c->ex);
         dfree(c->ex_orient);
@@ -531,9 +586,9 @@ extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
         c->ex_cap = n;
     } else {
         /* a fresh texture allocation of the new size: zero it */
-        HIP_CHECK(c, hipMemsetAsync(c->ex, 0, 2 * n * sizeof(float4), c->stream));
-        HIP_CHECK(c, hipMemsetAsync(c->ex_orient, 0, n * sizeof(float), c->stream));
-        HIP_CHECK(c, hipMemsetAsync(c->ex_dt, 0, n * sizeof(float), c->stream));
+        RC_CHECK(zero_fill(c, c->ex, 2 * n * sizeof(float4)));
+        RC_CHECK(zero_fill(c, c->ex_orient, n * sizeof(float)));
+        RC_CHECK(zero_fill(c, c->ex_dt, n * sizeof(float)));
     }
     c->ew = w;
     c->eh = h;
@@ -544,12 +599,12 @@ static int reset_advected(pt_ctx *c)
 {
     size_t n = (size_t)c->W * c->H;
     for (int k = 0; k < 2; k++) {
-        HIP_CHECK(c, hipMemsetAsync(c->adv[k].xyth, 0, n * sizeof(float4), c->stream));
-        HIP_CHECK(c, hipMemsetAsync(c->adv[k].sc, 0, n * sizeof(float2), c->stream));
-        HIP_CHECK(c, hipMemsetAsync(c->adv[k].base, 0, n * sizeof(float4), c->stream));
-        HIP_CHECK(c, hipMemsetAsync(c->adv[k].out, 0, n * sizeof(float4), c->stream));
-        HIP_CHECK(c, hipMemsetAsync(c->adv[k].res, 0, n * sizeof(float), c->stream));
-        HIP_CHECK(c, hipMemsetAsync(c->adv[k].ts, 0, n * sizeof(int32_t), c->stream));
+        RC_CHECK(zero_fill(c, c->adv[k].xyth, n * sizeof(float4)));
+        RC_CHECK(zero_fill(c, c->adv[k].sc, n * sizeof(float2)));
+        RC_CHECK(zero_fill(c, c->adv[k].base, n * sizeof(float4)));
+        RC_CHECK(zero_fill(c, c->adv[k].out, n * sizeof(float4)));
+        RC_CHECK(zero_fill(c, c->adv[k].res, n * sizeof(float)));
+        RC_CHECK(zero_fill(c, c->adv[k].ts, n * sizeof(int32_t)));
         c->adv_written[k] = false;
     }
     return PT_OK;
@@ -576,8 +631,12 @@ static int pyramid_rebuild(pt_ctx *c, pt_pyramid *p, int scale_mode)
 {
     int ch = elem_channels(p->elem);
     for (int l = 1; l < p->levels; l++) {
-        HIP_CHECK(c, pt_launch_downsample(p->data + p->off[l - 1], p->lw[l - 1], p->data + p->off[l], p->lw[l],
-                                          p->lh[l], ch, scale_mode, c->stream));
+        const float *src = p->data + p->off[l - 1];
+        float *dst = p->data + p->off[l];
+        int pw = p->lw[l - 1], lw = p->lw[l], lh = p->lh[l];
+        RC_CHECK(issue(c, "downsample", [=](hipStream_t s) {
+            return pt_launch_downsample(src, pw, dst, lw, lh, ch, scale_mode, s);
+        }));
     }
     return PT_OK;
 }
@@ -697,16 +756,18 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
     int w = p->lw[level], h = p->lh[level];
     const float *src = p->data + p->off[level];
     int prev = (c->cur + 1) % 2;
-    hipStream_t s = c->stream;
+    const int W = c->W;
     auto rect = [&](void *dst, int ch) -> int {
         if (w > c->W || h > c->H) return set_err(c, PT_ERR_INVALID_ARG, "level larger than image");
-        HIP_CHECK(c, hipMemcpy2DAsync(dst, (size_t)c->W * ch * sizeof(float), src, (size_t)w * ch * sizeof(float),
-                                      (size_t)w * ch * sizeof(float), h, hipMemcpyDeviceToDevice, s));
-        return PT_OK;
+        return issue(c, "image upload", [=](hipStream_t s) {
+            return hipMemcpy2DAsync(dst, (size_t)W * ch * sizeof(float), src, (size_t)w * ch * sizeof(float),
+                                    (size_t)w * ch * sizeof(float), h, hipMemcpyDeviceToDevice, s);
+        });
     };
     auto tight = [&](void *dst, int ch) -> int {
-        HIP_CHECK(c, hipMemcpyAsync(dst, src, (size_t)w * h * ch * sizeof(float), hipMemcpyDeviceToDevice, s));
-        return PT_OK;
+        return issue(c, "texture upload", [=](hipStream_t s) {
+            return hipMemcpyAsync(dst, src, (size_t)w * h * ch * sizeof(float), hipMemcpyDeviceToDevice, s);
+        });
     };
     switch (layer) {
     case PT_LAYER_INPUT_COLOR:
@@ -745,16 +806,22 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
     case PT_LAYER_EXEMPLAR_BASE:
     case PT_LAYER_EXEMPLAR_OUTPUT:
         if (p->elem != 4 || !c->ex || w > c->ew || h > c->eh) break;
-        HIP_CHECK(c, pt_launch_interleave((const float4 *)src, w, h, c->ex, c->ew,
-                                          layer == PT_LAYER_EXEMPLAR_BASE ? 0 : 1, s));
-        return PT_OK;
+    {
+        float4 *ex = c->ex;
+        int ew = c->ew, slot = layer == PT_LAYER_EXEMPLAR_BASE ? 0 : 1;
+        return issue(c, "exemplar upload", [=](hipStream_t s) {
+            return pt_launch_interleave((const float4 *)src, w, h, ex, ew, slot, s);
+        });
+    }
     case PT_LAYER_EXEMPLAR_ORIENTATION:
     case PT_LAYER_EXEMPLAR_DIST_TRANS: {
         if (p->elem != 1 || !c->ex || w > c->ew || h > c->eh) break;
         float *dst = layer == PT_LAYER_EXEMPLAR_ORIENTATION ? c->ex_orient : c->ex_dt;
-        HIP_CHECK(c, hipMemcpy2DAsync(dst, (size_t)c->ew * sizeof(float), src, (size_t)w * sizeof(float),
-                                      (size_t)w * sizeof(float), h, hipMemcpyDeviceToDevice, s));
-        return PT_OK;
+        int ew = c->ew;
+        return issue(c, "exemplar guide upload", [=](hipStream_t s) {
+            return hipMemcpy2DAsync(dst, (size_t)ew * sizeof(float), src, (size_t)w * sizeof(float),
+                                    (size_t)w * sizeof(float), h, hipMemcpyDeviceToDevice, s);
+        });
     }
     default:
         return set_err(c, PT_ERR_UNSUPPORTED, "layer %d not supported", layer);
@@ -785,11 +852,18 @@ static int random_pyramid(pt_ctx *c, int level, int frame, const pt_pyramid *ico
         return set_err(c, PT_ERR_INVALID_ARG, "pyramid sizes differ");
     if (ew < 2 || eh < 2) return set_err(c, PT_ERR_INVALID_ARG, "exemplar level too small");
     if (w > c->W || h > c->H) return set_err(c, PT_ERR_INVALID_ARG, "level larger than image");
-    HIP_CHECK(c, pt_launch_random_pyramid((const float4 *)(icol->data + icol->off[level]), iori->data + iori->off[level],
-                                          w, h, (const float4 *)(eout->data + eout->off[level]),
-                                          eori->data + eori->off[level], ew, eh, c->p.transparency_ok,
-                                          c->p.orientation_enabled, c->p.seed, (uint32_t)(frame * level), c->random,
-                                          c->stream));
+    {
+        const float4 *ic = (const float4 *)(icol->data + icol->off[level]);
+        const float *io = iori->data + iori->off[level];
+        const float4 *eo = (const float4 *)(eout->data + eout->off[level]);
+        const float *er = eori->data + eori->off[level];
+        int transp = c->p.transparency_ok, oe = c->p.orientation_enabled;
+        uint32_t seed = c->p.seed, k1 = (uint32_t)(frame * level);
+        float4 *out = c->random;
+        RC_CHECK(issue(c, "random pyramid", [=](hipStream_t s) {
+            return pt_launch_random_pyramid(ic, io, w, h, eo, er, ew, eh, transp, oe, seed, k1, out, s);
+        }));
+    }
     c->rw = w;
     c->rh = h;
     return PT_OK;
@@ -864,12 +938,12 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.key0 = p.seed ^ 0x5CA77E12u;
     a.key1 = (uint32_t)(p.frame_number * level);
     a.time_active = (c->adv_written[0] || c->adv_written[1]) ? 1 : 0;
-    a.early_exit = (p.hysteresis_weight == 0.f && p.input_analogy_weight >= 0.f && p.canvas_output_weight >= 0.f &&
-                    p.alpha_weight >= 0.f && p.dist_trans_weight >= 0.f && p.time_derivative_input_weight >= 0.f &&
-                    p.time_derivative_output_weight >= 0.f && p.temporal_coherence_weight >= 0.f && !c->no_early_exit &&
-                    (int64_t)a.lw * a.lh >= c->early_exit_min_pixels)
-                       ? 1
-                       : 0;
+    a.ee_ok = (p.hysteresis_weight == 0.f && p.input_analogy_weight >= 0.f && p.canvas_output_weight >= 0.f &&
+               p.alpha_weight >= 0.f && p.dist_trans_weight >= 0.f && p.time_derivative_input_weight >= 0.f &&
+               p.time_derivative_output_weight >= 0.f && p.temporal_coherence_weight >= 0.f && !c->no_early_exit)
+                  ? 1
+                  : 0;
+    a.early_exit = (a.ee_ok && (int64_t)a.lw * a.lh >= c->early_exit_min_pixels) ? 1 : 0;
     a.unit_parallel = c->unit_parallel ? 1 : 0;
     a.base = c->base;
     a.work = c->work;
@@ -901,11 +975,14 @@ static PtArgs make_args(const pt_ctx *c, int level)
 static int copy_nnf_rect(pt_ctx *c, const PtNnf &dst, const PtNnf &src, Rect r)
 {
     if (r.empty()) return PT_OK;
-    HIP_CHECK(c, hipMemcpy2DAsync(dst.xyth, (size_t)c->W * sizeof(float4), src.xyth, (size_t)c->W * sizeof(float4),
-                                  (size_t)r.w * sizeof(float4), r.h, hipMemcpyDeviceToDevice, c->stream));
-    HIP_CHECK(c, hipMemcpy2DAsync(dst.sc, (size_t)c->W * sizeof(float2), src.sc, (size_t)c->W * sizeof(float2),
-                                  (size_t)r.w * sizeof(float2), r.h, hipMemcpyDeviceToDevice, c->stream));
-    return PT_OK;
+    const size_t W = c->W;
+    return issue(c, "offset copy", [=](hipStream_t s) {
+        hipError_t e = hipMemcpy2DAsync(dst.xyth, W * sizeof(float4), src.xyth, W * sizeof(float4),
+                                        (size_t)r.w * sizeof(float4), r.h, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        return hipMemcpy2DAsync(dst.sc, W * sizeof(float2), src.sc, W * sizeof(float2), (size_t)r.w * sizeof(float2),
+                                r.h, hipMemcpyDeviceToDevice, s);
+    });
 }
 
 static int copy_working_to_base(pt_ctx *c)
@@ -926,7 +1003,7 @@ static int update_residual(pt_ctx *c, int level)
 {
     PtArgs a = make_args(c, level);
     int pr = prof_begin(c, PK_RESIDUAL, (uint64_t)a.lw * a.lh);
-    HIP_CHECK(c, pt_launch_update_residual(a, c->stream));
+    RC_CHECK(issue_op(c, PT_OP_RESIDUAL, a));
     prof_end(c, pr);
     c->res_diff.grow(a.lw, a.lh);
     c->pde += (uint64_t)a.lw * a.lh;
@@ -940,11 +1017,13 @@ static int sync_buffers(pt_ctx *c, int level, int flags)
     int pr = prof_begin(c, PK_CANVAS, (uint64_t)a.lw * a.lh);
     if (!(flags & PT_DONT_COPY_OFFSETS) && !c->off_diff.empty()) {
         /* copy of the differing rectangle fused with the canvas pass */
-        HIP_CHECK(c, pt_launch_sync_canvas(a, c->off_diff.w, c->off_diff.h, c->stream));
+        a.dw = c->off_diff.w;
+        a.dh = c->off_diff.h;
         c->off_diff = Rect();
     } else {
-        HIP_CHECK(c, pt_launch_canvas(a, c->stream));
+        a.dw = a.dh = 0; /* canvas only */
     }
+    RC_CHECK(issue_op(c, PT_OP_SYNC_CANVAS, a));
     prof_end(c, pr);
     if (!(flags & PT_DONT_UPDATE_RESIDUAL)) RC_CHECK(update_residual(c, level));
     return PT_OK;
@@ -955,7 +1034,7 @@ static int randomize_offsets(pt_ctx *c, int level)
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
     PtArgs a = make_args(c, level);
     int pr = prof_begin(c, PK_RANDOM, (uint64_t)a.lw * a.lh);
-    HIP_CHECK(c, pt_launch_copy_random(a, c->stream));
+    RC_CHECK(issue(c, "random copy", [=](hipStream_t s) { return pt_launch_copy_random(a, s); }));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
     return sync_buffers(c, level, 0);
@@ -964,8 +1043,9 @@ static int randomize_offsets(pt_ctx *c, int level)
 static int propagate(pt_ctx *c, int level, int jump)
 {
     PtArgs a = make_args(c, level);
+    a.jump = jump;
     int pr = prof_begin(c, PK_PROPAGATE, (uint64_t)a.lw * a.lh);
-    HIP_CHECK(c, pt_launch_propagate(a, jump, c->stream));
+    RC_CHECK(issue_op(c, PT_OP_PROPAGATE, a));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
     c->pde += (uint64_t)a.lw * a.lh * 4;
@@ -987,8 +1067,9 @@ static int scatter(pt_ctx *c, int level)
 {
     PtArgs a = make_args(c, level);
     a.call = c->scatter_calls++;
+    a.nscat = scatter_samples(c->ew, c->eh, a.samples);
     int pr = prof_begin(c, PK_SCATTER, (uint64_t)a.lw * a.lh);
-    HIP_CHECK(c, pt_launch_scatter(a, c->stream));
+    RC_CHECK(issue_op(c, PT_OP_SCATTER, a));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
     c->pde += (uint64_t)a.lw * a.lh * (uint64_t)(scatter_samples(c->ew, c->eh, a.samples) + 1);
@@ -999,15 +1080,19 @@ static int upsample(pt_ctx *c, int fine_level)
 {
     PtArgs a = make_args(c, fine_level);
     int pr = prof_begin(c, PK_UPSAMPLE, (uint64_t)a.lw * a.lh);
-    HIP_CHECK(c, pt_launch_upsample(a, c->stream));
+    RC_CHECK(issue(c, "upsample", [=](hipStream_t s) { return pt_launch_upsample(a, s); }));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
     c->res_diff.grow(a.lw, a.lh);
     RC_CHECK(copy_working_to_base(c));
     if (!c->res_diff.empty()) {
-        HIP_CHECK(c, hipMemcpy2DAsync(c->res_base, (size_t)c->W * sizeof(float), c->res_work,
-                                      (size_t)c->W * sizeof(float), (size_t)c->res_diff.w * sizeof(float),
-                                      c->res_diff.h, hipMemcpyDeviceToDevice, c->stream));
+        float *rb = c->res_base, *rw = c->res_work;
+        size_t W = c->W;
+        Rect r = c->res_diff;
+        RC_CHECK(issue(c, "residual copy", [=](hipStream_t s) {
+            return hipMemcpy2DAsync(rb, W * sizeof(float), rw, W * sizeof(float), (size_t)r.w * sizeof(float), r.h,
+                                    hipMemcpyDeviceToDevice, s);
+        }));
     }
     c->res_diff = Rect();
     return PT_OK;
@@ -1018,19 +1103,26 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
     PtArgs a = make_args(c, level);
     int pr = prof_begin(c, PK_ADVECT, (uint64_t)a.lw * a.lh);
-    HIP_CHECK(c, pt_launch_advect(a, forwards, time_step, flags, c->stream));
+    RC_CHECK(issue(c, "advect", [=](hipStream_t s) { return pt_launch_advect(a, forwards, time_step, flags, s); }));
     prof_end(c, pr);
     c->adv_written[forwards ? 0 : 1] = true;
-    DevAdv &d = c->adv[forwards ? 0 : 1];
+    DevAdv d = c->adv[forwards ? 0 : 1];
     size_t n = (size_t)c->W * c->H;
     /* dest.offsets -> base, whole buffer (texSynth_kernel.cu:1716) */
-    HIP_CHECK(c, hipMemcpyAsync(c->base.xyth, d.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    HIP_CHECK(c, hipMemcpyAsync(c->base.sc, d.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
+    PtNnf base = c->base;
+    RC_CHECK(issue(c, "advected offset copy", [=](hipStream_t s) {
+        hipError_t e = hipMemcpyAsync(base.xyth, d.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(base.sc, d.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
+    }));
     c->off_diff.grow(c->W, c->H);
     RC_CHECK(sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL));
     if (flags & PT_ADVECT_UPDATE_RESIDUAL) RC_CHECK(update_residual(c, level));
     /* residual -> dest.residual, whole buffer (:1724) */
-    HIP_CHECK(c, hipMemcpyAsync(d.res, c->res_base, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    float *rb = c->res_base;
+    RC_CHECK(issue(c, "advected residual copy", [=](hipStream_t s) {
+        return hipMemcpyAsync(d.res, rb, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+    }));
     return PT_OK;
 }
 
@@ -1038,7 +1130,7 @@ static int merge_randomly(pt_ctx *c, int level, int forwards, float w, int force
 {
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
     PtArgs a = make_args(c, level);
-    HIP_CHECK(c, pt_launch_merge_randomly(a, forwards, w, force, c->stream));
+    RC_CHECK(issue(c, "merge", [=](hipStream_t s) { return pt_launch_merge_randomly(a, forwards, w, force, s); }));
     c->off_diff.grow(a.lw, a.lh);
     return sync_buffers(c, level, 0);
 }
@@ -1046,7 +1138,7 @@ static int merge_randomly(pt_ctx *c, int level, int forwards, float w, int force
 static int merge_residual(pt_ctx *c, int level, int forwards)
 {
     PtArgs a = make_args(c, level);
-    HIP_CHECK(c, pt_launch_merge_residual(a, forwards, c->stream));
+    RC_CHECK(issue(c, "merge", [=](hipStream_t s) { return pt_launch_merge_residual(a, forwards, s); }));
     c->off_diff.grow(a.lw, a.lh);
     return sync_buffers(c, level, 0);
 }
@@ -1239,39 +1331,58 @@ static Snapshot &snap(pt_ctx *c, int l, int f, int p)
 }
 
 /* WorkingBuffers::saveTempImages (workingBuffers.cpp:242-296), kept on the device */
-static int snap_save(pt_ctx *c, int level, Snapshot &s)
+static int snap_save(pt_ctx *c, int level, Snapshot &sn)
 {
-    hipStream_t st = c->stream;
-    HIP_CHECK(c, hipMemcpy2DAsync(s.xyth, (size_t)s.w * sizeof(float4), c->base.xyth, (size_t)c->W * sizeof(float4),
-                                  (size_t)s.w * sizeof(float4), s.h, hipMemcpyDeviceToDevice, st));
-    HIP_CHECK(c, hipMemcpy2DAsync(s.sc, (size_t)s.w * sizeof(float2), c->base.sc, (size_t)c->W * sizeof(float2),
-                                  (size_t)s.w * sizeof(float2), s.h, hipMemcpyDeviceToDevice, st));
-    HIP_CHECK(c, hipMemcpy2DAsync(s.res, (size_t)s.w * sizeof(float), c->res_base, (size_t)c->W * sizeof(float),
-                                  (size_t)s.w * sizeof(float), s.h, hipMemcpyDeviceToDevice, st));
-    HIP_CHECK(c, hipMemcpy2DAsync(s.canvas, (size_t)s.w * sizeof(float4), c->canvas, (size_t)c->W * sizeof(float4),
-                                  (size_t)s.w * sizeof(float4), s.h, hipMemcpyDeviceToDevice, st));
+    Snapshot d = sn;
+    PtNnf base = c->base;
+    float *rb = c->res_base;
+    float4 *cv = c->canvas;
+    size_t W = c->W;
     (void)level;
-    return PT_OK;
+    return issue(c, "snapshot save", [=](hipStream_t st) {
+        hipError_t e = hipMemcpy2DAsync(d.xyth, (size_t)d.w * sizeof(float4), base.xyth, W * sizeof(float4),
+                                        (size_t)d.w * sizeof(float4), d.h, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(d.sc, (size_t)d.w * sizeof(float2), base.sc, W * sizeof(float2),
+                                 (size_t)d.w * sizeof(float2), d.h, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(d.res, (size_t)d.w * sizeof(float), rb, W * sizeof(float), (size_t)d.w * sizeof(float),
+                                 d.h, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(d.canvas, (size_t)d.w * sizeof(float4), cv, W * sizeof(float4),
+                                 (size_t)d.w * sizeof(float4), d.h, hipMemcpyDeviceToDevice, st);
+        return e;
+    });
 }
 
 /* WorkingBuffers::loadOffsets (workingBuffers.cpp:190-207) from a snapshot */
-static int snap_load_offsets(pt_ctx *c, const Snapshot &s)
+static int snap_load_offsets(pt_ctx *c, const Snapshot &sn)
 {
-    hipStream_t st = c->stream;
-    HIP_CHECK(c, hipMemcpy2DAsync(c->base.xyth, (size_t)c->W * sizeof(float4), s.xyth, (size_t)s.w * sizeof(float4),
-                                  (size_t)s.w * sizeof(float4), s.h, hipMemcpyDeviceToDevice, st));
-    HIP_CHECK(c, hipMemcpy2DAsync(c->base.sc, (size_t)c->W * sizeof(float2), s.sc, (size_t)s.w * sizeof(float2),
-                                  (size_t)s.w * sizeof(float2), s.h, hipMemcpyDeviceToDevice, st));
-    c->off_diff.grow(s.w, s.h);
+    Snapshot d = sn;
+    PtNnf base = c->base;
+    size_t W = c->W;
+    RC_CHECK(issue(c, "snapshot offset load", [=](hipStream_t st) {
+        hipError_t e = hipMemcpy2DAsync(base.xyth, W * sizeof(float4), d.xyth, (size_t)d.w * sizeof(float4),
+                                        (size_t)d.w * sizeof(float4), d.h, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+        return hipMemcpy2DAsync(base.sc, W * sizeof(float2), d.sc, (size_t)d.w * sizeof(float2),
+                                (size_t)d.w * sizeof(float2), d.h, hipMemcpyDeviceToDevice, st);
+    }));
+    c->off_diff.grow(sn.w, sn.h);
     return PT_OK;
 }
 
 /* WorkingBuffers::loadResidual (workingBuffers.cpp:166-188) from a snapshot */
-static int snap_load_residual(pt_ctx *c, const Snapshot &s)
+static int snap_load_residual(pt_ctx *c, const Snapshot &sn)
 {
-    HIP_CHECK(c, hipMemcpy2DAsync(c->res_base, (size_t)c->W * sizeof(float), s.res, (size_t)s.w * sizeof(float),
-                                  (size_t)s.w * sizeof(float), s.h, hipMemcpyDeviceToDevice, c->stream));
-    c->res_diff.grow(s.w, s.h);
+    Snapshot d = sn;
+    float *rb = c->res_base;
+    size_t W = c->W;
+    RC_CHECK(issue(c, "snapshot residual load", [=](hipStream_t st) {
+        return hipMemcpy2DAsync(rb, W * sizeof(float), d.res, (size_t)d.w * sizeof(float), (size_t)d.w * sizeof(float),
+                                d.h, hipMemcpyDeviceToDevice, st);
+    }));
+    c->res_diff.grow(sn.w, sn.h);
     return PT_OK;
 }
 
@@ -1399,20 +1510,27 @@ static void free_style(StylePyr &sp)
 static int pyramid_fill(pt_ctx *c, pt_pyramid *p, const float *src, int on_device, float defv, int scale_mode)
 {
     size_t n0 = (size_t)p->w * p->h * elem_channels(p->elem);
-    if (src)
-        HIP_CHECK(c, hipMemcpyAsync(p->data, src, n0 * sizeof(float),
-                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
-    else
-        HIP_CHECK(c, pt_launch_fill(p->data, n0, defv, c->stream));
+    float *dst = p->data;
+    if (src) {
+        hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        if (c->rec && !on_device) return set_err(c, PT_ERR_STATE, "recorded plans take device inputs only");
+        RC_CHECK(issue(c, "pyramid level-0 copy",
+                       [=](hipStream_t s) { return hipMemcpyAsync(dst, src, n0 * sizeof(float), kind, s); }));
+    } else {
+        RC_CHECK(issue(c, "pyramid fill", [=](hipStream_t s) { return pt_launch_fill(dst, n0, defv, s); }));
+    }
     return pyramid_rebuild(c, p, scale_mode);
 }
 
 static int orientation_fill(pt_ctx *c, pt_pyramid *tensor, pt_pyramid *angle, const float *src, int on_device)
 {
     RC_CHECK(pyramid_fill(c, tensor, src, on_device, 0.f, PT_DOWNSAMPLE_SCALE_CONSTANT));
-    for (int l = 0; l < tensor->levels; l++)
-        HIP_CHECK(c, pt_launch_tensor_angle(tensor->data + tensor->off[l], angle->data + angle->off[l],
-                                            (size_t)tensor->lw[l] * tensor->lh[l], c->stream));
+    for (int l = 0; l < tensor->levels; l++) {
+        const float *t = tensor->data + tensor->off[l];
+        float *a = angle->data + angle->off[l];
+        size_t n = (size_t)tensor->lw[l] * tensor->lh[l];
+        RC_CHECK(issue(c, "tensor angle", [=](hipStream_t s) { return pt_launch_tensor_angle(t, a, n, s); }));
+    }
     return PT_OK;
 }
 
@@ -1458,26 +1576,57 @@ static int prepare_style(pt_ctx *c, int w, int h, const pt_style_guides &g, int 
     return PT_OK;
 }
 
+/* One level-0 snapshot to the caller's buffers (each may be NULL). */
+static int read_snapshot(pt_ctx *c, const Snapshot &s, pt_patchxf *out_off, float *out_canvas, float *out_res,
+                         int on_device)
+{
+    hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    size_t n = (size_t)s.w * s.h;
+    if (out_off) {
+        if (!on_device && (!c->pack_tmp || c->pack_cap < n)) {
+            if (c->rec) return set_err(c, PT_ERR_STATE, "recorded plans write device outputs only");
+            HIP_CHECK(c, hipStreamSynchronize(c->stream));
+            dfree(c->pack_tmp);
+            RC_CHECK(dalloc(c, &c->pack_tmp, n));
+            c->pack_cap = n;
+        }
+        PtNnf nn{s.xyth, s.sc};
+        int w = s.w, h = s.h;
+        pt_patchxf *tmp = c->pack_tmp, *dst = out_off;
+        if (on_device) {
+            /* pack straight into the caller's device buffer */
+            RC_CHECK(issue(c, "offset pack", [=](hipStream_t st) { return pt_launch_pack_nnf(nn, w, w, h, dst, st); }));
+        } else {
+            RC_CHECK(issue(c, "offset pack", [=](hipStream_t st) {
+                hipError_t e = pt_launch_pack_nnf(nn, w, w, h, tmp, st);
+                if (e != hipSuccess) return e;
+                return hipMemcpyAsync(dst, tmp, n * sizeof(pt_patchxf), kind, st);
+            }));
+        }
+    }
+    if (out_canvas) {
+        float *dst = out_canvas;
+        const float4 *src = s.canvas;
+        RC_CHECK(issue(c, "canvas readback",
+                       [=](hipStream_t st) { return hipMemcpyAsync(dst, src, n * sizeof(float4), kind, st); }));
+    }
+    if (out_res) {
+        float *dst = out_res;
+        const float *src = s.res;
+        RC_CHECK(issue(c, "residual readback",
+                       [=](hipStream_t st) { return hipMemcpyAsync(dst, src, n * sizeof(float), kind, st); }));
+    }
+    return PT_OK;
+}
+
 static int read_results(pt_ctx *c, int F, int last_pass, pt_patchxf *out_off, float *out_canvas, float *out_res,
                         int on_device)
 {
-    hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     for (int f = 0; f < F; f++) {
         Snapshot &s = snap(c, 0, f, last_pass);
         size_t n = (size_t)s.w * s.h;
-        if (out_off) {
-            if (!c->pack_tmp || c->pack_cap < n) {
-                dfree(c->pack_tmp);
-                RC_CHECK(dalloc(c, &c->pack_tmp, n));
-                c->pack_cap = n;
-            }
-            PtNnf nn{s.xyth, s.sc};
-            HIP_CHECK(c, pt_launch_pack_nnf(nn, s.w, s.w, s.h, c->pack_tmp, c->stream));
-            HIP_CHECK(c, hipMemcpyAsync(out_off + f * n, c->pack_tmp, n * sizeof(pt_patchxf), kind, c->stream));
-        }
-        if (out_canvas)
-            HIP_CHECK(c, hipMemcpyAsync(out_canvas + f * n * 4, s.canvas, n * sizeof(float4), kind, c->stream));
-        if (out_res) HIP_CHECK(c, hipMemcpyAsync(out_res + f * n, s.res, n * sizeof(float), kind, c->stream));
+        RC_CHECK(read_snapshot(c, s, out_off ? out_off + f * n : nullptr, out_canvas ? out_canvas + f * n * 4 : nullptr,
+                               out_res ? out_res + f * n : nullptr, on_device));
     }
     return PT_OK;
 }
@@ -1488,19 +1637,18 @@ static int read_results(pt_ctx *c, int F, int last_pass, pt_patchxf *out_off, fl
 static int reset_working(pt_ctx *c)
 {
     size_t n = (size_t)c->W * c->H;
-    hipStream_t s = c->stream;
-    HIP_CHECK(c, hipMemsetAsync(c->base.xyth, 0, n * sizeof(float4), s));
-    HIP_CHECK(c, hipMemsetAsync(c->base.sc, 0, n * sizeof(float2), s));
-    HIP_CHECK(c, hipMemsetAsync(c->work.xyth, 0, n * sizeof(float4), s));
-    HIP_CHECK(c, hipMemsetAsync(c->work.sc, 0, n * sizeof(float2), s));
-    HIP_CHECK(c, hipMemsetAsync(c->res_base, 0, n * sizeof(float), s));
-    HIP_CHECK(c, hipMemsetAsync(c->res_work, 0, n * sizeof(float), s));
-    HIP_CHECK(c, hipMemsetAsync(c->canvas, 0, n * sizeof(float4), s));
+    RC_CHECK(zero_fill(c, c->base.xyth, n * sizeof(float4)));
+    RC_CHECK(zero_fill(c, c->base.sc, n * sizeof(float2)));
+    RC_CHECK(zero_fill(c, c->work.xyth, n * sizeof(float4)));
+    RC_CHECK(zero_fill(c, c->work.sc, n * sizeof(float2)));
+    RC_CHECK(zero_fill(c, c->res_base, n * sizeof(float)));
+    RC_CHECK(zero_fill(c, c->res_work, n * sizeof(float)));
+    RC_CHECK(zero_fill(c, c->canvas, n * sizeof(float4)));
     for (int k = 0; k < 2; k++) {
-        HIP_CHECK(c, hipMemsetAsync(c->color[k], 0, n * sizeof(float4), s));
-        HIP_CHECK(c, hipMemsetAsync(c->dt[k], 0, n * sizeof(float), s));
-        HIP_CHECK(c, hipMemsetAsync(c->velb[k], 0, n * sizeof(float2), s));
-        HIP_CHECK(c, hipMemsetAsync(c->velf[k], 0, n * sizeof(float2), s));
+        RC_CHECK(zero_fill(c, c->color[k], n * sizeof(float4)));
+        RC_CHECK(zero_fill(c, c->dt[k], n * sizeof(float)));
+        RC_CHECK(zero_fill(c, c->velb[k], n * sizeof(float2)));
+        RC_CHECK(zero_fill(c, c->velf[k], n * sizeof(float2)));
     }
     RC_CHECK(reset_advected(c));
     c->off_diff = Rect();
@@ -1508,6 +1656,43 @@ static int reset_working(pt_ctx *c)
     c->cur = 0;
     c->scatter_calls = 0;
     return PT_OK;
+}
+
+static int check_synthesis(pt_ctx *c, const pt_params *params, int W, int H, int SW, int SH)
+{
+    if (!params) return set_err(c, PT_ERR_INVALID_ARG, "NULL parameters");
+    if (W < 2 || H < 2 || SW < 2 || SH < 2) return set_err(c, PT_ERR_INVALID_ARG, "bad dimensions");
+    int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
+    int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
+    if (params->levels < 1 || params->levels > img_lv || params->levels > sty_lv)
+        return set_err(c, PT_ERR_INVALID_ARG, "levels exceeds the pyramid depth");
+    return check_supported(c, params);
+}
+
+/* Buffers of a one-shot synthesis, allocated before any work is issued or
+ * recorded (a recorded plan must not see its buffers move). */
+static int synthesis_alloc(pt_ctx *c, const pt_params *params, int W, int H, int F, int passes, int SW, int SH)
+{
+    RC_CHECK(pt_initialize(c, W, H, SW, SH));
+    RC_CHECK(pt_initialize_exemplars(c, SW, SH, 1));
+    RC_CHECK(snap_alloc(c, params->levels, F, passes));
+    return PT_OK;
+}
+
+static int synthesize_issue(pt_ctx *c, const pt_params *params, int scheme, int iterations, int fixup_passes,
+                            int W, int H, int F, const pt_frame_guides *frames, int SW, int SH,
+                            const pt_style_guides *style, int guides_on_device, pt_patchxf *out_off,
+                            float *out_canvas, float *out_res, int outputs_on_device)
+{
+    RC_CHECK(reset_working(c));
+    RC_CHECK(prepare_style(c, SW, SH, *style, guides_on_device, c->style_pyr));
+    if ((int)c->frame_pyrs.size() > F) {
+        for (size_t i = F; i < c->frame_pyrs.size(); i++) free_frame(c->frame_pyrs[i]);
+    }
+    c->frame_pyrs.resize(F);
+    for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], guides_on_device, c->frame_pyrs[f]));
+    RC_CHECK(run_schedule(c, params, scheme, iterations, fixup_passes, F, c->frame_pyrs.data(), c->style_pyr));
+    return read_results(c, F, fixup_passes, out_off, out_canvas, out_res, outputs_on_device);
 }
 
 static int synthesize(pt_ctx *c, const pt_params *params, int scheme, int iterations, int fixup_passes, int W, int H,
@@ -1518,23 +1703,292 @@ static int synthesize(pt_ctx *c, const pt_params *params, int scheme, int iterat
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
     if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM) return set_err(c, PT_ERR_UNSUPPORTED, "scheme");
-    int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
-    int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
-    if (params->levels < 1 || params->levels > img_lv || params->levels > sty_lv)
-        return set_err(c, PT_ERR_INVALID_ARG, "levels exceeds the pyramid depth");
-    RC_CHECK(check_supported(c, params));
+    RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
-    RC_CHECK(pt_initialize(c, W, H, SW, SH));
-    RC_CHECK(reset_working(c));
-    RC_CHECK(prepare_style(c, SW, SH, *style, guides_on_device, c->style_pyr));
-    if ((int)c->frame_pyrs.size() > F) {
-        for (size_t i = F; i < c->frame_pyrs.size(); i++) free_frame(c->frame_pyrs[i]);
-    }
-    c->frame_pyrs.resize(F);
-    for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], guides_on_device, c->frame_pyrs[f]));
-    RC_CHECK(run_schedule(c, params, scheme, iterations, fixup_passes, F, c->frame_pyrs.data(), c->style_pyr));
-    RC_CHECK(read_results(c, F, fixup_passes, out_off, out_canvas, out_res, outputs_on_device));
+    RC_CHECK(synthesis_alloc(c, params, W, H, F, fixup_passes + 1, SW, SH));
+    RC_CHECK(synthesize_issue(c, params, scheme, iterations, fixup_passes, W, H, F, frames, SW, SH, style,
+                              guides_on_device, out_off, out_canvas, out_res, outputs_on_device));
     return sync_stream(c);
+}
+
+/* Runs `body` with the context recording into *ops; the context's own
+ * stream is drained first so that allocation-time clears are complete
+ * before any executor stream uses the buffers. */
+template <class F> static int record_into(pt_ctx *c, std::vector<RecOp> *ops, F body)
+{
+    if (!c || !ops) return set_err(c, PT_ERR_INVALID_ARG, "NULL argument");
+    if (c->rec) return set_err(c, PT_ERR_STATE, "already recording");
+    c->rec = ops;
+    int rc = body();
+    c->rec = nullptr;
+    if (rc != PT_OK) return rc;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return PT_OK;
+}
+
+int pt_record_synthesis(pt_ctx *c, std::vector<RecOp> *ops, const pt_params *params, int scheme, int iterations,
+                        int fixup_passes, int W, int H, int F, const pt_frame_guides *frames, int SW, int SH,
+                        const pt_style_guides *style, pt_patchxf *out_off, float *out_canvas, float *out_res)
+{
+    if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
+    if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
+        return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
+    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM) return set_err(c, PT_ERR_UNSUPPORTED, "scheme");
+    RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
+    HIP_CHECK(c, hipSetDevice(c->device));
+    RC_CHECK(synthesis_alloc(c, params, W, H, F, fixup_passes + 1, SW, SH));
+    /* the frame and style pyramids are allocated here too, by a dry pass of
+     * their preparation with recording into a scratch list */
+    {
+        std::vector<RecOp> scratch;
+        RC_CHECK(record_into(c, &scratch, [&] {
+            RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr));
+            if ((int)c->frame_pyrs.size() > F)
+                for (size_t i = F; i < c->frame_pyrs.size(); i++) free_frame(c->frame_pyrs[i]);
+            c->frame_pyrs.resize(F);
+            for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], 1, c->frame_pyrs[f]));
+            return PT_OK;
+        }));
+    }
+    return record_into(c, ops, [&] {
+        return synthesize_issue(c, params, scheme, iterations, fixup_passes, W, H, F, frames, SW, SH, style, 1,
+                                out_off, out_canvas, out_res, 1);
+    });
+}
+
+hipError_t pt_run_op(int op, const PtArgs &a, hipStream_t s)
+{
+    switch (op) {
+    case PT_OP_PROPAGATE: return pt_launch_propagate(a, a.jump, s);
+    case PT_OP_SCATTER: return pt_launch_scatter(a, s);
+    case PT_OP_RESIDUAL: return pt_launch_update_residual(a, s);
+    case PT_OP_SYNC_CANVAS:
+        if (a.dw > 0 && a.dh > 0) return pt_launch_sync_canvas(a, a.dw, a.dh, s);
+        return pt_launch_canvas(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int pt_ctx_device(const pt_ctx *c) { return c ? c->device : -1; }
+
+int pt_ctx_fail(pt_ctx *c, int code, const char *msg) { return set_err(c, code, "%s", msg); }
+
+/* ------------------------------------------------------------------ */
+/* per-frame sequence schedule (streaming)                             */
+/* ------------------------------------------------------------------ */
+static void free_seq(pt_ctx *c)
+{
+    for (int k = 0; k < 2; k++) {
+        for (auto &s : c->seq_snap[k]) {
+            dfree(s.xyth);
+            dfree(s.sc);
+            dfree(s.res);
+            dfree(s.canvas);
+        }
+        c->seq_snap[k].clear();
+        free_frame(c->seq_pyr[k]);
+    }
+    c->seq_levels = 0;
+}
+
+static int seq_alloc(pt_ctx *c, int levels)
+{
+    if (c->seq_levels == levels && !c->seq_snap[0].empty() && c->seq_snap[0][0].w == c->W &&
+        c->seq_snap[0][0].h == c->H)
+        return PT_OK;
+    for (int k = 0; k < 2; k++) {
+        for (auto &s : c->seq_snap[k]) {
+            dfree(s.xyth);
+            dfree(s.sc);
+            dfree(s.res);
+            dfree(s.canvas);
+        }
+        c->seq_snap[k].assign(levels, Snapshot());
+        for (int l = 0; l < levels; l++) {
+            Snapshot &s = c->seq_snap[k][l];
+            s.w = c->W >> l;
+            s.h = c->H >> l;
+            size_t n = (size_t)s.w * s.h;
+            RC_CHECK(dalloc(c, &s.xyth, n));
+            RC_CHECK(dalloc(c, &s.sc, n));
+            RC_CHECK(dalloc(c, &s.res, n));
+            RC_CHECK(dalloc(c, &s.canvas, n));
+        }
+    }
+    c->seq_levels = levels;
+    return PT_OK;
+}
+
+/*
+ * One new frame of a sequence, all levels, before the next frame arrives
+ * (frame-major order).  Per level it runs the forward first pass of
+ * SynthesisFrame::synthesizeConsecutive (synthesisFrame.cpp:214-323) with
+ * FMBM_RM: the coarsest level advects the previous frame's result and keeps
+ * it; finer levels advect, upsample this frame's coarser result and merge
+ * the two at random (mergeRandomly, weight 0.5), then propagateAndScatter.
+ * The state the level-major reference schedule would hold at (frame, level)
+ * is restored from resident per-level snapshots: the previous frame's
+ * offsets and residual at this level, and its images in the other texture
+ * layer (flipAnimBuffers).  Level-0-sized buffers keep what earlier levels
+ * and frames of this sequence left outside the active rectangle, so a frame
+ * depends only on its sequence's own history.
+ */
+static int next_frame_schedule(pt_ctx *c, const pt_params *params, int iterations, const FramePyr &cur,
+                               const FramePyr *prev, const StylePyr &sp)
+{
+    const int levels = params->levels;
+    const int f = c->seq_frame;
+    std::vector<Snapshot> &now = c->seq_snap[f & 1];
+    std::vector<Snapshot> &before = c->seq_snap[(f + 1) & 1];
+    RC_CHECK(pt_setup(c, params));
+    pt_params fp = *params;
+    fp.direction = PT_FORWARD;
+    fp.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+    fp.frame_number = f;
+    RC_CHECK(pt_upload_parameters(c, &fp));
+    for (int level = levels - 1; level >= 0; level--) {
+        RC_CHECK(pt_initialize_exemplars(c, c->SW >> level, c->SH >> level, 1));
+        RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sp.base, 0));
+        RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, sp.out, 0));
+        RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sp.orient, 0));
+        RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sp.dt, 0));
+        RC_CHECK(reset_advected(c));
+        if (f > 0) {
+            c->cur = (f + 1) % 2;
+            RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_COLOR, prev->color, 0));
+            RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_B, prev->velb, 0));
+            RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_F, prev->velf, 0));
+            RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, prev->dt, 0));
+            RC_CHECK(snap_load_offsets(c, before[level]));
+            RC_CHECK(snap_load_residual(c, before[level]));
+        }
+        c->cur = f % 2;
+        RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_COLOR, cur.color, 0));
+        RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, cur.orient, 0));
+        RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_B, cur.velb, 0));
+        RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_F, cur.velf, 0));
+        RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, cur.dt, 0));
+        RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_SCALE, cur.scale, 0));
+        RC_CHECK(random_pyramid(c, level, f, cur.color, cur.orient, sp.out, sp.orient));
+        if (f == 0) {
+            if (level == levels - 1) {
+                RC_CHECK(randomize_offsets(c, level));
+            } else {
+                RC_CHECK(snap_load_offsets(c, now[level + 1]));
+                RC_CHECK(upsample(c, level));
+                RC_CHECK(sync_buffers(c, level, 0));
+            }
+        } else if (level == levels - 1) {
+            RC_CHECK(advect(c, level, 1, 1, 0));
+            RC_CHECK(copy_base_to_working(c));
+        } else {
+            RC_CHECK(advect(c, level, 1, 1, PT_ADVECT_IGNORE_RANDOM));
+            RC_CHECK(snap_load_offsets(c, now[level + 1]));
+            RC_CHECK(snap_load_residual(c, now[level + 1]));
+            RC_CHECK(upsample(c, level));
+            RC_CHECK(sync_buffers(c, level, 0));
+            RC_CHECK(merge_randomly(c, level, 1, 0.5f, 0));
+        }
+        RC_CHECK(sync_buffers(c, level, 0));
+        RC_CHECK(propagate_and_scatter(c, level, iterations));
+        RC_CHECK(snap_save(c, level, now[level]));
+    }
+    return PT_OK;
+}
+
+static int next_frame_issue(pt_ctx *c, const pt_params *params, int iterations, int W, int H,
+                            const pt_frame_guides *frame, int SW, int SH, const pt_style_guides *style,
+                            pt_patchxf *out_off, float *out_canvas, float *out_res)
+{
+    const int f = c->seq_frame;
+    if (f == 0) {
+        RC_CHECK(reset_working(c));
+        RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr));
+    }
+    RC_CHECK(prepare_frame(c, W, H, *frame, 1, c->seq_pyr[f & 1]));
+    RC_CHECK(next_frame_schedule(c, params, iterations, c->seq_pyr[f & 1], f > 0 ? &c->seq_pyr[(f + 1) & 1] : nullptr,
+                                 c->style_pyr));
+    /* results: this frame's level-0 snapshot */
+    RC_CHECK(read_snapshot(c, c->seq_snap[f & 1][0], out_off, out_canvas, out_res, 1));
+    c->seq_frame = f + 1;
+    return PT_OK;
+}
+
+static int next_frame_check(pt_ctx *c, const pt_params *params, int iterations, int W, int H,
+                            const pt_frame_guides *frame, int SW, int SH, const pt_style_guides *style)
+{
+    if (!params || !frame || !style || iterations < 1) return set_err(c, PT_ERR_INVALID_ARG, "bad frame arguments");
+    RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
+    if (c->seq_frame > 0 && (c->W != W || c->H != H || c->SW != SW || c->SH != SH || c->seq_levels != params->levels))
+        return set_err(c, PT_ERR_INVALID_ARG, "frame shape differs from the sequence; call pt_sequence_reset");
+    HIP_CHECK(c, hipSetDevice(c->device));
+    if (c->seq_frame == 0) {
+        RC_CHECK(pt_initialize(c, W, H, SW, SH));
+        RC_CHECK(pt_initialize_exemplars(c, SW, SH, 1));
+        RC_CHECK(seq_alloc(c, params->levels));
+    }
+    return PT_OK;
+}
+
+/* Allocates this frame's pyramids (and on frame 0 the style's) without
+ * issuing anything. */
+static int next_frame_pyramids(pt_ctx *c, int W, int H, const pt_frame_guides *frame, int SW, int SH,
+                               const pt_style_guides *style)
+{
+    std::vector<RecOp> scratch;
+    const int f = c->seq_frame;
+    return record_into(c, &scratch, [&] {
+        if (f == 0) RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr));
+        return prepare_frame(c, W, H, *frame, 1, c->seq_pyr[f & 1]);
+    });
+}
+
+int pt_record_next_frame(pt_ctx *c, std::vector<RecOp> *ops, const pt_params *params, int iterations, int W, int H,
+                         const pt_frame_guides *frame, int SW, int SH, const pt_style_guides *style,
+                         pt_patchxf *out_off, float *out_canvas, float *out_res)
+{
+    if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
+    RC_CHECK(next_frame_check(c, params, iterations, W, H, frame, SW, SH, style));
+    RC_CHECK(next_frame_pyramids(c, W, H, frame, SW, SH, style));
+    return record_into(c, ops, [&] {
+        return next_frame_issue(c, params, iterations, W, H, frame, SW, SH, style, out_off, out_canvas, out_res);
+    });
+}
+
+extern "C" int pt_synthesize_next_frame(pt_ctx *c, const pt_params *params, int iterations, int W, int H,
+                                        const pt_frame_guides *frame, int SW, int SH, const pt_style_guides *style,
+                                        pt_patchxf *out_off, float *out_canvas, float *out_res)
+{
+    if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
+    RC_CHECK(next_frame_check(c, params, iterations, W, H, frame, SW, SH, style));
+    RC_CHECK(next_frame_issue(c, params, iterations, W, H, frame, SW, SH, style, out_off, out_canvas, out_res));
+    return sync_stream(c);
+}
+
+extern "C" int pt_sequence_reset(pt_ctx *c)
+{
+    if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
+    c->seq_frame = 0;
+    c->scatter_calls = 0;
+    return PT_OK;
+}
+
+extern "C" int pt_sequence_frame(pt_ctx *c, int *frame)
+{
+    if (!c || !frame) return set_err(c, PT_ERR_INVALID_ARG, "NULL argument");
+    *frame = c->seq_frame;
+    return PT_OK;
+}
+
+size_t pt_sequence_bytes(int W, int H, int SW, int SH, int levels)
+{
+    (void)levels;
+    const double px = (double)W * H, ex = (double)SW * SH;
+    /* working buffers 308 B/px; two frame pyramids 2 x 60 B/px x 4/3;
+     * two snapshot sets 2 x 44 B/px x 4/3; exemplar textures 40 B/texel;
+     * style pyramids 52 B/texel x 4/3; plus 10% slack */
+    double b = px * (308.0 + 2 * 60.0 * 4.0 / 3.0 + 2 * 44.0 * 4.0 / 3.0) + ex * (40.0 + 52.0 * 4.0 / 3.0);
+    return (size_t)(b * 1.1) + ((size_t)8 << 20);
 }
 
 extern "C" int pt_synthesize(pt_ctx *c, const pt_params *params, int scheme, int iterations, int fixup_passes,

@@ -6,6 +6,8 @@ prefix, so ``OracleContext`` is a ``painttween_amd.Context`` bound to it.
 import ctypes as C
 import os
 
+import numpy as np
+
 from painttween_amd import _ffi
 from painttween_amd.texsynth import Context, PtError
 
@@ -34,6 +36,12 @@ def load_oracle():
         lib.ptref_atan2f.argtypes = [C.c_float, C.c_float]
         lib.ptref_tensor_to_angle.restype = C.c_float
         lib.ptref_tensor_to_angle.argtypes = [C.c_float, C.c_float, C.c_float]
+        lib.ptref_synthesize_next_frame.restype = C.c_int
+        lib.ptref_synthesize_next_frame.argtypes = [C.c_void_p, C.POINTER(_ffi.PtParams), C.c_int, C.c_int, C.c_int,
+                                                    C.POINTER(_ffi.FrameGuides), C.c_int, C.c_int,
+                                                    C.POINTER(_ffi.StyleGuides), C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ptref_sequence_reset.restype = C.c_int
+        lib.ptref_sequence_reset.argtypes = [C.c_void_p]
         lib.ptref_philox.restype = None
         lib.ptref_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         _LIB = lib
@@ -53,6 +61,33 @@ class OracleContext(Context):
         p = _ffi.PatchXF(*[cand[k] for k in ("x", "y", "theta", "hysteresis", "layer", "scale_u",
                                                "scale_v", "lum_y", "lum_a")])
         return self.lib.ptref_patch_distance(self.handle, x, y, level, C.byref(p))
+
+    def next_frame(self, params, iterations, frame, style):
+        """ptref_synthesize_next_frame on host arrays; returns (offsets,
+        canvas, residual) of the frame."""
+        keep = []
+
+        def arr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float32)
+            keep.append(a)
+            return a
+
+        h, w = frame["color"].shape[:2]
+        sh, sw = style["base"].shape[:2]
+        fg = _ffi.FrameGuides(_ffi.fptr(arr(frame["color"])), _ffi.fptr(arr(frame.get("orientation"))),
+                              _ffi.fptr(arr(frame.get("dist_trans"))), _ffi.fptr(arr(frame.get("scale"))),
+                              _ffi.fptr(arr(frame.get("vel_b"))), _ffi.fptr(arr(frame.get("vel_f"))))
+        sg = _ffi.StyleGuides(_ffi.fptr(arr(style["base"])), _ffi.fptr(arr(style["output"])),
+                              _ffi.fptr(arr(style.get("orientation"))), _ffi.fptr(arr(style.get("dist_trans"))))
+        off = np.zeros((h, w), _ffi.PATCHXF_DTYPE)
+        canvas = np.zeros((h, w, 4), np.float32)
+        res = np.zeros((h, w), np.float32)
+        self._check(self.lib.ptref_synthesize_next_frame(self.handle, C.byref(params), iterations, w, h, C.byref(fg),
+                                                         sw, sh, C.byref(sg), _ffi.vptr(off), _ffi.vptr(canvas),
+                                                         _ffi.vptr(res)))
+        return off, canvas, res
 
     def gaussian_weights(self, r):
         out = (C.c_float * (2 * r + 1))()

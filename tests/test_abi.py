@@ -43,6 +43,8 @@ def test_struct_layouts():
     assert C.sizeof(_ffi.PatchXF) == 36 == lib.pt_struct_size(1)      # PatchXF, types.h:98-114
     assert C.sizeof(_ffi.FrameGuides) == lib.pt_struct_size(2)
     assert C.sizeof(_ffi.StyleGuides) == lib.pt_struct_size(3)
+    for which, cls in _ffi.STRUCTS.items():
+        assert C.sizeof(cls) == lib.pt_struct_size(which), cls.__name__
     assert lib.pt_struct_size(99) == 0
 
 
@@ -66,3 +68,19 @@ def test_errors_are_returned_not_fatal():
     assert rc < 0
     assert lib.pt_global_last_error()
     assert lib.pt_propagate(None, 0, 1, -1) < 0
+    # batch and server entry points reject bad arguments without a GPU call
+    b = C.c_void_p()
+    assert lib.pt_batch_create(C.byref(b), 0, 0, None) < 0
+    assert lib.pt_batch_run(None) < 0
+    assert lib.pt_batch_last_error(None)
+    s = C.c_void_p()
+    assert lib.pt_server_create(C.byref(s), 0, 0, 1) < 0
+    assert lib.pt_server_submit(None, 0, None, None, None, None, None) < 0
+
+
+def test_batch_and_server_surface_declared():
+    names = declared_functions()
+    for n in ("pt_batch_create", "pt_batch_run", "pt_batch_tune", "pt_submit_batch", "pt_server_create",
+              "pt_server_open", "pt_server_submit", "pt_server_poll", "pt_server_drain", "pt_server_close",
+              "pt_synthesize_next_frame", "pt_sequence_reset"):
+        assert n in names, n
