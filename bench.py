@@ -4,13 +4,25 @@ One step = one stylized 1024x1024 still frame through the full reference
 schedule (device box pyramids of every guide, random init, 5 levels of
 jump-flood propagation + random search with 7x7 guided patches, 8 iterations
 per level, upsampling between levels), inputs already resident in HBM.
-Prints one JSON line.  See DESIGN.md for the roofline figures used.
+Beside it, on the same GPU in the same run:
+  batched -- 64 uneven single-frame jobs (128^2..1024^2, aspect 0.5..2,
+             5x5/7x7 patches, 3-5 levels, 6 iterations) planned as one
+             multi-job launch sequence, shapes autotuned, replayed as a HIP
+             graph; compared per job, bit for bit, with the plain path;
+  served  -- 32 streamed sequences (256..1920 x 256..1080, exemplars
+             512..1024, 7x7 patches, 4 levels, 3 iterations, temporal
+             coherence 0.5) under Poisson frame arrivals, continuous
+             batching in the server loop, p50/p99 latency and frames/s
+             against the plain per-frame path.
+Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-levels L]
+                  [--no-batch] [--no-serve] [--serve-rates 4,20,100,400]
 """
 import argparse
 import ctypes as C
 import json
+import multiprocessing as mp
 import os
 import sys
 import time
@@ -61,6 +73,191 @@ def cpu_baseline(style, frame, params, iterations, cpu_levels, threads):
     return dt, pdes
 
 
+def _pool_map(fn, items):
+    """Host-side input generation in worker processes (before any GPU use)."""
+    n = max(1, min(16, os.cpu_count() or 1, len(items)))
+    with mp.get_context("spawn").Pool(n) as pool:
+        return pool.map(fn, items)
+
+
+def _gen_batch(spec):
+    from painttween_amd import workloads as W
+    return W.batch_job(spec)
+
+
+def _gen_seq(arg):
+    from painttween_amd import workloads as W
+    spec, frames = arg
+    return W.serving_sequence(spec, frames)
+
+
+def run_batched(ctx, jobs_in, steps):
+    """Plain path (one context per job, jobs one after another) against the
+    batched form on the same device inputs."""
+    import torch
+    from painttween_amd import _ffi
+    from painttween_amd.forms import Batch, DeviceOutputs
+    lib = ctx.lib
+    jobs = []
+    for spec, (frame, style) in jobs_in:
+        kw = dict(residual_window_size=spec["r"], levels=spec["levels"], dist_trans_weight=0.5,
+                  input_analogy_weight=1.0, orientation_enabled=1, canvas_output_weight=1.0)
+        jobs.append(dict(params=ctx.default_params(**kw), frames=[frame], style=style,
+                         iterations=spec["iterations"], fixup_passes=0))
+    b = Batch(jobs)
+    # plain path on the same device inputs
+    from painttween_amd import Context
+    ctxs = [Context.gpu(0) for _ in jobs]
+    plain_out = [DeviceOutputs(o.w, o.h, 1) for o in b.outputs]
+
+    def plain_all():
+        for i, j in enumerate(jobs):
+            out = plain_out[i]
+            o, c, r = out.pointers()
+            ctxs[i]._check(lib.pt_synthesize_device(
+                ctxs[i].handle, C.byref(j["params"]), _ffi.PT_S_FMBM_RM, j["iterations"], 0, out.w, out.h, 1,
+                b._frames[i], jobs_in[i][0]["sw"], jobs_in[i][0]["sh"], C.pointer(b._styles[i]), o, c, r))
+
+    plain_all()                       # warm: allocations
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_all()
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    print("batched: plain path %.1f ms for %d jobs" % (plain_s * 1e3, len(jobs)), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b.tune(1)
+    tune_s = time.perf_counter() - t0
+    b.run()                           # captures the graph
+    times = []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        b.run()
+        times.append(time.perf_counter() - t0)
+        print("batched run %d: %.1f ms" % (k, times[-1] * 1e3), file=sys.stderr, flush=True)
+    exact = all(torch.equal(b.outputs[i].off, plain_out[i].off) and
+                torch.equal(b.outputs[i].canvas, plain_out[i].canvas) and
+                torch.equal(b.outputs[i].res, plain_out[i].res) for i in range(len(jobs)))
+    st = b.stats()
+    px = sum(o.w * o.h for o in b.outputs)
+    batch_s = float(np.median(times))
+    out = {"jobs": len(jobs), "pixels": px, "jobs_per_s": len(jobs) / batch_s, "ms_per_batch": batch_s * 1e3,
+           "plain_jobs_per_s": len(jobs) / plain_s, "plain_ms": plain_s * 1e3, "speedup_vs_plain": plain_s / batch_s,
+           "bit_exact_vs_plain": bool(exact), "launches": st["launches"], "multi_launches": st["multi_launches"],
+           "ticks": st["ticks"], "recorded_ops": st["ops"], "tuned_shapes": st["shapes"], "tune_s": tune_s,
+           "hip_graph": True}
+    b.close()
+    for c in ctxs:
+        c.close()
+    return out
+
+
+def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0):
+    """The serving loop under Poisson arrivals against the plain per-frame
+    path (one context per sequence, frames one at a time in arrival order)."""
+    import torch
+    from painttween_amd import Context
+    from painttween_amd.forms import DeviceImages, DeviceOutputs, Server, next_frame
+    iterations = 3
+    kw = dict(residual_window_size=3, levels=4, temporal_coherence_weight=0.5, dist_trans_weight=0.5,
+              input_analogy_weight=1.0, orientation_enabled=1, canvas_output_weight=1.0)
+    params = ctx.default_params(**kw)
+    img = DeviceImages(0)
+    dev = []
+    for spec, (fr, st) in seqs_in:
+        dev.append(dict(w=spec["w"], h=spec["h"], sw=spec["s"], sh=spec["s"], style=img.style(st),
+                        frames=[img.frame(f) for f in fr]))
+    N = len(dev)
+    order = [(k, f) for f in range(frames) for k in range(N)]       # arrival order
+    torch.cuda.synchronize()
+    # plain per-frame path
+    pctx = [Context.gpu(0) for _ in range(N)]
+    first_round = {}
+    service = []
+    for rnd in range(2):          # round 0 warms allocations, round 1 is timed
+        for (k, f) in order:
+            d = dev[k]
+            out = DeviceOutputs(d["w"], d["h"], 1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            next_frame(pctx[k], params, iterations, d["frames"][f], d["style"], d["sw"], d["sh"], out)
+            dt = time.perf_counter() - t0
+            if rnd == 0:
+                first_round[(k, f)] = out
+            else:
+                service.append(dt)
+    print("served: plain per-frame path %.1f ms for %d frames" % (sum(service) * 1e3, len(service)),
+          file=sys.stderr, flush=True)
+    for c in pctx:
+        c.close()
+    # server: one warm round (compared with the plain round 0), then the rates
+    srv = Server(int(budget_gb * (1 << 30)), N)
+    qs = [srv.open_device(params, iterations, d["w"], d["h"], d["style"], d["sw"], d["sh"]) for d in dev]
+    tickets = {}
+    for (k, f) in order:
+        tickets[srv.submit(qs[k], dev[k]["frames"][f])] = (k, f)
+    srv.drain()
+    exact = True
+    for t, (k, f) in tickets.items():
+        o = srv.outputs[t]
+        p = first_round[(k, f)]
+        exact = exact and torch.equal(o.off, p.off) and torch.equal(o.canvas, p.canvas) and torch.equal(o.res, p.res)
+    srv.outputs.clear()
+    while srv.poll(max_ticks=0, max_done=1024):   # completions of the warm round
+        pass
+    rng = np.random.default_rng(1234)
+    per_rate = []
+    for lam in rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(order)))
+        arrivals -= arrivals[0]
+        outs = {}
+        lat = []
+        t0 = time.perf_counter()
+        i = 0
+        done = 0
+        last = t0
+        while done < len(order):
+            now = time.perf_counter() - t0
+            while i < len(order) and arrivals[i] <= now:
+                k, f = order[i]
+                t = srv.submit(qs[k], dev[k]["frames"][f])
+                outs[t] = (k, f)
+                i += 1
+            got = srv.poll(max_ticks=4)
+            for t, ms in got:
+                if t in outs:
+                    lat.append(ms)
+                    done += 1
+                    last = time.perf_counter()
+            if not got and i < len(order) and srv.stats()["in_flight"] == 0:
+                time.sleep(max(0.0, min(0.002, arrivals[i] - (time.perf_counter() - t0))))
+        span = last - t0
+        srv.outputs.clear()
+        # plain FIFO single server with the measured service times
+        c_prev, plat = 0.0, []
+        for a, s_ in zip(arrivals, service):
+            c_prev = max(a, c_prev) + s_
+            plat.append((c_prev - a) * 1e3)
+        pspan = c_prev
+        r = {"rate_fps": lam, "frames": len(order), "served_fps": len(order) / span,
+             "served_p50_ms": float(np.percentile(lat, 50)), "served_p99_ms": float(np.percentile(lat, 99)),
+             "plain_fps": len(order) / pspan, "plain_p50_ms": float(np.percentile(plat, 50)),
+             "plain_p99_ms": float(np.percentile(plat, 99))}
+        r["served_vs_plain_fps"] = r["served_fps"] / r["plain_fps"]
+        per_rate.append(r)
+        print("served at %g fps offered: %.2f fps, p50 %.0f ms, p99 %.0f ms (plain %.2f fps, p99 %.0f ms)"
+              % (lam, r["served_fps"], r["served_p50_ms"], r["served_p99_ms"], r["plain_fps"], r["plain_p99_ms"]),
+              file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    px = sum(d["w"] * d["h"] for d in dev)
+    return {"sequences": N, "frames_per_sequence_per_round": frames, "pixels_per_round": px * frames,
+            "plain_fps": len(service) / sum(service), "bit_exact_vs_plain": bool(exact), "rates": per_rate,
+            "budget_gb": budget_gb, "resident_bytes": st["resident_bytes"], "launches": st["launches"],
+            "multi_launches": st["multi_launches"], "ticks": st["ticks"],
+            "schedule": "per-frame FMBM_RM first pass, 7x7 patches, 4 levels, 3 iterations, temporal coherence 0.5"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +265,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-levels", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve-seqs", type=int, default=32)
+    ap.add_argument("--serve-frames", type=int, default=3)
+    ap.add_argument("--serve-rates", default="4,20,100,400")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the synthesis path runs on a single MI355X "
@@ -79,6 +283,17 @@ def main():
     from painttween_amd._ffi import FrameGuides, StyleGuides, PATCHXF_DTYPE
 
     style, frame, pk, iterations = workload()
+    # inputs of the batched and served forms, generated before any GPU use
+    from painttween_amd import workloads as W
+    batch_in = serve_in = None
+    if not args.no_batch:
+        specs = W.batch_specs(args.batch_jobs)
+        batch_in = list(zip(specs, _pool_map(_gen_batch, specs)))
+        print("batched inputs generated", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        sspecs = W.serving_specs(args.serve_seqs)
+        serve_in = list(zip(sspecs, _pool_map(_gen_seq, [(sp, args.serve_frames) for sp in sspecs])))
+        print("served inputs generated", file=sys.stderr, flush=True)
     h, w = frame["color"].shape[:2]
     sh, sw = style["base"].shape[:2]
     dev = torch.device("cuda:0")
@@ -185,6 +400,11 @@ def main():
         "kernels": kstats,
         "cpu_baseline": cpu,
     }
+    if batch_in is not None:
+        line["batched"] = run_batched(ctx, batch_in, args.batch_steps)
+    if serve_in is not None:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        line["served"] = run_served(ctx, serve_in, args.serve_frames, rates)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -428,6 +428,9 @@ int pt_batch_tune(pt_batch *b, int rounds);
 #define PT_BATCH_USE_GRAPH 1 /* default 1: capture/replay the plan as a HIP graph */
 #define PT_BATCH_XCD_REMAP 2 /* default 1: deal contiguous tile ranges to each XCD */
 #define PT_BATCH_SHAPE 3     /* value = op*256 + class*16 + shape (shape 15 = default policy) */
+/* default 2048: a job step of at least this many workgroup tiles fills the
+ * GPU alone and gets a launch of its own instead of sharing one */
+#define PT_BATCH_ISOLATE_TILES 4
 int pt_batch_set(pt_batch *b, int key, int value);
 
 typedef struct pt_batch_stats {
@@ -435,6 +438,7 @@ typedef struct pt_batch_stats {
     double last_run_ms;  /* host wall time of the last pt_batch_run */
     double tune_ms;      /* total time spent in pt_batch_tune */
     int32_t shape[2][3]; /* chosen shape per (propagate, scatter) x size class; -1 = default */
+    int32_t xcd_remap, isolate_tiles;
 } pt_batch_stats;
 int pt_batch_get_stats(pt_batch *b, pt_batch_stats *out);
 
@@ -485,6 +489,9 @@ int pt_server_poll(pt_server *s, int max_ticks, uint64_t *done, double *latency_
 int pt_server_drain(pt_server *s);
 /* Releases a sequence's device state once its queued frames are done. */
 int pt_server_close(pt_server *s, int32_t seq);
+/* Launch settings of the engine, keys as pt_batch_set: PT_BATCH_XCD_REMAP
+ * (default 0 here), PT_BATCH_SHAPE, PT_BATCH_ISOLATE_TILES. */
+int pt_server_set(pt_server *s, int key, int value);
 
 typedef struct pt_server_stats {
     int64_t submitted, completed, ticks, launches, multi_launches;

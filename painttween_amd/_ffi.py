@@ -112,7 +112,8 @@ class PtJob(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("jobs", C.c_int64), ("ops", C.c_int64), ("launches", C.c_int64),
                 ("multi_launches", C.c_int64), ("ticks", C.c_int64), ("last_run_ms", C.c_double),
-                ("tune_ms", C.c_double), ("shape", (C.c_int32 * 3) * 2)]
+                ("tune_ms", C.c_double), ("shape", (C.c_int32 * 3) * 2), ("xcd_remap", C.c_int32),
+                ("isolate_tiles", C.c_int32)]
 
 
 class SequenceDesc(C.Structure):
@@ -132,7 +133,7 @@ class ServerStats(C.Structure):
 STRUCTS = {0: PtParams, 1: PatchXF, 2: FrameGuides, 3: StyleGuides, 4: PtJob, 5: BatchStats,
            6: SequenceDesc, 7: ServerStats}
 
-PT_BATCH_USE_GRAPH, PT_BATCH_XCD_REMAP, PT_BATCH_SHAPE = 1, 2, 3
+PT_BATCH_USE_GRAPH, PT_BATCH_XCD_REMAP, PT_BATCH_SHAPE, PT_BATCH_ISOLATE_TILES = 1, 2, 3, 4
 PT_OP_PROPAGATE, PT_OP_SCATTER = 0, 1
 SHAPE_NAMES = ["pixel", "pixel_ee", "unit_ee_a", "unit_a", "unit_ee_b", "unit_b"]
 
@@ -256,6 +257,7 @@ def load_library():
             ("pt_server_drain", I, [VP]),
             ("pt_server_close", I, [VP, C.c_int32]),
             ("pt_server_get_stats", I, [VP, C.POINTER(ServerStats)]),
+            ("pt_server_set", I, [VP, I, I]),
         ]
         for name, res, args in extra:
             fn = getattr(lib, name)

@@ -50,6 +50,10 @@ int size_class(const PtArgs &a)
 
 struct ShapePolicy {
     int shape[2][3]; /* [propagate, scatter][class]; -1 = default policy */
+    /* a job step with at least this many tiles fills the GPU alone: it gets
+     * a launch of its own (sharing one launch would only spread the gathers
+     * over several exemplars at once) */
+    int isolate_tiles = 2048;
     ShapePolicy()
     {
         for (auto &r : shape)
@@ -127,7 +131,8 @@ void build_tick(std::vector<Lane *> &lanes, const ShapePolicy &pol, std::vector<
         int op, shape, cls;
         std::vector<int> lanes;
     };
-    std::map<int, Group> groups; /* ordered: deterministic launch order */
+    /* ordered: deterministic launch order; second = lane of an isolated step */
+    std::map<std::pair<int, int>, Group> groups;
     for (size_t li = 0; li < lanes.size(); li++) {
         Lane *L = lanes[li];
         if (!L || L->done()) continue;
@@ -135,7 +140,8 @@ void build_tick(std::vector<Lane *> &lanes, const ShapePolicy &pol, std::vector<
         int shape = pol.pick(o.op, o.a);
         int cls = size_class(o.a);
         int key = pt_shape_key(o.op, shape, o.a) * 4 + cls;
-        Group &g = groups[key];
+        bool alone = pt_shape_tiles(o.op, shape, o.a, nullptr) >= pol.isolate_tiles;
+        Group &g = groups[std::make_pair(key, alone ? (int)li : -1)];
         g.op = o.op;
         g.shape = shape;
         g.cls = cls;
@@ -389,6 +395,7 @@ extern "C" int pt_batch_tune(pt_batch *b, int rounds)
     int64_t units[2][3] = {};
     for (int s = 0; s < PT_SHAPE_COUNT; s++) {
         ShapePolicy p;
+        p.isolate_tiles = keep.isolate_tiles;
         for (int op = 0; op < 2; op++)
             for (int c = 0; c < 3; c++) p.shape[op][c] = s;
         b->pol = p;
@@ -417,6 +424,7 @@ extern "C" int pt_batch_tune(pt_batch *b, int rounds)
     /* a shape qualifies for (op, class) only if it served every job-step
      * of that class in its run (valid everywhere) */
     ShapePolicy best;
+    best.isolate_tiles = keep.isolate_tiles;
     for (int op = 0; op < 2; op++)
         for (int c = 0; c < 3; c++) {
             double bt = 0.0;
@@ -431,6 +439,24 @@ extern "C" int pt_batch_tune(pt_batch *b, int rounds)
             best.shape[op][c] = bs;
         }
     b->pol = best;
+    /* then the XCD tile mapping, on whole-batch time */
+    int rc = build_program(b);
+    if (rc != PT_OK) return rc;
+    double tx[2] = {0.0, 0.0};
+    for (int x = 0; x < 2; x++) {
+        b->xcd = x;
+        for (int r = 0; r < rounds; r++) {
+            double t1 = now_ms();
+            rc = issue_program(b, nullptr);
+            if (rc == PT_OK) {
+                hipError_t e = hipStreamSynchronize(b->stream);
+                if (e != hipSuccess) rc = bfail(b, PT_ERR_HIP, "tune run failed: %s", hipGetErrorString(e));
+            }
+            if (rc != PT_OK) return rc;
+            tx[x] += now_ms() - t1;
+        }
+    }
+    b->xcd = tx[1] <= tx[0] ? 1 : 0;
     b->prog_valid = false;
     drop_graph(b);
     b->tune_ms += now_ms() - t0;
@@ -443,6 +469,10 @@ extern "C" int pt_batch_set(pt_batch *b, int key, int value)
     switch (key) {
     case PT_BATCH_USE_GRAPH: b->use_graph = value != 0; break;
     case PT_BATCH_XCD_REMAP: b->xcd = value != 0; break;
+    case PT_BATCH_ISOLATE_TILES:
+        if (value < 1) return bfail(b, PT_ERR_INVALID_ARG, "isolate threshold must be >= 1");
+        b->pol.isolate_tiles = value;
+        break;
     case PT_BATCH_SHAPE: {
         int op = value / 256, cls = (value / 16) % 16, shape = value % 16;
         if (op < 0 || op > 1 || cls > 2 || (shape >= PT_SHAPE_COUNT && shape != 15))
@@ -475,6 +505,8 @@ extern "C" int pt_batch_get_stats(pt_batch *b, pt_batch_stats *o)
     o->tune_ms = b->tune_ms;
     for (int op = 0; op < 2; op++)
         for (int c = 0; c < 3; c++) o->shape[op][c] = b->pol.shape[op][c];
+    o->xcd_remap = b->xcd;
+    o->isolate_tiles = b->pol.isolate_tiles;
     return PT_OK;
 }
 
@@ -538,6 +570,7 @@ struct pt_server {
     std::vector<hipEvent_t> slot_ev;
     std::vector<bool> slot_used;
     int slot_next = 0;
+    int xcd = 0;
     std::vector<std::pair<uint64_t, double>> completed;
     int64_t submitted = 0, ncompleted = 0, ticks = 0, launches = 0, multi = 0;
 };
@@ -721,14 +754,14 @@ static int tick(pt_server *s)
                                 hipMemcpyHostToDevice, s->stream));
         dev = s->ring_dev + k * s->slot_bytes;
         for (const Step &st : steps) {
-            hipError_t e = issue_step(st, lanes, dev, 1, s->stream);
+            hipError_t e = issue_step(st, lanes, dev, s->xcd, s->stream);
             if (e != hipSuccess) return sfail(s, PT_ERR_HIP, "tick launch failed: %s", hipGetErrorString(e));
         }
         S_HIP(s, hipEventRecord(s->slot_ev[k], s->stream));
         s->slot_used[k] = true;
     } else {
         for (const Step &st : steps) {
-            hipError_t e = issue_step(st, lanes, dev, 1, s->stream);
+            hipError_t e = issue_step(st, lanes, dev, s->xcd, s->stream);
             if (e != hipSuccess) return sfail(s, PT_ERR_HIP, "tick launch failed: %s", hipGetErrorString(e));
         }
     }
@@ -830,6 +863,26 @@ extern "C" int pt_server_close(pt_server *s, int32_t seq)
         release_seq(s, q);
     }
     return PT_OK;
+}
+
+extern "C" int pt_server_set(pt_server *s, int key, int value)
+{
+    if (!s) return sfail(nullptr, PT_ERR_INVALID_ARG, "NULL server");
+    switch (key) {
+    case PT_BATCH_XCD_REMAP: s->xcd = value != 0; return PT_OK;
+    case PT_BATCH_ISOLATE_TILES:
+        if (value < 1) return sfail(s, PT_ERR_INVALID_ARG, "isolate threshold must be >= 1");
+        s->pol.isolate_tiles = value;
+        return PT_OK;
+    case PT_BATCH_SHAPE: {
+        int op = value / 256, cls = (value / 16) % 16, shape = value % 16;
+        if (op < 0 || op > 1 || cls > 2 || (shape >= PT_SHAPE_COUNT && shape != 15))
+            return sfail(s, PT_ERR_INVALID_ARG, "bad shape setting %d", value);
+        s->pol.shape[op][cls] = shape == 15 ? -1 : shape;
+        return PT_OK;
+    }
+    default: return sfail(s, PT_ERR_INVALID_ARG, "unknown server key %d", key);
+    }
 }
 
 extern "C" int pt_server_get_stats(pt_server *s, pt_server_stats *o)

@@ -134,7 +134,8 @@ class Batch:
         for op, name in ((0, "propagate"), (1, "scatter")):
             shapes[name] = [(_ffi.SHAPE_NAMES[s] if s >= 0 else "default") for s in st.shape[op]]
         return dict(jobs=st.jobs, ops=st.ops, launches=st.launches, multi_launches=st.multi_launches,
-                    ticks=st.ticks, last_run_ms=st.last_run_ms, tune_ms=st.tune_ms, shapes=shapes)
+                    ticks=st.ticks, last_run_ms=st.last_run_ms, tune_ms=st.tune_ms, shapes=shapes,
+                    xcd_remap=st.xcd_remap, isolate_tiles=st.isolate_tiles)
 
     def result(self, j):
         return self.outputs[j].numpy()
@@ -202,6 +203,9 @@ class Server:
 
     def drain(self):
         self._check(self.lib.pt_server_drain(self.handle))
+
+    def set(self, key, value):
+        self._check(self.lib.pt_server_set(self.handle, key, value))
 
     def close_sequence(self, seq):
         self._check(self.lib.pt_server_close(self.handle, seq))

@@ -21,7 +21,8 @@ def _rng(seed, stream):
 
 def sin_noise(seed, stream, w, h, c=4):
     rng = _rng(seed, stream)
-    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    x = np.arange(w, dtype=np.float64)
+    y = np.arange(h, dtype=np.float64)
     out = np.empty((h, w, c), np.float32)
     for k in range(c):
         a = rng.uniform(0.5, 1.0, 4)
@@ -30,7 +31,10 @@ def sin_noise(seed, stream, w, h, c=4):
         ph = rng.uniform(0.0, 2 * np.pi, 4)
         s = np.zeros((h, w))
         for i in range(4):
-            s += a[i] * np.sin(2 * np.pi * (fx[i] * xx / w + fy[i] * yy / h) + ph[i])
+            # sin(u + v) = sin u cos v + cos u sin v with u along x, v along y
+            u = 2 * np.pi * fx[i] * x / w + ph[i]
+            v = 2 * np.pi * fy[i] * y / h
+            s += a[i] * (np.outer(np.cos(v), np.sin(u)) + np.outer(np.sin(v), np.cos(u)))
         v = 0.9 * (0.5 + 0.5 * s / a.sum()) + 0.1 * rng.uniform(0.0, 1.0, (h, w))
         out[..., k] = v
     return out
@@ -53,10 +57,10 @@ def distance_transform(seed, stream, w, h, points=64):
     rng = _rng(seed, stream)
     px = rng.uniform(0, w, points)
     py = rng.uniform(0, h, points)
+    from scipy.spatial import cKDTree
     yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
-    d = np.full((h, w), np.inf)
-    for i in range(points):
-        d = np.minimum(d, np.hypot(xx - px[i], yy - py[i]))
+    d, _ = cKDTree(np.stack([px, py], axis=1)).query(np.stack([xx.ravel(), yy.ravel()], axis=1))
+    d = d.reshape(h, w)
     return (d / d.max()).astype(np.float32)
 
 
@@ -97,6 +101,7 @@ def animation(w, h, frames, seed, guides=True):
     out = []
     shift = np.zeros(2)
     yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    dt = distance_transform(seed, 21, w, h) if guides else None
     for t in range(frames):
         m = _motion(seed, t, w, h)
         sx, sy = shift
@@ -115,7 +120,54 @@ def animation(w, h, frames, seed, guides=True):
             fr["vel_b"] = np.zeros((h, w, 2), np.float32)
         if guides:
             fr["orientation"] = orientation_tensor(seed, 20 + t, w, h)
-            fr["dist_trans"] = distance_transform(seed, 21, w, h)
+            fr["dist_trans"] = dt
         out.append(fr)
         shift += (3.0, 1.0)
     return out
+
+
+def _log_uniform(rng, lo, hi):
+    return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+
+def batch_specs(n=64, seed=4, iterations=6):
+    """Jobs of the batched workload: area log-uniform 128^2..1024^2, aspect
+    0.5..2, patch radius 2 or 3 (5x5 / 7x7), 3 to 5 levels, exemplar area
+    log-uniform 256^2..1024^2 (square); job j uses seed `seed + j`."""
+    rng = np.random.default_rng([int(seed), 0xBA7C])
+    specs = []
+    for j in range(n):
+        area = _log_uniform(rng, 128 * 128, 1024 * 1024)
+        aspect = _log_uniform(rng, 0.5, 2.0)
+        w = max(16, int(round(np.sqrt(area * aspect))))
+        h = max(16, int(round(np.sqrt(area / aspect))))
+        r = int(rng.choice([2, 3]))
+        levels = int(rng.integers(3, 6))
+        s = int(round(np.sqrt(_log_uniform(rng, 256 * 256, 1024 * 1024))))
+        levels = min(levels, int(np.floor(np.log2(min(w, h)))), int(np.floor(np.log2(s))))
+        specs.append(dict(w=w, h=h, r=r, levels=levels, sw=s, sh=s, seed=seed + j, iterations=iterations))
+    return specs
+
+
+def batch_job(spec):
+    """(frame, style) of one batched job, with orientation and DT guides."""
+    return (still_frame(spec["w"], spec["h"], seed=spec["seed"], guides=True),
+            style_pair(spec["sw"], spec["sh"], seed=spec["seed"], guides=True))
+
+
+def serving_specs(n=32, seed=5):
+    """Sequences of the serving workload: frame sizes uniform 256..1920 x
+    256..1080, square exemplars 512..1024; sequence k uses seed `seed + k`."""
+    rng = np.random.default_rng([int(seed), 0x5E7F])
+    specs = []
+    for k in range(n):
+        specs.append(dict(w=int(rng.integers(256, 1921)), h=int(rng.integers(256, 1081)),
+                          s=int(rng.integers(512, 1025)), seed=seed + k))
+    return specs
+
+
+def serving_sequence(spec, frames):
+    """(frames, style) of one served sequence: the animation motion model
+    (rotation 0.5 deg/frame, translation (3, 1), smooth noise 2 px)."""
+    return (animation(spec["w"], spec["h"], frames, seed=spec["seed"]),
+            style_pair(spec["s"], spec["s"], seed=spec["seed"], guides=True))
