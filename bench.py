@@ -39,6 +39,20 @@ FLOP_PER_TAP = 102          # canonical per-tap FLOP count of the patch distance
 NNF_BYTES_PER_PX = 96       # algorithmic HBM bytes per pixel per propagate step
 
 
+def measured_traffic(kernel="propagate"):
+    """HBM-side bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy), or
+    None when absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_still1024.md")
+    try:
+        for line in open(path):
+            if line.startswith(kernel + "_traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def workload():
     from painttween_amd import workloads as W
     style = W.style_pair(1024, 1024, seed=2, guides=True)
@@ -272,7 +286,19 @@ def main():
     ap.add_argument("--serve-seqs", type=int, default=32)
     ap.add_argument("--serve-frames", type=int, default=3)
     ap.add_argument("--serve-rates", default="4,20,100,400")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 256 MiB five times (calibrates rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE against known bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the synthesis path runs on a single MI355X "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -393,7 +419,9 @@ def main():
         "pde_per_s": pde_s,
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "roofline": {"bound": "hbm", "kernel": "propagate", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": "profiles/traffic_still1024.md (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
+                                       "calibrated by a plain copy)",
                      "avg_launch_ms": prop_avg_ms, "algorithmic_bytes_per_launch": prop_bytes},
         "valu": {"achieved": valu_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": valu_tflops / FP32_PEAK_TFLOPS, "flop_per_tap": FLOP_PER_TAP},
