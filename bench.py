@@ -5,6 +5,8 @@ schedule (device box pyramids of every guide, random init, 5 levels of
 jump-flood propagation + random search with 7x7 guided patches, 8 iterations
 per level, upsampling between levels), inputs already resident in HBM.
 Beside it, on the same GPU in the same run:
+  animation -- 24 frames 960x540, exemplar 512^2, FMBM_RM with forward and
+             backward passes, temporal coherence 0.5, 4 levels, 3 iterations;
   batched -- 64 uneven single-frame jobs (128^2..1024^2, aspect 0.5..2,
              5x5/7x7 patches, 3-5 levels, 6 iterations) planned as one
              multi-job launch sequence, shapes autotuned, replayed as a HIP
@@ -166,6 +168,53 @@ def run_batched(ctx, jobs_in, steps):
     return out
 
 
+def _gen_anim(arg):
+    from painttween_amd import workloads as W
+    w, h, frames, sw = arg
+    return W.animation(w, h, frames, seed=3), W.style_pair(sw, sw, seed=3, guides=True)
+
+
+def run_animation(ctx, anim_in, levels, iterations):
+    """The animation workload through pt_synthesize_device: FMBM_RM, two
+    passes (forward then backward advection), temporal coherence 0.5."""
+    import torch
+    from painttween_amd import _ffi
+    from painttween_amd.forms import DeviceImages, DeviceOutputs
+    frames, style = anim_in
+    h, w = frames[0]["color"].shape[:2]
+    sh, sw = style["base"].shape[:2]
+    F = len(frames)
+    img = DeviceImages(0)
+    fg = (_ffi.FrameGuides * F)()
+    for i, f in enumerate(frames):
+        fg[i] = img.frame(f)
+    sg = img.style(style)
+    out = DeviceOutputs(w, h, F)
+    kw = dict(residual_window_size=3, levels=levels, temporal_coherence_weight=0.5, dist_trans_weight=0.5,
+              input_analogy_weight=1.0, orientation_enabled=1, canvas_output_weight=1.0)
+    params = ctx.default_params(**kw)
+    o, c, r = out.pointers()
+
+    def run():
+        ctx._check(ctx.lib.pt_synthesize_device(ctx.handle, C.byref(params), _ffi.PT_S_FMBM_RM, iterations, 1, w, h,
+                                                F, fg, sw, sh, C.byref(sg), o, c, r))
+
+    run()
+    torch.cuda.synchronize()
+    pde0 = ctx.pde_count()
+    t0 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pdes = ctx.pde_count() - pde0
+    res = out.res.cpu().numpy()
+    assert np.isfinite(res).all() and (res >= 0).all(), "invalid residual"
+    return {"frames": F, "image": [w, h], "exemplar": [sw, sh], "levels": levels, "iterations": iterations,
+            "passes": 2, "seconds": dt, "frames_per_s": F / dt, "frame_passes_per_s": 2 * F / dt,
+            "pde_per_s": pdes / dt, "schedule": "FMBM_RM, fixup passes 1 (forward then backward), 7x7 patches, "
+                                                "temporal coherence 0.5, time derivatives 1"}
+
+
 def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0):
     """The serving loop under Poisson arrivals against the plain per-frame
     path (one context per sequence, frames one at a time in arrival order)."""
@@ -281,6 +330,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-anim", action="store_true")
+    ap.add_argument("--anim-levels", type=int, default=4)
+    ap.add_argument("--anim-iterations", type=int, default=3)
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--serve-seqs", type=int, default=32)
@@ -311,7 +363,10 @@ def main():
     style, frame, pk, iterations = workload()
     # inputs of the batched and served forms, generated before any GPU use
     from painttween_amd import workloads as W
-    batch_in = serve_in = None
+    batch_in = serve_in = anim_in = None
+    if not args.no_anim:
+        anim_in = _pool_map(_gen_anim, [(960, 540, 24, 512)])[0]
+        print("animation inputs generated", file=sys.stderr, flush=True)
     if not args.no_batch:
         specs = W.batch_specs(args.batch_jobs)
         batch_in = list(zip(specs, _pool_map(_gen_batch, specs)))
@@ -428,6 +483,9 @@ def main():
         "kernels": kstats,
         "cpu_baseline": cpu,
     }
+    if anim_in is not None:
+        line["animation"] = run_animation(ctx, anim_in, args.anim_levels, args.anim_iterations)
+        print("animation: %.2f frames/s" % line["animation"]["frames_per_s"], file=sys.stderr, flush=True)
     if batch_in is not None:
         line["batched"] = run_batched(ctx, batch_in, args.batch_steps)
     if serve_in is not None:
