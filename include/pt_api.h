@@ -287,6 +287,10 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * queue and resolve each pixel in candidate order afterwards; 0 evaluates
  * each pixel's candidates sequentially on its own lane.  Same results. */
 #define PT_TUNE_UNIT_PARALLEL 3
+/* PT_TUNE_INCREMENTAL_RESIDUAL (default 1): the residual refresh after a
+ * sync recomputes only pixels whose patch window holds an offset that sync
+ * changed; the others would reproduce their cached value bit for bit. */
+#define PT_TUNE_INCREMENTAL_RESIDUAL 4
 int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --

@@ -24,6 +24,7 @@ PT_ELEM_F32, PT_ELEM_F32X2, PT_ELEM_F32X3, PT_ELEM_F32X4 = 1, 2, 3, 4
 PT_TUNE_EARLY_EXIT = 1
 PT_TUNE_EARLY_EXIT_MIN_PIXELS = 2
 PT_TUNE_UNIT_PARALLEL = 3
+PT_TUNE_INCREMENTAL_RESIDUAL = 4
 PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT = 0, 1
 
 LAYER = dict(

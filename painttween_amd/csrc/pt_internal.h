@@ -58,6 +58,10 @@ struct PtArgs {
     int jump;          /* propagate: jump distance                     */
     int nscat;         /* scatter: random samples per pixel            */
     int dw, dh;        /* sync: rectangle where working and base differ */
+    int res_incremental; /* residual refresh: recompute only pixels whose
+                            patch window holds a changed offset */
+
+    uint8_t *changed;  /* per pixel: base offset changed by the last sync */
 
     PtNnf base, work;
     float *res_base, *res_work;

@@ -444,15 +444,70 @@ __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, floa
     return T;
 }
 
-/* updateResidualCache_kernel (texSynth_kernel.cu:661-693) */
+/*
+ * updateResidualCache_kernel (texSynth_kernel.cu:661-693).
+ *
+ * Incremental form: a pixel's residual depends on its own offset, the
+ * canvas inside its patch window (each canvas value depends only on that
+ * pixel's offset), and inputs that do not change between two refreshes of
+ * one level.  When the residual cache was exact before the last sync and no
+ * offset inside the window changed in that sync (A.changed, written by the
+ * sync pass), recomputing would reproduce the stored value bit for bit, so
+ * the pixel is skipped.  Pixels that need the refresh are compacted onto the
+ * first lanes of the workgroup so that whole waves skip.
+ */
 template <int RT, bool TK>
 __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int by)
 {
     __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
+    __shared__ unsigned char sflag[TileSize<RT>::N];
+    __shared__ unsigned short slist[PT_BX * PT_BY];
+    __shared__ int scount;
+    const int r = RT > 0 ? RT : A.r;
+    const bool inc = A.res_incremental != 0;
+    if (inc) {
+        const int tw = PT_BX + 2 * r, th = PT_BY + 2 * r;
+        const int x0 = bx * PT_BX - r, y0 = by * PT_BY - r;
+        for (int k = threadIdx.y * PT_BX + threadIdx.x; k < tw * th; k += PT_BX * PT_BY) {
+            int tx = clampi(x0 + k % tw, 0, A.lw - 1);
+            int ty = clampi(y0 + k / tw, 0, A.lh - 1);
+            sflag[k] = A.changed[ty * A.W + tx];
+        }
+        if (threadIdx.x == 0 && threadIdx.y == 0) scount = 0;
+    }
     int x, y;
     bool valid = pixel_of(A, bx, by, x, y);
-    Tile T = stage_tile<RT>(A, bx, by, scol, scan);
-    if (!valid) return;
+    Tile T = stage_tile<RT>(A, bx, by, scol, scan); /* its barrier also covers sflag and scount */
+    if (!inc) {
+        if (!valid) return;
+        int o = y * A.W + x;
+        if (image_culled(A, x, y)) {
+            A.res_base[o] = 0.0f;
+            return;
+        }
+        A.res_base[o] = calc_residual<RT, TK>(A, T, x, y, A.base.xyth[o], A.base.sc[o]);
+        return;
+    }
+    if (valid) {
+        /* clamped halo entries repeat edge flags: conservative */
+        bool need = false;
+        for (int j = 0; j <= 2 * r && !need; j++)
+            for (int i = 0; i <= 2 * r; i++)
+                if (sflag[(threadIdx.y + j) * T.tw + threadIdx.x + i]) {
+                    need = true;
+                    break;
+                }
+        if (need) slist[atomicAdd(&scount, 1)] = (unsigned short)(threadIdx.y * PT_BX + threadIdx.x);
+    }
+    __syncthreads();
+    const int n = scount;
+    const int lane = threadIdx.y * PT_BX + threadIdx.x;
+    if (lane >= n) return;
+    const int p = slist[lane];
+    x = bx * PT_BX + p % PT_BX;
+    y = by * PT_BY + p / PT_BX;
+    T.lx = p % PT_BX;
+    T.ly = p / PT_BX;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) {
         A.res_base[o] = 0.0f;
@@ -475,6 +530,7 @@ __device__ __forceinline__ void b_canvas(const PtArgs &A, int bx, int by)
     int o = y * A.W + x;
     float4 n = A.base.xyth[o];
     A.canvas[o] = tex_ex_out(A, n.x, n.y);
+    if (A.changed) A.changed[o] = 0;
 }
 
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_canvas(PtArgs A)
@@ -486,22 +542,32 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_canvas(PtArgs A)
  * (texSynth_kernel.cu:1244-1246,1274-1275): pixels of the copy rectangle
  * (dw x dh) take working -> base; pixels of the level rectangle then get
  * the canvas of their (new) base offset. */
+__device__ __forceinline__ bool bits_differ(float a, float b) { return __float_as_uint(a) != __float_as_uint(b); }
+
 __device__ __forceinline__ void b_sync_canvas(const PtArgs &A, int bx, int by, int dw, int dh)
 {
     int x = bx * PT_BX + threadIdx.x;
     int y = by * PT_BY + threadIdx.y;
     int o = y * A.W + x;
     float4 n;
-    bool have = false;
+    bool have = false, ch = false;
     if (x < dw && y < dh) {
         n = A.work.xyth[o];
+        float2 s = A.work.sc[o];
+        if (A.changed) {
+            float4 ob = A.base.xyth[o];
+            float2 os = A.base.sc[o];
+            ch = bits_differ(n.x, ob.x) || bits_differ(n.y, ob.y) || bits_differ(n.z, ob.z) ||
+                 bits_differ(n.w, ob.w) || bits_differ(s.x, os.x) || bits_differ(s.y, os.y);
+        }
         A.base.xyth[o] = n;
-        A.base.sc[o] = A.work.sc[o];
+        A.base.sc[o] = s;
         have = true;
     }
     if (x < A.lw && y < A.lh) {
         if (!have) n = A.base.xyth[o];
         A.canvas[o] = tex_ex_out(A, n.x, n.y);
+        if (A.changed) A.changed[o] = ch ? 1 : 0;
     }
 }
 

@@ -109,6 +109,12 @@ struct pt_ctx {
      * levels with enough waves to hide the latency. */
     int64_t early_exit_min_pixels = 512 * 512;
     bool unit_parallel = true; /* candidate units over workgroup lanes */
+    /* residual refresh: res_base is exact for the current offsets, canvas
+     * and inputs of level res_level (-1: unknown); `changed` marks the
+     * offsets the last sync changed */
+    uint8_t *changed = nullptr;
+    int res_level = -1;
+    bool incremental_residual = true;
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -322,6 +328,7 @@ static void free_working(pt_ctx *c)
     dfree(c->res_base);
     dfree(c->res_work);
     dfree(c->canvas);
+    dfree(c->changed);
     for (int k = 0; k < 2; k++) {
         dfree(c->adv[k].xyth);
         dfree(c->adv[k].sc);
@@ -358,6 +365,10 @@ extern "C" int pt_destroy(pt_ctx *c)
     delete c;
     return PT_OK;
 }
+
+/* Marks the residual cache as not known to be exact (any change of offsets,
+ * canvas inputs, textures or parameters other than a sync's copy). */
+static void res_invalidate(pt_ctx *c) { c->res_level = -1; }
 
 static int sync_stream(pt_ctx *c)
 {
@@ -486,6 +497,9 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
         if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative pixel count");
         c->early_exit_min_pixels = value;
         return PT_OK;
+    case PT_TUNE_INCREMENTAL_RESIDUAL:
+        c->incremental_residual = value != 0;
+        return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
@@ -497,6 +511,7 @@ extern "C" int pt_upload_parameters(pt_ctx *c, const pt_params *p)
     RC_CHECK(check_supported(c, p));
     c->p = *p;
     c->have_params = true;
+    res_invalidate(c);
     return PT_OK;
 }
 
@@ -543,6 +558,7 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
     RC_CHECK(dalloc(c, &c->res_base, n));
     RC_CHECK(dalloc(c, &c->res_work, n));
     RC_CHECK(dalloc(c, &c->canvas, n));
+    RC_CHECK(dalloc(c, &c->changed, n));
     for (int k = 0; k < 2; k++) {
         RC_CHECK(dalloc(c, &c->adv[k].xyth, n));
         RC_CHECK(dalloc(c, &c->adv[k].sc, n));
@@ -564,6 +580,7 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
     c->res_diff = Rect();
     c->adv_written[0] = c->adv_written[1] = false;
     c->cur = 0;
+    c->res_level = -1;
     return sync_stream(c);
 }
 
@@ -572,6 +589,7 @@ extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
     if (!c || w < 1 || h < 1) return set_err(c, PT_ERR_INVALID_ARG, "bad exemplar dimensions");
     if (num_layers != 1) return set_err(c, PT_ERR_UNSUPPORTED, "one style layer only");
     HIP_CHECK(c, hipSetDevice(c->device));
+    res_invalidate(c);
     if (c->ex && c->ew == w && c->eh == h) return PT_OK;
     size_t n = (size_t)w * h;
     if (n > c->ex_cap) {
@@ -597,6 +615,7 @@ extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
 
 static int reset_advected(pt_ctx *c)
 {
+    res_invalidate(c);
     size_t n = (size_t)c->W * c->H;
     for (int k = 0; k < 2; k++) {
         RC_CHECK(zero_fill(c, c->adv[k].xyth, n * sizeof(float4)));
@@ -621,6 +640,7 @@ extern "C" int pt_flip_anim_buffers(pt_ctx *c)
 {
     if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
     c->cur = (c->cur + 1) % 2;
+    res_invalidate(c);
     return PT_OK;
 }
 
@@ -756,6 +776,7 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
     int w = p->lw[level], h = p->lh[level];
     const float *src = p->data + p->off[level];
     int prev = (c->cur + 1) % 2;
+    res_invalidate(c);
     const int W = c->W;
     auto rect = [&](void *dst, int ch) -> int {
         if (w > c->W || h > c->H) return set_err(c, PT_ERR_INVALID_ARG, "level larger than image");
@@ -945,6 +966,8 @@ static PtArgs make_args(const pt_ctx *c, int level)
                   : 0;
     a.early_exit = (a.ee_ok && (int64_t)a.lw * a.lh >= c->early_exit_min_pixels) ? 1 : 0;
     a.unit_parallel = c->unit_parallel ? 1 : 0;
+    a.res_incremental = 0;
+    a.changed = c->changed;
     a.base = c->base;
     a.work = c->work;
     a.res_base = c->res_base;
@@ -999,9 +1022,11 @@ static int copy_base_to_working(pt_ctx *c)
     return PT_OK;
 }
 
-static int update_residual(pt_ctx *c, int level)
+static int update_residual(pt_ctx *c, int level, bool allow_incremental = false)
 {
     PtArgs a = make_args(c, level);
+    a.res_incremental = (allow_incremental && c->incremental_residual && c->res_level == level) ? 1 : 0;
+    c->res_level = level;
     int pr = prof_begin(c, PK_RESIDUAL, (uint64_t)a.lw * a.lh);
     RC_CHECK(issue_op(c, PT_OP_RESIDUAL, a));
     prof_end(c, pr);
@@ -1025,13 +1050,15 @@ static int sync_buffers(pt_ctx *c, int level, int flags)
     }
     RC_CHECK(issue_op(c, PT_OP_SYNC_CANVAS, a));
     prof_end(c, pr);
-    if (!(flags & PT_DONT_UPDATE_RESIDUAL)) RC_CHECK(update_residual(c, level));
+    if (!(flags & PT_DONT_UPDATE_RESIDUAL)) RC_CHECK(update_residual(c, level, true));
+    else res_invalidate(c);
     return PT_OK;
 }
 
 static int randomize_offsets(pt_ctx *c, int level)
 {
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
+    res_invalidate(c);
     PtArgs a = make_args(c, level);
     int pr = prof_begin(c, PK_RANDOM, (uint64_t)a.lw * a.lh);
     RC_CHECK(issue(c, "random copy", [=](hipStream_t s) { return pt_launch_copy_random(a, s); }));
@@ -1078,6 +1105,7 @@ static int scatter(pt_ctx *c, int level)
 
 static int upsample(pt_ctx *c, int fine_level)
 {
+    res_invalidate(c);
     PtArgs a = make_args(c, fine_level);
     int pr = prof_begin(c, PK_UPSAMPLE, (uint64_t)a.lw * a.lh);
     RC_CHECK(issue(c, "upsample", [=](hipStream_t s) { return pt_launch_upsample(a, s); }));
@@ -1101,6 +1129,7 @@ static int upsample(pt_ctx *c, int fine_level)
 static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
 {
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
+    res_invalidate(c);
     PtArgs a = make_args(c, level);
     int pr = prof_begin(c, PK_ADVECT, (uint64_t)a.lw * a.lh);
     RC_CHECK(issue(c, "advect", [=](hipStream_t s) { return pt_launch_advect(a, forwards, time_step, flags, s); }));
@@ -1129,6 +1158,7 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
 static int merge_randomly(pt_ctx *c, int level, int forwards, float w, int force)
 {
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
+    res_invalidate(c);
     PtArgs a = make_args(c, level);
     RC_CHECK(issue(c, "merge", [=](hipStream_t s) { return pt_launch_merge_randomly(a, forwards, w, force, s); }));
     c->off_diff.grow(a.lw, a.lh);
@@ -1137,6 +1167,7 @@ static int merge_randomly(pt_ctx *c, int level, int forwards, float w, int force
 
 static int merge_residual(pt_ctx *c, int level, int forwards)
 {
+    res_invalidate(c);
     PtArgs a = make_args(c, level);
     RC_CHECK(issue(c, "merge", [=](hipStream_t s) { return pt_launch_merge_residual(a, forwards, s); }));
     c->off_diff.grow(a.lw, a.lh);
@@ -1263,6 +1294,7 @@ extern "C" int pt_load_offsets(pt_ctx *c, int level, const pt_patchxf *in)
     if (!c || !c->base.xyth || !in || level < 0 || (c->W >> level) < 1 || (c->H >> level) < 1)
         return set_err(c, PT_ERR_INVALID_ARG, "bad load_offsets");
     HIP_CHECK(c, hipSetDevice(c->device));
+    res_invalidate(c);
     int lw = c->W >> level, lh = c->H >> level;
     for (size_t i = 0; i < (size_t)lw * lh; i++)
         if (in[i].layer != 0 || in[i].lum_y != 0.f || in[i].lum_a != 0.f)
@@ -1283,6 +1315,7 @@ extern "C" int pt_load_residual(pt_ctx *c, int level, const float *in)
     if (!c || !c->base.xyth || !in || level < 0 || (c->W >> level) < 1 || (c->H >> level) < 1)
         return set_err(c, PT_ERR_INVALID_ARG, "bad load_residual");
     HIP_CHECK(c, hipSetDevice(c->device));
+    res_invalidate(c);
     int lw = c->W >> level, lh = c->H >> level;
     HIP_CHECK(c, hipMemcpy2DAsync(c->res_base, (size_t)c->W * sizeof(float), in, (size_t)lw * sizeof(float),
                                   (size_t)lw * sizeof(float), lh, hipMemcpyHostToDevice, c->stream));
@@ -1358,6 +1391,7 @@ static int snap_save(pt_ctx *c, int level, Snapshot &sn)
 /* WorkingBuffers::loadOffsets (workingBuffers.cpp:190-207) from a snapshot */
 static int snap_load_offsets(pt_ctx *c, const Snapshot &sn)
 {
+    res_invalidate(c);
     Snapshot d = sn;
     PtNnf base = c->base;
     size_t W = c->W;
@@ -1375,6 +1409,7 @@ static int snap_load_offsets(pt_ctx *c, const Snapshot &sn)
 /* WorkingBuffers::loadResidual (workingBuffers.cpp:166-188) from a snapshot */
 static int snap_load_residual(pt_ctx *c, const Snapshot &sn)
 {
+    res_invalidate(c);
     Snapshot d = sn;
     float *rb = c->res_base;
     size_t W = c->W;
@@ -1655,6 +1690,7 @@ static int reset_working(pt_ctx *c)
     c->res_diff = Rect();
     c->cur = 0;
     c->scatter_calls = 0;
+    res_invalidate(c);
     return PT_OK;
 }
 

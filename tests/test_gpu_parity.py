@@ -190,3 +190,24 @@ def test_repeat_calls_identical():
         b = g.synthesize(p, [fr], st, iterations=2)
     for x, y in zip(a, b):
         assert x.tobytes() == y.tobytes()
+
+
+@pytest.mark.parametrize("anim", [False, True])
+def test_incremental_residual_matches_full_refresh(anim):
+    """Skipping pixels whose patch window saw no offset change reproduces the
+    full refresh bit for bit (still and animation with temporal terms)."""
+    if anim:
+        st = W.style_pair(96, 80, seed=91, guides=True)
+        frames = W.animation(128, 96, 3, seed=92)
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5)
+    else:
+        st = W.style_pair(192, 160, seed=93, guides=True)
+        frames = [W.still_frame(256, 224, seed=94, guides=True)]
+        kw = dict(residual_window_size=3, levels=4, dist_trans_weight=0.5)
+    outs = []
+    for inc in (1, 0):
+        with Context.gpu(0) as g:
+            g.set_tuning(_ffi.PT_TUNE_INCREMENTAL_RESIDUAL, inc)
+            outs.append(g.synthesize(g.default_params(**kw), frames, st, iterations=3, fixup_passes=1))
+    for a, b in zip(*outs):
+        assert a.tobytes() == b.tobytes()
