@@ -606,6 +606,25 @@ __device__ __forceinline__ void propagate_offset(const PtArgs &A, int tx, int ty
     c = make_float4(ex, ey, ang, 0.f);
 }
 
+/* calculatePropagateOffset with the pixel's image orientation `io` and
+ * scale `csc_px` already fetched; the same operations. */
+__device__ __forceinline__ float ex_angle_io(const PtArgs &A, float io, float ex, float ey);
+__device__ __forceinline__ void propagate_offset_io(const PtArgs &A, int tx, int ty, float x, float y, float io,
+                                                    float2 csc_px, float4 &c, float2 &csc)
+{
+    int wx = (int)((float)tx - (float)A.W * floorf((float)tx / (float)A.W));
+    int wy = (int)((float)ty - (float)A.H * floorf((float)ty / (float)A.H));
+    float4 t = A.base.xyth[wy * A.W + wx];
+    float2 sc = scale_at(A, (float)tx, (float)ty);
+    float b0x, b0y, b1x, b1y;
+    basis(t.z, A.su * sc.x, A.sv * sc.y, b0x, b0y, b1x, b1y);
+    float ex, ey;
+    new_coord(x - (float)tx, y - (float)ty, t.x, t.y, b0x, b0y, b1x, b1y, ex, ey);
+    clamp_ex(ex, ey, A.ew, A.eh);
+    c = make_float4(ex, ey, ex_angle_io(A, io, ex, ey), 0.f);
+    csc = csc_px;
+}
+
 /* propagate_kernel (texSynth_kernel.cu:1112-1219) */
 template <int RT, bool TK>
 __device__ __forceinline__ void b_propagate(const PtArgs &A, int bx, int by, int jump)
@@ -1180,16 +1199,15 @@ __device__ __forceinline__ void stage_tile_up(const PtArgs &A, int x0, int y0, f
     }
 }
 
-/* Evaluates one unit for pixel (x, y) at tile position (lx, ly): the exact
- * patch distance, or +inf when a partial sum proves it cannot beat `best`
- * (EE).  Runs the row state machine inline so lanes stay independent. */
+/* Evaluates one unit (candidate c with its basis) for pixel (x, y) at tile
+ * position (lx, ly): the exact patch distance, or +inf when a partial sum
+ * proves it cannot beat `best` (EE). */
 template <int RT, bool EE, int TX>
 __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
                                          Cand c, float best, float sw)
 {
     const int r = RT > 0 ? RT : A.r;
     if (!(best > 0.f)) return PT_INF;
-    basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
     float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float tb4[8];
     float acc = 0.f, swa = 0.f;
@@ -1226,9 +1244,9 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
  * Row-granular state machine over a queue of units (EE variant): a lane
  * pulls the next unit as soon as its current one is decided, so a wave's
  * time follows the mean cost of its units rather than the slowest one.
- * SETUP(u, c, x, y, p) prepares unit u (returns false when the unit needs no
- * evaluation; it has then stored its value itself); DONE(u, value) stores a
- * decided unit's value (+inf when rejected by the bound).
+ * SETUP(u, c, x, y, p) prepares unit u, basis included (returns false when
+ * the unit needs no evaluation; it has then stored its value itself);
+ * DONE(u, value) stores a decided unit's value (+inf when rejected).
  */
 template <int RT, int TX, class Setup, class Done>
 __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_buffer_rsrc_t er, int first_u,
@@ -1257,7 +1275,6 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
             T0 = ee_threshold(best, s_sw[p]);
             T.lx = p % TX;
             T.ly = p / TX;
-            basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
             ph = 0;
             step = 0;
             acc = 0.f;
@@ -1292,14 +1309,25 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
 }
 
 /* propagate_kernel (texSynth_kernel.cu:1112-1219), unit-parallel. */
+/*
+ * Two phases: every lane first sets up candidate units in lockstep (the
+ * neighbour's offset carried over, its angle, the acceptance test, the
+ * patch basis), storing them in LDS and listing the testable ones; then the
+ * lanes pull listed units through the row state machine, whose per-unit
+ * setup is a few LDS reads, so a wave does not pay the full setup whenever
+ * one of its lanes starts a new unit.
+ */
 template <int RT, bool EE, int TX, int TY>
 __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, int jump)
 {
     constexpr int P = TX * TY, U = 4 * P;
     __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
-    __shared__ float s_best[P], s_sw[P], s_ia[P];
+    __shared__ float s_best[P], s_sw[P], s_ia[P], s_io[P];
+    __shared__ float2 s_sc[P];
     __shared__ float s_r[U];
-    __shared__ int s_next;
+    __shared__ float s_cx[U], s_cy[U], s_b0x[U], s_b0y[U], s_b1x[U], s_b1y[U], s_scy[U];
+    __shared__ unsigned short s_list[U];
+    __shared__ int s_n, s_next;
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
@@ -1310,48 +1338,73 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         s_best[p] = live ? A.res_base[o] : -1.f; /* -1: nothing to search */
         s_sw[p] = live ? tap_weight_sum(A, r, x, y) : 0.f;
         s_ia[p] = live ? A.color[A.cur][o].w : 0.f;
+        s_io[p] = live ? image_orient_at(A, x, y) : 0.f;
+        s_sc[p] = live ? scale_at(A, (float)x, (float)y) : make_float2(1.f, 1.f);
     }
-    if (threadIdx.x == 0) s_next = PT_UP_THREADS;
+    if (threadIdx.x == 0) {
+        s_n = 0;
+        s_next = PT_UP_THREADS;
+    }
     __syncthreads();
-    __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
-    Tile T;
-    T.col = scol;
-    T.can = scan;
-    T.tw = TX + 2 * r;
-    auto setup = [&](int u, Cand &c, int &x, int &y, int &p) -> bool {
-        p = u >> 2;
-        int k = u & 3;
-        x = x0 + p % TX;
-        y = y0 + p / TX;
-        if (!(s_best[p] > 0.f)) {
-            s_r[u] = PT_INF;
-            return false;
-        }
+    /* phase 1: candidate set-up in lockstep */
+    for (int u = threadIdx.x; u < U; u += PT_UP_THREADS) {
+        int p = u >> 2, k = u & 3;
+        int x = x0 + p % TX, y = y0 + p / TX;
+        s_r[u] = PT_INF;
+        if (!(s_best[p] > 0.f)) continue;
         int tx = x, ty = y;
         if (k == 0) tx = x + jump;
         else if (k == 1) tx = x - jump;
         else if (k == 2) ty = y + jump;
         else ty = y - jump;
-        propagate_offset(A, tx, ty, (float)x, (float)y, c.t, c.sc);
-        if (!candidate_testable(A, c.t, s_ia[p])) {
-            s_r[u] = PT_INF;
-            return false;
-        }
+        float4 c;
+        float2 csc;
+        propagate_offset_io(A, tx, ty, (float)x, (float)y, s_io[p], s_sc[p], c, csc);
+        if (!candidate_testable(A, c, s_ia[p])) continue;
+        float b0x, b0y, b1x, b1y;
+        basis(c.z, A.su * csc.x, A.sv * csc.y, b0x, b0y, b1x, b1y);
+        s_cx[u] = c.x;
+        s_cy[u] = c.y;
+        s_b0x[u] = b0x;
+        s_b0y[u] = b0y;
+        s_b1x[u] = b1x;
+        s_b1y[u] = b1y;
+        s_scy[u] = csc.y;
+        s_list[atomicAdd(&s_n, 1)] = (unsigned short)u;
+    }
+    __syncthreads();
+    /* phase 2: evaluation of the listed units */
+    const int n = s_n;
+    __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
+    Tile T;
+    T.col = scol;
+    T.can = scan;
+    T.tw = TX + 2 * r;
+    auto setup = [&](int idx, Cand &c, int &x, int &y, int &p) -> bool {
+        int u = s_list[idx];
+        p = u >> 2;
+        x = x0 + p % TX;
+        y = y0 + p / TX;
+        c.t = make_float4(s_cx[u], s_cy[u], 0.f, 0.f);
+        c.sc = make_float2(1.f, s_scy[u]);
+        c.b0x = s_b0x[u];
+        c.b0y = s_b0y[u];
+        c.b1x = s_b1x[u];
+        c.b1y = s_b1y[u];
         return true;
     };
+    auto done = [&](int idx, float v) { s_r[s_list[idx]] = v; };
     if (EE) {
-        up_queue_ee<RT, TX>(A, T, er, threadIdx.x, U, &s_next, s_best, s_sw, setup,
-                            [&](int u, float v) { s_r[u] = v; });
+        up_queue_ee<RT, TX>(A, T, er, threadIdx.x, n, &s_next, s_best, s_sw, setup, done);
     } else {
-        for (int u = threadIdx.x; u < U;) {
+        for (int idx = threadIdx.x; idx < n;) {
             Cand c;
             int x, y, p;
-            if (setup(u, c, x, y, p)) {
-                T.lx = p % TX;
-                T.ly = p / TX;
-                s_r[u] = up_eval<RT, false, TX>(A, T, er, x, y, c, s_best[p], s_sw[p]);
-            }
-            u = atomicAdd(&s_next, 1);
+            setup(idx, c, x, y, p);
+            T.lx = p % TX;
+            T.ly = p / TX;
+            done(idx, up_eval<RT, false, TX>(A, T, er, x, y, c, s_best[p], s_sw[p]));
+            idx = atomicAdd(&s_next, 1);
         }
     }
     __syncthreads();
@@ -1384,7 +1437,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
             else ty = y - jump;
             float4 c;
             float2 csc;
-            propagate_offset(A, tx, ty, (float)x, (float)y, c, csc);
+            propagate_offset_io(A, tx, ty, (float)x, (float)y, s_io[p], s_sc[p], c, csc);
             A.work.xyth[o] = c;
             A.work.sc[o] = csc;
         }
@@ -1469,6 +1522,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
                 s_r[p * S + k] = PT_INF;
                 return false;
             }
+            basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
             return true;
         };
         auto done = [&](int u, float v) { s_r[(u / SU) * S + (u % SU)] = v; };
@@ -2020,8 +2074,12 @@ int pt_shape_valid(int op, int shape, const PtArgs &a)
 int pt_default_shape(int op, const PtArgs &a)
 {
     if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return PT_SHAPE_PIXEL;
-    if (a.unit_parallel && pt_shape_valid(op, PT_SHAPE_UP_A, a))
-        return a.early_exit ? PT_SHAPE_UP_EE_A : PT_SHAPE_UP_A;
+    if (a.unit_parallel && pt_shape_valid(op, PT_SHAPE_UP_A, a)) {
+        /* propagate keeps a candidate table per unit in LDS: the 8x8 tile
+         * keeps occupancy up */
+        if (a.early_exit) return op == PT_OP_PROPAGATE ? PT_SHAPE_UP_EE_B : PT_SHAPE_UP_EE_A;
+        return PT_SHAPE_UP_A;
+    }
     return a.early_exit ? PT_SHAPE_PIXEL_EE : PT_SHAPE_PIXEL;
 }
 
@@ -2069,17 +2127,17 @@ hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
     bool tk = a.time_active != 0;
 #define L_PIX(R) hipLaunchKernelGGL((tk ? k_propagate<R, true> : k_propagate<R, false>), g, kBlock, 0, s, a, jump)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? k_propagate_ee<R, true> : k_propagate_ee<R, false>), g, kBlock, 0, s, a, jump)
-#define L_UEA(R) hipLaunchKernelGGL((k_propagate_up<R, true, 16, 16>), g, bu, 0, s, a, jump)
+#define L_UEB(R) hipLaunchKernelGGL((k_propagate_up<R, true, 8, 8>), g, bu, 0, s, a, jump)
 #define L_UA(R) hipLaunchKernelGGL((k_propagate_up<R, false, 8, 8>), g, bu, 0, s, a, jump)
     switch (shape) {
     case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
     case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
-    case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
+    case PT_SHAPE_UP_EE_B: PT_RT_SWITCH(a.r, L_UEB); break;
     default: PT_RT_SWITCH(a.r, L_UA); break;
     }
 #undef L_PIX
 #undef L_PEE
-#undef L_UEA
+#undef L_UEB
 #undef L_UA
     return hipGetLastError();
 }
