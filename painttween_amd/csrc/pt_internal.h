@@ -87,8 +87,8 @@ enum PtOp { PT_OP_PROPAGATE = 0, PT_OP_SCATTER = 1, PT_OP_RESIDUAL = 2, PT_OP_SY
  * lanes.  Propagate/scatter: PIXEL = one lane per pixel, candidates in
  * sequence; PIXEL_EE = the same with exact early rejection; UP_* =
  * candidate units spread over a 256-lane workgroup owning a pixel tile
- * (EE_A: propagate 16x16 / scatter 16x8 with early rejection; A: 8x8 / 4x4;
- * EE_B: 8x8 / 8x8 with early rejection; B: 16x16 / 8x8).  Residual and
+ * (EE_A: propagate 16x8 / scatter 16x8 with early rejection; A: 8x8 / 4x4;
+ * EE_B: 8x8 / 8x8 with early rejection; B: 16x8 / 8x8).  Residual and
  * sync have the PIXEL shape only. */
 enum PtShape {
     PT_SHAPE_PIXEL = 0,

@@ -857,6 +857,68 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
     }
 }
 
+/* row_accum split in two for a fixed radius: the tap residuals of row i
+ * (no temporal terms) ... */
+template <int RT>
+__device__ __forceinline__ void row_taps(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y, int i,
+                                         const Cand &c, float (&rv)[2 * RT + 1])
+{
+    constexpr int r = RT;
+    const int ew = A.ew, eh = A.eh;
+    const float hw = (float)A.hw, hh = (float)A.hh;
+    const float ibx = (float)i * c.b0x, iby = (float)i * c.b0y;
+    const float4 *tcol = T.col + (T.ly + r) * T.tw + (T.lx + r + i);
+    const float4 *tcan = T.can + (T.ly + r) * T.tw + (T.lx + r + i);
+#pragma unroll
+    for (int j = -r; j <= r; j++) {
+        rv[j + r] = 0.f;
+        if (y + j < 0 || y + j >= A.lh)
+            continue;
+        float4 base = tcol[j * T.tw];
+        float4 canvas = tcan[j * T.tw];
+        float tx = c.t.x + (ibx + (float)j * c.b1x);
+        float ty = c.t.y + (iby + (float)j * c.b1y);
+        if (tx >= hw) tx -= hw;
+        else if (tx < 0) tx += hw;
+        if (ty >= hh) ty -= hh;
+        else if (ty < 0) ty += hh;
+        float4 tb, to;
+        ex_pair(er, ew, eh, tx, ty, tb, to);
+        float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
+        if (i == 0 && j == 0) {
+            if (A.dt_w > 0.f) {
+                float de = tex1_lin(A.ex_dt, A.ew, A.ew, A.eh, tx, ty);
+                de /= c.sc.y;
+                float di = A.dt[A.cur][y * A.W + x];
+                float diff = fabsf(de - di);
+                diff = fminf(fmaxf(diff, 0.0f), 10.0f);
+                res = res + diff * A.dt_w;
+            }
+        } else {
+            res = fmaf(ssd_alpha(canvas, to, A.alpha_w), A.canvas_w, res);
+        }
+        rv[j + r] = res;
+    }
+}
+
+/* ... and their accumulation in the reference order, with the same skips
+ * and weights as row_accum. */
+template <int RT>
+__device__ __forceinline__ void row_sum(const PtArgs &A, int y, int i, const float (&rv)[2 * RT + 1], float &acc,
+                                        float &swa)
+{
+    constexpr int r = RT;
+    const float gi = A.gauss[i + r];
+#pragma unroll
+    for (int j = -r; j <= r; j++) {
+        if (y + j < 0 || y + j >= A.lh)
+            continue;
+        float wgt = gi * A.gauss[j + r];
+        acc = fmaf(rv[j + r], wgt, acc);
+        swa += wgt;
+    }
+}
+
 /* Temporal bases of the advected offsets at (x, y) (texSynth_kernel.cu:552-557). */
 template <bool TIME>
 __device__ __forceinline__ void time_setup(const PtArgs &A, int x, int y, float4 &pf, float4 &pb, bool &cpv,
@@ -1248,7 +1310,7 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
  * the unit needs no evaluation; it has then stored its value itself);
  * DONE(u, value) stores a decided unit's value (+inf when rejected).
  */
-template <int RT, int TX, class Setup, class Done>
+template <int RT, int TX, bool KEEP, class Setup, class Done>
 __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_buffer_rsrc_t er, int first_u,
                                             int U, int *s_next, const float *s_best, const float *s_sw, Setup setup,
                                             Done done)
@@ -1260,6 +1322,10 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
     int u = first_u, x = 0, y = 0, p = 0, ph = 0, step = 0;
     float acc = 0.f, swa = 0.f, T0 = 0.f;
     bool need = true;
+    /* KEEP: the tap residuals of rows 0, -1, +1 from the first phase, so the
+     * in-order phase replays them instead of gathering them again */
+    constexpr int NK = KEEP ? 2 * RT + 1 : 1;
+    float k0[NK], km[NK], kp[NK];
     while (true) {
         if (need) {
             bool found = false;
@@ -1282,8 +1348,31 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
             need = false;
         }
         int i = ph == 0 ? (step == 0 ? 0 : (step == 1 ? -1 : 1)) : step - r;
-        if (x + i >= 0 && x + i < A.lw)
-            row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+        if constexpr (KEEP) {
+            if (x + i >= 0 && x + i < A.lw) {
+                if (ph == 0) {
+                    float rv[NK];
+                    row_taps<RT>(A, T, er, x, y, i, c, rv);
+                    row_sum<RT>(A, y, i, rv, acc, swa);
+#pragma unroll
+                    for (int j = 0; j < NK; j++) {
+                        k0[j] = step == 0 ? rv[j] : k0[j];
+                        km[j] = step == 1 ? rv[j] : km[j];
+                        kp[j] = step == 2 ? rv[j] : kp[j];
+                    }
+                } else if (i >= -1 && i <= 1) {
+                    float rv[NK];
+#pragma unroll
+                    for (int j = 0; j < NK; j++) rv[j] = i == 0 ? k0[j] : (i < 0 ? km[j] : kp[j]);
+                    row_sum<RT>(A, y, i, rv, acc, swa);
+                } else {
+                    row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+                }
+            }
+        } else {
+            if (x + i >= 0 && x + i < A.lw)
+                row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+        }
         float val = 0.f;
         bool decided = false;
         if (acc >= T0) {
@@ -1395,7 +1484,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     };
     auto done = [&](int idx, float v) { s_r[s_list[idx]] = v; };
     if (EE) {
-        up_queue_ee<RT, TX>(A, T, er, threadIdx.x, n, &s_next, s_best, s_sw, setup, done);
+        up_queue_ee<RT, TX, (RT > 0)>(A, T, er, threadIdx.x, n, &s_next, s_best, s_sw, setup, done);
     } else {
         for (int idx = threadIdx.x; idx < n;) {
             Cand c;
@@ -1527,7 +1616,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
         };
         auto done = [&](int u, float v) { s_r[(u / SU) * S + (u % SU)] = v; };
         if (EE) {
-            up_queue_ee<RT, TX>(A, T, er, threadIdx.x, P * SU, &s_next, s_best, s_sw, setup, done);
+            up_queue_ee<RT, TX, false>(A, T, er, threadIdx.x, P * SU, &s_next, s_best, s_sw, setup, done);
         } else {
             for (int u = threadIdx.x; u < P * SU;) {
                 Cand c;
@@ -2043,7 +2132,7 @@ static void shape_tile(int op, int shape, int &tx, int &ty)
     tx = PT_BX;
     ty = PT_BY;
     if (op == PT_OP_PROPAGATE) {
-        if (shape == PT_SHAPE_UP_EE_A || shape == PT_SHAPE_UP_B) tx = ty = 16;
+        if (shape == PT_SHAPE_UP_EE_A || shape == PT_SHAPE_UP_B) { tx = 16; ty = 8; }
         else if (shape == PT_SHAPE_UP_A || shape == PT_SHAPE_UP_EE_B) tx = ty = 8;
     } else if (op == PT_OP_SCATTER) {
         if (shape == PT_SHAPE_UP_EE_A) { tx = 16; ty = 8; }
@@ -2186,10 +2275,10 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
     case PT_OP_PROPAGATE: {
 #define L_PIX(R) hipLaunchKernelGGL((tk ? km_propagate<R, true> : km_propagate<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? km_propagate_ee<R, true> : km_propagate_ee<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UEA(R) hipLaunchKernelGGL((km_propagate_up<R, true, 16, 16>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEA(R) hipLaunchKernelGGL((km_propagate_up<R, true, 16, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_UA(R) hipLaunchKernelGGL((km_propagate_up<R, false, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_UEB(R) hipLaunchKernelGGL((km_propagate_up<R, true, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UB(R) hipLaunchKernelGGL((km_propagate_up<R, false, 16, 16>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UB(R) hipLaunchKernelGGL((km_propagate_up<R, false, 16, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
         switch (shape) {
         case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
         case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
