@@ -279,7 +279,7 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * results are bit-identical either way.  0 forces the plain kernels. */
 #define PT_TUNE_EARLY_EXIT 1
 /* Early rejection is used on levels with at least this many pixels
- * (default 512*512); smaller levels have too few waves to hide the longer
+ * (default 256*256); smaller levels have too few waves to hide the longer
  * per-lane dependency chain. */
 #define PT_TUNE_EARLY_EXIT_MIN_PIXELS 2
 /* PT_TUNE_UNIT_PARALLEL (default 1): propagate and scatter spread every

@@ -780,6 +780,16 @@ struct Cand {
     float b0x, b0y, b1x, b1y;
 };
 
+/* Temporal state of the pixel a unit belongs to: the advected offsets and
+ * their bases (time_setup) and the sum's start value, minus the hysteresis
+ * term when the forward advected offset is valid (calc_residual_impl). */
+struct TimeCand {
+    float4 pf, pb;
+    bool cpv, cnv;
+    float tb4[8];
+    float acc0;
+};
+
 __device__ __forceinline__ float ee_threshold(float best, float sw) { return best * sw * 1.0001f; }
 
 /* Sum of the in-bounds tap weights of pixel (x, y), in the reference order. */
@@ -1264,15 +1274,13 @@ __device__ __forceinline__ void stage_tile_up(const PtArgs &A, int x0, int y0, f
 /* Evaluates one unit (candidate c with its basis) for pixel (x, y) at tile
  * position (lx, ly): the exact patch distance, or +inf when a partial sum
  * proves it cannot beat `best` (EE). */
-template <int RT, bool EE, int TX>
+template <int RT, bool EE, int TX, bool TM>
 __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
-                                         Cand c, float best, float sw)
+                                         Cand c, const TimeCand &tc, float best, float sw)
 {
     const int r = RT > 0 ? RT : A.r;
-    if (!(best > 0.f)) return PT_INF;
-    float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float tb4[8];
-    float acc = 0.f, swa = 0.f;
+    const float a0 = TM ? tc.acc0 : 0.f;
+    float acc = a0, swa = 0.f;
     if (EE) {
         const float T0 = ee_threshold(best, sw);
         /* phase 0: the three heaviest rows */
@@ -1281,22 +1289,22 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
             int i = step == 0 ? 0 : (step == 1 ? -1 : 1);
             if (i > r) break;
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+                row_accum<RT, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
             if (acc >= T0) return PT_INF;
         }
-        acc = 0.f;
+        acc = a0;
         swa = 0.f;
 #pragma unroll 1
         for (int i = -r; i <= r; i++) {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+                row_accum<RT, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
             if (acc >= T0) return PT_INF;
         }
     } else {
 #pragma unroll 1
         for (int i = -r; i <= r; i++) {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+                row_accum<RT, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
         }
     }
     return (swa < 0.000001f) ? 1e31f : acc / swa;
@@ -1310,15 +1318,17 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
  * the unit needs no evaluation; it has then stored its value itself);
  * DONE(u, value) stores a decided unit's value (+inf when rejected).
  */
-template <int RT, int TX, bool KEEP, class Setup, class Done>
+template <int RT, int TX, bool KEEP, bool TM, class Setup, class Done>
 __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_buffer_rsrc_t er, int first_u,
                                             int U, int *s_next, const float *s_best, const float *s_sw, Setup setup,
                                             Done done)
 {
+    static_assert(!(KEEP && TM), "the replayed rows carry no temporal terms");
     const int r = RT > 0 ? RT : A.r;
     float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float tb4[8];
     Cand c;
+    TimeCand tc;
     int u = first_u, x = 0, y = 0, p = 0, ph = 0, step = 0;
     float acc = 0.f, swa = 0.f, T0 = 0.f;
     bool need = true;
@@ -1330,7 +1340,7 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
         if (need) {
             bool found = false;
             while (u < U) {
-                if (setup(u, c, x, y, p)) {
+                if (setup(u, c, tc, x, y, p)) {
                     found = true;
                     break;
                 }
@@ -1343,7 +1353,7 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
             T.ly = p / TX;
             ph = 0;
             step = 0;
-            acc = 0.f;
+            acc = TM ? tc.acc0 : 0.f;
             swa = 0.f;
             need = false;
         }
@@ -1369,6 +1379,9 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
                     row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
                 }
             }
+        } else if constexpr (TM) {
+            if (x + i >= 0 && x + i < A.lw)
+                row_accum<RT, true>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
         } else {
             if (x + i >= 0 && x + i < A.lw)
                 row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
@@ -1381,7 +1394,7 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
         } else if (ph == 0 && step == 2) {
             ph = 1;
             step = 0;
-            acc = 0.f;
+            acc = TM ? tc.acc0 : 0.f;
             swa = 0.f;
         } else if (ph == 1 && step == 2 * r) {
             val = (swa < 0.000001f) ? 1e31f : acc / swa;
@@ -1398,6 +1411,53 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
 }
 
 /* propagate_kernel (texSynth_kernel.cu:1112-1219), unit-parallel. */
+/* Whether a unit kernel must evaluate candidates of a pixel: it is inside
+ * the level and not culled (its tap weight sum is positive), and a
+ * candidate could score below its best.  Without a positive hysteresis
+ * weight every residual is >= 0, so a best <= 0 (or NaN) cannot be beaten. */
+__device__ __forceinline__ bool up_open(const PtArgs &A, float sw, float best)
+{
+    return sw > 0.f && (best > 0.f || A.hyst_w > 0.f);
+}
+
+/* Per-pixel temporal state kept in LDS by the unit kernels. */
+template <int P> struct UpTime {
+    float2 pf[P], pb[P];
+    float tb4[P][8];
+    float acc0[P];
+    int flags[P]; /* bit 0: forward advected offset valid, bit 1: backward */
+};
+
+template <int P>
+__device__ __forceinline__ void up_time_init(const PtArgs &A, UpTime<P> &S, int p, int x, int y, bool live)
+{
+    float4 pf, pb;
+    bool cpv = false, cnv = false;
+    float tb4[8];
+    if (live) time_setup<true>(A, x, y, pf, pb, cpv, cnv, tb4);
+    else {
+        pf = pb = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = 0; k < 8; k++) tb4[k] = 0.f;
+    }
+    S.pf[p] = make_float2(pf.x, pf.y);
+    S.pb[p] = make_float2(pb.x, pb.y);
+    for (int k = 0; k < 8; k++) S.tb4[p][k] = tb4[k];
+    float a0 = 0.f;
+    if (cpv) a0 -= hysteresis_term(A, x, y);
+    S.acc0[p] = a0;
+    S.flags[p] = (cpv ? 1 : 0) | (cnv ? 2 : 0);
+}
+
+template <int P> __device__ __forceinline__ void up_time_load(const UpTime<P> &S, int p, TimeCand &tc)
+{
+    tc.pf = make_float4(S.pf[p].x, S.pf[p].y, 0.f, 0.f);
+    tc.pb = make_float4(S.pb[p].x, S.pb[p].y, 0.f, 0.f);
+    for (int k = 0; k < 8; k++) tc.tb4[k] = S.tb4[p][k];
+    tc.acc0 = S.acc0[p];
+    tc.cpv = (S.flags[p] & 1) != 0;
+    tc.cnv = (S.flags[p] & 2) != 0;
+}
+
 /*
  * Two phases: every lane first sets up candidate units in lockstep (the
  * neighbour's offset carried over, its angle, the acceptance test, the
@@ -1406,10 +1466,11 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
  * setup is a few LDS reads, so a wave does not pay the full setup whenever
  * one of its lanes starts a new unit.
  */
-template <int RT, bool EE, int TX, int TY>
+template <int RT, bool EE, int TX, int TY, bool TM>
 __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, int jump)
 {
     constexpr int P = TX * TY, U = 4 * P;
+    __shared__ UpTime<TM ? P : 1> s_tm;
     __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
     __shared__ float s_best[P], s_sw[P], s_ia[P], s_io[P];
     __shared__ float2 s_sc[P];
@@ -1424,11 +1485,12 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         int x = x0 + p % TX, y = y0 + p / TX;
         bool live = x < A.lw && y < A.lh && !image_culled(A, x, y);
         int o = y * A.W + x;
-        s_best[p] = live ? A.res_base[o] : -1.f; /* -1: nothing to search */
+        s_best[p] = live ? A.res_base[o] : -1.f;
         s_sw[p] = live ? tap_weight_sum(A, r, x, y) : 0.f;
         s_ia[p] = live ? A.color[A.cur][o].w : 0.f;
         s_io[p] = live ? image_orient_at(A, x, y) : 0.f;
         s_sc[p] = live ? scale_at(A, (float)x, (float)y) : make_float2(1.f, 1.f);
+        if constexpr (TM) up_time_init<P>(A, s_tm, p, x, y, live);
     }
     if (threadIdx.x == 0) {
         s_n = 0;
@@ -1440,7 +1502,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         int p = u >> 2, k = u & 3;
         int x = x0 + p % TX, y = y0 + p / TX;
         s_r[u] = PT_INF;
-        if (!(s_best[p] > 0.f)) continue;
+        if (!up_open(A, s_sw[p], s_best[p])) continue;
         int tx = x, ty = y;
         if (k == 0) tx = x + jump;
         else if (k == 1) tx = x - jump;
@@ -1469,7 +1531,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     T.col = scol;
     T.can = scan;
     T.tw = TX + 2 * r;
-    auto setup = [&](int idx, Cand &c, int &x, int &y, int &p) -> bool {
+    auto setup = [&](int idx, Cand &c, TimeCand &tc, int &x, int &y, int &p) -> bool {
         int u = s_list[idx];
         p = u >> 2;
         x = x0 + p % TX;
@@ -1480,19 +1542,21 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         c.b0y = s_b0y[u];
         c.b1x = s_b1x[u];
         c.b1y = s_b1y[u];
+        if constexpr (TM) up_time_load<P>(s_tm, p, tc);
         return true;
     };
     auto done = [&](int idx, float v) { s_r[s_list[idx]] = v; };
     if (EE) {
-        up_queue_ee<RT, TX, (RT > 0)>(A, T, er, threadIdx.x, n, &s_next, s_best, s_sw, setup, done);
+        up_queue_ee<RT, TX, (RT > 0) && !TM, TM>(A, T, er, threadIdx.x, n, &s_next, s_best, s_sw, setup, done);
     } else {
         for (int idx = threadIdx.x; idx < n;) {
             Cand c;
+            TimeCand tc;
             int x, y, p;
-            setup(idx, c, x, y, p);
+            setup(idx, c, tc, x, y, p);
             T.lx = p % TX;
             T.ly = p / TX;
-            done(idx, up_eval<RT, false, TX>(A, T, er, x, y, c, s_best[p], s_sw[p]));
+            done(idx, up_eval<RT, false, TX, TM>(A, T, er, x, y, c, tc, s_best[p], s_sw[p]));
             idx = atomicAdd(&s_next, 1);
         }
     }
@@ -1533,20 +1597,21 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     }
 }
 
-template <int RT, bool EE, int TX, int TY>
+template <int RT, bool EE, int TX, int TY, bool TM>
 __global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int jump)
 {
-    b_propagate_up<RT, EE, TX, TY>(A, blockIdx.x, blockIdx.y, jump);
+    b_propagate_up<RT, EE, TX, TY, TM>(A, blockIdx.x, blockIdx.y, jump);
 }
 
 /* scatter_kernel (texSynth_kernel.cu:865-952), unit-parallel in rounds.
  * n = number of random samples (the reference's loop bound, identical for
  * every pixel); unit k == n is the averaged-neighbour candidate. */
-template <int RT, bool EE, int TX, int TY>
+template <int RT, bool EE, int TX, int TY, bool TM>
 __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, int n)
 {
     constexpr int P = TX * TY;
     constexpr int S = PT_UP_MAX_SAMPLES + 1;
+    __shared__ UpTime<TM ? P : 1> s_tm;
     __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
     __shared__ float4 s_ow[P];
     __shared__ float2 s_osc[P];
@@ -1581,6 +1646,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
         s_iori[p] = live ? image_orient_at(A, x, y) : 0.f;
         s_sc[p] = live ? scale_at(A, (float)x, (float)y) : make_float2(1.f, 1.f);
         s_start[p] = live ? 0 : SU;
+        if constexpr (TM) up_time_init<P>(A, s_tm, p, x, y, live);
     }
     __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
     Tile T;
@@ -1592,7 +1658,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     while (true) {
         if (threadIdx.x == 0) s_next = PT_UP_THREADS;
         __syncthreads();
-        auto setup = [&](int u, Cand &c, int &x, int &y, int &p) -> bool {
+        auto setup = [&](int u, Cand &c, TimeCand &tc, int &x, int &y, int &p) -> bool {
             p = u / SU;
             int k = u % SU;
             int st = s_start[p];
@@ -1607,24 +1673,26 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
                 c.t = average_candidate(A, x, y, s_iori[p]);
                 c.sc = one;
             }
-            if (!(s_best[p] > 0.f) || !candidate_testable(A, c.t, s_ia[p])) {
+            if (!up_open(A, s_sw[p], s_best[p]) || !candidate_testable(A, c.t, s_ia[p])) {
                 s_r[p * S + k] = PT_INF;
                 return false;
             }
             basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
+            if constexpr (TM) up_time_load<P>(s_tm, p, tc);
             return true;
         };
         auto done = [&](int u, float v) { s_r[(u / SU) * S + (u % SU)] = v; };
         if (EE) {
-            up_queue_ee<RT, TX, false>(A, T, er, threadIdx.x, P * SU, &s_next, s_best, s_sw, setup, done);
+            up_queue_ee<RT, TX, false, TM>(A, T, er, threadIdx.x, P * SU, &s_next, s_best, s_sw, setup, done);
         } else {
             for (int u = threadIdx.x; u < P * SU;) {
                 Cand c;
+                TimeCand tc;
                 int x, y, p;
-                if (setup(u, c, x, y, p)) {
+                if (setup(u, c, tc, x, y, p)) {
                     T.lx = p % TX;
                     T.ly = p / TX;
-                    done(u, up_eval<RT, false, TX>(A, T, er, x, y, c, s_best[p], s_sw[p]));
+                    done(u, up_eval<RT, false, TX, TM>(A, T, er, x, y, c, tc, s_best[p], s_sw[p]));
                 }
                 u = atomicAdd(&s_next, 1);
             }
@@ -1678,10 +1746,10 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     }
 }
 
-template <int RT, bool EE, int TX, int TY>
+template <int RT, bool EE, int TX, int TY, bool TM>
 __global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
 {
-    b_scatter_up<RT, EE, TX, TY>(A, blockIdx.x, blockIdx.y, n);
+    b_scatter_up<RT, EE, TX, TY, TM>(A, blockIdx.x, blockIdx.y, n);
 }
 
 /* upsampleBilinearKernel (texSynth_kernel.cu:1453-1510) with
@@ -2020,12 +2088,12 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void km_propagate_ee(PT_MULTI_PARAMS)
     b_propagate_ee<RT, TK>(A, t.bx, t.by, A.jump);
 }
 
-template <int RT, bool EE, int TX, int TY>
+template <int RT, bool EE, int TX, int TY, bool TM>
 __global__ __launch_bounds__(PT_UP_THREADS) void km_propagate_up(PT_MULTI_PARAMS)
 {
     MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
     const PtArgs &A = PT_JOB(t.j);
-    b_propagate_up<RT, EE, TX, TY>(A, t.bx, t.by, A.jump);
+    b_propagate_up<RT, EE, TX, TY, TM>(A, t.bx, t.by, A.jump);
 }
 
 template <int RT, bool TK>
@@ -2042,12 +2110,12 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void km_scatter_ee(PT_MULTI_PARAMS)
     b_scatter_ee<RT, TK>(PT_JOB(t.j), t.bx, t.by);
 }
 
-template <int RT, bool EE, int TX, int TY>
+template <int RT, bool EE, int TX, int TY, bool TM>
 __global__ __launch_bounds__(PT_UP_THREADS) void km_scatter_up(PT_MULTI_PARAMS)
 {
     MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
     const PtArgs &A = PT_JOB(t.j);
-    b_scatter_up<RT, EE, TX, TY>(A, t.bx, t.by, A.nscat);
+    b_scatter_up<RT, EE, TX, TY, TM>(A, t.bx, t.by, A.nscat);
 }
 
 /* ------------------------------------------------------------------ */
@@ -2154,7 +2222,6 @@ int pt_shape_valid(int op, int shape, const PtArgs &a)
     if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return 0;
     if (shape_is_ee(shape) && !a.ee_ok) return 0;
     if (shape_is_up(shape)) {
-        if (a.time_active) return 0; /* the unit kernels carry no temporal terms */
         if (op == PT_OP_SCATTER && a.nscat > PT_UP_MAX_SAMPLES) return 0;
     }
     return 1;
@@ -2216,8 +2283,8 @@ hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
     bool tk = a.time_active != 0;
 #define L_PIX(R) hipLaunchKernelGGL((tk ? k_propagate<R, true> : k_propagate<R, false>), g, kBlock, 0, s, a, jump)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? k_propagate_ee<R, true> : k_propagate_ee<R, false>), g, kBlock, 0, s, a, jump)
-#define L_UEB(R) hipLaunchKernelGGL((k_propagate_up<R, true, 8, 8>), g, bu, 0, s, a, jump)
-#define L_UA(R) hipLaunchKernelGGL((k_propagate_up<R, false, 8, 8>), g, bu, 0, s, a, jump)
+#define L_UEB(R) hipLaunchKernelGGL((tk ? k_propagate_up<R, true, 8, 8, true> : k_propagate_up<R, true, 8, 8, false>), g, bu, 0, s, a, jump)
+#define L_UA(R) hipLaunchKernelGGL((tk ? k_propagate_up<R, false, 8, 8, true> : k_propagate_up<R, false, 8, 8, false>), g, bu, 0, s, a, jump)
     switch (shape) {
     case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
     case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
@@ -2243,8 +2310,8 @@ hipError_t pt_launch_scatter(const PtArgs &a0, hipStream_t s)
     bool tk = a.time_active != 0;
 #define L_PIX(R) hipLaunchKernelGGL((tk ? k_scatter<R, true> : k_scatter<R, false>), g, kBlock, 0, s, a)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? k_scatter_ee<R, true> : k_scatter_ee<R, false>), g, kBlock, 0, s, a)
-#define L_UEA(R) hipLaunchKernelGGL((k_scatter_up<R, true, 16, 8>), g, bu, 0, s, a, n)
-#define L_UA(R) hipLaunchKernelGGL((k_scatter_up<R, false, 4, 4>), g, bu, 0, s, a, n)
+#define L_UEA(R) hipLaunchKernelGGL((tk ? k_scatter_up<R, true, 16, 8, true> : k_scatter_up<R, true, 16, 8, false>), g, bu, 0, s, a, n)
+#define L_UA(R) hipLaunchKernelGGL((tk ? k_scatter_up<R, false, 4, 4, true> : k_scatter_up<R, false, 4, 4, false>), g, bu, 0, s, a, n)
     switch (shape) {
     case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
     case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
@@ -2275,10 +2342,10 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
     case PT_OP_PROPAGATE: {
 #define L_PIX(R) hipLaunchKernelGGL((tk ? km_propagate<R, true> : km_propagate<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? km_propagate_ee<R, true> : km_propagate_ee<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UEA(R) hipLaunchKernelGGL((km_propagate_up<R, true, 16, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UA(R) hipLaunchKernelGGL((km_propagate_up<R, false, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UEB(R) hipLaunchKernelGGL((km_propagate_up<R, true, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UB(R) hipLaunchKernelGGL((km_propagate_up<R, false, 16, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEA(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, true, 16, 8, true> : km_propagate_up<R, true, 16, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UA(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, false, 8, 8, true> : km_propagate_up<R, false, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEB(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, true, 8, 8, true> : km_propagate_up<R, true, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UB(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, false, 16, 8, true> : km_propagate_up<R, false, 16, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
         switch (shape) {
         case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
         case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
@@ -2298,10 +2365,10 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
     case PT_OP_SCATTER: {
 #define L_PIX(R) hipLaunchKernelGGL((tk ? km_scatter<R, true> : km_scatter<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? km_scatter_ee<R, true> : km_scatter_ee<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UEA(R) hipLaunchKernelGGL((km_scatter_up<R, true, 16, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UA(R) hipLaunchKernelGGL((km_scatter_up<R, false, 4, 4>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UEB(R) hipLaunchKernelGGL((km_scatter_up<R, true, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-#define L_UB(R) hipLaunchKernelGGL((km_scatter_up<R, false, 8, 8>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEA(R) hipLaunchKernelGGL((tk ? km_scatter_up<R, true, 16, 8, true> : km_scatter_up<R, true, 16, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UA(R) hipLaunchKernelGGL((tk ? km_scatter_up<R, false, 4, 4, true> : km_scatter_up<R, false, 4, 4, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UEB(R) hipLaunchKernelGGL((tk ? km_scatter_up<R, true, 8, 8, true> : km_scatter_up<R, true, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+#define L_UB(R) hipLaunchKernelGGL((tk ? km_scatter_up<R, false, 8, 8, true> : km_scatter_up<R, false, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
         switch (shape) {
         case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
         case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;

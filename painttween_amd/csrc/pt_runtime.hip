@@ -107,7 +107,7 @@ struct pt_ctx {
     bool no_early_exit = false;            /* force the plain candidate kernels (tests) */
     /* Early rejection lengthens each lane's dependent chain; it pays only on
      * levels with enough waves to hide the latency. */
-    int64_t early_exit_min_pixels = 512 * 512;
+    int64_t early_exit_min_pixels = 256 * 256;
     bool unit_parallel = true; /* candidate units over workgroup lanes */
     /* residual refresh: res_base is exact for the current offsets, canvas
      * and inputs of level res_level (-1: unknown); `changed` marks the
