@@ -291,6 +291,17 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * sync recomputes only pixels whose patch window holds an offset that sync
  * changed; the others would reproduce their cached value bit for bit. */
 #define PT_TUNE_INCREMENTAL_RESIDUAL 4
+/* PT_TUNE_TAP_PARALLEL_MAX_PIXELS (default 128*128): on levels up to this
+ * many pixels, propagate and the residual refresh give every tap of every
+ * candidate its own lane and sum each candidate's taps in the reference
+ * order afterwards (shorter dependency chains where the GPU would idle);
+ * 0 disables it.  Same results. */
+#define PT_TUNE_TAP_PARALLEL_MAX_PIXELS 5
+/* PT_TUNE_TAP_PARALLEL_EVALS (default 1024, range 1..1024): tap evaluations
+ * per tap-parallel workgroup; a workgroup covers this many / (2r+1)^2 pixels
+ * in the refresh and / (4 (2r+1)^2) in propagate (at least one).  Same
+ * results. */
+#define PT_TUNE_TAP_PARALLEL_EVALS 6
 int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --

@@ -25,6 +25,8 @@ PT_TUNE_EARLY_EXIT = 1
 PT_TUNE_EARLY_EXIT_MIN_PIXELS = 2
 PT_TUNE_UNIT_PARALLEL = 3
 PT_TUNE_INCREMENTAL_RESIDUAL = 4
+PT_TUNE_TAP_PARALLEL_MAX_PIXELS = 5
+PT_TUNE_TAP_PARALLEL_EVALS = 6
 PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT = 0, 1
 
 LAYER = dict(
@@ -136,7 +138,7 @@ STRUCTS = {0: PtParams, 1: PatchXF, 2: FrameGuides, 3: StyleGuides, 4: PtJob, 5:
 
 PT_BATCH_USE_GRAPH, PT_BATCH_XCD_REMAP, PT_BATCH_SHAPE, PT_BATCH_ISOLATE_TILES = 1, 2, 3, 4
 PT_OP_PROPAGATE, PT_OP_SCATTER = 0, 1
-SHAPE_NAMES = ["pixel", "pixel_ee", "unit_ee_a", "unit_a", "unit_ee_b", "unit_b"]
+SHAPE_NAMES = ["pixel", "pixel_ee", "unit_ee_a", "unit_a", "unit_ee_b", "unit_b", "tap"]
 
 
 def fptr(a):

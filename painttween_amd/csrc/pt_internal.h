@@ -60,6 +60,9 @@ struct PtArgs {
     int dw, dh;        /* sync: rectangle where working and base differ */
     int res_incremental; /* residual refresh: recompute only pixels whose
                             patch window holds a changed offset */
+    int tap_max_px;    /* default policy: tap-parallel shapes on levels up to this size */
+    int tap_np_res;    /* tap-parallel refresh: pixels per workgroup (pt_tap_configure)   */
+    int tap_np_prop;   /* tap-parallel propagate: pixels per workgroup (pt_tap_configure) */
 
     uint8_t *changed;  /* per pixel: base offset changed by the last sync */
 
@@ -88,8 +91,11 @@ enum PtOp { PT_OP_PROPAGATE = 0, PT_OP_SCATTER = 1, PT_OP_RESIDUAL = 2, PT_OP_SY
  * sequence; PIXEL_EE = the same with exact early rejection; UP_* =
  * candidate units spread over a 256-lane workgroup owning a pixel tile
  * (EE_A: propagate 16x8 / scatter 16x8 with early rejection; A: 8x8 / 4x4;
- * EE_B: 8x8 / 8x8 with early rejection; B: 16x8 / 8x8).  Residual and
- * sync have the PIXEL shape only. */
+ * EE_B: 8x8 / 8x8 with early rejection; B: 16x8 / 8x8).  TAP (propagate
+ * and residual refresh, for small levels): every tap of every candidate is a
+ * lane, the workgroup then sums each candidate's taps in the reference
+ * order; it trades work per lane for latency.  Sync has the PIXEL shape
+ * only. */
 enum PtShape {
     PT_SHAPE_PIXEL = 0,
     PT_SHAPE_PIXEL_EE = 1,
@@ -97,7 +103,8 @@ enum PtShape {
     PT_SHAPE_UP_A = 3,
     PT_SHAPE_UP_EE_B = 4,
     PT_SHAPE_UP_B = 5,
-    PT_SHAPE_COUNT = 6
+    PT_SHAPE_TAP = 6, /* propagate and residual: one lane per (candidate, tap) */
+    PT_SHAPE_COUNT = 7
 };
 
 /* A multi-job launch: workgroup g serves job j with prefix[j] <= g <
@@ -111,6 +118,11 @@ struct PtJobTable {
     int xcd_remap;      /* spread contiguous tile ranges over the 8 XCDs */
 };
 
+/* Tap evaluations per tap-parallel workgroup: the LDS capacity of those
+ * kernels.  pt_tap_configure sets a.tap_np_* for a target of `evals`
+ * evaluations per workgroup (clamped to 1 .. PT_TAP_EVALS). */
+#define PT_TAP_EVALS 1024
+void pt_tap_configure(PtArgs &a, int evals);
 int pt_default_shape(int op, const PtArgs &a);
 int pt_shape_valid(int op, int shape, const PtArgs &a);
 /* Tiles of job `a` for (op, shape); *tiles_x receives tiles per row. */

@@ -2008,6 +2008,273 @@ __global__ void k_unpack_nnf(const PatchXF36 *s, int lw, int lh, PtNnf d, int W)
 }
 
 /* ------------------------------------------------------------------ */
+/* tap-parallel shapes (small levels)                                  */
+/* ------------------------------------------------------------------ */
+/*
+ * On small levels every lane-serial patch distance is a long chain of
+ * dependent gathers while most of the GPU idles.  Here each lane computes one
+ * tap's residual (the body of calculateResidual's tap loop, identical
+ * arithmetic), the values go to LDS, and one lane per candidate then sums
+ * them in the reference order (x outer, y inner, sequential fmaf), so the
+ * result is the same bit for bit.
+ */
+template <bool TIME>
+__device__ __forceinline__ float tap_residual(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int x, int y, int i, int j,
+                                              const Cand &c, const TimeCand &tc)
+{
+    const int oo = (y + j) * A.W + (x + i);
+    const float hw = (float)A.hw, hh = (float)A.hh;
+    float4 base = A.color[A.cur][oo];
+    float4 canvas = A.canvas[oo];
+    const float ibx = (float)i * c.b0x, iby = (float)i * c.b0y;
+    float tx = c.t.x + (ibx + (float)j * c.b1x);
+    float ty = c.t.y + (iby + (float)j * c.b1y);
+    if (tx >= hw) tx -= hw;
+    else if (tx < 0) tx += hw;
+    if (ty >= hh) ty -= hh;
+    else if (ty < 0) ty += hh;
+    float4 tb, to;
+    ex_pair(er, A.ew, A.eh, tx, ty, tb, to);
+    float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
+    if (TIME) {
+        bool any_time_w = A.tdi > 0.f || A.tdo > 0.f || A.tcw > 0.f;
+        bool fwd_dir = A.direction == PT_FORWARD || A.direction == PT_BIDIRECTIONAL;
+        bool bwd_dir = A.direction == PT_BACKWARD || A.direction == PT_BIDIRECTIONAL;
+        if (any_time_w) {
+            if (fwd_dir && tc.cpv && A.adv[0].xyth[oo].w > 0)
+                res = res + time_terms_b(A, er, A.adv[0], tc.pf, tc.tb4[0], tc.tb4[1], tc.tb4[2], tc.tb4[3], base, tb,
+                                         canvas, to, oo, i, j);
+            if (bwd_dir && tc.cnv && A.adv[1].xyth[oo].w > 0)
+                res = res + time_terms_b(A, er, A.adv[1], tc.pb, tc.tb4[4], tc.tb4[5], tc.tb4[6], tc.tb4[7], base, tb,
+                                         canvas, to, oo, i, j);
+        }
+    }
+    if (i == 0 && j == 0) {
+        if (A.dt_w > 0.f) {
+            float de = tex1_lin(A.ex_dt, A.ew, A.ew, A.eh, tx, ty);
+            de /= c.sc.y;
+            float di = A.dt[A.cur][y * A.W + x];
+            float diff = fabsf(de - di);
+            diff = fminf(fmaxf(diff, 0.0f), 10.0f);
+            res = res + diff * A.dt_w;
+        }
+    } else {
+        res = fmaf(ssd_alpha(canvas, to, A.alpha_w), A.canvas_w, res);
+    }
+    return res;
+}
+
+/* The reference-order sum of one candidate's tap residuals v[t] (t = (i+r)
+ * * (2r+1) + (j+r)), starting from acc0. */
+template <int RT>
+__device__ __forceinline__ float tap_sum(const PtArgs &A, int x, int y, const float *v, float acc0)
+{
+    constexpr int r = RT, D = 2 * RT + 1;
+    float sum_r = acc0, sum_w = 0.f;
+    for (int i = -r; i <= r; i++) {
+        if (x + i < 0 || x + i >= A.lw) continue;
+        for (int j = -r; j <= r; j++) {
+            if (y + j < 0 || y + j >= A.lh) continue;
+            float wgt = A.gauss[i + r] * A.gauss[j + r];
+            sum_r = fmaf(v[(i + r) * D + (j + r)], wgt, sum_r);
+            sum_w += wgt;
+        }
+    }
+    if (sum_w < 0.000001f) return 1e31f;
+    return sum_r / sum_w;
+}
+
+/* Tap-parallel workgroups cover A.tap_np_res (refresh) or A.tap_np_prop
+ * (propagate) pixels; the host keeps these within the LDS capacity below
+ * (PT_TAP_EVALS tap evaluations per workgroup at most). */
+template <int RT> struct TapCfg {
+    static constexpr int D = 2 * RT + 1, T = D * D;
+    static constexpr int RES_MAX = PT_TAP_EVALS / T;        /* refresh: pixels per workgroup   */
+    static constexpr int PROP_MAX = PT_TAP_EVALS / (4 * T); /* propagate: pixels per workgroup */
+};
+
+/* per-candidate state of a tap-parallel workgroup; the temporal part only
+ * when temporal terms are compiled in */
+template <int NU, bool TK> struct TapUnits {
+    Cand c[NU];
+    int x[NU], y[NU], on[NU];
+    TimeCand tc[TK ? NU : 1];
+};
+
+template <bool TK, int NU>
+__device__ __forceinline__ const TimeCand &tap_tc(const TapUnits<NU, TK> &su, int u)
+{
+    return su.tc[TK ? u : 0];
+}
+
+/* Evaluates taps e = tid, tid + 256, ... of the NU units in `su`: unit e / T,
+ * tap e % T (i outer, j inner), into v[e].  Taps outside the level are left
+ * unwritten; tap_sum never reads them. */
+template <int RT, bool TK, int NUM>
+__device__ __forceinline__ void tap_phase(const PtArgs &A, const TapUnits<NUM, TK> &su, int nu, float *v)
+{
+    constexpr int T = TapCfg<RT>::T, D = 2 * RT + 1;
+    __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
+    for (int e = threadIdx.x; e < nu * T; e += PT_UP_THREADS) {
+        int u = e / T, t = e - u * T;
+        if (!su.on[u]) continue;
+        int i = t / D - RT, j = t - (t / D) * D - RT;
+        int x = su.x[u], y = su.y[u];
+        if (x + i < 0 || x + i >= A.lw || y + j < 0 || y + j >= A.lh) continue;
+        const TimeCand &tc = tap_tc<TK>(su, u);
+        if (TK && (tc.cpv || tc.cnv)) v[e] = tap_residual<true>(A, er, x, y, i, j, su.c[u], tc);
+        else v[e] = tap_residual<false>(A, er, x, y, i, j, su.c[u], tc);
+    }
+}
+
+/* updateResidualCache_kernel, tap-parallel: workgroup g refreshes pixels
+ * g*NP .. g*NP+NP-1 of the level in row-major order. */
+template <int RT, bool TK>
+__device__ __forceinline__ void b_update_residual_tp(const PtArgs &A, int g)
+{
+    constexpr int T = TapCfg<RT>::T, NPM = TapCfg<RT>::RES_MAX;
+    __shared__ TapUnits<NPM, TK> su;
+    __shared__ float sv[NPM * T];
+    const int tid = threadIdx.x, NP = A.tap_np_res;
+    if (tid < NP) {
+        int q = g * NP + tid;
+        int on = 0;
+        if (q < A.lw * A.lh) {
+            int x = q % A.lw, y = q / A.lw, o = y * A.W + x;
+            su.x[tid] = x;
+            su.y[tid] = y;
+            if (image_culled(A, x, y)) {
+                A.res_base[o] = 0.0f;
+            } else {
+                on = 1;
+                if (A.res_incremental) {
+                    on = 0;
+                    for (int i = -RT; i <= RT && !on; i++)
+                        for (int j = -RT; j <= RT; j++) {
+                            int xx = clampi(x + i, 0, A.lw - 1), yy = clampi(y + j, 0, A.lh - 1);
+                            if (A.changed[yy * A.W + xx]) {
+                                on = 1;
+                                break;
+                            }
+                        }
+                }
+                if (on) {
+                    Cand &c = su.c[tid];
+                    c.t = A.base.xyth[o];
+                    c.sc = A.base.sc[o];
+                    basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
+                    if (TK) {
+                        TimeCand &tc = su.tc[TK ? tid : 0];
+                        tc.acc0 = 0.f;
+                        tc.cpv = tc.cnv = false;
+                        time_setup<true>(A, x, y, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4);
+                        if (tc.cpv) tc.acc0 -= hysteresis_term(A, x, y);
+                    }
+                }
+            }
+        }
+        su.on[tid] = on;
+    }
+    __syncthreads();
+    tap_phase<RT, TK>(A, su, NP, sv);
+    __syncthreads();
+    if (tid < NP && su.on[tid]) {
+        int x = su.x[tid], y = su.y[tid];
+        A.res_base[y * A.W + x] = tap_sum<RT>(A, x, y, sv + tid * T, TK ? tap_tc<TK>(su, tid).acc0 : 0.f);
+    }
+}
+
+/* propagate_kernel, tap-parallel: workgroup g searches pixels g*NP ..
+ * g*NP+NP-1; unit u = 4*pixel + candidate. */
+template <int RT, bool TK>
+__device__ __forceinline__ void b_propagate_tp(const PtArgs &A, int g, int jump)
+{
+    constexpr int T = TapCfg<RT>::T, NPM = TapCfg<RT>::PROP_MAX, NUM = 4 * NPM;
+    static_assert(NPM >= 1, "tap-parallel propagate needs 4 (2r+1)^2 <= PT_TAP_EVALS");
+    __shared__ TapUnits<NUM, TK> su;
+    __shared__ float sv[NUM * T];
+    __shared__ float sr[NUM];
+    const int tid = threadIdx.x, NP = A.tap_np_prop, NU = 4 * NP;
+    if (tid < NU) {
+        int k = tid >> 2, cand = tid & 3;
+        int q = g * NP + k;
+        int on = 0;
+        sr[tid] = PT_INF;
+        if (q < A.lw * A.lh) {
+            int x = q % A.lw, y = q / A.lw, o = y * A.W + x;
+            su.x[tid] = x;
+            su.y[tid] = y;
+            if (!image_culled(A, x, y)) {
+                int tx = x, ty = y;
+                if (cand == 0) tx = x + jump;
+                else if (cand == 1) tx = x - jump;
+                else if (cand == 2) ty = y + jump;
+                else ty = y - jump;
+                Cand &c = su.c[tid];
+                propagate_offset(A, tx, ty, (float)x, (float)y, c.t, c.sc);
+                if (candidate_testable(A, c.t, A.color[A.cur][o].w)) {
+                    on = 1;
+                    basis(c.t.z, A.su * c.sc.x, A.sv * c.sc.y, c.b0x, c.b0y, c.b1x, c.b1y);
+                    if (TK) {
+                        TimeCand &tc = su.tc[TK ? tid : 0];
+                        tc.acc0 = 0.f;
+                        tc.cpv = tc.cnv = false;
+                        time_setup<true>(A, x, y, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4);
+                        if (tc.cpv) tc.acc0 -= hysteresis_term(A, x, y);
+                    }
+                }
+            }
+        }
+        su.on[tid] = on;
+    }
+    __syncthreads();
+    tap_phase<RT, TK>(A, su, NU, sv);
+    __syncthreads();
+    if (tid < NU && su.on[tid])
+        sr[tid] = tap_sum<RT>(A, su.x[tid], su.y[tid], sv + tid * T, TK ? tap_tc<TK>(su, tid).acc0 : 0.f);
+    __syncthreads();
+    if (tid < NP) {
+        int q = g * NP + tid;
+        if (q < A.lw * A.lh) {
+            int x = q % A.lw, y = q / A.lw, o = y * A.W + x;
+            if (image_culled(A, x, y)) {
+                A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+                A.work.sc[o] = make_float2(1.f, 1.f);
+            } else {
+                float best = A.res_base[o];
+                int win = -1;
+                for (int k = 0; k < 4; k++) {
+                    float v = sr[4 * tid + k];
+                    if (v < best) {
+                        best = v;
+                        win = k;
+                    }
+                }
+                if (win < 0) {
+                    A.work.xyth[o] = A.base.xyth[o];
+                    A.work.sc[o] = A.base.sc[o];
+                } else {
+                    A.work.xyth[o] = su.c[4 * tid + win].t;
+                    A.work.sc[o] = su.c[4 * tid + win].sc;
+                }
+            }
+        }
+    }
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_UP_THREADS) void k_update_residual_tp(PtArgs A)
+{
+    b_update_residual_tp<RT, TK>(A, blockIdx.x);
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_tp(PtArgs A, int jump)
+{
+    b_propagate_tp<RT, TK>(A, blockIdx.x, jump);
+}
+
+/* ------------------------------------------------------------------ */
 /* multi-job kernels                                                   */
 /* ------------------------------------------------------------------ */
 /*
@@ -2063,6 +2330,21 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void km_update_residual(PT_MULTI_PARA
 {
     MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
     b_update_residual<RT, TK>(PT_JOB(t.j), t.bx, t.by);
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_UP_THREADS) void km_update_residual_tp(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    b_update_residual_tp<RT, TK>(PT_JOB(t.j), t.bx);
+}
+
+template <int RT, bool TK>
+__global__ __launch_bounds__(PT_UP_THREADS) void km_propagate_tp(PT_MULTI_PARAMS)
+{
+    MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
+    const PtArgs &A = PT_JOB(t.j);
+    b_propagate_tp<RT, TK>(A, t.bx, A.jump);
 }
 
 __global__ __launch_bounds__(PT_BX *PT_BY) void km_sync_canvas(PT_MULTI_PARAMS)
@@ -2146,6 +2428,16 @@ static const dim3 kBlock(PT_BX, PT_BY);
 
 hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s)
 {
+    if (pt_default_shape(PT_OP_RESIDUAL, a) == PT_SHAPE_TAP) {
+        dim3 g(pt_shape_tiles(PT_OP_RESIDUAL, PT_SHAPE_TAP, a, nullptr)), bu(PT_UP_THREADS);
+        bool tk = a.time_active != 0;
+        switch (a.r) {
+        case 2: hipLaunchKernelGGL((tk ? k_update_residual_tp<2, true> : k_update_residual_tp<2, false>), g, bu, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((tk ? k_update_residual_tp<3, true> : k_update_residual_tp<3, false>), g, bu, 0, s, a); break;
+        default: hipLaunchKernelGGL((tk ? k_update_residual_tp<4, true> : k_update_residual_tp<4, false>), g, bu, 0, s, a); break;
+        }
+        return hipGetLastError();
+    }
     PT_DISPATCH_R(k_update_residual, a);
     return hipGetLastError();
 }
@@ -2215,9 +2507,27 @@ static bool shape_is_ee(int shape)
     return shape == PT_SHAPE_PIXEL_EE || shape == PT_SHAPE_UP_EE_A || shape == PT_SHAPE_UP_EE_B;
 }
 
+/* pixels per tap-parallel workgroup for (op, a.r); 0 when unsupported */
+static int tap_pixels(int op, const PtArgs &a)
+{
+    if (op == PT_OP_RESIDUAL) return (a.r >= 2 && a.r <= 4) ? a.tap_np_res : 0;
+    if (op == PT_OP_PROPAGATE) return (a.r >= 2 && a.r <= 3) ? a.tap_np_prop : 0;
+    return 0;
+}
+
+void pt_tap_configure(PtArgs &a, int evals)
+{
+    int t = (2 * a.r + 1) * (2 * a.r + 1);
+    evals = evals < 1 ? 1 : (evals > PT_TAP_EVALS ? PT_TAP_EVALS : evals);
+    int nr = evals / t, np = evals / (4 * t);
+    a.tap_np_res = nr < 1 ? 1 : nr;
+    a.tap_np_prop = np < 1 ? 1 : np;
+}
+
 int pt_shape_valid(int op, int shape, const PtArgs &a)
 {
     if (shape < 0 || shape >= PT_SHAPE_COUNT) return 0;
+    if (shape == PT_SHAPE_TAP) return tap_pixels(op, a) > 0;
     if (op == PT_OP_RESIDUAL || op == PT_OP_SYNC_CANVAS) return shape == PT_SHAPE_PIXEL;
     if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return 0;
     if (shape_is_ee(shape) && !a.ee_ok) return 0;
@@ -2229,6 +2539,7 @@ int pt_shape_valid(int op, int shape, const PtArgs &a)
 
 int pt_default_shape(int op, const PtArgs &a)
 {
+    if ((int64_t)a.lw * a.lh <= a.tap_max_px && pt_shape_valid(op, PT_SHAPE_TAP, a)) return PT_SHAPE_TAP;
     if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return PT_SHAPE_PIXEL;
     if (a.unit_parallel && pt_shape_valid(op, PT_SHAPE_UP_A, a)) {
         /* propagate keeps a candidate table per unit in LDS: the 8x8 tile
@@ -2241,6 +2552,12 @@ int pt_default_shape(int op, const PtArgs &a)
 
 int pt_shape_tiles(int op, int shape, const PtArgs &a, int *tiles_x)
 {
+    if (shape == PT_SHAPE_TAP) {
+        int np = tap_pixels(op, a);
+        int n = np > 0 ? (a.lw * a.lh + np - 1) / np : 0;
+        if (tiles_x) *tiles_x = n > 0 ? n : 1;
+        return n;
+    }
     int tx, ty;
     shape_tile(op, shape, tx, ty);
     int w = a.lw, h = a.lh;
@@ -2277,10 +2594,16 @@ hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
     PtArgs a = a0;
     a.jump = jump;
     int shape = pt_default_shape(PT_OP_PROPAGATE, a);
+    bool tk = a.time_active != 0;
+    if (shape == PT_SHAPE_TAP) {
+        dim3 gt(pt_shape_tiles(PT_OP_PROPAGATE, PT_SHAPE_TAP, a, nullptr)), bt(PT_UP_THREADS);
+        if (a.r == 2) hipLaunchKernelGGL((tk ? k_propagate_tp<2, true> : k_propagate_tp<2, false>), gt, bt, 0, s, a, jump);
+        else hipLaunchKernelGGL((tk ? k_propagate_tp<3, true> : k_propagate_tp<3, false>), gt, bt, 0, s, a, jump);
+        return hipGetLastError();
+    }
     int tx, ty;
     shape_tile(PT_OP_PROPAGATE, shape, tx, ty);
     dim3 g((a.lw + tx - 1) / tx, (a.lh + ty - 1) / ty), bu(PT_UP_THREADS);
-    bool tk = a.time_active != 0;
 #define L_PIX(R) hipLaunchKernelGGL((tk ? k_propagate<R, true> : k_propagate<R, false>), g, kBlock, 0, s, a, jump)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? k_propagate_ee<R, true> : k_propagate_ee<R, false>), g, kBlock, 0, s, a, jump)
 #define L_UEB(R) hipLaunchKernelGGL((tk ? k_propagate_up<R, true, 8, 8, true> : k_propagate_up<R, true, 8, 8, false>), g, bu, 0, s, a, jump)
@@ -2333,6 +2656,14 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
     bool tk = a.time_active != 0;
     switch (op) {
     case PT_OP_RESIDUAL: {
+        if (shape == PT_SHAPE_TAP) {
+#define LT(R) hipLaunchKernelGGL((tk ? km_update_residual_tp<R, true> : km_update_residual_tp<R, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+            if (a.r == 2) LT(2);
+            else if (a.r == 3) LT(3);
+            else LT(4);
+#undef LT
+            break;
+        }
 #define L(R) hipLaunchKernelGGL((tk ? km_update_residual<R, true> : km_update_residual<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
         PT_RT_SWITCH(a.r, L);
 #undef L
@@ -2352,6 +2683,10 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
         case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
         case PT_SHAPE_UP_A: PT_RT_SWITCH(a.r, L_UA); break;
         case PT_SHAPE_UP_EE_B: PT_RT_SWITCH(a.r, L_UEB); break;
+        case PT_SHAPE_TAP:
+            if (a.r == 2) hipLaunchKernelGGL((tk ? km_propagate_tp<2, true> : km_propagate_tp<2, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap);
+            else hipLaunchKernelGGL((tk ? km_propagate_tp<3, true> : km_propagate_tp<3, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap);
+            break;
         default: PT_RT_SWITCH(a.r, L_UB); break;
         }
 #undef L_PIX

@@ -115,6 +115,8 @@ struct pt_ctx {
     uint8_t *changed = nullptr;
     int res_level = -1;
     bool incremental_residual = true;
+    int64_t tap_max_pixels = 128 * 128; /* tap-parallel shapes up to this level size */
+    int tap_evals = 1024;               /* tap-parallel: tap evaluations per workgroup */
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -499,6 +501,15 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
         return PT_OK;
     case PT_TUNE_INCREMENTAL_RESIDUAL:
         c->incremental_residual = value != 0;
+        return PT_OK;
+    case PT_TUNE_TAP_PARALLEL_MAX_PIXELS:
+        if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative pixel count");
+        c->tap_max_pixels = value;
+        return PT_OK;
+    case PT_TUNE_TAP_PARALLEL_EVALS:
+        if (value < 1 || value > PT_TAP_EVALS)
+            return set_err(c, PT_ERR_INVALID_ARG, "tap evaluations per workgroup must be 1..%d", PT_TAP_EVALS);
+        c->tap_evals = (int)value;
         return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
@@ -967,6 +978,8 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.early_exit = (a.ee_ok && (int64_t)a.lw * a.lh >= c->early_exit_min_pixels) ? 1 : 0;
     a.unit_parallel = c->unit_parallel ? 1 : 0;
     a.res_incremental = 0;
+    a.tap_max_px = (int)std::min<int64_t>(c->tap_max_pixels, 1 << 30);
+    pt_tap_configure(a, c->tap_evals);
     a.changed = c->changed;
     a.base = c->base;
     a.work = c->work;

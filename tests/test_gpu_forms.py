@@ -127,7 +127,7 @@ def test_batch_equals_each_job_alone():
         b.close()
 
 
-@pytest.mark.parametrize("shape", range(6))
+@pytest.mark.parametrize("shape", range(7))
 def test_batch_every_shape_equals_alone(shape):
     """Each kernel shape, forced on every level class, gives the same jobs."""
     jobs = _batch_jobs()[:3]

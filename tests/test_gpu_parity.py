@@ -21,11 +21,18 @@ def gpu():
     ctx.close()
 
 
-def _both(style, frames, overrides, ops, early_exit=1, unit_parallel=1):
+def _both(style, frames, overrides, ops, early_exit=1, unit_parallel=1, tap=0):
+    """Run `ops` on the GPU and on the oracle.  tap=N > 0 sends every
+    propagate and residual refresh with a supported radius to the
+    tap-parallel kernels with N tap evaluations per workgroup; tap=0 keeps
+    them off so the pixel and unit-parallel kernels run."""
     g = Context.gpu(0)
     g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, early_exit)
     g.set_tuning(_ffi.PT_TUNE_UNIT_PARALLEL, unit_parallel)
     g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)  # exercise the early-exit kernels on small levels
+    g.set_tuning(_ffi.PT_TUNE_TAP_PARALLEL_MAX_PIXELS, (1 << 30) if tap else 0)
+    if tap:
+        g.set_tuning(_ffi.PT_TUNE_TAP_PARALLEL_EVALS, tap)
     o = OracleContext()
     try:
         sg = Session(g, style, frames, g.default_params(**overrides))
@@ -69,33 +76,35 @@ def _still_ops(levels, iterations=2):
     return ops
 
 
-@pytest.mark.parametrize("early_exit,unit_parallel", [(1, 1), (0, 1), (1, 0), (0, 0)])
+@pytest.mark.parametrize("early_exit,unit_parallel,tap", [(1, 1, 0), (0, 1, 0), (1, 0, 0), (0, 0, 0), (1, 1, 1024),
+                                                         (1, 1, 300)])
 @pytest.mark.parametrize("r,orient,dtw", [(2, False, 0.0), (3, True, 0.5), (1, True, 0.0), (4, False, 0.0),
                                           (5, True, 0.0)])
-def test_still_operator_sequence(r, orient, dtw, early_exit, unit_parallel):
+def test_still_operator_sequence(r, orient, dtw, early_exit, unit_parallel, tap):
     st = W.style_pair(40, 36, seed=5, guides=orient or dtw > 0)
     fr = W.still_frame(44, 34, seed=6, guides=orient or dtw > 0)
     ov = dict(residual_window_size=r, levels=3, orientation_enabled=int(orient), dist_trans_weight=dtw)
-    ga, oa = _both(st, [fr], ov, _still_ops(3), early_exit, unit_parallel)
+    ga, oa = _both(st, [fr], ov, _still_ops(3), early_exit, unit_parallel, tap)
     assert_same_states(ga, oa)
 
 
-@pytest.mark.parametrize("early_exit,unit_parallel", [(1, 1), (0, 1), (1, 0)])
-def test_counter_rng_scatter(early_exit, unit_parallel):
+@pytest.mark.parametrize("early_exit,unit_parallel,tap", [(1, 1, 0), (0, 1, 0), (1, 0, 0), (1, 1, 1024)])
+def test_counter_rng_scatter(early_exit, unit_parallel, tap):
     st = W.style_pair(64, 64, seed=7)
     fr = W.still_frame(48, 40, seed=8)
     ov = dict(residual_window_size=2, levels=2, rng_mode=_ffi.PT_RNG_COUNTER, seed=1234, orientation_enabled=0)
-    ga, oa = _both(st, [fr], ov, _still_ops(2, iterations=3), early_exit, unit_parallel)
+    ga, oa = _both(st, [fr], ov, _still_ops(2, iterations=3), early_exit, unit_parallel, tap)
     assert_same_states(ga, oa)
 
 
-def test_odd_sizes_and_transparency():
+@pytest.mark.parametrize("tap", [0, 1024, 1])
+def test_odd_sizes_and_transparency(tap):
     st = W.style_pair(37, 29, seed=9)
     st["output"][..., 3] = np.where(st["output"][..., 3] > 0.6, 1.0, 0.2).astype(np.float32)
     fr = W.still_frame(45, 27, seed=10)
     fr["color"][..., 3] = 1.0
     ov = dict(residual_window_size=2, levels=3)
-    ga, oa = _both(st, [fr], ov, _still_ops(3))
+    ga, oa = _both(st, [fr], ov, _still_ops(3), tap=tap)
     assert_same_states(ga, oa)
 
 
@@ -119,26 +128,29 @@ def _anim_ops(levels, frames, scheme_merge="merge_random"):
     return ops
 
 
-@pytest.mark.parametrize("merge,hyst", [("merge_random", 0.3), ("merge_residual", 0.3), ("merge_random", 0.0)])
-def test_animation_operator_sequence(merge, hyst):
+@pytest.mark.parametrize("merge,hyst,tap", [("merge_random", 0.3, 0), ("merge_residual", 0.3, 0),
+                                            ("merge_random", 0.0, 0), ("merge_random", 0.3, 1024),
+                                            ("merge_residual", 0.3, 200)])
+def test_animation_operator_sequence(merge, hyst, tap):
     st = W.style_pair(48, 48, seed=12, guides=True)
     frames = W.animation(40, 32, 3, seed=13)
     ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=hyst)
-    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, merge))
+    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, merge), tap=tap)
     assert_same_states(ga, oa)
 
 
 def test_kernel_variants_agree_at_scale():
-    """Early rejection and unit-parallel search on and off, on a 256^2 still
-    with guides: offsets, canvas and residual identical."""
+    """Early rejection, unit-parallel and tap-parallel search on and off, on
+    a 256^2 still with guides: offsets, canvas and residual identical."""
     st = W.style_pair(256, 256, seed=51, guides=True)
     fr = W.still_frame(256, 256, seed=52, guides=True)
     outs = []
-    for ee, up in ((1, 1), (0, 0), (1, 0), (0, 1)):
+    for ee, up, tap in ((1, 1, 128 * 128), (0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 1 << 30)):
         with Context.gpu(0) as g:
             g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, ee)
             g.set_tuning(_ffi.PT_TUNE_UNIT_PARALLEL, up)
             g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)
+            g.set_tuning(_ffi.PT_TUNE_TAP_PARALLEL_MAX_PIXELS, tap)
             p = g.default_params(residual_window_size=3, levels=4, dist_trans_weight=0.5)
             outs.append(g.synthesize(p, [fr], st, iterations=4))
     for other in outs[1:]:
