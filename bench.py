@@ -11,6 +11,9 @@ Beside it, on the same GPU in the same run:
              5x5/7x7 patches, 3-5 levels, 6 iterations) planned as one
              multi-job launch sequence, shapes autotuned, replayed as a HIP
              graph; compared per job, bit for bit, with the plain path;
+  still256 -- the CPU-runnable still (256^2, 5x5 patches, 3 levels, 4
+             iterations, no orientation): GPU frame time, the oracle on all
+             host cores (median of 5) and on 1 core, offsets compared;
   served  -- 32 streamed sequences (256..1920 x 256..1080, exemplars
              512..1024, 7x7 patches, 4 levels, 3 iterations, temporal
              coherence 0.5) under Poisson frame arrivals, continuous
@@ -105,6 +108,62 @@ def _gen_seq(arg):
     from painttween_amd import workloads as W
     spec, frames = arg
     return W.serving_sequence(spec, frames)
+
+
+def run_still256(ctx, inputs, cpu_runs, cpu_threads):
+    """The 256^2 still (exemplar pair and target 256^2, 5x5 patches, 3 levels,
+    4 iterations, orientation and distance transform off) on the GPU and,
+    unless cpu_runs is 0, on the oracle with `cpu_threads` host threads
+    (median of cpu_runs) and with one thread (one run).  The offsets of the
+    two are compared byte for byte."""
+    import torch
+    from painttween_amd import _ffi
+    from painttween_amd.forms import DeviceImages, DeviceOutputs
+    style, frame = inputs
+    kw = dict(residual_window_size=2, levels=3, orientation_enabled=0, dist_trans_weight=0.0)
+    iterations = 4
+    img = DeviceImages(0)
+    fg = (_ffi.FrameGuides * 1)()
+    fg[0] = img.frame(frame)
+    sg = img.style(style)
+    out = DeviceOutputs(256, 256, 1)
+    params = ctx.default_params(**kw)
+    o, c, r = out.pointers()
+
+    def run():
+        ctx._check(ctx.lib.pt_synthesize_device(ctx.handle, C.byref(params), _ffi.PT_S_FMBM_RM, iterations, 1, 256,
+                                                256, 1, fg, 256, 256, C.byref(sg), o, c, r))
+
+    run()
+    torch.cuda.synchronize()
+    ts = []
+    p0 = ctx.pde_count()
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    pdes = (ctx.pde_count() - p0) // 5
+    gpu_ms = float(np.median(ts)) * 1e3
+    res = {"image": [256, 256], "exemplar": [256, 256], "patch": 25, "levels": 3, "iterations": iterations,
+           "pde_per_frame": int(pdes), "gpu_ms": gpu_ms, "gpu_frames_per_s": 1e3 / gpu_ms}
+    if cpu_runs > 0:
+        from tests.oracle_ctx import OracleContext
+        gpu_off = out.numpy()[0]
+        cpu = {}
+        for th, runs in ((cpu_threads, cpu_runs), (1, 1)):
+            with OracleContext(th) as oc:
+                tt = []
+                for _ in range(runs):
+                    t0 = time.perf_counter()
+                    ref = oc.synthesize(oc.default_params(**kw), [frame], style, iterations=iterations)
+                    tt.append(time.perf_counter() - t0)
+            cpu[th] = float(np.median(tt))
+        res.update({"cpu_threads": cpu_threads, "cpu_s": cpu[cpu_threads], "cpu_1thread_s": cpu[1],
+                    "cpu_frames_per_s": 1.0 / cpu[cpu_threads], "cpu_1thread_frames_per_s": 1.0 / cpu[1],
+                    "cpu_runs": cpu_runs, "gpu_vs_cpu": cpu[cpu_threads] / (gpu_ms * 1e-3),
+                    "offsets_identical": bool(ref[0].tobytes() == gpu_off.tobytes())})
+    return res
 
 
 def run_batched(ctx, jobs_in, steps):
@@ -331,6 +390,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-anim", action="store_true")
+    ap.add_argument("--no-still256", action="store_true")
     ap.add_argument("--anim-levels", type=int, default=4)
     ap.add_argument("--anim-iterations", type=int, default=3)
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -364,6 +424,7 @@ def main():
     # inputs of the batched and served forms, generated before any GPU use
     from painttween_amd import workloads as W
     batch_in = serve_in = anim_in = None
+    still256_in = None if args.no_still256 else (W.style_pair(256, 256, seed=1), W.still_frame(256, 256, seed=1))
     if not args.no_anim:
         anim_in = _pool_map(_gen_anim, [(960, 540, 24, 512)])[0]
         print("animation inputs generated", file=sys.stderr, flush=True)
@@ -486,6 +547,9 @@ def main():
         "kernels": kstats,
         "cpu_baseline": cpu,
     }
+    if still256_in is not None:
+        line["still256"] = run_still256(ctx, still256_in, 0 if args.no_cpu else 5, min(os.cpu_count() or 1, 16))
+        print("still256: %.2f ms on the GPU" % line["still256"]["gpu_ms"], file=sys.stderr, flush=True)
     if anim_in is not None:
         line["animation"] = run_animation(ctx, anim_in, args.anim_levels, args.anim_iterations)
         print("animation: %.2f frames/s" % line["animation"]["frames_per_s"], file=sys.stderr, flush=True)

@@ -1497,9 +1497,13 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         s_next = PT_UP_THREADS;
     }
     __syncthreads();
-    /* phase 1: candidate set-up in lockstep */
-    for (int u = threadIdx.x; u < U; u += PT_UP_THREADS) {
-        int p = u >> 2, k = u & 3;
+    /* phase 1: candidate set-up in lockstep.  Without temporal terms,
+     * neighbouring lanes take the same candidate direction of neighbouring
+     * pixels, so the list (and the first units each lane evaluates) holds
+     * coherent exemplar positions and a wave's gathers share cache lines;
+     * with them, the pixel-major order measured faster. */
+    for (int v = threadIdx.x; v < U; v += PT_UP_THREADS) {
+        int p = TM ? v >> 2 : v % P, k = TM ? v & 3 : v / P, u = 4 * p + k;
         int x = x0 + p % TX, y = y0 + p / TX;
         s_r[u] = PT_INF;
         if (!up_open(A, s_sw[p], s_best[p])) continue;
@@ -1658,9 +1662,12 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     while (true) {
         if (threadIdx.x == 0) s_next = PT_UP_THREADS;
         __syncthreads();
+        /* unit u: pixel p, sample k.  Without temporal terms units run
+         * sample-major (neighbouring lanes: the same sample of neighbouring
+         * pixels, coherent exemplar positions), with them pixel-major. */
         auto setup = [&](int u, Cand &c, TimeCand &tc, int &x, int &y, int &p) -> bool {
-            p = u / SU;
-            int k = u % SU;
+            p = TM ? u / SU : u % P;
+            int k = TM ? u % SU : u / P;
             int st = s_start[p];
             bool todo = (k < n) ? (k >= st && st < n) : (first && st <= n);
             if (!todo) return false;
@@ -1681,7 +1688,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
             if constexpr (TM) up_time_load<P>(s_tm, p, tc);
             return true;
         };
-        auto done = [&](int u, float v) { s_r[(u / SU) * S + (u % SU)] = v; };
+        auto done = [&](int u, float v) { s_r[TM ? (u / SU) * S + (u % SU) : (u % P) * S + (u / P)] = v; };
         if (EE) {
             up_queue_ee<RT, TX, false, TM>(A, T, er, threadIdx.x, P * SU, &s_next, s_best, s_sw, setup, done);
         } else {
