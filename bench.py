@@ -110,6 +110,39 @@ def _gen_seq(arg):
     return W.serving_sequence(spec, frames)
 
 
+BANDWIDTH_KINDS = ("canvas", "upsample", "random", "pyramid", "advect", "merge")
+
+
+def profile_kinds(ctx, kinds):
+    """HIP-event totals and algorithmic bytes of the recorded launches of
+    each kernel class; achieved GB/s and the fraction of the HBM peak."""
+    out = {}
+    for name in kinds:
+        n, ms, px, by = C.c_uint64(), C.c_double(), C.c_uint64(), C.c_uint64()
+        ctx._check(ctx.lib.pt_profile_get(ctx.handle, name.encode(), C.byref(n), C.byref(ms), C.byref(px)))
+        ctx._check(ctx.lib.pt_profile_bytes(ctx.handle, name.encode(), C.byref(by)))
+        if n.value == 0:
+            continue
+        gbs = by.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0
+        out[name] = {"launches": n.value, "total_ms": ms.value, "avg_us": ms.value * 1e3 / n.value,
+                     "pixels": px.value, "bytes_per_launch": by.value / n.value, "achieved_gbs": gbs,
+                     "hbm_frac": gbs / HBM_PEAK_GBS}
+        # the launches at the largest level size alone (small levels are
+        # bound by launch latency, not bandwidth)
+        cnt = C.c_int()
+        m = n.value
+        pa, ba, ta = (C.c_uint64 * m)(), (C.c_uint64 * m)(), (C.c_float * m)()
+        ctx._check(ctx.lib.pt_profile_records(ctx.handle, name.encode(), m, pa, ba, ta, C.byref(cnt)))
+        top = max(pa[i] for i in range(m))
+        sel = [i for i in range(m) if pa[i] == top]
+        tb, tt = sum(ba[i] for i in sel), sum(ta[i] for i in sel)
+        g = tb / (tt * 1e-3) / 1e9 if tt > 0 else 0.0
+        out[name]["largest_level"] = {"pixels": int(top), "launches": len(sel), "avg_us": tt * 1e3 / len(sel),
+                                      "bytes_per_launch": tb / len(sel), "achieved_gbs": g,
+                                      "hbm_frac": g / HBM_PEAK_GBS}
+    return out
+
+
 def run_still256(ctx, inputs, cpu_runs, cpu_threads):
     """The 256^2 still (exemplar pair and target 256^2, 5x5 patches, 3 levels,
     4 iterations, orientation and distance transform off) on the GPU and,
@@ -258,8 +291,13 @@ def run_animation(ctx, anim_in, levels, iterations):
         ctx._check(ctx.lib.pt_synthesize_device(ctx.handle, C.byref(params), _ffi.PT_S_FMBM_RM, iterations, 1, w, h,
                                                 F, fg, sw, sh, C.byref(sg), o, c, r))
 
+    # the warm run is recorded with HIP events (bandwidth-bound kernels:
+    # advection and merges run only here); the timed run is not
+    ctx._check(ctx.lib.pt_profile_enable(ctx.handle, 1))
     run()
     torch.cuda.synchronize()
+    kinds = profile_kinds(ctx, BANDWIDTH_KINDS)
+    ctx._check(ctx.lib.pt_profile_enable(ctx.handle, 0))
     pde0 = ctx.pde_count()
     t0 = time.perf_counter()
     run()
@@ -268,7 +306,7 @@ def run_animation(ctx, anim_in, levels, iterations):
     pdes = ctx.pde_count() - pde0
     res = out.res.cpu().numpy()
     assert np.isfinite(res).all() and (res >= 0).all(), "invalid residual"
-    return {"frames": F, "image": [w, h], "exemplar": [sw, sh], "levels": levels, "iterations": iterations,
+    return {"bandwidth_kernels": kinds, "frames": F, "image": [w, h], "exemplar": [sw, sh], "levels": levels, "iterations": iterations,
             "passes": 2, "seconds": dt, "frames_per_s": F / dt, "frame_passes_per_s": 2 * F / dt,
             "pde_per_s": pdes / dt, "schedule": "FMBM_RM, fixup passes 1 (forward then backward), 7x7 patches, "
                                                 "temporal coherence 0.5, time derivatives 1"}
@@ -498,6 +536,7 @@ def main():
         n, ms, px = C.c_uint64(), C.c_double(), C.c_uint64()
         ctx._check(lib.pt_profile_get(ctx.handle, name.encode(), C.byref(n), C.byref(ms), C.byref(px)))
         kstats[name] = dict(launches=n.value, total_ms=ms.value, pixels=px.value)
+    still_bw = profile_kinds(ctx, BANDWIDTH_KINDS)
     ctx._check(lib.pt_profile_enable(ctx.handle, 0))
 
     # sanity of the produced frame (not timed)
@@ -545,6 +584,7 @@ def main():
         "valu": {"achieved": valu_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": valu_tflops / FP32_PEAK_TFLOPS, "flop_per_tap": FLOP_PER_TAP},
         "kernels": kstats,
+        "bandwidth_kernels": still_bw,
         "cpu_baseline": cpu,
     }
     if still256_in is not None:
@@ -552,6 +592,11 @@ def main():
         print("still256: %.2f ms on the GPU" % line["still256"]["gpu_ms"], file=sys.stderr, flush=True)
     if anim_in is not None:
         line["animation"] = run_animation(ctx, anim_in, args.anim_levels, args.anim_iterations)
+        # advection and merges run only in the animation: their HBM rooflines
+        # come from its (untimed, event-recorded) warm run
+        for k in ("advect", "merge"):
+            if k in line["animation"]["bandwidth_kernels"]:
+                line["bandwidth_kernels"][k] = line["animation"]["bandwidth_kernels"][k]
         print("animation: %.2f frames/s" % line["animation"]["frames_per_s"], file=sys.stderr, flush=True)
     if batch_in is not None:
         line["batched"] = run_batched(ctx, batch_in, args.batch_steps)

@@ -306,13 +306,23 @@ int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --
  * pt_profile_enable(ctx, 1) clears the log and starts recording a pair of HIP
- * events on the context's stream around every NNF-path launch;
+ * events on the context's stream around every synthesis-path launch;
  * pt_profile_get() sums the launches of one kernel class ("propagate",
- * "scatter", "update_residual", "canvas", "upsample", "advect", "random")
- * since then: launch count, total GPU milliseconds, total pixels covered. */
+ * "scatter", "update_residual", "canvas" (the fused sync/canvas pass),
+ * "upsample", "advect", "merge", "random", "pyramid") since then: launch
+ * count, total GPU milliseconds, total pixels covered.  pt_profile_bytes()
+ * gives the same launches' algorithmic HBM bytes (each operand read or
+ * written once, the exemplar counted once per pixel), from which a caller
+ * forms achieved GB/s against the HBM roofline. */
 int pt_profile_enable(pt_ctx *ctx, int enable);
 int pt_profile_get(pt_ctx *ctx, const char *kernel, uint64_t *launches, double *total_ms,
                    uint64_t *pixels);
+int pt_profile_bytes(pt_ctx *ctx, const char *kernel, uint64_t *bytes);
+/* The recorded launches of one class one by one, in issue order: pixels,
+ * algorithmic bytes and GPU milliseconds of the first max_records of them;
+ * *count receives how many there are. */
+int pt_profile_records(pt_ctx *ctx, const char *kernel, int max_records, uint64_t *pixels, uint64_t *bytes,
+                       float *ms, int *count);
 
 /* ---- one-shot schedules ------------------------------------------------- */
 

@@ -270,6 +270,11 @@ def load_library():
         lib.pt_set_tuning.argtypes = [VP, I, I]
         lib.pt_profile_enable.restype = I
         lib.pt_profile_enable.argtypes = [VP, I]
+        lib.pt_profile_bytes.restype = I
+        lib.pt_profile_bytes.argtypes = [VP, C.c_char_p, C.POINTER(C.c_uint64)]
+        lib.pt_profile_records.restype = I
+        lib.pt_profile_records.argtypes = [VP, C.c_char_p, I, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_float), C.POINTER(I)]
         lib.pt_profile_get.restype = I
         lib.pt_profile_get.argtypes = [VP, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                                        C.POINTER(C.c_uint64)]

@@ -15,6 +15,11 @@
 
 #define PT_BX 16
 #define PT_BY 16
+/* Bandwidth-bound per-pixel kernels (sync/canvas, upsample, advection,
+ * merges, random init) run 64 x 4 blocks: a wave covers 64 consecutive
+ * pixels of one row, so its float4 / float2 accesses are whole cache lines. */
+#define PT_RX 64
+#define PT_RY 4
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -406,6 +411,13 @@ __device__ __forceinline__ float test_offset(const PtArgs &A, const Tile &T, int
 /* ------------------------------------------------------------------ */
 /* kernels                                                             */
 /* ------------------------------------------------------------------ */
+__device__ __forceinline__ bool pixel_of_rows(const PtArgs &A, int bx, int by, int &x, int &y)
+{
+    x = bx * PT_RX + threadIdx.x;
+    y = by * PT_RY + threadIdx.y;
+    return x < A.lw && y < A.lh;
+}
+
 __device__ __forceinline__ bool pixel_of(const PtArgs &A, int bx, int by, int &x, int &y)
 {
     x = bx * PT_BX + threadIdx.x;
@@ -526,7 +538,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_update_residual(PtArgs A)
 __device__ __forceinline__ void b_canvas(const PtArgs &A, int bx, int by)
 {
     int x, y;
-    if (!pixel_of(A, bx, by, x, y)) return;
+    if (!pixel_of_rows(A, bx, by, x, y)) return;
     int o = y * A.W + x;
     float4 n = A.base.xyth[o];
     A.canvas[o] = tex_ex_out(A, n.x, n.y);
@@ -546,8 +558,8 @@ __device__ __forceinline__ bool bits_differ(float a, float b) { return __float_a
 
 __device__ __forceinline__ void b_sync_canvas(const PtArgs &A, int bx, int by, int dw, int dh)
 {
-    int x = bx * PT_BX + threadIdx.x;
-    int y = by * PT_BY + threadIdx.y;
+    int x = bx * PT_RX + threadIdx.x;
+    int y = by * PT_RY + threadIdx.y;
     int o = y * A.W + x;
     float4 n;
     bool have = false, ch = false;
@@ -580,7 +592,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_sync_canvas(PtArgs A, int dw, 
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A)
 {
     int x, y;
-    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     float4 r = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
     A.work.xyth[o] = make_float4(r.x, r.y, r.z, 0.f);
@@ -1764,7 +1776,7 @@ __global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A)
 {
     int x, y;
-    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
     if (!in_bounds(x, y, A.lw, A.lh, 1)) return;
     float cxf = (float)((double)x - 0.5) / 2.0f;
     float cyf = (float)((double)y - 0.5) / 2.0f;
@@ -1829,7 +1841,7 @@ __device__ __forceinline__ bool translated_position(const PtArgs &A, int fwd, in
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int time_step, int flags)
 {
     int x, y;
-    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     const PtAdv &d = A.adv[fwd ? 0 : 1];
     if (image_culled(A, x, y)) {
@@ -1879,7 +1891,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int fwd, float weight, int force)
 {
     int x, y;
-    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     const PtAdv &d = A.adv[fwd ? 0 : 1];
     float4 a = d.xyth[o];
@@ -1898,7 +1910,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int f
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_residual(PtArgs A, int fwd)
 {
     int x, y;
-    if (!pixel_of(A, blockIdx.x, blockIdx.y, x, y)) return;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     const PtAdv &d = A.adv[fwd ? 0 : 1];
     if (d.res[o] < A.res_base[o]) {
@@ -1914,12 +1926,61 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_residual(PtArgs A, int f
     }
 }
 
-/* downsampleAverage (downsample.cpp:41-160): one level, `ch` floats per
- * element. */
-__global__ void k_downsample(const float *src, int pw, float *dst, int lw, int lh, int ch, int scale_mode)
+/* downsampleAverage (downsample.cpp:41-160): one level, CH floats per
+ * element, ((a + b) + c) + d then * 0.25 (and * 0.5 for velocities) per
+ * channel.  An element's CH floats are loaded together (float2 / float4
+ * where CH allows), 64 x 4 threads per block. */
+template <int CH> struct DsVec;
+template <> struct DsVec<1> { typedef float T; };
+template <> struct DsVec<2> { typedef float2 T; };
+template <> struct DsVec<4> { typedef float4 T; };
+
+__device__ __forceinline__ float ds_get(const float &v, int) { return v; }
+__device__ __forceinline__ float ds_get(const float2 &v, int k) { return k == 0 ? v.x : v.y; }
+__device__ __forceinline__ float ds_get(const float4 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+__device__ __forceinline__ void ds_set(float &v, int, float f) { v = f; }
+__device__ __forceinline__ void ds_set(float2 &v, int k, float f) { if (k == 0) v.x = f; else v.y = f; }
+__device__ __forceinline__ void ds_set(float4 &v, int k, float f)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (k == 0) v.x = f;
+    else if (k == 1) v.y = f;
+    else if (k == 2) v.z = f;
+    else v.w = f;
+}
+
+template <int CH>
+__global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_v(const float *src, int pw, float *dst, int lw, int lh,
+                                                              int scale_mode)
+{
+    typedef typename DsVec<CH>::T V;
+    int x = blockIdx.x * PT_RX + threadIdx.x;
+    int y = blockIdx.y * PT_RY + threadIdx.y;
+    if (x >= lw || y >= lh) return;
+    const V *s = (const V *)src;
+    V a = s[(y * 2) * pw + x * 2], b = s[(y * 2) * pw + x * 2 + 1];
+    V c = s[(y * 2 + 1) * pw + x * 2], d = s[(y * 2 + 1) * pw + x * 2 + 1];
+    V o;
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        float v = 0.f;
+        v += ds_get(a, k);
+        v += ds_get(b, k);
+        v += ds_get(c, k);
+        v += ds_get(d, k);
+        v *= 0.25f;
+        if (scale_mode == PT_DOWNSAMPLE_SCALE_WITH_LEVELS)
+            v *= 0.5f;
+        ds_set(o, k, v);
+    }
+    ((V *)dst)[y * lw + x] = o;
+}
+
+/* other element widths (the structure tensor, 3 floats) */
+__global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample(const float *src, int pw, float *dst, int lw, int lh,
+                                                            int ch, int scale_mode)
+{
+    int x = blockIdx.x * PT_RX + threadIdx.x;
+    int y = blockIdx.y * PT_RY + threadIdx.y;
     if (x >= lw || y >= lh) return;
     for (int k = 0; k < ch; k++) {
         float s = 0.f;
@@ -2412,6 +2473,8 @@ __global__ __launch_bounds__(PT_UP_THREADS) void km_scatter_up(PT_MULTI_PARAMS)
 /* ------------------------------------------------------------------ */
 static dim3 grid_for(int w, int h) { return dim3((w + PT_BX - 1) / PT_BX, (h + PT_BY - 1) / PT_BY); }
 static const dim3 kBlock(PT_BX, PT_BY);
+static dim3 grid_rows(int w, int h) { return dim3((w + PT_RX - 1) / PT_RX, (h + PT_RY - 1) / PT_RY); }
+static const dim3 kRowBlock(PT_RX, PT_RY);
 
 #define PT_DISPATCH_R(KERNEL, A, ...)                                                                     \
     do {                                                                                                  \
@@ -2451,20 +2514,20 @@ hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s)
 
 hipError_t pt_launch_canvas(const PtArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_canvas, grid_for(a.lw, a.lh), kBlock, 0, s, a);
+    hipLaunchKernelGGL(k_canvas, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_sync_canvas(const PtArgs &a, int dw, int dh, hipStream_t s)
 {
     int gw = dw > a.lw ? dw : a.lw, gh = dh > a.lh ? dh : a.lh;
-    hipLaunchKernelGGL(k_sync_canvas, grid_for(gw, gh), kBlock, 0, s, a, dw, dh);
+    hipLaunchKernelGGL(k_sync_canvas, grid_rows(gw, gh), kRowBlock, 0, s, a, dw, dh);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_copy_random(const PtArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_copy_random, grid_for(a.lw, a.lh), kBlock, 0, s, a);
+    hipLaunchKernelGGL(k_copy_random, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a);
     return hipGetLastError();
 }
 
@@ -2498,6 +2561,11 @@ static void shape_tile(int op, int shape, int &tx, int &ty)
 {
     tx = PT_BX;
     ty = PT_BY;
+    if (op == PT_OP_SYNC_CANVAS) {
+        tx = PT_RX;
+        ty = PT_RY;
+        return;
+    }
     if (op == PT_OP_PROPAGATE) {
         if (shape == PT_SHAPE_UP_EE_A || shape == PT_SHAPE_UP_B) { tx = 16; ty = 8; }
         else if (shape == PT_SHAPE_UP_A || shape == PT_SHAPE_UP_EE_B) tx = ty = 8;
@@ -2676,7 +2744,7 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
 #undef L
         break;
     }
-    case PT_OP_SYNC_CANVAS: hipLaunchKernelGGL(km_sync_canvas, g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap); break;
+    case PT_OP_SYNC_CANVAS: hipLaunchKernelGGL(km_sync_canvas, g, kRowBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap); break;
     case PT_OP_PROPAGATE: {
 #define L_PIX(R) hipLaunchKernelGGL((tk ? km_propagate<R, true> : km_propagate<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? km_propagate_ee<R, true> : km_propagate_ee<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
@@ -2734,33 +2802,38 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
 
 hipError_t pt_launch_upsample(const PtArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_upsample, grid_for(a.lw, a.lh), kBlock, 0, s, a);
+    hipLaunchKernelGGL(k_upsample, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_advect(const PtArgs &a, int forwards, int time_step, int flags, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_advect, grid_for(a.lw, a.lh), kBlock, 0, s, a, forwards, time_step, flags);
+    hipLaunchKernelGGL(k_advect, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, forwards, time_step, flags);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_merge_randomly(const PtArgs &a, int forwards, float weight, int force, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_merge_randomly, grid_for(a.lw, a.lh), kBlock, 0, s, a, forwards, weight, force);
+    hipLaunchKernelGGL(k_merge_randomly, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, forwards, weight, force);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_merge_residual(const PtArgs &a, int forwards, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_merge_residual, grid_for(a.lw, a.lh), kBlock, 0, s, a, forwards);
+    hipLaunchKernelGGL(k_merge_residual, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, forwards);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_downsample(const float *src, int pw, float *dst, int lw, int lh, int ch, int scale_mode,
                                 hipStream_t s)
 {
-    dim3 b(16, 16), g((lw + 15) / 16, (lh + 15) / 16);
-    hipLaunchKernelGGL(k_downsample, g, b, 0, s, src, pw, dst, lw, lh, ch, scale_mode);
+    dim3 b = kRowBlock, g = grid_rows(lw, lh);
+    switch (ch) {
+    case 1: hipLaunchKernelGGL(k_downsample_v<1>, g, b, 0, s, src, pw, dst, lw, lh, scale_mode); break;
+    case 2: hipLaunchKernelGGL(k_downsample_v<2>, g, b, 0, s, src, pw, dst, lw, lh, scale_mode); break;
+    case 4: hipLaunchKernelGGL(k_downsample_v<4>, g, b, 0, s, src, pw, dst, lw, lh, scale_mode); break;
+    default: hipLaunchKernelGGL(k_downsample, g, b, 0, s, src, pw, dst, lw, lh, ch, scale_mode); break;
+    }
     return hipGetLastError();
 }
 

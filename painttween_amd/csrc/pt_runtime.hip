@@ -62,6 +62,15 @@ struct StylePyr {
 /* Kernel classes timed by pt_profile_*; names in kProfNames. */
 enum ProfKind { PK_PROPAGATE = 0, PK_SCATTER, PK_RESIDUAL, PK_CANVAS, PK_UPSAMPLE, PK_ADVECT, PK_MERGE, PK_RANDOM,
                 PK_COPY, PK_PYRAMID, PK_COUNT };
+/* Algorithmic HBM bytes per pixel of each profiled launch class (every
+ * operand read or written once; the exemplar counted once per pixel): the
+ * NNF searches 96 (offsets 16+16+8, residual 4, canvas 16, target 16,
+ * orientation 4, distance transform 4, exemplar amortised), the residual
+ * refresh 60, random init 40 (random texel 16, offsets 24), upsample 40,
+ * advection 152, merges 64; the fused sync/canvas pass and the pyramid are
+ * counted per launch (sync_buffers, pyramid_rebuild). */
+enum : uint64_t { PB_NNF = 96, PB_RESIDUAL = 60, PB_RANDOM = 40, PB_UPSAMPLE = 40, PB_ADVECT = 152, PB_MERGE = 64 };
+
 static const char *kProfNames[PK_COUNT] = {"propagate", "scatter", "update_residual", "canvas", "upsample",
                                            "advect", "merge", "random", "copy", "pyramid"};
 
@@ -69,6 +78,7 @@ struct ProfRec {
     int kind;
     int ev0, ev1;
     uint64_t pixels;
+    uint64_t bytes; /* algorithmic HBM bytes of the launch */
 };
 
 struct Snapshot {
@@ -425,12 +435,13 @@ static int prof_event(pt_ctx *c)
 }
 
 /* Records the start of a timed launch; returns the record index or -1. */
-static int prof_begin(pt_ctx *c, int kind, uint64_t pixels)
+static int prof_begin(pt_ctx *c, int kind, uint64_t pixels, uint64_t bytes)
 {
     if (!c->prof_on || c->rec) return -1;
     ProfRec r;
     r.kind = kind;
     r.pixels = pixels;
+    r.bytes = bytes;
     r.ev0 = prof_event(c);
     r.ev1 = prof_event(c);
     if (r.ev0 < 0 || r.ev1 < 0) return -1;
@@ -482,6 +493,46 @@ extern "C" int pt_profile_get(pt_ctx *c, const char *kernel, uint64_t *launches,
         *total_ms += ms;
         *pixels += r.pixels;
     }
+    return PT_OK;
+}
+
+extern "C" int pt_profile_bytes(pt_ctx *c, const char *kernel, uint64_t *bytes)
+{
+    if (!c || !kernel || !bytes) return set_err(c, PT_ERR_INVALID_ARG, "NULL argument");
+    int kind = -1;
+    for (int k = 0; k < PK_COUNT; k++)
+        if (strcmp(kernel, kProfNames[k]) == 0) kind = k;
+    if (kind < 0) return set_err(c, PT_ERR_INVALID_ARG, "unknown kernel class '%s'", kernel);
+    *bytes = 0;
+    for (const auto &r : c->prof_recs)
+        if (r.kind == kind) *bytes += r.bytes;
+    return PT_OK;
+}
+
+extern "C" int pt_profile_records(pt_ctx *c, const char *kernel, int max_records, uint64_t *pixels, uint64_t *bytes,
+                                  float *ms, int *count)
+{
+    if (!c || !kernel || !count || max_records < 0 || (max_records > 0 && (!pixels || !bytes || !ms)))
+        return set_err(c, PT_ERR_INVALID_ARG, "bad profile record arguments");
+    int kind = -1;
+    for (int k = 0; k < PK_COUNT; k++)
+        if (strcmp(kernel, kProfNames[k]) == 0) kind = k;
+    if (kind < 0) return set_err(c, PT_ERR_INVALID_ARG, "unknown kernel class '%s'", kernel);
+    HIP_CHECK(c, hipSetDevice(c->device));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    int n = 0;
+    for (const auto &r : c->prof_recs) {
+        if (r.kind != kind) continue;
+        if (n < max_records) {
+            float t = 0.f;
+            HIP_CHECK(c, hipEventElapsedTime(&t, c->prof_events[r.ev0], c->prof_events[r.ev1]));
+            pixels[n] = r.pixels;
+            bytes[n] = r.bytes;
+            ms[n] = t;
+        }
+        n++;
+    }
+    *count = n;
     return PT_OK;
 }
 
@@ -665,9 +716,12 @@ static int pyramid_rebuild(pt_ctx *c, pt_pyramid *p, int scale_mode)
         const float *src = p->data + p->off[l - 1];
         float *dst = p->data + p->off[l];
         int pw = p->lw[l - 1], lw = p->lw[l], lh = p->lh[l];
+        /* a 2x2 block read and one element written per output pixel */
+        int pr = prof_begin(c, PK_PYRAMID, (uint64_t)lw * lh, (uint64_t)lw * lh * ch * 4 * 5);
         RC_CHECK(issue(c, "downsample", [=](hipStream_t s) {
             return pt_launch_downsample(src, pw, dst, lw, lh, ch, scale_mode, s);
         }));
+        prof_end(c, pr);
     }
     return PT_OK;
 }
@@ -1040,7 +1094,7 @@ static int update_residual(pt_ctx *c, int level, bool allow_incremental = false)
     PtArgs a = make_args(c, level);
     a.res_incremental = (allow_incremental && c->incremental_residual && c->res_level == level) ? 1 : 0;
     c->res_level = level;
-    int pr = prof_begin(c, PK_RESIDUAL, (uint64_t)a.lw * a.lh);
+    int pr = prof_begin(c, PK_RESIDUAL, (uint64_t)a.lw * a.lh, PB_RESIDUAL * a.lw * a.lh);
     RC_CHECK(issue_op(c, PT_OP_RESIDUAL, a));
     prof_end(c, pr);
     c->res_diff.grow(a.lw, a.lh);
@@ -1052,7 +1106,6 @@ static int update_residual(pt_ctx *c, int level, bool allow_incremental = false)
 static int sync_buffers(pt_ctx *c, int level, int flags)
 {
     PtArgs a = make_args(c, level);
-    int pr = prof_begin(c, PK_CANVAS, (uint64_t)a.lw * a.lh);
     if (!(flags & PT_DONT_COPY_OFFSETS) && !c->off_diff.empty()) {
         /* copy of the differing rectangle fused with the canvas pass */
         a.dw = c->off_diff.w;
@@ -1061,6 +1114,13 @@ static int sync_buffers(pt_ctx *c, int level, int flags)
     } else {
         a.dw = a.dh = 0; /* canvas only */
     }
+    /* copied pixels: working offsets read, base offsets read (change
+     * flags) and written, 24 B each; level pixels: canvas written, one
+     * exemplar texel, the change flag, and the base offset read where it
+     * was not copied */
+    uint64_t lvl = (uint64_t)a.lw * a.lh, cp = (uint64_t)a.dw * a.dh;
+    uint64_t cpl = (uint64_t)std::min(a.dw, a.lw) * std::min(a.dh, a.lh);
+    int pr = prof_begin(c, PK_CANVAS, lvl, cp * 72 + lvl * 33 + (lvl - cpl) * 16);
     RC_CHECK(issue_op(c, PT_OP_SYNC_CANVAS, a));
     prof_end(c, pr);
     if (!(flags & PT_DONT_UPDATE_RESIDUAL)) RC_CHECK(update_residual(c, level, true));
@@ -1073,7 +1133,7 @@ static int randomize_offsets(pt_ctx *c, int level)
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
     res_invalidate(c);
     PtArgs a = make_args(c, level);
-    int pr = prof_begin(c, PK_RANDOM, (uint64_t)a.lw * a.lh);
+    int pr = prof_begin(c, PK_RANDOM, (uint64_t)a.lw * a.lh, PB_RANDOM * a.lw * a.lh);
     RC_CHECK(issue(c, "random copy", [=](hipStream_t s) { return pt_launch_copy_random(a, s); }));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
@@ -1084,7 +1144,7 @@ static int propagate(pt_ctx *c, int level, int jump)
 {
     PtArgs a = make_args(c, level);
     a.jump = jump;
-    int pr = prof_begin(c, PK_PROPAGATE, (uint64_t)a.lw * a.lh);
+    int pr = prof_begin(c, PK_PROPAGATE, (uint64_t)a.lw * a.lh, PB_NNF * a.lw * a.lh);
     RC_CHECK(issue_op(c, PT_OP_PROPAGATE, a));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
@@ -1108,7 +1168,7 @@ static int scatter(pt_ctx *c, int level)
     PtArgs a = make_args(c, level);
     a.call = c->scatter_calls++;
     a.nscat = scatter_samples(c->ew, c->eh, a.samples);
-    int pr = prof_begin(c, PK_SCATTER, (uint64_t)a.lw * a.lh);
+    int pr = prof_begin(c, PK_SCATTER, (uint64_t)a.lw * a.lh, PB_NNF * a.lw * a.lh);
     RC_CHECK(issue_op(c, PT_OP_SCATTER, a));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
@@ -1120,7 +1180,7 @@ static int upsample(pt_ctx *c, int fine_level)
 {
     res_invalidate(c);
     PtArgs a = make_args(c, fine_level);
-    int pr = prof_begin(c, PK_UPSAMPLE, (uint64_t)a.lw * a.lh);
+    int pr = prof_begin(c, PK_UPSAMPLE, (uint64_t)a.lw * a.lh, PB_UPSAMPLE * a.lw * a.lh);
     RC_CHECK(issue(c, "upsample", [=](hipStream_t s) { return pt_launch_upsample(a, s); }));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
@@ -1144,7 +1204,7 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
     res_invalidate(c);
     PtArgs a = make_args(c, level);
-    int pr = prof_begin(c, PK_ADVECT, (uint64_t)a.lw * a.lh);
+    int pr = prof_begin(c, PK_ADVECT, (uint64_t)a.lw * a.lh, PB_ADVECT * a.lw * a.lh);
     RC_CHECK(issue(c, "advect", [=](hipStream_t s) { return pt_launch_advect(a, forwards, time_step, flags, s); }));
     prof_end(c, pr);
     c->adv_written[forwards ? 0 : 1] = true;
@@ -1173,7 +1233,9 @@ static int merge_randomly(pt_ctx *c, int level, int forwards, float w, int force
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
     res_invalidate(c);
     PtArgs a = make_args(c, level);
+    int pr = prof_begin(c, PK_MERGE, (uint64_t)a.lw * a.lh, PB_MERGE * a.lw * a.lh);
     RC_CHECK(issue(c, "merge", [=](hipStream_t s) { return pt_launch_merge_randomly(a, forwards, w, force, s); }));
+    prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
     return sync_buffers(c, level, 0);
 }
@@ -1182,7 +1244,9 @@ static int merge_residual(pt_ctx *c, int level, int forwards)
 {
     res_invalidate(c);
     PtArgs a = make_args(c, level);
+    int pr = prof_begin(c, PK_MERGE, (uint64_t)a.lw * a.lh, PB_MERGE * a.lw * a.lh);
     RC_CHECK(issue(c, "merge", [=](hipStream_t s) { return pt_launch_merge_residual(a, forwards, s); }));
+    prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
     return sync_buffers(c, level, 0);
 }
