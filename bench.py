@@ -113,6 +113,20 @@ def _gen_seq(arg):
 BANDWIDTH_KINDS = ("canvas", "upsample", "random", "pyramid", "advect", "merge")
 
 
+def host_cpu():
+    """The host's logical CPU count and model name (reported beside the CPU
+    timings)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"logical_cpus": os.cpu_count(), "model": model}
+
+
 def profile_kinds(ctx, kinds):
     """HIP-event totals and algorithmic bytes of the recorded launches of
     each kernel class; achieved GB/s and the fraction of the HBM peak."""
@@ -192,7 +206,7 @@ def run_still256(ctx, inputs, cpu_runs, cpu_threads):
                     ref = oc.synthesize(oc.default_params(**kw), [frame], style, iterations=iterations)
                     tt.append(time.perf_counter() - t0)
             cpu[th] = float(np.median(tt))
-        res.update({"cpu_threads": cpu_threads, "cpu_s": cpu[cpu_threads], "cpu_1thread_s": cpu[1],
+        res.update({"cpu_threads": cpu_threads, "host": host_cpu(), "cpu_s": cpu[cpu_threads], "cpu_1thread_s": cpu[1],
                     "cpu_frames_per_s": 1.0 / cpu[cpu_threads], "cpu_1thread_frames_per_s": 1.0 / cpu[1],
                     "cpu_runs": cpu_runs, "gpu_vs_cpu": cpu[cpu_threads] / (gpu_ms * 1e-3),
                     "offsets_identical": bool(ref[0].tobytes() == gpu_off.tobytes())})
@@ -560,6 +574,7 @@ def main():
         dt, cpdes = cpu_baseline(style, frame, pk, iterations, args.cpu_levels, threads)
         pde_per_frame = pdes / args.steps
         cpu = {"value": (cpdes / dt) / pde_per_frame, "unit": "frames/s", "cores": threads, "kind": "port",
+               "host": host_cpu(),
                "pde_per_s": cpdes / dt,
                "sample": "oracle (CPU restatement), coarsest %d levels of the same 1024^2 schedule: %d PDEs in "
                          "%.1f s, frames/s extrapolated by the %d PDEs of a full frame"
