@@ -124,6 +124,13 @@ static int fail(ptref_ctx *c, int code, const char *fmt, ...)
 /* ------------------------------------------------------------------ */
 void ptref_sincosf(float x, float *s, float *c) { pt_sincosf(x, s, c); }
 float ptref_atan2f(float y, float x) { return pt_atan2f(y, x); }
+float ptref_logf(float x) { return pt_logf(x); }
+float ptref_acosf(float x) { return pt_acosf(x); }
+float ptref_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
+                           float ratio_range, float angle_range)
+{
+    return pt_coherence_term(cx, cy, tcx, tcy, tx, ty, i, j, level, ratio_range, angle_range);
+}
 float ptref_tensor_to_angle(float xx, float xy, float yy) { return pt_tensor_to_angle(xx, xy, yy); }
 void ptref_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
 {
@@ -410,6 +417,13 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
                 }
             } else {
                 res = fmaf(ssd_alpha(canvas, to, p->alpha_weight), p->canvas_output_weight, res);
+                /* coherence term (texSynth_kernel.cu:642-646) */
+                if (p->coherence_weight > 0.f) {
+                    const nnf *cn = &c->off_base[oo];
+                    float ct = pt_coherence_term(cn->x, cn->y, t.x, t.y, tx, ty, i, j, level,
+                                                 p->coherence_ratio_range, p->coherence_angle_range);
+                    res = res + ct * p->coherence_weight;
+                }
             }
             float wgt = c->gauss[i + r] * c->gauss[j + r];
             sum_r = fmaf(res, wgt, sum_r);
@@ -531,8 +545,8 @@ static int check_supported(ptref_ctx *c, const pt_params *p)
         return fail(c, PT_ERR_INVALID_ARG, "residual_window_size must be in [1, 15]");
     if (p->num_styles != 1)
         return fail(c, PT_ERR_UNSUPPORTED, "only one style layer is supported");
-    if (p->coherence_weight != 0.f)
-        return fail(c, PT_ERR_UNSUPPORTED, "coherence term is not supported");
+    if (p->coherence_weight > 0.f && !(p->coherence_ratio_range > 1.f && p->coherence_angle_range > 0.f))
+        return fail(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
     if (p->colorspace != 0)
         return fail(c, PT_ERR_UNSUPPORTED, "only the RGB colourspace is supported");
     if (p->lum_shift_enable)

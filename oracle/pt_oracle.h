@@ -83,6 +83,10 @@ int ptref_gaussian_weights(ptref_ctx *ctx, float *out, int n);
 /* Deterministic math entry points for unit tests. */
 void ptref_sincosf(float x, float *s, float *c);
 float ptref_atan2f(float y, float x);
+float ptref_logf(float x);
+float ptref_acosf(float x);
+float ptref_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
+                           float ratio_range, float angle_range);
 float ptref_tensor_to_angle(float xx, float xy, float yy);
 void ptref_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -47,6 +47,7 @@ struct PtArgs {
     float alpha_w, in_w, canvas_w, dt_w, su, sv;
     int cull, transp, orient_en, direction;
     float tdi, tdo, tcw, hyst_w, max_dist, jitter;
+    float coh_w, coh_ratio, coh_angle; /* coherence term: weight (0 = off), ratio and angle ranges */
     int samples, rng_mode;
     uint32_t key0, key1, call;
     int time_active;   /* some advected offset may be valid */

@@ -285,6 +285,19 @@ __device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, const 
     return r;
 }
 
+/* The coherence term of tap (i, j) (texSynth_kernel.cu:642-646): against
+ * the current offset of pixel (x+i, y+j); (tcx, tcy) is the candidate's
+ * centre and (tx, ty) the wrapped tap position. */
+__device__ __forceinline__ float coherence_add(const PtArgs &A, float res, int x, int y, int i, int j, float tcx,
+                                               float tcy, float tx, float ty)
+{
+    if (A.coh_w > 0.f) {
+        float4 n = A.base.xyth[(y + j) * A.W + (x + i)];
+        res = res + pt_coherence_term(n.x, n.y, tcx, tcy, tx, ty, i, j, A.level, A.coh_ratio, A.coh_angle) * A.coh_w;
+    }
+    return res;
+}
+
 /*
  * calculateResidual (texSynth_kernel.cu:537-659): one patch distance of
  * candidate (t, tsc) at pixel (x, y).  Taps run i over x outer, j over y
@@ -364,6 +377,7 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
                 }
             } else {
                 res = fmaf(ssd_alpha(canvas, to, A.alpha_w), A.canvas_w, res);
+                res = coherence_add(A, res, x, y, i, j, t.x, t.y, tx, ty);
             }
             float wgt = A.gauss[i + r] * A.gauss[j + r];
             sum_r = fmaf(res, wgt, sum_r);
@@ -2602,6 +2616,9 @@ void pt_tap_configure(PtArgs &a, int evals)
 int pt_shape_valid(int op, int shape, const PtArgs &a)
 {
     if (shape < 0 || shape >= PT_SHAPE_COUNT) return 0;
+    /* the coherence term is evaluated by the per-pixel in-order kernels only
+     * (calc_residual_impl), which keeps its registers out of the others */
+    if (a.coh_w > 0.f && shape != PT_SHAPE_PIXEL) return 0;
     if (shape == PT_SHAPE_TAP) return tap_pixels(op, a) > 0;
     if (op == PT_OP_RESIDUAL || op == PT_OP_SYNC_CANVAS) return shape == PT_SHAPE_PIXEL;
     if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return 0;
@@ -2622,7 +2639,7 @@ int pt_default_shape(int op, const PtArgs &a)
         if (a.early_exit) return op == PT_OP_PROPAGATE ? PT_SHAPE_UP_EE_B : PT_SHAPE_UP_EE_A;
         return PT_SHAPE_UP_A;
     }
-    return a.early_exit ? PT_SHAPE_PIXEL_EE : PT_SHAPE_PIXEL;
+    return (a.early_exit && pt_shape_valid(op, PT_SHAPE_PIXEL_EE, a)) ? PT_SHAPE_PIXEL_EE : PT_SHAPE_PIXEL;
 }
 
 int pt_shape_tiles(int op, int shape, const PtArgs &a, int *tiles_x)

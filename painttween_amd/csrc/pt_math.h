@@ -22,6 +22,19 @@
  *      and the Taylor series of atan to t^13 on |t| <= 0.268.  The reference
  *      calls atan2f when converting a structure tensor to an angle
  *      (imageIO.h:136-139).
+ *  - pt_logf:
+ *      x = m * 2^e with m in [sqrt(1/2), sqrt(2)), log(m) = 2 atanh(s) with
+ *      s = (m-1)/(m+1), |s| <= 0.1716, by the series 2s (1 + s^2/3 + s^4/5 +
+ *      ... + s^10/11) (truncation below 5e-11 relative), plus e*ln2 with ln2
+ *      split in a 16-bit head and a tail.  The reference calls logf in the
+ *      coherence term (texSynth_kernel.cu:396,400).
+ *  - pt_acosf:
+ *      acos(x) = pi/2 - asin(x) for |x| <= 1/2, 2 asin(sqrt((1-x)/2)) above
+ *      and pi - 2 asin(sqrt((1+x)/2)) below, asin(z) on |z| <= 1/2 by its
+ *      Taylor series to z^25 (truncation below 4e-10 relative); NaN outside
+ *      [-1, 1].  The reference calls acosf in the coherence term
+ *      (texSynth_kernel.cu:417).
+ *  - pt_coherence_term: the coherence energy of one tap (below).
  *  - pt_philox4x32_10:
  *      Philox4x32-10 counter-based generator (J. K. Salmon, M. A. Moraes,
  *      R. O. Dror, D. E. Shaw, "Parallel Random Numbers: As Easy as 1, 2, 3",
@@ -132,6 +145,121 @@ PT_HD float pt_tensor_to_angle(float xx, float xy, float yy)
     if ((double)angle >= 1.57079632679489661923)
         return (float)((double)angle - 3.14159265358979323846);
     return angle;
+}
+
+/* ln 2 = PT_LN2_HI + PT_LN2_LO; the head has 16 significant bits, so
+ * e * PT_LN2_HI is exact for |e| < 256. */
+#define PT_LN2_HI 0.693145751953125f
+#define PT_LN2_LO 1.42860682030941723212e-06f
+#define PT_SQRT1_2_F 0.707106781186547524f
+
+PT_HD float pt_logf(float x)
+{
+    if (!(x >= 0.0f)) return NAN; /* negative or NaN */
+    if (x == 0.0f) return -INFINITY;
+    if (isinf(x)) return x;
+    int e = 0;
+    if (x < 1.17549435e-38f) { /* subnormal: scale by 2^24 (exact) */
+        x *= 16777216.0f;
+        e = -24;
+    }
+    union {
+        float f;
+        uint32_t u;
+    } b;
+    b.f = x;
+    e += (int)((b.u >> 23) & 0xffu) - 127;
+    b.u = (b.u & 0x007fffffu) | 0x3f800000u; /* m in [1, 2) */
+    float m = b.f;
+    if (m >= 1.41421356237309505f) {
+        m *= 0.5f; /* exact */
+        e += 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f); /* m - 1 is exact */
+    float z = s * s;
+    float p = fmaf(z, 9.09090909090909091e-02f, 1.11111111111111111e-01f);
+    p = fmaf(z, p, 1.42857142857142857e-01f);
+    p = fmaf(z, p, 2.00000000000000000e-01f);
+    p = fmaf(z, p, 3.33333333333333333e-01f);
+    float lm = 2.0f * fmaf(s * z, p, s); /* 2 (s + s^3 p) */
+    float ef = (float)e;
+    return ef * PT_LN2_HI + fmaf(ef, PT_LN2_LO, lm);
+}
+
+/* asin(z) for |z| <= 1/2: z + z^3 * P(z^2), P's coefficients the Taylor
+ * series (2k)! / (4^k (k!)^2 (2k+1)), k = 1..12. */
+PT_HD float pt_asin_half(float z)
+{
+    float t = z * z;
+    float p = fmaf(t, 6.44721031188964875e-03f, 7.31252587359884545e-03f);
+    p = fmaf(t, p, 8.39033580961681506e-03f);
+    p = fmaf(t, p, 9.76160952919407840e-03f);
+    p = fmaf(t, p, 1.15518008961397051e-02f);
+    p = fmaf(t, p, 1.39648437500000007e-02f);
+    p = fmaf(t, p, 1.73527644230769239e-02f);
+    p = fmaf(t, p, 2.23721590909090919e-02f);
+    p = fmaf(t, p, 3.03819444444444441e-02f);
+    p = fmaf(t, p, 4.46428571428571438e-02f);
+    p = fmaf(t, p, 7.49999999999999972e-02f);
+    p = fmaf(t, p, 1.66666666666666657e-01f);
+    return fmaf(z * t, p, z);
+}
+
+/* pi/2 = PT_PIO2_HI + PT_PIO2_LO */
+#define PT_PIO2_HI 1.57079637050628662109375f
+#define PT_PIO2_LO (-4.37113882867379e-08f)
+
+PT_HD float pt_acosf(float x)
+{
+    if (!(x >= -1.0f && x <= 1.0f)) return NAN;
+    if (x > 0.5f) {
+        float z = (1.0f - x) * 0.5f; /* exact */
+        return 2.0f * pt_asin_half(sqrtf(z));
+    }
+    if (x < -0.5f) {
+        float z = (1.0f + x) * 0.5f; /* exact */
+        float a = 2.0f * pt_asin_half(sqrtf(z));
+        return (2.0f * PT_PIO2_HI - a) + 2.0f * PT_PIO2_LO;
+    }
+    return (PT_PIO2_HI - pt_asin_half(x)) + PT_PIO2_LO;
+}
+
+/*
+ * coherenceTerm (texSynth_kernel.cu:377-423) for tap (i, j) != (0, 0) of a
+ * candidate centred at (tcx, tcy) whose tap lands at (tx, ty) (after the
+ * toroidal wrap), with (cx, cy) the current offset of pixel (x+i, y+j):
+ * the squared log of the exemplar/image distance ratio (clamped to
+ * [1/ratio_range, ratio_range], normalised by log(ratio_range)^2) and the
+ * angle between the two directions (acos clamped to [-1, 1], as the
+ * reference's clamp does, then relative to angle_range*pi), combined as
+ * 1 - (1 - Eratio)(1 - Eangle).  The reference forms angle_range*pi and the
+ * min/division in double (M_PI is a double) and stores the result in a
+ * float; the same is done here.  Both sides include this one definition.
+ */
+PT_HD float pt_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
+                              float ratio_range, float angle_range)
+{
+    float dx = cx - tcx, dy = cy - tcy;
+    float lvl = (float)(1 << level);
+    float dist_ex = (dx * dx + dy * dy) * lvl;
+    float dist_im = (float)(i * i + j * j) * lvl;
+    float ratio = fmaxf(1.0f / ratio_range, fminf(dist_ex / dist_im, ratio_range));
+    float lr = pt_logf(ratio);
+    float lz = pt_logf(ratio_range);
+    float z = lz * lz;
+    float e_ratio = (lr * lr) / z;
+    float ux = tx - tcx, uy = ty - tcy;
+    float lt = sqrtf(ux * ux + uy * uy);
+    ux = (lt > 1e-4f) ? ux * (1.0f / lt) : 1.0f;
+    uy = (lt > 1e-4f) ? uy * (1.0f / lt) : 0.0f;
+    float lc = sqrtf(dx * dx + dy * dy);
+    float vx = (lc > 1e-4f) ? dx * (1.0f / lc) : 1.0f;
+    float vy = (lc > 1e-4f) ? dy * (1.0f / lc) : 0.0f;
+    float d = ux * vx + uy * vy;
+    float angle = fmaxf(-1.0f, fminf(pt_acosf(d), 1.0f));
+    double lim = (double)angle_range * 3.14159265358979323846;
+    float e_angle = (float)(fmin((double)angle, lim) / lim);
+    return 1.0f - (1.0f - e_ratio) * (1.0f - e_angle);
 }
 
 /* Philox4x32-10. ctr[4], key[2] in; out[4]. */

@@ -34,6 +34,11 @@ def load_oracle():
         lib.ptref_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         lib.ptref_atan2f.restype = C.c_float
         lib.ptref_atan2f.argtypes = [C.c_float, C.c_float]
+        for f in ("ptref_logf", "ptref_acosf"):
+            getattr(lib, f).restype = C.c_float
+            getattr(lib, f).argtypes = [C.c_float]
+        lib.ptref_coherence_term.restype = C.c_float
+        lib.ptref_coherence_term.argtypes = [C.c_float] * 6 + [C.c_int] * 3 + [C.c_float] * 2
         lib.ptref_tensor_to_angle.restype = C.c_float
         lib.ptref_tensor_to_angle.argtypes = [C.c_float, C.c_float, C.c_float]
         lib.ptref_synthesize_next_frame.restype = C.c_int

@@ -139,6 +139,29 @@ def test_animation_operator_sequence(merge, hyst, tap):
     assert_same_states(ga, oa)
 
 
+@pytest.mark.parametrize("early_exit,unit_parallel,tap", [(1, 1, 0), (0, 1, 0), (1, 0, 0), (0, 0, 0), (1, 1, 1024)])
+@pytest.mark.parametrize("r", [2, 3])
+def test_coherence_term_operator_sequence(r, early_exit, unit_parallel, tap):
+    """The coherence term (against the current offsets of the tap's pixel)
+    in every kernel shape, still schedule with orientation."""
+    st = W.style_pair(40, 36, seed=15, guides=True)
+    fr = W.still_frame(44, 34, seed=16, guides=True)
+    ov = dict(residual_window_size=r, levels=3, coherence_weight=0.4, coherence_ratio_range=2.0,
+              coherence_angle_range=0.25, orientation_enabled=1, dist_trans_weight=0.5)
+    ga, oa = _both(st, [fr], ov, _still_ops(3), early_exit, unit_parallel, tap)
+    assert_same_states(ga, oa)
+
+
+@pytest.mark.parametrize("tap", [0, 1024])
+def test_coherence_term_with_temporal_terms(tap):
+    st = W.style_pair(48, 48, seed=17, guides=True)
+    frames = W.animation(40, 32, 3, seed=18)
+    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=0.3,
+              coherence_weight=0.3)
+    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, "merge_random"), tap=tap)
+    assert_same_states(ga, oa)
+
+
 def test_kernel_variants_agree_at_scale():
     """Early rejection, unit-parallel and tap-parallel search on and off, on
     a 256^2 still with guides: offsets, canvas and residual identical."""

@@ -88,3 +88,36 @@ def test_zero_tensor_angle():
     lib = load_oracle()
     got = lib.ptref_tensor_to_angle(0.0, 0.0, 0.0)
     assert got == np.float32(float(np.float32(np.pi / 2)) - np.pi)
+
+
+def _ulps(got, ref):
+    return abs(got - ref) / np.spacing(np.float32(max(abs(ref), 2 ** -126)))
+
+
+def test_log_within_two_ulp_of_libm():
+    lib = load_oracle()
+    xs = np.concatenate([np.geomspace(1e-30, 1e30, 20001), np.linspace(0.25, 4.0, 20001),
+                         [1.0, 2.0, 0.5, np.sqrt(2.0), 1 - 2 ** -24, 1 + 2 ** -23, 1e-40, 3.4e38]]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        got = lib.ptref_logf(float(x))
+        ref = np.log(float(x))
+        worst = max(worst, _ulps(got, ref) if ref != 0 else abs(got) / 2 ** -149)
+    assert worst <= 2.0, worst
+    assert lib.ptref_logf(1.0) == 0.0
+    assert lib.ptref_logf(0.0) == -np.inf
+    assert np.isnan(lib.ptref_logf(-1.0))
+
+
+def test_acos_within_three_ulp_of_libm():
+    lib = load_oracle()
+    xs = np.concatenate([np.linspace(-1.0, 1.0, 40001), 1 - np.geomspace(1e-7, 1e-1, 2001),
+                         -1 + np.geomspace(1e-7, 1e-1, 2001), [0.5, -0.5, 0.0, -0.0]]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        got = lib.ptref_acosf(float(x))
+        ref = np.arccos(float(x))
+        worst = max(worst, _ulps(got, ref))
+    assert worst <= 3.0, worst
+    assert lib.ptref_acosf(1.0) == 0.0
+    assert np.isnan(lib.ptref_acosf(1.0000001)) and np.isnan(lib.ptref_acosf(-1.0000001))

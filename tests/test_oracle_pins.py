@@ -9,7 +9,7 @@ import pytest
 from painttween_amd import _ffi
 from painttween_amd import workloads as W
 from tests.harness import Session
-from tests.oracle_ctx import OracleContext
+from tests.oracle_ctx import OracleContext, load_oracle
 
 E1, E2, E3, E4 = 1, 2, 3, 4
 
@@ -85,9 +85,25 @@ def _bilinear(tex, x, y):
     return ((1 - a) * (1 - b) * t[ya, xa] + a * (1 - b) * t[ya, xb] + (1 - a) * b * t[yb, xa] + a * b * t[yb, xb])
 
 
-def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm):
+def _coh_double(cx, cy, tcx, tcy, tx, ty, i, j, level, rr, ar):
+    """coherenceTerm (texSynth_kernel.cu:377-423) in float64."""
+    dx, dy = cx - tcx, cy - tcy
+    ratio = min(max((dx * dx + dy * dy) / float(i * i + j * j), 1.0 / rr), rr)
+    er = np.log(ratio) ** 2 / np.log(rr) ** 2
+    ux, uy = tx - tcx, ty - tcy
+    lt = np.hypot(ux, uy)
+    ux, uy = (ux / lt, uy / lt) if lt > 1e-4 else (1.0, 0.0)
+    lc = np.hypot(dx, dy)
+    vx, vy = (dx / lc, dy / lc) if lc > 1e-4 else (1.0, 0.0)
+    ang = min(np.arccos(np.clip(ux * vx + uy * vy, -1.0, 1.0)), 1.0)
+    ea = min(ang, ar * np.pi) / (ar * np.pi)
+    return 1.0 - (1.0 - er) * (1.0 - ea)
+
+
+def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, off=None, level=0):
     """Independent float64 statement of calculateResidual
-    (texSynth_kernel.cu:537-659) without temporal terms."""
+    (texSynth_kernel.cu:537-659) without temporal terms; the coherence term
+    (:642-646) against the offset field `off` when its weight is positive."""
     r = prm.residual_window_size
     lh, lw = target.shape[:2]
     hh, hw = exb.shape[:2]
@@ -121,6 +137,11 @@ def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm):
             else:
                 d = canvas[y + j, x + i].astype(np.float64) - to
                 res += (d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + prm.alpha_weight * d[3] ** 2) * prm.canvas_output_weight
+                if prm.coherence_weight > 0:
+                    n = off[y + j, x + i]
+                    res += _coh_double(float(n["x"]), float(n["y"]), float(cand["x"]), float(cand["y"]), tx, ty, i,
+                                       j, level, prm.coherence_ratio_range,
+                                       prm.coherence_angle_range) * prm.coherence_weight
             wgt = float(g[i + r]) * float(g[j + r])
             num += res * wgt
             den += wgt
@@ -157,6 +178,52 @@ def test_patch_distance_matches_float64_brute_force(r, level):
             got = o.patch_distance(x, y, level, cand)
             exp = _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm)
             assert abs(got - exp) <= 3e-6 * max(exp, 1e-3), (x, y, got, exp)
+
+
+@pytest.mark.parametrize("r,level", [(2, 0), (3, 1)])
+def test_patch_distance_with_coherence_matches_float64(r, level):
+    """The coherence term against the float64 statement, random candidates
+    over a random offset field (directions away from the acos edge)."""
+    st = W.style_pair(24, 20, seed=44, guides=True)
+    fr = W.still_frame(22, 18, seed=45, guides=True)
+    rng = np.random.default_rng(46)
+    with OracleContext() as o:
+        prm = o.default_params(residual_window_size=r, levels=2, coherence_weight=0.6, coherence_ratio_range=2.5,
+                               coherence_angle_range=0.3)
+        s = Session(o, st, [fr], prm)
+        s.run([("style", level), ("frame", level, 0), ("randomize",)])
+        off, canvas, _ = s.state()
+        exb, exo = s.sb.download(level), s.so.download(level)
+        exdt = s.sdt.download(level)
+        target = s.frames[0]["color"].download(level)
+        imgdt = s.frames[0]["dt"].download(level)
+        g = o.gaussian_weights(r)
+        lh, lw = target.shape[:2]
+        for x, y in [(0, 0), (lw - 1, lh - 1)] + [(int(a), int(b)) for a, b in
+                                                  zip(rng.integers(0, lw, 10), rng.integers(0, lh, 10))]:
+            cand = np.zeros((), _ffi.PATCHXF_DTYPE)
+            cand["x"] = rng.uniform(2, exb.shape[1] - 2)
+            cand["y"] = rng.uniform(2, exb.shape[0] - 2)
+            cand["theta"] = rng.uniform(-1.5, 1.5)
+            cand["scale_u"] = cand["scale_v"] = 1.0
+            got = o.patch_distance(x, y, level, cand)
+            exp = _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, off=off, level=level)
+            assert abs(got - exp) <= 3e-5 * max(exp, 1e-3), (x, y, got, exp)
+
+
+def test_coherence_term_special_values():
+    """Consistent neighbour (same distance and direction) costs exactly 0; a
+    neighbour at twice the distance (ratio clamped to the range) or in the
+    opposite direction (angle clamped to 1 rad > range*pi) costs exactly 1."""
+    lib = load_oracle()
+    f = lib.ptref_coherence_term
+    assert f(11.0, 10.0, 10.0, 10.0, 11.0, 10.0, 1, 0, 0, 2.0, 0.25) == 0.0
+    assert f(10.0, 11.0, 10.0, 10.0, 10.0, 11.0, 0, 1, 1, 2.0, 0.25) == 0.0
+    assert f(12.0, 10.0, 10.0, 10.0, 11.0, 10.0, 1, 0, 0, 2.0, 0.25) == 1.0
+    assert f(9.0, 10.0, 10.0, 10.0, 11.0, 10.0, 1, 0, 0, 2.0, 0.25) == 1.0
+    # a right angle with ratio 1: Eangle = min(pi/2 -> 1, 0.5*pi) / (0.5*pi)
+    got = f(10.0, 11.0, 10.0, 10.0, 11.0, 10.0, 1, 0, 2, 2.0, 0.5)
+    assert abs(got - 1.0 / (0.5 * np.pi)) < 1e-6, got
 
 
 def _identity_nnf(w, h, dx=0, dy=0, ew=None, eh=None):
