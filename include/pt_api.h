@@ -44,6 +44,9 @@ extern "C" {
 /* ---- SynthesisScheme (types.h:134-140) -------------------------------- */
 #define PT_S_FMBM 0
 #define PT_S_FMBM_RM 1
+#define PT_S_FBMO 2 /* not supported (PT_ERR_UNSUPPORTED) */
+#define PT_S_TCTF 3 /* not supported (PT_ERR_UNSUPPORTED) */
+#define PT_S_IND 4  /* every frame on its own (renderAnimationIND) */
 
 /* ---- TsImageLayer (texSynth_interface.h:52-75), same values ----------- */
 #define PT_LAYER_EXEMPLAR_BASE 0

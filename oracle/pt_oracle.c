@@ -1419,7 +1419,8 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
 {
     if (!c || !params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return fail(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM) return fail(c, PT_ERR_UNSUPPORTED, "scheme");
+    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND)
+        return fail(c, PT_ERR_UNSUPPORTED, "scheme");
     int levels = params->levels;
     int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
     int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
@@ -1445,7 +1446,59 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
     snap_t *snaps = (snap_t *)calloc((size_t)levels * F * passes, sizeof(snap_t));
 #define SNAP(l, f, p) snaps[((size_t)(l) * F + (f)) * passes + (p)]
     pt_params fparams = *params;
-    for (int level = levels - 1; level >= 0 && !rc; level--) {
+    /* renderAnimationIND (synthesisProcessor.cpp:806-839): every frame on its
+     * own, levels inside the frame loop, synthesizeConsecutive with
+     * is_first_frame and forward time for every pass, no layer flips */
+    for (int f = 0; f < F && !rc && scheme == PT_S_IND; f++) {
+        for (int level = levels - 1; level >= 0 && !rc; level--) {
+            rc = ptref_initialize_exemplars(c, SW >> level, SH >> level, 1);
+            if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sb, 0);
+            if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);
+            if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sor, 0);
+            if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sdt, 0);
+            if (!rc) rc = ptref_reset_advected_buffers(c);
+            for (int pass = 0; pass < passes && !rc; pass++) {
+                rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[f].color, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[f].orient, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[f].velb, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[f].velf, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[f].dt, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_SCALE, fp[f].scale, 0);
+                if (rc) break;
+                fparams.direction = PT_FORWARD;
+                fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+                fparams.frame_number = f;
+                rc = ptref_upload_parameters(c, &fparams);
+                if (rc) break;
+                if (pass == 0) {
+                    /* firstPass: random pyramid, then random offsets (coarsest)
+                     * or this frame's pass-0 result of the coarser level */
+                    rc = ptref_create_random_pyramid(c, level, f, fp[f].color, fp[f].orient, so, sor);
+                    if (!rc && level == levels - 1) {
+                        rc = ptref_randomize_offsets(c, level);
+                    } else if (!rc) {
+                        snap_t *s = &SNAP(level + 1, f, 0);
+                        rc = ptref_load_offsets(c, level + 1, s->off);
+                        if (!rc) rc = ptref_upsample(c, level);
+                        if (!rc) rc = ptref_sync_buffers(c, level, 0);
+                    }
+                } else {
+                    /* later passes: advect (from the never-loaded other layer),
+                     * previous pass, merge by residual */
+                    snap_t *s = &SNAP(level, f, pass - 1);
+                    rc = ptref_advect_offsets(c, level, 1, 1, PT_ADVECT_UPDATE_RESIDUAL | PT_ADVECT_IGNORE_RANDOM);
+                    if (!rc) rc = ptref_load_offsets(c, level, s->off);
+                    if (!rc) rc = ptref_load_residual(c, level, s->res);
+                    if (!rc) rc = ptref_sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL);
+                    if (!rc) rc = ptref_merge_using_residual(c, level, 1);
+                }
+                if (!rc) rc = ptref_sync_buffers(c, level, 0);
+                if (!rc) rc = propagate_and_scatter(c, level, iterations);
+                if (!rc) snap_save(c, level, &SNAP(level, f, pass));
+            }
+        }
+    }
+    for (int level = levels - 1; level >= 0 && !rc && scheme != PT_S_IND; level--) {
         rc = ptref_initialize_exemplars(c, SW >> level, SH >> level, 1);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sb, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);

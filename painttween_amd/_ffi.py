@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libpainttween.so")
 
 PT_OK = 0
 PT_FORWARD, PT_BACKWARD, PT_BIDIRECTIONAL = 0, 1, 2
-PT_S_FMBM, PT_S_FMBM_RM = 0, 1
+PT_S_FMBM, PT_S_FMBM_RM, PT_S_FBMO, PT_S_TCTF, PT_S_IND = 0, 1, 2, 3, 4
 PT_RNG_REFERENCE, PT_RNG_COUNTER = 0, 1
 PT_DONT_COPY_OFFSETS = 0x01
 PT_DONT_UPDATE_RESIDUAL = 0x10

@@ -1514,6 +1514,75 @@ static int propagate_and_scatter(pt_ctx *c, int level, int iterations)
     return PT_OK;
 }
 
+/* Style::load (style.cpp:62-140): the exemplar layers of one level */
+static int load_style_level(pt_ctx *c, int level, const StylePyr &sp)
+{
+    RC_CHECK(pt_initialize_exemplars(c, c->SW >> level, c->SH >> level, 1));
+    RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sp.base, 0));
+    RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, sp.out, 0));
+    RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sp.orient, 0));
+    return upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sp.dt, 0);
+}
+
+/* SynthesisFrame::loadImages / loadAnimImages (synthesisFrame.cpp:562-656):
+ * frame f's guides of one level into the current layer */
+static int load_frame_level(pt_ctx *c, int level, const FramePyr &f)
+{
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_COLOR, f.color, 0));
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, f.orient, 0));
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_B, f.velb, 0));
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_F, f.velf, 0));
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, f.dt, 0));
+    return upload_image(c, level, PT_LAYER_INPUT_SCALE, f.scale, 0);
+}
+
+/* renderAnimationIND (synthesisProcessor.cpp:806-839): every frame on its
+ * own, levels coarse to fine inside the frame loop, each pass through
+ * synthesizeConsecutive with is_first_frame and forward time
+ * (synthesisFrame.cpp:214-323).  The image layers are never flipped, so a
+ * later pass advects from the other (never loaded) layer, as in the
+ * reference; the merge is mergeUsingResidual (the scheme is not FMBM_RM). */
+static int run_schedule_ind(pt_ctx *c, const pt_params *params, int iterations, int passes, int F,
+                            const FramePyr *fp, const StylePyr &sp)
+{
+    int levels = params->levels;
+    pt_params fparams = *params;
+    for (int f = 0; f < F; f++) {
+        for (int level = levels - 1; level >= 0; level--) {
+            RC_CHECK(load_style_level(c, level, sp));
+            RC_CHECK(reset_advected(c));
+            for (int pass = 0; pass < passes; pass++) {
+                RC_CHECK(load_frame_level(c, level, fp[f]));
+                fparams.direction = PT_FORWARD;
+                fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+                fparams.frame_number = f;
+                RC_CHECK(pt_upload_parameters(c, &fparams));
+                if (pass == 0) {
+                    RC_CHECK(random_pyramid(c, level, f, fp[f].color, fp[f].orient, sp.out, sp.orient));
+                    if (level == levels - 1) {
+                        RC_CHECK(randomize_offsets(c, level));
+                    } else {
+                        RC_CHECK(snap_load_offsets(c, snap(c, level + 1, f, 0)));
+                        RC_CHECK(upsample(c, level));
+                        RC_CHECK(sync_buffers(c, level, 0));
+                    }
+                } else {
+                    Snapshot &s = snap(c, level, f, pass - 1);
+                    RC_CHECK(advect(c, level, 1, 1, PT_ADVECT_UPDATE_RESIDUAL | PT_ADVECT_IGNORE_RANDOM));
+                    RC_CHECK(snap_load_offsets(c, s));
+                    RC_CHECK(snap_load_residual(c, s));
+                    RC_CHECK(sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL));
+                    RC_CHECK(merge_residual(c, level, 1));
+                }
+                RC_CHECK(sync_buffers(c, level, 0));
+                RC_CHECK(propagate_and_scatter(c, level, iterations));
+                RC_CHECK(snap_save(c, level, snap(c, level, f, pass)));
+            }
+        }
+    }
+    return PT_OK;
+}
+
 static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iterations, int fixup_passes, int F,
                         const FramePyr *fp, const StylePyr &sp)
 {
@@ -1521,6 +1590,7 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
     int passes = fixup_passes + 1;
     RC_CHECK(pt_setup(c, params));
     RC_CHECK(snap_alloc(c, levels, F, passes));
+    if (scheme == PT_S_IND) return run_schedule_ind(c, params, iterations, passes, F, fp, sp);
     pt_params fparams = *params;
     for (int level = levels - 1; level >= 0; level--) {
         /* Style::load (style.cpp:62-140) */
@@ -1819,7 +1889,8 @@ static int synthesize(pt_ctx *c, const pt_params *params, int scheme, int iterat
 {
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM) return set_err(c, PT_ERR_UNSUPPORTED, "scheme");
+    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND)
+        return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
     RC_CHECK(synthesis_alloc(c, params, W, H, F, fixup_passes + 1, SW, SH));
@@ -1850,7 +1921,8 @@ int pt_record_synthesis(pt_ctx *c, std::vector<RecOp> *ops, const pt_params *par
     if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM) return set_err(c, PT_ERR_UNSUPPORTED, "scheme");
+    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND)
+        return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
     RC_CHECK(synthesis_alloc(c, params, W, H, F, fixup_passes + 1, SW, SH));

@@ -204,6 +204,19 @@ def test_synthesize_animation_bit_exact(scheme):
                    o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
 
 
+@pytest.mark.parametrize("fixup", [0, 1])
+def test_synthesize_ind_bit_exact(fixup):
+    """renderAnimationIND: every frame on its own, levels inside frames."""
+    st = W.style_pair(64, 64, seed=65, guides=True)
+    frames = W.animation(48, 40, 3, seed=66)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, hysteresis_weight=0.2)
+        _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, scheme=_ffi.PT_S_IND, iterations=2,
+                                fixup_passes=fixup),
+                   o.synthesize(o.default_params(**kw), frames, st, scheme=_ffi.PT_S_IND, iterations=2,
+                                fixup_passes=fixup))
+
+
 def test_config1_still_256_bit_exact():
     """The CPU reference workload: 256^2 exemplar pair and target, 3 levels,
     5x5 patches, 4 iterations, orientation off, no temporal term."""

@@ -447,3 +447,32 @@ def test_random_pyramid_ranges_and_alpha_rule():
     draw_y = np.where(fy == 0, oy - 1, oy).clip(0)
     opaque = (draw_x >= 5) & (draw_x < 25) & (draw_y >= 10) & (draw_y < 20)
     assert opaque.mean() > 0.99
+
+
+# ---------------------------------------------------------------- IND schedule
+def test_ind_first_frame_is_the_one_frame_schedule():
+    """renderAnimationIND synthesizes frame 0 exactly like a one-frame FMBM_RM
+    run (fresh buffers, random init at the coarsest level, upsampling from
+    the same frame's pass 0); later frames run too and stay finite."""
+    st = W.style_pair(40, 36, seed=61, guides=True)
+    frames = W.animation(36, 32, 3, seed=62)
+    with OracleContext() as o:
+        p = o.default_params(residual_window_size=2, levels=3, temporal_coherence_weight=0.5)
+        ind = o.synthesize(p, frames, st, scheme=_ffi.PT_S_IND, iterations=2, fixup_passes=0)
+        one = o.synthesize(p, frames[:1], st, scheme=_ffi.PT_S_FMBM_RM, iterations=2, fixup_passes=0)
+        ind2 = o.synthesize(p, frames, st, scheme=_ffi.PT_S_IND, iterations=2, fixup_passes=1)
+    for a, b in zip(ind, one):
+        assert a[0].tobytes() == b[0].tobytes()
+    for a in ind2[1:]:
+        assert np.isfinite(a).all()
+    assert (ind2[2] >= 0).all()
+
+
+def test_unsupported_schemes_are_rejected():
+    st = W.style_pair(32, 32, seed=63)
+    fr = W.still_frame(32, 32, seed=64)
+    with OracleContext() as o:
+        p = o.default_params(residual_window_size=2, levels=2)
+        for scheme in (_ffi.PT_S_FBMO, _ffi.PT_S_TCTF):
+            with pytest.raises(Exception):
+                o.synthesize(p, [fr], st, scheme=scheme, iterations=1, fixup_passes=0)
