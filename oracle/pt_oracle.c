@@ -1419,7 +1419,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
 {
     if (!c || !params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return fail(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND)
+    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND && scheme != PT_S_FBMO)
         return fail(c, PT_ERR_UNSUPPORTED, "scheme");
     int levels = params->levels;
     int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
@@ -1442,7 +1442,8 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
     ptref_pyramid_destroy(st);
     frame_pyr *fp = (frame_pyr *)calloc(F, sizeof(frame_pyr));
     for (int f = 0; f < F && !rc; f++) rc = build_frame_pyramids(c, W, H, &frames[f], &fp[f]);
-    int passes = fixup_passes + 1;
+    /* FBMO saves its two directed passes, their merge and the fixups */
+    int passes = scheme == PT_S_FBMO ? fixup_passes + 3 : fixup_passes + 1;
     snap_t *snaps = (snap_t *)calloc((size_t)levels * F * passes, sizeof(snap_t));
 #define SNAP(l, f, p) snaps[((size_t)(l) * F + (f)) * passes + (p)]
     pt_params fparams = *params;
@@ -1498,7 +1499,130 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
             }
         }
     }
-    for (int level = levels - 1; level >= 0 && !rc && scheme != PT_S_IND; level--) {
+    /* renderAnimationFBMO (synthesisProcessor.cpp:902-970) and its fixup
+     * passes (renderInterpolatedFixupPass :771-805, synthesizeInterpolated
+     * and advectNeighbors, synthesisFrame.cpp:325-407) */
+    int *last = (int *)calloc(F, sizeof(int)); /* SynthesisFrame::_last_saved_pass at this level */
+    for (int level = levels - 1; level >= 0 && !rc && scheme == PT_S_FBMO; level--) {
+        int merged = fixup_passes + 2;
+        rc = ptref_initialize_exemplars(c, SW >> level, SH >> level, 1);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sb, 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sor, 0);
+        if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sdt, 0);
+        /* passes 0 and 1: forward, then backward, every frame through
+         * synthesizeConsecutive's first-pass branch */
+        for (int pass = 0; pass < 2 && !rc; pass++) {
+            rc = ptref_reset_advected_buffers(c);
+            int fwd = pass == 0;
+            for (int k = 0; k < F && !rc; k++) {
+                int f = fwd ? k : F - 1 - k;
+                int is_first = k == 0;
+                if (!is_first || pass > 0) ptref_flip_anim_buffers(c);
+                rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[f].color, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[f].orient, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[f].velb, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[f].velf, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[f].dt, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_SCALE, fp[f].scale, 0);
+                if (rc) break;
+                fparams.direction = fwd ? PT_FORWARD : PT_BACKWARD;
+                fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+                fparams.frame_number = f;
+                rc = ptref_upload_parameters(c, &fparams);
+                if (!rc) rc = ptref_create_random_pyramid(c, level, f, fp[f].color, fp[f].orient, so, sor);
+                if (rc) break;
+                if (is_first) {
+                    if (level == levels - 1) {
+                        rc = ptref_randomize_offsets(c, level);
+                    } else {
+                        rc = ptref_load_offsets(c, level + 1, SNAP(level + 1, f, merged).off);
+                        if (!rc) rc = ptref_upsample(c, level);
+                        if (!rc) rc = ptref_sync_buffers(c, level, 0);
+                    }
+                } else if (level == levels - 1) {
+                    rc = ptref_advect_offsets(c, level, 1, fwd, PT_ADVECT_UPDATE_RESIDUAL);
+                    if (!rc) memcpy(c->off_work, c->off_base, sizeof(nnf) * c->W * c->H);
+                } else {
+                    snap_t *s = &SNAP(level + 1, f, merged);
+                    rc = ptref_advect_offsets(c, level, 1, fwd, PT_ADVECT_UPDATE_RESIDUAL | PT_ADVECT_IGNORE_RANDOM);
+                    if (!rc) rc = ptref_load_offsets(c, level + 1, s->off);
+                    if (!rc) rc = ptref_load_residual(c, level + 1, s->res);
+                    if (!rc) rc = ptref_upsample(c, level);
+                    if (!rc) rc = ptref_sync_buffers(c, level, 0);
+                    if (!rc) rc = ptref_merge_using_residual(c, level, fwd);
+                }
+                if (!rc) rc = ptref_sync_buffers(c, level, 0);
+                if (!rc) rc = propagate_and_scatter(c, level, iterations);
+                if (!rc) snap_save(c, level, &SNAP(level, f, pass));
+                last[f] = pass;
+            }
+        }
+        /* pass 2: the forward result becomes the advected-forward offsets and
+         * residual, the backward result the current ones, force-valid merge */
+        for (int f = 0; f < F && !rc; f++) {
+            size_t n = (size_t)c->W * c->H;
+            rc = ptref_load_offsets(c, level, SNAP(level, f, 0).off);
+            if (!rc) rc = ptref_load_residual(c, level, SNAP(level, f, 0).res);
+            if (rc) break;
+            memcpy(c->adv[0].off, c->off_base, sizeof(nnf) * n);
+            memcpy(c->adv[0].res, c->res_base, sizeof(float) * n);
+            rc = ptref_sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL);
+            if (!rc) rc = ptref_load_offsets(c, level, SNAP(level, f, 1).off);
+            if (!rc) rc = ptref_load_residual(c, level, SNAP(level, f, 1).res);
+            if (!rc) rc = ptref_sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL);
+            if (!rc) rc = ptref_merge_randomly(c, level, 1, 0.5f, 1);
+            if (!rc) snap_save(c, level, &SNAP(level, f, 2));
+        }
+        fparams.direction = PT_BIDIRECTIONAL;
+        if (!rc) rc = ptref_upload_parameters(c, &fparams);
+        int temporal = F > 1 && (params->time_derivative_input_weight > 0.f ||
+                                 params->time_derivative_output_weight > 0.f ||
+                                 params->temporal_coherence_weight > 0.f);
+        for (int pass = 3; pass < fixup_passes + 3 && !rc; pass++) {
+            for (int f = 0; f < F && !rc; f++) {
+                rc = ptref_create_random_pyramid(c, level, f, fp[f].color, fp[f].orient, so, sor);
+                fparams.direction = PT_BIDIRECTIONAL;
+                fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+                fparams.frame_number = f;
+                if (!rc) rc = ptref_upload_parameters(c, &fparams);
+                if (!rc) rc = ptref_reset_advected_buffers(c);
+                for (int side = 0; side < 2 && !rc && temporal; side++) {
+                    /* prior frame advected forward, next frame backward */
+                    int g = side == 0 ? f - 1 : f + 1;
+                    if (g < 0 || g >= F) continue;
+                    rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[g].color, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[g].orient, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[g].velb, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[g].velf, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[g].dt, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_SCALE, fp[g].scale, 0);
+                    if (!rc) ptref_flip_anim_buffers(c);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[f].velb, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[f].velf, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[f].dt, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_SCALE, fp[f].scale, 0);
+                    if (!rc) rc = ptref_load_offsets(c, level, SNAP(level, g, last[g]).off);
+                    if (!rc) rc = ptref_advect_offsets(c, level, 1, side == 0, 0);
+                }
+                if (!rc && !temporal) {
+                    rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[f].velb, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[f].velf, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[f].dt, 0);
+                    if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_SCALE, fp[f].scale, 0);
+                }
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[f].color, 0);
+                if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[f].orient, 0);
+                if (!rc) rc = ptref_load_offsets(c, level, SNAP(level, f, pass - 1).off);
+                if (!rc) rc = ptref_sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL);
+                if (!rc) rc = propagate_and_scatter(c, level, iterations);
+                if (!rc) snap_save(c, level, &SNAP(level, f, pass));
+                last[f] = pass;
+            }
+        }
+    }
+    free(last);
+    for (int level = levels - 1; level >= 0 && !rc && scheme != PT_S_IND && scheme != PT_S_FBMO; level--) {
         rc = ptref_initialize_exemplars(c, SW >> level, SH >> level, 1);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sb, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);

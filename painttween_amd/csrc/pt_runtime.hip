@@ -1536,6 +1536,170 @@ static int load_frame_level(pt_ctx *c, int level, const FramePyr &f)
     return upload_image(c, level, PT_LAYER_INPUT_SCALE, f.scale, 0);
 }
 
+/* Saved passes per (level, frame) of a scheme; the last one is the output
+ * (copyOutputToFinal): FBMO saves its two directed passes, their merge and
+ * the fixup passes. */
+static int scheme_passes(int scheme, int fixup_passes)
+{
+    return scheme == PT_S_FBMO ? fixup_passes + 3 : fixup_passes + 1;
+}
+
+/* SynthesisFrame::loadImages (synthesisFrame.cpp:562-578): colour and
+ * orientation of frame f into the current layer */
+static int load_images(pt_ctx *c, int level, const FramePyr &f)
+{
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_COLOR, f.color, 0));
+    return upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, f.orient, 0);
+}
+
+/* SynthesisFrame::loadAnimImages (synthesisFrame.cpp:581-656), time step 1:
+ * velocities, distance transform and scale of frame f */
+static int load_anim_images(pt_ctx *c, int level, const FramePyr &f)
+{
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_B, f.velb, 0));
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_F, f.velf, 0));
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, f.dt, 0));
+    return upload_image(c, level, PT_LAYER_INPUT_SCALE, f.scale, 0);
+}
+
+/* offsets.base().copyTo(advectedF.offsets()) and
+ * residualCache.base().copyTo(advectedF.residual())
+ * (synthesisProcessor.cpp:939-940): whole level-0-sized buffers */
+static int copy_base_to_advected_f(pt_ctx *c)
+{
+    PtNnf base = c->base;
+    DevAdv d = c->adv[0];
+    float *rb = c->res_base;
+    size_t n = (size_t)c->W * c->H;
+    RC_CHECK(issue(c, "copy to advected", [=](hipStream_t st) {
+        hipError_t e = hipMemcpyAsync(d.xyth, base.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d.sc, base.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d.res, rb, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+        return e;
+    }));
+    c->adv_written[0] = true;
+    res_invalidate(c);
+    return PT_OK;
+}
+
+/* renderAnimationFBMO (synthesisProcessor.cpp:902-970) with its fixup passes
+ * (renderInterpolatedFixupPass :771-805, synthesizeInterpolated and
+ * advectNeighbors, synthesisFrame.cpp:325-407).  Passes 0 and 1 are
+ * independent forward and backward consecutive passes (every frame through
+ * synthesizeConsecutive's first-pass branch, coarser levels read at pass
+ * fixup+2); pass 2 merges them per frame (forward result as the advected
+ * offsets, force-valid random merge); passes 3.. re-optimise every frame
+ * with both neighbours advected into it (their last saved pass at this
+ * level: the neighbour before has already run the current pass, the one
+ * after has not). */
+static int run_schedule_fbmo(pt_ctx *c, const pt_params *params, int iterations, int fixup, int F,
+                             const FramePyr *fp, const StylePyr &sp)
+{
+    int levels = params->levels;
+    int merged = fixup + 2;
+    pt_params fparams = *params;
+    std::vector<int> last(F, 0);
+    for (int level = levels - 1; level >= 0; level--) {
+        RC_CHECK(load_style_level(c, level, sp));
+        for (int pass = 0; pass < 2; pass++) {
+            RC_CHECK(reset_advected(c));
+            bool fwd = pass == 0;
+            for (int k = 0; k < F; k++) {
+                int f = fwd ? k : F - 1 - k;
+                bool is_first = k == 0;
+                if (!is_first || pass > 0) RC_CHECK(pt_flip_anim_buffers(c));
+                RC_CHECK(load_images(c, level, fp[f]));
+                RC_CHECK(load_anim_images(c, level, fp[f]));
+                /* synthesizeConsecutive(level, pass, 1, is_first, fwd, fixup+2, firstPass=true) */
+                fparams.direction = fwd ? PT_FORWARD : PT_BACKWARD;
+                fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+                fparams.frame_number = f;
+                RC_CHECK(pt_upload_parameters(c, &fparams));
+                RC_CHECK(random_pyramid(c, level, f, fp[f].color, fp[f].orient, sp.out, sp.orient));
+                if (is_first) {
+                    if (level == levels - 1) {
+                        RC_CHECK(randomize_offsets(c, level));
+                    } else {
+                        RC_CHECK(snap_load_offsets(c, snap(c, level + 1, f, merged)));
+                        RC_CHECK(upsample(c, level));
+                        RC_CHECK(sync_buffers(c, level, 0));
+                    }
+                } else if (level == levels - 1) {
+                    RC_CHECK(advect(c, level, 1, fwd, PT_ADVECT_UPDATE_RESIDUAL));
+                    RC_CHECK(copy_base_to_working(c));
+                } else {
+                    Snapshot &s = snap(c, level + 1, f, merged);
+                    RC_CHECK(advect(c, level, 1, fwd, PT_ADVECT_UPDATE_RESIDUAL | PT_ADVECT_IGNORE_RANDOM));
+                    RC_CHECK(snap_load_offsets(c, s));
+                    RC_CHECK(snap_load_residual(c, s));
+                    RC_CHECK(upsample(c, level));
+                    RC_CHECK(sync_buffers(c, level, 0));
+                    RC_CHECK(merge_residual(c, level, fwd));
+                }
+                RC_CHECK(sync_buffers(c, level, 0));
+                RC_CHECK(propagate_and_scatter(c, level, iterations));
+                RC_CHECK(snap_save(c, level, snap(c, level, f, pass)));
+                last[f] = pass;
+            }
+        }
+        /* pass 2: merge the forward and backward passes */
+        for (int f = 0; f < F; f++) {
+            RC_CHECK(snap_load_offsets(c, snap(c, level, f, 0)));
+            RC_CHECK(snap_load_residual(c, snap(c, level, f, 0)));
+            RC_CHECK(copy_base_to_advected_f(c));
+            RC_CHECK(sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL));
+            RC_CHECK(snap_load_offsets(c, snap(c, level, f, 1)));
+            RC_CHECK(snap_load_residual(c, snap(c, level, f, 1)));
+            RC_CHECK(sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL));
+            RC_CHECK(merge_randomly(c, level, 1, 0.5f, 1));
+            RC_CHECK(snap_save(c, level, snap(c, level, f, 2)));
+        }
+        fparams.direction = PT_BIDIRECTIONAL;
+        RC_CHECK(pt_upload_parameters(c, &fparams));
+        const bool temporal = F > 1 && (params->time_derivative_input_weight > 0.f ||
+                                        params->time_derivative_output_weight > 0.f ||
+                                        params->temporal_coherence_weight > 0.f);
+        for (int pass = 3; pass < fixup + 3; pass++) {
+            for (int f = 0; f < F; f++) {
+                RC_CHECK(random_pyramid(c, level, f, fp[f].color, fp[f].orient, sp.out, sp.orient));
+                /* synthesizeInterpolated(level, pass, pass-1, 1, prior, next) */
+                fparams.direction = PT_BIDIRECTIONAL;
+                fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+                fparams.frame_number = f;
+                RC_CHECK(pt_upload_parameters(c, &fparams));
+                RC_CHECK(reset_advected(c));
+                if (temporal) {
+                    if (f > 0) {
+                        RC_CHECK(load_images(c, level, fp[f - 1]));
+                        RC_CHECK(load_anim_images(c, level, fp[f - 1]));
+                        RC_CHECK(pt_flip_anim_buffers(c));
+                        RC_CHECK(load_anim_images(c, level, fp[f]));
+                        RC_CHECK(snap_load_offsets(c, snap(c, level, f - 1, last[f - 1])));
+                        RC_CHECK(advect(c, level, 1, 1, 0));
+                    }
+                    if (f < F - 1) {
+                        RC_CHECK(load_images(c, level, fp[f + 1]));
+                        RC_CHECK(load_anim_images(c, level, fp[f + 1]));
+                        RC_CHECK(pt_flip_anim_buffers(c));
+                        RC_CHECK(load_anim_images(c, level, fp[f]));
+                        RC_CHECK(snap_load_offsets(c, snap(c, level, f + 1, last[f + 1])));
+                        RC_CHECK(advect(c, level, 1, 0, 0));
+                    }
+                } else {
+                    RC_CHECK(load_anim_images(c, level, fp[f]));
+                }
+                RC_CHECK(load_images(c, level, fp[f]));
+                RC_CHECK(snap_load_offsets(c, snap(c, level, f, pass - 1)));
+                RC_CHECK(sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL));
+                RC_CHECK(propagate_and_scatter(c, level, iterations));
+                RC_CHECK(snap_save(c, level, snap(c, level, f, pass)));
+                last[f] = pass;
+            }
+        }
+    }
+    return PT_OK;
+}
+
 /* renderAnimationIND (synthesisProcessor.cpp:806-839): every frame on its
  * own, levels coarse to fine inside the frame loop, each pass through
  * synthesizeConsecutive with is_first_frame and forward time
@@ -1587,10 +1751,11 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
                         const FramePyr *fp, const StylePyr &sp)
 {
     int levels = params->levels;
-    int passes = fixup_passes + 1;
+    int passes = scheme_passes(scheme, fixup_passes);
     RC_CHECK(pt_setup(c, params));
     RC_CHECK(snap_alloc(c, levels, F, passes));
     if (scheme == PT_S_IND) return run_schedule_ind(c, params, iterations, passes, F, fp, sp);
+    if (scheme == PT_S_FBMO) return run_schedule_fbmo(c, params, iterations, fixup_passes, F, fp, sp);
     pt_params fparams = *params;
     for (int level = levels - 1; level >= 0; level--) {
         /* Style::load (style.cpp:62-140) */
@@ -1879,7 +2044,8 @@ static int synthesize_issue(pt_ctx *c, const pt_params *params, int scheme, int 
     c->frame_pyrs.resize(F);
     for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], guides_on_device, c->frame_pyrs[f]));
     RC_CHECK(run_schedule(c, params, scheme, iterations, fixup_passes, F, c->frame_pyrs.data(), c->style_pyr));
-    return read_results(c, F, fixup_passes, out_off, out_canvas, out_res, outputs_on_device);
+    return read_results(c, F, scheme_passes(scheme, fixup_passes) - 1, out_off, out_canvas, out_res,
+                        outputs_on_device);
 }
 
 static int synthesize(pt_ctx *c, const pt_params *params, int scheme, int iterations, int fixup_passes, int W, int H,
@@ -1889,11 +2055,11 @@ static int synthesize(pt_ctx *c, const pt_params *params, int scheme, int iterat
 {
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND)
+    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND && scheme != PT_S_FBMO)
         return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
-    RC_CHECK(synthesis_alloc(c, params, W, H, F, fixup_passes + 1, SW, SH));
+    RC_CHECK(synthesis_alloc(c, params, W, H, F, scheme_passes(scheme, fixup_passes), SW, SH));
     RC_CHECK(synthesize_issue(c, params, scheme, iterations, fixup_passes, W, H, F, frames, SW, SH, style,
                               guides_on_device, out_off, out_canvas, out_res, outputs_on_device));
     return sync_stream(c);
@@ -1921,11 +2087,11 @@ int pt_record_synthesis(pt_ctx *c, std::vector<RecOp> *ops, const pt_params *par
     if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND)
+    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND && scheme != PT_S_FBMO)
         return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
-    RC_CHECK(synthesis_alloc(c, params, W, H, F, fixup_passes + 1, SW, SH));
+    RC_CHECK(synthesis_alloc(c, params, W, H, F, scheme_passes(scheme, fixup_passes), SW, SH));
     /* the frame and style pyramids are allocated here too, by a dry pass of
      * their preparation with recording into a scratch list */
     {

@@ -468,11 +468,27 @@ def test_ind_first_frame_is_the_one_frame_schedule():
     assert (ind2[2] >= 0).all()
 
 
+def test_fbmo_single_level_single_frame_is_the_one_frame_schedule():
+    """renderAnimationFBMO on one frame and one level: the forward and the
+    backward pass run the same first-frame synthesis, and the force-valid
+    merge of two equal fields returns it, so the output offsets are the
+    one-frame FMBM_RM result.  (With more levels the two directed passes of
+    a finer level differ through the upsample's untouched one-pixel ring,
+    which holds what the previous pass left there.)"""
+    st = W.style_pair(40, 36, seed=61, guides=True)
+    frames = W.animation(36, 32, 1, seed=62)
+    with OracleContext() as o:
+        p = o.default_params(residual_window_size=2, levels=1, temporal_coherence_weight=0.5)
+        a = o.synthesize(p, frames, st, scheme=_ffi.PT_S_FBMO, iterations=2, fixup_passes=0)
+        b = o.synthesize(p, frames, st, scheme=_ffi.PT_S_FMBM_RM, iterations=2, fixup_passes=0)
+    assert a[0].tobytes() == b[0].tobytes()
+
+
 def test_unsupported_schemes_are_rejected():
     st = W.style_pair(32, 32, seed=63)
     fr = W.still_frame(32, 32, seed=64)
     with OracleContext() as o:
         p = o.default_params(residual_window_size=2, levels=2)
-        for scheme in (_ffi.PT_S_FBMO, _ffi.PT_S_TCTF):
+        for scheme in (_ffi.PT_S_TCTF,):
             with pytest.raises(Exception):
                 o.synthesize(p, [fr], st, scheme=scheme, iterations=1, fixup_passes=0)
