@@ -45,7 +45,7 @@ extern "C" {
 #define PT_S_FMBM 0
 #define PT_S_FMBM_RM 1
 #define PT_S_FBMO 2 /* forward and backward passes, merge, fixups (renderAnimationFBMO) */
-#define PT_S_TCTF 3 /* not supported (PT_ERR_UNSUPPORTED) */
+#define PT_S_TCTF 3 /* consecutive pass, then refine passes coarse to fine (renderAnimationTCTF) */
 #define PT_S_IND 4  /* every frame on its own (renderAnimationIND) */
 
 /* ---- TsImageLayer (texSynth_interface.h:52-75), same values ----------- */

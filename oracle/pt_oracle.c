@@ -1419,8 +1419,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
 {
     if (!c || !params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return fail(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND && scheme != PT_S_FBMO)
-        return fail(c, PT_ERR_UNSUPPORTED, "scheme");
+    if (scheme < PT_S_FMBM || scheme > PT_S_IND) return fail(c, PT_ERR_UNSUPPORTED, "scheme");
     int levels = params->levels;
     int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
     int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
@@ -1442,8 +1441,12 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
     ptref_pyramid_destroy(st);
     frame_pyr *fp = (frame_pyr *)calloc(F, sizeof(frame_pyr));
     for (int f = 0; f < F && !rc; f++) rc = build_frame_pyramids(c, W, H, &frames[f], &fp[f]);
-    /* FBMO saves its two directed passes, their merge and the fixups */
-    int passes = scheme == PT_S_FBMO ? fixup_passes + 3 : fixup_passes + 1;
+    /* FBMO saves its two directed passes, their merge and the fixups; TCTF a
+     * consecutive pass and, per level, one refine pass (+ fixups + 1 from
+     * three frames), numbered on across levels */
+    int passes = scheme == PT_S_FBMO ? fixup_passes + 3
+                 : scheme == PT_S_TCTF ? 1 + levels * (1 + (F > 2 ? fixup_passes + 1 : 0))
+                                       : fixup_passes + 1;
     snap_t *snaps = (snap_t *)calloc((size_t)levels * F * passes, sizeof(snap_t));
 #define SNAP(l, f, p) snaps[((size_t)(l) * F + (f)) * passes + (p)]
     pt_params fparams = *params;
@@ -1622,7 +1625,119 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
         }
     }
     free(last);
-    for (int level = levels - 1; level >= 0 && !rc && scheme != PT_S_IND && scheme != PT_S_FBMO; level--) {
+    /* renderAnimationTCTF (synthesisProcessor.cpp:1151-1312), direct
+     * execution, no keyframes or ribbons (largest time step 1): a forward
+     * consecutive pass on the coarsest level, then per level coarse to fine
+     * refine passes over all frames (refine, synthesisFrame.cpp:409-457) */
+    if (scheme == PT_S_TCTF && !rc) {
+        int L = levels - 1;
+        int *lastp = (int *)calloc((size_t)F * levels, sizeof(int)); /* [f * levels + level] */
+        int temporal = F > 1 && (params->time_derivative_input_weight > 0.f ||
+                                 params->time_derivative_output_weight > 0.f ||
+                                 params->temporal_coherence_weight > 0.f);
+        int passes_this = 0, passes_last = 0, last_level = -1;
+        rc = ptref_reset_advected_buffers(c);
+#define ENTER_LEVEL(l)                                                                               \
+        do {                                                                                         \
+            if ((l) != last_level && !rc) {                                                          \
+                rc = ptref_initialize_exemplars(c, SW >> (l), SH >> (l), 1);                          \
+                if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_EXEMPLAR_BASE, sb, 0);              \
+                if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_EXEMPLAR_OUTPUT, so, 0);            \
+                if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_EXEMPLAR_ORIENTATION, sor, 0);      \
+                if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_EXEMPLAR_DIST_TRANS, sdt, 0);       \
+                passes_last = passes_this;                                                           \
+                last_level = (l);                                                                    \
+            }                                                                                        \
+        } while (0)
+#define LOAD_IMAGES(l, g)                                                                            \
+        do {                                                                                         \
+            if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_COLOR, fp[g].color, 0);           \
+            if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_ORIENTATION, fp[g].orient, 0);    \
+        } while (0)
+#define LOAD_ANIM(l, g)                                                                              \
+        do {                                                                                         \
+            if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_VEL_B, fp[g].velb, 0);            \
+            if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_VEL_F, fp[g].velf, 0);            \
+            if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_DIST_TRANS, fp[g].dt, 0);         \
+            if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_SCALE, fp[g].scale, 0);           \
+        } while (0)
+        /* a pass that was never saved loads nothing (missing temporary file) */
+#define LOAD_IF_SAVED(l, g, p)                                                                       \
+        do {                                                                                         \
+            if (!rc && SNAP((l), (g), (p)).off) rc = ptref_load_offsets(c, (l), SNAP((l), (g), (p)).off); \
+        } while (0)
+        ENTER_LEVEL(L);
+        for (int f = 0; f < F && !rc; f++) {
+            int is_first = f == 0;
+            if (!is_first) ptref_flip_anim_buffers(c);
+            LOAD_IMAGES(L, f);
+            LOAD_ANIM(L, f);
+            fparams.direction = PT_FORWARD;
+            fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+            fparams.frame_number = f;
+            if (!rc) rc = ptref_upload_parameters(c, &fparams);
+            if (!rc) rc = ptref_create_random_pyramid(c, L, f, fp[f].color, fp[f].orient, so, sor);
+            if (!rc && is_first) {
+                rc = ptref_randomize_offsets(c, L);
+            } else if (!rc) {
+                rc = ptref_advect_offsets(c, L, 1, 1, PT_ADVECT_UPDATE_RESIDUAL);
+                if (!rc) memcpy(c->off_work, c->off_base, sizeof(nnf) * c->W * c->H);
+            }
+            if (!rc) rc = ptref_sync_buffers(c, L, 0);
+            if (!rc) rc = propagate_and_scatter(c, L, iterations);
+            if (!rc) snap_save(c, L, &SNAP(L, f, 0));
+            lastp[f * levels + L] = 0;
+        }
+        passes_this++;
+        for (int level = L; level >= 0 && !rc; level--) {
+            int count = 1 + (F > 2 ? fixup_passes + 1 : 0);
+            for (int r = 0; r < count && !rc; r++) {
+                ENTER_LEVEL(level);
+                int this_pass = passes_this, last_coarse = passes_last - 1;
+                int spatial = last_coarse == this_pass - 1;
+                for (int f = 0; f < F && !rc; f++) {
+                    fparams.direction = PT_BIDIRECTIONAL;
+                    fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+                    fparams.frame_number = f;
+                    rc = ptref_upload_parameters(c, &fparams);
+                    if (!rc) rc = ptref_reset_advected_buffers(c);
+                    if (temporal && (f > 0 || f < F - 1)) {
+                        for (int side = 0; side < 2 && !rc; side++) {
+                            int g = side == 0 ? f - 1 : f + 1;
+                            if (g < 0 || g >= F || this_pass - 1 < 0) continue;
+                            LOAD_IMAGES(level, g);
+                            LOAD_ANIM(level, g);
+                            if (!rc) ptref_flip_anim_buffers(c);
+                            LOAD_ANIM(level, f);
+                            LOAD_IF_SAVED(level, g, lastp[g * levels + level]);
+                            if (!rc) rc = ptref_advect_offsets(c, level, 1, side == 0, 0);
+                        }
+                    } else {
+                        LOAD_ANIM(level, f);
+                    }
+                    if (spatial) {
+                        LOAD_IF_SAVED(level + 1, f, lastp[f * levels + level + 1]);
+                        if (!rc) rc = ptref_upsample(c, level);
+                    } else {
+                        LOAD_IF_SAVED(level, f, lastp[f * levels + level]);
+                    }
+                    if (!rc) rc = ptref_sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL);
+                    LOAD_IMAGES(level, f);
+                    if (!rc) rc = propagate_and_scatter(c, level, iterations);
+                    if (!rc) snap_save(c, level, &SNAP(level, f, this_pass));
+                    lastp[f * levels + level] = this_pass;
+                }
+                passes_this++;
+            }
+        }
+#undef ENTER_LEVEL
+#undef LOAD_IMAGES
+#undef LOAD_ANIM
+#undef LOAD_IF_SAVED
+        free(lastp);
+    }
+    for (int level = levels - 1; level >= 0 && !rc && scheme != PT_S_IND && scheme != PT_S_FBMO &&
+                                 scheme != PT_S_TCTF; level--) {
         rc = ptref_initialize_exemplars(c, SW >> level, SH >> level, 1);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sb, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);

@@ -87,6 +87,7 @@ struct Snapshot {
     float *res = nullptr;
     float4 *canvas = nullptr;
     int w = 0, h = 0;
+    bool saved = false; /* written in the current run (the reference's temporary file exists) */
 };
 
 } // namespace
@@ -1447,6 +1448,7 @@ static Snapshot &snap(pt_ctx *c, int l, int f, int p)
 /* WorkingBuffers::saveTempImages (workingBuffers.cpp:242-296), kept on the device */
 static int snap_save(pt_ctx *c, int level, Snapshot &sn)
 {
+    sn.saved = true;
     Snapshot d = sn;
     PtNnf base = c->base;
     float *rb = c->res_base;
@@ -1539,8 +1541,10 @@ static int load_frame_level(pt_ctx *c, int level, const FramePyr &f)
 /* Saved passes per (level, frame) of a scheme; the last one is the output
  * (copyOutputToFinal): FBMO saves its two directed passes, their merge and
  * the fixup passes. */
-static int scheme_passes(int scheme, int fixup_passes)
+static int scheme_passes(int scheme, int fixup_passes, int levels, int F)
 {
+    if (scheme == PT_S_TCTF) /* a consecutive pass, then per level one refine (+ fixups + 1 from 3 frames) */
+        return 1 + levels * (1 + (F > 2 ? fixup_passes + 1 : 0));
     return scheme == PT_S_FBMO ? fixup_passes + 3 : fixup_passes + 1;
 }
 
@@ -1700,6 +1704,114 @@ static int run_schedule_fbmo(pt_ctx *c, const pt_params *params, int iterations,
     return PT_OK;
 }
 
+/* WorkingBuffers::loadOffsets of a pass that may never have been saved: the
+ * reference reads a temporary file and, when it is missing, leaves the
+ * buffers as they are (synthesisFrame.cpp:769-772 ignores the failure). */
+static int snap_load_offsets_if_saved(pt_ctx *c, const Snapshot &sn)
+{
+    return sn.saved ? snap_load_offsets(c, sn) : PT_OK;
+}
+
+/* renderAnimationTCTF (synthesisProcessor.cpp:1151-1312), direct execution,
+ * without keyframes or ribbons (so the largest time step is 1): a
+ * consecutive forward pass on the coarsest level, then on every level from
+ * the coarsest one refine pass over all frames (from the coarser level's
+ * last pass, upsampled, or from this level's previous pass), followed by
+ * fixup+1 more when there are more than two frames.  Pass indices run on
+ * across levels.  refine (synthesisFrame.cpp:409-457) advects both
+ * neighbours into the frame from their last saved pass at this level; a
+ * neighbour that has not saved one here yet (its record still 0) loads
+ * nothing, as the reference's missing temporary file does. */
+static int run_schedule_tctf(pt_ctx *c, const pt_params *params, int iterations, int fixup, int F,
+                             const FramePyr *fp, const StylePyr &sp)
+{
+    int levels = params->levels, L = levels - 1;
+    pt_params fparams = *params;
+    std::vector<int> last((size_t)F * levels, 0); /* [f * levels + level] */
+    auto LAST = [&](int f, int l) -> int & { return last[(size_t)f * levels + l]; };
+    const bool temporal = F > 1 && (params->time_derivative_input_weight > 0.f ||
+                                    params->time_derivative_output_weight > 0.f ||
+                                    params->temporal_coherence_weight > 0.f);
+    RC_CHECK(reset_advected(c));
+    int passes_this = 0, passes_last = 0, last_level = -1;
+    auto enter = [&](int level) -> int {
+        if (level != last_level) {
+            RC_CHECK(load_style_level(c, level, sp));
+            passes_last = passes_this;
+            last_level = level;
+        }
+        return PT_OK;
+    };
+    /* renderConsecutivePass(L, 0, -1, 1): forward, firstPass */
+    RC_CHECK(enter(L));
+    for (int f = 0; f < F; f++) {
+        bool is_first = f == 0;
+        if (!is_first) RC_CHECK(pt_flip_anim_buffers(c));
+        RC_CHECK(load_images(c, L, fp[f]));
+        RC_CHECK(load_anim_images(c, L, fp[f]));
+        fparams.direction = PT_FORWARD;
+        fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+        fparams.frame_number = f;
+        RC_CHECK(pt_upload_parameters(c, &fparams));
+        RC_CHECK(random_pyramid(c, L, f, fp[f].color, fp[f].orient, sp.out, sp.orient));
+        if (is_first) {
+            RC_CHECK(randomize_offsets(c, L));
+        } else {
+            RC_CHECK(advect(c, L, 1, 1, PT_ADVECT_UPDATE_RESIDUAL));
+            RC_CHECK(copy_base_to_working(c));
+        }
+        RC_CHECK(sync_buffers(c, L, 0));
+        RC_CHECK(propagate_and_scatter(c, L, iterations));
+        RC_CHECK(snap_save(c, L, snap(c, L, f, 0)));
+        LAST(f, L) = 0;
+    }
+    passes_this++;
+    /* refine passes, coarse to fine */
+    for (int level = L; level >= 0; level--) {
+        int count = 1 + (F > 2 ? fixup + 1 : 0);
+        for (int r = 0; r < count; r++) {
+            RC_CHECK(enter(level));
+            int this_pass = passes_this, last_coarse = passes_last - 1;
+            bool spatial = last_coarse == this_pass - 1;
+            for (int f = 0; f < F; f++) {
+                fparams.direction = PT_BIDIRECTIONAL;
+                fparams.advection_jitter = (f % 3 == 0) ? params->advection_jitter : 0.f;
+                fparams.frame_number = f;
+                RC_CHECK(pt_upload_parameters(c, &fparams));
+                /* advectNeighbors(level, this_pass - 1, 1, prior, next) */
+                RC_CHECK(reset_advected(c));
+                if (temporal && (f > 0 || f < F - 1)) {
+                    for (int side = 0; side < 2; side++) {
+                        int g = side == 0 ? f - 1 : f + 1;
+                        if (g < 0 || g >= F || this_pass - 1 < 0) continue;
+                        RC_CHECK(load_images(c, level, fp[g]));
+                        RC_CHECK(load_anim_images(c, level, fp[g]));
+                        RC_CHECK(pt_flip_anim_buffers(c));
+                        RC_CHECK(load_anim_images(c, level, fp[f]));
+                        RC_CHECK(snap_load_offsets_if_saved(c, snap(c, level, g, LAST(g, level))));
+                        RC_CHECK(advect(c, level, 1, side == 0, 0));
+                    }
+                } else {
+                    RC_CHECK(load_anim_images(c, level, fp[f]));
+                }
+                if (spatial) { /* spatialUpsample(level+1, level, ...) */
+                    RC_CHECK(snap_load_offsets_if_saved(c, snap(c, level + 1, f, LAST(f, level + 1))));
+                    RC_CHECK(upsample(c, level));
+                } else {
+                    RC_CHECK(snap_load_offsets_if_saved(c, snap(c, level, f, LAST(f, level))));
+                }
+                RC_CHECK(sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL));
+                RC_CHECK(load_images(c, level, fp[f]));
+                RC_CHECK(propagate_and_scatter(c, level, iterations));
+                RC_CHECK(snap_save(c, level, snap(c, level, f, this_pass)));
+                LAST(f, level) = this_pass;
+            }
+            passes_this++;
+        }
+    }
+    return PT_OK;
+}
+
 /* renderAnimationIND (synthesisProcessor.cpp:806-839): every frame on its
  * own, levels coarse to fine inside the frame loop, each pass through
  * synthesizeConsecutive with is_first_frame and forward time
@@ -1751,9 +1863,11 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
                         const FramePyr *fp, const StylePyr &sp)
 {
     int levels = params->levels;
-    int passes = scheme_passes(scheme, fixup_passes);
+    int passes = scheme_passes(scheme, fixup_passes, levels, F);
     RC_CHECK(pt_setup(c, params));
     RC_CHECK(snap_alloc(c, levels, F, passes));
+    for (auto &sn : c->snaps) sn.saved = false;
+    if (scheme == PT_S_TCTF) return run_schedule_tctf(c, params, iterations, fixup_passes, F, fp, sp);
     if (scheme == PT_S_IND) return run_schedule_ind(c, params, iterations, passes, F, fp, sp);
     if (scheme == PT_S_FBMO) return run_schedule_fbmo(c, params, iterations, fixup_passes, F, fp, sp);
     pt_params fparams = *params;
@@ -2044,7 +2158,7 @@ static int synthesize_issue(pt_ctx *c, const pt_params *params, int scheme, int 
     c->frame_pyrs.resize(F);
     for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], guides_on_device, c->frame_pyrs[f]));
     RC_CHECK(run_schedule(c, params, scheme, iterations, fixup_passes, F, c->frame_pyrs.data(), c->style_pyr));
-    return read_results(c, F, scheme_passes(scheme, fixup_passes) - 1, out_off, out_canvas, out_res,
+    return read_results(c, F, scheme_passes(scheme, fixup_passes, params->levels, F) - 1, out_off, out_canvas, out_res,
                         outputs_on_device);
 }
 
@@ -2055,11 +2169,10 @@ static int synthesize(pt_ctx *c, const pt_params *params, int scheme, int iterat
 {
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND && scheme != PT_S_FBMO)
-        return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
+    if (scheme < PT_S_FMBM || scheme > PT_S_IND) return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
-    RC_CHECK(synthesis_alloc(c, params, W, H, F, scheme_passes(scheme, fixup_passes), SW, SH));
+    RC_CHECK(synthesis_alloc(c, params, W, H, F, scheme_passes(scheme, fixup_passes, params->levels, F), SW, SH));
     RC_CHECK(synthesize_issue(c, params, scheme, iterations, fixup_passes, W, H, F, frames, SW, SH, style,
                               guides_on_device, out_off, out_canvas, out_res, outputs_on_device));
     return sync_stream(c);
@@ -2087,11 +2200,10 @@ int pt_record_synthesis(pt_ctx *c, std::vector<RecOp> *ops, const pt_params *par
     if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM && scheme != PT_S_IND && scheme != PT_S_FBMO)
-        return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
+    if (scheme < PT_S_FMBM || scheme > PT_S_IND) return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
-    RC_CHECK(synthesis_alloc(c, params, W, H, F, scheme_passes(scheme, fixup_passes), SW, SH));
+    RC_CHECK(synthesis_alloc(c, params, W, H, F, scheme_passes(scheme, fixup_passes, params->levels, F), SW, SH));
     /* the frame and style pyramids are allocated here too, by a dry pass of
      * their preparation with recording into a scratch list */
     {

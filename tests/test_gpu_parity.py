@@ -232,6 +232,20 @@ def test_synthesize_fbmo_bit_exact(fixup):
                                 fixup_passes=fixup))
 
 
+@pytest.mark.parametrize("fixup", [0, 1])
+def test_synthesize_tctf_bit_exact(fixup):
+    """renderAnimationTCTF: a consecutive pass, then refine passes coarse to
+    fine with both neighbours advected in."""
+    st = W.style_pair(64, 64, seed=71, guides=True)
+    frames = W.animation(48, 40, 3, seed=72)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, hysteresis_weight=0.2)
+        _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, scheme=_ffi.PT_S_TCTF, iterations=2,
+                                fixup_passes=fixup),
+                   o.synthesize(o.default_params(**kw), frames, st, scheme=_ffi.PT_S_TCTF, iterations=2,
+                                fixup_passes=fixup))
+
+
 def test_config1_still_256_bit_exact():
     """The CPU reference workload: 256^2 exemplar pair and target, 3 levels,
     5x5 patches, 4 iterations, orientation off, no temporal term."""

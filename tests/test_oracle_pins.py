@@ -484,11 +484,29 @@ def test_fbmo_single_level_single_frame_is_the_one_frame_schedule():
     assert a[0].tobytes() == b[0].tobytes()
 
 
-def test_unsupported_schemes_are_rejected():
+def test_unknown_schemes_are_rejected():
     st = W.style_pair(32, 32, seed=63)
     fr = W.still_frame(32, 32, seed=64)
     with OracleContext() as o:
         p = o.default_params(residual_window_size=2, levels=2)
-        for scheme in (_ffi.PT_S_TCTF,):
+        for scheme in (-1, 5):
             with pytest.raises(Exception):
                 o.synthesize(p, [fr], st, scheme=scheme, iterations=1, fixup_passes=0)
+
+
+@pytest.mark.parametrize("nframes,fixup", [(1, 0), (3, 1)])
+def test_tctf_runs_and_keeps_offsets_inside_the_exemplar(nframes, fixup):
+    """renderAnimationTCTF: every frame's offsets end within a texel of the
+    exemplar and its energies are finite and non-negative (the schedule is
+    pinned through its operators and the GPU parity; it has no closed
+    form)."""
+    st = W.style_pair(40, 36, seed=69, guides=True)
+    frames = W.animation(36, 32, nframes, seed=70)
+    with OracleContext() as o:
+        p = o.default_params(residual_window_size=2, levels=2, temporal_coherence_weight=0.5)
+        off, canvas, res = o.synthesize(p, frames, st, scheme=_ffi.PT_S_TCTF, iterations=2, fixup_passes=fixup)
+    assert off.shape[0] == nframes
+    # candidates are clamped to [0, size-1]; upsampled (2c+0.5) and averaged
+    # offsets may lie up to a texel beyond, as in the other schedules
+    assert (off["x"] >= 0).all() and (off["x"] <= 41).all() and (off["y"] >= 0).all() and (off["y"] <= 37).all()
+    assert np.isfinite(canvas).all() and np.isfinite(res).all() and (res >= 0).all()
