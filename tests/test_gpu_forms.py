@@ -148,6 +148,29 @@ def test_batch_every_shape_equals_alone(shape):
         b.close()
 
 
+def test_batch_of_every_schedule_equals_each_job_alone():
+    """Jobs of different schedules (FMBM, FMBM_RM, IND, FBMO, TCTF) planned
+    together, graph-replayed: each equals the same job synthesized alone."""
+    st = W.style_pair(48, 40, seed=90, guides=True)
+    frames = W.animation(40, 32, 3, seed=91)
+    jobs = []
+    for scheme in (_ffi.PT_S_FMBM, _ffi.PT_S_FMBM_RM, _ffi.PT_S_IND, _ffi.PT_S_FBMO, _ffi.PT_S_TCTF):
+        jobs.append(dict(kw=dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5),
+                         frames=frames, style=st, iterations=2, scheme=scheme, fixup_passes=1))
+    with Context.gpu(0) as g:
+        alone = [g.synthesize(g.default_params(**j["kw"]), j["frames"], j["style"], scheme=j["scheme"], iterations=2,
+                              fixup_passes=1) for j in jobs]
+        for j in jobs:
+            j["params"] = g.default_params(**j["kw"])
+    b = Batch(jobs)
+    try:
+        b.run()
+        for i in range(len(jobs)):
+            _same(b.result(i), alone[i], "job %d (scheme %d)" % (i, jobs[i]["scheme"]))
+    finally:
+        b.close()
+
+
 def _sequences():
     seqs = []
     for i, (w, h, sw, sh) in enumerate([(48, 40, 64, 56), (64, 48, 48, 48), (40, 40, 56, 64)]):
