@@ -126,6 +126,9 @@ void ptref_sincosf(float x, float *s, float *c) { pt_sincosf(x, s, c); }
 float ptref_atan2f(float y, float x) { return pt_atan2f(y, x); }
 float ptref_logf(float x) { return pt_logf(x); }
 float ptref_acosf(float x) { return pt_acosf(x); }
+double ptref_log_d(double x) { return pt_log_d(x); }
+double ptref_exp_d(double x) { return pt_exp_d(x); }
+void ptref_rgb_to_lab(float r, float g, float b, float *out) { pt_rgb_to_lab(r, g, b, &out[0], &out[1], &out[2]); }
 float ptref_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
                            float ratio_range, float angle_range)
 {
@@ -336,6 +339,15 @@ static float hysteresis_term(const ptref_ctx *c, int x, int y, int level)
 }
 
 /* timeDerivativeTerms (texSynth_kernel.cu:438-469) */
+/* RGBtoLAB (texSynth_util.h:220-249), alpha unchanged */
+static f4 lab4(f4 v)
+{
+    f4 o;
+    pt_rgb_to_lab(v.x, v.y, v.z, &o.x, &o.y, &o.z);
+    o.w = v.w;
+    return o;
+}
+
 static float time_terms(const ptref_ctx *c, const adv_t *a, nnf prev, float pb0x, float pb0y,
                         float pb1x, float pb1y, f4 base, f4 test_base, f4 canvas, f4 test_out,
                         int x, int y, int i, int j)
@@ -347,6 +359,12 @@ static float time_terms(const ptref_ctx *c, const adv_t *a, nnf prev, float pb0x
     new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, &ptx, &pty);
     f4 peb = tex4_lin(c->ex_base, c->ew, c->ew, c->eh, ptx, pty);
     f4 peo = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, ptx, pty);
+    if (c->p.colorspace == 1) { /* texSynth_kernel.cu:454-459 */
+        prev_base = lab4(prev_base);
+        prev_out = lab4(prev_out);
+        peb = lab4(peb);
+        peo = lab4(peo);
+    }
     float r = ssd4(sub4(base, prev_base), sub4(test_base, peb)) * c->p.time_derivative_input_weight;
     r = fmaf(ssd4(sub4(canvas, prev_out), sub4(test_out, peo)), c->p.time_derivative_output_weight, r);
     r = fmaf(ssd4(prev_out, test_out), c->p.temporal_coherence_weight, r);
@@ -390,6 +408,10 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
             int oo = (y + j) * c->W + (x + i);
             f4 base = img_color_at(c, c->cur, x + i, y + j);
             f4 canvas = c->canvas[oo];
+            if (p->colorspace == 1) { /* texSynth_kernel.cu:575-578 */
+                base = lab4(base);
+                canvas = lab4(canvas);
+            }
             int pv = cpv && c->adv[0].off[oo].hy > 0;
             int nv = cnv && c->adv[1].off[oo].hy > 0;
             float tx, ty;
@@ -400,6 +422,10 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
             else if (ty < 0) ty += (float)hh;
             f4 tb = tex4_lin(c->ex_base, c->ew, c->ew, c->eh, tx, ty);
             f4 to = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, tx, ty);
+            if (p->colorspace == 1) { /* texSynth_kernel.cu:603-606 */
+                tb = lab4(tb);
+                to = lab4(to);
+            }
             float res = ssd_alpha(base, tb, p->alpha_weight) * p->input_analogy_weight;
             if (fwd_dir && pv && any_time_w)
                 res = res + time_terms(c, &c->adv[0], pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, x, y, i, j);
@@ -547,8 +573,8 @@ static int check_supported(ptref_ctx *c, const pt_params *p)
         return fail(c, PT_ERR_UNSUPPORTED, "only one style layer is supported");
     if (p->coherence_weight > 0.f && !(p->coherence_ratio_range > 1.f && p->coherence_angle_range > 0.f))
         return fail(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
-    if (p->colorspace != 0)
-        return fail(c, PT_ERR_UNSUPPORTED, "only the RGB colourspace is supported");
+    if (p->colorspace != 0 && p->colorspace != 1)
+        return fail(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
     if (p->lum_shift_enable)
         return fail(c, PT_ERR_UNSUPPORTED, "luminance shift is not supported");
     if (p->style_obj_id_weight != 0.f)

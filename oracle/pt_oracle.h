@@ -85,6 +85,9 @@ void ptref_sincosf(float x, float *s, float *c);
 float ptref_atan2f(float y, float x);
 float ptref_logf(float x);
 float ptref_acosf(float x);
+double ptref_log_d(double x);
+double ptref_exp_d(double x);
+void ptref_rgb_to_lab(float r, float g, float b, float *out);
 float ptref_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
                            float ratio_range, float angle_range);
 float ptref_tensor_to_angle(float xx, float xy, float yy);

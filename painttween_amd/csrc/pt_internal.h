@@ -48,6 +48,7 @@ struct PtArgs {
     int cull, transp, orient_en, direction;
     float tdi, tdo, tcw, hyst_w, max_dist, jitter;
     float coh_w, coh_ratio, coh_angle; /* coherence term: weight (0 = off), ratio and angle ranges */
+    int lab;           /* colour distances in the LAB space (colorspace 1) */
     int samples, rng_mode;
     uint32_t key0, key1, call;
     int time_active;   /* some advected offset may be valid */

@@ -137,6 +137,14 @@ __device__ __forceinline__ float ssd_alpha(float4 a, float4 b, float alpha)
     return s;
 }
 
+/* RGBtoLAB of a colour (texSynth_util.h:220-249), alpha unchanged */
+__device__ __forceinline__ float4 lab4(float4 c)
+{
+    float l, a, b;
+    pt_rgb_to_lab(c.x, c.y, c.z, &l, &a, &b);
+    return make_float4(l, a, b, c.w);
+}
+
 __device__ __forceinline__ float ssd4(float4 a, float4 b)
 {
     float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
@@ -268,7 +276,10 @@ __device__ __forceinline__ void ex_pair(__amdgpu_buffer_rsrc_t er, int ew, int e
     o = mix4(q, ld4(er, o00 + 16), ld4(er, o10 + 16), ld4(er, o01 + 16), ld4(er, o11 + 16));
 }
 
-/* timeDerivativeTerms (texSynth_kernel.cu:438-469) */
+/* timeDerivativeTerms (texSynth_kernel.cu:438-469); EXT: the kernel
+ * variant with the extended terms, where the LAB colour space converts the
+ * previous frame's colours and exemplar samples (:454-459) */
+template <bool EXT = false>
 __device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, const PtAdv &d, float4 prev, float pb0x,
                               float pb0y, float pb1x, float pb1y, float4 base, float4 tb, float4 canvas, float4 to,
                               int oo, int i, int j)
@@ -279,6 +290,12 @@ __device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, const 
     new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, ptx, pty);
     float4 peb, peo;
     ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo);
+    if (EXT && A.lab) {
+        prev_base = lab4(prev_base);
+        prev_out = lab4(prev_out);
+        peb = lab4(peb);
+        peo = lab4(peo);
+    }
     float r = ssd4(sub4(base, prev_base), sub4(tb, peb)) * A.tdi;
     r = fmaf(ssd4(sub4(canvas, prev_out), sub4(to, peo)), A.tdo, r);
     r = fmaf(ssd4(prev_out, to), A.tcw, r);
@@ -299,6 +316,10 @@ __device__ __forceinline__ float coherence_add(const PtArgs &A, float res, int x
 }
 
 /*
+ * RT < 0 selects the variant of the per-pixel kernels with the extended
+ * terms (coherence, LAB colour space) at a run-time radius; the default
+ * kernels (RT >= 0) leave them out and keep their registers.
+ *
  * calculateResidual (texSynth_kernel.cu:537-659): one patch distance of
  * candidate (t, tsc) at pixel (x, y).  Taps run i over x outer, j over y
  * inner and are accumulated sequentially per lane in that order.  TIME
@@ -350,6 +371,10 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
                 continue;
             float4 base = tcol[j * T.tw];
             float4 canvas = tcan[j * T.tw];
+            if (RT < 0 && A.lab) {
+                base = lab4(base);
+                canvas = lab4(canvas);
+            }
             float tx = t.x + (ibx + (float)j * b1x);
             float ty = t.y + (iby + (float)j * b1y);
             if (tx >= hw) tx -= hw;
@@ -358,13 +383,17 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
             else if (ty < 0) ty += hh;
             float4 tb, to;
             ex_pair(er, ew, eh, tx, ty, tb, to);
+            if (RT < 0 && A.lab) {
+                tb = lab4(tb);
+                to = lab4(to);
+            }
             float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
             if (TIME && any_time_w) {
                 int oo = (y + j) * A.W + (x + i);
                 if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
-                    res = res + time_terms_b(A, er, A.adv[0], pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo, i, j);
+                    res = res + time_terms_b<(RT < 0)>(A, er, A.adv[0], pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo, i, j);
                 if (bwd_dir && cnv && A.adv[1].xyth[oo].w > 0)
-                    res = res + time_terms_b(A, er, A.adv[1], pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo, i, j);
+                    res = res + time_terms_b<(RT < 0)>(A, er, A.adv[1], pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo, i, j);
             }
             if (i == 0 && j == 0) {
                 if (A.dt_w > 0.f) {
@@ -377,7 +406,7 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
                 }
             } else {
                 res = fmaf(ssd_alpha(canvas, to, A.alpha_w), A.canvas_w, res);
-                res = coherence_add(A, res, x, y, i, j, t.x, t.y, tx, ty);
+                if (RT < 0) res = coherence_add(A, res, x, y, i, j, t.x, t.y, tx, ty);
             }
             float wgt = A.gauss[i + r] * A.gauss[j + r];
             sum_r = fmaf(res, wgt, sum_r);
@@ -2485,6 +2514,10 @@ __global__ __launch_bounds__(PT_UP_THREADS) void km_scatter_up(PT_MULTI_PARAMS)
 /* ------------------------------------------------------------------ */
 /* launchers                                                           */
 /* ------------------------------------------------------------------ */
+/* the extended patch-distance terms (coherence, LAB) run in the per-pixel
+ * kernels' RT = -1 variant */
+static bool ext_terms(const PtArgs &a) { return a.coh_w > 0.f || a.lab; }
+
 static dim3 grid_for(int w, int h) { return dim3((w + PT_BX - 1) / PT_BX, (h + PT_BY - 1) / PT_BY); }
 static const dim3 kBlock(PT_BX, PT_BY);
 static dim3 grid_rows(int w, int h) { return dim3((w + PT_RX - 1) / PT_RX, (h + PT_RY - 1) / PT_RY); }
@@ -2520,6 +2553,12 @@ hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s)
         case 3: hipLaunchKernelGGL((tk ? k_update_residual_tp<3, true> : k_update_residual_tp<3, false>), g, bu, 0, s, a); break;
         default: hipLaunchKernelGGL((tk ? k_update_residual_tp<4, true> : k_update_residual_tp<4, false>), g, bu, 0, s, a); break;
         }
+        return hipGetLastError();
+    }
+    if (ext_terms(a)) {
+        bool tk = a.time_active != 0;
+        hipLaunchKernelGGL((tk ? k_update_residual<-1, true> : k_update_residual<-1, false>), grid_for(a.lw, a.lh),
+                           kBlock, 0, s, a);
         return hipGetLastError();
     }
     PT_DISPATCH_R(k_update_residual, a);
@@ -2616,9 +2655,9 @@ void pt_tap_configure(PtArgs &a, int evals)
 int pt_shape_valid(int op, int shape, const PtArgs &a)
 {
     if (shape < 0 || shape >= PT_SHAPE_COUNT) return 0;
-    /* the coherence term is evaluated by the per-pixel in-order kernels only
-     * (calc_residual_impl), which keeps its registers out of the others */
-    if (a.coh_w > 0.f && shape != PT_SHAPE_PIXEL) return 0;
+    /* the coherence term and the LAB colour space are evaluated by the
+     * per-pixel in-order kernels only (their RT < 0 variant) */
+    if ((a.coh_w > 0.f || a.lab) && shape != PT_SHAPE_PIXEL) return 0;
     if (shape == PT_SHAPE_TAP) return tap_pixels(op, a) > 0;
     if (op == PT_OP_RESIDUAL || op == PT_OP_SYNC_CANVAS) return shape == PT_SHAPE_PIXEL;
     if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return 0;
@@ -2667,7 +2706,7 @@ static int rt_of(int r) { return (r == 2 || r == 3 || r == 4) ? r : 0; }
 int pt_shape_key(int op, int shape, const PtArgs &a)
 {
     int tk = (op == PT_OP_SYNC_CANVAS) ? 0 : (a.time_active ? 1 : 0);
-    int rt = (op == PT_OP_SYNC_CANVAS) ? 0 : rt_of(a.r);
+    int rt = (op == PT_OP_SYNC_CANVAS) ? 0 : (ext_terms(a) ? 1 : rt_of(a.r));
     return ((op * PT_SHAPE_COUNT + shape) * 8 + rt) * 2 + tk;
 }
 
@@ -2679,6 +2718,12 @@ int pt_shape_key(int op, int shape, const PtArgs &a)
     case 3: F(3); break;                                                                                  \
     case 4: F(4); break;                                                                                  \
     default: F(0); break;                                                                                 \
+    }
+#define PT_RTX_SWITCH(a, F)                                                                               \
+    if (ext_terms(a)) {                                                                                   \
+        F(-1);                                                                                            \
+    } else {                                                                                              \
+        PT_RT_SWITCH((a).r, F)                                                                            \
     }
 
 hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
@@ -2701,7 +2746,7 @@ hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
 #define L_UEB(R) hipLaunchKernelGGL((tk ? k_propagate_up<R, true, 8, 8, true> : k_propagate_up<R, true, 8, 8, false>), g, bu, 0, s, a, jump)
 #define L_UA(R) hipLaunchKernelGGL((tk ? k_propagate_up<R, false, 8, 8, true> : k_propagate_up<R, false, 8, 8, false>), g, bu, 0, s, a, jump)
     switch (shape) {
-    case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
+    case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
     case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
     case PT_SHAPE_UP_EE_B: PT_RT_SWITCH(a.r, L_UEB); break;
     default: PT_RT_SWITCH(a.r, L_UA); break;
@@ -2728,7 +2773,7 @@ hipError_t pt_launch_scatter(const PtArgs &a0, hipStream_t s)
 #define L_UEA(R) hipLaunchKernelGGL((tk ? k_scatter_up<R, true, 16, 8, true> : k_scatter_up<R, true, 16, 8, false>), g, bu, 0, s, a, n)
 #define L_UA(R) hipLaunchKernelGGL((tk ? k_scatter_up<R, false, 4, 4, true> : k_scatter_up<R, false, 4, 4, false>), g, bu, 0, s, a, n)
     switch (shape) {
-    case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
+    case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
     case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
     case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
     default: PT_RT_SWITCH(a.r, L_UA); break;
@@ -2757,7 +2802,7 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
             break;
         }
 #define L(R) hipLaunchKernelGGL((tk ? km_update_residual<R, true> : km_update_residual<R, false>), g, kBlock, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
-        PT_RT_SWITCH(a.r, L);
+        PT_RTX_SWITCH(a, L);
 #undef L
         break;
     }
@@ -2770,7 +2815,7 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
 #define L_UEB(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, true, 8, 8, true> : km_propagate_up<R, true, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_UB(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, false, 16, 8, true> : km_propagate_up<R, false, 16, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
         switch (shape) {
-        case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
+        case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
         case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
         case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
         case PT_SHAPE_UP_A: PT_RT_SWITCH(a.r, L_UA); break;
@@ -2797,7 +2842,7 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
 #define L_UEB(R) hipLaunchKernelGGL((tk ? km_scatter_up<R, true, 8, 8, true> : km_scatter_up<R, true, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_UB(R) hipLaunchKernelGGL((tk ? km_scatter_up<R, false, 8, 8, true> : km_scatter_up<R, false, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
         switch (shape) {
-        case PT_SHAPE_PIXEL: PT_RT_SWITCH(a.r, L_PIX); break;
+        case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
         case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
         case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
         case PT_SHAPE_UP_A: PT_RT_SWITCH(a.r, L_UA); break;

@@ -35,6 +35,15 @@
  *      [-1, 1].  The reference calls acosf in the coherence term
  *      (texSynth_kernel.cu:417).
  *  - pt_coherence_term: the coherence energy of one tap (below).
+ *  - pt_log_d / pt_exp_d / pt_pow_d (double): log as pt_logf with the series
+ *      to s^25; exp by k = rint(x/ln2), r = x - k ln2 (two-part ln2) and the
+ *      Taylor series to r^14 (|r| <= 0.35, truncation below 1e-19), scaled by
+ *      2^k; pow(x, y) = exp(y log x) for x > 0.  Used by the colour
+ *      conversion below, where the reference calls pow in double and powf.
+ *  - pt_rgb_to_lab: RGBtoLAB (texSynth_util.h:220-249) with Clinear and f_Lab
+ *      (:166-195), keeping the reference's mix of double and float
+ *      arithmetic; powf(c, 3) and powf(c, 2) of its constants are the
+ *      correctly rounded float values.
  *  - pt_philox4x32_10:
  *      Philox4x32-10 counter-based generator (J. K. Salmon, M. A. Moraes,
  *      R. O. Dror, D. E. Shaw, "Parallel Random Numbers: As Easy as 1, 2, 3",
@@ -260,6 +269,124 @@ PT_HD float pt_coherence_term(float cx, float cy, float tcx, float tcy, float tx
     double lim = (double)angle_range * 3.14159265358979323846;
     float e_angle = (float)(fmin((double)angle, lim) / lim);
     return 1.0f - (1.0f - e_ratio) * (1.0f - e_angle);
+}
+
+/* ln 2 = PT_LN2_HI_D + PT_LN2_LO_D, the head with 32 significant bits */
+#define PT_LN2_HI_D 6.93147180369123816490e-01
+#define PT_LN2_LO_D 1.90821492927058770002e-10
+
+PT_HD double pt_log_d(double x)
+{
+    if (!(x >= 0.0)) return NAN;
+    if (x == 0.0) return -INFINITY;
+    if (isinf(x)) return x;
+    int e = 0;
+    if (x < 2.2250738585072014e-308) { /* subnormal: scale by 2^54 (exact) */
+        x *= 18014398509481984.0;
+        e = -54;
+    }
+    union {
+        double d;
+        uint64_t u;
+    } b;
+    b.d = x;
+    e += (int)((b.u >> 52) & 0x7ffu) - 1023;
+    b.u = (b.u & 0x000fffffffffffffull) | 0x3ff0000000000000ull; /* m in [1, 2) */
+    double m = b.d;
+    if (m >= 1.4142135623730951) {
+        m *= 0.5;
+        e += 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double z = s * s;
+    double p = 1.0 / 27.0;
+    p = fma(z, p, 1.0 / 25.0);
+    p = fma(z, p, 1.0 / 23.0);
+    p = fma(z, p, 1.0 / 21.0);
+    p = fma(z, p, 1.0 / 19.0);
+    p = fma(z, p, 1.0 / 17.0);
+    p = fma(z, p, 1.0 / 15.0);
+    p = fma(z, p, 1.0 / 13.0);
+    p = fma(z, p, 1.0 / 11.0);
+    p = fma(z, p, 1.0 / 9.0);
+    p = fma(z, p, 1.0 / 7.0);
+    p = fma(z, p, 1.0 / 5.0);
+    p = fma(z, p, 1.0 / 3.0);
+    double lm = 2.0 * fma(s * z, p, s);
+    double ef = (double)e;
+    return ef * PT_LN2_HI_D + fma(ef, PT_LN2_LO_D, lm);
+}
+
+PT_HD double pt_exp_d(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return INFINITY;
+    if (x < -745.1332191019412) return 0.0;
+    double k = rint(x * 1.4426950408889634);
+    double r = fma(-k, PT_LN2_HI_D, x);
+    r = fma(-k, PT_LN2_LO_D, r);
+    double p = 1.0 / 87178291200.0; /* 1/14! */
+    p = fma(r, p, 1.0 / 6227020800.0);
+    p = fma(r, p, 1.0 / 479001600.0);
+    p = fma(r, p, 1.0 / 39916800.0);
+    p = fma(r, p, 1.0 / 3628800.0);
+    p = fma(r, p, 1.0 / 362880.0);
+    p = fma(r, p, 1.0 / 40320.0);
+    p = fma(r, p, 1.0 / 5040.0);
+    p = fma(r, p, 1.0 / 720.0);
+    p = fma(r, p, 1.0 / 120.0);
+    p = fma(r, p, 1.0 / 24.0);
+    p = fma(r, p, 1.0 / 6.0);
+    p = fma(r, p, 0.5);
+    p = fma(r, p, 1.0);
+    p = fma(r, p, 1.0);
+    return ldexp(p, (int)k);
+}
+
+/* x^y for x > 0 (0 for x == 0 and y > 0) */
+PT_HD double pt_pow_d(double x, double y)
+{
+    if (x == 0.0) return y > 0.0 ? 0.0 : (y == 0.0 ? 1.0 : INFINITY);
+    if (!(x > 0.0)) return NAN;
+    return pt_exp_d(y * pt_log_d(x));
+}
+
+PT_HD float pt_powf(float x, float y) { return (float)pt_pow_d((double)x, (double)y); }
+
+/* Clinear (texSynth_util.h:175-183): (c + a) / (1.0 + a) and the division by
+ * 12.92 in double, pow in double */
+PT_HD float pt_srgb_to_linear(float c)
+{
+    const float a = 0.055f;
+    if ((double)c > 0.04045) return (float)pt_pow_d((double)(c + a) / (1.0 + (double)a), 2.4);
+    return (float)((double)c / 12.92);
+}
+
+/* f_Lab (texSynth_util.h:185-192): powf((float)(6/29), 3) and
+ * powf((float)(29/6), 2) correctly rounded; the linear branch in double */
+#define PT_LAB_T0 8.856453001e-03f
+#define PT_LAB_K2 2.336111259e+01f
+PT_HD float pt_f_lab(float t)
+{
+    if (t > PT_LAB_T0) return pt_powf(t, 0.3333333432674408f);
+    return (float)(((1.0 / 3.0) * (double)PT_LAB_K2) * (double)t + (4.0 / 29.0));
+}
+
+/* RGBtoLAB (texSynth_util.h:220-249): D65 white, L/100, (a+100)/200,
+ * (b+100)/200; the XYZ matrix products in double; alpha unchanged */
+PT_HD void pt_rgb_to_lab(float r, float g, float b, float *lo, float *ao, float *bo)
+{
+    const float ref_x = 95.0469970703125f, ref_y = 100.0f, ref_z = 108.88300323486328f;
+    float rl = 100 * pt_srgb_to_linear(r);
+    float gl = 100 * pt_srgb_to_linear(g);
+    float bl = 100 * pt_srgb_to_linear(b);
+    float x = (float)(0.4124 * (double)rl + 0.3576 * (double)gl + 0.1805 * (double)bl);
+    float y = (float)(0.2126 * (double)rl + 0.7152 * (double)gl + 0.0722 * (double)bl);
+    float z = (float)(0.0193 * (double)rl + 0.1192 * (double)gl + 0.9505 * (double)bl);
+    float fy = pt_f_lab(y / ref_y);
+    *lo = (116.f * fy - 16) / 100.f;
+    *ao = (500.f * (pt_f_lab(x / ref_x) - fy) + 100) / 200.f;
+    *bo = (200.f * (fy - pt_f_lab(z / ref_z)) + 100) / 200.f;
 }
 
 /* Philox4x32-10. ctr[4], key[2] in; out[4]. */

@@ -229,7 +229,8 @@ static int check_supported(pt_ctx *c, const pt_params *p)
     if (p->num_styles != 1) return set_err(c, PT_ERR_UNSUPPORTED, "only one style layer is supported");
     if (p->coherence_weight > 0.f && !(p->coherence_ratio_range > 1.f && p->coherence_angle_range > 0.f))
         return set_err(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
-    if (p->colorspace != 0) return set_err(c, PT_ERR_UNSUPPORTED, "only the RGB colourspace is supported");
+    if (p->colorspace != 0 && p->colorspace != 1)
+        return set_err(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
     if (p->lum_shift_enable) return set_err(c, PT_ERR_UNSUPPORTED, "luminance shift is not supported");
     if (p->style_obj_id_weight != 0.f) return set_err(c, PT_ERR_UNSUPPORTED, "object-id term is not supported");
     if (p->interpolate_key_frame)
@@ -1021,6 +1022,7 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.coh_w = p.coherence_weight > 0.f ? p.coherence_weight : 0.f;
     a.coh_ratio = p.coherence_ratio_range;
     a.coh_angle = p.coherence_angle_range;
+    a.lab = p.colorspace == 1 ? 1 : 0;
     a.hyst_w = p.hysteresis_weight;
     a.max_dist = p.max_distortion;
     a.jitter = p.advection_jitter;

@@ -152,6 +152,25 @@ def test_coherence_term_operator_sequence(r, early_exit, unit_parallel, tap):
     assert_same_states(ga, oa)
 
 
+@pytest.mark.parametrize("early_exit,unit_parallel,tap", [(1, 1, 0), (0, 0, 0), (1, 1, 1024)])
+def test_lab_colour_space_operator_sequence(early_exit, unit_parallel, tap):
+    """colorspace 1: every colour distance in L*a*b* (texSynth_kernel.cu:575-606)."""
+    st = W.style_pair(40, 36, seed=19, guides=True)
+    fr = W.still_frame(44, 34, seed=20, guides=True)
+    ov = dict(residual_window_size=2, levels=3, colorspace=1, orientation_enabled=1, dist_trans_weight=0.5)
+    ga, oa = _both(st, [fr], ov, _still_ops(3), early_exit, unit_parallel, tap)
+    assert_same_states(ga, oa)
+
+
+def test_lab_with_coherence_and_temporal_terms():
+    st = W.style_pair(48, 48, seed=21, guides=True)
+    frames = W.animation(40, 32, 3, seed=22)
+    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=0.3,
+              coherence_weight=0.3, colorspace=1)
+    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, "merge_random"))
+    assert_same_states(ga, oa)
+
+
 @pytest.mark.parametrize("tap", [0, 1024])
 def test_coherence_term_with_temporal_terms(tap):
     st = W.style_pair(48, 48, seed=17, guides=True)

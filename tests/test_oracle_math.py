@@ -121,3 +121,54 @@ def test_acos_within_three_ulp_of_libm():
     assert worst <= 3.0, worst
     assert lib.ptref_acosf(1.0) == 0.0
     assert np.isnan(lib.ptref_acosf(1.0000001)) and np.isnan(lib.ptref_acosf(-1.0000001))
+
+
+def test_double_log_and_exp_within_two_ulp_of_libm():
+    lib = load_oracle()
+    worst = 0.0
+    for x in np.concatenate([np.geomspace(1e-300, 1e300, 4001), np.linspace(0.05, 3.0, 4001)]):
+        got, ref = lib.ptref_log_d(float(x)), np.log(float(x))
+        if ref != 0:
+            worst = max(worst, abs(got - ref) / np.spacing(abs(ref)))
+    for x in np.linspace(-700, 700, 8001):
+        got, ref = lib.ptref_exp_d(float(x)), np.exp(float(x))
+        worst = max(worst, abs(got - ref) / np.spacing(ref))
+    assert worst <= 2.0, worst
+    assert lib.ptref_log_d(1.0) == 0.0 and lib.ptref_exp_d(0.0) == 1.0
+
+
+def _lab64(r, g, b):
+    """RGBtoLAB (texSynth_util.h:220-249) in float64."""
+    def lin(c):
+        return ((c + 0.055) / 1.055) ** 2.4 if c > 0.04045 else c / 12.92
+
+    def f(t):
+        return t ** (1.0 / 3.0) if t > (6.0 / 29.0) ** 3 else (29.0 / 6.0) ** 2 / 3.0 * t + 4.0 / 29.0
+    rl, gl, bl = 100 * lin(r), 100 * lin(g), 100 * lin(b)
+    x = 0.4124 * rl + 0.3576 * gl + 0.1805 * bl
+    y = 0.2126 * rl + 0.7152 * gl + 0.0722 * bl
+    z = 0.0193 * rl + 0.1192 * gl + 0.9505 * bl
+    fx, fy, fz = f(x / 95.047), f(y / 100.0), f(z / 108.883)
+    return (116 * fy - 16) / 100, (500 * (fx - fy) + 100) / 200, (200 * (fy - fz) + 100) / 200
+
+
+def test_rgb_to_lab_against_float64_and_exact_values():
+    """The colour conversion against its float64 statement (both branches of
+    the sRGB curve and of f); black maps to (0, 1/2, 1/2) exactly; sRGB red,
+    green and blue land on their textbook L*a*b* coordinates."""
+    lib = load_oracle()
+    out = (C.c_float * 3)()
+    rng = np.random.default_rng(7)
+    pts = np.concatenate([rng.uniform(0, 1, (3000, 3)), rng.uniform(0, 0.05, (500, 3)), np.eye(3), np.ones((1, 3))])
+    worst = 0.0
+    for r, g, b in pts.astype(np.float32):
+        lib.ptref_rgb_to_lab(float(r), float(g), float(b), out)
+        worst = max(worst, max(abs(a - e) for a, e in zip(out, _lab64(float(r), float(g), float(b)))))
+    assert worst < 3e-6, worst
+    lib.ptref_rgb_to_lab(0.0, 0.0, 0.0, out)
+    assert list(out) == [0.0, 0.5, 0.5]
+    for rgb, lab in (((1, 0, 0), (53.24, 80.09, 67.20)), ((0, 1, 0), (87.73, -86.18, 83.18)),
+                     ((0, 0, 1), (32.30, 79.19, -107.86))):
+        lib.ptref_rgb_to_lab(*map(float, rgb), out)
+        assert abs(out[0] * 100 - lab[0]) < 0.05 and abs(out[1] * 200 - 100 - lab[1]) < 0.1 \
+            and abs(out[2] * 200 - 100 - lab[2]) < 0.1, (rgb, list(out))
