@@ -1,0 +1,137 @@
+"""EXR reader/writer for offset dumps and RGBA images (painttween_amd/exr.py),
+the files of ImageIO (src/PaintTweenCUDA/imageIO.cpp:45,340-428).
+
+No OpenEXR library and no EXR file exists in this image or in the reference,
+so interoperability with OpenEXR itself is parity unpinned.  The tests pin
+the module against byte values worked out by hand from the published file
+layout (magic bytes, channel-list records, the ZIP predictor, RLE runs) and
+check round trips and the half rounding."""
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from painttween_amd import exr, PATCHXF_DTYPE
+
+
+def test_magic_and_version_bytes(tmp_path):
+    p = tmp_path / "a.exr"
+    exr.write_exr(p, {"R": (np.zeros((1, 1), np.float32), exr.FLOAT)}, exr.NO_COMPRESSION)
+    b = p.read_bytes()
+    # 20000630 little-endian, then version 2 with no flags
+    assert b[:8] == bytes([0x76, 0x2F, 0x31, 0x01, 2, 0, 0, 0])
+
+
+def test_uncompressed_file_layout(tmp_path):
+    """A 2x1 FLOAT image with channels B and A: the bytes after the header are
+    the offset table (one u64 per line) and per line y, size, then each
+    channel's samples in name order (A before B)."""
+    p = tmp_path / "b.exr"
+    a = np.array([[1.0, 2.0]], np.float32)
+    bb = np.array([[3.0, 4.0]], np.float32)
+    exr.write_exr(p, {"B": (bb, exr.FLOAT), "A": (a, exr.FLOAT)}, exr.NO_COMPRESSION)
+    buf = p.read_bytes()
+    # channel list: "A\0", type 2, pLinear 0, 3 reserved, sampling 1, 1; then "B"; then "\0"
+    chl = b"A\0" + bytes([2, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]) + \
+          b"B\0" + bytes([2, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]) + b"\0"
+    assert b"channels\0chlist\0" + struct.pack("<i", len(chl)) + chl in buf
+    assert b"compression\0compression\0" + bytes([1, 0, 0, 0, 0]) in buf
+    assert b"dataWindow\0box2i\0" + struct.pack("<i4i", 16, 0, 0, 1, 0) in buf
+    chunk = struct.pack("<ii", 0, 16) + struct.pack("<4f", 1.0, 2.0, 3.0, 4.0)
+    assert buf.endswith(chunk)
+    table_at = len(buf) - len(chunk) - 8
+    assert struct.unpack_from("<Q", buf, table_at)[0] == table_at + 8
+    assert buf[table_at - 1] == 0  # end of header
+
+
+def test_zip_predictor_known_bytes():
+    # bytes 1,2,3,4 -> interleaved 1,3,2,4 -> deltas (+128 mod 256) 1,130,127,130
+    assert exr._interleave_predict(bytes([1, 2, 3, 4])) == bytes([1, 130, 127, 130])
+    assert exr._unpredict_deinterleave(bytes([1, 130, 127, 130])) == bytes([1, 2, 3, 4])
+    # odd length: the first half holds the extra even-indexed byte; wrap-around deltas
+    # (255, 0, 1 -> 255, 1, 0 -> 255, 1-255+128 = 130 mod 256, 0-1+128 = 127)
+    assert exr._interleave_predict(bytes([255, 0, 1])) == bytes([255, 130, 127])
+    assert exr._unpredict_deinterleave(bytes([255, 130, 127])) == bytes([255, 0, 1])
+
+
+def test_zip_chunk_is_zlib_of_predicted_block(tmp_path):
+    p = tmp_path / "c.exr"
+    v = np.arange(64, dtype=np.float32).reshape(2, 32)
+    exr.write_exr(p, {"Y": (v, exr.FLOAT)}, exr.ZIP_COMPRESSION)
+    buf = p.read_bytes()
+    planes, hdr = exr.read_exr(p)
+    assert hdr["compression"] == exr.ZIP_COMPRESSION
+    # one 16-line block holds both lines: find it through the offset table
+    head_end = buf.index(b"screenWindowWidth\0float\0") + len(b"screenWindowWidth\0float\0") + 8 + 1
+    off = struct.unpack_from("<Q", buf, head_end)[0]
+    y, size = struct.unpack_from("<ii", buf, off)
+    assert y == 0
+    raw = zlib.decompress(buf[off + 8:off + 8 + size])
+    assert raw == exr._interleave_predict(v.tobytes())
+    np.testing.assert_array_equal(planes["Y"], v)
+
+
+def test_rle_runs():
+    # +2 -> next byte 3 times; -3 -> 3 literal bytes
+    assert exr._rle_decode(bytes([2, 7, 0xFD, 1, 2, 3]), 6) == bytes([7, 7, 7, 1, 2, 3])
+    with pytest.raises(exr.ExrError):
+        exr._rle_decode(bytes([2, 7]), 4)
+
+
+@pytest.mark.parametrize("comp", [exr.NO_COMPRESSION, exr.ZIPS_COMPRESSION, exr.ZIP_COMPRESSION])
+def test_offsets_round_trip(tmp_path, comp):
+    rng = np.random.default_rng(3)
+    h, w = 37, 21  # not a multiple of the 16-line ZIP block
+    o = np.zeros((h, w), PATCHXF_DTYPE)
+    for f in ("x", "y", "theta", "hysteresis", "scale_u", "scale_v", "lum_y", "lum_a"):
+        o[f] = rng.uniform(-50, 300, (h, w)).astype(np.float32)
+    o["layer"] = rng.integers(0, 4, (h, w))
+    p = tmp_path / "off.exr"
+    exr.write_offsets(p, o, comp)
+    planes, hdr = exr.read_exr(p)
+    assert sorted(planes) == sorted("PatchXF." + c for c in exr.PATCHXF_CHANNELS)
+    assert all(t == exr.FLOAT for _, t in hdr["channels"])
+    back = exr.read_offsets(p)
+    assert back.tobytes() == o.tobytes()
+
+
+def test_offsets_layer_truncates_like_float_to_int(tmp_path):
+    p = tmp_path / "l.exr"
+    o = np.zeros((1, 3), PATCHXF_DTYPE)
+    exr.write_offsets(p, o)
+    planes, _ = exr.read_exr(p)
+    planes["PatchXF.layer"][:] = [2.9, -0.5, 1.0]
+    exr.write_exr(p, {k: (v, exr.FLOAT) for k, v in planes.items()})
+    assert list(exr.read_offsets(p)["layer"][0]) == [2, 0, 1]
+
+
+def test_rgba_rounds_to_half(tmp_path):
+    rng = np.random.default_rng(5)
+    img = rng.uniform(0, 1, (9, 13, 4)).astype(np.float32)
+    img[0, 0] = [1 + 2 ** -11, 1 + 3 * 2 ** -11, 65520.0, 1e-8]  # ties to even, overflow, subnormal
+    p = tmp_path / "rgba.exr"
+    with np.errstate(over="ignore"):
+        exr.write_rgba(p, img)
+        want = img.astype(np.float16).astype(np.float32)
+    back = exr.read_rgba(p)
+    np.testing.assert_array_equal(back, want)
+    assert back[0, 0, 0] == 1.0 and back[0, 0, 1] == 1 + 2 ** -9 and np.isinf(back[0, 0, 2])
+    _, hdr = exr.read_exr(p)
+    assert [n for n, _ in hdr["channels"]] == ["A", "B", "G", "R"]
+    assert all(t == exr.HALF for _, t in hdr["channels"])
+
+
+def test_rejects_other_files(tmp_path):
+    p = tmp_path / "x.exr"
+    p.write_bytes(b"\0" * 16)
+    with pytest.raises(exr.ExrError):
+        exr.read_exr(p)
+    exr.write_exr(p, {"R": (np.zeros((2, 2), np.float32), exr.FLOAT)}, exr.NO_COMPRESSION)
+    b = bytearray(p.read_bytes())
+    b[5] |= 0x02  # tiled flag (bit 9)
+    p.write_bytes(bytes(b))
+    with pytest.raises(exr.ExrError):
+        exr.read_exr(p)
+    with pytest.raises(exr.ExrError):
+        exr.write_exr(p, {"R": (np.zeros((2, 2), np.float32), exr.FLOAT)}, exr.PIZ_COMPRESSION)
