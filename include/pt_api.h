@@ -355,12 +355,13 @@ typedef struct pt_style_guides {
 } pt_style_guides;
 
 /*
- * Synthesis of a sequence of `num_frames` frames (1 = a still) with the
- * reference FMBM / FMBM_RM schedule: SynthesisProcessor::renderAnimationFMBM
- * (synthesisProcessor.cpp:846-878) over renderConsecutivePass (:703-753) and
- * SynthesisFrame::synthesizeConsecutive (synthesisFrame.cpp:214-323) with
- * propagateAndScatter (:483-508), at maxLevel 0.  Frames are processed in
- * the reference order (levels coarse to fine; passes alternate direction).
+ * Synthesis of a sequence of `num_frames` frames (1 = a still) with one of
+ * the reference schedules (`scheme`, PT_S_*): FMBM / FMBM_RM,
+ * SynthesisProcessor::renderAnimationFMBM (synthesisProcessor.cpp:846-878)
+ * over renderConsecutivePass (:703-753) and SynthesisFrame::synthesizeConsecutive
+ * (synthesisFrame.cpp:214-323) with propagateAndScatter (:483-508); IND
+ * (:806-839), FBMO (:902-970) or TCTF (:1151-1312); all at maxLevel 0.
+ * Frames are processed in the reference order of the scheme.
  * out_offsets / out_canvas / out_residual (each may be NULL) receive every
  * frame's level-0 result of the final pass: num_frames*w*h elements.
  * Each call starts from zeroed working buffers (a fresh WorkingBuffers), so

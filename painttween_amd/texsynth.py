@@ -226,7 +226,7 @@ class Context:
     # -- one-shot schedule ----------------------------------------------
     def synthesize(self, params, frames, style, scheme=_ffi.PT_S_FMBM_RM, iterations=3, fixup_passes=1,
                    want=("offsets", "canvas", "residual")):
-        """Run the FMBM, FMBM_RM or IND schedule over `frames` (list of dicts with
+        """Run a schedule (PT_S_FMBM, _FMBM_RM, _FBMO, _TCTF, _IND) over `frames` (list of dicts with
         keys color, orientation, dist_trans, scale, vel_b, vel_f; arrays are
         (H, W, C) float32 or None) against `style` (dict: base, output,
         orientation, dist_trans).  Returns level-0 results of the final pass,
