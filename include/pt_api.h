@@ -86,6 +86,7 @@ extern "C" {
 #define PT_ELEM_F32X2 2 /* float2         */
 #define PT_ELEM_F32X3 3 /* float3 tensor  */
 #define PT_ELEM_F32X4 4 /* Color4 / float4 */
+#define PT_ELEM_I32 5   /* int32 object IDs (ImagePyramid<int>) */
 /* DownsampleScaleMode (downsample.h:35-38), same values. */
 #define PT_DOWNSAMPLE_SCALE_WITH_LEVELS 0
 #define PT_DOWNSAMPLE_SCALE_CONSTANT 1
@@ -203,10 +204,14 @@ int pt_flip_anim_buffers(pt_ctx *ctx);
 /* ImagePyramid<T>::initialize (imagePyramid.cpp:81-106) with
  * downsampleAverage (downsample.cpp:41-160) run on the GPU: level L+1 is the
  * 2x2 box mean of level L, times 0.5 more for PT_DOWNSAMPLE_SCALE_WITH_LEVELS.
+ * PT_ELEM_I32 pyramids use downsampleVote (downsample.cpp:162-210) instead:
+ * the most frequent of the 2x2 IDs, a tie going to the first of them in the
+ * order top-left, top-right, bottom-left, bottom-right (the reference breaks
+ * ties by QHash iteration order, which depends on a per-process hash seed).
  * `host_level0` is w*h elements of kind `elem`, tightly packed; NULL fills
  * level 0 with `default_value` in every channel (the absent-input defaults of
- * G/nlDataAccess.cpp:96-155).  The pyramid has min(floor(log2 w),
- * floor(log2 h)) levels. */
+ * G/nlDataAccess.cpp:96-155; converted to int for PT_ELEM_I32).  The pyramid
+ * has min(floor(log2 w), floor(log2 h)) levels. */
 int pt_pyramid_create(pt_ctx *ctx, pt_pyramid **out, int elem, int w, int h,
                       int scale_mode, const void *host_level0, float default_value);
 /* NLDataAccess::toOrientationPyramid (G/nlDataAccess.cpp:82-93): converts a
@@ -221,10 +226,12 @@ int pt_pyramid_dims(const pt_pyramid *pyr, int level, int *w, int *h);
 /* Copies one level to the host (w*h elements, tightly packed). */
 int pt_pyramid_download(const pt_pyramid *pyr, int level, void *host);
 
-/* TexSynth::uploadImage_{float,float2,Color4} (texSynth_kernel.cu:2128-2253):
+/* TexSynth::uploadImage_{float,float2,Color4,int} (texSynth_kernel.cu:2128-2253):
  * binds `level` of `pyr` to image layer `layer`, device to device.  The
  * element kind must match the layer (color/exemplar/random: F32X4, velocity
- * and scale: F32X2, orientation and distance transform: F32). */
+ * and scale: F32X2, orientation and distance transform: F32, INPUT_ID and
+ * EXEMPLAR_OBJECT_IDS: I32).  Until INPUT_ID is uploaded the image ID
+ * texture is a single 0 texel. */
 int pt_upload_image(pt_ctx *ctx, int level, int layer, const pt_pyramid *pyr, int style_layer);
 
 /* SynthesisFrame::createRandomPyramid (synthesisFrame.cpp:66-150) on the GPU
@@ -342,6 +349,8 @@ typedef struct pt_frame_guides {
     const float *scale_uv;          /* w*h*2 or NULL                        */
     const float *vel_b;             /* w*h*2 pixels/frame or NULL           */
     const float *vel_f;             /* w*h*2 pixels/frame or NULL           */
+    const int32_t *object_id;       /* w*h object IDs or NULL (all 0); read only
+                                       when style_obj_id_weight > 0           */
 } pt_frame_guides;
 
 /* The exemplar ("style") pair with its guides.  NULL orientation or distance
@@ -352,6 +361,8 @@ typedef struct pt_style_guides {
     const float *output_rgba;       /* required, w*h*4 exemplar output     */
     const float *orientation_xx_xy_yy; /* w*h*3 or NULL                     */
     const float *dist_trans;        /* w*h or NULL                          */
+    const int32_t *object_ids;      /* w*h object IDs or NULL (all 0); read only
+                                       when style_obj_id_weight > 0           */
 } pt_style_guides;
 
 /*

@@ -64,7 +64,7 @@ struct ptref_pyramid {
 };
 
 typedef struct {
-    ptref_pyramid *color, *orient, *dt, *scale, *velb, *velf;
+    ptref_pyramid *color, *orient, *dt, *scale, *velb, *velf, *id;
 } frame_pyr;
 
 typedef struct {
@@ -95,9 +95,11 @@ struct ptref_ctx {
     float *img_orient; int ow, oh;
     f2 *img_scale; int scw, sch;
     f4 *random; int rw, rh;
+    int32_t *img_id; int idw, idh; /* gImageId, a single 0 texel until uploaded */
     /* exemplar textures, one level (texSynth_kernel.cu:2119-2126) */
     f4 *ex_base, *ex_out;
     float *ex_orient, *ex_dt;
+    int32_t *ex_id;                 /* gExemplarObjectIDs */
     int ew, eh;
     uint32_t scatter_calls;
     uint64_t pde;
@@ -105,7 +107,7 @@ struct ptref_ctx {
     int seq_frame, seq_levels;
     snap_t *seq_snap[2];       /* [frame parity][level] */
     frame_pyr seq_pyr[2];      /* [frame parity] */
-    ptref_pyramid *seq_style[4]; /* base, output, orientation angle, distance transform */
+    ptref_pyramid *seq_style[5]; /* base, output, orientation angle, distance transform, object IDs */
 };
 
 static int fail(ptref_ctx *c, int code, const char *fmt, ...)
@@ -200,6 +202,13 @@ static inline float tex1_point(const float *t, int w, int h, float x, float y)
     int xi = clampi((int)floorf(x + 0.5f), 0, w - 1);
     int yi = clampi((int)floorf(y + 0.5f), 0, h - 1);
     return t[yi * w + xi];
+}
+
+static inline int point_index(int w, int h, float x, float y)
+{
+    int xi = clampi((int)floorf(x + 0.5f), 0, w - 1);
+    int yi = clampi((int)floorf(y + 0.5f), 0, h - 1);
+    return yi * w + xi;
 }
 
 static inline f4 tex4_point(const f4 *t, int w, int h, float x, float y)
@@ -441,6 +450,13 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
                     diff = fminf(fmaxf(diff, 0.0f), 10.0f);
                     res = res + diff * p->dist_trans_weight;
                 }
+                /* objectIdTerm (texSynth_kernel.cu:497-504, 637-638): point
+                 * fetches, the image ID at the pixel, the style's at the tap */
+                if (p->style_obj_id_weight > 0.f) {
+                    int iid = c->img_id[point_index(c->idw, c->idh, (float)x, (float)y)];
+                    int sid = c->ex_id[point_index(c->ew, c->eh, tx, ty)];
+                    if (iid != sid) res = res + p->style_obj_id_weight;
+                }
             } else {
                 res = fmaf(ssd_alpha(canvas, to, p->alpha_weight), p->canvas_output_weight, res);
                 /* coherence term (texSynth_kernel.cu:642-646) */
@@ -557,8 +573,8 @@ int ptref_destroy(ptref_ctx *c)
     if (!c) return PT_OK;
     free_sequence(c);
     free_buffers(c);
-    free(c->img_orient); free(c->img_scale); free(c->random);
-    free(c->ex_base); free(c->ex_out); free(c->ex_orient); free(c->ex_dt);
+    free(c->img_orient); free(c->img_scale); free(c->random); free(c->img_id);
+    free(c->ex_base); free(c->ex_out); free(c->ex_orient); free(c->ex_dt); free(c->ex_id);
     free(c);
     return PT_OK;
 }
@@ -577,8 +593,6 @@ static int check_supported(ptref_ctx *c, const pt_params *p)
         return fail(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
     if (p->lum_shift_enable)
         return fail(c, PT_ERR_UNSUPPORTED, "luminance shift is not supported");
-    if (p->style_obj_id_weight != 0.f)
-        return fail(c, PT_ERR_UNSUPPORTED, "object-id term is not supported");
     if (p->interpolate_key_frame)
         return fail(c, PT_ERR_UNSUPPORTED, "keyframe interpolation is not supported");
     for (int i = 0; i < 5; i++)
@@ -656,6 +670,9 @@ int ptref_initialize(ptref_ctx *c, int iw, int ih, int sw, int sh)
         if (!c->adv[k].ts || !c->img_velf[k]) return fail(c, PT_ERR_OUT_OF_MEMORY, "out of memory");
     }
     if (!c->canvas) return fail(c, PT_ERR_OUT_OF_MEMORY, "out of memory");
+    free(c->img_id); /* a fresh image-ID texture: one 0 texel until uploaded */
+    c->img_id = (int32_t *)calloc(1, sizeof(int32_t));
+    c->idw = c->idh = 1;
     c->cur = 0;
     return PT_OK;
 }
@@ -664,8 +681,9 @@ int ptref_initialize_exemplars(ptref_ctx *c, int w, int h, int num_layers)
 {
     if (!c || w < 1 || h < 1 || num_layers != 1) return PT_ERR_INVALID_ARG;
     if (c->ex_base && c->ew == w && c->eh == h) return PT_OK;
-    free(c->ex_base); free(c->ex_out); free(c->ex_orient); free(c->ex_dt);
+    free(c->ex_base); free(c->ex_out); free(c->ex_orient); free(c->ex_dt); free(c->ex_id);
     c->ew = w; c->eh = h;
+    c->ex_id = (int32_t *)calloc((size_t)w * h, sizeof(int32_t));
     c->ex_base = (f4 *)calloc((size_t)w * h, sizeof(f4));
     c->ex_out = (f4 *)calloc((size_t)w * h, sizeof(f4));
     c->ex_orient = (float *)calloc((size_t)w * h, sizeof(float));
@@ -697,7 +715,26 @@ int ptref_flip_anim_buffers(ptref_ctx *c)
 /* ------------------------------------------------------------------ */
 /* pyramids (imagePyramid.cpp:81-106, downsample.cpp:41-160)           */
 /* ------------------------------------------------------------------ */
-static int elem_channels(int elem) { return (elem >= 1 && elem <= 4) ? elem : 0; }
+static int elem_channels(int elem) { return elem == PT_ELEM_I32 ? 1 : (elem >= 1 && elem <= 4) ? elem : 0; }
+
+/* downsampleVote (downsample.cpp:162-210): the most frequent of the four
+ * IDs; a tie goes to the first of them in scan order (the reference's
+ * winner among ties follows QHash iteration order). */
+static void downsample_vote(const int32_t *prev, int pw, int32_t *dst, int lw, int lh)
+{
+    for (int y = 0; y < lh; y++)
+        for (int x = 0; x < lw; x++) {
+            int32_t v[4] = {prev[(2 * y) * pw + 2 * x], prev[(2 * y) * pw + 2 * x + 1],
+                            prev[(2 * y + 1) * pw + 2 * x], prev[(2 * y + 1) * pw + 2 * x + 1]};
+            int best = 0, best_n = 0;
+            for (int k = 0; k < 4; k++) {
+                int n = 0;
+                for (int m = 0; m < 4; m++) n += v[m] == v[k];
+                if (n > best_n) { best = k; best_n = n; }
+            }
+            dst[y * lw + x] = v[best];
+        }
+}
 
 int ptref_pyramid_create(ptref_ctx *c, ptref_pyramid **out, int elem, int w, int h,
                          int scale_mode, const void *host0, float defv)
@@ -715,7 +752,14 @@ int ptref_pyramid_create(ptref_ctx *c, ptref_pyramid **out, int elem, int w, int
     }
     size_t n0 = (size_t)w * h * ch;
     if (host0) memcpy(p->data[0], host0, n0 * sizeof(float));
+    else if (elem == PT_ELEM_I32) for (size_t i = 0; i < n0; i++) ((int32_t *)p->data[0])[i] = (int32_t)defv;
     else for (size_t i = 0; i < n0; i++) p->data[0][i] = defv;
+    if (elem == PT_ELEM_I32) {
+        for (int l = 1; l < lv; l++)
+            downsample_vote((const int32_t *)p->data[l - 1], p->lw[l - 1], (int32_t *)p->data[l], p->lw[l], p->lh[l]);
+        *out = p;
+        return PT_OK;
+    }
     for (int l = 1; l < lv; l++) {
         int lw = p->lw[l], lh = p->lh[l], pw = p->lw[l - 1];
         const float *prev = p->data[l - 1];
@@ -838,6 +882,16 @@ int ptref_upload_image(ptref_ctx *c, int level, int layer, const ptref_pyramid *
         c->random = (f4 *)realloc_tex((float *)c->random, (size_t)w * h * 4);
         memcpy(c->random, src, sizeof(float) * 4 * w * h);
         c->rw = w; c->rh = h;
+        return PT_OK;
+    case PT_LAYER_INPUT_ID:
+        if (pyr->elem != PT_ELEM_I32) break;
+        c->img_id = (int32_t *)realloc_tex((float *)c->img_id, (size_t)w * h);
+        memcpy(c->img_id, src, sizeof(int32_t) * w * h);
+        c->idw = w; c->idh = h;
+        return PT_OK;
+    case PT_LAYER_EXEMPLAR_OBJECT_IDS:
+        if (pyr->elem != PT_ELEM_I32 || !c->ex_id || w > c->ew || h > c->eh) break;
+        copy_rect((float *)c->ex_id, c->ew, src, w, h, 1);
         return PT_OK;
     case PT_LAYER_EXEMPLAR_BASE:
     case PT_LAYER_EXEMPLAR_OUTPUT:
@@ -1406,7 +1460,9 @@ static void snap_save(ptref_ctx *c, int level, snap_t *s)
     ptref_get_output(c, level, s->off, s->canvas, s->res);
 }
 
-static int build_frame_pyramids(ptref_ctx *c, int w, int h, const pt_frame_guides *g, frame_pyr *fp)
+/* Object-ID pyramids are built only when the term is on (style_obj_id_weight
+ * > 0); an absent ID image is all 0 (NLDataAccess::getInputElement). */
+static int build_frame_pyramids(ptref_ctx *c, int w, int h, const pt_frame_guides *g, frame_pyr *fp, int ids)
 {
     ptref_pyramid *ten = NULL;
     int rc = ptref_pyramid_create(c, &fp->color, PT_ELEM_F32X4, w, h, PT_DOWNSAMPLE_SCALE_CONSTANT, g->color_rgba, 0.f);
@@ -1417,6 +1473,7 @@ static int build_frame_pyramids(ptref_ctx *c, int w, int h, const pt_frame_guide
     if (!rc) rc = ptref_pyramid_create(c, &fp->scale, PT_ELEM_F32X2, w, h, PT_DOWNSAMPLE_SCALE_CONSTANT, g->scale_uv, 1.0f);
     if (!rc) rc = ptref_pyramid_create(c, &fp->velb, PT_ELEM_F32X2, w, h, PT_DOWNSAMPLE_SCALE_WITH_LEVELS, g->vel_b, 0.f);
     if (!rc) rc = ptref_pyramid_create(c, &fp->velf, PT_ELEM_F32X2, w, h, PT_DOWNSAMPLE_SCALE_WITH_LEVELS, g->vel_f, 0.f);
+    if (!rc && ids) rc = ptref_pyramid_create(c, &fp->id, PT_ELEM_I32, w, h, 0, g->object_id, 0.f);
     return rc;
 }
 
@@ -1424,6 +1481,7 @@ static void free_frame_pyramids(frame_pyr *fp)
 {
     ptref_pyramid_destroy(fp->color); ptref_pyramid_destroy(fp->orient); ptref_pyramid_destroy(fp->dt);
     ptref_pyramid_destroy(fp->scale); ptref_pyramid_destroy(fp->velb); ptref_pyramid_destroy(fp->velf);
+    ptref_pyramid_destroy(fp->id);
 }
 
 /* propagateAndScatter (synthesisFrame.cpp:483-508) */
@@ -1458,15 +1516,17 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
     if (rc) return rc;
     c->scatter_calls = 0;
     /* style pyramids (Style::load, style.cpp:62-140) */
-    ptref_pyramid *sb = NULL, *so = NULL, *st = NULL, *sor = NULL, *sdt = NULL;
+    ptref_pyramid *sb = NULL, *so = NULL, *st = NULL, *sor = NULL, *sdt = NULL, *sid = NULL;
+    const int ids = params->style_obj_id_weight > 0.f;
     rc = ptref_pyramid_create(c, &sb, PT_ELEM_F32X4, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT, style->base_rgba, 0.f);
     if (!rc) rc = ptref_pyramid_create(c, &so, PT_ELEM_F32X4, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT, style->output_rgba, 0.f);
     if (!rc) rc = ptref_pyramid_create(c, &st, PT_ELEM_F32X3, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT, style->orientation_xx_xy_yy, 0.f);
     if (!rc) rc = ptref_pyramid_orientation(c, &sor, st);
     if (!rc) rc = ptref_pyramid_create(c, &sdt, PT_ELEM_F32, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT, style->dist_trans, 0.f);
+    if (!rc && ids) rc = ptref_pyramid_create(c, &sid, PT_ELEM_I32, SW, SH, 0, style->object_ids, 0.f);
     ptref_pyramid_destroy(st);
     frame_pyr *fp = (frame_pyr *)calloc(F, sizeof(frame_pyr));
-    for (int f = 0; f < F && !rc; f++) rc = build_frame_pyramids(c, W, H, &frames[f], &fp[f]);
+    for (int f = 0; f < F && !rc; f++) rc = build_frame_pyramids(c, W, H, &frames[f], &fp[f], ids);
     /* FBMO saves its two directed passes, their merge and the fixups; TCTF a
      * consecutive pass and, per level, one refine pass (+ fixups + 1 from
      * three frames), numbered on across levels */
@@ -1486,10 +1546,12 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
             if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);
             if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sor, 0);
             if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sdt, 0);
+            if (!rc && sid) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sid, 0);
             if (!rc) rc = ptref_reset_advected_buffers(c);
             for (int pass = 0; pass < passes && !rc; pass++) {
                 rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[f].color, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[f].orient, 0);
+                if (!rc && fp[f].id) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ID, fp[f].id, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[f].velb, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[f].velf, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[f].dt, 0);
@@ -1539,6 +1601,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sor, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sdt, 0);
+        if (!rc && sid) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sid, 0);
         /* passes 0 and 1: forward, then backward, every frame through
          * synthesizeConsecutive's first-pass branch */
         for (int pass = 0; pass < 2 && !rc; pass++) {
@@ -1550,6 +1613,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
                 if (!is_first || pass > 0) ptref_flip_anim_buffers(c);
                 rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[f].color, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[f].orient, 0);
+                if (!rc && fp[f].id) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ID, fp[f].id, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[f].velb, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[f].velf, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[f].dt, 0);
@@ -1622,6 +1686,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
                     if (g < 0 || g >= F) continue;
                     rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[g].color, 0);
                     if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[g].orient, 0);
+                    if (!rc && fp[g].id) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ID, fp[g].id, 0);
                     if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[g].velb, 0);
                     if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[g].velf, 0);
                     if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[g].dt, 0);
@@ -1642,6 +1707,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
                 }
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[f].color, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[f].orient, 0);
+                if (!rc && fp[f].id) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ID, fp[f].id, 0);
                 if (!rc) rc = ptref_load_offsets(c, level, SNAP(level, f, pass - 1).off);
                 if (!rc) rc = ptref_sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL);
                 if (!rc) rc = propagate_and_scatter(c, level, iterations);
@@ -1671,6 +1737,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
                 if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_EXEMPLAR_OUTPUT, so, 0);            \
                 if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_EXEMPLAR_ORIENTATION, sor, 0);      \
                 if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_EXEMPLAR_DIST_TRANS, sdt, 0);       \
+                if (!rc && sid) rc = ptref_upload_image(c, (l), PT_LAYER_EXEMPLAR_OBJECT_IDS, sid, 0); \
                 passes_last = passes_this;                                                           \
                 last_level = (l);                                                                    \
             }                                                                                        \
@@ -1679,6 +1746,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
         do {                                                                                         \
             if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_COLOR, fp[g].color, 0);           \
             if (!rc) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_ORIENTATION, fp[g].orient, 0);    \
+            if (!rc && fp[g].id) rc = ptref_upload_image(c, (l), PT_LAYER_INPUT_ID, fp[g].id, 0);     \
         } while (0)
 #define LOAD_ANIM(l, g)                                                                              \
         do {                                                                                         \
@@ -1769,6 +1837,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sor, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sdt, 0);
+        if (!rc && sid) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sid, 0);
         if (!rc) rc = ptref_reset_advected_buffers(c);
         for (int pass = 0; pass < passes && !rc; pass++) {
             /* renderConsecutivePass (synthesisProcessor.cpp:703-753), firstPassDirection 0 */
@@ -1789,6 +1858,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
                 /* loadImages / loadAnimImages (synthesisFrame.cpp:562-656) */
                 rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[f].color, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[f].orient, 0);
+                if (!rc && fp[f].id) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ID, fp[f].id, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[f].velb, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[f].velf, 0);
                 if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[f].dt, 0);
@@ -1859,6 +1929,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
     for (int f = 0; f < F; f++) free_frame_pyramids(&fp[f]);
     free(fp);
     ptref_pyramid_destroy(sb); ptref_pyramid_destroy(so); ptref_pyramid_destroy(sor); ptref_pyramid_destroy(sdt);
+    ptref_pyramid_destroy(sid);
     return rc;
 }
 
@@ -1878,7 +1949,7 @@ static void free_sequence(ptref_ctx *c)
         free_frame_pyramids(&c->seq_pyr[k]);
         memset(&c->seq_pyr[k], 0, sizeof(frame_pyr));
     }
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 5; k++) {
         ptref_pyramid_destroy(c->seq_style[k]);
         c->seq_style[k] = NULL;
     }
@@ -1930,6 +2001,8 @@ int ptref_synthesize_next_frame(ptref_ctx *c, const pt_params *params, int itera
         ptref_pyramid_destroy(ten);
         if (!rc) rc = ptref_pyramid_create(c, &c->seq_style[3], PT_ELEM_F32, SW, SH, PT_DOWNSAMPLE_SCALE_CONSTANT,
                                            style->dist_trans, 0.f);
+        if (!rc && params->style_obj_id_weight > 0.f)
+            rc = ptref_pyramid_create(c, &c->seq_style[4], PT_ELEM_I32, SW, SH, 0, style->object_ids, 0.f);
         if (rc) return rc;
     } else if (c->W != W || c->H != H || c->SW != SW || c->SH != SH || c->seq_levels != levels) {
         return fail(c, PT_ERR_INVALID_ARG, "frame shape differs from the sequence");
@@ -1937,7 +2010,7 @@ int ptref_synthesize_next_frame(ptref_ctx *c, const pt_params *params, int itera
     frame_pyr *cur = &c->seq_pyr[f & 1], *prev = &c->seq_pyr[(f + 1) & 1];
     free_frame_pyramids(cur);
     memset(cur, 0, sizeof(frame_pyr));
-    rc = build_frame_pyramids(c, W, H, frame, cur);
+    rc = build_frame_pyramids(c, W, H, frame, cur, params->style_obj_id_weight > 0.f);
     if (!rc) rc = ptref_setup(c, params);
     if (rc) return rc;
     pt_params fp = *params;
@@ -1953,6 +2026,7 @@ int ptref_synthesize_next_frame(ptref_ctx *c, const pt_params *params, int itera
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, sp[1], 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sp[2], 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sp[3], 0);
+        if (!rc && sp[4]) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sp[4], 0);
         if (!rc) rc = ptref_reset_advected_buffers(c);
         if (!rc && f > 0) {
             c->cur = (f + 1) % 2;
@@ -1966,6 +2040,7 @@ int ptref_synthesize_next_frame(ptref_ctx *c, const pt_params *params, int itera
         c->cur = f % 2;
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_COLOR, cur->color, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, cur->orient, 0);
+        if (!rc && cur->id) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_ID, cur->id, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_B, cur->velb, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_VEL_F, cur->velf, 0);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, cur->dt, 0);

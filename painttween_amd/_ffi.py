@@ -20,7 +20,7 @@ PT_DONT_UPDATE_RESIDUAL = 0x10
 PT_ADVECT_IGNORE_RANDOM = 0x1
 PT_ADVECT_DO_NOT_CHECK_PREVIOUS = 0x10
 PT_ADVECT_UPDATE_RESIDUAL = 0x80
-PT_ELEM_F32, PT_ELEM_F32X2, PT_ELEM_F32X3, PT_ELEM_F32X4 = 1, 2, 3, 4
+PT_ELEM_F32, PT_ELEM_F32X2, PT_ELEM_F32X3, PT_ELEM_F32X4, PT_ELEM_I32 = 1, 2, 3, 4, 5
 PT_TUNE_EARLY_EXIT = 1
 PT_TUNE_EARLY_EXIT_MIN_PIXELS = 2
 PT_TUNE_UNIT_PARALLEL = 3
@@ -30,8 +30,8 @@ PT_TUNE_TAP_PARALLEL_EVALS = 6
 PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT = 0, 1
 
 LAYER = dict(
-    EXEMPLAR_BASE=0, EXEMPLAR_OUTPUT=1, EXEMPLAR_ORIENTATION=2, EXEMPLAR_DIST_TRANS=3,
-    INPUT_COLOR=5, INPUT_ORIENTATION=6, INPUT_DIST_TRANS=7, INPUT_SCALE=8,
+    EXEMPLAR_BASE=0, EXEMPLAR_OUTPUT=1, EXEMPLAR_ORIENTATION=2, EXEMPLAR_DIST_TRANS=3, EXEMPLAR_OBJECT_IDS=4,
+    INPUT_COLOR=5, INPUT_ORIENTATION=6, INPUT_DIST_TRANS=7, INPUT_SCALE=8, INPUT_ID=9,
     INPUT_VEL_B=11, INPUT_VEL_F=12, INPUT_VEL_B_PREVIOUS=13, INPUT_VEL_F_PREVIOUS=14,
     RANDOM=21,
 )
@@ -93,15 +93,17 @@ PATCHXF_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("theta", "<f4"), ("hyster
 assert PATCHXF_DTYPE.itemsize == C.sizeof(PatchXF) == 36
 
 FP = C.POINTER(C.c_float)
+IP = C.POINTER(C.c_int32)
 
 
 class FrameGuides(C.Structure):
     _fields_ = [("color_rgba", FP), ("orientation_xx_xy_yy", FP), ("dist_trans", FP),
-                ("scale_uv", FP), ("vel_b", FP), ("vel_f", FP)]
+                ("scale_uv", FP), ("vel_b", FP), ("vel_f", FP), ("object_id", IP)]
 
 
 class StyleGuides(C.Structure):
-    _fields_ = [("base_rgba", FP), ("output_rgba", FP), ("orientation_xx_xy_yy", FP), ("dist_trans", FP)]
+    _fields_ = [("base_rgba", FP), ("output_rgba", FP), ("orientation_xx_xy_yy", FP), ("dist_trans", FP),
+                ("object_ids", IP)]
 
 
 class PtJob(C.Structure):
@@ -147,6 +149,14 @@ def fptr(a):
         return FP()
     assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(FP)
+
+
+def iptr(a):
+    """int32* into a C-contiguous int32 array, or NULL for None."""
+    if a is None:
+        return IP()
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(IP)
 
 
 def vptr(a):

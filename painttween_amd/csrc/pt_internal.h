@@ -49,6 +49,8 @@ struct PtArgs {
     float tdi, tdo, tcw, hyst_w, max_dist, jitter;
     float coh_w, coh_ratio, coh_angle; /* coherence term: weight (0 = off), ratio and angle ranges */
     int lab;           /* colour distances in the LAB space (colorspace 1) */
+    float obj_w;       /* object-ID term weight (0 = off)              */
+    int idw, idh;      /* image object-ID texture size                 */
     int samples, rng_mode;
     uint32_t key0, key1, call;
     int time_active;   /* some advected offset may be valid */
@@ -82,6 +84,8 @@ struct PtArgs {
     const float4 *ex; /* interleaved exemplar: ex[2i] base, ex[2i+1] output */
     const float *ex_orient;
     const float *ex_dt;
+    const int *img_id; /* image object IDs, idw x idh                  */
+    const int *ex_id;  /* exemplar object IDs, pitch ew                */
 };
 
 /* Operators with several kernel shapes and a multi-job form. */
@@ -146,6 +150,7 @@ hipError_t pt_launch_merge_randomly(const PtArgs &a, int forwards, float weight,
 hipError_t pt_launch_merge_residual(const PtArgs &a, int forwards, hipStream_t s);
 hipError_t pt_launch_downsample(const float *src, int pw, float *dst, int lw, int lh, int ch,
                                 int scale_mode, hipStream_t s);
+hipError_t pt_launch_downsample_vote(const int *src, int pw, int *dst, int lw, int lh, hipStream_t s);
 hipError_t pt_launch_fill(float *dst, size_t n, float v, hipStream_t s);
 hipError_t pt_launch_tensor_angle(const float *tensor, float *angle, size_t n, hipStream_t s);
 hipError_t pt_launch_interleave(const float4 *src, int w, int h, float4 *ex, int ew, int slot, hipStream_t s);

@@ -317,7 +317,7 @@ __device__ __forceinline__ float coherence_add(const PtArgs &A, float res, int x
 
 /*
  * RT < 0 selects the variant of the per-pixel kernels with the extended
- * terms (coherence, LAB colour space) at a run-time radius; the default
+ * terms (coherence, LAB colour space, object IDs) at a run-time radius; the default
  * kernels (RT >= 0) leave them out and keep their registers.
  *
  * calculateResidual (texSynth_kernel.cu:537-659): one patch distance of
@@ -403,6 +403,13 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
                     float diff = fabsf(de - di);
                     diff = fminf(fmaxf(diff, 0.0f), 10.0f);
                     res = res + diff * A.dt_w;
+                }
+                /* objectIdTerm (texSynth_kernel.cu:497-504, 637-638): point
+                 * fetches, the image ID at the pixel, the style's at the tap */
+                if (RT < 0 && A.obj_w > 0.f) {
+                    int iid = A.img_id[point_index(A.idw, A.idh, (float)x, (float)y)];
+                    int sid = A.ex_id[point_index(A.ew, A.eh, tx, ty)];
+                    if (iid != sid) res = res + A.obj_w;
                 }
             } else {
                 res = fmaf(ssd_alpha(canvas, to, A.alpha_w), A.canvas_w, res);
@@ -2038,6 +2045,27 @@ __global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample(const float *src, i
     }
 }
 
+/* downsampleVote (downsample.cpp:162-210) for object-ID pyramids: the most
+ * frequent of the four IDs, a tie going to the first in scan order. */
+__global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_vote(const int *src, int pw, int *dst, int lw, int lh)
+{
+    int x = blockIdx.x * PT_RX + threadIdx.x;
+    int y = blockIdx.y * PT_RY + threadIdx.y;
+    if (x >= lw || y >= lh) return;
+    int v[4] = {src[(y * 2) * pw + x * 2], src[(y * 2) * pw + x * 2 + 1], src[(y * 2 + 1) * pw + x * 2],
+                src[(y * 2 + 1) * pw + x * 2 + 1]};
+    int best = v[0], best_n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int n = (v[0] == v[k]) + (v[1] == v[k]) + (v[2] == v[k]) + (v[3] == v[k]);
+        if (n > best_n) {
+            best = v[k];
+            best_n = n;
+        }
+    }
+    dst[y * lw + x] = best;
+}
+
 __global__ void k_fill(float *dst, size_t n, float v)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2516,7 +2544,7 @@ __global__ __launch_bounds__(PT_UP_THREADS) void km_scatter_up(PT_MULTI_PARAMS)
 /* ------------------------------------------------------------------ */
 /* the extended patch-distance terms (coherence, LAB) run in the per-pixel
  * kernels' RT = -1 variant */
-static bool ext_terms(const PtArgs &a) { return a.coh_w > 0.f || a.lab; }
+static bool ext_terms(const PtArgs &a) { return a.coh_w > 0.f || a.lab || a.obj_w > 0.f; }
 
 static dim3 grid_for(int w, int h) { return dim3((w + PT_BX - 1) / PT_BX, (h + PT_BY - 1) / PT_BY); }
 static const dim3 kBlock(PT_BX, PT_BY);
@@ -2655,9 +2683,9 @@ void pt_tap_configure(PtArgs &a, int evals)
 int pt_shape_valid(int op, int shape, const PtArgs &a)
 {
     if (shape < 0 || shape >= PT_SHAPE_COUNT) return 0;
-    /* the coherence term and the LAB colour space are evaluated by the
-     * per-pixel in-order kernels only (their RT < 0 variant) */
-    if ((a.coh_w > 0.f || a.lab) && shape != PT_SHAPE_PIXEL) return 0;
+    /* the coherence term, the LAB colour space and the object-ID term are
+     * evaluated by the per-pixel in-order kernels only (their RT < 0 variant) */
+    if (ext_terms(a) && shape != PT_SHAPE_PIXEL) return 0;
     if (shape == PT_SHAPE_TAP) return tap_pixels(op, a) > 0;
     if (op == PT_OP_RESIDUAL || op == PT_OP_SYNC_CANVAS) return shape == PT_SHAPE_PIXEL;
     if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return 0;
@@ -2896,6 +2924,12 @@ hipError_t pt_launch_downsample(const float *src, int pw, float *dst, int lw, in
     case 4: hipLaunchKernelGGL(k_downsample_v<4>, g, b, 0, s, src, pw, dst, lw, lh, scale_mode); break;
     default: hipLaunchKernelGGL(k_downsample, g, b, 0, s, src, pw, dst, lw, lh, ch, scale_mode); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_downsample_vote(const int *src, int pw, int *dst, int lw, int lh, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_downsample_vote, grid_rows(lw, lh), kRowBlock, 0, s, src, pw, dst, lw, lh);
     return hipGetLastError();
 }
 

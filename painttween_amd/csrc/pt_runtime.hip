@@ -50,13 +50,14 @@ struct DevAdv {
     int32_t *ts = nullptr;
 };
 
+/* `id` (object IDs) exists only while the object-ID term is on */
 struct FramePyr {
     pt_pyramid *color = nullptr, *tensor = nullptr, *orient = nullptr, *dt = nullptr, *scale = nullptr,
-               *velb = nullptr, *velf = nullptr;
+               *velb = nullptr, *velf = nullptr, *id = nullptr;
 };
 
 struct StylePyr {
-    pt_pyramid *base = nullptr, *out = nullptr, *tensor = nullptr, *orient = nullptr, *dt = nullptr;
+    pt_pyramid *base = nullptr, *out = nullptr, *tensor = nullptr, *orient = nullptr, *dt = nullptr, *id = nullptr;
 };
 
 /* Kernel classes timed by pt_profile_*; names in kProfNames. */
@@ -140,9 +141,12 @@ struct pt_ctx {
     int scw = 0, sch = 0;
     float4 *random = nullptr;
     int rw = 0, rh = 0;
+    int32_t *img_id = nullptr; /* object IDs, idw x idh: one 0 texel until uploaded */
+    int idw = 0, idh = 0;
     /* exemplar textures (capacity ex_cap texels) */
     float4 *ex = nullptr;
     float *ex_orient = nullptr, *ex_dt = nullptr;
+    int32_t *ex_id = nullptr;
     size_t ex_cap = 0;
     int ew = 0, eh = 0;
     uint32_t scatter_calls = 0;
@@ -220,7 +224,7 @@ template <class T> static void dfree(T *&p)
     p = nullptr;
 }
 
-static int elem_channels(int elem) { return (elem >= 1 && elem <= 4) ? elem : 0; }
+static int elem_channels(int elem) { return elem == PT_ELEM_I32 ? 1 : (elem >= 1 && elem <= 4) ? elem : 0; }
 
 static int check_supported(pt_ctx *c, const pt_params *p)
 {
@@ -232,7 +236,6 @@ static int check_supported(pt_ctx *c, const pt_params *p)
     if (p->colorspace != 0 && p->colorspace != 1)
         return set_err(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
     if (p->lum_shift_enable) return set_err(c, PT_ERR_UNSUPPORTED, "luminance shift is not supported");
-    if (p->style_obj_id_weight != 0.f) return set_err(c, PT_ERR_UNSUPPORTED, "object-id term is not supported");
     if (p->interpolate_key_frame)
         return set_err(c, PT_ERR_UNSUPPORTED, "keyframe interpolation is not supported");
     for (int i = 0; i < 5; i++)
@@ -359,6 +362,7 @@ static void free_working(pt_ctx *c)
     dfree(c->orient);
     dfree(c->scale);
     dfree(c->random);
+    dfree(c->img_id);
 }
 
 extern "C" int pt_destroy(pt_ctx *c)
@@ -376,6 +380,7 @@ extern "C" int pt_destroy(pt_ctx *c)
     dfree(c->ex);
     dfree(c->ex_orient);
     dfree(c->ex_dt);
+    dfree(c->ex_id);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return PT_OK;
@@ -640,6 +645,9 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
     RC_CHECK(dalloc(c, &c->orient, n));
     RC_CHECK(dalloc(c, &c->scale, n));
     RC_CHECK(dalloc(c, &c->random, n));
+    RC_CHECK(dalloc(c, &c->img_id, n));
+    RC_CHECK(zero_fill(c, c->img_id, sizeof(int32_t)));
+    c->idw = c->idh = 1;
     c->ow = c->oh = c->scw = c->sch = c->rw = c->rh = 0;
     c->off_diff = Rect();
     c->res_diff = Rect();
@@ -663,15 +671,18 @@ extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
         dfree(c->ex);
         dfree(c->ex_orient);
         dfree(c->ex_dt);
+        dfree(c->ex_id);
         RC_CHECK(dalloc(c, &c->ex, 2 * n));
         RC_CHECK(dalloc(c, &c->ex_orient, n));
         RC_CHECK(dalloc(c, &c->ex_dt, n));
+        RC_CHECK(dalloc(c, &c->ex_id, n));
         c->ex_cap = n;
     } else {
         /* a fresh texture allocation of the new size: zero it */
         RC_CHECK(zero_fill(c, c->ex, 2 * n * sizeof(float4)));
         RC_CHECK(zero_fill(c, c->ex_orient, n * sizeof(float)));
         RC_CHECK(zero_fill(c, c->ex_dt, n * sizeof(float)));
+        RC_CHECK(zero_fill(c, c->ex_id, n * sizeof(int32_t)));
     }
     c->ew = w;
     c->eh = h;
@@ -715,6 +726,7 @@ extern "C" int pt_flip_anim_buffers(pt_ctx *c)
 static int pyramid_rebuild(pt_ctx *c, pt_pyramid *p, int scale_mode)
 {
     int ch = elem_channels(p->elem);
+    const bool vote = p->elem == PT_ELEM_I32;
     for (int l = 1; l < p->levels; l++) {
         const float *src = p->data + p->off[l - 1];
         float *dst = p->data + p->off[l];
@@ -722,11 +734,23 @@ static int pyramid_rebuild(pt_ctx *c, pt_pyramid *p, int scale_mode)
         /* a 2x2 block read and one element written per output pixel */
         int pr = prof_begin(c, PK_PYRAMID, (uint64_t)lw * lh, (uint64_t)lw * lh * ch * 4 * 5);
         RC_CHECK(issue(c, "downsample", [=](hipStream_t s) {
+            if (vote) return pt_launch_downsample_vote((const int *)src, pw, (int *)dst, lw, lh, s);
             return pt_launch_downsample(src, pw, dst, lw, lh, ch, scale_mode, s);
         }));
         prof_end(c, pr);
     }
     return PT_OK;
+}
+
+/* The fill pattern of a default level 0: the value itself, or for object-ID
+ * pyramids the bits of (int)v. */
+static float fill_bits(int elem, float v)
+{
+    if (elem != PT_ELEM_I32) return v;
+    int32_t i = (int32_t)v;
+    float f;
+    memcpy(&f, &i, sizeof(f));
+    return f;
 }
 
 static int pyramid_alloc(pt_ctx *c, pt_pyramid **out, int elem, int w, int h)
@@ -765,7 +789,7 @@ extern "C" int pt_pyramid_create(pt_ctx *c, pt_pyramid **out, int elem, int w, i
     RC_CHECK(pyramid_alloc(c, &p, elem, w, h));
     size_t n0 = (size_t)w * h * elem_channels(elem);
     hipError_t e = host0 ? hipMemcpyAsync(p->data, host0, n0 * sizeof(float), hipMemcpyHostToDevice, c->stream)
-                         : pt_launch_fill(p->data, n0, defv, c->stream);
+                         : pt_launch_fill(p->data, n0, fill_bits(elem, defv), c->stream);
     if (e != hipSuccess) {
         pt_pyramid_destroy(p);
         return set_err(c, PT_ERR_HIP, "level-0 upload failed: %s", hipGetErrorString(e));
@@ -892,6 +916,20 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
         c->rw = w;
         c->rh = h;
         return tight(c->random, 4);
+    case PT_LAYER_INPUT_ID:
+        if (p->elem != PT_ELEM_I32 || w > c->W || h > c->H) break;
+        c->idw = w;
+        c->idh = h;
+        return tight(c->img_id, 1);
+    case PT_LAYER_EXEMPLAR_OBJECT_IDS: {
+        if (p->elem != PT_ELEM_I32 || !c->ex || w > c->ew || h > c->eh) break;
+        int32_t *dst = c->ex_id;
+        int ew = c->ew;
+        return issue(c, "exemplar ID upload", [=](hipStream_t s) {
+            return hipMemcpy2DAsync(dst, (size_t)ew * sizeof(int32_t), src, (size_t)w * sizeof(int32_t),
+                                    (size_t)w * sizeof(int32_t), h, hipMemcpyDeviceToDevice, s);
+        });
+    }
     case PT_LAYER_EXEMPLAR_BASE:
     case PT_LAYER_EXEMPLAR_OUTPUT:
         if (p->elem != 4 || !c->ex || w > c->ew || h > c->eh) break;
@@ -1023,6 +1061,9 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.coh_ratio = p.coherence_ratio_range;
     a.coh_angle = p.coherence_angle_range;
     a.lab = p.colorspace == 1 ? 1 : 0;
+    a.obj_w = p.style_obj_id_weight > 0.f ? p.style_obj_id_weight : 0.f;
+    a.idw = c->idw;
+    a.idh = c->idh;
     a.hyst_w = p.hysteresis_weight;
     a.max_dist = p.max_distortion;
     a.jitter = p.advection_jitter;
@@ -1065,6 +1106,8 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.ex = c->ex;
     a.ex_orient = c->ex_orient;
     a.ex_dt = c->ex_dt;
+    a.img_id = c->img_id;
+    a.ex_id = c->ex_id;
     return a;
 }
 
@@ -1525,7 +1568,8 @@ static int load_style_level(pt_ctx *c, int level, const StylePyr &sp)
     RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sp.base, 0));
     RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, sp.out, 0));
     RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sp.orient, 0));
-    return upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sp.dt, 0);
+    RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sp.dt, 0));
+    return sp.id ? upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sp.id, 0) : PT_OK;
 }
 
 /* SynthesisFrame::loadImages / loadAnimImages (synthesisFrame.cpp:562-656):
@@ -1534,6 +1578,7 @@ static int load_frame_level(pt_ctx *c, int level, const FramePyr &f)
 {
     RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_COLOR, f.color, 0));
     RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, f.orient, 0));
+    if (f.id) RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ID, f.id, 0));
     RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_B, f.velb, 0));
     RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_F, f.velf, 0));
     RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, f.dt, 0));
@@ -1550,12 +1595,13 @@ static int scheme_passes(int scheme, int fixup_passes, int levels, int F)
     return scheme == PT_S_FBMO ? fixup_passes + 3 : fixup_passes + 1;
 }
 
-/* SynthesisFrame::loadImages (synthesisFrame.cpp:562-578): colour and
- * orientation of frame f into the current layer */
+/* SynthesisFrame::loadImages (synthesisFrame.cpp:562-578): colour,
+ * orientation and object IDs of frame f into the current layer */
 static int load_images(pt_ctx *c, int level, const FramePyr &f)
 {
     RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_COLOR, f.color, 0));
-    return upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, f.orient, 0);
+    RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, f.orient, 0));
+    return f.id ? upload_image(c, level, PT_LAYER_INPUT_ID, f.id, 0) : PT_OK;
 }
 
 /* SynthesisFrame::loadAnimImages (synthesisFrame.cpp:581-656), time step 1:
@@ -1880,6 +1926,7 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
         RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, sp.out, 0));
         RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sp.orient, 0));
         RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sp.dt, 0));
+        if (sp.id) RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sp.id, 0));
         RC_CHECK(reset_advected(c));
         for (int pass = 0; pass < passes; pass++) {
             /* renderConsecutivePass (synthesisProcessor.cpp:703-753), firstPassDirection 0 */
@@ -1898,6 +1945,7 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
                 /* loadImages / loadAnimImages (synthesisFrame.cpp:562-656) */
                 RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_COLOR, fp[f].color, 0));
                 RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, fp[f].orient, 0));
+                if (fp[f].id) RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ID, fp[f].id, 0));
                 RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_B, fp[f].velb, 0));
                 RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_F, fp[f].velf, 0));
                 RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, fp[f].dt, 0));
@@ -1959,6 +2007,7 @@ static void free_frame(FramePyr &f)
     pt_pyramid_destroy(f.scale);
     pt_pyramid_destroy(f.velb);
     pt_pyramid_destroy(f.velf);
+    pt_pyramid_destroy(f.id);
     f = FramePyr();
 }
 
@@ -1969,6 +2018,7 @@ static void free_style(StylePyr &sp)
     pt_pyramid_destroy(sp.orient);
     pt_pyramid_destroy(sp.tensor);
     pt_pyramid_destroy(sp.dt);
+    pt_pyramid_destroy(sp.id);
     sp = StylePyr();
 }
 
@@ -1984,7 +2034,8 @@ static int pyramid_fill(pt_ctx *c, pt_pyramid *p, const float *src, int on_devic
         RC_CHECK(issue(c, "pyramid level-0 copy",
                        [=](hipStream_t s) { return hipMemcpyAsync(dst, src, n0 * sizeof(float), kind, s); }));
     } else {
-        RC_CHECK(issue(c, "pyramid fill", [=](hipStream_t s) { return pt_launch_fill(dst, n0, defv, s); }));
+        float v = fill_bits(p->elem, defv);
+        RC_CHECK(issue(c, "pyramid fill", [=](hipStream_t s) { return pt_launch_fill(dst, n0, v, s); }));
     }
     return pyramid_rebuild(c, p, scale_mode);
 }
@@ -2001,6 +2052,8 @@ static int orientation_fill(pt_ctx *c, pt_pyramid *tensor, pt_pyramid *angle, co
     return PT_OK;
 }
 
+static bool wants_ids(const pt_params *p) { return p->style_obj_id_weight > 0.f; }
+
 static int ensure_pyr(pt_ctx *c, pt_pyramid **p, int elem, int w, int h)
 {
     if (*p && (*p)->w == w && (*p)->h == h && (*p)->elem == elem) return PT_OK;
@@ -2009,7 +2062,20 @@ static int ensure_pyr(pt_ctx *c, pt_pyramid **p, int elem, int w, int h)
     return pyramid_alloc(c, p, elem, w, h);
 }
 
-static int prepare_frame(pt_ctx *c, int w, int h, const pt_frame_guides &g, int on_device, FramePyr &fp)
+/* Object-ID pyramids are kept only while the term is on (`ids`); an absent
+ * ID image is all 0 (NLDataAccess::getInputElement). */
+static int prepare_ids(pt_ctx *c, int w, int h, const int32_t *src, int on_device, bool ids, pt_pyramid **p)
+{
+    if (!ids) {
+        pt_pyramid_destroy(*p);
+        *p = nullptr;
+        return PT_OK;
+    }
+    RC_CHECK(ensure_pyr(c, p, PT_ELEM_I32, w, h));
+    return pyramid_fill(c, *p, (const float *)src, on_device, 0.f, PT_DOWNSAMPLE_SCALE_CONSTANT);
+}
+
+static int prepare_frame(pt_ctx *c, int w, int h, const pt_frame_guides &g, int on_device, FramePyr &fp, bool ids)
 {
     if (!g.color_rgba) return set_err(c, PT_ERR_INVALID_ARG, "frame colour is required");
     RC_CHECK(ensure_pyr(c, &fp.color, PT_ELEM_F32X4, w, h));
@@ -2025,10 +2091,10 @@ static int prepare_frame(pt_ctx *c, int w, int h, const pt_frame_guides &g, int 
     RC_CHECK(pyramid_fill(c, fp.scale, g.scale_uv, on_device, 1.0f, PT_DOWNSAMPLE_SCALE_CONSTANT));
     RC_CHECK(pyramid_fill(c, fp.velb, g.vel_b, on_device, 0.f, PT_DOWNSAMPLE_SCALE_WITH_LEVELS));
     RC_CHECK(pyramid_fill(c, fp.velf, g.vel_f, on_device, 0.f, PT_DOWNSAMPLE_SCALE_WITH_LEVELS));
-    return PT_OK;
+    return prepare_ids(c, w, h, g.object_id, on_device, ids, &fp.id);
 }
 
-static int prepare_style(pt_ctx *c, int w, int h, const pt_style_guides &g, int on_device, StylePyr &sp)
+static int prepare_style(pt_ctx *c, int w, int h, const pt_style_guides &g, int on_device, StylePyr &sp, bool ids)
 {
     if (!g.base_rgba || !g.output_rgba) return set_err(c, PT_ERR_INVALID_ARG, "exemplar pair required");
     RC_CHECK(ensure_pyr(c, &sp.base, PT_ELEM_F32X4, w, h));
@@ -2040,7 +2106,7 @@ static int prepare_style(pt_ctx *c, int w, int h, const pt_style_guides &g, int 
     RC_CHECK(pyramid_fill(c, sp.out, g.output_rgba, on_device, 0.f, PT_DOWNSAMPLE_SCALE_CONSTANT));
     RC_CHECK(orientation_fill(c, sp.tensor, sp.orient, g.orientation_xx_xy_yy, on_device));
     RC_CHECK(pyramid_fill(c, sp.dt, g.dist_trans, on_device, 0.f, PT_DOWNSAMPLE_SCALE_CONSTANT));
-    return PT_OK;
+    return prepare_ids(c, w, h, g.object_ids, on_device, ids, &sp.id);
 }
 
 /* One level-0 snapshot to the caller's buffers (each may be NULL). */
@@ -2153,12 +2219,12 @@ static int synthesize_issue(pt_ctx *c, const pt_params *params, int scheme, int 
                             float *out_canvas, float *out_res, int outputs_on_device)
 {
     RC_CHECK(reset_working(c));
-    RC_CHECK(prepare_style(c, SW, SH, *style, guides_on_device, c->style_pyr));
+    RC_CHECK(prepare_style(c, SW, SH, *style, guides_on_device, c->style_pyr, wants_ids(params)));
     if ((int)c->frame_pyrs.size() > F) {
         for (size_t i = F; i < c->frame_pyrs.size(); i++) free_frame(c->frame_pyrs[i]);
     }
     c->frame_pyrs.resize(F);
-    for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], guides_on_device, c->frame_pyrs[f]));
+    for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], guides_on_device, c->frame_pyrs[f], wants_ids(params)));
     RC_CHECK(run_schedule(c, params, scheme, iterations, fixup_passes, F, c->frame_pyrs.data(), c->style_pyr));
     return read_results(c, F, scheme_passes(scheme, fixup_passes, params->levels, F) - 1, out_off, out_canvas, out_res,
                         outputs_on_device);
@@ -2211,11 +2277,11 @@ int pt_record_synthesis(pt_ctx *c, std::vector<RecOp> *ops, const pt_params *par
     {
         std::vector<RecOp> scratch;
         RC_CHECK(record_into(c, &scratch, [&] {
-            RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr));
+            RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr, wants_ids(params)));
             if ((int)c->frame_pyrs.size() > F)
                 for (size_t i = F; i < c->frame_pyrs.size(); i++) free_frame(c->frame_pyrs[i]);
             c->frame_pyrs.resize(F);
-            for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], 1, c->frame_pyrs[f]));
+            for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], 1, c->frame_pyrs[f], wants_ids(params)));
             return PT_OK;
         }));
     }
@@ -2321,6 +2387,7 @@ static int next_frame_schedule(pt_ctx *c, const pt_params *params, int iteration
         RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_OUTPUT, sp.out, 0));
         RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_ORIENTATION, sp.orient, 0));
         RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_DIST_TRANS, sp.dt, 0));
+        if (sp.id) RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sp.id, 0));
         RC_CHECK(reset_advected(c));
         if (f > 0) {
             c->cur = (f + 1) % 2;
@@ -2334,6 +2401,7 @@ static int next_frame_schedule(pt_ctx *c, const pt_params *params, int iteration
         c->cur = f % 2;
         RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_COLOR, cur.color, 0));
         RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ORIENTATION, cur.orient, 0));
+        if (cur.id) RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_ID, cur.id, 0));
         RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_B, cur.velb, 0));
         RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_VEL_F, cur.velf, 0));
         RC_CHECK(upload_image(c, level, PT_LAYER_INPUT_DIST_TRANS, cur.dt, 0));
@@ -2372,9 +2440,9 @@ static int next_frame_issue(pt_ctx *c, const pt_params *params, int iterations, 
     const int f = c->seq_frame;
     if (f == 0) {
         RC_CHECK(reset_working(c));
-        RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr));
+        RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr, wants_ids(params)));
     }
-    RC_CHECK(prepare_frame(c, W, H, *frame, 1, c->seq_pyr[f & 1]));
+    RC_CHECK(prepare_frame(c, W, H, *frame, 1, c->seq_pyr[f & 1], wants_ids(params)));
     RC_CHECK(next_frame_schedule(c, params, iterations, c->seq_pyr[f & 1], f > 0 ? &c->seq_pyr[(f + 1) & 1] : nullptr,
                                  c->style_pyr));
     /* results: this frame's level-0 snapshot */
@@ -2401,14 +2469,14 @@ static int next_frame_check(pt_ctx *c, const pt_params *params, int iterations, 
 
 /* Allocates this frame's pyramids (and on frame 0 the style's) without
  * issuing anything. */
-static int next_frame_pyramids(pt_ctx *c, int W, int H, const pt_frame_guides *frame, int SW, int SH,
-                               const pt_style_guides *style)
+static int next_frame_pyramids(pt_ctx *c, const pt_params *params, int W, int H, const pt_frame_guides *frame,
+                               int SW, int SH, const pt_style_guides *style)
 {
     std::vector<RecOp> scratch;
     const int f = c->seq_frame;
     return record_into(c, &scratch, [&] {
-        if (f == 0) RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr));
-        return prepare_frame(c, W, H, *frame, 1, c->seq_pyr[f & 1]);
+        if (f == 0) RC_CHECK(prepare_style(c, SW, SH, *style, 1, c->style_pyr, wants_ids(params)));
+        return prepare_frame(c, W, H, *frame, 1, c->seq_pyr[f & 1], wants_ids(params));
     });
 }
 
@@ -2418,7 +2486,7 @@ int pt_record_next_frame(pt_ctx *c, std::vector<RecOp> *ops, const pt_params *pa
 {
     if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
     RC_CHECK(next_frame_check(c, params, iterations, W, H, frame, SW, SH, style));
-    RC_CHECK(next_frame_pyramids(c, W, H, frame, SW, SH, style));
+    RC_CHECK(next_frame_pyramids(c, params, W, H, frame, SW, SH, style));
     return record_into(c, ops, [&] {
         return next_frame_issue(c, params, iterations, W, H, frame, SW, SH, style, out_off, out_canvas, out_res);
     });

@@ -33,21 +33,25 @@ class DeviceImages:
         self.dev = torch.device("cuda:%d" % device)
         self.keep = []
 
-    def ptr(self, a):
+    def ptr(self, a, dt=np.float32, kind=_ffi.FP):
         if a is None:
-            return _ffi.FP()
+            return kind()
         torch = _torch()
-        t = torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(self.dev)
+        t = torch.from_numpy(np.ascontiguousarray(a, dt)).to(self.dev)
         self.keep.append(t)
-        return C.cast(C.c_void_p(t.data_ptr()), _ffi.FP)
+        return C.cast(C.c_void_p(t.data_ptr()), kind)
+
+    def ids(self, a):
+        return self.ptr(a, np.int32, _ffi.IP)
 
     def frame(self, f):
         return FrameGuides(self.ptr(f["color"]), self.ptr(f.get("orientation")), self.ptr(f.get("dist_trans")),
-                           self.ptr(f.get("scale")), self.ptr(f.get("vel_b")), self.ptr(f.get("vel_f")))
+                           self.ptr(f.get("scale")), self.ptr(f.get("vel_b")), self.ptr(f.get("vel_f")),
+                           self.ids(f.get("object_id")))
 
     def style(self, s):
         return StyleGuides(self.ptr(s["base"]), self.ptr(s["output"]), self.ptr(s.get("orientation")),
-                           self.ptr(s.get("dist_trans")))
+                           self.ptr(s.get("dist_trans")), self.ids(s.get("object_ids")))
 
 
 class DeviceOutputs:

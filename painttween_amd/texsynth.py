@@ -37,7 +37,7 @@ def default_params(lib=None, prefix="pt_", **overrides):
 class Pyramid:
     """Device- (or host-) resident ImagePyramid<T> (imagePyramid.h)."""
 
-    CHANNELS = {1: 1, 2: 2, 3: 3, 4: 4}
+    CHANNELS = {1: 1, 2: 2, 3: 3, 4: 4, 5: 1}
 
     def __init__(self, ctx, handle, elem):
         self.ctx, self.handle, self.elem = ctx, handle, elem
@@ -55,7 +55,8 @@ class Pyramid:
     def download(self, level):
         w, h = self.dims(level)
         ch = self.CHANNELS[self.elem]
-        out = np.empty((h, w, ch) if ch > 1 else (h, w), np.float32)
+        dt = np.int32 if self.elem == _ffi.PT_ELEM_I32 else np.float32
+        out = np.empty((h, w, ch) if ch > 1 else (h, w), dt)
         self.ctx._check(self.ctx._fn("pyramid_download")(self.handle, level, _ffi.vptr(out)))
         return out
 
@@ -141,10 +142,11 @@ class Context:
         self.call("flip_anim_buffers")
 
     def pyramid(self, elem, level0, scale_mode=_ffi.PT_DOWNSAMPLE_SCALE_CONSTANT, default=0.0, shape=None):
-        """Build a pyramid from a level-0 float32 array (H, W[, C]) or, with
-        level0=None and shape=(W, H), from a constant default."""
+        """Build a pyramid from a level-0 float32 array (H, W[, C]) -- int32
+        (H, W) for PT_ELEM_I32 -- or, with level0=None and shape=(W, H), from a
+        constant default."""
         if level0 is not None:
-            level0 = np.ascontiguousarray(level0, np.float32)
+            level0 = np.ascontiguousarray(level0, np.int32 if elem == _ffi.PT_ELEM_I32 else np.float32)
             h, w = level0.shape[:2]
         else:
             w, h = shape
@@ -227,16 +229,16 @@ class Context:
     def synthesize(self, params, frames, style, scheme=_ffi.PT_S_FMBM_RM, iterations=3, fixup_passes=1,
                    want=("offsets", "canvas", "residual")):
         """Run a schedule (PT_S_FMBM, _FMBM_RM, _FBMO, _TCTF, _IND) over `frames` (list of dicts with
-        keys color, orientation, dist_trans, scale, vel_b, vel_f; arrays are
-        (H, W, C) float32 or None) against `style` (dict: base, output,
-        orientation, dist_trans).  Returns level-0 results of the final pass,
-        one entry per frame."""
+        keys color, orientation, dist_trans, scale, vel_b, vel_f, object_id; arrays are
+        (H, W, C) float32, int32 (H, W) object IDs, or None) against `style` (dict: base,
+        output, orientation, dist_trans, object_ids).  Returns level-0 results of the
+        final pass, one entry per frame."""
         keep = []
 
-        def arr(a):
+        def arr(a, dt=np.float32):
             if a is None:
                 return None
-            a = np.ascontiguousarray(a, np.float32)
+            a = np.ascontiguousarray(a, dt)
             keep.append(a)
             return a
 
@@ -247,9 +249,11 @@ class Context:
         for i, f in enumerate(frames):
             fg[i] = _ffi.FrameGuides(_ffi.fptr(arr(f["color"])), _ffi.fptr(arr(f.get("orientation"))),
                                      _ffi.fptr(arr(f.get("dist_trans"))), _ffi.fptr(arr(f.get("scale"))),
-                                     _ffi.fptr(arr(f.get("vel_b"))), _ffi.fptr(arr(f.get("vel_f"))))
+                                     _ffi.fptr(arr(f.get("vel_b"))), _ffi.fptr(arr(f.get("vel_f"))),
+                                     _ffi.iptr(arr(f.get("object_id"), np.int32)))
         sg = _ffi.StyleGuides(_ffi.fptr(arr(style["base"])), _ffi.fptr(arr(style["output"])),
-                              _ffi.fptr(arr(style.get("orientation"))), _ffi.fptr(arr(style.get("dist_trans"))))
+                              _ffi.fptr(arr(style.get("orientation"))), _ffi.fptr(arr(style.get("dist_trans"))),
+                              _ffi.iptr(arr(style.get("object_ids"), np.int32)))
         off = np.zeros((F, h, w), PATCHXF_DTYPE) if "offsets" in want else None
         canvas = np.zeros((F, h, w, 4), np.float32) if "canvas" in want else None
         res = np.zeros((F, h, w), np.float32) if "residual" in want else None

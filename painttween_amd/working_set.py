@@ -233,9 +233,11 @@ class WorkingSet:
         """The guides of `frame` in the form Context.synthesize takes."""
         get = self.read_input
         dt = get("dist_trans", frame)
+        ids = get("id", frame)
         return {"color": get("input", frame), "orientation": get("orientation", frame),
                 "dist_trans": None if dt is None else dt[..., 0], "scale": get("scale", frame),
-                "vel_b": get("vel_b", frame), "vel_f": get("vel_f", frame)}
+                "vel_b": get("vel_b", frame), "vel_f": get("vel_f", frame),
+                "object_id": None if ids is None else ids_from_rgb(ids, style=False)}
 
     def read_style(self, index=0):
         """Style `index` in the form Context.synthesize takes."""
@@ -245,8 +247,10 @@ class WorkingSet:
             if key in st:
                 out[key] = _read_layer(st[key], layer, chans)
         dt = out.get("dist_trans")
+        ids = out.get("object_ids")
         return {"base": out["input"], "output": out["output"], "orientation": out.get("orientation"),
-                "dist_trans": None if dt is None else dt[..., 0]}
+                "dist_trans": None if dt is None else dt[..., 0],
+                "object_ids": None if ids is None else ids_from_rgb(ids, style=True)}
 
     # -- writing ---------------------------------------------------------
     def save(self, path):
@@ -289,6 +293,17 @@ class WorkingSet:
             if name in self.params:
                 ET.SubElement(params, name, value=str(int(self.params[name])))
         ET.ElementTree(root).write(path, encoding="utf-8", xml_declaration=False)
+
+
+def ids_from_rgb(rgb, style):
+    """Object IDs from an RGB ID image: r + 255 g + 255^2 b of the channels
+    scaled to 0..255, rounded half up for shot inputs (floorf(v*255 + 0.5),
+    imageIO.cpp:298-317) and half away from zero for styles (round(v*255),
+    G/nlDataAccess.cpp:378-395)."""
+    v = np.asarray(rgb, np.float32)[..., :3] * np.float32(255)
+    q = np.where(v < 0, -np.floor(-v + 0.5), np.floor(v + 0.5)) if style else np.floor(v + np.float32(0.5))
+    q = q.astype(np.int64)
+    return (q[..., 0] + q[..., 1] * 255 + q[..., 2] * 255 * 255).astype(np.int32)
 
 
 def _read_layer(path, layer, channels):

@@ -53,6 +53,19 @@ def orientation_tensor(seed, stream, w, h):
     return np.stack([c * c, c * s, s * s], axis=-1).astype(np.float32)
 
 
+def object_ids(seed, stream, w, h, regions=4):
+    """Object-ID image (int32): the Voronoi cell of `regions` uniform seed
+    points, cell k carrying the ID 1009 * k + 7 (IDs are arbitrary integers,
+    as the reference's r + 255 g + 255^2 b codes are)."""
+    rng = _rng(seed, stream)
+    px = rng.uniform(0, w, regions)
+    py = rng.uniform(0, h, regions)
+    from scipy.spatial import cKDTree
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    _, k = cKDTree(np.stack([px, py], axis=1)).query(np.stack([xx.ravel(), yy.ravel()], axis=1))
+    return (1009 * k.reshape(h, w) + 7).astype(np.int32)
+
+
 def distance_transform(seed, stream, w, h, points=64):
     rng = _rng(seed, stream)
     px = rng.uniform(0, w, points)

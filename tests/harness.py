@@ -5,6 +5,7 @@ import numpy as np
 from painttween_amd import _ffi
 
 E1, E2, E3, E4 = _ffi.PT_ELEM_F32, _ffi.PT_ELEM_F32X2, _ffi.PT_ELEM_F32X3, _ffi.PT_ELEM_F32X4
+I32 = _ffi.PT_ELEM_I32
 CONST, WITH_LEVELS = _ffi.PT_DOWNSAMPLE_SCALE_CONSTANT, _ffi.PT_DOWNSAMPLE_SCALE_WITH_LEVELS
 
 
@@ -24,6 +25,7 @@ class Session:
         self.st = ctx.pyramid(E3, style.get("orientation"), shape=(sw, sh))
         self.sor = ctx.orientation_pyramid(self.st)
         self.sdt = ctx.pyramid(E1, style.get("dist_trans"), shape=(sw, sh), default=0.0)
+        self.sid = None if style.get("object_ids") is None else ctx.pyramid(I32, style["object_ids"])
         self.frames = []
         for f in frames:
             fp = {
@@ -33,6 +35,7 @@ class Session:
                 "scale": ctx.pyramid(E2, f.get("scale"), shape=(w, h), default=1.0),
                 "velb": ctx.pyramid(E2, f.get("vel_b"), WITH_LEVELS, shape=(w, h)),
                 "velf": ctx.pyramid(E2, f.get("vel_f"), WITH_LEVELS, shape=(w, h)),
+                "id": None if f.get("object_id") is None else ctx.pyramid(I32, f["object_id"]),
             }
             fp["orient"] = ctx.orientation_pyramid(fp["tensor"])
             self.frames.append(fp)
@@ -45,6 +48,8 @@ class Session:
         c.upload_image(level, "EXEMPLAR_OUTPUT", self.so)
         c.upload_image(level, "EXEMPLAR_ORIENTATION", self.sor)
         c.upload_image(level, "EXEMPLAR_DIST_TRANS", self.sdt)
+        if self.sid is not None:
+            c.upload_image(level, "EXEMPLAR_OBJECT_IDS", self.sid)
         self.level = level
 
     def load_frame(self, level, f, random=True, frame_number=None):
@@ -52,6 +57,8 @@ class Session:
         fp = self.frames[f]
         c.upload_image(level, "INPUT_COLOR", fp["color"])
         c.upload_image(level, "INPUT_ORIENTATION", fp["orient"])
+        if fp["id"] is not None:
+            c.upload_image(level, "INPUT_ID", fp["id"])
         c.upload_image(level, "INPUT_VEL_B", fp["velb"])
         c.upload_image(level, "INPUT_VEL_F", fp["velf"])
         c.upload_image(level, "INPUT_DIST_TRANS", fp["dt"])

@@ -77,10 +77,10 @@ class OracleContext(Context):
         canvas, residual) of the frame."""
         keep = []
 
-        def arr(a):
+        def arr(a, dt=np.float32):
             if a is None:
                 return None
-            a = np.ascontiguousarray(a, np.float32)
+            a = np.ascontiguousarray(a, dt)
             keep.append(a)
             return a
 
@@ -88,9 +88,11 @@ class OracleContext(Context):
         sh, sw = style["base"].shape[:2]
         fg = _ffi.FrameGuides(_ffi.fptr(arr(frame["color"])), _ffi.fptr(arr(frame.get("orientation"))),
                               _ffi.fptr(arr(frame.get("dist_trans"))), _ffi.fptr(arr(frame.get("scale"))),
-                              _ffi.fptr(arr(frame.get("vel_b"))), _ffi.fptr(arr(frame.get("vel_f"))))
+                              _ffi.fptr(arr(frame.get("vel_b"))), _ffi.fptr(arr(frame.get("vel_f"))),
+                              _ffi.iptr(arr(frame.get("object_id"), np.int32)))
         sg = _ffi.StyleGuides(_ffi.fptr(arr(style["base"])), _ffi.fptr(arr(style["output"])),
-                              _ffi.fptr(arr(style.get("orientation"))), _ffi.fptr(arr(style.get("dist_trans"))))
+                              _ffi.fptr(arr(style.get("orientation"))), _ffi.fptr(arr(style.get("dist_trans"))),
+                              _ffi.iptr(arr(style.get("object_ids"), np.int32)))
         off = np.zeros((h, w), _ffi.PATCHXF_DTYPE)
         canvas = np.zeros((h, w, 4), np.float32)
         res = np.zeros((h, w), np.float32)

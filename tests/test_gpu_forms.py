@@ -52,6 +52,18 @@ def test_next_frame_matches_oracle():
         _same(streamed[f], ref[f], "frame %d vs oracle" % f)
 
 
+def test_next_frame_with_object_ids_matches_oracle():
+    st = dict(W.style_pair(64, 56, seed=43, guides=True), object_ids=W.object_ids(43, 30, 64, 56))
+    frames = [dict(f, object_id=W.object_ids(44, 31 + i, 48, 40)) for i, f in enumerate(W.animation(48, 40, 3, seed=44))]
+    kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, style_obj_id_weight=0.6)
+    with Context.gpu(0) as g, OracleContext() as o:
+        streamed = _stream(g, g.default_params(**kw), 2, frames, st)
+        po = o.default_params(**kw)
+        ref = [o.next_frame(po, 2, f, st) for f in frames]
+    for f in range(len(frames)):
+        _same(streamed[f], ref[f], "frame %d vs oracle" % f)
+
+
 def test_first_streamed_frame_is_the_still_schedule():
     """Frame 0 of a sequence runs exactly the one-frame reference schedule."""
     st = W.style_pair(64, 56, seed=45, guides=True)
@@ -169,6 +181,31 @@ def test_batch_of_every_schedule_equals_each_job_alone():
             _same(b.result(i), alone[i], "job %d (scheme %d)" % (i, jobs[i]["scheme"]))
     finally:
         b.close()
+
+
+def test_batch_mixing_object_id_jobs_equals_each_alone():
+    """A job with the object-ID term (extended-terms kernels) and one without
+    planned together: each equals the same job synthesized alone."""
+    st = W.style_pair(48, 40, seed=92, guides=True)
+    frames = W.animation(40, 32, 2, seed=93)
+    st_ids = dict(st, object_ids=W.object_ids(92, 30, 48, 40))
+    fr_ids = [dict(f, object_id=W.object_ids(93, 31 + i, 40, 32)) for i, f in enumerate(frames)]
+    jobs = [dict(kw=dict(residual_window_size=2, levels=2, style_obj_id_weight=0.6), frames=fr_ids, style=st_ids),
+            dict(kw=dict(residual_window_size=2, levels=2), frames=frames, style=st)]
+    for j in jobs:
+        j.update(iterations=2, scheme=_ffi.PT_S_FMBM_RM, fixup_passes=1)
+    with Context.gpu(0) as g:
+        alone = [g.synthesize(g.default_params(**j["kw"]), j["frames"], j["style"], iterations=2) for j in jobs]
+        for j in jobs:
+            j["params"] = g.default_params(**j["kw"])
+    b = Batch(jobs)
+    try:
+        b.run()
+        for i in range(len(jobs)):
+            _same(b.result(i), alone[i], "job %d" % i)
+    finally:
+        b.close()
+    assert alone[0][0].tobytes() != alone[1][0].tobytes()
 
 
 def _sequences():

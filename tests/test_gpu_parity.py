@@ -181,6 +181,48 @@ def test_coherence_term_with_temporal_terms(tap):
     assert_same_states(ga, oa)
 
 
+def _with_ids(st, frames, seed):
+    """Attach object-ID images (Voronoi cells) to a style and its frames."""
+    sh, sw = st["base"].shape[:2]
+    st = dict(st, object_ids=W.object_ids(seed, 30, sw, sh))
+    out = []
+    for i, f in enumerate(frames):
+        h, w = f["color"].shape[:2]
+        out.append(dict(f, object_id=W.object_ids(seed, 31 + i, w, h)))
+    return st, out
+
+
+def test_object_id_pyramid_bit_exact():
+    """downsampleVote on the GPU equals the oracle's, odd sizes included."""
+    with Context.gpu(0) as g, OracleContext() as o:
+        for (w, h), seed in (((45, 27), 1), ((64, 64), 2)):
+            ids = W.object_ids(seed, 3, w, h, regions=9)
+            ids[::3, ::2] = 5  # ties and majorities of every kind
+            pg, po = g.pyramid(_ffi.PT_ELEM_I32, ids), o.pyramid(_ffi.PT_ELEM_I32, ids)
+            assert pg.levels == po.levels
+            for l in range(pg.levels):
+                assert pg.download(l).tobytes() == po.download(l).tobytes(), ((w, h), l)
+
+
+@pytest.mark.parametrize("early_exit,unit_parallel,tap", [(1, 1, 0), (0, 0, 0), (1, 1, 1024)])
+def test_object_id_term_operator_sequence(early_exit, unit_parallel, tap):
+    """styleObjIdWeight > 0: the object-ID term at the centre tap
+    (texSynth_kernel.cu:497-504, 637-638), still schedule."""
+    st, frs = _with_ids(W.style_pair(40, 36, seed=23, guides=True), [W.still_frame(44, 34, seed=24, guides=True)],
+                        25)
+    ov = dict(residual_window_size=2, levels=3, style_obj_id_weight=0.6, orientation_enabled=1, dist_trans_weight=0.5)
+    ga, oa = _both(st, frs, ov, _still_ops(3), early_exit, unit_parallel, tap)
+    assert_same_states(ga, oa)
+
+
+def test_object_ids_with_temporal_and_coherence_terms():
+    st, frames = _with_ids(W.style_pair(48, 48, seed=26, guides=True), W.animation(40, 32, 3, seed=27), 28)
+    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=0.3,
+              coherence_weight=0.3, style_obj_id_weight=0.5)
+    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, "merge_random"))
+    assert_same_states(ga, oa)
+
+
 def test_kernel_variants_agree_at_scale():
     """Early rejection, unit-parallel and tap-parallel search on and off, on
     a 256^2 still with guides: offsets, canvas and residual identical."""
@@ -219,6 +261,17 @@ def test_synthesize_animation_bit_exact(scheme):
     frames = W.animation(48, 40, 4, seed=24)
     with Context.gpu(0) as g, OracleContext() as o:
         kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5)
+        _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1),
+                   o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
+
+
+@pytest.mark.parametrize("scheme", [_ffi.PT_S_FMBM_RM, _ffi.PT_S_TCTF])
+def test_synthesize_object_ids_bit_exact(scheme):
+    """Object-ID pyramids built from the guides (vote downsample) and
+    uploaded beside the orientation and distance transform each level."""
+    st, frames = _with_ids(W.style_pair(64, 64, seed=29, guides=True), W.animation(48, 40, 3, seed=30), 31)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, style_obj_id_weight=0.7)
         _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1),
                    o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
 

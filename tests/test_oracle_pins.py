@@ -100,10 +100,12 @@ def _coh_double(cx, cy, tcx, tcy, tx, ty, i, j, level, rr, ar):
     return 1.0 - (1.0 - er) * (1.0 - ea)
 
 
-def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, off=None, level=0):
+def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, off=None, level=0, ids=None):
     """Independent float64 statement of calculateResidual
     (texSynth_kernel.cu:537-659) without temporal terms; the coherence term
-    (:642-646) against the offset field `off` when its weight is positive."""
+    (:642-646) against the offset field `off` when its weight is positive; the
+    object-ID term (:497-504) with ids = (image IDs, exemplar IDs) when
+    style_obj_id_weight is positive."""
     r = prm.residual_window_size
     lh, lw = target.shape[:2]
     hh, hw = exb.shape[:2]
@@ -134,6 +136,12 @@ def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, off=N
                 if prm.dist_trans_weight > 0:
                     de = _bilinear(exdt[..., None], tx, ty)[0] / float(cand["scale_v"])
                     res += min(max(abs(de - float(imgdt[y, x])), 0.0), 10.0) * prm.dist_trans_weight
+                if prm.style_obj_id_weight > 0:
+                    # point sampling at coordinate + 0.5, clamped (tex2D, cudaFilterModePoint)
+                    ex = min(max(int(np.floor(tx + 0.5)), 0), hw - 1)
+                    ey = min(max(int(np.floor(ty + 0.5)), 0), hh - 1)
+                    if ids[0][y, x] != ids[1][ey, ex]:
+                        res += prm.style_obj_id_weight
             else:
                 d = canvas[y + j, x + i].astype(np.float64) - to
                 res += (d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + prm.alpha_weight * d[3] ** 2) * prm.canvas_output_weight
@@ -510,3 +518,106 @@ def test_tctf_runs_and_keeps_offsets_inside_the_exemplar(nframes, fixup):
     # offsets may lie up to a texel beyond, as in the other schedules
     assert (off["x"] >= 0).all() and (off["x"] <= 41).all() and (off["y"] >= 0).all() and (off["y"] <= 37).all()
     assert np.isfinite(canvas).all() and np.isfinite(res).all() and (res >= 0).all()
+
+
+# ---------------------------------------------------------------- object IDs
+def test_vote_downsample_known_answers():
+    """downsampleVote (downsample.cpp:162-210) on hand-made 2x2 blocks: a
+    3-1 and a 2-1-1 majority are the reference's answers; the 2-2 and the
+    all-distinct ties go to the first ID in scan order (the reference's tie
+    winner depends on QHash iteration order)."""
+    ids = np.array([[5, 5, 1, 2],
+                    [5, 9, 2, 3],
+                    [4, 7, 8, 6],
+                    [7, 4, 3, 1]], np.int32)
+    with OracleContext() as o:
+        p = o.pyramid(_ffi.PT_ELEM_I32, ids)
+        assert p.levels == 2
+        np.testing.assert_array_equal(p.download(0), ids)
+        np.testing.assert_array_equal(p.download(1), [[5, 2], [4, 8]])
+        q = o.pyramid(_ffi.PT_ELEM_I32, None, default=3.0, shape=(8, 4))
+        assert (q.download(1) == 3).all()
+
+
+def test_object_id_term_point_fetch_and_weight():
+    """The term is added at the centre tap only, weight * g_c / sum(g) of the
+    patch distance, when the image ID at the pixel differs from the style ID
+    at the candidate's centre texel floor(x + 0.5): a candidate at x = 5.49
+    reads texel 5, one at x = 5.5 texel 6."""
+    st = W.style_pair(16, 12, seed=47)
+    fr = W.still_frame(12, 10, seed=48)
+    st["object_ids"] = np.zeros((12, 16), np.int32)
+    st["object_ids"][:, 5] = 1
+    st["object_ids"][:, 6] = 2
+    fr["object_id"] = np.ones((10, 12), np.int32)
+    r, wt = 2, 0.75
+    with OracleContext() as o:
+        g = None
+        d = {}
+        for w in (0.0, wt):
+            s = Session(o, st, [fr], o.default_params(residual_window_size=r, levels=1, style_obj_id_weight=w))
+            s.run([("style", 0), ("frame", 0, 0), ("randomize",)])
+            g = o.gaussian_weights(r)
+            for cx in (5.49, 5.5):
+                cand = np.zeros((), _ffi.PATCHXF_DTYPE)
+                cand["x"], cand["y"], cand["scale_u"], cand["scale_v"] = cx, 4.0, 1.0, 1.0
+                d[w, cx] = o.patch_distance(6, 5, 0, cand)
+        assert d[wt, 5.49] == d[0.0, 5.49]
+        exp = wt * float(g[r]) ** 2 / float(np.sum(g)) ** 2
+        assert abs((d[wt, 5.5] - d[0.0, 5.5]) - exp) <= 1e-6 * exp + 1e-6 * d[0.0, 5.5]
+
+
+@pytest.mark.parametrize("r,level", [(2, 0), (3, 1)])
+def test_patch_distance_with_object_ids_matches_float64(r, level):
+    st = W.style_pair(24, 20, seed=49, guides=True)
+    fr = W.still_frame(22, 18, seed=50, guides=True)
+    st["object_ids"] = W.object_ids(49, 30, 24, 20)
+    fr["object_id"] = W.object_ids(50, 31, 22, 18)
+    rng = np.random.default_rng(51)
+    with OracleContext() as o:
+        prm = o.default_params(residual_window_size=r, levels=2, style_obj_id_weight=0.9, dist_trans_weight=0.4)
+        s = Session(o, st, [fr], prm)
+        s.run([("style", level), ("frame", level, 0), ("randomize",)])
+        off, canvas, _ = s.state()
+        exb, exo = s.sb.download(level), s.so.download(level)
+        exdt = s.sdt.download(level)
+        target = s.frames[0]["color"].download(level)
+        imgdt = s.frames[0]["dt"].download(level)
+        ids = (s.frames[0]["id"].download(level), s.sid.download(level))
+        g = o.gaussian_weights(r)
+        lh, lw = target.shape[:2]
+        hits = 0
+        for x, y in [(0, 0), (lw - 1, lh - 1)] + [(int(a), int(b)) for a, b in
+                                                  zip(rng.integers(0, lw, 14), rng.integers(0, lh, 14))]:
+            cand = np.zeros((), _ffi.PATCHXF_DTYPE)
+            cand["x"] = rng.uniform(0, exb.shape[1])
+            cand["y"] = rng.uniform(0, exb.shape[0])
+            cand["theta"] = rng.uniform(-1.5, 1.5)
+            cand["scale_u"] = cand["scale_v"] = 1.0
+            got = o.patch_distance(x, y, level, cand)
+            exp = _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, ids=ids)
+            assert abs(got - exp) <= 3e-6 * max(exp, 1e-3), (x, y, got, exp)
+            hits += _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, ids=ids) != \
+                _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g,
+                            o.default_params(residual_window_size=r, levels=2, dist_trans_weight=0.4))
+        assert hits >= 3  # the term fired on some of the candidates
+
+
+def test_object_ids_absent_or_equal_change_nothing():
+    """With the weight on but no ID images (all 0), or with equal IDs
+    everywhere, the schedule gives the weight-0 result bit for bit."""
+    st = W.style_pair(32, 32, seed=52, guides=True)
+    frames = W.animation(24, 20, 2, seed=53)
+    kw = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5)
+    with OracleContext() as o:
+        base = o.synthesize(o.default_params(**kw), frames, st, iterations=2)
+        off = o.synthesize(o.default_params(style_obj_id_weight=0.8, **kw), frames, st, iterations=2)
+        same = [dict(f, object_id=np.full((20, 24), 11, np.int32)) for f in frames]
+        st2 = dict(st, object_ids=np.full((32, 32), 11, np.int32))
+        eq = o.synthesize(o.default_params(style_obj_id_weight=0.8, **kw), same, st2, iterations=2)
+        diff = o.synthesize(o.default_params(style_obj_id_weight=0.8, **kw),
+                            [dict(f, object_id=W.object_ids(53, 40 + i, 24, 20)) for i, f in enumerate(frames)],
+                            dict(st, object_ids=W.object_ids(52, 41, 32, 32)), iterations=2)
+    for a, b, c in zip(base, off, eq):
+        assert a.tobytes() == b.tobytes() == c.tobytes()
+    assert base[0].tobytes() != diff[0].tobytes()

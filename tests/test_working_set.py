@@ -231,3 +231,27 @@ def test_synthesize_gpu_matches_oracle(tmp_path):
         got, want = wsm.synthesize(g, ws), wsm.synthesize(o, ws)
     for a, b in zip(got, want):
         assert a.tobytes() == b.tobytes()
+
+
+def test_object_id_elements(tmp_path):
+    """The shot's and the style's ID images become r + 255 g + 255^2 b codes
+    and reach the object-ID guides."""
+    xml = XML.replace('velFElem="vf"', 'velFElem="vf" idMergedElem="ids"').replace(
+        '<styleOrientation path="style/or.exr"/>', '<styleOrientation path="style/or.exr"/><styleObjectIDs path="style/id.exr"/>')
+    path, _, _ = _make(tmp_path, xml=xml)
+    rng = np.random.default_rng(9)
+    q = rng.integers(0, 256, (H, Wd, 3))
+    for f in (1, 2):
+        _layer(tmp_path / ("shots/shotA.ids.%d.exr" % f), "id", (q / 255.0).astype(np.float32), ("red", "green", "blue"))
+    qs = rng.integers(0, 256, (SH, SW, 3))
+    _layer(tmp_path / "style/id.exr", "id", (qs / 255.0).astype(np.float32), ("red", "green", "blue"))
+    ws = wsm.load(path)
+    want = (q[..., 0] + 255 * q[..., 1] + 255 * 255 * q[..., 2]).astype(np.int32)
+    np.testing.assert_array_equal(ws.read_frame(1)["object_id"], want)
+    want_s = (qs[..., 0] + 255 * qs[..., 1] + 255 * 255 * qs[..., 2]).astype(np.int32)
+    np.testing.assert_array_equal(ws.read_style(0)["object_ids"], want_s)
+    # the two rounding rules differ only below zero: -0.5 rounds to 0 (half up) and -1 (half away)
+    half = np.array([[[-0.5 / 255, 0, 0]]], np.float32)
+    v = float(half[0, 0, 0] * np.float32(255))
+    if v == -0.5:
+        assert wsm.ids_from_rgb(half, style=False)[0, 0] == 0 and wsm.ids_from_rgb(half, style=True)[0, 0] == -1
