@@ -67,6 +67,7 @@ struct PtArgs {
     int tap_max_px;    /* default policy: tap-parallel shapes on levels up to this size */
     int tap_np_res;    /* tap-parallel refresh: pixels per workgroup (pt_tap_configure)   */
     int tap_np_prop;   /* tap-parallel propagate: pixels per workgroup (pt_tap_configure) */
+    int ex_window;     /* XL kernel variants: exemplar windows in LDS */
 
     uint8_t *changed;  /* per pixel: base offset changed by the last sync */
 

@@ -238,6 +238,12 @@ struct Tile {
     const float4 *col;
     const float4 *can;
     int tw, lx, ly;
+    /* exemplar window: ww x ww texels from (wx, wy), interleaved base/output
+     * in LDS; exl: every bilinear fetch of the lane's current candidate
+     * lies inside it (read only by the XL kernel variants) */
+    const float4 *exw;
+    int wx, wy, ww;
+    bool exl;
 };
 
 typedef float pt_f4v __attribute__((ext_vector_type(4)));
@@ -274,6 +280,60 @@ __device__ __forceinline__ void ex_pair(__amdgpu_buffer_rsrc_t er, int ew, int e
     int o00 = (ra + xa) << 5, o10 = (ra + xb) << 5, o01 = (rb + xa) << 5, o11 = (rb + xb) << 5;
     b = mix4(q, ld4(er, o00), ld4(er, o10), ld4(er, o01), ld4(er, o11));
     o = mix4(q, ld4(er, o00 + 16), ld4(er, o10 + 16), ld4(er, o01 + 16), ld4(er, o11 + 16));
+}
+
+/* ex_pair from the workgroup's LDS window when the lane's candidate lies
+ * inside it (same texels, weights and order as the global fetch). */
+template <bool XL>
+__device__ __forceinline__ void ex_pair_t(const Tile &T, __amdgpu_buffer_rsrc_t er, int ew, int eh, float x, float y,
+                                          float4 &b, float4 &o)
+{
+    if (XL && T.exl) {
+        float fx = floorf(x), fy = floorf(y);
+        float a = x - fx, c = y - fy;
+        Bilin q;
+        q.w00 = (1.0f - a) * (1.0f - c);
+        q.w10 = a * (1.0f - c);
+        q.w01 = (1.0f - a) * c;
+        q.w11 = a * c;
+        int i00 = (((int)fy - T.wy) * T.ww + ((int)fx - T.wx)) * 2;
+        int i01 = i00 + 2 * T.ww;
+        const float4 *e = T.exw;
+        b = mix4(q, e[i00], e[i00 + 2], e[i01], e[i01 + 2]);
+        o = mix4(q, e[i00 + 1], e[i00 + 3], e[i01 + 1], e[i01 + 3]);
+    } else {
+        ex_pair(er, ew, eh, x, y, b, o);
+    }
+}
+
+/* Whether every tap of a candidate centred at (cx, cy) with basis (b0, b1)
+ * reads its bilinear texels inside the window at (wx, wy) of side ww,
+ * without wrap-around (taps in [0, hw) x [0, hh)) and without edge clamping
+ * (texels inside the exemplar).  Taps lie within r(|b0|+|b1|) of the centre
+ * per axis; the margin covers their rounding.  False for NaN. */
+__device__ __forceinline__ bool in_window(const PtArgs &A, int r, float cx, float cy, float b0x, float b0y, float b1x,
+                                          float b1y, int wx, int wy, int ww)
+{
+    float ex = (float)r * (fabsf(b0x) + fabsf(b1x)) + 0.01f;
+    float ey = (float)r * (fabsf(b0y) + fabsf(b1y)) + 0.01f;
+    float lx = cx - ex, hx = cx + ex, ly = cy - ey, hy = cy + ey;
+    return lx >= (float)wx && hx < (float)(wx + ww - 1) && ly >= (float)wy && hy < (float)(wy + ww - 1) &&
+           lx >= 0.f && ly >= 0.f && hx < (float)A.hw && hy < (float)A.hh && hx < (float)(A.ew - 1) &&
+           hy < (float)(A.eh - 1);
+}
+
+/* Stages the exemplar window of side ww anchored on the offset (ax, ay):
+ * its origin puts the anchor near the centre, clamped inside the exemplar
+ * (the caller guarantees ew, eh >= ww). */
+__device__ __forceinline__ void stage_window(const PtArgs &A, float ax, float ay, int ww, float4 *sw, int &wx, int &wy,
+                                             int tid, int nthreads)
+{
+    wx = clampi((int)floorf(ax) - ww / 2, 0, A.ew - ww);
+    wy = clampi((int)floorf(ay) - ww / 2, 0, A.eh - ww);
+    for (int k = tid; k < ww * ww * 2; k += nthreads) {
+        int t = k >> 1;
+        sw[k] = A.ex[2 * ((wy + t / ww) * A.ew + wx + t % ww) + (k & 1)];
+    }
 }
 
 /* timeDerivativeTerms (texSynth_kernel.cu:438-469); EXT: the kernel
@@ -328,7 +388,7 @@ __device__ __forceinline__ float coherence_add(const PtArgs &A, float res, int x
  * inactive.  Target and canvas come from the LDS tile; i*b0 is formed once
  * per row, exactly the product the reference forms per tap.
  */
-template <int RT, bool TIME>
+template <int RT, bool TIME, bool XL = false>
 __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc, float4 pf,
                                     float4 pb, bool cpv, bool cnv)
 {
@@ -382,7 +442,7 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
             if (ty >= hh) ty -= hh;
             else if (ty < 0) ty += hh;
             float4 tb, to;
-            ex_pair(er, ew, eh, tx, ty, tb, to);
+            ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to);
             if (RT < 0 && A.lab) {
                 tb = lab4(tb);
                 to = lab4(to);
@@ -428,7 +488,7 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
 /* TK (a kernel-level switch) is false when the host knows no advected
  * offset is valid (the advected buffers were reset and not written since);
  * the per-pixel test then always takes the plain variant. */
-template <int RT, bool TK>
+template <int RT, bool TK, bool XL = false>
 __device__ __forceinline__ float calc_residual(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc)
 {
     if (TK) {
@@ -436,10 +496,10 @@ __device__ __forceinline__ float calc_residual(const PtArgs &A, const Tile &T, i
         float4 pf = A.adv[0].xyth[o], pb = A.adv[1].xyth[o];
         bool cpv = pf.w > 0, cnv = pb.w > 0;
         if (cpv || cnv)
-            return calc_residual_impl<RT, true>(A, T, x, y, t, tsc, pf, pb, cpv, cnv);
+            return calc_residual_impl<RT, true, XL>(A, T, x, y, t, tsc, pf, pb, cpv, cnv);
     }
     float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    return calc_residual_impl<RT, false>(A, T, x, y, t, tsc, z, z, false, false);
+    return calc_residual_impl<RT, false, XL>(A, T, x, y, t, tsc, z, z, false, false);
 }
 
 /* test_offset (texSynth_kernel.cu:695-721) */
@@ -518,10 +578,13 @@ __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, floa
  * the pixel is skipped.  Pixels that need the refresh are compacted onto the
  * first lanes of the workgroup so that whole waves skip.
  */
-template <int RT, bool TK>
+#define PT_XW_RES 32 /* exemplar window side of the XL residual refresh */
+
+template <int RT, bool TK, bool XL>
 __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int by)
 {
     __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
+    __shared__ float4 sexw[XL ? PT_XW_RES * PT_XW_RES * 2 : 1];
     __shared__ unsigned char sflag[TileSize<RT>::N];
     __shared__ unsigned short slist[PT_BX * PT_BY];
     __shared__ int scount;
@@ -540,14 +603,34 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
     int x, y;
     bool valid = pixel_of(A, bx, by, x, y);
     Tile T = stage_tile<RT>(A, bx, by, scol, scan); /* its barrier also covers sflag and scount */
+    /* XL: the window is anchored on the offset of the tile's middle pixel
+     * (clamped into the level) and read by the pixels whose patch lies in it */
+    auto window = [&]() {
+        const int tid = threadIdx.y * PT_BX + threadIdx.x;
+        int ax = min(bx * PT_BX + PT_BX / 2, A.lw - 1), ay = min(by * PT_BY + PT_BY / 2, A.lh - 1);
+        float4 an = A.base.xyth[ay * A.W + ax];
+        stage_window(A, an.x, an.y, PT_XW_RES, sexw, T.wx, T.wy, tid, PT_BX * PT_BY);
+        T.exw = sexw;
+        T.ww = PT_XW_RES;
+        __syncthreads();
+    };
+    auto fits = [&](float4 t, float2 tsc) {
+        float b0x, b0y, b1x, b1y;
+        basis(t.z, A.su * tsc.x, A.sv * tsc.y, b0x, b0y, b1x, b1y);
+        return in_window(A, r, t.x, t.y, b0x, b0y, b1x, b1y, T.wx, T.wy, T.ww);
+    };
     if (!inc) {
+        if (XL) window();
         if (!valid) return;
         int o = y * A.W + x;
         if (image_culled(A, x, y)) {
             A.res_base[o] = 0.0f;
             return;
         }
-        A.res_base[o] = calc_residual<RT, TK>(A, T, x, y, A.base.xyth[o], A.base.sc[o]);
+        float4 t = A.base.xyth[o];
+        float2 tsc = A.base.sc[o];
+        if (XL) T.exl = fits(t, tsc);
+        A.res_base[o] = calc_residual<RT, TK, XL>(A, T, x, y, t, tsc);
         return;
     }
     if (valid) {
@@ -563,6 +646,7 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
     }
     __syncthreads();
     const int n = scount;
+    if (XL && n > 0) window();
     const int lane = threadIdx.y * PT_BX + threadIdx.x;
     if (lane >= n) return;
     const int p = slist[lane];
@@ -575,13 +659,16 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
         A.res_base[o] = 0.0f;
         return;
     }
-    A.res_base[o] = calc_residual<RT, TK>(A, T, x, y, A.base.xyth[o], A.base.sc[o]);
+    float4 t = A.base.xyth[o];
+    float2 tsc = A.base.sc[o];
+    if (XL) T.exl = fits(t, tsc);
+    A.res_base[o] = calc_residual<RT, TK, XL>(A, T, x, y, t, tsc);
 }
 
-template <int RT, bool TK>
+template <int RT, bool TK, bool XL = false>
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_update_residual(PtArgs A)
 {
-    b_update_residual<RT, TK>(A, blockIdx.x, blockIdx.y);
+    b_update_residual<RT, TK, XL>(A, blockIdx.x, blockIdx.y);
 }
 
 /* offsetsToOutput (texSynth_kernel.cu:263-274) */
@@ -2468,7 +2555,7 @@ template <int RT, bool TK>
 __global__ __launch_bounds__(PT_BX *PT_BY) void km_update_residual(PT_MULTI_PARAMS)
 {
     MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
-    b_update_residual<RT, TK>(PT_JOB(t.j), t.bx, t.by);
+    b_update_residual<RT, TK, false>(PT_JOB(t.j), t.bx, t.by);
 }
 
 template <int RT, bool TK>
@@ -2583,10 +2670,20 @@ hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s)
         }
         return hipGetLastError();
     }
+    bool tk = a.time_active != 0;
     if (ext_terms(a)) {
-        bool tk = a.time_active != 0;
         hipLaunchKernelGGL((tk ? k_update_residual<-1, true> : k_update_residual<-1, false>), grid_for(a.lw, a.lh),
                            kBlock, 0, s, a);
+        return hipGetLastError();
+    }
+    if (a.ex_window && a.ew >= PT_XW_RES && a.eh >= PT_XW_RES && (a.r == 2 || a.r == 3)) {
+        dim3 g = grid_for(a.lw, a.lh);
+        if (a.r == 2)
+            hipLaunchKernelGGL((tk ? k_update_residual<2, true, true> : k_update_residual<2, false, true>), g, kBlock,
+                               0, s, a);
+        else
+            hipLaunchKernelGGL((tk ? k_update_residual<3, true, true> : k_update_residual<3, false, true>), g, kBlock,
+                               0, s, a);
         return hipGetLastError();
     }
     PT_DISPATCH_R(k_update_residual, a);

@@ -129,6 +129,7 @@ struct pt_ctx {
     bool incremental_residual = true;
     int64_t tap_max_pixels = 128 * 128; /* tap-parallel shapes up to this level size */
     int tap_evals = 1024;               /* tap-parallel: tap evaluations per workgroup */
+    bool ex_window = true;              /* LDS exemplar windows in the XL kernel variants */
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -569,6 +570,9 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
         if (value < 1 || value > PT_TAP_EVALS)
             return set_err(c, PT_ERR_INVALID_ARG, "tap evaluations per workgroup must be 1..%d", PT_TAP_EVALS);
         c->tap_evals = (int)value;
+        return PT_OK;
+    case PT_TUNE_EXEMPLAR_WINDOW:
+        c->ex_window = value != 0;
         return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
@@ -1082,6 +1086,7 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.res_incremental = 0;
     a.tap_max_px = (int)std::min<int64_t>(c->tap_max_pixels, 1 << 30);
     pt_tap_configure(a, c->tap_evals);
+    a.ex_window = c->ex_window ? 1 : 0;
     a.changed = c->changed;
     a.base = c->base;
     a.work = c->work;
