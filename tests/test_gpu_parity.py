@@ -264,6 +264,54 @@ def test_kernel_variants_agree_at_scale():
         _cmp_synth(outs[0], other)
 
 
+def test_headline_config_coarse_levels_bit_exact():
+    """The benchmark still's configuration (1024^2 inputs, 7x7 patches, 5
+    levels, 8 iterations, guides), default kernel policy: every state of
+    levels 4 to 1 (64^2 to 512^2) equals the oracle's; level 0 is checked by
+    the full-size consistency test below."""
+    st = W.style_pair(1024, 1024, seed=2, guides=True)
+    fr = W.still_frame(1024, 1024, seed=2, guides=True)
+    kw = dict(residual_window_size=3, levels=5, dist_trans_weight=0.5, orientation_enabled=1)
+    ops = _still_ops(5, iterations=8)
+    ops = ops[:ops.index(("style", 0))]  # levels 4 to 1
+    g, o = Context.gpu(0), OracleContext()
+    try:
+        ga = Session(g, st, [fr], g.default_params(**kw)).run(ops)
+        oa = Session(o, st, [fr], o.default_params(**kw)).run(ops)
+    finally:
+        g.close()
+        o.close()
+    assert_same_states(ga, oa)
+
+
+def test_headline_still_full_size_consistent_with_oracle():
+    """The benchmark still at its full size (1024^2 frame and exemplar pair,
+    7x7 patches, 5 levels, 8 iterations, orientation and distance-transform
+    guides): the oracle, given the GPU's final offsets, recomputes the canvas
+    and every pixel's residual bit for bit; a second GPU run reproduces the
+    first; offsets stay inside the exemplar."""
+    st = W.style_pair(1024, 1024, seed=2, guides=True)
+    fr = W.still_frame(1024, 1024, seed=2, guides=True)
+    kw = dict(residual_window_size=3, levels=5, dist_trans_weight=0.5, orientation_enabled=1)
+    with Context.gpu(0) as g:
+        p = g.default_params(**kw)
+        a = g.synthesize(p, [fr], st, iterations=8)
+        b = g.synthesize(p, [fr], st, iterations=8)
+    _cmp_synth(a, b)
+    off, canvas, res = (x[0] for x in a)
+    for f in ("x", "y"):
+        assert off[f].min() >= 0.0 and off[f].max() <= 1025.0
+    assert np.isfinite(res).all() and (res >= 0).all()
+    with OracleContext() as o:
+        s = Session(o, st, [fr], o.default_params(**kw))
+        s.run([("style", 0), ("frame", 0, 0)])
+        o.load_offsets(0, off)
+        o.sync_buffers(0, _ffi.PT_DONT_COPY_OFFSETS)
+        _, oc, orr = s.state(0)
+    assert first_difference(canvas, oc) is None, "canvas"
+    assert first_difference(res, orr) is None, "residual"
+
+
 def _cmp_synth(g_out, o_out):
     for label, a, b in zip(("offsets", "canvas", "residual"), g_out, o_out):
         assert first_difference(a, b) is None, label
