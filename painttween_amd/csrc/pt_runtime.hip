@@ -129,7 +129,7 @@ struct pt_ctx {
     bool incremental_residual = true;
     int64_t tap_max_pixels = 128 * 128; /* tap-parallel shapes up to this level size */
     int tap_evals = 1024;               /* tap-parallel: tap evaluations per workgroup */
-    bool ex_window = true;              /* LDS exemplar windows in the XL kernel variants */
+    int64_t ex_window_min_pixels = 400000; /* XL refresh on levels of at least this size (0: never) */
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -572,7 +572,8 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
         c->tap_evals = (int)value;
         return PT_OK;
     case PT_TUNE_EXEMPLAR_WINDOW:
-        c->ex_window = value != 0;
+        if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative pixel count");
+        c->ex_window_min_pixels = value;
         return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
@@ -1086,7 +1087,7 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.res_incremental = 0;
     a.tap_max_px = (int)std::min<int64_t>(c->tap_max_pixels, 1 << 30);
     pt_tap_configure(a, c->tap_evals);
-    a.ex_window = c->ex_window ? 1 : 0;
+    a.ex_window = (c->ex_window_min_pixels > 0 && (int64_t)a.lw * a.lh >= c->ex_window_min_pixels) ? 1 : 0;
     a.changed = c->changed;
     a.base = c->base;
     a.work = c->work;
