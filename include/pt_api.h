@@ -269,7 +269,9 @@ int pt_merge_using_residual(pt_ctx *ctx, int level, int forwards);
  * NULL. */
 int pt_get_output(pt_ctx *ctx, int level, pt_patchxf *offsets, float *canvas_rgba, float *residual);
 /* WorkingBuffers::loadOffsets / loadResidual (workingBuffers.cpp:166-207):
- * writes the level rectangle of the base offsets / base residual. */
+ * writes the level rectangle of the base offsets / base residual.  Offsets
+ * must have layer 0 and lum_a 0 (PT_ERR_UNSUPPORTED otherwise); a nonzero
+ * lum_y (luminance shift) is kept and added to the canvas. */
 int pt_load_offsets(pt_ctx *ctx, int level, const pt_patchxf *offsets);
 int pt_load_residual(pt_ctx *ctx, int level, const float *residual);
 /* Reads an advected buffer set (advectedBuffers.h) for tests: which = 0 for

@@ -47,7 +47,8 @@ typedef struct { float x, y; } f2;
 /* PatchXF restricted to one style layer and no luminance shift: layer is
  * always 0 and the shift always (0,0) on this path (multi-style and
  * luminance shift are rejected by setup). */
-typedef struct { float x, y, th, hy, su, sv; } nnf;
+/* PatchXF without layer: ly is luminanceShift.y (its .a is always 0 here) */
+typedef struct { float x, y, th, hy, su, sv, ly; } nnf;
 
 typedef struct {
     nnf *off;
@@ -96,6 +97,7 @@ struct ptref_ctx {
     f2 *img_scale; int scw, sch;
     f4 *random; int rw, rh;
     int32_t *img_id; int idw, idh; /* gImageId, a single 0 texel until uploaded */
+    int lum_live; /* some luminance shift may be nonzero: the canvas adds it */
     /* exemplar textures, one level (texSynth_kernel.cu:2119-2126) */
     f4 *ex_base, *ex_out;
     float *ex_orient, *ex_dt;
@@ -131,6 +133,10 @@ float ptref_acosf(float x) { return pt_acosf(x); }
 double ptref_log_d(double x) { return pt_log_d(x); }
 double ptref_exp_d(double x) { return pt_exp_d(x); }
 void ptref_rgb_to_lab(float r, float g, float b, float *out) { pt_rgb_to_lab(r, g, b, &out[0], &out[1], &out[2]); }
+float ptref_luminance_delta(float cr, float cg, float cb, float pr, float pg, float pb)
+{
+    return pt_luminance_delta(cr, cg, cb, pr, pg, pb);
+}
 float ptref_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
                            float ratio_range, float angle_range)
 {
@@ -591,8 +597,6 @@ static int check_supported(ptref_ctx *c, const pt_params *p)
         return fail(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
     if (p->colorspace != 0 && p->colorspace != 1)
         return fail(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
-    if (p->lum_shift_enable)
-        return fail(c, PT_ERR_UNSUPPORTED, "luminance shift is not supported");
     if (p->interpolate_key_frame)
         return fail(c, PT_ERR_UNSUPPORTED, "keyframe interpolation is not supported");
     for (int i = 0; i < 5; i++)
@@ -623,6 +627,7 @@ int ptref_upload_parameters(ptref_ctx *c, const pt_params *p)
     if (rc) return rc;
     c->p = *p;
     c->have_params = 1;
+    if (p->lum_shift_enable) c->lum_live = 1;
     return PT_OK;
 }
 
@@ -670,6 +675,7 @@ int ptref_initialize(ptref_ctx *c, int iw, int ih, int sw, int sh)
         if (!c->adv[k].ts || !c->img_velf[k]) return fail(c, PT_ERR_OUT_OF_MEMORY, "out of memory");
     }
     if (!c->canvas) return fail(c, PT_ERR_OUT_OF_MEMORY, "out of memory");
+    c->lum_live = c->have_params && c->p.lum_shift_enable;
     free(c->img_id); /* a fresh image-ID texture: one 0 texel until uploaded */
     c->img_id = (int32_t *)calloc(1, sizeof(int32_t));
     c->idw = c->idh = 1;
@@ -998,7 +1004,14 @@ static void offsets_to_output(ptref_ctx *c, int level)
     for (int y = 0; y < lh; y++)
         for (int x = 0; x < lw; x++) {
             nnf n = c->off_base[y * c->W + x];
-            c->canvas[y * c->W + x] = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, n.x, n.y);
+            f4 v = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, n.x, n.y);
+            if (c->lum_live) { /* toColor4(output, luminanceShift) (texSynth_kernel.cu:272) */
+                v.x = v.x + n.ly;
+                v.y = v.y + n.ly;
+                v.z = v.z + n.ly;
+                v.w = v.w + 0.0f;
+            }
+            c->canvas[y * c->W + x] = v;
         }
 }
 
@@ -1295,6 +1308,11 @@ int ptref_advect_offsets(ptref_ctx *c, int level, int time_step, int forwards, i
                 out.y = iy;
                 out.th = nr.th;
                 out.hy = 1.f;
+                /* luminanceShift = nearest's + (color - prev_color).toLuminance(), .a = 0 (:1626-1631) */
+                if (c->p.lum_shift_enable) {
+                    f4 col = c->img_color[c->cur][o];
+                    out.ly = nr.ly + pt_luminance_delta(col.x, col.y, col.z, prev_color.x, prev_color.y, prev_color.z);
+                }
                 d->base[o] = prev_color;
                 d->out[o] = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, out.x, out.y);
                 d->ts[o] = time_step;
@@ -1372,7 +1390,7 @@ static pt_patchxf to_patchxf(nnf n)
 {
     pt_patchxf p;
     p.x = n.x; p.y = n.y; p.theta = n.th; p.hysteresis = n.hy; p.layer = 0;
-    p.scale_u = n.su; p.scale_v = n.sv; p.lum_y = 0.f; p.lum_a = 0.f;
+    p.scale_u = n.su; p.scale_v = n.sv; p.lum_y = n.ly; p.lum_a = 0.f;
     return p;
 }
 
@@ -1415,9 +1433,10 @@ int ptref_load_offsets(ptref_ctx *c, int level, const pt_patchxf *in)
     for (int y = 0; y < lh; y++)
         for (int x = 0; x < lw; x++) {
             const pt_patchxf *p = &in[y * lw + x];
-            if (p->layer != 0 || p->lum_y != 0.f || p->lum_a != 0.f)
-                return fail(c, PT_ERR_UNSUPPORTED, "offsets with layer or luminance shift");
-            nnf n = {p->x, p->y, p->theta, p->hysteresis, p->scale_u, p->scale_v};
+            if (p->layer != 0 || p->lum_a != 0.f)
+                return fail(c, PT_ERR_UNSUPPORTED, "offsets with a style layer or an alpha luminance shift");
+            if (p->lum_y != 0.f) c->lum_live = 1;
+            nnf n = {p->x, p->y, p->theta, p->hysteresis, p->scale_u, p->scale_v, p->lum_y};
             c->off_base[y * c->W + x] = n;
         }
     return PT_OK;

@@ -88,6 +88,7 @@ float ptref_acosf(float x);
 double ptref_log_d(double x);
 double ptref_exp_d(double x);
 void ptref_rgb_to_lab(float r, float g, float b, float *out);
+float ptref_luminance_delta(float cr, float cg, float cb, float pr, float pg, float pb);
 float ptref_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
                            float ratio_range, float angle_range);
 float ptref_tensor_to_angle(float xx, float xy, float yy);

@@ -16,6 +16,7 @@
 struct PtNnf {
     float4 *xyth;
     float2 *sc;
+    float *lum; /* luminance shift y (PatchXF::luminanceShift); read only while lum_live */
 };
 
 /* AdvectedBuffersDevice (advectedBuffers.h:37-52). */
@@ -26,6 +27,7 @@ struct PtAdv {
     float4 *out;
     float *res;
     int32_t *ts;
+    float *lum;
 };
 
 #define PT_MAX_R 7
@@ -50,6 +52,8 @@ struct PtArgs {
     float coh_w, coh_ratio, coh_angle; /* coherence term: weight (0 = off), ratio and angle ranges */
     int lab;           /* colour distances in the LAB space (colorspace 1) */
     float obj_w;       /* object-ID term weight (0 = off)              */
+    int lum_live;      /* luminance-shift planes maintained (some shift may be nonzero) */
+    int lum_enable;    /* lumShiftEnable: advection accumulates the shift */
     int idw, idh;      /* image object-ID texture size                 */
     int samples, rng_mode;
     uint32_t key0, key1, call;

@@ -91,6 +91,24 @@ __device__ __forceinline__ int point_index(int w, int h, float x, float y)
     return yi * w + xi;
 }
 
+/* Luminance shift (PatchXF::luminanceShift, types.h:98-114): a per-pixel
+ * offset y added to the R, G and B of the exemplar output in the canvas
+ * (toColor4(c, lum), texSynth_util.h:207-209; its alpha part is always 0
+ * here).  The plane is kept only while A.lum_live. */
+__device__ __forceinline__ float4 lum_shifted(const PtArgs &A, float4 c, int o)
+{
+    if (!A.lum_live) return c;
+    float l = A.base.lum[o];
+    return make_float4(c.x + l, c.y + l, c.z + l, c.w + 0.0f);
+}
+
+/* The working offset's shift: the base offset's when the pixel keeps it,
+ * else 0 (every new candidate carries Luminance()). */
+__device__ __forceinline__ void lum_set(const PtArgs &A, int o, bool keep)
+{
+    if (A.lum_live) A.work.lum[o] = keep ? A.base.lum[o] : 0.f;
+}
+
 __device__ __forceinline__ float4 tex_ex_out(const PtArgs &A, float x, float y)
 {
     Bilin q = bilin(A.ew, A.ew, A.eh, x, y);
@@ -678,7 +696,7 @@ __device__ __forceinline__ void b_canvas(const PtArgs &A, int bx, int by)
     if (!pixel_of_rows(A, bx, by, x, y)) return;
     int o = y * A.W + x;
     float4 n = A.base.xyth[o];
-    A.canvas[o] = tex_ex_out(A, n.x, n.y);
+    A.canvas[o] = lum_shifted(A, tex_ex_out(A, n.x, n.y), o);
     if (A.changed) A.changed[o] = 0;
 }
 
@@ -711,11 +729,16 @@ __device__ __forceinline__ void b_sync_canvas(const PtArgs &A, int bx, int by, i
         }
         A.base.xyth[o] = n;
         A.base.sc[o] = s;
+        if (A.lum_live) {
+            float l = A.work.lum[o];
+            ch = ch || (A.changed && bits_differ(l, A.base.lum[o]));
+            A.base.lum[o] = l;
+        }
         have = true;
     }
     if (x < A.lw && y < A.lh) {
         if (!have) n = A.base.xyth[o];
-        A.canvas[o] = tex_ex_out(A, n.x, n.y);
+        A.canvas[o] = lum_shifted(A, tex_ex_out(A, n.x, n.y), o);
         if (A.changed) A.changed[o] = ch ? 1 : 0;
     }
 }
@@ -734,6 +757,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A)
     float4 r = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
     A.work.xyth[o] = make_float4(r.x, r.y, r.z, 0.f);
     A.work.sc[o] = make_float2(1.f, 1.f);
+    lum_set(A, o, false);
 }
 
 /* calculatePropagateOffset (texSynth_kernel.cu:956-982); pixelWrap
@@ -787,12 +811,14 @@ __device__ __forceinline__ void b_propagate(const PtArgs &A, int bx, int by, int
     if (image_culled(A, x, y)) {
         A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
         A.work.sc[o] = make_float2(1.f, 1.f);
+        lum_set(A, o, false);
         return;
     }
     float best = A.res_base[o];
     float ia = A.color[A.cur][o].w;
     float4 ow = A.base.xyth[o];
     float2 osc = A.base.sc[o];
+    bool moved = false;
 #pragma unroll 1
     for (int i = 0; i < 4; i++) {
         int tx = x, ty = y;
@@ -808,10 +834,12 @@ __device__ __forceinline__ void b_propagate(const PtArgs &A, int bx, int by, int
             ow = c;
             osc = csc;
             best = r;
+            moved = true;
         }
     }
     A.work.xyth[o] = ow;
     A.work.sc[o] = osc;
+    lum_set(A, o, !moved);
 }
 
 template <int RT, bool TK>
@@ -894,6 +922,7 @@ __device__ __forceinline__ void b_scatter(const PtArgs &A, int bx, int by)
     if (best < cached) {
         A.work.xyth[o] = ow;
         A.work.sc[o] = osc;
+        lum_set(A, o, false);
     }
 }
 
@@ -1185,12 +1214,14 @@ __device__ __forceinline__ void b_propagate_ee(const PtArgs &A, int bx, int by, 
     if (image_culled(A, x, y)) {
         A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
         A.work.sc[o] = make_float2(1.f, 1.f);
+        lum_set(A, o, false);
         return;
     }
     float best = A.res_base[o];
     float ia = A.color[A.cur][o].w;
     float4 ow = A.base.xyth[o];
     float2 osc = A.base.sc[o];
+    bool moved = false;
     int kk = 0;
     auto next_seq = [&](Cand &c) -> bool {
         while (kk < 4) {
@@ -1208,6 +1239,7 @@ __device__ __forceinline__ void b_propagate_ee(const PtArgs &A, int bx, int by, 
     auto accept = [&](const Cand &c) {
         ow = c.t;
         osc = c.sc;
+        moved = true;
     };
     if (TK) {
         float4 pf = A.adv[0].xyth[o], pb = A.adv[1].xyth[o];
@@ -1220,6 +1252,7 @@ __device__ __forceinline__ void b_propagate_ee(const PtArgs &A, int bx, int by, 
     }
     A.work.xyth[o] = ow;
     A.work.sc[o] = osc;
+    lum_set(A, o, !moved);
 }
 
 template <int RT, bool TK>
@@ -1310,6 +1343,7 @@ __device__ __forceinline__ void b_scatter_ee(const PtArgs &A, int bx, int by)
     if (best < cached) {
         A.work.xyth[o] = ow;
         A.work.sc[o] = osc;
+        lum_set(A, o, false);
     }
 }
 
@@ -1721,6 +1755,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         if (image_culled(A, x, y)) {
             A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
             A.work.sc[o] = make_float2(1.f, 1.f);
+            lum_set(A, o, false);
             continue;
         }
         float best = A.res_base[o];
@@ -1732,6 +1767,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
                 win = k;
             }
         }
+        lum_set(A, o, win < 0);
         if (win < 0) {
             A.work.xyth[o] = A.base.xyth[o];
             A.work.sc[o] = A.base.sc[o];
@@ -1898,6 +1934,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
         if (best < s_cached[p]) {
             A.work.xyth[o] = ow;
             A.work.sc[o] = osc;
+            lum_set(A, o, false);
         }
     }
 }
@@ -1947,6 +1984,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A)
     int o = y * A.W + x;
     A.work.xyth[o] = make_float4(ux, uy, ang, 0.f);
     A.work.sc[o] = sc;
+    lum_set(A, o, false);
     A.res_work[o] = rw;
 }
 
@@ -1986,6 +2024,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
         d.out[o] = make_float4(0, 0, 0, 0);
         d.xyth[o] = make_float4(0, 0, 0, 0);
         d.sc[o] = make_float2(1.f, 1.f);
+        if (A.lum_live) d.lum[o] = 0.f;
         return;
     }
     int prev = (A.cur + 1) % 2;
@@ -1993,6 +2032,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
     float4 frac = make_float4(rnd.x - floorf(rnd.x), rnd.y - floorf(rnd.y), rnd.z - floorf(rnd.z),
                               rnd.w - floorf(rnd.w));
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    float lum = 0.f;
     float tx, ty;
     if (!(flags & PT_ADVECT_DO_NOT_CHECK_PREVIOUS) && translated_position(A, fwd, x, y, tx, ty) &&
         !(tex1_lin(A.dt[prev], A.W, A.W, A.H, tx, ty) > ((float)A.cull - 8.0f))) {
@@ -2005,12 +2045,21 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
         clamp_ex(ix, iy, A.ew, A.eh);
         float4 prev_color = tex4_lin(A.color[prev], A.W, A.W, A.H, tx, ty);
         out = make_float4(ix, iy, nr.z, 1.f);
+        if (A.lum_live) {
+            /* luminanceShift: the nearest source's shift plus the luminance
+             * change of the image between the frames (:1628-1631) */
+            float4 col = A.color[A.cur][o];
+            lum = A.lum_enable ? A.base.lum[rty * A.W + rtx] + pt_luminance_delta(col.x, col.y, col.z, prev_color.x,
+                                                                                    prev_color.y, prev_color.z)
+                               : 0.f;
+        }
         d.base[o] = prev_color;
         d.out[o] = tex_ex_out(A, out.x, out.y);
         d.ts[o] = time_step;
     }
     if (out.w == 0.f) {
         out = make_float4(rnd.x, rnd.y, rnd.z, 0.f);
+        lum = 0.f;
         d.base[o] = make_float4(0, 0, 0, 0);
         d.out[o] = make_float4(0, 0, 0, 0);
         d.ts[o] = -1;
@@ -2021,6 +2070,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
     out.z = screen_to_ex_angle(A, (float)x, (float)y, out.x, out.y);
     d.xyth[o] = out;
     d.sc[o] = make_float2(1.f, 1.f);
+    if (A.lum_live) d.lum[o] = lum;
 }
 
 /* mergeRandomlyKernel (texSynth_kernel.cu:1891-1936); the ribbon textures
@@ -2037,9 +2087,11 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int f
     if ((a.w > 0 || force) && r < weight) {
         A.work.xyth[o] = a;
         A.work.sc[o] = d.sc[o];
+        if (A.lum_live) A.work.lum[o] = d.lum[o];
     } else {
         A.work.xyth[o] = A.base.xyth[o];
         A.work.sc[o] = A.base.sc[o];
+        lum_set(A, o, true);
     }
 }
 
@@ -2055,11 +2107,13 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_residual(PtArgs A, int f
         a.w = 1.f;
         A.work.xyth[o] = a;
         A.work.sc[o] = d.sc[o];
+        if (A.lum_live) A.work.lum[o] = d.lum[o];
     } else {
         float4 u = A.base.xyth[o];
         u.w = 0.f;
         A.work.xyth[o] = u;
         A.work.sc[o] = A.base.sc[o];
+        lum_set(A, o, true);
     }
 }
 
@@ -2219,7 +2273,7 @@ __global__ void k_pack_nnf(PtNnf s, int W, int lw, int lh, PatchXF36 *d)
     if (x >= lw || y >= lh) return;
     float4 v = s.xyth[y * W + x];
     float2 c = s.sc[y * W + x];
-    PatchXF36 p = {v.x, v.y, v.z, v.w, 0, c.x, c.y, 0.f, 0.f};
+    PatchXF36 p = {v.x, v.y, v.z, v.w, 0, c.x, c.y, s.lum ? s.lum[y * W + x] : 0.f, 0.f};
     d[y * lw + x] = p;
 }
 
@@ -2231,6 +2285,7 @@ __global__ void k_unpack_nnf(const PatchXF36 *s, int lw, int lh, PtNnf d, int W)
     PatchXF36 p = s[y * lw + x];
     d.xyth[y * W + x] = make_float4(p.x, p.y, p.theta, p.hysteresis);
     d.sc[y * W + x] = make_float2(p.su, p.sv);
+    if (d.lum) d.lum[y * W + x] = p.ly;
 }
 
 /* ------------------------------------------------------------------ */
@@ -2466,6 +2521,7 @@ __device__ __forceinline__ void b_propagate_tp(const PtArgs &A, int g, int jump)
             if (image_culled(A, x, y)) {
                 A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
                 A.work.sc[o] = make_float2(1.f, 1.f);
+                lum_set(A, o, false);
             } else {
                 float best = A.res_base[o];
                 int win = -1;
@@ -2476,6 +2532,7 @@ __device__ __forceinline__ void b_propagate_tp(const PtArgs &A, int g, int jump)
                         win = k;
                     }
                 }
+                lum_set(A, o, win < 0);
                 if (win < 0) {
                     A.work.xyth[o] = A.base.xyth[o];
                     A.work.sc[o] = A.base.sc[o];

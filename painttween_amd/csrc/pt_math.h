@@ -147,6 +147,18 @@ PT_HD float pt_atan2f(float y, float x)
  * (imageIO.h:136-139): theta = 0.5*atan2(2xy, xx-yy) + pi/2 evaluated in
  * double and rounded to float, then folded from [pi/2, ...) down by pi.
  */
+/* Color4::toLuminance (types.h:87-89) of the per-channel difference of two
+ * colours: 0.2126 r + 0.7152 g + 0.0722 b, the products and sums in double
+ * (the constants are double literals), rounded to float. */
+PT_HD float pt_luminance_delta(float cr, float cg, float cb, float pr, float pg, float pb)
+{
+    float r = cr - pr, g = cg - pg, b = cb - pb;
+    double y = 0.2126 * (double)r;
+    y = y + 0.7152 * (double)g;
+    y = y + 0.0722 * (double)b;
+    return (float)y;
+}
+
 PT_HD float pt_tensor_to_angle(float xx, float xy, float yy)
 {
     float at = pt_atan2f((float)(2.0 * (double)xy), xx - yy);

@@ -48,6 +48,7 @@ struct DevAdv {
     float4 *out = nullptr;
     float *res = nullptr;
     int32_t *ts = nullptr;
+    float *lum = nullptr;
 };
 
 /* `id` (object IDs) exists only while the object-ID term is on */
@@ -85,6 +86,7 @@ struct ProfRec {
 struct Snapshot {
     float4 *xyth = nullptr;
     float2 *sc = nullptr;
+    float *lum = nullptr; /* luminance shift, saved and loaded while lum_live */
     float *res = nullptr;
     float4 *canvas = nullptr;
     int w = 0, h = 0;
@@ -116,6 +118,10 @@ struct pt_ctx {
     float4 *canvas = nullptr;
     DevAdv adv[2];
     bool adv_written[2] = {false, false}; /* advected set k may hold valid offsets */
+    /* luminance-shift planes are maintained from the first parameters with
+     * lum_shift_enable (or loaded nonzero shifts) until the working buffers
+     * are reset; before that every shift is 0 */
+    bool lum_live = false;
     bool no_early_exit = false;            /* force the plain candidate kernels (tests) */
     /* Early rejection lengthens each lane's dependent chain; it pays only on
      * levels with enough waves to hide the latency. */
@@ -236,7 +242,6 @@ static int check_supported(pt_ctx *c, const pt_params *p)
         return set_err(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
     if (p->colorspace != 0 && p->colorspace != 1)
         return set_err(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
-    if (p->lum_shift_enable) return set_err(c, PT_ERR_UNSUPPORTED, "luminance shift is not supported");
     if (p->interpolate_key_frame)
         return set_err(c, PT_ERR_UNSUPPORTED, "keyframe interpolation is not supported");
     for (int i = 0; i < 5; i++)
@@ -331,6 +336,7 @@ static void free_snaps(pt_ctx *c)
     for (auto &s : c->snaps) {
         dfree(s.xyth);
         dfree(s.sc);
+        dfree(s.lum);
         dfree(s.res);
         dfree(s.canvas);
     }
@@ -344,6 +350,8 @@ static void free_working(pt_ctx *c)
     dfree(c->base.sc);
     dfree(c->work.xyth);
     dfree(c->work.sc);
+    dfree(c->base.lum);
+    dfree(c->work.lum);
     dfree(c->res_base);
     dfree(c->res_work);
     dfree(c->canvas);
@@ -355,6 +363,7 @@ static void free_working(pt_ctx *c)
         dfree(c->adv[k].out);
         dfree(c->adv[k].res);
         dfree(c->adv[k].ts);
+        dfree(c->adv[k].lum);
         dfree(c->color[k]);
         dfree(c->dt[k]);
         dfree(c->velb[k]);
@@ -586,6 +595,7 @@ extern "C" int pt_upload_parameters(pt_ctx *c, const pt_params *p)
     RC_CHECK(check_supported(c, p));
     c->p = *p;
     c->have_params = true;
+    if (p->lum_shift_enable) c->lum_live = true;
     res_invalidate(c);
     return PT_OK;
 }
@@ -630,6 +640,8 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
     RC_CHECK(dalloc(c, &c->base.sc, n));
     RC_CHECK(dalloc(c, &c->work.xyth, n));
     RC_CHECK(dalloc(c, &c->work.sc, n));
+    RC_CHECK(dalloc(c, &c->base.lum, n));
+    RC_CHECK(dalloc(c, &c->work.lum, n));
     RC_CHECK(dalloc(c, &c->res_base, n));
     RC_CHECK(dalloc(c, &c->res_work, n));
     RC_CHECK(dalloc(c, &c->canvas, n));
@@ -641,6 +653,7 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
         RC_CHECK(dalloc(c, &c->adv[k].out, n));
         RC_CHECK(dalloc(c, &c->adv[k].res, n));
         RC_CHECK(dalloc(c, &c->adv[k].ts, n));
+        RC_CHECK(dalloc(c, &c->adv[k].lum, n));
         RC_CHECK(dalloc(c, &c->color[k], n));
         RC_CHECK(dalloc(c, &c->dt[k], n));
         RC_CHECK(dalloc(c, &c->velb[k], n));
@@ -659,6 +672,7 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
     c->adv_written[0] = c->adv_written[1] = false;
     c->cur = 0;
     c->res_level = -1;
+    c->lum_live = c->have_params && c->p.lum_shift_enable;
     return sync_stream(c);
 }
 
@@ -705,6 +719,7 @@ static int reset_advected(pt_ctx *c)
         RC_CHECK(zero_fill(c, c->adv[k].out, n * sizeof(float4)));
         RC_CHECK(zero_fill(c, c->adv[k].res, n * sizeof(float)));
         RC_CHECK(zero_fill(c, c->adv[k].ts, n * sizeof(int32_t)));
+        RC_CHECK(zero_fill(c, c->adv[k].lum, n * sizeof(float)));
         c->adv_written[k] = false;
     }
     return PT_OK;
@@ -1067,6 +1082,8 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.coh_angle = p.coherence_angle_range;
     a.lab = p.colorspace == 1 ? 1 : 0;
     a.obj_w = p.style_obj_id_weight > 0.f ? p.style_obj_id_weight : 0.f;
+    a.lum_live = c->lum_live ? 1 : 0;
+    a.lum_enable = p.lum_shift_enable ? 1 : 0;
     a.idw = c->idw;
     a.idh = c->idh;
     a.hyst_w = p.hysteresis_weight;
@@ -1101,6 +1118,7 @@ static PtArgs make_args(const pt_ctx *c, int level)
         a.adv[k].out = c->adv[k].out;
         a.adv[k].res = c->adv[k].res;
         a.adv[k].ts = c->adv[k].ts;
+        a.adv[k].lum = c->adv[k].lum;
         a.color[k] = c->color[k];
         a.dt[k] = c->dt[k];
         a.velb[k] = c->velb[k];
@@ -1122,9 +1140,13 @@ static int copy_nnf_rect(pt_ctx *c, const PtNnf &dst, const PtNnf &src, Rect r)
 {
     if (r.empty()) return PT_OK;
     const size_t W = c->W;
+    const bool lum = c->lum_live;
     return issue(c, "offset copy", [=](hipStream_t s) {
         hipError_t e = hipMemcpy2DAsync(dst.xyth, W * sizeof(float4), src.xyth, W * sizeof(float4),
                                         (size_t)r.w * sizeof(float4), r.h, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && lum)
+            e = hipMemcpy2DAsync(dst.lum, W * sizeof(float), src.lum, W * sizeof(float), (size_t)r.w * sizeof(float),
+                                 r.h, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
         return hipMemcpy2DAsync(dst.sc, W * sizeof(float2), src.sc, W * sizeof(float2), (size_t)r.w * sizeof(float2),
                                 r.h, hipMemcpyDeviceToDevice, s);
@@ -1268,8 +1290,10 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
     size_t n = (size_t)c->W * c->H;
     /* dest.offsets -> base, whole buffer (texSynth_kernel.cu:1716) */
     PtNnf base = c->base;
+    const bool lum = c->lum_live;
     RC_CHECK(issue(c, "advected offset copy", [=](hipStream_t s) {
         hipError_t e = hipMemcpyAsync(base.xyth, d.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && lum) e = hipMemcpyAsync(base.lum, d.lum, n * sizeof(float), hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
         return hipMemcpyAsync(base.sc, d.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
     }));
@@ -1399,7 +1423,9 @@ extern "C" int pt_get_output(pt_ctx *c, int level, pt_patchxf *offsets, float *c
         return set_err(c, PT_ERR_INVALID_ARG, "bad get_output");
     HIP_CHECK(c, hipSetDevice(c->device));
     int lw = c->W >> level, lh = c->H >> level;
-    if (offsets) RC_CHECK(get_nnf(c, c->base, lw, lh, offsets));
+    PtNnf b = c->base;
+    if (!c->lum_live) b.lum = nullptr;
+    if (offsets) RC_CHECK(get_nnf(c, b, lw, lh, offsets));
     if (canvas) RC_CHECK(get_plane(c, c->canvas, lw, lh, (float4 *)canvas));
     if (residual) RC_CHECK(get_plane(c, c->res_base, lw, lh, residual));
     return sync_stream(c);
@@ -1413,7 +1439,7 @@ extern "C" int pt_get_advected(pt_ctx *c, int level, int which, pt_patchxf *offs
     HIP_CHECK(c, hipSetDevice(c->device));
     int lw = c->W >> level, lh = c->H >> level;
     DevAdv &d = c->adv[which];
-    PtNnf n{d.xyth, d.sc};
+    PtNnf n{d.xyth, d.sc, c->lum_live ? d.lum : nullptr};
     if (offsets) RC_CHECK(get_nnf(c, n, lw, lh, offsets));
     if (base) RC_CHECK(get_plane(c, d.base, lw, lh, (float4 *)base));
     if (output) RC_CHECK(get_plane(c, d.out, lw, lh, (float4 *)output));
@@ -1429,13 +1455,19 @@ extern "C" int pt_load_offsets(pt_ctx *c, int level, const pt_patchxf *in)
     HIP_CHECK(c, hipSetDevice(c->device));
     res_invalidate(c);
     int lw = c->W >> level, lh = c->H >> level;
-    for (size_t i = 0; i < (size_t)lw * lh; i++)
-        if (in[i].layer != 0 || in[i].lum_y != 0.f || in[i].lum_a != 0.f)
-            return set_err(c, PT_ERR_UNSUPPORTED, "offsets with layer or luminance shift");
+    bool shifted = false;
+    for (size_t i = 0; i < (size_t)lw * lh; i++) {
+        if (in[i].layer != 0 || in[i].lum_a != 0.f)
+            return set_err(c, PT_ERR_UNSUPPORTED, "offsets with a style layer or an alpha luminance shift");
+        shifted = shifted || in[i].lum_y != 0.f;
+    }
+    if (shifted) c->lum_live = true;
+    PtNnf b = c->base;
+    if (!c->lum_live) b.lum = nullptr;
     pt_patchxf *tmp = nullptr;
     RC_CHECK(dalloc(c, &tmp, (size_t)lw * lh));
     hipError_t e = hipMemcpyAsync(tmp, in, (size_t)lw * lh * sizeof(pt_patchxf), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = pt_launch_unpack_nnf(tmp, lw, lh, c->base, c->W, c->stream);
+    if (e == hipSuccess) e = pt_launch_unpack_nnf(tmp, lw, lh, b, c->W, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(tmp);
     if (e != hipSuccess) return set_err(c, PT_ERR_HIP, "offset load failed: %s", hipGetErrorString(e));
@@ -1485,6 +1517,7 @@ static int snap_alloc(pt_ctx *c, int levels, int frames, int passes)
                 size_t n = (size_t)s.w * s.h;
                 RC_CHECK(dalloc(c, &s.xyth, n));
                 RC_CHECK(dalloc(c, &s.sc, n));
+                RC_CHECK(dalloc(c, &s.lum, n));
                 RC_CHECK(dalloc(c, &s.res, n));
                 RC_CHECK(dalloc(c, &s.canvas, n));
             }
@@ -1505,10 +1538,14 @@ static int snap_save(pt_ctx *c, int level, Snapshot &sn)
     float *rb = c->res_base;
     float4 *cv = c->canvas;
     size_t W = c->W;
+    const bool lum = c->lum_live;
     (void)level;
     return issue(c, "snapshot save", [=](hipStream_t st) {
         hipError_t e = hipMemcpy2DAsync(d.xyth, (size_t)d.w * sizeof(float4), base.xyth, W * sizeof(float4),
                                         (size_t)d.w * sizeof(float4), d.h, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && lum)
+            e = hipMemcpy2DAsync(d.lum, (size_t)d.w * sizeof(float), base.lum, W * sizeof(float),
+                                 (size_t)d.w * sizeof(float), d.h, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess)
             e = hipMemcpy2DAsync(d.sc, (size_t)d.w * sizeof(float2), base.sc, W * sizeof(float2),
                                  (size_t)d.w * sizeof(float2), d.h, hipMemcpyDeviceToDevice, st);
@@ -1529,9 +1566,13 @@ static int snap_load_offsets(pt_ctx *c, const Snapshot &sn)
     Snapshot d = sn;
     PtNnf base = c->base;
     size_t W = c->W;
+    const bool lum = c->lum_live;
     RC_CHECK(issue(c, "snapshot offset load", [=](hipStream_t st) {
         hipError_t e = hipMemcpy2DAsync(base.xyth, W * sizeof(float4), d.xyth, (size_t)d.w * sizeof(float4),
                                         (size_t)d.w * sizeof(float4), d.h, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && lum)
+            e = hipMemcpy2DAsync(base.lum, W * sizeof(float), d.lum, (size_t)d.w * sizeof(float),
+                                 (size_t)d.w * sizeof(float), d.h, hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) return e;
         return hipMemcpy2DAsync(base.sc, W * sizeof(float2), d.sc, (size_t)d.w * sizeof(float2),
                                 (size_t)d.w * sizeof(float2), d.h, hipMemcpyDeviceToDevice, st);
@@ -1629,8 +1670,10 @@ static int copy_base_to_advected_f(pt_ctx *c)
     DevAdv d = c->adv[0];
     float *rb = c->res_base;
     size_t n = (size_t)c->W * c->H;
+    const bool lum = c->lum_live;
     RC_CHECK(issue(c, "copy to advected", [=](hipStream_t st) {
         hipError_t e = hipMemcpyAsync(d.xyth, base.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && lum) e = hipMemcpyAsync(d.lum, base.lum, n * sizeof(float), hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(d.sc, base.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(d.res, rb, n * sizeof(float), hipMemcpyDeviceToDevice, st);
         return e;
@@ -2129,7 +2172,7 @@ static int read_snapshot(pt_ctx *c, const Snapshot &s, pt_patchxf *out_off, floa
             RC_CHECK(dalloc(c, &c->pack_tmp, n));
             c->pack_cap = n;
         }
-        PtNnf nn{s.xyth, s.sc};
+        PtNnf nn{s.xyth, s.sc, c->lum_live ? s.lum : nullptr};
         int w = s.w, h = s.h;
         pt_patchxf *tmp = c->pack_tmp, *dst = out_off;
         if (on_device) {
@@ -2180,6 +2223,9 @@ static int reset_working(pt_ctx *c)
     RC_CHECK(zero_fill(c, c->base.sc, n * sizeof(float2)));
     RC_CHECK(zero_fill(c, c->work.xyth, n * sizeof(float4)));
     RC_CHECK(zero_fill(c, c->work.sc, n * sizeof(float2)));
+    RC_CHECK(zero_fill(c, c->base.lum, n * sizeof(float)));
+    RC_CHECK(zero_fill(c, c->work.lum, n * sizeof(float)));
+    c->lum_live = false;
     RC_CHECK(zero_fill(c, c->res_base, n * sizeof(float)));
     RC_CHECK(zero_fill(c, c->res_work, n * sizeof(float)));
     RC_CHECK(zero_fill(c, c->canvas, n * sizeof(float4)));
@@ -2323,6 +2369,7 @@ static void free_seq(pt_ctx *c)
         for (auto &s : c->seq_snap[k]) {
             dfree(s.xyth);
             dfree(s.sc);
+            dfree(s.lum);
             dfree(s.res);
             dfree(s.canvas);
         }
@@ -2341,6 +2388,7 @@ static int seq_alloc(pt_ctx *c, int levels)
         for (auto &s : c->seq_snap[k]) {
             dfree(s.xyth);
             dfree(s.sc);
+            dfree(s.lum);
             dfree(s.res);
             dfree(s.canvas);
         }
@@ -2352,6 +2400,7 @@ static int seq_alloc(pt_ctx *c, int levels)
             size_t n = (size_t)s.w * s.h;
             RC_CHECK(dalloc(c, &s.xyth, n));
             RC_CHECK(dalloc(c, &s.sc, n));
+            RC_CHECK(dalloc(c, &s.lum, n));
             RC_CHECK(dalloc(c, &s.res, n));
             RC_CHECK(dalloc(c, &s.canvas, n));
         }
@@ -2527,10 +2576,11 @@ size_t pt_sequence_bytes(int W, int H, int SW, int SH, int levels)
 {
     (void)levels;
     const double px = (double)W * H, ex = (double)SW * SH;
-    /* working buffers 308 B/px; two frame pyramids 2 x 60 B/px x 4/3;
-     * two snapshot sets 2 x 44 B/px x 4/3; exemplar textures 40 B/texel;
+    /* working buffers 328 B/px (luminance-shift planes and the image-ID
+     * texture included); two frame pyramids 2 x 60 B/px x 4/3;
+     * two snapshot sets 2 x 48 B/px x 4/3; exemplar textures 40 B/texel;
      * style pyramids 52 B/texel x 4/3; plus 10% slack */
-    double b = px * (308.0 + 2 * 60.0 * 4.0 / 3.0 + 2 * 44.0 * 4.0 / 3.0) + ex * (40.0 + 52.0 * 4.0 / 3.0);
+    double b = px * (328.0 + 2 * 60.0 * 4.0 / 3.0 + 2 * 48.0 * 4.0 / 3.0) + ex * (40.0 + 52.0 * 4.0 / 3.0);
     return (size_t)(b * 1.1) + ((size_t)8 << 20);
 }
 

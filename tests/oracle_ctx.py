@@ -42,6 +42,8 @@ def load_oracle():
             getattr(lib, f).argtypes = [C.c_double]
         lib.ptref_rgb_to_lab.restype = None
         lib.ptref_rgb_to_lab.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)]
+        lib.ptref_luminance_delta.restype = C.c_float
+        lib.ptref_luminance_delta.argtypes = [C.c_float] * 6
         lib.ptref_coherence_term.restype = C.c_float
         lib.ptref_coherence_term.argtypes = [C.c_float] * 6 + [C.c_int] * 3 + [C.c_float] * 2
         lib.ptref_tensor_to_angle.restype = C.c_float

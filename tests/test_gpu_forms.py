@@ -64,6 +64,19 @@ def test_next_frame_with_object_ids_matches_oracle():
         _same(streamed[f], ref[f], "frame %d vs oracle" % f)
 
 
+def test_next_frame_with_luminance_shift_matches_oracle():
+    st = W.style_pair(64, 56, seed=47, guides=True)
+    frames = W.animation(48, 40, 3, seed=48)
+    kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, lum_shift_enable=1)
+    with Context.gpu(0) as g, OracleContext() as o:
+        streamed = _stream(g, g.default_params(**kw), 2, frames, st)
+        po = o.default_params(**kw)
+        ref = [o.next_frame(po, 2, f, st) for f in frames]
+    for f in range(len(frames)):
+        _same(streamed[f], ref[f], "frame %d vs oracle" % f)
+    assert (streamed[2][0]["lum_y"] != 0).any()
+
+
 def test_first_streamed_frame_is_the_still_schedule():
     """Frame 0 of a sequence runs exactly the one-frame reference schedule."""
     st = W.style_pair(64, 56, seed=45, guides=True)

@@ -204,6 +204,46 @@ def test_exemplar_window_refresh(r, window):
     assert_same_states(ga, oa)
 
 
+@pytest.mark.parametrize("merge,early_exit,unit_parallel,tap", [("merge_random", 1, 1, 0), ("merge_residual", 1, 1, 0),
+                                                               ("merge_random", 0, 0, 0), ("merge_random", 1, 1, 1024)])
+def test_luminance_shift_operator_sequence(merge, early_exit, unit_parallel, tap):
+    """lumShiftEnable: advection accumulates each offset's luminance shift,
+    propagation and scatter reset it on the pixels they move, merges carry it,
+    the canvas adds it (texSynth_kernel.cu:272, 1626-1631)."""
+    st = W.style_pair(48, 48, seed=35, guides=True)
+    frames = W.animation(40, 32, 3, seed=36)
+    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=0.3,
+              lum_shift_enable=1)
+    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, merge), early_exit, unit_parallel, tap)
+    assert_same_states(ga, oa)
+    assert any((state[0]["lum_y"] != 0).any() for _, state in ga)
+
+
+def test_loaded_luminance_shifts():
+    """Offsets loaded with shifts (pt_load_offsets) through propagate and
+    scatter, against the oracle."""
+    st = W.style_pair(40, 36, seed=37, guides=True)
+    fr = W.still_frame(44, 34, seed=38, guides=True)
+    rng = np.random.default_rng(39)
+    outs = []
+    for ctx in (Context.gpu(0), OracleContext()):
+        try:
+            s = Session(ctx, st, [fr], ctx.default_params(residual_window_size=2, levels=1))
+            s.run([("style", 0), ("frame", 0, 0), ("randomize",)])
+            off, _, _ = s.state(0)
+            off["lum_y"] = rng.uniform(-0.2, 0.2, off.shape).astype(np.float32) if not outs else outs[0][0]["lum_y"]
+            ctx.load_offsets(0, off)
+            ctx.sync_buffers(0, _ffi.PT_DONT_COPY_OFFSETS)
+            res = [s.state(0)]
+            res += [st_ for _, st_ in s.run([("propagate", 4), ("propagate", 1), ("scatter",), ("propagate", 1)])]
+            outs.append((off, res))
+        finally:
+            ctx.close()
+    for a, b in zip(outs[0][1], outs[1][1]):
+        for x, y in zip(a, b):
+            assert first_difference(x, y) is None
+
+
 def _with_ids(st, frames, seed):
     """Attach object-ID images (Voronoi cells) to a style and its frames."""
     sh, sw = st["base"].shape[:2]
@@ -345,6 +385,17 @@ def test_synthesize_object_ids_bit_exact(scheme):
         kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, style_obj_id_weight=0.7)
         _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1),
                    o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
+
+
+@pytest.mark.parametrize("scheme", [_ffi.PT_S_FMBM_RM, _ffi.PT_S_FMBM, _ffi.PT_S_FBMO])
+def test_synthesize_luminance_shift_bit_exact(scheme):
+    st = W.style_pair(64, 64, seed=40, guides=True)
+    frames = W.animation(48, 40, 3, seed=41)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, lum_shift_enable=1)
+        a = g.synthesize(g.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1)
+        _cmp_synth(a, o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
+    assert (a[0]["lum_y"] != 0).any()
 
 
 @pytest.mark.parametrize("fixup", [0, 1])

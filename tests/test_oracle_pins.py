@@ -621,3 +621,81 @@ def test_object_ids_absent_or_equal_change_nothing():
     for a, b, c in zip(base, off, eq):
         assert a.tobytes() == b.tobytes() == c.tobytes()
     assert base[0].tobytes() != diff[0].tobytes()
+
+
+# ---------------------------------------------------------------- luminance shift
+def test_luminance_delta_matches_float64():
+    """Color4::toLuminance of a colour difference (types.h:87-89): the float
+    differences, then 0.2126 r + 0.7152 g + 0.0722 b in double, to float."""
+    lib = load_oracle()
+    rng = np.random.default_rng(61)
+    for _ in range(2000):
+        c = rng.uniform(-0.5, 1.5, 3).astype(np.float32)
+        p = rng.uniform(-0.5, 1.5, 3).astype(np.float32)
+        d = (c - p).astype(np.float32)
+        want = np.float32((0.2126 * float(d[0]) + 0.7152 * float(d[1])) + 0.0722 * float(d[2]))
+        got = lib.ptref_luminance_delta(*(float(v) for v in c), *(float(v) for v in p))
+        assert np.float32(got) == want
+    assert lib.ptref_luminance_delta(0.3, 0.4, 0.5, 0.3, 0.4, 0.5) == 0.0
+
+
+def _lum_frames():
+    f0 = W.still_frame(24, 20, seed=62)
+    f1 = dict(f0, color=f0["color"].copy())
+    f1["color"][..., :3] += np.float32(0.1)
+    return f0, f1
+
+
+def test_luminance_shift_advection_and_canvas():
+    """With lumShiftEnable, an offset advected along a zero velocity field
+    carries the source's shift plus the luminance change of its pixel
+    between the frames (texSynth_kernel.cu:1626-1631), and the canvas adds
+    the shift to R, G and B (toColor4(c, lum), :272)."""
+    st = W.style_pair(32, 28, seed=63)
+    f0, f1 = _lum_frames()
+    ops = [("style", 0), ("reset_adv",), ("frame", 0, 0), ("randomize",), ("sync", 0), ("flip",), ("frame", 0, 1),
+           ("params", dict(direction=_ffi.PT_FORWARD, frame_number=1)), ("advect", 1, 0)]
+    out = {}
+    for on in (0, 1):
+        with OracleContext() as o:
+            s = Session(o, st, [f0, f1], o.default_params(residual_window_size=2, levels=1, lum_shift_enable=on))
+            out[on] = s.run(ops)[-1][1]
+    off1, can1, _ = out[1]
+    off0, can0, _ = out[0]
+    lib = load_oracle()
+    valid = off1["hysteresis"] > 0
+    assert valid.mean() > 0.9
+    for y, x in zip(*np.nonzero(valid)):
+        c, p = f1["color"][y, x], f0["color"][y, x]
+        assert off1["lum_y"][y, x] == np.float32(lib.ptref_luminance_delta(*map(float, c[:3]), *map(float, p[:3])))
+    assert (off1["lum_y"][~valid] == 0).all() and (off0["lum_y"] == 0).all()
+    assert abs(off1["lum_y"][valid].mean() - 0.1) < 1e-6  # the coefficients sum to 1
+    for k in off0.dtype.names:
+        if k != "lum_y":
+            assert off0[k].tobytes() == off1[k].tobytes(), k
+    np.testing.assert_array_equal(can1[..., :3], (can0[..., :3] + off1["lum_y"][..., None]).astype(np.float32))
+    np.testing.assert_array_equal(can1[..., 3], can0[..., 3])
+
+
+def test_luminance_shift_kept_or_reset_by_propagation():
+    """A pixel that keeps its offset keeps its shift; one that takes a
+    propagated candidate gets Luminance() (0)."""
+    st = W.style_pair(32, 28, seed=64)
+    f0, _ = _lum_frames()
+    with OracleContext() as o:
+        s = Session(o, st, [f0], o.default_params(residual_window_size=2, levels=1, lum_shift_enable=1))
+        s.run([("style", 0), ("frame", 0, 0), ("randomize",)])
+        off, _, _ = s.state(0)
+        off["lum_y"] = np.float32(0.25)
+        o.load_offsets(0, off)
+        o.sync_buffers(0, _ffi.PT_DONT_COPY_OFFSETS)
+        o.propagate(0, 1)
+        o.sync_buffers(0)
+        new, _, _ = s.state(0)
+        moved = (new["x"] != off["x"]) | (new["y"] != off["y"]) | (new["theta"] != off["theta"])
+        assert 0.05 < moved.mean() < 0.95
+        assert (new["lum_y"][moved] == 0).all() and (new["lum_y"][~moved] == np.float32(0.25)).all()
+        bad = off.copy()
+        bad["lum_a"][0, 0] = 0.5
+        with pytest.raises(Exception):
+            o.load_offsets(0, bad)
