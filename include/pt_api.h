@@ -192,7 +192,12 @@ int pt_upload_parameters(pt_ctx *ctx, const pt_params *params);
  * offsets-histogram buffer whose dimensions drive the patch wrap). */
 int pt_initialize(pt_ctx *ctx, int image_w, int image_h, int style_w, int style_h);
 /* TexSynth::initialize_exemplars (texSynth_kernel.cu:2119-2126): exemplar
- * textures of one level (w, h are that level's dimensions). */
+ * textures of one level (w, h are that level's dimensions), num_layers in
+ * 1..4 (the number of styles).  Without keyframe interpolation every offset
+ * stays in style layer 0 (random initialisation, scatter and the average
+ * candidate use style 0, synthesisFrame.cpp:107-110, texSynth_kernel.cu:925,
+ * 939), so only layer 0 is stored; more styles enlarge the patch wrap height
+ * to style_h * num_styles, as the reference's offsets histogram does. */
 int pt_initialize_exemplars(pt_ctx *ctx, int w, int h, int num_layers);
 /* WorkingBuffers::resetAdvectedBuffers (workingBuffers.cpp:113-117). */
 int pt_reset_advected_buffers(pt_ctx *ctx);
@@ -231,7 +236,9 @@ int pt_pyramid_download(const pt_pyramid *pyr, int level, void *host);
  * element kind must match the layer (color/exemplar/random: F32X4, velocity
  * and scale: F32X2, orientation and distance transform: F32, INPUT_ID and
  * EXEMPLAR_OBJECT_IDS: I32).  Until INPUT_ID is uploaded the image ID
- * texture is a single 0 texel. */
+ * texture is a single 0 texel.  style_layer (0..3) selects the style of an
+ * exemplar layer; layers above 0 are accepted and never sampled (see
+ * pt_initialize_exemplars); image layers ignore it. */
 int pt_upload_image(pt_ctx *ctx, int level, int layer, const pt_pyramid *pyr, int style_layer);
 
 /* SynthesisFrame::createRandomPyramid (synthesisFrame.cpp:66-150) on the GPU

@@ -591,8 +591,8 @@ static int check_supported(ptref_ctx *c, const pt_params *p)
 {
     if (p->residual_window_size < 1 || p->residual_window_size > 15)
         return fail(c, PT_ERR_INVALID_ARG, "residual_window_size must be in [1, 15]");
-    if (p->num_styles != 1)
-        return fail(c, PT_ERR_UNSUPPORTED, "only one style layer is supported");
+    if (p->num_styles < 1 || p->num_styles > 4)
+        return fail(c, PT_ERR_INVALID_ARG, "num_styles must be in [1, 4]");
     if (p->coherence_weight > 0.f && !(p->coherence_ratio_range > 1.f && p->coherence_angle_range > 0.f))
         return fail(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
     if (p->colorspace != 0 && p->colorspace != 1)
@@ -685,7 +685,7 @@ int ptref_initialize(ptref_ctx *c, int iw, int ih, int sw, int sh)
 
 int ptref_initialize_exemplars(ptref_ctx *c, int w, int h, int num_layers)
 {
-    if (!c || w < 1 || h < 1 || num_layers != 1) return PT_ERR_INVALID_ARG;
+    if (!c || w < 1 || h < 1 || num_layers < 1 || num_layers > 4) return PT_ERR_INVALID_ARG;
     if (c->ex_base && c->ew == w && c->eh == h) return PT_OK;
     free(c->ex_base); free(c->ex_out); free(c->ex_orient); free(c->ex_dt); free(c->ex_id);
     c->ew = w; c->eh = h;
@@ -847,7 +847,11 @@ static float *realloc_tex(float *old, size_t n)
 int ptref_upload_image(ptref_ctx *c, int level, int layer, const ptref_pyramid *pyr, int style_layer)
 {
     if (!c || !pyr || level < 0 || level >= pyr->levels) return fail(c, PT_ERR_INVALID_ARG, "bad upload");
-    if (style_layer != 0) return fail(c, PT_ERR_UNSUPPORTED, "one style layer only");
+    if (style_layer < 0 || style_layer > 3) return fail(c, PT_ERR_INVALID_ARG, "style layer must be in [0, 3]");
+    /* style layers above 0 are never sampled without keyframe interpolation
+     * (every offset keeps layer 0: synthesisFrame.cpp:106-110,
+     * texSynth_kernel.cu:925,939) */
+    if (style_layer > 0 && layer >= PT_LAYER_EXEMPLAR_BASE && layer <= PT_LAYER_EXEMPLAR_OBJECT_IDS) return PT_OK;
     int w = pyr->lw[level], h = pyr->lh[level];
     const float *src = pyr->data[level];
     int prev = (c->cur + 1) % 2;

@@ -237,7 +237,8 @@ static int check_supported(pt_ctx *c, const pt_params *p)
 {
     if (p->residual_window_size < 1 || p->residual_window_size > PT_MAX_R)
         return set_err(c, PT_ERR_INVALID_ARG, "residual_window_size must be in [1, %d]", PT_MAX_R);
-    if (p->num_styles != 1) return set_err(c, PT_ERR_UNSUPPORTED, "only one style layer is supported");
+    /* NUM_MAX_STYLES 5, numStyles < 5 (synthesisProcessor.cpp:192) */
+    if (p->num_styles < 1 || p->num_styles > 4) return set_err(c, PT_ERR_INVALID_ARG, "num_styles must be in [1, 4]");
     if (p->coherence_weight > 0.f && !(p->coherence_ratio_range > 1.f && p->coherence_angle_range > 0.f))
         return set_err(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
     if (p->colorspace != 0 && p->colorspace != 1)
@@ -679,7 +680,7 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
 extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
 {
     if (!c || w < 1 || h < 1) return set_err(c, PT_ERR_INVALID_ARG, "bad exemplar dimensions");
-    if (num_layers != 1) return set_err(c, PT_ERR_UNSUPPORTED, "one style layer only");
+    if (num_layers < 1 || num_layers > 4) return set_err(c, PT_ERR_INVALID_ARG, "num_layers must be in [1, 4]");
     HIP_CHECK(c, hipSetDevice(c->device));
     res_invalidate(c);
     if (c->ex && c->ew == w && c->eh == h) return PT_OK;
@@ -884,7 +885,13 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
 {
     if (!c || !p || level < 0 || level >= p->levels) return set_err(c, PT_ERR_INVALID_ARG, "bad upload");
     if (!c->base.xyth) return set_err(c, PT_ERR_STATE, "not initialized");
-    if (style_layer != 0) return set_err(c, PT_ERR_UNSUPPORTED, "one style layer only");
+    if (style_layer < 0 || style_layer > 3) return set_err(c, PT_ERR_INVALID_ARG, "style layer must be in [0, 3]");
+    /* Style layers above 0 are never sampled without keyframe interpolation:
+     * random initialisation, scatter and the average candidate all use style
+     * 0 (synthesisFrame.cpp:106-110, texSynth_kernel.cu:925,939) and every
+     * other offset inherits its layer, so their uploads are accepted and
+     * have nothing to bind. */
+    if (style_layer > 0 && layer >= PT_LAYER_EXEMPLAR_BASE && layer <= PT_LAYER_EXEMPLAR_OBJECT_IDS) return PT_OK;
     int w = p->lw[level], h = p->lh[level];
     const float *src = p->data + p->off[level];
     int prev = (c->cur + 1) % 2;

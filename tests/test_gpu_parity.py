@@ -244,6 +244,28 @@ def test_loaded_luminance_shifts():
             assert first_difference(x, y) is None
 
 
+@pytest.mark.parametrize("styles", [2, 4])
+def test_several_styles_operator_sequence(styles):
+    """numStyles > 1 without keyframes: offsets stay in style 0, taps wrap at
+    style height x numStyles (clamping to the exemplar's last row below it)."""
+    st = W.style_pair(40, 36, seed=42, guides=True)
+    fr = W.still_frame(44, 34, seed=43, guides=True)
+    ov = dict(residual_window_size=3, levels=3, num_styles=styles, orientation_enabled=1, dist_trans_weight=0.5)
+    ga, oa = _both(st, [fr], ov, _still_ops(3))
+    assert_same_states(ga, oa)
+    ga1, _ = _both(st, [fr], dict(ov, num_styles=1), _still_ops(3)[:5])
+    assert ga[4][1][2].tobytes() != ga1[4][1][2].tobytes()  # the wrap changed the residuals
+
+
+def test_synthesize_several_styles_bit_exact():
+    st = W.style_pair(64, 64, seed=44, guides=True)
+    frames = W.animation(48, 40, 3, seed=45)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, num_styles=3)
+        _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, iterations=2),
+                   o.synthesize(o.default_params(**kw), frames, st, iterations=2))
+
+
 def _with_ids(st, frames, seed):
     """Attach object-ID images (Voronoi cells) to a style and its frames."""
     sh, sw = st["base"].shape[:2]

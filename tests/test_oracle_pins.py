@@ -100,7 +100,7 @@ def _coh_double(cx, cy, tcx, tcy, tx, ty, i, j, level, rr, ar):
     return 1.0 - (1.0 - er) * (1.0 - ea)
 
 
-def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, off=None, level=0, ids=None):
+def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, off=None, level=0, ids=None, styles=1):
     """Independent float64 statement of calculateResidual
     (texSynth_kernel.cu:537-659) without temporal terms; the coherence term
     (:642-646) against the offset field `off` when its weight is positive; the
@@ -109,6 +109,7 @@ def _pde_double(x, y, cand, target, canvas, exb, exo, exdt, imgdt, g, prm, off=N
     r = prm.residual_window_size
     lh, lw = target.shape[:2]
     hh, hw = exb.shape[:2]
+    hh *= styles  # the wrap height is the offsets histogram's: style height x numStyles
     th = float(cand["theta"])
     su = prm.residual_window_scale_u * float(cand["scale_u"])
     sv = prm.residual_window_scale_v * float(cand["scale_v"])
@@ -699,3 +700,53 @@ def test_luminance_shift_kept_or_reset_by_propagation():
         bad["lum_a"][0, 0] = 0.5
         with pytest.raises(Exception):
             o.load_offsets(0, bad)
+
+
+# ---------------------------------------------------------------- several styles
+@pytest.mark.parametrize("styles", [2, 3])
+def test_patch_distance_wraps_at_style_height_times_styles(styles):
+    """With numStyles > 1 the tap coordinates wrap at the offsets histogram's
+    height, styleHeight x numStyles (workingBuffers.cpp:57,86,
+    texSynth_kernel.cu:595-600): taps below the exemplar clamp to its last row
+    instead of wrapping to the top."""
+    st = W.style_pair(24, 20, seed=53)
+    fr = W.still_frame(22, 18, seed=54)
+    rng = np.random.default_rng(55)
+    with OracleContext() as o:
+        vals = {}
+        for n in (1, styles):
+            prm = o.default_params(residual_window_size=3, levels=1, num_styles=n)
+            s = Session(o, st, [fr], prm)
+            s.run([("style", 0), ("frame", 0, 0), ("randomize",)])
+            off, canvas, _ = s.state()
+            exb, exo = s.sb.download(0), s.so.download(0)
+            target = s.frames[0]["color"].download(0)
+            g = o.gaussian_weights(3)
+            got = []
+            for k in range(12):
+                cand = np.zeros((), _ffi.PATCHXF_DTYPE)
+                cand["x"] = rng.uniform(4, 20)
+                cand["y"] = rng.uniform(17.5, 19.5)  # taps cross the bottom edge
+                cand["theta"] = rng.uniform(-0.3, 0.3)
+                cand["scale_u"] = cand["scale_v"] = 1.0
+                x, y = int(rng.integers(3, 19)), int(rng.integers(3, 15))
+                v = o.patch_distance(x, y, 0, cand)
+                exp = _pde_double(x, y, cand, target, canvas, exb, exo, None, None, g, prm, styles=n)
+                assert abs(v - exp) <= 3e-6 * max(exp, 1e-3), (n, x, y, v, exp)
+                got.append(v)
+            vals[n] = got
+            rng = np.random.default_rng(55)
+        assert vals[1] != vals[styles]
+
+
+def test_style_layers_above_zero_are_accepted():
+    with OracleContext() as o:
+        st = W.style_pair(24, 20, seed=56)
+        fr = W.still_frame(22, 18, seed=57)
+        s = Session(o, st, [fr], o.default_params(residual_window_size=2, levels=1, num_styles=3))
+        o.initialize_exemplars(24, 20, 3)
+        o.upload_image(0, "EXEMPLAR_BASE", s.sb, style_layer=2)
+        with pytest.raises(Exception):
+            o.upload_image(0, "EXEMPLAR_BASE", s.sb, style_layer=4)
+        with pytest.raises(Exception):
+            o.setup(o.default_params(num_styles=5))
