@@ -139,6 +139,12 @@ typedef struct pt_params {
     int32_t rng_mode;                  /* PT_RNG_REFERENCE / PT_RNG_COUNTER  */
     uint32_t seed;                     /* Philox key word 0                  */
     int32_t frame_number;              /* frame used to key the RNG          */
+    int32_t first_pass_direction;      /* firstPassDirection of the schedules
+                                          (SynthesisProcessor, 0 = PT_FORWARD
+                                          first, 1 = backward first): pass p
+                                          runs forwards iff p % 2 equals it
+                                          (FMBM, FMBM_RM, FBMO; TCTF and the
+                                          per-frame schedule take 0 only)   */
 } pt_params;
 
 /*

@@ -604,6 +604,8 @@ static int check_supported(ptref_ctx *c, const pt_params *p)
             return fail(c, PT_ERR_UNSUPPORTED, "offset histogram term is not supported");
     if (p->rng_mode != PT_RNG_REFERENCE && p->rng_mode != PT_RNG_COUNTER)
         return fail(c, PT_ERR_INVALID_ARG, "unknown rng_mode");
+    if (p->first_pass_direction != 0 && p->first_pass_direction != 1)
+        return fail(c, PT_ERR_INVALID_ARG, "first_pass_direction must be 0 or 1");
     return PT_OK;
 }
 
@@ -1527,6 +1529,8 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
     if (!c || !params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return fail(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
     if (scheme < PT_S_FMBM || scheme > PT_S_IND) return fail(c, PT_ERR_UNSUPPORTED, "scheme");
+    if (scheme == PT_S_TCTF && params->first_pass_direction != 0)
+        return fail(c, PT_ERR_UNSUPPORTED, "TCTF with a backward first pass");
     int levels = params->levels;
     int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
     int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
@@ -1629,7 +1633,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
          * synthesizeConsecutive's first-pass branch */
         for (int pass = 0; pass < 2 && !rc; pass++) {
             rc = ptref_reset_advected_buffers(c);
-            int fwd = pass == 0;
+            int fwd = pass == params->first_pass_direction; /* synthesisProcessor.cpp:938 */
             for (int k = 0; k < F && !rc; k++) {
                 int f = fwd ? k : F - 1 - k;
                 int is_first = k == 0;
@@ -1863,8 +1867,8 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
         if (!rc && sid) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sid, 0);
         if (!rc) rc = ptref_reset_advected_buffers(c);
         for (int pass = 0; pass < passes && !rc; pass++) {
-            /* renderConsecutivePass (synthesisProcessor.cpp:703-753), firstPassDirection 0 */
-            int fwd = (pass % 2 == 0);
+            /* renderConsecutivePass (synthesisProcessor.cpp:703-753) */
+            int fwd = (pass % 2 == params->first_pass_direction); /* synthesisProcessor.cpp:707 */
             int order[4096];
             int nf = 0;
             if (F > 4096) { rc = fail(c, PT_ERR_INVALID_ARG, "too many frames"); break; }
@@ -1998,6 +2002,7 @@ int ptref_synthesize_next_frame(ptref_ctx *c, const pt_params *params, int itera
                                 pt_patchxf *out_off, float *out_canvas, float *out_res)
 {
     if (!c || !params || !frame || !style || iterations < 1) return fail(c, PT_ERR_INVALID_ARG, "bad frame arguments");
+    if (params->first_pass_direction != 0) return fail(c, PT_ERR_UNSUPPORTED, "forwards only");
     int levels = params->levels;
     int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
     int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));

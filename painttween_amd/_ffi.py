@@ -79,6 +79,7 @@ class PtParams(C.Structure):
         ("rng_mode", C.c_int32),
         ("seed", C.c_uint32),
         ("frame_number", C.c_int32),
+        ("first_pass_direction", C.c_int32),
     ]
 
 

@@ -250,6 +250,8 @@ static int check_supported(pt_ctx *c, const pt_params *p)
             return set_err(c, PT_ERR_UNSUPPORTED, "offset histogram term is not supported");
     if (p->rng_mode != PT_RNG_REFERENCE && p->rng_mode != PT_RNG_COUNTER)
         return set_err(c, PT_ERR_INVALID_ARG, "unknown rng_mode");
+    if (p->first_pass_direction != 0 && p->first_pass_direction != 1)
+        return set_err(c, PT_ERR_INVALID_ARG, "first_pass_direction must be 0 or 1");
     return PT_OK;
 }
 
@@ -1711,7 +1713,7 @@ static int run_schedule_fbmo(pt_ctx *c, const pt_params *params, int iterations,
         RC_CHECK(load_style_level(c, level, sp));
         for (int pass = 0; pass < 2; pass++) {
             RC_CHECK(reset_advected(c));
-            bool fwd = pass == 0;
+            bool fwd = pass == params->first_pass_direction; /* synthesisProcessor.cpp:938 */
             for (int k = 0; k < F; k++) {
                 int f = fwd ? k : F - 1 - k;
                 bool is_first = k == 0;
@@ -1971,7 +1973,11 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
     RC_CHECK(pt_setup(c, params));
     RC_CHECK(snap_alloc(c, levels, F, passes));
     for (auto &sn : c->snaps) sn.saved = false;
-    if (scheme == PT_S_TCTF) return run_schedule_tctf(c, params, iterations, fixup_passes, F, fp, sp);
+    if (scheme == PT_S_TCTF) {
+        if (params->first_pass_direction != 0)
+            return set_err(c, PT_ERR_UNSUPPORTED, "TCTF with a backward first pass is not supported");
+        return run_schedule_tctf(c, params, iterations, fixup_passes, F, fp, sp);
+    }
     if (scheme == PT_S_IND) return run_schedule_ind(c, params, iterations, passes, F, fp, sp);
     if (scheme == PT_S_FBMO) return run_schedule_fbmo(c, params, iterations, fixup_passes, F, fp, sp);
     pt_params fparams = *params;
@@ -1985,8 +1991,8 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
         if (sp.id) RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_OBJECT_IDS, sp.id, 0));
         RC_CHECK(reset_advected(c));
         for (int pass = 0; pass < passes; pass++) {
-            /* renderConsecutivePass (synthesisProcessor.cpp:703-753), firstPassDirection 0 */
-            int fwd = (pass % 2 == 0);
+            /* renderConsecutivePass (synthesisProcessor.cpp:703-753) */
+            int fwd = (pass % 2 == params->first_pass_direction);
             std::vector<int> order;
             for (int i = 0; i < F; i++) order.push_back(fwd ? i : F - 1 - i);
             size_t start = 0;
@@ -2517,6 +2523,8 @@ static int next_frame_check(pt_ctx *c, const pt_params *params, int iterations, 
                             const pt_frame_guides *frame, int SW, int SH, const pt_style_guides *style)
 {
     if (!params || !frame || !style || iterations < 1) return set_err(c, PT_ERR_INVALID_ARG, "bad frame arguments");
+    if (params->first_pass_direction != 0)
+        return set_err(c, PT_ERR_UNSUPPORTED, "the per-frame schedule runs forwards only");
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     if (c->seq_frame > 0 && (c->W != W || c->H != H || c->SW != SW || c->SH != SH || c->seq_levels != params->levels))
         return set_err(c, PT_ERR_INVALID_ARG, "frame shape differs from the sequence; call pt_sequence_reset");

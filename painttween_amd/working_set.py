@@ -198,19 +198,18 @@ class WorkingSet:
         p.num_frames = self.last_frame - self.first_frame
         p.use_guide = int(bool(g("guidedInterpolation")))
         p.interpolate_key_frame = int(bool(g("keyFrameInterpolation")))
+        p.first_pass_direction = int(g("firstPassDirection"))  # synthesisProcessor.cpp:262
         return p
 
     def schedule(self):
         """The processor settings of an offline render (synthesisProcessor.cpp:225-262):
-        scheme, iterations per level, fixup passes and the frame range.  Settings
-        this path does not implement are rejected here; parameters it does not
-        implement are rejected by pt_setup."""
+        scheme, iterations per level, fixup passes and the frame range (the
+        first pass direction travels in pt_params).  Settings this path does
+        not implement are rejected here; parameters it does not implement are
+        rejected by pt_setup or pt_synthesize."""
         g = self.get
         if int(g("maxLevel")) != 0:
             raise WorkingSetError("maxLevel %d: only maxLevel 0 is implemented" % g("maxLevel"))
-        if int(g("firstPassDirection")) != 0:
-            raise WorkingSetError("firstPassDirection %d: only 0 (forwards first) is implemented"
-                                  % g("firstPassDirection"))
         if g("keyFrameInterpolation"):
             raise WorkingSetError("keyFrameInterpolation is not implemented")
         return dict(scheme=int(g("synthesisScheme")), iterations=int(g("iterations")),

@@ -420,6 +420,17 @@ def test_synthesize_luminance_shift_bit_exact(scheme):
     assert (a[0]["lum_y"] != 0).any()
 
 
+@pytest.mark.parametrize("scheme", [_ffi.PT_S_FMBM_RM, _ffi.PT_S_FMBM, _ffi.PT_S_FBMO])
+def test_synthesize_backward_first_pass_bit_exact(scheme):
+    """firstPassDirection 1: pass p runs forwards iff p % 2 == 1."""
+    st = W.style_pair(64, 64, seed=46, guides=True)
+    frames = W.animation(48, 40, 3, seed=47)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, first_pass_direction=1)
+        _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1),
+                   o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
+
+
 @pytest.mark.parametrize("fixup", [0, 1])
 def test_synthesize_ind_bit_exact(fixup):
     """renderAnimationIND: every frame on its own, levels inside frames."""

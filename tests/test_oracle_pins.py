@@ -750,3 +750,29 @@ def test_style_layers_above_zero_are_accepted():
             o.upload_image(0, "EXEMPLAR_BASE", s.sb, style_layer=4)
         with pytest.raises(Exception):
             o.setup(o.default_params(num_styles=5))
+
+
+# ---------------------------------------------------------------- first pass direction
+def test_backward_first_pass_starts_from_the_last_frame():
+    """firstPassDirection 1 (synthesisProcessor.cpp:707): pass 0 runs
+    backwards, so the last frame is synthesized first, as a first frame (random
+    initialisation, no advection): with one level (whose random pyramid does
+    not depend on the frame number) its result is the one-frame synthesis of
+    that frame, as frame 0's is with the forward first pass."""
+    st = W.style_pair(40, 36, seed=58, guides=True)
+    frames = W.animation(32, 28, 3, seed=59)
+    kw = dict(residual_window_size=2, levels=1, temporal_coherence_weight=0.5)
+    with OracleContext() as o:
+        out = {d: o.synthesize(o.default_params(first_pass_direction=d, **kw), frames, st,
+                               scheme=_ffi.PT_S_FMBM_RM, iterations=2, fixup_passes=0) for d in (0, 1)}
+        alone = {f: o.synthesize(o.default_params(**kw), [frames[f]], st, scheme=_ffi.PT_S_FMBM_RM, iterations=2,
+                                 fixup_passes=0) for f in (0, 2)}
+        for a, b in zip(out[1], alone[2]):
+            assert a[2].tobytes() == b[0].tobytes()
+        for a, b in zip(out[0], alone[0]):
+            assert a[0].tobytes() == b[0].tobytes()
+        assert out[1][0][2].tobytes() != out[0][0][2].tobytes()
+        with pytest.raises(Exception):
+            o.synthesize(o.default_params(first_pass_direction=1, **kw), frames, st, scheme=_ffi.PT_S_TCTF)
+        with pytest.raises(Exception):
+            o.setup(o.default_params(first_pass_direction=2))

@@ -159,7 +159,10 @@ def test_schedule_rejects_unimplemented_settings(tmp_path):
     path, _, _ = _make(tmp_path)
     ws = wsm.load(path)
     assert ws.schedule() == dict(scheme=_ffi.PT_S_FMBM, iterations=1, fixup_passes=1, first_frame=1, last_frame=2)
-    for k, v in (("maxLevel", 1), ("firstPassDirection", 1), ("keyFrameInterpolation", True)):
+    ws1 = wsm.load(path)
+    ws1.params["firstPassDirection"] = 1
+    assert ws1.pt_params().first_pass_direction == 1
+    for k, v in (("maxLevel", 1), ("keyFrameInterpolation", True)):
         ws2 = wsm.load(path)
         ws2.params[k] = v
         with pytest.raises(wsm.WorkingSetError, match=k):
