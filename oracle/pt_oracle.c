@@ -1045,7 +1045,7 @@ int ptref_randomize_offsets(ptref_ctx *c, int level)
     for (int y = 0; y < lh; y++)
         for (int x = 0; x < lw; x++) {
             f4 r = tex4_point(c->random, c->rw, c->rh, (float)x, (float)y);
-            nnf n = {r.x, r.y, r.z, 0.f, 1.f, 1.f};
+            nnf n = {r.x, r.y, r.z, 0.f, 1.f, 1.f, 0.f};
             c->off_work[y * c->W + x] = n;
         }
     return ptref_sync_buffers(c, level, 0);
@@ -1071,7 +1071,7 @@ static nnf propagate_offset(const ptref_ctx *c, int tx, int ty, float x, float y
     clamp_ex(&ex, &ey, c->ew, c->eh);
     float ang = screen_to_ex_angle(c, x, y, ex, ey);
     f2 s2 = img_scale_at(c, x, y);
-    nnf r = {ex, ey, ang, 0.f, s2.x, s2.y};
+    nnf r = {ex, ey, ang, 0.f, s2.x, s2.y, 0.f};
     return r;
 }
 
@@ -1088,7 +1088,7 @@ int ptref_propagate(ptref_ctx *c, int level, int jump, int key_index)
         for (int x = 0; x < lw; x++) {
             int o = y * c->W + x;
             if (image_culled(c, x, y)) {
-                nnf z = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f};
+                nnf z = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0.f};
                 c->off_work[o] = z;
                 continue;
             }
@@ -1116,7 +1116,7 @@ int ptref_propagate(ptref_ctx *c, int level, int jump, int key_index)
 /* averageOffset (texSynth_kernel.cu:841-862) */
 static nnf average_offset(const ptref_ctx *c, int x, int y, int lw, int lh)
 {
-    nnf a = {0.f, 0.f, 0.f, 1.f, 1.f, 1.f};
+    nnf a = {0.f, 0.f, 0.f, 1.f, 1.f, 1.f, 0.f};
     int sum = 0;
     for (int i = -1; i <= 1; i++)
         for (int j = -1; j <= 1; j++) {
@@ -1175,7 +1175,7 @@ int ptref_scatter(ptref_ctx *c, int level, int key_index)
                 float ey = ow.y + s * len;
                 clamp_ex(&ex, &ey, ew, eh);
                 float ang = screen_to_ex_angle(c, (float)x, (float)y, ex, ey);
-                nnf cand = {ex, ey, ang, 0.f, sc.x, sc.y};
+                nnf cand = {ex, ey, ang, 0.f, sc.x, sc.y, 0.f};
                 float res = test_offset(c, x, y, cand, level, ia);
                 if (res < best) { ow = cand; best = res; }
                 dist *= factor;
@@ -1242,7 +1242,7 @@ int ptref_upsample(ptref_ctx *c, int fine_level)
             float rw = c->res_base[lcy * c->W + lcx];
             float ang = screen_to_ex_angle(c, (float)x, (float)y, ux, uy);
             f2 sc = img_scale_at(c, (float)x, (float)y);
-            nnf n = {ux, uy, ang, 0.f, sc.x, sc.y};
+            nnf n = {ux, uy, ang, 0.f, sc.x, sc.y, 0.f};
             c->off_work[y * c->W + x] = n;
             c->res_work[y * c->W + x] = rw;
         }
@@ -1291,13 +1291,13 @@ int ptref_advect_offsets(ptref_ctx *c, int level, int time_step, int forwards, i
             int o = y * c->W + x;
             if (image_culled(c, x, y)) {
                 f4 z = {0, 0, 0, 0};
-                nnf dn = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f};
+                nnf dn = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0.f};
                 d->base[o] = z; d->out[o] = z; d->off[o] = dn;
                 continue;
             }
             f4 rnd = tex4_point(c->random, c->rw, c->rh, (float)x, (float)y);
             f4 frac = {rnd.x - floorf(rnd.x), rnd.y - floorf(rnd.y), rnd.z - floorf(rnd.z), rnd.w - floorf(rnd.w)};
-            nnf out = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f};
+            nnf out = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0.f};
             float tx, ty;
             if (!(flags & PT_ADVECT_DO_NOT_CHECK_PREVIOUS) &&
                 translated_position(c, forwards, x, y, &tx, &ty, lw, lh) &&
@@ -1324,7 +1324,7 @@ int ptref_advect_offsets(ptref_ctx *c, int level, int time_step, int forwards, i
                 d->ts[o] = time_step;
             }
             if (out.hy == 0.f) {
-                nnf rn = {rnd.x, rnd.y, rnd.z, 0.f, 1.f, 1.f};
+                nnf rn = {rnd.x, rnd.y, rnd.z, 0.f, 1.f, 1.f, 0.f};
                 f4 z = {0, 0, 0, 0};
                 out = rn;
                 d->base[o] = z;
@@ -1466,7 +1466,7 @@ int ptref_pde_count(ptref_ctx *c, uint64_t *out)
 
 float ptref_patch_distance(ptref_ctx *c, int x, int y, int level, const pt_patchxf *cand)
 {
-    nnf n = {cand->x, cand->y, cand->theta, cand->hysteresis, cand->scale_u, cand->scale_v};
+    nnf n = {cand->x, cand->y, cand->theta, cand->hysteresis, cand->scale_u, cand->scale_v, 0.f};
     return calc_residual(c, x, y, level, n);
 }
 

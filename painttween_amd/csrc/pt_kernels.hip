@@ -2686,7 +2686,7 @@ __global__ __launch_bounds__(PT_UP_THREADS) void km_scatter_up(PT_MULTI_PARAMS)
 /* ------------------------------------------------------------------ */
 /* launchers                                                           */
 /* ------------------------------------------------------------------ */
-/* the extended patch-distance terms (coherence, LAB) run in the per-pixel
+/* the extended patch-distance terms (coherence, LAB, object IDs) run in the per-pixel
  * kernels' RT = -1 variant */
 static bool ext_terms(const PtArgs &a) { return a.coh_w > 0.f || a.lab || a.obj_w > 0.f; }
 
