@@ -145,6 +145,12 @@ typedef struct pt_params {
                                           runs forwards iff p % 2 equals it
                                           (FMBM, FMBM_RM, FBMO; TCTF and the
                                           per-frame schedule take 0 only)   */
+    int32_t max_level;                 /* maxLevel (SynthesisProcessor::
+                                          _finalLevel): FMBM / FMBM_RM
+                                          synthesize levels down to it, then
+                                          upsample the result to level 0
+                                          without synthesis; 0 = all levels,
+                                          other schedules take 0 only       */
 } pt_params;
 
 /*
@@ -392,7 +398,9 @@ typedef struct pt_style_guides {
  * SynthesisProcessor::renderAnimationFMBM (synthesisProcessor.cpp:846-878)
  * over renderConsecutivePass (:703-753) and SynthesisFrame::synthesizeConsecutive
  * (synthesisFrame.cpp:214-323) with propagateAndScatter (:483-508); IND
- * (:806-839), FBMO (:902-970) or TCTF (:1151-1312); all at maxLevel 0.
+ * (:806-839), FBMO (:902-970) or TCTF (:1151-1312).  params->max_level > 0
+ * is taken by FMBM / FMBM_RM only (their tail, :863-870: every frame's
+ * result upsampled level by level to level 0 without synthesis).
  * Frames are processed in the reference order of the scheme.
  * out_offsets / out_canvas / out_residual (each may be NULL) receive every
  * frame's level-0 result of the final pass: num_frames*w*h elements.

@@ -606,6 +606,8 @@ static int check_supported(ptref_ctx *c, const pt_params *p)
         return fail(c, PT_ERR_INVALID_ARG, "unknown rng_mode");
     if (p->first_pass_direction != 0 && p->first_pass_direction != 1)
         return fail(c, PT_ERR_INVALID_ARG, "first_pass_direction must be 0 or 1");
+    if (p->max_level < 0 || p->max_level >= p->levels)
+        return fail(c, PT_ERR_INVALID_ARG, "max_level must be in [0, levels)");
     return PT_OK;
 }
 
@@ -1531,6 +1533,8 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
     if (scheme < PT_S_FMBM || scheme > PT_S_IND) return fail(c, PT_ERR_UNSUPPORTED, "scheme");
     if (scheme == PT_S_TCTF && params->first_pass_direction != 0)
         return fail(c, PT_ERR_UNSUPPORTED, "TCTF with a backward first pass");
+    if (params->max_level != 0 && scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM)
+        return fail(c, PT_ERR_UNSUPPORTED, "max_level above 0 with FMBM and FMBM_RM only");
     int levels = params->levels;
     int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
     int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
@@ -1857,7 +1861,7 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
 #undef LOAD_IF_SAVED
         free(lastp);
     }
-    for (int level = levels - 1; level >= 0 && !rc && scheme != PT_S_IND && scheme != PT_S_FBMO &&
+    for (int level = levels - 1; level >= params->max_level && !rc && scheme != PT_S_IND && scheme != PT_S_FBMO &&
                                  scheme != PT_S_TCTF; level--) {
         rc = ptref_initialize_exemplars(c, SW >> level, SH >> level, 1);
         if (!rc) rc = ptref_upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sb, 0);
@@ -1939,10 +1943,27 @@ int ptref_synthesize(ptref_ctx *c, const pt_params *params, int scheme, int iter
             }
         }
     }
+    /* maxLevel > 0 (FMBM, FMBM_RM): renderAnimationFMBM's tail
+     * (synthesisProcessor.cpp:863-870), SynthesisFrame::spatialUpsample
+     * (synthesisFrame.cpp:465-481) for every frame, level by level */
+    for (int level = params->max_level; level > 0 && !rc; level--) {
+        rc = ptref_initialize_exemplars(c, SW >> (level - 1), SH >> (level - 1), 1);
+        if (!rc) rc = ptref_upload_image(c, level - 1, PT_LAYER_EXEMPLAR_BASE, sb, 0);
+        if (!rc) rc = ptref_upload_image(c, level - 1, PT_LAYER_EXEMPLAR_OUTPUT, so, 0);
+        if (!rc) rc = ptref_upload_image(c, level - 1, PT_LAYER_EXEMPLAR_ORIENTATION, sor, 0);
+        if (!rc) rc = ptref_upload_image(c, level - 1, PT_LAYER_EXEMPLAR_DIST_TRANS, sdt, 0);
+        if (!rc && sid) rc = ptref_upload_image(c, level - 1, PT_LAYER_EXEMPLAR_OBJECT_IDS, sid, 0);
+        for (int f = 0; f < F && !rc; f++) {
+            rc = ptref_load_offsets(c, level, SNAP(level, f, level == params->max_level ? passes - 1 : 0).off);
+            if (!rc) rc = ptref_upsample(c, level - 1);
+            if (!rc) rc = ptref_sync_buffers(c, level - 1, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL);
+            if (!rc) snap_save(c, level - 1, &SNAP(level - 1, f, 0));
+        }
+    }
     if (!rc) {
         size_t n = (size_t)W * H;
         for (int f = 0; f < F; f++) {
-            snap_t *s = &SNAP(0, f, passes - 1);
+            snap_t *s = &SNAP(0, f, params->max_level > 0 ? 0 : passes - 1); /* copyOutputToFinal (:872) */
             if (out_off) memcpy(out_off + f * n, s->off, sizeof(pt_patchxf) * n);
             if (out_canvas) memcpy(out_canvas + f * n * 4, s->canvas, sizeof(float) * 4 * n);
             if (out_res) memcpy(out_res + f * n, s->res, sizeof(float) * n);
@@ -2002,7 +2023,8 @@ int ptref_synthesize_next_frame(ptref_ctx *c, const pt_params *params, int itera
                                 pt_patchxf *out_off, float *out_canvas, float *out_res)
 {
     if (!c || !params || !frame || !style || iterations < 1) return fail(c, PT_ERR_INVALID_ARG, "bad frame arguments");
-    if (params->first_pass_direction != 0) return fail(c, PT_ERR_UNSUPPORTED, "forwards only");
+    if (params->first_pass_direction != 0 || params->max_level != 0)
+        return fail(c, PT_ERR_UNSUPPORTED, "forwards and down to level 0 only");
     int levels = params->levels;
     int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
     int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));

@@ -80,6 +80,7 @@ class PtParams(C.Structure):
         ("seed", C.c_uint32),
         ("frame_number", C.c_int32),
         ("first_pass_direction", C.c_int32),
+        ("max_level", C.c_int32),
     ]
 
 

@@ -252,6 +252,8 @@ static int check_supported(pt_ctx *c, const pt_params *p)
         return set_err(c, PT_ERR_INVALID_ARG, "unknown rng_mode");
     if (p->first_pass_direction != 0 && p->first_pass_direction != 1)
         return set_err(c, PT_ERR_INVALID_ARG, "first_pass_direction must be 0 or 1");
+    if (p->max_level < 0 || p->max_level >= p->levels)
+        return set_err(c, PT_ERR_INVALID_ARG, "max_level must be in [0, levels)");
     return PT_OK;
 }
 
@@ -1965,6 +1967,18 @@ static int run_schedule_ind(pt_ctx *c, const pt_params *params, int iterations, 
     return PT_OK;
 }
 
+/* The scheme and the schedule settings it implements: a backward first pass
+ * with FMBM, FMBM_RM and FBMO, maxLevel above 0 with FMBM and FMBM_RM. */
+static int check_scheme(pt_ctx *c, const pt_params *params, int scheme)
+{
+    if (scheme < PT_S_FMBM || scheme > PT_S_IND) return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
+    if (scheme == PT_S_TCTF && params->first_pass_direction != 0)
+        return set_err(c, PT_ERR_UNSUPPORTED, "TCTF with a backward first pass is not supported");
+    if (params->max_level != 0 && scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM)
+        return set_err(c, PT_ERR_UNSUPPORTED, "max_level above 0 is implemented for FMBM and FMBM_RM only");
+    return PT_OK;
+}
+
 static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iterations, int fixup_passes, int F,
                         const FramePyr *fp, const StylePyr &sp)
 {
@@ -1973,15 +1987,12 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
     RC_CHECK(pt_setup(c, params));
     RC_CHECK(snap_alloc(c, levels, F, passes));
     for (auto &sn : c->snaps) sn.saved = false;
-    if (scheme == PT_S_TCTF) {
-        if (params->first_pass_direction != 0)
-            return set_err(c, PT_ERR_UNSUPPORTED, "TCTF with a backward first pass is not supported");
-        return run_schedule_tctf(c, params, iterations, fixup_passes, F, fp, sp);
-    }
+    if (scheme == PT_S_TCTF) return run_schedule_tctf(c, params, iterations, fixup_passes, F, fp, sp);
     if (scheme == PT_S_IND) return run_schedule_ind(c, params, iterations, passes, F, fp, sp);
     if (scheme == PT_S_FBMO) return run_schedule_fbmo(c, params, iterations, fixup_passes, F, fp, sp);
     pt_params fparams = *params;
-    for (int level = levels - 1; level >= 0; level--) {
+    const int final_level = params->max_level;
+    for (int level = levels - 1; level >= final_level; level--) {
         /* Style::load (style.cpp:62-140) */
         RC_CHECK(pt_initialize_exemplars(c, c->SW >> level, c->SH >> level, 1));
         RC_CHECK(upload_image(c, level, PT_LAYER_EXEMPLAR_BASE, sp.base, 0));
@@ -2055,6 +2066,27 @@ static int run_schedule(pt_ctx *c, const pt_params *params, int scheme, int iter
                 RC_CHECK(propagate_and_scatter(c, level, iterations));
                 RC_CHECK(snap_save(c, level, snap(c, level, f, pass)));
             }
+        }
+    }
+    /* renderAnimationFMBM's tail (synthesisProcessor.cpp:863-870) with
+     * renderSpatialUpsample / SynthesisFrame::spatialUpsample
+     * (synthesisFrame.cpp:465-481): from maxLevel down, every frame's last
+     * saved offsets are upsampled one level, their canvas synced and the
+     * result saved as pass 0 of the finer level.  No frame images are loaded,
+     * so the upsample reads the image orientation and scale textures as the
+     * last synthesized frame left them. */
+    for (int level = final_level; level > 0; level--) {
+        RC_CHECK(pt_initialize_exemplars(c, c->SW >> (level - 1), c->SH >> (level - 1), 1));
+        RC_CHECK(upload_image(c, level - 1, PT_LAYER_EXEMPLAR_BASE, sp.base, 0));
+        RC_CHECK(upload_image(c, level - 1, PT_LAYER_EXEMPLAR_OUTPUT, sp.out, 0));
+        RC_CHECK(upload_image(c, level - 1, PT_LAYER_EXEMPLAR_ORIENTATION, sp.orient, 0));
+        RC_CHECK(upload_image(c, level - 1, PT_LAYER_EXEMPLAR_DIST_TRANS, sp.dt, 0));
+        if (sp.id) RC_CHECK(upload_image(c, level - 1, PT_LAYER_EXEMPLAR_OBJECT_IDS, sp.id, 0));
+        for (int f = 0; f < F; f++) {
+            RC_CHECK(snap_load_offsets(c, snap(c, level, f, level == final_level ? passes - 1 : 0)));
+            RC_CHECK(upsample(c, level - 1));
+            RC_CHECK(sync_buffers(c, level - 1, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL));
+            RC_CHECK(snap_save(c, level - 1, snap(c, level - 1, f, 0)));
         }
     }
     return PT_OK;
@@ -2291,7 +2323,9 @@ static int synthesize_issue(pt_ctx *c, const pt_params *params, int scheme, int 
     c->frame_pyrs.resize(F);
     for (int f = 0; f < F; f++) RC_CHECK(prepare_frame(c, W, H, frames[f], guides_on_device, c->frame_pyrs[f], wants_ids(params)));
     RC_CHECK(run_schedule(c, params, scheme, iterations, fixup_passes, F, c->frame_pyrs.data(), c->style_pyr));
-    return read_results(c, F, scheme_passes(scheme, fixup_passes, params->levels, F) - 1, out_off, out_canvas, out_res,
+    /* copyOutputToFinal(final_pass): pass 0 when maxLevel > 0 (synthesisProcessor.cpp:872) */
+    int final_pass = params->max_level > 0 ? 0 : scheme_passes(scheme, fixup_passes, params->levels, F) - 1;
+    return read_results(c, F, final_pass, out_off, out_canvas, out_res,
                         outputs_on_device);
 }
 
@@ -2302,7 +2336,7 @@ static int synthesize(pt_ctx *c, const pt_params *params, int scheme, int iterat
 {
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme < PT_S_FMBM || scheme > PT_S_IND) return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
+    RC_CHECK(check_scheme(c, params, scheme));
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
     RC_CHECK(synthesis_alloc(c, params, W, H, F, scheme_passes(scheme, fixup_passes, params->levels, F), SW, SH));
@@ -2333,7 +2367,7 @@ int pt_record_synthesis(pt_ctx *c, std::vector<RecOp> *ops, const pt_params *par
     if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
     if (!params || !frames || !style || F < 1 || iterations < 1 || fixup_passes < 0)
         return set_err(c, PT_ERR_INVALID_ARG, "bad synthesize arguments");
-    if (scheme < PT_S_FMBM || scheme > PT_S_IND) return set_err(c, PT_ERR_UNSUPPORTED, "scheme %d is not supported", scheme);
+    RC_CHECK(check_scheme(c, params, scheme));
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     HIP_CHECK(c, hipSetDevice(c->device));
     RC_CHECK(synthesis_alloc(c, params, W, H, F, scheme_passes(scheme, fixup_passes, params->levels, F), SW, SH));
@@ -2523,8 +2557,8 @@ static int next_frame_check(pt_ctx *c, const pt_params *params, int iterations, 
                             const pt_frame_guides *frame, int SW, int SH, const pt_style_guides *style)
 {
     if (!params || !frame || !style || iterations < 1) return set_err(c, PT_ERR_INVALID_ARG, "bad frame arguments");
-    if (params->first_pass_direction != 0)
-        return set_err(c, PT_ERR_UNSUPPORTED, "the per-frame schedule runs forwards only");
+    if (params->first_pass_direction != 0 || params->max_level != 0)
+        return set_err(c, PT_ERR_UNSUPPORTED, "the per-frame schedule runs forwards and down to level 0 only");
     RC_CHECK(check_synthesis(c, params, W, H, SW, SH));
     if (c->seq_frame > 0 && (c->W != W || c->H != H || c->SW != SW || c->SH != SH || c->seq_levels != params->levels))
         return set_err(c, PT_ERR_INVALID_ARG, "frame shape differs from the sequence; call pt_sequence_reset");

@@ -199,17 +199,16 @@ class WorkingSet:
         p.use_guide = int(bool(g("guidedInterpolation")))
         p.interpolate_key_frame = int(bool(g("keyFrameInterpolation")))
         p.first_pass_direction = int(g("firstPassDirection"))  # synthesisProcessor.cpp:262
+        p.max_level = int(g("maxLevel"))  # synthesisProcessor.cpp:227 (_finalLevel)
         return p
 
     def schedule(self):
         """The processor settings of an offline render (synthesisProcessor.cpp:225-262):
         scheme, iterations per level, fixup passes and the frame range (the
-        first pass direction travels in pt_params).  Settings this path does
-        not implement are rejected here; parameters it does not implement are
-        rejected by pt_setup or pt_synthesize."""
+        first pass direction and maxLevel travel in pt_params).  Settings this
+        path does not implement are rejected here; parameters it does not
+        implement are rejected by pt_setup or pt_synthesize."""
         g = self.get
-        if int(g("maxLevel")) != 0:
-            raise WorkingSetError("maxLevel %d: only maxLevel 0 is implemented" % g("maxLevel"))
         if g("keyFrameInterpolation"):
             raise WorkingSetError("keyFrameInterpolation is not implemented")
         return dict(scheme=int(g("synthesisScheme")), iterations=int(g("iterations")),
@@ -444,8 +443,8 @@ def load(path, check_files=True):
 
 def synthesize(ctx, ws, want=("offsets", "canvas", "residual")):
     """Render the working set's frame range with its first style on `ctx`
-    (a painttween_amd.Context): the offline render of SynthesisProcessor at
-    maxLevel 0.  Returns what Context.synthesize returns."""
+    (a painttween_amd.Context): the offline render of SynthesisProcessor.
+    Returns what Context.synthesize returns."""
     s = ws.schedule()
     frames = [ws.read_frame(f) for f in range(s["first_frame"], s["last_frame"] + 1)]
     return ctx.synthesize(ws.pt_params(ctx.default_params()), frames, ws.read_style(0), scheme=s["scheme"],

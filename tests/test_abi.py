@@ -39,7 +39,7 @@ def test_struct_layouts():
     lib = _ffi.load_library()
     lib.pt_struct_size.restype = C.c_size_t
     lib.pt_struct_size.argtypes = [C.c_int]
-    assert C.sizeof(_ffi.PtParams) == 48 * 4 == lib.pt_struct_size(0)  # 38 scalars + two float[5]
+    assert C.sizeof(_ffi.PtParams) == 49 * 4 == lib.pt_struct_size(0)  # 39 scalars + two float[5]
     assert C.sizeof(_ffi.PatchXF) == 36 == lib.pt_struct_size(1)      # PatchXF, types.h:98-114
     assert C.sizeof(_ffi.FrameGuides) == lib.pt_struct_size(2)
     assert C.sizeof(_ffi.StyleGuides) == lib.pt_struct_size(3)

@@ -431,6 +431,22 @@ def test_synthesize_backward_first_pass_bit_exact(scheme):
                    o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
 
 
+@pytest.mark.parametrize("scheme,max_level", [(_ffi.PT_S_FMBM_RM, 1), (_ffi.PT_S_FMBM, 2), (_ffi.PT_S_FMBM, 1)])
+def test_synthesize_max_level_bit_exact(scheme, max_level):
+    """maxLevel > 0: synthesis stops at that level, then every frame's result
+    is upsampled level by level to the full size (renderAnimationFMBM's
+    tail, synthesisProcessor.cpp:863-870); other schemes reject it."""
+    st = W.style_pair(64, 64, seed=48, guides=True)
+    frames = W.animation(48, 40, 3, seed=49)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, max_level=max_level)
+        _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1),
+                   o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
+        for other in (_ffi.PT_S_IND, _ffi.PT_S_FBMO, _ffi.PT_S_TCTF):
+            with pytest.raises(Exception):
+                g.synthesize(g.default_params(**kw), frames, st, scheme=other, iterations=2, fixup_passes=1)
+
+
 @pytest.mark.parametrize("fixup", [0, 1])
 def test_synthesize_ind_bit_exact(fixup):
     """renderAnimationIND: every frame on its own, levels inside frames."""

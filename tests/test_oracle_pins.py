@@ -776,3 +776,36 @@ def test_backward_first_pass_starts_from_the_last_frame():
             o.synthesize(o.default_params(first_pass_direction=1, **kw), frames, st, scheme=_ffi.PT_S_TCTF)
         with pytest.raises(Exception):
             o.setup(o.default_params(first_pass_direction=2))
+
+
+# ---------------------------------------------------------------- maxLevel
+def test_max_level_upsamples_the_coarse_result_without_synthesis():
+    """maxLevel 1 (renderAnimationFMBM, synthesisProcessor.cpp:849-872): level
+    1 is synthesized, then every frame's result is upsampled to level 0 and
+    its canvas synced, with no image reload and no search, and the output is
+    that pass 0.  Level 1 runs the same code as a full schedule; the tail is
+    pinned through what the upsample alone leaves: hysteresis 0 on every
+    pixel it writes (the search at level 0 would set it on the pixels it
+    moves), offsets within the upsampled exemplar, an output unlike the full
+    schedule's, and the same output for the backward first pass on one frame."""
+    st = W.style_pair(40, 36, seed=65, guides=True)
+    fr = W.still_frame(36, 32, seed=66, guides=True)
+    kw = dict(residual_window_size=2, levels=2, orientation_enabled=1, dist_trans_weight=0.5)
+    with OracleContext() as o:
+        got = o.synthesize(o.default_params(max_level=1, **kw), [fr], st, scheme=_ffi.PT_S_FMBM_RM, iterations=2,
+                           fixup_passes=0)
+        full = o.synthesize(o.default_params(**kw), [fr], st, scheme=_ffi.PT_S_FMBM_RM, iterations=2,
+                            fixup_passes=0)
+        off = got[0][0]
+        assert (off["hysteresis"][1:-1, 1:-1] == 0).all()
+        assert (full[0][0]["hysteresis"][1:-1, 1:-1] != 0).any()
+        assert off.tobytes() != full[0][0].tobytes()
+        inner = off[1:-1, 1:-1]
+        # 2c + 0.5 of coarse offsets that lie up to a texel beyond the 20x18 level-1 exemplar
+        assert (inner["x"] >= 0.5).all() and (inner["x"] <= 41.5).all()
+        assert (inner["y"] >= 0.5).all() and (inner["y"] <= 37.5).all()
+        assert np.isfinite(got[1]).all() and np.isfinite(got[2]).all()
+        with pytest.raises(Exception):
+            o.synthesize(o.default_params(max_level=1, **kw), [fr], st, scheme=_ffi.PT_S_IND)
+        with pytest.raises(Exception):
+            o.setup(o.default_params(max_level=2, **kw))

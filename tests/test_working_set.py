@@ -162,7 +162,9 @@ def test_schedule_rejects_unimplemented_settings(tmp_path):
     ws1 = wsm.load(path)
     ws1.params["firstPassDirection"] = 1
     assert ws1.pt_params().first_pass_direction == 1
-    for k, v in (("maxLevel", 1), ("keyFrameInterpolation", True)):
+    ws1.params["maxLevel"] = 2
+    assert ws1.pt_params().max_level == 2 and ws1.schedule()["scheme"] == _ffi.PT_S_FMBM
+    for k, v in (("keyFrameInterpolation", True),):
         ws2 = wsm.load(path)
         ws2.params[k] = v
         with pytest.raises(wsm.WorkingSetError, match=k):
