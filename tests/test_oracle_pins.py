@@ -783,11 +783,11 @@ def test_max_level_upsamples_the_coarse_result_without_synthesis():
     """maxLevel 1 (renderAnimationFMBM, synthesisProcessor.cpp:849-872): level
     1 is synthesized, then every frame's result is upsampled to level 0 and
     its canvas synced, with no image reload and no search, and the output is
-    that pass 0.  Level 1 runs the same code as a full schedule; the tail is
-    pinned through what the upsample alone leaves: hysteresis 0 on every
-    pixel it writes (the search at level 0 would set it on the pixels it
-    moves), offsets within the upsampled exemplar, an output unlike the full
-    schedule's, and the same output for the backward first pass on one frame."""
+    that pass 0.  One frame: the same as the operator sequence replayed on a
+    fresh context (a used one holds stale level-0-sized buffers that the
+    coarse level reads).  Also what the upsample alone leaves: hysteresis 0
+    on every pixel it writes (the full schedule's search does not), offsets
+    within the upsampled exemplar and an output unlike the full schedule's."""
     st = W.style_pair(40, 36, seed=65, guides=True)
     fr = W.still_frame(36, 32, seed=66, guides=True)
     kw = dict(residual_window_size=2, levels=2, orientation_enabled=1, dist_trans_weight=0.5)
@@ -805,6 +805,19 @@ def test_max_level_upsamples_the_coarse_result_without_synthesis():
         assert (inner["x"] >= 0.5).all() and (inner["x"] <= 41.5).all()
         assert (inner["y"] >= 0.5).all() and (inner["y"] <= 37.5).all()
         assert np.isfinite(got[1]).all() and np.isfinite(got[2]).all()
+    with OracleContext() as o:
+        s = Session(o, st, [fr], o.default_params(**kw))
+        sweep = [("propagate", j) for j in (128, 64, 32, 16, 8, 4, 2, 1, 1)]
+        s.run([("style", 1), ("reset_adv",), ("frame", 1, 0), ("randomize",), ("sync", 0)] + sweep +
+              [("scatter",)] + sweep)
+        off1, _, _ = s.state(1)
+        s.load_style(0)
+        o.load_offsets(1, off1)
+        o.upsample(0)
+        o.sync_buffers(0, _ffi.PT_DONT_COPY_OFFSETS | _ffi.PT_DONT_UPDATE_RESIDUAL)
+        for a, b in zip(got, s.state(0)):
+            assert a[0].tobytes() == b.tobytes()
+    with OracleContext() as o:
         with pytest.raises(Exception):
             o.synthesize(o.default_params(max_level=1, **kw), [fr], st, scheme=_ffi.PT_S_IND)
         with pytest.raises(Exception):
