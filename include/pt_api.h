@@ -107,7 +107,11 @@ typedef struct pt_params {
     float input_analogy_weight;
     float canvas_output_weight;
     int32_t num_styles;
-    float offsets_histogram_slope[5];
+    float offsets_histogram_slope[5];     /* offsetsHistogramSlope: any value;
+                                             the reference's histogram stays
+                                             zero, so the term is 0 unless the
+                                             layer-0 threshold is negative
+                                             (PT_ERR_UNSUPPORTED then)      */
     float offsets_histogram_threshold[5];
     float max_distortion;
     float alpha_weight;

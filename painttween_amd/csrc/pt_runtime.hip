@@ -245,9 +245,22 @@ static int check_supported(pt_ctx *c, const pt_params *p)
         return set_err(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
     if (p->interpolate_key_frame)
         return set_err(c, PT_ERR_UNSUPPORTED, "keyframe interpolation is not supported");
-    for (int i = 0; i < 5; i++)
-        if (p->offsets_histogram_slope[i] != 0.f)
-            return set_err(c, PT_ERR_UNSUPPORTED, "offset histogram term is not supported");
+    /* The offsets-histogram term (histogramTerm, texSynth_kernel.cu:486-494)
+     * reads a histogram that stays all zero: its accumulation is commented out
+     * (:258), the buffer is reset before it and the two blur passes of zeros
+     * give zeros (:1250-1276).  The term is therefore the constant
+     * (0 > threshold) ? (0 - threshold) * slope : 0 of the candidate's style
+     * layer, which is layer 0 without keyframes; it is exactly 0 for every
+     * slope when that threshold is not negative (the GUI's range is
+     * [0, 1000], nlDialsAndKnobs.cpp:279), and adding 0 to a non-negative
+     * float sum changes no bit.  A negative layer-0 threshold with a nonzero
+     * slope is not implemented. */
+    {
+        float thr = p->offsets_histogram_threshold[0], slope = p->offsets_histogram_slope[0];
+        float term = (0.f > thr) ? (0.f - thr) * slope : 0.f;
+        if (term != 0.f)
+            return set_err(c, PT_ERR_UNSUPPORTED, "offsets histogram with a negative threshold is not supported");
+    }
     if (p->rng_mode != PT_RNG_REFERENCE && p->rng_mode != PT_RNG_COUNTER)
         return set_err(c, PT_ERR_INVALID_ARG, "unknown rng_mode");
     if (p->first_pass_direction != 0 && p->first_pass_direction != 1)

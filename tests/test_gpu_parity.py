@@ -447,6 +447,26 @@ def test_synthesize_max_level_bit_exact(scheme, max_level):
                 g.synthesize(g.default_params(**kw), frames, st, scheme=other, iterations=2, fixup_passes=1)
 
 
+def test_synthesize_histogram_slope_bit_exact():
+    """Offsets-histogram slopes are accepted where the reference's term is 0
+    (an all-zero histogram, thresholds not negative) and change nothing; a
+    negative layer-0 threshold with a slope is refused."""
+    st = W.style_pair(64, 64, seed=50, guides=True)
+    frames = W.animation(48, 40, 2, seed=51)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5)
+        pg, po, p0 = g.default_params(**kw), o.default_params(**kw), g.default_params(**kw)
+        for p in (pg, po):
+            p.offsets_histogram_slope[0] = 2.0
+            p.offsets_histogram_threshold[0] = 0.3
+        got = g.synthesize(pg, frames, st, scheme=_ffi.PT_S_FMBM, iterations=2, fixup_passes=1)
+        _cmp_synth(got, o.synthesize(po, frames, st, scheme=_ffi.PT_S_FMBM, iterations=2, fixup_passes=1))
+        _cmp_synth(got, g.synthesize(p0, frames, st, scheme=_ffi.PT_S_FMBM, iterations=2, fixup_passes=1))
+        pg.offsets_histogram_threshold[0] = -0.5
+        with pytest.raises(Exception):
+            g.synthesize(pg, frames, st, scheme=_ffi.PT_S_FMBM, iterations=2, fixup_passes=1)
+
+
 @pytest.mark.parametrize("fixup", [0, 1])
 def test_synthesize_ind_bit_exact(fixup):
     """renderAnimationIND: every frame on its own, levels inside frames."""

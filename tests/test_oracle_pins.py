@@ -822,3 +822,33 @@ def test_max_level_upsamples_the_coarse_result_without_synthesis():
             o.synthesize(o.default_params(max_level=1, **kw), [fr], st, scheme=_ffi.PT_S_IND)
         with pytest.raises(Exception):
             o.setup(o.default_params(max_level=2, **kw))
+
+
+def test_offsets_histogram_term_is_zero_as_in_the_reference():
+    """histogramTerm (texSynth_kernel.cu:486-494) reads a histogram that is
+    never accumulated (:258) and is reset and blurred from zeros
+    (:1250-1276): any slope with a non-negative threshold changes no bit; a
+    negative layer-0 threshold with a nonzero slope, which would add a
+    constant, is refused; thresholds of layers the candidates never use
+    (styles above 0 without keyframes) do not matter."""
+    st = W.style_pair(40, 36, seed=91, guides=True)
+    fr = W.still_frame(36, 32, seed=92, guides=True)
+    kw = dict(residual_window_size=2, levels=2, dist_trans_weight=0.5)
+    runs = []
+    for slope, thr in ((0.0, 0.0), (2.0, 0.3), (7.5, 0.0), (0.0, -0.5)):
+        with OracleContext() as o:
+            p = o.default_params(**kw)
+            p.offsets_histogram_slope[0] = slope
+            p.offsets_histogram_threshold[0] = thr
+            p.offsets_histogram_slope[3] = 1.0
+            p.offsets_histogram_threshold[3] = -4.0
+            runs.append(o.synthesize(p, [fr], st, scheme=_ffi.PT_S_FMBM_RM, iterations=2, fixup_passes=0))
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert a.tobytes() == b.tobytes()
+    with OracleContext() as o:
+        p = o.default_params(**kw)
+        p.offsets_histogram_slope[0] = 2.0
+        p.offsets_histogram_threshold[0] = -0.5
+        with pytest.raises(Exception):
+            o.setup(p)

@@ -209,8 +209,9 @@ def test_synthesize_on_the_oracle(tmp_path):
     """The glue end to end on the CPU restatement: the loaded working set
     gives what Context.synthesize gives with the same arrays by hand."""
     from tests.oracle_ctx import OracleContext
-    # the offsets-histogram term is rejected by pt_setup: leave it out
-    path, st, fr = _make(tmp_path, xml=XML.replace('<offsetsHistogramSlope value0="0.25" value1="9"/>', ""))
+    # the histogram slope 0.25 stays in: with the reference's all-zero histogram
+    # and threshold 0 its term is 0, so the by-hand parameters leave it out
+    path, st, fr = _make(tmp_path)
     ws = wsm.load(path)
     with OracleContext() as o:
         off, canvas, res = wsm.synthesize(o, ws)
@@ -229,7 +230,7 @@ def test_synthesize_gpu_matches_oracle(tmp_path):
     """A working set rendered through the HIP library equals the oracle's render."""
     from painttween_amd import Context
     from tests.oracle_ctx import OracleContext
-    path, _, _ = _make(tmp_path, xml=XML.replace('<offsetsHistogramSlope value0="0.25" value1="9"/>', ""))
+    path, _, _ = _make(tmp_path)
     ws = wsm.load(path)
     print("working set", path, "frames 1-2, 28x24, FMBM", flush=True)
     with Context.gpu(0) as g, OracleContext() as o:
