@@ -221,6 +221,33 @@ def test_batch_mixing_object_id_jobs_equals_each_alone():
     assert alone[0][0].tobytes() != alone[1][0].tobytes()
 
 
+
+def test_batch_with_max_level_and_backward_first_pass_equals_each_alone():
+    """maxLevel and the first pass direction travel in each job's parameters:
+    a maxLevel-1 FMBM job, a backward-first FMBM_RM job and a plain job
+    planned together each equal the same job synthesized alone."""
+    st = W.style_pair(48, 40, seed=94, guides=True)
+    frames = W.animation(40, 32, 3, seed=95)
+    base = dict(residual_window_size=2, levels=3, temporal_coherence_weight=0.5)
+    jobs = [dict(kw=dict(base, max_level=1), scheme=_ffi.PT_S_FMBM),
+            dict(kw=dict(base, first_pass_direction=1), scheme=_ffi.PT_S_FMBM_RM),
+            dict(kw=dict(base), scheme=_ffi.PT_S_FMBM_RM)]
+    for j in jobs:
+        j.update(iterations=2, fixup_passes=1, frames=frames, style=st)
+    with Context.gpu(0) as g:
+        alone = [g.synthesize(g.default_params(**j["kw"]), frames, st, scheme=j["scheme"], iterations=2,
+                              fixup_passes=1) for j in jobs]
+        for j in jobs:
+            j["params"] = g.default_params(**j["kw"])
+    b = Batch(jobs)
+    try:
+        b.run()
+        for i in range(len(jobs)):
+            _same(b.result(i), alone[i], "job %d" % i)
+    finally:
+        b.close()
+    assert alone[0][0].tobytes() != alone[2][0].tobytes() and alone[1][0].tobytes() != alone[2][0].tobytes()
+
 def _sequences():
     seqs = []
     for i, (w, h, sw, sh) in enumerate([(48, 40, 64, 56), (64, 48, 48, 48), (40, 40, 56, 64)]):
