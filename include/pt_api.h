@@ -36,6 +36,14 @@ extern "C" {
 #define PT_ERR_STATE (-5)
 #define PT_ERR_BUDGET (-6)
 
+/* ---- size limits (no reference counterpart) ---------------------------
+ * The exemplar of one level is gathered through a buffer resource with
+ * 32-bit byte offsets over its interleaved 32-byte texels, and image buffers
+ * are indexed with 32-bit pixel offsets: pt_initialize_exemplars and the
+ * one-shot schedules refuse larger sizes with PT_ERR_UNSUPPORTED. */
+#define PT_MAX_EXEMPLAR_TEXELS 67108863 /* (2^31 - 1) / 32: below 8192 x 8192 */
+#define PT_MAX_IMAGE_PIXELS 2147483647
+
 /* ---- PassDirection (types.h:128-132) ---------------------------------- */
 #define PT_FORWARD 0
 #define PT_BACKWARD 1

@@ -327,10 +327,12 @@ static inline int image_culled(const ptref_ctx *c, int x, int y)
     return d > ((float)c->p.dist_trans_cull - 0.0f);
 }
 
-/* isExemplarPixelCulled (texSynth_kernel.cu:200-205) */
+/* isExemplarPixelCulled (texSynth_kernel.cu:200-205): int parameters, so
+ * the position is truncated toward zero and the texel there read exactly
+ * (clamp addressing). */
 static inline int exemplar_culled(const ptref_ctx *c, float x, float y)
 {
-    float d = tex1_lin(c->ex_dt, c->ew, c->ew, c->eh, x, y);
+    float d = c->ex_dt[clampi((int)y, 0, c->eh - 1) * c->ew + clampi((int)x, 0, c->ew - 1)];
     return (double)d > (double)c->p.dist_trans_cull * 1.25;
 }
 
@@ -1314,9 +1316,12 @@ int ptref_advect_offsets(ptref_ctx *c, int level, int time_step, int forwards, i
             f4 frac = {rnd.x - floorf(rnd.x), rnd.y - floorf(rnd.y), rnd.z - floorf(rnd.z), rnd.w - floorf(rnd.w)};
             nnf out = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0.f};
             float tx, ty;
+            /* isImagePixelCulledPreviousFrame (texSynth_kernel.cu:188-198)
+             * has int parameters: (tx, ty) is truncated (it is in bounds,
+             * so non-negative) and the texel read exactly. */
             if (!(flags & PT_ADVECT_DO_NOT_CHECK_PREVIOUS) &&
                 translated_position(c, forwards, x, y, &tx, &ty, lw, lh) &&
-                !(tex1_lin(c->img_dt[prev], c->W, c->W, c->H, tx, ty) > ((float)c->p.dist_trans_cull - 8.0f))) {
+                !(c->img_dt[prev][(int)ty * c->W + (int)tx] > ((float)c->p.dist_trans_cull - 8.0f))) {
                 int rtx = (int)roundf(tx), rty = (int)roundf(ty);
                 nnf nr = c->off_base[rty * c->W + rtx];
                 float b0x, b0y, b1x, b1y;
@@ -1483,6 +1488,11 @@ float ptref_patch_distance(ptref_ctx *c, int x, int y, int level, const pt_patch
 {
     nnf n = {cand->x, cand->y, cand->theta, cand->hysteresis, cand->scale_u, cand->scale_v, 0.f};
     return calc_residual(c, x, y, level, n);
+}
+
+float ptref_screen_to_exemplar_angle(ptref_ctx *c, float x, float y, float ex, float ey)
+{
+    return screen_to_ex_angle(c, x, y, ex, ey);
 }
 
 /* ------------------------------------------------------------------ */

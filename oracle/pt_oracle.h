@@ -78,6 +78,9 @@ int ptref_sequence_reset(ptref_ctx *ctx);
 
 /* Single patch distance at (x, y) for a candidate, for unit tests. */
 float ptref_patch_distance(ptref_ctx *ctx, int x, int y, int level, const pt_patchxf *cand);
+/* screenToExemplarAngle (texSynth_kernel.cu:425-436) at image (x, y) and
+ * exemplar (ex, ey) with the textures bound now, for unit tests. */
+float ptref_screen_to_exemplar_angle(ptref_ctx *ctx, float x, float y, float ex, float ey);
 /* Gaussian tap weights of the current setup (2r+1 floats). */
 int ptref_gaussian_weights(ptref_ctx *ctx, float *out, int n);
 /* Deterministic math entry points for unit tests. */

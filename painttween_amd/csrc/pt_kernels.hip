@@ -220,10 +220,12 @@ __device__ __forceinline__ bool image_culled(const PtArgs &A, int x, int y)
     return A.dt[A.cur][y * A.W + x] > ((float)A.cull - 0.0f);
 }
 
-/* isExemplarPixelCulled (texSynth_kernel.cu:200-205) */
+/* isExemplarPixelCulled (texSynth_kernel.cu:200-205): its parameters are
+ * int, so the candidate position is truncated toward zero and the texel at
+ * that integer position read exactly (clamped). */
 __device__ __forceinline__ bool exemplar_culled(const PtArgs &A, float x, float y)
 {
-    float d = tex1_lin(A.ex_dt, A.ew, A.ew, A.eh, x, y);
+    float d = A.ex_dt[clampi((int)y, 0, A.eh - 1) * A.ew + clampi((int)x, 0, A.ew - 1)];
     return (double)d > (double)A.cull * 1.25;
 }
 
@@ -2034,8 +2036,11 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
     float lum = 0.f;
     float tx, ty;
+    /* isImagePixelCulledPreviousFrame(tx, ty, level, 8.0) takes int
+     * coordinates (texSynth_kernel.cu:188-198): the position is truncated and
+     * the texel read exactly; translated_position has put it in bounds. */
     if (!(flags & PT_ADVECT_DO_NOT_CHECK_PREVIOUS) && translated_position(A, fwd, x, y, tx, ty) &&
-        !(tex1_lin(A.dt[prev], A.W, A.W, A.H, tx, ty) > ((float)A.cull - 8.0f))) {
+        !(A.dt[prev][(int)ty * A.W + (int)tx] > ((float)A.cull - 8.0f))) {
         int rtx = (int)roundf(tx), rty = (int)roundf(ty);
         float4 nr = A.base.xyth[rty * A.W + rtx];
         float b0x, b0y, b1x, b1y;

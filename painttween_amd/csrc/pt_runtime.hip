@@ -643,6 +643,10 @@ extern "C" int pt_setup(pt_ctx *c, const pt_params *p)
 
 extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
 {
+    if ((int64_t)iw * ih > PT_MAX_IMAGE_PIXELS)
+        return set_err(c, PT_ERR_UNSUPPORTED, "image larger than PT_MAX_IMAGE_PIXELS");
+    if ((int64_t)sw * sh > PT_MAX_EXEMPLAR_TEXELS)
+        return set_err(c, PT_ERR_UNSUPPORTED, "exemplar larger than PT_MAX_EXEMPLAR_TEXELS");
     if (!c || iw < 2 || ih < 2 || sw < 2 || sh < 2) return set_err(c, PT_ERR_INVALID_ARG, "bad dimensions");
     HIP_CHECK(c, hipSetDevice(c->device));
     if (c->base.xyth && c->W == iw && c->H == ih && c->SW == sw && c->SH == sh) return PT_OK;
@@ -696,6 +700,10 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
 
 extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
 {
+    /* the gathers address the interleaved 32-byte texels with 32-bit byte
+     * offsets (ex_rsrc, ex_pair); checked before anything else */
+    if ((int64_t)w * h > PT_MAX_EXEMPLAR_TEXELS)
+        return set_err(c, PT_ERR_UNSUPPORTED, "exemplar larger than PT_MAX_EXEMPLAR_TEXELS");
     if (!c || w < 1 || h < 1) return set_err(c, PT_ERR_INVALID_ARG, "bad exemplar dimensions");
     if (num_layers < 1 || num_layers > 4) return set_err(c, PT_ERR_INVALID_ARG, "num_layers must be in [1, 4]");
     HIP_CHECK(c, hipSetDevice(c->device));
@@ -2306,6 +2314,10 @@ static int check_synthesis(pt_ctx *c, const pt_params *params, int W, int H, int
 {
     if (!params) return set_err(c, PT_ERR_INVALID_ARG, "NULL parameters");
     if (W < 2 || H < 2 || SW < 2 || SH < 2) return set_err(c, PT_ERR_INVALID_ARG, "bad dimensions");
+    if ((int64_t)W * H > PT_MAX_IMAGE_PIXELS)
+        return set_err(c, PT_ERR_UNSUPPORTED, "image larger than PT_MAX_IMAGE_PIXELS");
+    if ((int64_t)SW * SH > PT_MAX_EXEMPLAR_TEXELS)
+        return set_err(c, PT_ERR_UNSUPPORTED, "exemplar larger than PT_MAX_EXEMPLAR_TEXELS");
     int img_lv = (int)fmin(floor(log2f((float)W)), floor(log2f((float)H)));
     int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
     if (params->levels < 1 || params->levels > img_lv || params->levels > sty_lv)

@@ -89,7 +89,10 @@ def _decompress(data, size, compression):
     if len(data) == size:  # stored uncompressed (any codec may do so)
         return data
     if compression in (ZIP_COMPRESSION, ZIPS_COMPRESSION):
-        return _unpredict_deinterleave(zlib.decompress(data))
+        try:
+            return _unpredict_deinterleave(zlib.decompress(data))
+        except zlib.error as e:
+            raise ExrError("corrupt ZIP chunk: %s" % e)
     if compression == RLE_COMPRESSION:
         return _unpredict_deinterleave(_rle_decode(data, size))
     raise ExrError("compression %d is not supported for reading" % compression)
@@ -192,24 +195,42 @@ def write_exr(path, channels, compression=ZIP_COMPRESSION):
 
 
 def read_exr(path):
-    """Returns ({name: array (H, W)}, header dict)."""
+    """Returns ({name: array (H, W)}, header dict).  The arrays hold the data
+    window, row 0 / column 0 at its minimum corner, as ImageIO::readImage
+    does with its frame buffer at pixels - dx - dy * dw (imageIO.cpp:106-117),
+    so a data window away from the origin is read without a shift.  Malformed
+    chunk tables or chunks raise ExrError."""
     with open(path, "rb") as f:
         buf = f.read()
     hdr, pos = _parse_header(buf)
     w, h, comp = hdr["width"], hdr["height"], hdr["compression"]
+    if w < 1 or h < 1:
+        raise ExrError("empty data window")
     if comp not in _LINES:
         raise ExrError("compression %d is not supported for reading" % comp)
     lines = _LINES[comp]
     nblocks = (h + lines - 1) // lines
+    if pos + 8 * nblocks > len(buf):
+        raise ExrError("truncated offset table")
     offsets = struct.unpack_from("<%dQ" % nblocks, buf, pos)
     chans = hdr["channels"]
     planes = {n: np.empty((h, w), _NP[t]) for n, t in chans}
+    seen = set()
     for off in offsets:
+        if off + 8 > len(buf):
+            raise ExrError("chunk offset %d beyond the file" % off)
         y, size = struct.unpack_from("<ii", buf, off)
         y -= hdr["ymin"]
+        if y < 0 or y >= h or y % lines or y in seen:
+            raise ExrError("chunk at line %d outside the data window or repeated" % (y + hdr["ymin"]))
+        if size < 0 or off + 8 + size > len(buf):
+            raise ExrError("chunk of %d bytes at %d runs past the file" % (size, off))
+        seen.add(y)
         n = min(lines, h - y)
         expect = sum(_NP[t].itemsize for _, t in chans) * w * n
         raw = _decompress(buf[off + 8:off + 8 + size], expect, comp)
+        if len(raw) != expect:
+            raise ExrError("chunk at line %d holds %d bytes, expected %d" % (y + hdr["ymin"], len(raw), expect))
         p = 0
         for yy in range(y, y + n):
             for name, t in chans:

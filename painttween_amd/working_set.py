@@ -298,8 +298,14 @@ def ids_from_rgb(rgb, style):
     scaled to 0..255, rounded half up for shot inputs (floorf(v*255 + 0.5),
     imageIO.cpp:298-317) and half away from zero for styles (round(v*255),
     G/nlDataAccess.cpp:378-395)."""
-    v = np.asarray(rgb, np.float32)[..., :3] * np.float32(255)
-    q = np.where(v < 0, -np.floor(-v + 0.5), np.floor(v + 0.5)) if style else np.floor(v + np.float32(0.5))
+    v = np.asarray(rgb, np.float32)[..., :3] * np.float32(255)  # the float product
+    if style:
+        # round() of the float: half away from zero, exact in float64 (a
+        # float32 v + 0.5 would round 0.49999997 up to 1)
+        d = v.astype(np.float64)
+        q = np.sign(d) * np.floor(np.abs(d) + 0.5)
+    else:
+        q = np.floor(v + np.float32(0.5))  # floorf(v * 255 + 0.5f) in float
     q = q.astype(np.int64)
     return (q[..., 0] + q[..., 1] * 255 + q[..., 2] * 255 * 255).astype(np.int32)
 

@@ -77,6 +77,16 @@ def distance_transform(seed, stream, w, h, points=64):
     return (d / d.max()).astype(np.float32)
 
 
+def culling_dist_trans(seed, stream, w, h, lo=30.0, hi=70.0):
+    """A distance transform in pixel units, lo + (hi - lo) * SmoothNoise, for
+    inputs that reach the culling rules: with the cull distance 50 an image
+    pixel is culled above 50 (isImagePixelCulled), a previous-frame pixel
+    invalidates advection above 42 (cull - 8) and an exemplar texel rejects
+    candidates above 62.5 (1.25 * cull); the default range straddles all
+    three."""
+    return (lo + (hi - lo) * smooth_noise(seed, stream, w, h)).astype(np.float32)
+
+
 def style_pair(w, h, seed, guides=False):
     """Exemplar input/output pair (streams 0, 1) and optional guides."""
     st = {"base": sin_noise(seed, 0, w, h), "output": sin_noise(seed, 1, w, h)}

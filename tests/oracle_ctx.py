@@ -28,6 +28,8 @@ def load_oracle():
         lib.ptref_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
         lib.ptref_patch_distance.restype = C.c_float
         lib.ptref_patch_distance.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(_ffi.PatchXF)]
+        lib.ptref_screen_to_exemplar_angle.restype = C.c_float
+        lib.ptref_screen_to_exemplar_angle.argtypes = [C.c_void_p] + [C.c_float] * 4
         lib.ptref_gaussian_weights.restype = C.c_int
         lib.ptref_gaussian_weights.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]
         lib.ptref_sincosf.restype = None
@@ -102,6 +104,9 @@ class OracleContext(Context):
                                                          sw, sh, C.byref(sg), _ffi.vptr(off), _ffi.vptr(canvas),
                                                          _ffi.vptr(res)))
         return off, canvas, res
+
+    def screen_to_exemplar_angle(self, x, y, ex, ey):
+        return self.lib.ptref_screen_to_exemplar_angle(self.handle, x, y, ex, ey)
 
     def gaussian_weights(self, r):
         out = (C.c_float * (2 * r + 1))()

@@ -84,3 +84,23 @@ def test_batch_and_server_surface_declared():
               "pt_server_open", "pt_server_submit", "pt_server_poll", "pt_server_drain", "pt_server_close",
               "pt_synthesize_next_frame", "pt_sequence_reset"):
         assert n in names, n
+
+
+def test_exemplar_and_image_size_limits_are_refused_before_any_device_work():
+    """The exemplar gathers use 32-bit byte offsets over 32-byte texels, so an
+    exemplar of PT_MAX_EXEMPLAR_TEXELS + 1 texels (8192 x 8192 is above it) is
+    refused with PT_ERR_UNSUPPORTED; the check runs before the context is
+    touched, so it needs no GPU."""
+    lib = _ffi.load_library()
+    lib.pt_global_last_error.restype = C.c_char_p
+    assert _ffi.PT_MAX_EXEMPLAR_TEXELS == (2 ** 31 - 1) // 32
+    assert lib.pt_initialize_exemplars(None, 8192, 8192, 1) == _ffi.PT_ERR_UNSUPPORTED
+    assert b"PT_MAX_EXEMPLAR_TEXELS" in lib.pt_global_last_error()
+    assert lib.pt_initialize_exemplars(None, 8192, 8193, 1) == _ffi.PT_ERR_UNSUPPORTED
+    assert lib.pt_initialize(None, 64, 64, 8192, 8192) == _ffi.PT_ERR_UNSUPPORTED
+    assert lib.pt_initialize(None, 65536, 32768, 64, 64) == _ffi.PT_ERR_UNSUPPORTED
+    assert b"PT_MAX_IMAGE_PIXELS" in lib.pt_global_last_error()
+    # sizes inside the limits reach the context check; 8191 x 8193 is the limit exactly
+    assert 8191 * 8193 == _ffi.PT_MAX_EXEMPLAR_TEXELS
+    assert lib.pt_initialize_exemplars(None, 8191, 8193, 1) == _ffi.PT_ERR_INVALID_ARG
+    assert lib.pt_initialize(None, 64, 64, 64, 64) == _ffi.PT_ERR_INVALID_ARG

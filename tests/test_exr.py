@@ -135,3 +135,46 @@ def test_rejects_other_files(tmp_path):
         exr.read_exr(p)
     with pytest.raises(exr.ExrError):
         exr.write_exr(p, {"R": (np.zeros((2, 2), np.float32), exr.FLOAT)}, exr.PIZ_COMPRESSION)
+
+
+def _patch_data_window(buf, box):
+    key = b"dataWindow\0box2i\0" + struct.pack("<i", 16)
+    at = buf.index(key) + len(key)
+    return buf[:at] + struct.pack("<4i", *box) + buf[at + 16:]
+
+
+def test_data_window_away_from_the_origin_reads_unshifted(tmp_path):
+    """ImageIO::readImage points its frame buffer at pixels - dx - dy * dw
+    (imageIO.cpp:106-117): the array is the data window itself.  A file whose
+    data window starts at x = 3 (chunk lines are data-window lines, so only
+    the box changes) reads back the same array."""
+    p = tmp_path / "w.exr"
+    a = np.arange(12, dtype=np.float32).reshape(3, 4)
+    exr.write_exr(p, {"R": (a, exr.FLOAT)}, exr.NO_COMPRESSION)
+    p.write_bytes(_patch_data_window(p.read_bytes(), (3, 0, 6, 2)))
+    planes, hdr = exr.read_exr(p)
+    assert (hdr["width"], hdr["height"]) == (4, 3)
+    assert planes["R"].tobytes() == a.tobytes()
+
+
+def test_malformed_chunks_raise_exr_error(tmp_path):
+    p = tmp_path / "m.exr"
+    a = np.arange(12, dtype=np.float32).reshape(3, 4)
+    exr.write_exr(p, {"R": (a, exr.FLOAT)}, exr.NO_COMPRESSION)
+    good = p.read_bytes()
+    # the first chunk (line 0) sits right after the three u64 offsets
+    first = struct.unpack_from("<Q", good, len(good) - 3 * (8 + 16) - 3 * 8)[0]
+    for patch in (struct.pack("<ii", 7, 16),      # line outside the data window
+                  struct.pack("<ii", 1, 16),      # line 1 twice
+                  struct.pack("<ii", 0, 10 ** 6),  # size past the end of the file
+                  struct.pack("<ii", 0, 12)):     # fewer bytes than a line holds
+        b = bytearray(good)
+        b[first:first + 8] = patch
+        p.write_bytes(bytes(b))
+        with pytest.raises(exr.ExrError):
+            exr.read_exr(p)
+    b = bytearray(good)
+    b[len(good) - 3 * (8 + 16) - 24:len(good) - 3 * (8 + 16) - 16] = struct.pack("<Q", 10 ** 9)
+    p.write_bytes(bytes(b))
+    with pytest.raises(exr.ExrError):
+        exr.read_exr(p)

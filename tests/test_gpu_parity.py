@@ -551,3 +551,56 @@ def test_incremental_residual_matches_full_refresh(anim):
             outs.append(g.synthesize(g.default_params(**kw), frames, st, iterations=3, fixup_passes=1))
     for a, b in zip(*outs):
         assert a.tobytes() == b.tobytes()
+
+
+# ---------------------------------------------------------------- culling
+def _culling_inputs(w, h, sw, sh, frames=1, seed=60):
+    """Distance transforms in pixel units straddling the culling thresholds:
+    image 30..70 around the cull 50, exemplar 40..80 around 1.25 * 50, and
+    (animations) previous frames around cull - 8 = 42."""
+    st = W.style_pair(sw, sh, seed=seed, guides=True)
+    st["dist_trans"] = W.culling_dist_trans(seed, 1, sw, sh, lo=40.0, hi=80.0)
+    if frames == 1:
+        fr = [W.still_frame(w, h, seed=seed + 1, guides=True)]
+    else:
+        fr = W.animation(w, h, frames, seed=seed + 1)
+    for k, f in enumerate(fr):
+        f["dist_trans"] = W.culling_dist_trans(seed, 10 + k, w, h)
+    return st, fr
+
+
+@pytest.mark.parametrize("early_exit,unit_parallel,tap", [(1, 1, 0), (0, 1, 0), (1, 0, 0), (0, 0, 0), (1, 1, 1024),
+                                                         (1, 1, 300)])
+@pytest.mark.parametrize("r", [2, 3])
+def test_culling_still_operator_sequence(r, early_exit, unit_parallel, tap):
+    """Culled pixels (zero offset, residual 0, no scatter write) and culled
+    exemplar texels (candidates rejected) in every kernel shape."""
+    st, fr = _culling_inputs(44, 34, 40, 36)
+    ov = dict(residual_window_size=r, levels=3, dist_trans_weight=0.5)
+    ga, oa = _both(st, fr, ov, _still_ops(3), early_exit, unit_parallel, tap)
+    assert_same_states(ga, oa)
+    off, _, res = ga[-1][1]
+    culled = W.culling_dist_trans(60, 10, 44, 34) > 50
+    assert culled.any() and (~culled).any()
+    assert np.all(off["x"][culled] == 0) and np.all(res[culled] == 0)
+
+
+@pytest.mark.parametrize("merge,tap", [("merge_random", 0), ("merge_residual", 0), ("merge_random", 1024)])
+def test_culling_animation_operator_sequence(merge, tap):
+    """Advection with previous-frame culling (cull - 8) and the culled
+    current pixels, temporal terms on."""
+    st, frames = _culling_inputs(40, 32, 48, 48, frames=3)
+    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=0.3,
+              dist_trans_weight=0.5)
+    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, merge), tap=tap)
+    assert_same_states(ga, oa)
+
+
+@pytest.mark.parametrize("scheme", [_ffi.PT_S_FMBM_RM, _ffi.PT_S_FMBM])
+def test_culling_synthesize_animation_bit_exact(scheme):
+    st, frames = _culling_inputs(48, 40, 64, 64, frames=3, seed=70)
+    with Context.gpu(0) as g, OracleContext() as o:
+        kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5, dist_trans_weight=0.5)
+        a = g.synthesize(g.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1)
+        _cmp_synth(a, o.synthesize(o.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
+    assert (a[2] == 0).any()

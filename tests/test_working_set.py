@@ -261,3 +261,21 @@ def test_object_id_elements(tmp_path):
     v = float(half[0, 0, 0] * np.float32(255))
     if v == -0.5:
         assert wsm.ids_from_rgb(half, style=False)[0, 0] == 0 and wsm.ids_from_rgb(half, style=True)[0, 0] == -1
+
+
+def test_style_id_rounding_is_exact_just_below_a_half():
+    """round() of the float product v*255 rounds half away from zero; a float
+    just below .5 (0.49999997) must give 0, which a float32 v*255 + 0.5 would
+    round up to 1 (G/nlDataAccess.cpp:390).  Shot inputs keep floorf(x + 0.5f)
+    in float, where the same value does round to 1 (imageIO.cpp:298-317)."""
+    below = np.nextafter(np.float32(0.5), np.float32(0))
+    for p, want_style in ((below, 0), (np.float32(0.5), 1), (-below, 0), (np.float32(-0.5), -1),
+                          (np.float32(2 ** 23 - 0.5), 2 ** 23)):
+        v = np.float32(p) / np.float32(255)
+        if v * np.float32(255) != np.float32(p):
+            continue  # p is not a product the division reaches exactly
+        rgb = np.array([[[v, 0, 0]]], np.float32)
+        assert wsm.ids_from_rgb(rgb, style=True)[0, 0] == want_style, p
+    rgb = np.array([[[below / np.float32(255), 0, 0]]], np.float32)
+    if rgb[0, 0, 0] * np.float32(255) == below:
+        assert wsm.ids_from_rgb(rgb, style=False)[0, 0] == 1  # floorf(0.49999997f + 0.5f) = 1 in float
