@@ -299,6 +299,20 @@ int pt_merge_using_residual(pt_ctx *ctx, int level, int forwards);
  * of the base offsets, the canvas (RGBA) and the residual; any pointer may be
  * NULL. */
 int pt_get_output(pt_ctx *ctx, int level, pt_patchxf *offsets, float *canvas_rgba, float *residual);
+/* pt_get_output without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  pt_get_output stages the packed offsets in a buffer it
+ * allocates and frees per call; this variant stages them in a file-scope
+ * device array of PT_NOALLOC_SCRATCH_BYTES per GPU (allocated when the
+ * library's code loads on that GPU, held until the process ends), so a
+ * level is limited to PT_NOALLOC_SCRATCH_BYTES / 36 offsets (3 728 270:
+ * 2048 x 1820, larger levels return PT_ERR_UNSUPPORTED).  All calls on one
+ * GPU share that array; the library serializes them with a process-wide
+ * lock, and callers in other processes have arrays of their own.  Results
+ * equal pt_get_output's. */
+#define PT_NOALLOC_SCRATCH_BYTES (128u << 20)
+int pt_get_output_noalloc(pt_ctx *ctx, int level, pt_patchxf *offsets, float *canvas_rgba, float *residual);
 /* WorkingBuffers::loadOffsets / loadResidual (workingBuffers.cpp:166-207):
  * writes the level rectangle of the base offsets / base residual.  Offsets
  * must have layer 0 and lum_a 0 (PT_ERR_UNSUPPORTED otherwise); a nonzero

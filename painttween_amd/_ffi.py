@@ -16,6 +16,7 @@ PT_ERR_INVALID_ARG, PT_ERR_UNSUPPORTED, PT_ERR_HIP, PT_ERR_OUT_OF_MEMORY, PT_ERR
     -1, -2, -3, -4, -5, -6
 PT_MAX_EXEMPLAR_TEXELS = 67108863
 PT_MAX_IMAGE_PIXELS = 2147483647
+PT_NOALLOC_SCRATCH_BYTES = 128 << 20
 PT_FORWARD, PT_BACKWARD, PT_BIDIRECTIONAL = 0, 1, 2
 PT_S_FMBM, PT_S_FMBM_RM, PT_S_FBMO, PT_S_TCTF, PT_S_IND = 0, 1, 2, 3, 4
 PT_RNG_REFERENCE, PT_RNG_COUNTER = 0, 1

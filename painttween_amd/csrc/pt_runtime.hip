@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -326,6 +327,34 @@ extern "C" size_t pt_struct_size(int which)
 
 extern "C" const char *pt_last_error(pt_ctx *c) { return c ? c->err.c_str() : g_last_error; }
 
+/* ------------------------------------------------------------------ */
+/* allocation-free readback (pt_get_output_noalloc)                    */
+/* ------------------------------------------------------------------ */
+/* Staging for the packed offsets of pt_get_output_noalloc: one array per
+ * GPU, held until the process ends.  Host code never passes the array's
+ * name to a kernel or a copy (on the host it is only a stand-in); it uses
+ * the device address hipGetSymbolAddress returns for each device, resolved
+ * once per device by pt_create with that device current. */
+static __device__ __align__(256) unsigned char scratch[PT_NOALLOC_SCRATCH_BYTES];
+
+#define PT_SCRATCH_MAX_DEVICES 64
+static std::mutex g_scratch_mu;                        /* guards the table and the arrays' users */
+static void *g_scratch_ptr[PT_SCRATCH_MAX_DEVICES];    /* device address of `scratch` per device */
+
+/* With `device` current: records that device's address of `scratch`. */
+static hipError_t scratch_resolve(int device)
+{
+    if (device < 0 || device >= PT_SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
 extern "C" int pt_create(pt_ctx **out, int device)
 {
     if (!out) return set_err(nullptr, PT_ERR_INVALID_ARG, "out is NULL");
@@ -345,6 +374,12 @@ extern "C" int pt_create(pt_ctx **out, int device)
         return set_err(nullptr, PT_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
     pt_params_default(&c->p);
+    e = scratch_resolve(device);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return set_err(nullptr, PT_ERR_HIP, "resolving the readback staging array failed: %s", hipGetErrorString(e));
+    }
     *out = c;
     return PT_OK;
 }
@@ -1458,6 +1493,34 @@ extern "C" int pt_get_output(pt_ctx *c, int level, pt_patchxf *offsets, float *c
     PtNnf b = c->base;
     if (!c->lum_live) b.lum = nullptr;
     if (offsets) RC_CHECK(get_nnf(c, b, lw, lh, offsets));
+    if (canvas) RC_CHECK(get_plane(c, c->canvas, lw, lh, (float4 *)canvas));
+    if (residual) RC_CHECK(get_plane(c, c->res_base, lw, lh, residual));
+    return sync_stream(c);
+}
+
+extern "C" int pt_get_output_noalloc(pt_ctx *c, int level, pt_patchxf *offsets, float *canvas, float *residual)
+{
+    if (!c || !c->base.xyth || level < 0 || (c->W >> level) < 1 || (c->H >> level) < 1)
+        return set_err(c, PT_ERR_INVALID_ARG, "bad get_output_noalloc");
+    int lw = c->W >> level, lh = c->H >> level;
+    size_t bytes = (size_t)lw * lh * sizeof(pt_patchxf);
+    if (offsets && bytes > PT_NOALLOC_SCRATCH_BYTES)
+        return set_err(c, PT_ERR_UNSUPPORTED, "%d x %d offsets need %zu staging bytes, above "
+                       "PT_NOALLOC_SCRATCH_BYTES (%u)", lw, lh, bytes, (unsigned)PT_NOALLOC_SCRATCH_BYTES);
+    HIP_CHECK(c, hipSetDevice(c->device));
+    PtNnf b = c->base;
+    if (!c->lum_live) b.lum = nullptr;
+    if (offsets) {
+        /* every call on this GPU shares its array: one user at a time */
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        pt_patchxf *stage = (pt_patchxf *)g_scratch_ptr[c->device];
+        if (!stage) return set_err(c, PT_ERR_STATE, "no staging array for device %d", c->device);
+        /* the pack writes every staged element: nothing needs clearing */
+        hipError_t e = pt_launch_pack_nnf(b, c->W, lw, lh, stage, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(offsets, stage, bytes, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return set_err(c, PT_ERR_HIP, "offset readback failed: %s", hipGetErrorString(e));
+    }
     if (canvas) RC_CHECK(get_plane(c, c->canvas, lw, lh, (float4 *)canvas));
     if (residual) RC_CHECK(get_plane(c, c->res_base, lw, lh, residual));
     return sync_stream(c);
