@@ -213,7 +213,7 @@ def run_still256(ctx, inputs, cpu_runs, cpu_threads):
     return res
 
 
-def run_batched(ctx, jobs_in, steps):
+def run_batched(ctx, jobs_in, steps, cpu_threads=0):
     """Plain path (one context per job, jobs one after another) against the
     batched form on the same device inputs."""
     import torch
@@ -242,11 +242,17 @@ def run_batched(ctx, jobs_in, steps):
 
     plain_all()                       # warm: allocations
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain_all()
-    torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t0
-    print("batched: plain path %.1f ms for %d jobs" % (plain_s * 1e3, len(jobs)), file=sys.stderr, flush=True)
+    pde0 = [c.pde_count() for c in ctxs]
+    plain_times = []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        plain_all()
+        torch.cuda.synchronize()
+        plain_times.append(time.perf_counter() - t0)
+    job_pdes = [(c.pde_count() - p0) // steps for c, p0 in zip(ctxs, pde0)]
+    plain_s = float(np.median(plain_times))
+    print("batched: plain path %.1f ms (median of %d) for %d jobs" % (plain_s * 1e3, steps, len(jobs)),
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b.tune(1)
     tune_s = time.perf_counter() - t0
@@ -264,14 +270,49 @@ def run_batched(ctx, jobs_in, steps):
     px = sum(o.w * o.h for o in b.outputs)
     batch_s = float(np.median(times))
     out = {"jobs": len(jobs), "pixels": px, "jobs_per_s": len(jobs) / batch_s, "ms_per_batch": batch_s * 1e3,
-           "plain_jobs_per_s": len(jobs) / plain_s, "plain_ms": plain_s * 1e3, "speedup_vs_plain": plain_s / batch_s,
+           "runs": steps, "plain_jobs_per_s": len(jobs) / plain_s, "plain_ms": plain_s * 1e3,
+           "plain_runs": steps, "speedup_vs_plain": plain_s / batch_s, "pde_per_batch": int(sum(job_pdes)),
            "bit_exact_vs_plain": bool(exact), "launches": st["launches"], "multi_launches": st["multi_launches"],
            "ticks": st["ticks"], "recorded_ops": st["ops"], "tuned_shapes": st["shapes"], "tune_s": tune_s,
            "hip_graph": True}
     b.close()
     for c in ctxs:
         c.close()
+    if cpu_threads:
+        out["cpu_baseline"] = batch_cpu_baseline(jobs_in, job_pdes, cpu_threads)
     return out
+
+
+def batch_cpu_baseline(jobs_in, job_pdes, threads, budget_s=10.0):
+    """The oracle on the host cores over whole jobs of the batch, smallest
+    first, until about `budget_s` of CPU work; its rate in patch taps per
+    second (patch distances times (2r+1)^2) extrapolates to the whole batch
+    with the GPU's per-job patch-distance counts."""
+    from tests.oracle_ctx import OracleContext
+    order = sorted(range(len(jobs_in)), key=lambda i: job_pdes[i])
+    taps = lambda i: (2 * jobs_in[i][0]["r"] + 1) ** 2
+    done, spent, tap_sum = [], 0.0, 0
+    with OracleContext(threads) as o:
+        for i in order:
+            spec, (frame, style) = jobs_in[i]
+            kw = dict(residual_window_size=spec["r"], levels=spec["levels"], dist_trans_weight=0.5,
+                      input_analogy_weight=1.0, orientation_enabled=1, canvas_output_weight=1.0)
+            p0 = o.pde_count()
+            t0 = time.perf_counter()
+            o.synthesize(o.default_params(**kw), [frame], style, iterations=spec["iterations"], fixup_passes=0)
+            spent += time.perf_counter() - t0
+            tap_sum += (o.pde_count() - p0) * taps(i)
+            done.append(i)
+            if spent >= budget_s:
+                break
+    rate = tap_sum / spent
+    total_taps = sum(job_pdes[i] * taps(i) for i in range(len(jobs_in)))
+    batch_s = total_taps / rate
+    return {"value": len(jobs_in) / batch_s, "unit": "jobs/s", "cores": threads, "kind": "port",
+            "host": host_cpu(), "taps_per_s": rate,
+            "sample": "oracle on the %d smallest jobs of the batch in full (%.1f s), jobs/s extrapolated to the "
+                      "%d jobs by patch taps (the GPU's per-job patch-distance counts times (2r+1)^2)"
+                      % (len(done), spent, len(jobs_in))}
 
 
 def _gen_anim(arg):
@@ -280,7 +321,7 @@ def _gen_anim(arg):
     return W.animation(w, h, frames, seed=3), W.style_pair(sw, sw, seed=3, guides=True)
 
 
-def run_animation(ctx, anim_in, levels, iterations):
+def run_animation(ctx, anim_in, levels, iterations, cpu_threads=0):
     """The animation workload through pt_synthesize_device: FMBM_RM, two
     passes (forward then backward advection), temporal coherence 0.5."""
     import torch
@@ -320,13 +361,40 @@ def run_animation(ctx, anim_in, levels, iterations):
     pdes = ctx.pde_count() - pde0
     res = out.res.cpu().numpy()
     assert np.isfinite(res).all() and (res >= 0).all(), "invalid residual"
-    return {"bandwidth_kernels": kinds, "frames": F, "image": [w, h], "exemplar": [sw, sh], "levels": levels, "iterations": iterations,
+    cpu = None
+    if cpu_threads:
+        cpu = anim_cpu_baseline(frames, style, kw, levels, iterations, cpu_threads, pdes / F)
+    return {"bandwidth_kernels": kinds, "frames": F, "cpu_baseline": cpu, "image": [w, h], "exemplar": [sw, sh], "levels": levels, "iterations": iterations,
             "passes": 2, "seconds": dt, "frames_per_s": F / dt, "frame_passes_per_s": 2 * F / dt,
             "pde_per_s": pdes / dt, "schedule": "FMBM_RM, fixup passes 1 (forward then backward), 7x7 patches, "
                                                 "temporal coherence 0.5, time derivatives 1"}
 
 
-def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0):
+def anim_cpu_baseline(frames, style, kw, levels, iterations, threads, gpu_pde_per_frame):
+    """The oracle on the host cores over a bounded sample of the animation:
+    its first three frames cropped to their top-left 240x136 (velocities,
+    guides and the 512^2 exemplar unchanged), the same schedule and
+    parameters; its patch-distance rate (temporal terms included)
+    extrapolates to frames/s of the full sequence with the GPU's count of
+    patch distances per frame."""
+    from tests.oracle_ctx import OracleContext
+    cw, ch = 240, 136
+    crop = [{k: (None if v is None else np.ascontiguousarray(v[:ch, :cw])) for k, v in f.items()} for f in frames[:3]]
+    with OracleContext(threads) as o:
+        p0 = o.pde_count()
+        t0 = time.perf_counter()
+        o.synthesize(o.default_params(**kw), crop, style, iterations=iterations, fixup_passes=1)
+        dt = time.perf_counter() - t0
+        pdes = o.pde_count() - p0
+    rate = pdes / dt
+    return {"value": rate / gpu_pde_per_frame, "unit": "frames/s", "cores": threads, "kind": "port",
+            "host": host_cpu(), "pde_per_s": rate,
+            "sample": "oracle on the first 3 frames cropped to 240x136 (%d patch distances with temporal terms in "
+                      "%.1f s), frames/s extrapolated by the GPU's %.0f patch distances per 960x540 frame"
+                      % (pdes, dt, gpu_pde_per_frame)}
+
+
+def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0, cpu_pde_rate=None):
     """The serving loop under Poisson arrivals against the plain per-frame
     path (one context per sequence, frames one at a time in arrival order)."""
     import torch
@@ -348,7 +416,10 @@ def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0):
     pctx = [Context.gpu(0) for _ in range(N)]
     first_round = {}
     service = []
+    pde0 = None
     for rnd in range(2):          # round 0 warms allocations, round 1 is timed
+        if rnd == 1:
+            pde0 = sum(c.pde_count() for c in pctx)
         for (k, f) in order:
             d = dev[k]
             out = DeviceOutputs(d["w"], d["h"], 1)
@@ -360,6 +431,7 @@ def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0):
                 first_round[(k, f)] = out
             else:
                 service.append(dt)
+    served_pdes = sum(c.pde_count() for c in pctx) - pde0
     print("served: plain per-frame path %.1f ms for %d frames" % (sum(service) * 1e3, len(service)),
           file=sys.stderr, flush=True)
     for c in pctx:
@@ -429,7 +501,14 @@ def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0):
             "plain_fps": len(service) / sum(service), "bit_exact_vs_plain": bool(exact), "rates": per_rate,
             "budget_gb": budget_gb, "resident_bytes": st["resident_bytes"], "launches": st["launches"],
             "multi_launches": st["multi_launches"], "ticks": st["ticks"],
-            "schedule": "per-frame FMBM_RM first pass, 7x7 patches, 4 levels, 3 iterations, temporal coherence 0.5"}
+            "schedule": "per-frame FMBM_RM first pass, 7x7 patches, 4 levels, 3 iterations, temporal coherence 0.5",
+            "pde_per_frame": served_pdes / len(service),
+            "cpu_baseline": None if not cpu_pde_rate else {
+                "value": cpu_pde_rate["pde_per_s"] / (served_pdes / len(service)), "unit": "frames/s",
+                "cores": cpu_pde_rate["cores"], "kind": "port",
+                "sample": "the animation leg's oracle sample (7x7 patches with temporal terms, %.3g patch "
+                          "distances/s), extrapolated by the served frames' mean patch-distance count"
+                          % cpu_pde_rate["pde_per_s"]}}
 
 
 def main():
@@ -606,7 +685,8 @@ def main():
         line["still256"] = run_still256(ctx, still256_in, 0 if args.no_cpu else 5, min(os.cpu_count() or 1, 16))
         print("still256: %.2f ms on the GPU" % line["still256"]["gpu_ms"], file=sys.stderr, flush=True)
     if anim_in is not None:
-        line["animation"] = run_animation(ctx, anim_in, args.anim_levels, args.anim_iterations)
+        line["animation"] = run_animation(ctx, anim_in, args.anim_levels, args.anim_iterations,
+                                          0 if args.no_cpu else min(os.cpu_count() or 1, 16))
         # advection and merges run only in the animation: their HBM rooflines
         # come from its (untimed, event-recorded) warm run
         for k in ("advect", "merge"):
@@ -614,10 +694,12 @@ def main():
                 line["bandwidth_kernels"][k] = line["animation"]["bandwidth_kernels"][k]
         print("animation: %.2f frames/s" % line["animation"]["frames_per_s"], file=sys.stderr, flush=True)
     if batch_in is not None:
-        line["batched"] = run_batched(ctx, batch_in, args.batch_steps)
+        line["batched"] = run_batched(ctx, batch_in, args.batch_steps,
+                                      0 if args.no_cpu else min(os.cpu_count() or 1, 16))
     if serve_in is not None:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        line["served"] = run_served(ctx, serve_in, args.serve_frames, rates)
+        anim_cpu = (line.get("animation") or {}).get("cpu_baseline")
+        line["served"] = run_served(ctx, serve_in, args.serve_frames, rates, cpu_pde_rate=anim_cpu)
     ctx.close()
     print(json.dumps(line), flush=True)
 
