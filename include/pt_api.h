@@ -365,6 +365,13 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * there (smaller levels measured slower with it); 0 reads every tap from
  * global memory.  Same results. */
 #define PT_TUNE_EXEMPLAR_WINDOW 7
+/* PT_TUNE_TIME_TABLE_MAX_TAPS (default 81): with temporal terms active and
+ * patches of at most this many taps, the exemplar samples at every tap of
+ * every pixel's advected offset (the candidate-independent half of
+ * timeDerivativeTerms, texSynth_kernel.cu:448-452) are gathered once per
+ * level into a table of 32 B per tap and pixel, which every candidate then
+ * reads instead of gathering them again; 0 disables it.  Same results. */
+#define PT_TUNE_TIME_TABLE_MAX_TAPS 8
 int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --

@@ -659,7 +659,8 @@ extern "C" int pt_server_open(pt_server *s, const pt_sequence_desc *d, int32_t *
     q.style = *d->style;
     q.desc.style = nullptr;
     q.open = true;
-    q.bytes = pt_sequence_bytes(d->image_w, d->image_h, d->style_w, d->style_h, d->params.levels);
+    q.bytes = pt_sequence_bytes(d->image_w, d->image_h, d->style_w, d->style_h, d->params.levels,
+                                d->params.residual_window_size);
     if (q.bytes > s->budget) return sfail(s, PT_ERR_BUDGET, "sequence needs %zu bytes, budget is %llu", q.bytes,
                                           (unsigned long long)s->budget);
     s->seqs.push_back(q);

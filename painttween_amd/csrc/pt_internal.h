@@ -74,6 +74,10 @@ struct PtArgs {
     int ex_window;     /* XL kernel variants: exemplar windows in LDS */
 
     uint8_t *changed;  /* per pixel: base offset changed by the last sync */
+    /* time tables of the advected sets (k_time_table) or null: entry
+     * time_table_index(x, y, i, j) holds the exemplar base and output at tap
+     * (i, j) of pixel (x, y)'s advected offset */
+    const float4 *tt[2];
 
     PtNnf base, work;
     float *res_base, *res_work;
@@ -151,6 +155,8 @@ hipError_t pt_launch_propagate(const PtArgs &a, int jump, hipStream_t s);
 hipError_t pt_launch_scatter(const PtArgs &a, hipStream_t s);
 hipError_t pt_launch_upsample(const PtArgs &a, hipStream_t s);
 hipError_t pt_launch_advect(const PtArgs &a, int forwards, int time_step, int flags, hipStream_t s);
+/* Time table of advected set k at a's level: (2r+1)^2 * lw * lh * 2 float4. */
+hipError_t pt_launch_time_table(const PtArgs &a, int k, float4 *tt, hipStream_t s);
 hipError_t pt_launch_merge_randomly(const PtArgs &a, int forwards, float weight, int force, hipStream_t s);
 hipError_t pt_launch_merge_residual(const PtArgs &a, int forwards, hipStream_t s);
 hipError_t pt_launch_downsample(const float *src, int pw, float *dst, int lw, int lh, int ch,

@@ -247,6 +247,14 @@ __device__ float hysteresis_term(const PtArgs &A, int x, int y)
     return 0.f;
 }
 
+/* Index of tap (i, j) of pixel (x, y) in a time table: tap-major planes of
+ * the level's pixels, two float4 (exemplar base, output) per entry. */
+__device__ __forceinline__ int time_table_index(const PtArgs &A, int x, int y, int i, int j)
+{
+    const int r = A.r, t = (i + r) * (2 * r + 1) + (j + r);
+    return (t * (A.lw * A.lh) + y * A.lw + x) * 2;
+}
+
 /* ------------------------------------------------------------------ */
 /* patch distance                                                      */
 /* ------------------------------------------------------------------ */
@@ -360,16 +368,25 @@ __device__ __forceinline__ void stage_window(const PtArgs &A, float ax, float ay
  * variant with the extended terms, where the LAB colour space converts the
  * previous frame's colours and exemplar samples (:454-459) */
 template <bool EXT = false>
-__device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, const PtAdv &d, float4 prev, float pb0x,
-                              float pb0y, float pb1x, float pb1y, float4 base, float4 tb, float4 canvas, float4 to,
-                              int oo, int i, int j)
+__device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k, float4 prev, float pb0x, float pb0y,
+                              float pb1x, float pb1y, float4 base, float4 tb, float4 canvas, float4 to, int oo, int x,
+                              int y, int i, int j)
 {
+    const PtAdv &d = A.adv[k];
     float4 prev_base = d.base[oo];
     float4 prev_out = d.out[oo];
-    float ptx, pty;
-    new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, ptx, pty);
     float4 peb, peo;
-    ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo);
+    if (A.tt[k]) {
+        /* the exemplar samples at the advected offset's tap, gathered once
+         * per level by k_time_table with this same arithmetic */
+        const float4 *e = A.tt[k] + time_table_index(A, x, y, i, j);
+        peb = e[0];
+        peo = e[1];
+    } else {
+        float ptx, pty;
+        new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, ptx, pty);
+        ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo);
+    }
     if (EXT && A.lab) {
         prev_base = lab4(prev_base);
         prev_out = lab4(prev_out);
@@ -471,9 +488,9 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
             if (TIME && any_time_w) {
                 int oo = (y + j) * A.W + (x + i);
                 if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
-                    res = res + time_terms_b<(RT < 0)>(A, er, A.adv[0], pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo, i, j);
+                    res = res + time_terms_b<(RT < 0)>(A, er, 0, pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo, x, y, i, j);
                 if (bwd_dir && cnv && A.adv[1].xyth[oo].w > 0)
-                    res = res + time_terms_b<(RT < 0)>(A, er, A.adv[1], pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo, i, j);
+                    res = res + time_terms_b<(RT < 0)>(A, er, 1, pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo, x, y, i, j);
             }
             if (i == 0 && j == 0) {
                 if (A.dt_w > 0.f) {
@@ -1022,11 +1039,11 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
             if (any_time_w) {
                 int oo = (y + j) * A.W + (x + i);
                 if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
-                    res = res + time_terms_b(A, er, A.adv[0], pf, tb4[0], tb4[1], tb4[2], tb4[3], base, tb, canvas,
-                                             to, oo, i, j);
+                    res = res + time_terms_b(A, er, 0, pf, tb4[0], tb4[1], tb4[2], tb4[3], base, tb, canvas,
+                                             to, oo, x, y, i, j);
                 if (bwd_dir && cnv && A.adv[1].xyth[oo].w > 0)
-                    res = res + time_terms_b(A, er, A.adv[1], pb, tb4[4], tb4[5], tb4[6], tb4[7], base, tb, canvas,
-                                             to, oo, i, j);
+                    res = res + time_terms_b(A, er, 1, pb, tb4[4], tb4[5], tb4[6], tb4[7], base, tb, canvas,
+                                             to, oo, x, y, i, j);
             }
         }
         if (i == 0 && j == 0) {
@@ -1595,6 +1612,21 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
     }
 }
 
+/* Whether the temporal terms of this launch still gather the advected
+ * offsets' exemplar samples per candidate (a direction in use has no time
+ * table).  Those gathers are shared by the candidates of one pixel, so the
+ * unit kernels then order units pixel-major; otherwise candidate-major, where
+ * neighbouring lanes take the same candidate of neighbouring pixels (coherent
+ * exemplar lines, and neighbouring time-table entries).  The order changes
+ * no result. */
+template <bool TM> __device__ __forceinline__ bool up_pixel_major(const PtArgs &A)
+{
+    if (!TM) return false;
+    const bool fwd = A.direction == PT_FORWARD || A.direction == PT_BIDIRECTIONAL;
+    const bool bwd = A.direction == PT_BACKWARD || A.direction == PT_BIDIRECTIONAL;
+    return (fwd && !A.tt[0]) || (bwd && !A.tt[1]);
+}
+
 /* propagate_kernel (texSynth_kernel.cu:1112-1219), unit-parallel. */
 /* Whether a unit kernel must evaluate candidates of a pixel: it is inside
  * the level and not culled (its tap weight sum is positive), and a
@@ -1682,13 +1714,11 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         s_next = PT_UP_THREADS;
     }
     __syncthreads();
-    /* phase 1: candidate set-up in lockstep.  Without temporal terms,
-     * neighbouring lanes take the same candidate direction of neighbouring
-     * pixels, so the list (and the first units each lane evaluates) holds
-     * coherent exemplar positions and a wave's gathers share cache lines;
-     * with them, the pixel-major order measured faster. */
+    /* phase 1: candidate set-up in lockstep, in the order of up_pixel_major
+     * (the list, and the first units each lane evaluates, follow it) */
+    const bool pmaj = up_pixel_major<TM>(A);
     for (int v = threadIdx.x; v < U; v += PT_UP_THREADS) {
-        int p = TM ? v >> 2 : v % P, k = TM ? v & 3 : v / P, u = 4 * p + k;
+        int p = pmaj ? v >> 2 : v % P, k = pmaj ? v & 3 : v / P, u = 4 * p + k;
         int x = x0 + p % TX, y = y0 + p / TX;
         s_r[u] = PT_INF;
         if (!up_open(A, s_sw[p], s_best[p])) continue;
@@ -1815,6 +1845,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     const int SU = n + 1; /* units per pixel */
+    const bool pmaj = up_pixel_major<TM>(A);
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
     if (threadIdx.x == 0) {
         float d = (float)(A.ew > A.eh ? A.ew : A.eh);
@@ -1849,12 +1880,11 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     while (true) {
         if (threadIdx.x == 0) s_next = PT_UP_THREADS;
         __syncthreads();
-        /* unit u: pixel p, sample k.  Without temporal terms units run
-         * sample-major (neighbouring lanes: the same sample of neighbouring
-         * pixels, coherent exemplar positions), with them pixel-major. */
+        /* unit u: pixel p, sample k, sample-major or pixel-major as
+         * up_pixel_major says */
         auto setup = [&](int u, Cand &c, TimeCand &tc, int &x, int &y, int &p) -> bool {
-            p = TM ? u / SU : u % P;
-            int k = TM ? u % SU : u / P;
+            p = pmaj ? u / SU : u % P;
+            int k = pmaj ? u % SU : u / P;
             int st = s_start[p];
             bool todo = (k < n) ? (k >= st && st < n) : (first && st <= n);
             if (!todo) return false;
@@ -1875,7 +1905,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
             if constexpr (TM) up_time_load<P>(s_tm, p, tc);
             return true;
         };
-        auto done = [&](int u, float v) { s_r[TM ? (u / SU) * S + (u % SU) : (u % P) * S + (u / P)] = v; };
+        auto done = [&](int u, float v) { s_r[pmaj ? (u / SU) * S + (u % SU) : (u % P) * S + (u / P)] = v; };
         if (EE) {
             up_queue_ee<RT, TX, false, TM>(A, T, er, threadIdx.x, P * SU, &s_next, s_best, s_sw, setup, done);
         } else {
@@ -2076,6 +2106,38 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
     d.xyth[o] = out;
     d.sc[o] = make_float2(1.f, 1.f);
     if (A.lum_live) d.lum[o] = lum;
+}
+
+/* The candidate-independent exemplar samples of timeDerivativeTerms
+ * (texSynth_kernel.cu:448-452) for advected set k: for every pixel whose
+ * advected offset is valid, the exemplar base and output at each tap of that
+ * offset's window, with the basis and coordinates of time_setup /
+ * calc_residual_impl (same operations, so the same values).  Tap-major
+ * planes: for a fixed tap, neighbouring pixels are neighbouring entries. */
+__global__ __launch_bounds__(PT_RX *PT_RY) void k_time_table(PtArgs A, int k, float4 *tt)
+{
+    int x, y;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
+    const int o = y * A.W + x;
+    const float4 prev = A.adv[k].xyth[o];
+    if (!(prev.w > 0)) return; /* never read: the centre offset is not valid */
+    const float2 ps = A.adv[k].sc[o];
+    float b0x, b0y, b1x, b1y;
+    float ang = screen_to_ex_angle(A, (float)x, (float)y, prev.x, prev.y);
+    basis(ang, A.su * ps.x, A.sv * ps.y, b0x, b0y, b1x, b1y);
+    __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
+    const int r = A.r;
+    for (int i = -r; i <= r; i++) {
+        for (int j = -r; j <= r; j++) {
+            float ptx, pty;
+            new_coord((float)i, (float)j, prev.x, prev.y, b0x, b0y, b1x, b1y, ptx, pty);
+            float4 peb, peo;
+            ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo);
+            float4 *e = tt + time_table_index(A, x, y, i, j);
+            e[0] = peb;
+            e[1] = peo;
+        }
+    }
 }
 
 /* mergeRandomlyKernel (texSynth_kernel.cu:1891-1936); the ribbon textures
@@ -2328,11 +2390,11 @@ __device__ __forceinline__ float tap_residual(const PtArgs &A, __amdgpu_buffer_r
         bool bwd_dir = A.direction == PT_BACKWARD || A.direction == PT_BIDIRECTIONAL;
         if (any_time_w) {
             if (fwd_dir && tc.cpv && A.adv[0].xyth[oo].w > 0)
-                res = res + time_terms_b(A, er, A.adv[0], tc.pf, tc.tb4[0], tc.tb4[1], tc.tb4[2], tc.tb4[3], base, tb,
-                                         canvas, to, oo, i, j);
+                res = res + time_terms_b(A, er, 0, tc.pf, tc.tb4[0], tc.tb4[1], tc.tb4[2], tc.tb4[3], base, tb,
+                                         canvas, to, oo, x, y, i, j);
             if (bwd_dir && tc.cnv && A.adv[1].xyth[oo].w > 0)
-                res = res + time_terms_b(A, er, A.adv[1], tc.pb, tc.tb4[4], tc.tb4[5], tc.tb4[6], tc.tb4[7], base, tb,
-                                         canvas, to, oo, i, j);
+                res = res + time_terms_b(A, er, 1, tc.pb, tc.tb4[4], tc.tb4[5], tc.tb4[6], tc.tb4[7], base, tb,
+                                         canvas, to, oo, x, y, i, j);
         }
     }
     if (i == 0 && j == 0) {
@@ -3058,6 +3120,12 @@ hipError_t pt_launch_upsample(const PtArgs &a, hipStream_t s)
 hipError_t pt_launch_advect(const PtArgs &a, int forwards, int time_step, int flags, hipStream_t s)
 {
     hipLaunchKernelGGL(k_advect, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, forwards, time_step, flags);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_time_table(const PtArgs &a, int k, float4 *tt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_time_table, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, k, tt);
     return hipGetLastError();
 }
 

@@ -44,9 +44,10 @@ int pt_record_next_frame(pt_ctx *ctx, std::vector<RecOp> *ops, const pt_params *
                          const pt_style_guides *style, pt_patchxf *out_offsets, float *out_canvas_rgba,
                          float *out_residual);
 
-/* Device bytes a context holds for a sequence of this shape (working
- * buffers, pyramids, per-level snapshots): the serving admission estimate. */
-size_t pt_sequence_bytes(int image_w, int image_h, int style_w, int style_h, int levels);
+/* Device bytes a context holds for a sequence of this shape and patch
+ * radius (working buffers, forward time table, pyramids, per-level
+ * snapshots): the serving admission estimate. */
+size_t pt_sequence_bytes(int image_w, int image_h, int style_w, int style_h, int levels, int r);
 
 /* The context's HIP device and last error (for the executor). */
 int pt_ctx_device(const pt_ctx *ctx);

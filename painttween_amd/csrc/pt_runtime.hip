@@ -119,6 +119,16 @@ struct pt_ctx {
     float4 *canvas = nullptr;
     DevAdv adv[2];
     bool adv_written[2] = {false, false}; /* advected set k may hold valid offsets */
+    /* time tables (k_time_table): for advected set k, the exemplar base and
+     * output at every tap of every pixel's advected offset, gathered once and
+     * read by the temporal terms of every candidate afterwards.  Valid while
+     * the set, the exemplar textures, the image orientation, the parameters
+     * and the level stay as they were when it was built. */
+    float4 *tt[2] = {nullptr, nullptr};
+    size_t tt_cap[2] = {0, 0}; /* float4 elements */
+    bool tt_valid[2] = {false, false};
+    int tt_level[2] = {-1, -1}, tt_r[2] = {-1, -1};
+    int64_t tt_max_taps = 81; /* tables for patches of at most this many taps (0: never) */
     /* luminance-shift planes are maintained from the first parameters with
      * lum_shift_enable (or loaded nonzero shifts) until the working buffers
      * are reset; before that every shift is 0 */
@@ -439,6 +449,8 @@ extern "C" int pt_destroy(pt_ctx *c)
     for (auto &f : c->frame_pyrs) free_frame(f);
     free_style(c->style_pyr);
     dfree(c->pack_tmp);
+    dfree(c->tt[0]);
+    dfree(c->tt[1]);
     prof_clear(c);
     dfree(c->ex);
     dfree(c->ex_orient);
@@ -452,6 +464,10 @@ extern "C" int pt_destroy(pt_ctx *c)
 /* Marks the residual cache as not known to be exact (any change of offsets,
  * canvas inputs, textures or parameters other than a sync's copy). */
 static void res_invalidate(pt_ctx *c) { c->res_level = -1; }
+
+/* Marks the time tables stale (an advected set, an exemplar texture, the
+ * image orientation or the parameters changed). */
+static void tt_invalidate(pt_ctx *c) { c->tt_valid[0] = c->tt_valid[1] = false; }
 
 static int sync_stream(pt_ctx *c)
 {
@@ -637,6 +653,11 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
         if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative pixel count");
         c->ex_window_min_pixels = value;
         return PT_OK;
+    case PT_TUNE_TIME_TABLE_MAX_TAPS:
+        if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative tap count");
+        c->tt_max_taps = value;
+        tt_invalidate(c);
+        return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
     }
@@ -646,6 +667,12 @@ extern "C" int pt_upload_parameters(pt_ctx *c, const pt_params *p)
 {
     if (!c || !p) return set_err(c, PT_ERR_INVALID_ARG, "NULL argument");
     RC_CHECK(check_supported(c, p));
+    /* what the time tables depend on: the patch basis and the angle fold */
+    if (!c->have_params || p->residual_window_size != c->p.residual_window_size ||
+        p->residual_window_scale_u != c->p.residual_window_scale_u ||
+        p->residual_window_scale_v != c->p.residual_window_scale_v ||
+        p->orientation_enabled != c->p.orientation_enabled)
+        tt_invalidate(c);
     c->p = *p;
     c->have_params = true;
     if (p->lum_shift_enable) c->lum_live = true;
@@ -727,6 +754,7 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
     c->off_diff = Rect();
     c->res_diff = Rect();
     c->adv_written[0] = c->adv_written[1] = false;
+    tt_invalidate(c);
     c->cur = 0;
     c->res_level = -1;
     c->lum_live = c->have_params && c->p.lum_shift_enable;
@@ -743,6 +771,7 @@ extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
     if (num_layers < 1 || num_layers > 4) return set_err(c, PT_ERR_INVALID_ARG, "num_layers must be in [1, 4]");
     HIP_CHECK(c, hipSetDevice(c->device));
     res_invalidate(c);
+    tt_invalidate(c);
     if (c->ex && c->ew == w && c->eh == h) return PT_OK;
     size_t n = (size_t)w * h;
     if (n > c->ex_cap) {
@@ -783,6 +812,7 @@ static int reset_advected(pt_ctx *c)
         RC_CHECK(zero_fill(c, c->adv[k].lum, n * sizeof(float)));
         c->adv_written[k] = false;
     }
+    tt_invalidate(c);
     return PT_OK;
 }
 
@@ -956,6 +986,9 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
     const float *src = p->data + p->off[level];
     int prev = (c->cur + 1) % 2;
     res_invalidate(c);
+    if (layer == PT_LAYER_EXEMPLAR_BASE || layer == PT_LAYER_EXEMPLAR_OUTPUT || layer == PT_LAYER_EXEMPLAR_ORIENTATION ||
+        layer == PT_LAYER_INPUT_ORIENTATION)
+        tt_invalidate(c);
     const int W = c->W;
     auto rect = [&](void *dst, int ch) -> int {
         if (w > c->W || h > c->H) return set_err(c, PT_ERR_INVALID_ARG, "level larger than image");
@@ -1234,9 +1267,57 @@ static int copy_base_to_working(pt_ctx *c)
     return PT_OK;
 }
 
+/* Sets a.tt[k] to the time table of advected set k for this launch,
+ * building it first when it is stale: only for sets the temporal terms read
+ * (valid offsets written, the pass direction uses them, a temporal weight
+ * positive), patches of at most tt_max_taps taps, and tables that fit a
+ * 32-bit element index.  While a plan is recorded a table is never
+ * reallocated (recorded work may hold its address); a launch without a table
+ * gathers the same samples itself, with the same result. */
+static int time_tables(pt_ctx *c, int level, PtArgs &a)
+{
+    a.tt[0] = a.tt[1] = nullptr;
+    if (!a.time_active) return PT_OK;
+    if (!(a.tdi > 0.f || a.tdo > 0.f || a.tcw > 0.f)) return PT_OK;
+    const int64_t taps = (int64_t)(2 * a.r + 1) * (2 * a.r + 1);
+    if (taps > c->tt_max_taps) return PT_OK;
+    const bool dir[2] = {a.direction == PT_FORWARD || a.direction == PT_BIDIRECTIONAL,
+                         a.direction == PT_BACKWARD || a.direction == PT_BIDIRECTIONAL};
+    for (int k = 0; k < 2; k++) {
+        if (!dir[k] || !c->adv_written[k]) continue;
+        const size_t need = (size_t)taps * a.lw * a.lh * 2;
+        if (need > (size_t)INT32_MAX) continue;
+        if (c->tt_cap[k] < need) {
+            if (c->rec) continue;
+            /* capacity for level 0, so finer levels of the same radius fit */
+            const size_t cap = std::max(need, (size_t)taps * c->W * c->H * 2);
+            if (cap > (size_t)INT32_MAX) continue;
+            HIP_CHECK(c, hipStreamSynchronize(c->stream));
+            dfree(c->tt[k]);
+            c->tt[k] = nullptr;
+            c->tt_cap[k] = 0;
+            c->tt_valid[k] = false;
+            RC_CHECK(dalloc(c, &c->tt[k], cap));
+            c->tt_cap[k] = cap;
+        }
+        if (!c->tt_valid[k] || c->tt_level[k] != level || c->tt_r[k] != a.r) {
+            PtArgs b = a;
+            float4 *t = c->tt[k];
+            const int dk = k;
+            RC_CHECK(issue(c, "time table", [=](hipStream_t st) { return pt_launch_time_table(b, dk, t, st); }));
+            c->tt_valid[k] = true;
+            c->tt_level[k] = level;
+            c->tt_r[k] = a.r;
+        }
+        a.tt[k] = c->tt[k];
+    }
+    return PT_OK;
+}
+
 static int update_residual(pt_ctx *c, int level, bool allow_incremental = false)
 {
     PtArgs a = make_args(c, level);
+    RC_CHECK(time_tables(c, level, a));
     a.res_incremental = (allow_incremental && c->incremental_residual && c->res_level == level) ? 1 : 0;
     c->res_level = level;
     int pr = prof_begin(c, PK_RESIDUAL, (uint64_t)a.lw * a.lh, PB_RESIDUAL * a.lw * a.lh);
@@ -1288,6 +1369,7 @@ static int randomize_offsets(pt_ctx *c, int level)
 static int propagate(pt_ctx *c, int level, int jump)
 {
     PtArgs a = make_args(c, level);
+    RC_CHECK(time_tables(c, level, a));
     a.jump = jump;
     int pr = prof_begin(c, PK_PROPAGATE, (uint64_t)a.lw * a.lh, PB_NNF * a.lw * a.lh);
     RC_CHECK(issue_op(c, PT_OP_PROPAGATE, a));
@@ -1311,6 +1393,7 @@ static int scatter_samples(int ew, int eh, int samples)
 static int scatter(pt_ctx *c, int level)
 {
     PtArgs a = make_args(c, level);
+    RC_CHECK(time_tables(c, level, a));
     a.call = c->scatter_calls++;
     a.nscat = scatter_samples(c->ew, c->eh, a.samples);
     int pr = prof_begin(c, PK_SCATTER, (uint64_t)a.lw * a.lh, PB_NNF * a.lw * a.lh);
@@ -1353,6 +1436,7 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
     RC_CHECK(issue(c, "advect", [=](hipStream_t s) { return pt_launch_advect(a, forwards, time_step, flags, s); }));
     prof_end(c, pr);
     c->adv_written[forwards ? 0 : 1] = true;
+    tt_invalidate(c);
     DevAdv d = c->adv[forwards ? 0 : 1];
     size_t n = (size_t)c->W * c->H;
     /* dest.offsets -> base, whole buffer (texSynth_kernel.cu:1716) */
@@ -1775,6 +1859,7 @@ static int copy_base_to_advected_f(pt_ctx *c)
     }));
     c->adv_written[0] = true;
     res_invalidate(c);
+    tt_invalidate(c);
     return PT_OK;
 }
 
@@ -2709,15 +2794,18 @@ extern "C" int pt_sequence_frame(pt_ctx *c, int *frame)
     return PT_OK;
 }
 
-size_t pt_sequence_bytes(int W, int H, int SW, int SH, int levels)
+size_t pt_sequence_bytes(int W, int H, int SW, int SH, int levels, int r)
 {
     (void)levels;
     const double px = (double)W * H, ex = (double)SW * SH;
+    /* the forward time table: 32 B per tap and pixel (patches up to 81 taps) */
+    const double taps = (double)(2 * r + 1) * (2 * r + 1);
+    const double tt = taps <= 81 ? 32.0 * taps : 0.0;
     /* working buffers 328 B/px (luminance-shift planes and the image-ID
      * texture included); two frame pyramids 2 x 60 B/px x 4/3;
      * two snapshot sets 2 x 48 B/px x 4/3; exemplar textures 40 B/texel;
      * style pyramids 52 B/texel x 4/3; plus 10% slack */
-    double b = px * (328.0 + 2 * 60.0 * 4.0 / 3.0 + 2 * 48.0 * 4.0 / 3.0) + ex * (40.0 + 52.0 * 4.0 / 3.0);
+    double b = px * (328.0 + tt + 2 * 60.0 * 4.0 / 3.0 + 2 * 48.0 * 4.0 / 3.0) + ex * (40.0 + 52.0 * 4.0 / 3.0);
     return (size_t)(b * 1.1) + ((size_t)8 << 20);
 }
 

@@ -21,12 +21,15 @@ def gpu():
     ctx.close()
 
 
-def _both(style, frames, overrides, ops, early_exit=1, unit_parallel=1, tap=0):
+def _both(style, frames, overrides, ops, early_exit=1, unit_parallel=1, tap=0, time_table=None):
     """Run `ops` on the GPU and on the oracle.  tap=N > 0 sends every
     propagate and residual refresh with a supported radius to the
     tap-parallel kernels with N tap evaluations per workgroup; tap=0 keeps
-    them off so the pixel and unit-parallel kernels run."""
+    them off so the pixel and unit-parallel kernels run.  time_table=0
+    turns the per-level tables of the temporal terms off."""
     g = Context.gpu(0)
+    if time_table is not None:
+        g.set_tuning(_ffi.PT_TUNE_TIME_TABLE_MAX_TAPS, time_table)
     g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, early_exit)
     g.set_tuning(_ffi.PT_TUNE_UNIT_PARALLEL, unit_parallel)
     g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)  # exercise the early-exit kernels on small levels
@@ -128,15 +131,35 @@ def _anim_ops(levels, frames, scheme_merge="merge_random"):
     return ops
 
 
-@pytest.mark.parametrize("merge,hyst,tap", [("merge_random", 0.3, 0), ("merge_residual", 0.3, 0),
-                                            ("merge_random", 0.0, 0), ("merge_random", 0.3, 1024),
-                                            ("merge_residual", 0.3, 200)])
-def test_animation_operator_sequence(merge, hyst, tap):
+@pytest.mark.parametrize("merge,hyst,tap,tt", [("merge_random", 0.3, 0, None), ("merge_residual", 0.3, 0, None),
+                                               ("merge_random", 0.0, 0, None), ("merge_random", 0.3, 1024, None),
+                                               ("merge_residual", 0.3, 200, None), ("merge_random", 0.3, 0, 0),
+                                               ("merge_random", 0.0, 1024, 0)])
+def test_animation_operator_sequence(merge, hyst, tap, tt):
+    """Temporal terms against the oracle, with the per-level time tables
+    (default) and without them (tt=0: every candidate gathers the advected
+    offset's exemplar samples itself)."""
     st = W.style_pair(48, 48, seed=12, guides=True)
     frames = W.animation(40, 32, 3, seed=13)
     ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=hyst)
-    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, merge), tap=tap)
+    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, merge), tap=tap, time_table=tt)
     assert_same_states(ga, oa)
+
+
+@pytest.mark.parametrize("scheme", [_ffi.PT_S_FBMO, _ffi.PT_S_TCTF])
+def test_time_tables_on_and_off_agree_bidirectional(scheme):
+    """Fixup passes advect both neighbours in (bidirectional temporal terms,
+    both tables live): the synthesis with and without the tables is the same
+    bit for bit."""
+    st = W.style_pair(64, 64, seed=14, guides=True)
+    frames = W.animation(48, 40, 4, seed=15)
+    kw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5)
+    outs = []
+    for tt in (81, 0):
+        with Context.gpu(0) as g:
+            g.set_tuning(_ffi.PT_TUNE_TIME_TABLE_MAX_TAPS, tt)
+            outs.append(g.synthesize(g.default_params(**kw), frames, st, scheme=scheme, iterations=2, fixup_passes=1))
+    _cmp_synth(outs[0], outs[1])
 
 
 @pytest.mark.parametrize("early_exit,unit_parallel,tap", [(1, 1, 0), (0, 1, 0), (1, 0, 0), (0, 0, 0), (1, 1, 1024)])
