@@ -10,9 +10,11 @@ line-offset table and one chunk per scanline block.  Channels are stored in
 name order, each line holding every channel's samples in turn.
 Compressions: NONE, RLE, ZIPS (one line per block) and ZIP (16 lines per
 block), the latter two as zlib streams over the byte-interleaved,
-delta-predicted block.  PIZ and the lossy codecs are not read (the
-reference writes half RGBA images with PIZ by default; this module writes
-them with ZIP, which every OpenEXR reader accepts).
+delta-predicted block, and PIZ (32 lines per block, wavelet + Huffman,
+painttween_amd/exr_piz.py), the codec of the reference's half RGBA images
+(ImageIO::writeImage's default, imageIO.h:131-133); its float layers use
+the OpenEXR header's default, ZIP.  The lossy codecs (PXR24, B44, DWA) are
+neither read nor written.
 """
 import struct
 import zlib
@@ -20,11 +22,13 @@ import zlib
 import numpy as np
 
 from . import _ffi
+from . import exr_piz
 
 MAGIC = 20000630
 NO_COMPRESSION, RLE_COMPRESSION, ZIPS_COMPRESSION, ZIP_COMPRESSION, PIZ_COMPRESSION = 0, 1, 2, 3, 4
 UINT, HALF, FLOAT = 0, 1, 2
-_LINES = {NO_COMPRESSION: 1, RLE_COMPRESSION: 1, ZIPS_COMPRESSION: 1, ZIP_COMPRESSION: 16}
+_LINES = {NO_COMPRESSION: 1, RLE_COMPRESSION: 1, ZIPS_COMPRESSION: 1, ZIP_COMPRESSION: 16, PIZ_COMPRESSION: 32}
+_WORDS = {UINT: 2, HALF: 1, FLOAT: 2}  # 16-bit words per sample (PIZ)
 _NP = {UINT: np.dtype("<u4"), HALF: np.dtype("<f2"), FLOAT: np.dtype("<f4")}
 
 #: channel names of the reference's PatchXF layer (imageIO.cpp:45), in the
@@ -163,7 +167,7 @@ def write_exr(path, channels, compression=ZIP_COMPRESSION):
     """channels: {name: (array (H, W), HALF or FLOAT or UINT)}; all the same size."""
     if not channels:
         raise ExrError("no channels")
-    if compression not in (NO_COMPRESSION, ZIPS_COMPRESSION, ZIP_COMPRESSION):
+    if compression not in (NO_COMPRESSION, ZIPS_COMPRESSION, ZIP_COMPRESSION, PIZ_COMPRESSION):
         raise ExrError("writing compression %d is not supported" % compression)
     names = sorted(channels)
     h, w = np.asarray(channels[names[0]][0]).shape
@@ -183,7 +187,12 @@ def write_exr(path, channels, compression=ZIP_COMPRESSION):
     for bi in range(nblocks):
         y0, y1 = bi * lines, min(h, bi * lines + lines)
         raw = b"".join(p[y].tobytes() for y in range(y0, y1) for p in planes)
-        data = _compress(raw, compression)
+        if compression == PIZ_COMPRESSION:
+            data = exr_piz.compress(raw, [_WORDS[t] for _, t in chans], w, y1 - y0)
+            if len(data) >= len(raw):
+                data = raw  # stored as is when the codec does not shrink it
+        else:
+            data = _compress(raw, compression)
         chunks.append(struct.pack("<ii", y0, len(data)) + data)
     off = len(head) + 8 * nblocks
     table = b""
@@ -228,7 +237,13 @@ def read_exr(path):
         seen.add(y)
         n = min(lines, h - y)
         expect = sum(_NP[t].itemsize for _, t in chans) * w * n
-        raw = _decompress(buf[off + 8:off + 8 + size], expect, comp)
+        if comp == PIZ_COMPRESSION and size != expect:
+            try:
+                raw = exr_piz.decompress(buf[off + 8:off + 8 + size], [_WORDS[t] for _, t in chans], w, n)
+            except exr_piz.PizError as e:
+                raise ExrError("PIZ chunk at line %d: %s" % (y + hdr["ymin"], e))
+        else:
+            raw = _decompress(buf[off + 8:off + 8 + size], expect, comp)
         if len(raw) != expect:
             raise ExrError("chunk at line %d holds %d bytes, expected %d" % (y + hdr["ymin"], len(raw), expect))
         p = 0
@@ -261,9 +276,10 @@ def read_offsets(path):
     return out
 
 
-def write_rgba(path, rgba, compression=ZIP_COMPRESSION):
-    """An RGBA float image as half-float R, G, B, A channels (the reference
-    rounds canvas and residual images to half, imageIO.cpp:356-371)."""
+def write_rgba(path, rgba, compression=PIZ_COMPRESSION):
+    """An RGBA float image as half-float R, G, B, A channels with the
+    reference's PIZ compression (it rounds canvas and residual images to
+    half, imageIO.cpp:356-371)."""
     a = np.asarray(rgba, np.float32)
     with np.errstate(over="ignore"):  # beyond 65504 rounds to infinity, as OpenEXR's half does
         write_exr(path, {c: (a[..., k].astype(np.float16), HALF) for k, c in enumerate("RGBA")},

@@ -134,7 +134,7 @@ def test_rejects_other_files(tmp_path):
     with pytest.raises(exr.ExrError):
         exr.read_exr(p)
     with pytest.raises(exr.ExrError):
-        exr.write_exr(p, {"R": (np.zeros((2, 2), np.float32), exr.FLOAT)}, exr.PIZ_COMPRESSION)
+        exr.write_exr(p, {"R": (np.zeros((2, 2), np.float32), exr.FLOAT)}, 6)  # B44: lossy, not written
 
 
 def _patch_data_window(buf, box):
@@ -178,3 +178,112 @@ def test_malformed_chunks_raise_exr_error(tmp_path):
     p.write_bytes(bytes(b))
     with pytest.raises(exr.ExrError):
         exr.read_exr(p)
+
+
+# ---------------------------------------------------------------- PIZ
+from painttween_amd import exr_piz  # noqa: E402
+
+
+def test_piz_wavelet_steps_worked_by_hand():
+    """14-bit step: (a, b) -> ((a + b) >> 1, a - b) in signed 16-bit; a 2x2
+    block runs it along rows, then along columns of the row results.
+    [[10, 4], [7, 1]]: rows (7, 6), (4, 6); columns (5, 3), (6, 0).  16-bit
+    step: a' = (a + 2^15) mod 2^16, m = (a' + b) >> 1, d = a' - b (+2^15 on m
+    when d < 0), both mod 2^16: (40000, 1000) -> (4116, 6232)."""
+    a = np.array([[10, 4], [7, 1]], np.int64)
+    exr_piz.wavelet_encode(a, 100)
+    assert a.tolist() == [[5, 6], [3, 0]]
+    exr_piz.wavelet_decode(a, 100)
+    assert a.tolist() == [[10, 4], [7, 1]]
+    lo, hi = exr_piz._enc(np.array([40000]), np.array([1000]), False)
+    assert (int(lo[0]), int(hi[0])) == (4116, 6232)
+    lo, hi = exr_piz._enc(np.array([1000]), np.array([40000]), False)  # d < 0: m shifted by 2^15
+    a2 = (1000 + 32768) & 0xFFFF
+    assert (int(lo[0]), int(hi[0])) == ((((a2 + 40000) >> 1) + 32768) & 0xFFFF, (a2 - 40000) & 0xFFFF)
+    lo, hi = exr_piz._enc(np.array([3]), np.array([65535]), True)  # -1 as a signed short
+    assert (int(lo[0]), int(hi[0])) == ((3 - 1) >> 1, 4)
+
+
+def test_piz_huffman_streams_worked_by_hand():
+    """Header u32 min symbol, max symbol (the repeat symbol, one past the
+    largest word), table bytes, bit count, 0; 6-bit code lengths; canonical
+    codes with the longer codes numerically lower: lengths 1, 2, 2 give
+    codes 1, 00, 01."""
+    # [0, 0, 0, 1]: lengths 0:1 1:2 rep:2; table 000001 000010 000010; data 1 1 1 00
+    assert exr_piz.huffman_encode(np.array([0, 0, 0, 1])).hex() == \
+        "00000000" "02000000" "03000000" "05000000" "00000000" "042080" "e0"
+    # [0]*5 + [10]: zeros 1..9 as a long run (63, then 9 - 6 in 8 bits)
+    h = exr_piz.huffman_encode(np.array([0] * 5 + [10]))
+    assert h[:20] == bytes.fromhex("00000000" "0b000000" "04000000" "07000000" "00000000")
+    assert h[20:].hex() == "07f03082" "f8"
+    # [0]*5 + [3]: zeros 1..2 as a short run (59 + 2 - 2)
+    assert exr_piz.huffman_encode(np.array([0] * 5 + [3]))[20:].hex() == "07b082" "f8"
+    # [5]*300 + [7]: the run goes out as 256 (symbol, repeat, 255) and 44 (symbol, repeat, 43)
+    h = exr_piz.huffman_encode(np.array([5] * 300 + [7]))
+    assert h.hex() == "05000000" "08000000" "03000000" "18000000" "00000000" "040082" "bff4ac"
+    for w in ([0, 0, 0, 1], [0] * 5 + [10], [5] * 300 + [7]):
+        assert exr_piz.huffman_decode(exr_piz.huffman_encode(np.array(w)), len(w)).tolist() == w
+    assert exr_piz.canonical_codes([1, 2, 2]).tolist() == [1, 0, 1]
+    assert exr_piz.canonical_codes([2, 0, 3, 3, 1]).tolist() == [1, 0, 0, 1, 1]
+
+
+def test_piz_round_trips_random_words_and_planes():
+    rng = np.random.default_rng(8)
+    for trial in range(20):
+        n = int(rng.integers(1, 4000))
+        w = rng.integers(0, int(rng.integers(2, 70000)), n)
+        w[rng.random(n) < 0.4] = w[0]
+        w = np.minimum(w, 65535)
+        assert (exr_piz.huffman_decode(exr_piz.huffman_encode(w), n) == w).all()
+    for mx in (9, 16383, 16384, 65535):
+        for ny, nx in ((1, 1), (1, 9), (6, 1), (3, 5), (32, 17), (7, 64), (32, 33), (31, 1000)):
+            a = rng.integers(0, mx + 1, (ny, nx))
+            b = a.copy()
+            exr_piz.wavelet_encode(b, mx)
+            exr_piz.wavelet_decode(b, mx)
+            assert (a == b).all(), (mx, ny, nx)
+
+
+def test_piz_files_round_trip(tmp_path):
+    """Half RGBA (three 32-line chunks, odd width), FLOAT and UINT channels
+    (two words per sample), and constant images; write_rgba writes PIZ."""
+    rng = np.random.default_rng(9)
+    rgba = rng.uniform(-2, 2, (70, 45, 4)).astype(np.float32)
+    rgba[10:40, 5:30] = 0.25  # runs for the repeat symbol
+    p = tmp_path / "rgba.exr"
+    exr.write_rgba(p, rgba)
+    assert b"compression\0compression\0" + struct.pack("<i", 1) + bytes([exr.PIZ_COMPRESSION]) in p.read_bytes()
+    assert exr.read_rgba(p).tobytes() == rgba.astype(np.float16).astype(np.float32).tobytes()
+    f = rng.uniform(0, 1000, (37, 23)).astype(np.float32)
+    u = rng.integers(0, 2 ** 32, (37, 23), dtype=np.uint64).astype(np.uint32)
+    exr.write_exr(p, {"a": (f, exr.FLOAT), "b": (u, exr.UINT), "c": (f.astype(np.float16), exr.HALF)},
+                  exr.PIZ_COMPRESSION)
+    planes, hdr = exr.read_exr(p)
+    assert hdr["compression"] == exr.PIZ_COMPRESSION
+    assert planes["a"].tobytes() == f.tobytes() and planes["b"].tobytes() == u.tobytes()
+    assert planes["c"].tobytes() == f.astype(np.float16).tobytes()
+    for v in (0.0, 1.0):
+        exr.write_rgba(p, np.full((33, 3, 4), v, np.float32))
+        assert np.all(exr.read_rgba(p) == v)
+
+
+def test_piz_chunk_layout(tmp_path):
+    """A chunk: u16 first and last nonzero bitmap byte, those bytes (bit v & 7
+    of byte v >> 3 marks word v; word 0 is never marked), i32 Huffman length,
+    the Huffman stream.  One 2x2 HALF channel of 1.0 (word 0x3C00) and 0:
+    bitmap byte 0x780 holds bit 0."""
+    p = tmp_path / "c.exr"
+    a = np.array([[1.0, 0.0], [0.0, 1.0]], np.float16)
+    exr.write_exr(p, {"R": (a, exr.HALF)}, exr.PIZ_COMPRESSION)
+    planes, _ = exr.read_exr(p)
+    assert planes["R"].tobytes() == a.tobytes()
+    chunk = exr_piz.compress(a.tobytes(), [1], 2, 2)
+    lo, hi = struct.unpack_from("<HH", chunk, 0)
+    assert (lo, hi) == (0x3C00 >> 3, 0x3C00 >> 3) and chunk[4] == 1
+    (n,) = struct.unpack_from("<i", chunk, 5)
+    assert n == len(chunk) - 9
+    assert exr_piz.decompress(chunk, [1], 2, 2) == a.tobytes()
+    z = exr_piz.compress(np.zeros((2, 2), np.float16).tobytes(), [1], 2, 2)
+    assert struct.unpack_from("<HH", z, 0) == (exr_piz.BITMAP_SIZE - 1, 0)
+    with pytest.raises(exr_piz.PizError):
+        exr_piz.decompress(chunk[:-1], [1], 2, 2)
