@@ -394,6 +394,45 @@ def anim_cpu_baseline(frames, style, kw, levels, iterations, threads, gpu_pde_pe
                       % (pdes, dt, gpu_pde_per_frame)}
 
 
+def served_alone_1024(params, iterations, img, frames=4):
+    """One 1024x1024 sequence (exemplar 1024^2, the served motion model)
+    on an otherwise idle server: per-frame latency from submit to completion,
+    beside the plain per-frame path's time for the same frames."""
+    import torch
+    from painttween_amd import Context, workloads as W
+    from painttween_amd.forms import DeviceOutputs, Server, next_frame
+    fr, st = W.serving_sequence(dict(w=1024, h=1024, s=1024, seed=99), frames)
+    sg = img.style(st)
+    fg = [img.frame(f) for f in fr]
+    plain = []
+    with Context.gpu(0) as c:
+        for rnd in range(2):
+            c.lib.pt_sequence_reset(c.handle)
+            for f in fg:
+                out = DeviceOutputs(1024, 1024, 1)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                next_frame(c, params, iterations, f, sg, 1024, 1024, out)
+                if rnd == 1:
+                    plain.append((time.perf_counter() - t0) * 1e3)
+    srv = Server(200 << 30, 4)
+    lat = []
+    try:
+        q = srv.open_device(params, iterations, 1024, 1024, sg, 1024, 1024)
+        for rnd in range(2):  # round 0 makes the sequence resident
+            for f in fg:
+                t = srv.submit(q, f)
+                srv.drain()
+                for tk, ms in srv.poll(max_ticks=0, max_done=16):
+                    if rnd == 1 and tk == t:
+                        lat.append(ms)
+            srv.outputs.clear()
+    finally:
+        srv.close()
+    return {"frames": frames, "served_ms": lat, "plain_ms": plain,
+            "served_mean_ms": float(np.mean(lat)) if lat else None, "plain_mean_ms": float(np.mean(plain))}
+
+
 def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0, cpu_pde_rate=None):
     """The serving loop under Poisson arrivals against the plain per-frame
     path (one context per sequence, frames one at a time in arrival order)."""
@@ -496,12 +535,14 @@ def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0, cpu_pde_rate=None):
               file=sys.stderr, flush=True)
     st = srv.stats()
     srv.close()
+    alone = served_alone_1024(params, iterations, img)
     px = sum(d["w"] * d["h"] for d in dev)
     return {"sequences": N, "frames_per_sequence_per_round": frames, "pixels_per_round": px * frames,
             "plain_fps": len(service) / sum(service), "bit_exact_vs_plain": bool(exact), "rates": per_rate,
             "budget_gb": budget_gb, "resident_bytes": st["resident_bytes"], "launches": st["launches"],
             "multi_launches": st["multi_launches"], "ticks": st["ticks"],
             "schedule": "per-frame FMBM_RM first pass, 7x7 patches, 4 levels, 3 iterations, temporal coherence 0.5",
+            "alone_1024": alone, "fill_tiles": 2048,
             "pde_per_frame": served_pdes / len(service),
             "cpu_baseline": None if not cpu_pde_rate else {
                 "value": cpu_pde_rate["pde_per_s"] / (served_pdes / len(service)), "unit": "frames/s",

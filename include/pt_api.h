@@ -591,8 +591,15 @@ int pt_server_poll(pt_server *s, int max_ticks, uint64_t *done, double *latency_
 int pt_server_drain(pt_server *s);
 /* Releases a sequence's device state once its queued frames are done. */
 int pt_server_close(pt_server *s, int32_t seq);
+/* Latency policy (default 2048): each engine tick advances the in-flight
+ * frames oldest first, the oldest always, younger ones while the workgroup
+ * tiles taken so far are below this value, so a step that fills the GPU runs
+ * alone and small (coarse-level) steps are joined by younger frames' steps;
+ * 0 advances every in-flight frame each tick (processor sharing). */
+#define PT_SERVER_FILL_TILES 5
 /* Launch settings of the engine, keys as pt_batch_set: PT_BATCH_XCD_REMAP
- * (default 0 here), PT_BATCH_SHAPE, PT_BATCH_ISOLATE_TILES. */
+ * (default 0 here), PT_BATCH_SHAPE, PT_BATCH_ISOLATE_TILES, and
+ * PT_SERVER_FILL_TILES. */
 int pt_server_set(pt_server *s, int key, int value);
 
 typedef struct pt_server_stats {

@@ -54,6 +54,9 @@ struct ShapePolicy {
      * a launch of its own (sharing one launch would only spread the gathers
      * over several exemplars at once) */
     int isolate_tiles = 2048;
+    /* a launch group of one job uses the single-job kernel (arguments by
+     * value, no launch table) when its shape is the default one */
+    bool single_direct = true;
     ShapePolicy()
     {
         for (auto &r : shape)
@@ -77,7 +80,7 @@ struct Lane {
 
 /* One entry of a launch program. */
 struct Step {
-    int kind = 0; /* 0: closure ops[opi] of lane; 1: multi-job launch */
+    int kind = 0; /* 0: closure ops[opi] of lane; 1: multi-job launch; 2: one job's op, default shape */
     int lane = -1;
     size_t opi = 0;
     int op = -1, shape = 0, cls = 0, n = 0, ntiles = 0;
@@ -149,6 +152,23 @@ void build_tick(std::vector<Lane *> &lanes, const ShapePolicy &pol, std::vector<
     }
     for (auto &kv : groups) {
         Group &g = kv.second;
+        if (g.lanes.size() == 1 && pol.single_direct) {
+            /* one job: the plain single-job kernel with its arguments by
+             * value when the policy's shape is the default one (no table) */
+            Lane *L = lanes[g.lanes[0]];
+            const PtArgs &a = L->ops[L->pos].a;
+            if (g.shape == pt_default_shape(g.op, a)) {
+                Step s;
+                s.kind = 2;
+                s.op = g.op;
+                s.shape = g.shape;
+                s.n = 1;
+                s.first = a;
+                out.push_back(s);
+                L->pos++;
+                continue;
+            }
+        }
         std::vector<const PtArgs *> args;
         std::vector<int> tiles, tx;
         int ntiles = 0;
@@ -181,6 +201,7 @@ void build_tick(std::vector<Lane *> &lanes, const ShapePolicy &pol, std::vector<
 hipError_t issue_step(const Step &s, std::vector<Lane *> &lanes, const char *tables_dev, int xcd, hipStream_t st)
 {
     if (s.kind == 0) return lanes[s.lane]->ops[s.opi].fn(st);
+    if (s.kind == 2) return pt_run_op(s.op, s.first, st);
     const char *p = tables_dev + s.table_off;
     PtJobTable t;
     t.args = (const PtArgs *)p;
@@ -571,6 +592,12 @@ struct pt_server {
     std::vector<bool> slot_used;
     int slot_next = 0;
     int xcd = 0;
+    /* latency policy: a tick advances the in-flight frames oldest first while
+     * their steps fit this many workgroup tiles (the oldest always goes);
+     * 0 advances every in-flight frame each tick */
+    int fill_tiles = 2048;
+    /* ticks planned per table upload (one host-to-device copy for all of them) */
+    int ticks_ahead = 16;
     std::vector<std::pair<uint64_t, double>> completed;
     int64_t submitted = 0, ncompleted = 0, ticks = 0, launches = 0, multi = 0;
 };
@@ -601,7 +628,8 @@ extern "C" int pt_server_create(pt_server **out, int device, uint64_t budget, in
     s->device = device;
     s->budget = budget;
     s->max_in_flight = max_in_flight;
-    s->slot_bytes = (size_t)max_in_flight * (sizeof(PtArgs) + 3 * sizeof(int)) + 64 * kTableAlign;
+    /* worst case of one tick: every in-flight frame in a launch group of its own */
+    s->slot_bytes = (size_t)s->ticks_ahead * max_in_flight * (sizeof(PtArgs) + 3 * sizeof(int) + kTableAlign);
     s->slot_bytes = (s->slot_bytes + kTableAlign - 1) / kTableAlign * kTableAlign;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -735,47 +763,98 @@ static int admit(pt_server *s)
     return PT_OK;
 }
 
-static int tick(pt_server *s)
+/* Workgroup tiles of the next multi-job step of a lane (its leading
+ * closures skipped), 0 when none is left. */
+static int next_step_tiles(const Lane &L, const ShapePolicy &pol)
 {
-    std::vector<Lane *> lanes;
-    for (auto *f : s->flights) lanes.push_back(f->issued ? nullptr : &f->lane);
+    size_t p = L.pos;
+    while (p < L.ops.size() && L.ops[p].op < 0) p++;
+    if (p >= L.ops.size()) return 0;
+    const RecOp &o = L.ops[p];
+    return pt_shape_tiles(o.op, pol.pick(o.op, o.a), o.a, nullptr);
+}
+
+/* The lanes one tick advances: oldest frame first, its step always;
+ * younger frames join while the tiles taken so far are below fill_tiles, so
+ * a step that fills the GPU runs alone and the GPU left idle by a small
+ * (coarse-level) step is filled with the steps of younger frames.  Empty when
+ * no in-flight frame has work left. */
+static std::vector<Lane *> tick_lanes(pt_server *s, const std::vector<int> &order)
+{
+    std::vector<Lane *> lanes(s->flights.size(), nullptr);
+    int used = 0;
+    bool any = false;
+    for (int i : order) {
+        Flight *f = s->flights[i];
+        if (f->issued || f->lane.done()) continue;
+        if (any && s->fill_tiles > 0 && used >= s->fill_tiles) continue;
+        lanes[i] = &f->lane;
+        used += next_step_tiles(f->lane, s->pol);
+        any = true;
+    }
+    if (!any) lanes.clear();
+    return lanes;
+}
+
+/* Plans up to min(ticks_ahead, limit) ticks, uploads their launch tables
+ * with one copy and issues them; a frame's completion event is recorded
+ * right after the tick that issued its last step.  *ran receives the number
+ * of ticks issued. */
+static int tick(pt_server *s, int limit, int *ran)
+{
+    *ran = 0;
+    std::vector<int> order(s->flights.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::sort(order.begin(), order.end(),
+              [&](int a, int b) { return s->flights[a]->req.ticket < s->flights[b]->req.ticket; });
     std::vector<Step> steps;
     std::vector<char> tables;
+    std::vector<size_t> tick_end;            /* steps issued by the first k+1 ticks */
+    std::vector<int> done_tick(s->flights.size(), -1);
     size_t toff = 0;
-    build_tick(lanes, s->pol, steps, tables, &toff);
+    std::vector<Lane *> all(s->flights.size(), nullptr);
+    for (size_t i = 0; i < s->flights.size(); i++) all[i] = &s->flights[i]->lane;
+    for (int t = 0; t < std::min(s->ticks_ahead, limit); t++) {
+        std::vector<Lane *> lanes = tick_lanes(s, order);
+        if (lanes.empty()) break;
+        build_tick(lanes, s->pol, steps, tables, &toff);
+        tick_end.push_back(steps.size());
+        for (size_t i = 0; i < s->flights.size(); i++)
+            if (done_tick[i] < 0 && !s->flights[i]->issued && s->flights[i]->lane.done()) done_tick[i] = t;
+    }
     if (steps.empty()) return PT_OK;
     const char *dev = s->ring_dev;
+    int k = -1;
     if (toff > 0) {
         if (toff > s->slot_bytes) return sfail(s, PT_ERR_STATE, "tick tables exceed the staging slot");
-        int k = s->slot_next;
+        k = s->slot_next;
         s->slot_next = (k + 1) % s->nslots;
         if (s->slot_used[k]) S_HIP(s, hipEventSynchronize(s->slot_ev[k]));
         memcpy(s->ring_host + k * s->slot_bytes, tables.data(), toff);
         S_HIP(s, hipMemcpyAsync(s->ring_dev + k * s->slot_bytes, s->ring_host + k * s->slot_bytes, toff,
                                 hipMemcpyHostToDevice, s->stream));
         dev = s->ring_dev + k * s->slot_bytes;
-        for (const Step &st : steps) {
-            hipError_t e = issue_step(st, lanes, dev, s->xcd, s->stream);
+    }
+    size_t si = 0;
+    for (size_t t = 0; t < tick_end.size(); t++) {
+        for (; si < tick_end[t]; si++) {
+            hipError_t e = issue_step(steps[si], all, dev, s->xcd, s->stream);
             if (e != hipSuccess) return sfail(s, PT_ERR_HIP, "tick launch failed: %s", hipGetErrorString(e));
+            s->launches++;
+            s->multi += steps[si].kind == 1;
         }
+        s->ticks++;
+        (*ran)++;
+        for (size_t i = 0; i < s->flights.size(); i++)
+            if (done_tick[i] == (int)t) {
+                S_HIP(s, hipEventRecord(s->flights[i]->done_ev, s->stream));
+                s->flights[i]->issued = true;
+            }
+    }
+    if (k >= 0) {
         S_HIP(s, hipEventRecord(s->slot_ev[k], s->stream));
         s->slot_used[k] = true;
-    } else {
-        for (const Step &st : steps) {
-            hipError_t e = issue_step(st, lanes, dev, s->xcd, s->stream);
-            if (e != hipSuccess) return sfail(s, PT_ERR_HIP, "tick launch failed: %s", hipGetErrorString(e));
-        }
     }
-    for (const Step &st : steps) {
-        s->launches++;
-        s->multi += st.kind == 1;
-    }
-    s->ticks++;
-    for (auto *f : s->flights)
-        if (!f->issued && f->lane.done()) {
-            S_HIP(s, hipEventRecord(f->done_ev, s->stream));
-            f->issued = true;
-        }
     return PT_OK;
 }
 
@@ -808,13 +887,15 @@ extern "C" int pt_server_poll(pt_server *s, int max_ticks, uint64_t *done, doubl
     if (!s || max_ticks < 0 || (max_done > 0 && !done) || !n_done) return sfail(s, PT_ERR_INVALID_ARG, "bad poll");
     S_HIP(s, hipSetDevice(s->device));
     int rc;
-    for (int t = 0; t < max_ticks; t++) {
+    for (int t = 0; t < max_ticks;) {
         if ((rc = collect(s)) != PT_OK) return rc;
         if ((rc = admit(s)) != PT_OK) return rc;
         bool work = false;
         for (auto *f : s->flights) work = work || !f->issued;
         if (!work) break;
-        if ((rc = tick(s)) != PT_OK) return rc;
+        int ran = 0;
+        if ((rc = tick(s, max_ticks - t, &ran)) != PT_OK) return rc;
+        t += std::max(ran, 1);
     }
     if ((rc = collect(s)) != PT_OK) return rc;
     int n = std::min<int>(max_done, (int)s->completed.size());
@@ -841,7 +922,8 @@ extern "C" int pt_server_drain(pt_server *s)
         bool work = false;
         for (auto *f : s->flights) work = work || !f->issued;
         if (work) {
-            rc = tick(s);
+            int ran = 0;
+            rc = tick(s, s->ticks_ahead, &ran);
             if (rc != PT_OK) return rc;
         } else if (!s->flights.empty()) {
             S_HIP(s, hipStreamSynchronize(s->stream));
@@ -871,6 +953,10 @@ extern "C" int pt_server_set(pt_server *s, int key, int value)
     if (!s) return sfail(nullptr, PT_ERR_INVALID_ARG, "NULL server");
     switch (key) {
     case PT_BATCH_XCD_REMAP: s->xcd = value != 0; return PT_OK;
+    case PT_SERVER_FILL_TILES:
+        if (value < 0) return sfail(s, PT_ERR_INVALID_ARG, "fill threshold must be >= 0");
+        s->fill_tiles = value;
+        return PT_OK;
     case PT_BATCH_ISOLATE_TILES:
         if (value < 1) return sfail(s, PT_ERR_INVALID_ARG, "isolate threshold must be >= 1");
         s->pol.isolate_tiles = value;

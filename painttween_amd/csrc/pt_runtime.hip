@@ -1271,9 +1271,9 @@ static int copy_base_to_working(pt_ctx *c)
  * building it first when it is stale: only for sets the temporal terms read
  * (valid offsets written, the pass direction uses them, a temporal weight
  * positive), patches of at most tt_max_taps taps, and tables that fit a
- * 32-bit element index.  While a plan is recorded a table is never
- * reallocated (recorded work may hold its address); a launch without a table
- * gathers the same samples itself, with the same result. */
+ * 32-bit element index.  While a plan is recorded an existing table is
+ * never reallocated (recorded work may hold its address); a launch without a
+ * table gathers the same samples itself, with the same result. */
 static int time_tables(pt_ctx *c, int level, PtArgs &a)
 {
     a.tt[0] = a.tt[1] = nullptr;
@@ -1288,7 +1288,7 @@ static int time_tables(pt_ctx *c, int level, PtArgs &a)
         const size_t need = (size_t)taps * a.lw * a.lh * 2;
         if (need > (size_t)INT32_MAX) continue;
         if (c->tt_cap[k] < need) {
-            if (c->rec) continue;
+            if (c->rec && c->tt[k]) continue; /* recorded work may hold the old table */
             /* capacity for level 0, so finer levels of the same radius fit */
             const size_t cap = std::max(need, (size_t)taps * c->W * c->H * 2);
             if (cap > (size_t)INT32_MAX) continue;
