@@ -43,6 +43,10 @@ extern "C" {
  * one-shot schedules refuse larger sizes with PT_ERR_UNSUPPORTED. */
 #define PT_MAX_EXEMPLAR_TEXELS 67108863 /* (2^31 - 1) / 32: below 8192 x 8192 */
 #define PT_MAX_IMAGE_PIXELS 2147483647
+/* Style layers of the exemplar textures (pt_initialize_exemplars,
+ * pt_initialize_cumulative_fields); layers * texels of one exemplar level
+ * stays below PT_MAX_EXEMPLAR_TEXELS. */
+#define PT_MAX_STYLE_LAYERS 4
 
 /* ---- PassDirection (types.h:128-132) ---------------------------------- */
 #define PT_FORWARD 0
@@ -72,13 +76,20 @@ extern "C" {
 #define PT_LAYER_INPUT_VEL_F 12
 #define PT_LAYER_INPUT_VEL_B_PREVIOUS 13
 #define PT_LAYER_INPUT_VEL_F_PREVIOUS 14
+#define PT_LAYER_CUMUL_FIELD_B 17         /* key-to-key ribbons, layer = style_layer */
+#define PT_LAYER_CUMUL_FIELD_F 18
+#define PT_LAYER_FRAME_TO_KEY_RIBBON_B 19 /* frame-to-key ribbons of the frame */
+#define PT_LAYER_FRAME_TO_KEY_RIBBON_F 20
 #define PT_LAYER_RANDOM 21
 
 /* ---- operation flags (texSynth_kernel.h:79-91), same bits ------------- */
 #define PT_DONT_COPY_OFFSETS 0x01
 #define PT_DONT_UPDATE_RESIDUAL 0x10
+#define PT_MERGE_FALLBACK_RANDOM 0x20
+#define PT_MERGE_FALLBACK_LAST_PASS 0x40
 #define PT_ADVECT_IGNORE_RANDOM 0x1
 #define PT_ADVECT_DO_NOT_CHECK_PREVIOUS 0x10
+#define PT_ADVECT_CHECK_FRAME_TO_KEY 0x20
 #define PT_ADVECT_UPDATE_RESIDUAL 0x80
 
 /* ---- scatter random-number modes (no counterpart in the reference) ----
@@ -98,6 +109,11 @@ extern "C" {
 /* DownsampleScaleMode (downsample.h:35-38), same values. */
 #define PT_DOWNSAMPLE_SCALE_WITH_LEVELS 0
 #define PT_DOWNSAMPLE_SCALE_CONSTANT 1
+/* Ribbon pyramids (initializeRibbonPyramid, imagePyramid.cpp:211-235):
+ * PT_ELEM_F32X4 texels packed as RibbonP's Color4 (time_step, y, x,
+ * layer); level L+1 takes the top-left texel of each 2x2 block with x and
+ * y halved. */
+#define PT_DOWNSAMPLE_RIBBON 2
 
 /*
  * pt_params -- TsParameters (texSynth_interface.h:79-152) as a fixed-width
@@ -177,6 +193,14 @@ typedef struct pt_patchxf {
     float lum_y, lum_a;
 } pt_patchxf;
 
+/* pt_ribbonp -- RibbonP (types.h:116-126), 16 bytes: the offset from a pixel
+ * to where a keyframe's texel travelled, its time step, and the keyframe's
+ * style layer (-1: invalid). */
+typedef struct pt_ribbonp {
+    float x, y;
+    int32_t time_step, layer;
+} pt_ribbonp;
+
 typedef struct pt_ctx pt_ctx;
 typedef struct pt_pyramid pt_pyramid;
 
@@ -215,14 +239,21 @@ int pt_upload_parameters(pt_ctx *ctx, const pt_params *params);
  * style_w/style_h are the level-0 exemplar dimensions (they size the
  * offsets-histogram buffer whose dimensions drive the patch wrap). */
 int pt_initialize(pt_ctx *ctx, int image_w, int image_h, int style_w, int style_h);
-/* TexSynth::initialize_exemplars (texSynth_kernel.cu:2119-2126): exemplar
- * textures of one level (w, h are that level's dimensions), num_layers in
- * 1..4 (the number of styles).  Without keyframe interpolation every offset
- * stays in style layer 0 (random initialisation, scatter and the average
- * candidate use style 0, synthesisFrame.cpp:107-110, texSynth_kernel.cu:925,
- * 939), so only layer 0 is stored; more styles enlarge the patch wrap height
- * to style_h * num_styles, as the reference's offsets histogram does. */
+/* TexSynth::initialize_exemplars (texSynth_kernel.cu:2119-2126): layered
+ * exemplar textures of one level (w, h are that level's dimensions),
+ * num_layers in 1..PT_MAX_STYLE_LAYERS (the styles, which are the keyframes
+ * with keyframe interpolation).  An offset samples the layer it carries
+ * (PatchXF::layer), the index clamped into the range as a layered texture
+ * does.  Without keyframe interpolation every offset stays in layer 0
+ * (random initialisation, scatter and the average candidate use style 0,
+ * synthesisFrame.cpp:107-110, texSynth_kernel.cu:925,939); more styles then
+ * only enlarge the patch wrap height to style_h * num_styles, as the
+ * reference's offsets histogram does. */
 int pt_initialize_exemplars(pt_ctx *ctx, int w, int h, int num_layers);
+/* TexSynth::initialize_cumulativeFields (texSynth_kernel.cu:2136-2140): the
+ * layered key-to-key fields B and F (w, h: their level dimensions), zero
+ * filled; PT_LAYER_CUMUL_FIELD_B/F uploads fill layer style_layer. */
+int pt_initialize_cumulative_fields(pt_ctx *ctx, int w, int h, int num_layers);
 /* WorkingBuffers::resetAdvectedBuffers (workingBuffers.cpp:113-117). */
 int pt_reset_advected_buffers(pt_ctx *ctx);
 /* TexSynth::flipAnimBuffers (texSynth_kernel.cu:1534-1538). */
@@ -259,10 +290,12 @@ int pt_pyramid_download(const pt_pyramid *pyr, int level, void *host);
  * binds `level` of `pyr` to image layer `layer`, device to device.  The
  * element kind must match the layer (color/exemplar/random: F32X4, velocity
  * and scale: F32X2, orientation and distance transform: F32, INPUT_ID and
- * EXEMPLAR_OBJECT_IDS: I32).  Until INPUT_ID is uploaded the image ID
- * texture is a single 0 texel.  style_layer (0..3) selects the style of an
- * exemplar layer; layers above 0 are accepted and never sampled (see
- * pt_initialize_exemplars); image layers ignore it. */
+ * EXEMPLAR_OBJECT_IDS: I32, cumulative fields and frame-to-key ribbons:
+ * F32X4 RibbonP texels).  Until INPUT_ID is uploaded the image ID texture
+ * is a single 0 texel, and so are the frame-to-key ribbons (a valid layer-0
+ * ribbon, as the reference's unbound textures read).  style_layer (0..3)
+ * selects the layer of an exemplar texture or a cumulative field; image
+ * layers ignore it. */
 int pt_upload_image(pt_ctx *ctx, int level, int layer, const pt_pyramid *pyr, int style_layer);
 
 /* SynthesisFrame::createRandomPyramid (synthesisFrame.cpp:66-150) on the GPU
@@ -273,14 +306,33 @@ int pt_upload_image(pt_ctx *ctx, int level, int layer, const pt_pyramid *pyr, in
 int pt_create_random_pyramid(pt_ctx *ctx, int level, int frame_number,
                              const pt_pyramid *image_color, const pt_pyramid *image_orientation,
                              const pt_pyramid *exemplar_output, const pt_pyramid *exemplar_orientation);
+/* The same with keyframe interpolation (setupWithKeys, synthesisFrame.cpp:
+ * 94-110): each pixel first draws its style thisStyle = rand() % (number of
+ * keys) + first_key_index (params of the context), then its texel in that
+ * style; styles[k] are the output and orientation pyramids of style k
+ * (num_styles of them).  The style draw is Philox counter (x, y, 51, 0). */
+int pt_create_random_pyramid_keys(pt_ctx *ctx, int level, int frame_number, const pt_pyramid *image_color,
+                                  const pt_pyramid *image_orientation, int num_styles,
+                                  const pt_pyramid *const *exemplar_outputs,
+                                  const pt_pyramid *const *exemplar_orientations);
 
 /* ---- NNF operators (TexSynth::*) --------------------------------------- */
 
 /* TexSynth::randomizeOffsets (texSynth_kernel.cu:2100-2109). */
 int pt_randomize_offsets(pt_ctx *ctx, int level);
-/* TexSynth::propagate (texSynth_kernel.cu:1306-1357); key_index must be -1. */
+/* TexSynth::offsetsFromFrameToKeyRibbon (texSynth_kernel.cu:2061-2118):
+ * offsets from the frame-to-key ribbon F (look_forwards) or B, the random
+ * texel where the ribbon is invalid, then syncBuffers. */
+int pt_offsets_from_frame_to_key_ribbon(pt_ctx *ctx, int level, int look_forwards);
+/* TexSynth::propagate (texSynth_kernel.cu:1306-1357): propagate_kernel, or
+ * propagate_kernel_keys when params.interpolate_key_frame is set (every
+ * neighbour candidate also followed through the cumulative fields to the
+ * other keys, :984-1110).  key_index >= 0: pixels where that style layer's
+ * output is painted (alpha > 0) take their own position in it. */
 int pt_propagate(pt_ctx *ctx, int level, int jump, int key_index);
-/* TexSynth::scatter (texSynth_kernel.cu:1360-1401); key_index must be -1. */
+/* TexSynth::scatter (texSynth_kernel.cu:1360-1401): scatter_kernel, or
+ * scatter_kernel_keys with params.interpolate_key_frame (:723-839); key_index
+ * as for pt_propagate (painted pixels keep their offset). */
 int pt_scatter(pt_ctx *ctx, int level, int key_index);
 /* TexSynth::upsample (texSynth_kernel.cu:1513-1532). */
 int pt_upsample(pt_ctx *ctx, int fine_level);
@@ -294,6 +346,21 @@ int pt_advect_offsets(pt_ctx *ctx, int level, int time_step, int forwards, int f
 int pt_merge_randomly(pt_ctx *ctx, int level, int forwards, float weight_advected, int force_valid);
 /* TexSynth::mergeUsingResidual (texSynth_kernel.cu:1978-2001). */
 int pt_merge_using_residual(pt_ctx *ctx, int level, int forwards);
+/* TexSynth::mergeAdvectedBuffers (texSynth_kernel.cu:1938-2042): the forward
+ * and backward advected offsets merged, merge_fallback
+ * PT_MERGE_FALLBACK_RANDOM or PT_MERGE_FALLBACK_LAST_PASS where neither is
+ * valid. */
+int pt_merge_advected_buffers(pt_ctx *ctx, int level, int merge_fallback);
+/* TexSynth::accumulateRibbonField (texSynth_kernel.cu:1728-1854): the
+ * ribbon field advected one step of time_step frames along the uploaded
+ * velocities (overwrite: replace every texel; else keep the shorter ribbon),
+ * and the canvas texels the ribbons reach written to the forward advected
+ * output colours (clamped into the canvas). */
+int pt_accumulate_ribbon_field(pt_ctx *ctx, int level, int forwards, int time_step, int overwrite);
+/* The ribbon field (WorkingBuffers::ribbonField, level-0 sized, invalid
+ * after pt_initialize): level-rectangle copies in and out. */
+int pt_load_ribbon_field(pt_ctx *ctx, int level, const pt_ribbonp *ribbons);
+int pt_get_ribbon_field(pt_ctx *ctx, int level, pt_ribbonp *ribbons);
 
 /* WorkingBuffers::getOutput (workingBuffers.cpp:125-142): level-sized copies
  * of the base offsets, the canvas (RGBA) and the residual; any pointer may be
@@ -315,8 +382,9 @@ int pt_get_output(pt_ctx *ctx, int level, pt_patchxf *offsets, float *canvas_rgb
 int pt_get_output_noalloc(pt_ctx *ctx, int level, pt_patchxf *offsets, float *canvas_rgba, float *residual);
 /* WorkingBuffers::loadOffsets / loadResidual (workingBuffers.cpp:166-207):
  * writes the level rectangle of the base offsets / base residual.  Offsets
- * must have layer 0 and lum_a 0 (PT_ERR_UNSUPPORTED otherwise); a nonzero
- * lum_y (luminance shift) is kept and added to the canvas. */
+ * must have lum_a 0 (PT_ERR_UNSUPPORTED otherwise); a nonzero lum_y
+ * (luminance shift) is kept and added to the canvas; the style layer is
+ * kept. */
 int pt_load_offsets(pt_ctx *ctx, int level, const pt_patchxf *offsets);
 int pt_load_residual(pt_ctx *ctx, int level, const float *residual);
 /* Reads an advected buffer set (advectedBuffers.h) for tests: which = 0 for

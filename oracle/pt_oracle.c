@@ -47,8 +47,13 @@ typedef struct { float x, y; } f2;
 /* PatchXF restricted to one style layer and no luminance shift: layer is
  * always 0 and the shift always (0,0) on this path (multi-style and
  * luminance shift are rejected by setup). */
-/* PatchXF without layer: ly is luminanceShift.y (its .a is always 0 here) */
-typedef struct { float x, y, th, hy, su, sv, ly; } nnf;
+/* PatchXF: ly is luminanceShift.y (its .a is always 0 here), la the style
+ * layer (0 except with keyframe interpolation) */
+typedef struct { float x, y, th, hy, su, sv, ly; int la; } nnf;
+
+/* RibbonP (types.h:116-126): offset to a keyframe, time step, style layer
+ * (-1: invalid).  Packed in textures as Color4(time_step, y, x, layer). */
+typedef struct { float x, y; int32_t ts, layer; } ribbon;
 
 typedef struct {
     nnf *off;
@@ -98,11 +103,18 @@ struct ptref_ctx {
     f4 *random; int rw, rh;
     int32_t *img_id; int idw, idh; /* gImageId, a single 0 texel until uploaded */
     int lum_live; /* some luminance shift may be nonzero: the canvas adds it */
-    /* exemplar textures, one level (texSynth_kernel.cu:2119-2126) */
+    /* exemplar textures, one level, nl style layers of ew x eh texels each
+     * (texSynth_kernel.cu:2119-2126) */
     f4 *ex_base, *ex_out;
     float *ex_orient, *ex_dt;
     int32_t *ex_id;                 /* gExemplarObjectIDs */
-    int ew, eh;
+    int ew, eh, nl;
+    /* keyframe interpolation: cumulative key-to-key fields (layered, cl
+     * layers of cw x ch, linear filter), frame-to-key ribbons (level sized,
+     * one 0 texel until uploaded) and the ribbon field buffer pair */
+    f4 *cumul[2]; int cw, ch, cl;   /* [0] = gCumulFieldB, [1] = gCumulFieldF */
+    f4 *ftk[2]; int fw[2], fh[2];   /* [0] = gFrameToKeyRibbonB, [1] = ...F */
+    ribbon *rib_base, *rib_work;
     uint32_t scatter_calls;
     uint64_t pde;
     /* sequence state of ptref_synthesize_next_frame */
@@ -239,6 +251,19 @@ static inline f2 img_scale_at(const ptref_ctx *c, float x, float y)
     return tex2_lin(c->img_scale, c->scw, c->scw, c->sch, x, y);
 }
 
+/* Style layer `l` of the exemplar textures: a layered texture clamps the
+ * layer index into its range. */
+static inline size_t ex_layer(const ptref_ctx *c, int l)
+{
+    l = l < 0 ? 0 : (l >= c->nl ? c->nl - 1 : l);
+    return (size_t)l * c->ew * c->eh;
+}
+#define EXB(c, l) ((c)->ex_base + ex_layer((c), (l)))
+#define EXO(c, l) ((c)->ex_out + ex_layer((c), (l)))
+#define EXOR(c, l) ((c)->ex_orient + ex_layer((c), (l)))
+#define EXDT(c, l) ((c)->ex_dt + ex_layer((c), (l)))
+#define EXID(c, l) ((c)->ex_id + ex_layer((c), (l)))
+
 /* ------------------------------------------------------------------ */
 /* helpers (texSynth_util.h)                                           */
 /* ------------------------------------------------------------------ */
@@ -305,11 +330,12 @@ static inline void new_coord(float i, float j, float ex, float ey, float b0x, fl
     *oy = ey + cy;
 }
 
-/* screenToExemplarAngle (texSynth_kernel.cu:425-436) */
-static inline float screen_to_ex_angle(const ptref_ctx *c, float x, float y, float ex, float ey)
+/* screenToExemplarAngle (texSynth_kernel.cu:425-436), exemplar orientation
+ * of style layer `layer` */
+static inline float screen_to_ex_angle_l(const ptref_ctx *c, float x, float y, float ex, float ey, int layer)
 {
     if (c->p.orientation_enabled) {
-        float theta = tex1_point(c->ex_orient, c->ew, c->eh, ex, ey) -
+        float theta = tex1_point(EXOR(c, layer), c->ew, c->eh, ex, ey) -
                       tex1_point(c->img_orient, c->ow, c->oh, x, y);
         if ((double)theta > 1.57079632679489661923)
             return (float)((double)theta - 3.14159265358979323846);
@@ -318,6 +344,11 @@ static inline float screen_to_ex_angle(const ptref_ctx *c, float x, float y, flo
         return theta;
     }
     return 0.0f;
+}
+
+static inline float screen_to_ex_angle(const ptref_ctx *c, float x, float y, float ex, float ey)
+{
+    return screen_to_ex_angle_l(c, x, y, ex, ey, 0);
 }
 
 /* isImagePixelCulled (texSynth_kernel.cu:176-186) */
@@ -330,9 +361,9 @@ static inline int image_culled(const ptref_ctx *c, int x, int y)
 /* isExemplarPixelCulled (texSynth_kernel.cu:200-205): int parameters, so
  * the position is truncated toward zero and the texel there read exactly
  * (clamp addressing). */
-static inline int exemplar_culled(const ptref_ctx *c, float x, float y)
+static inline int exemplar_culled(const ptref_ctx *c, float x, float y, int layer)
 {
-    float d = c->ex_dt[clampi((int)y, 0, c->eh - 1) * c->ew + clampi((int)x, 0, c->ew - 1)];
+    float d = EXDT(c, layer)[clampi((int)y, 0, c->eh - 1) * c->ew + clampi((int)x, 0, c->ew - 1)];
     return (double)d > (double)c->p.dist_trans_cull * 1.25;
 }
 
@@ -374,8 +405,8 @@ static float time_terms(const ptref_ctx *c, const adv_t *a, nnf prev, float pb0x
     f4 prev_out = a->out[o];
     float ptx, pty;
     new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, &ptx, &pty);
-    f4 peb = tex4_lin(c->ex_base, c->ew, c->ew, c->eh, ptx, pty);
-    f4 peo = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, ptx, pty);
+    f4 peb = tex4_lin(EXB(c, prev.la), c->ew, c->ew, c->eh, ptx, pty);
+    f4 peo = tex4_lin(EXO(c, prev.la), c->ew, c->ew, c->eh, ptx, pty);
     if (c->p.colorspace == 1) { /* texSynth_kernel.cu:454-459 */
         prev_base = lab4(prev_base);
         prev_out = lab4(prev_out);
@@ -403,12 +434,12 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
     int bwd_dir = p->direction == PT_BACKWARD || p->direction == PT_BIDIRECTIONAL;
     float fb0x = 0, fb0y = 0, fb1x = 0, fb1y = 0, bb0x = 0, bb0y = 0, bb1x = 0, bb1y = 0;
     if (cpv) {
-        float ang = screen_to_ex_angle(c, (float)x, (float)y, pf.x, pf.y);
+        float ang = screen_to_ex_angle_l(c, (float)x, (float)y, pf.x, pf.y, pf.la);
         basis(ang, p->residual_window_scale_u * pf.su, p->residual_window_scale_v * pf.sv,
               &fb0x, &fb0y, &fb1x, &fb1y);
     }
     if (cnv) {
-        float ang = screen_to_ex_angle(c, (float)x, (float)y, pb.x, pb.y);
+        float ang = screen_to_ex_angle_l(c, (float)x, (float)y, pb.x, pb.y, pb.la);
         basis(ang, p->residual_window_scale_u * pb.su, p->residual_window_scale_v * pb.sv,
               &bb0x, &bb0y, &bb1x, &bb1y);
     }
@@ -437,8 +468,8 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
             else if (tx < 0) tx += (float)hw;
             if (ty >= (float)hh) ty -= (float)hh;
             else if (ty < 0) ty += (float)hh;
-            f4 tb = tex4_lin(c->ex_base, c->ew, c->ew, c->eh, tx, ty);
-            f4 to = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, tx, ty);
+            f4 tb = tex4_lin(EXB(c, t.la), c->ew, c->ew, c->eh, tx, ty);
+            f4 to = tex4_lin(EXO(c, t.la), c->ew, c->ew, c->eh, tx, ty);
             if (p->colorspace == 1) { /* texSynth_kernel.cu:603-606 */
                 tb = lab4(tb);
                 to = lab4(to);
@@ -451,7 +482,7 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
             if (i == 0 && j == 0) {
                 if (p->dist_trans_weight > 0.f) {
                     /* distTransformTerm (texSynth_kernel.cu:471-485) */
-                    float de = tex1_lin(c->ex_dt, c->ew, c->ew, c->eh, tx, ty);
+                    float de = tex1_lin(EXDT(c, t.la), c->ew, c->ew, c->eh, tx, ty);
                     de /= t.sv;
                     float di = img_dt_at(c, c->cur, x, y);
                     float diff = fabsf(de - di);
@@ -462,7 +493,7 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
                  * fetches, the image ID at the pixel, the style's at the tap */
                 if (p->style_obj_id_weight > 0.f) {
                     int iid = c->img_id[point_index(c->idw, c->idh, (float)x, (float)y)];
-                    int sid = c->ex_id[point_index(c->ew, c->eh, tx, ty)];
+                    int sid = EXID(c, t.la)[point_index(c->ew, c->eh, tx, ty)];
                     if (iid != sid) res = res + p->style_obj_id_weight;
                 }
             } else {
@@ -488,9 +519,9 @@ static float calc_residual(const ptref_ctx *c, int x, int y, int level, nnf t)
 /* test_offset (texSynth_kernel.cu:695-721) */
 static float test_offset(const ptref_ctx *c, int x, int y, nnf t, int level, float image_alpha)
 {
-    if (!exemplar_culled(c, t.x, t.y)) {
+    if (!exemplar_culled(c, t.x, t.y, t.la)) {
         if (!c->p.transparency_ok) {
-            float ea = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, t.x, t.y).w;
+            float ea = tex4_lin(EXO(c, t.la), c->ew, c->ew, c->eh, t.x, t.y).w;
             if (ea < 0.99f && image_alpha > 0.99f)
                 return 1e31f;
         }
@@ -572,6 +603,8 @@ static void free_buffers(ptref_ctx *c)
         c->img_color[k] = NULL; c->img_dt[k] = NULL; c->img_velb[k] = NULL; c->img_velf[k] = NULL;
     }
     c->off_base = c->off_work = NULL; c->res_base = c->res_work = NULL; c->canvas = NULL;
+    free(c->rib_base); free(c->rib_work);
+    c->rib_base = c->rib_work = NULL;
 }
 
 static void free_sequence(ptref_ctx *c);
@@ -583,6 +616,7 @@ int ptref_destroy(ptref_ctx *c)
     free_buffers(c);
     free(c->img_orient); free(c->img_scale); free(c->random); free(c->img_id);
     free(c->ex_base); free(c->ex_out); free(c->ex_orient); free(c->ex_dt); free(c->ex_id);
+    for (int k = 0; k < 2; k++) { free(c->cumul[k]); free(c->ftk[k]); }
     free(c);
     return PT_OK;
 }
@@ -599,8 +633,10 @@ static int check_supported(ptref_ctx *c, const pt_params *p)
         return fail(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
     if (p->colorspace != 0 && p->colorspace != 1)
         return fail(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
-    if (p->interpolate_key_frame)
-        return fail(c, PT_ERR_UNSUPPORTED, "keyframe interpolation is not supported");
+    if (p->interpolate_key_frame &&
+        (p->first_key_index < 0 || p->last_key_index < p->first_key_index || p->last_key_index >= PT_MAX_STYLE_LAYERS))
+        return fail(c, PT_ERR_INVALID_ARG, "keyframe interpolation needs 0 <= first_key_index <= last_key_index < %d",
+                    PT_MAX_STYLE_LAYERS);
     /* The offsets-histogram term (histogramTerm, texSynth_kernel.cu:486-494)
      * reads a histogram that stays all zero: its accumulation is commented out
      * (:258), the buffer is reset before it and the two blur passes of zeros
@@ -694,6 +730,19 @@ int ptref_initialize(ptref_ctx *c, int iw, int ih, int sw, int sh)
         if (!c->adv[k].ts || !c->img_velf[k]) return fail(c, PT_ERR_OUT_OF_MEMORY, "out of memory");
     }
     if (!c->canvas) return fail(c, PT_ERR_OUT_OF_MEMORY, "out of memory");
+    c->rib_base = (ribbon *)malloc(n * sizeof(ribbon));
+    c->rib_work = (ribbon *)malloc(n * sizeof(ribbon));
+    if (!c->rib_base || !c->rib_work) return fail(c, PT_ERR_OUT_OF_MEMORY, "out of memory");
+    for (size_t i = 0; i < n; i++) {
+        ribbon r0 = {0.f, 0.f, 0, -1}; /* RibbonP(): invalid */
+        c->rib_base[i] = r0;
+        c->rib_work[i] = r0;
+    }
+    for (int k = 0; k < 2; k++) { /* unbound frame-to-key ribbons read as 0 */
+        free(c->ftk[k]);
+        c->ftk[k] = (f4 *)calloc(1, sizeof(f4));
+        c->fw[k] = c->fh[k] = 1;
+    }
     c->lum_live = c->have_params && c->p.lum_shift_enable;
     free(c->img_id); /* a fresh image-ID texture: one 0 texel until uploaded */
     c->img_id = (int32_t *)calloc(1, sizeof(int32_t));
@@ -704,15 +753,30 @@ int ptref_initialize(ptref_ctx *c, int iw, int ih, int sw, int sh)
 
 int ptref_initialize_exemplars(ptref_ctx *c, int w, int h, int num_layers)
 {
-    if (!c || w < 1 || h < 1 || num_layers < 1 || num_layers > 4) return PT_ERR_INVALID_ARG;
-    if (c->ex_base && c->ew == w && c->eh == h) return PT_OK;
+    if (!c || w < 1 || h < 1 || num_layers < 1 || num_layers > PT_MAX_STYLE_LAYERS) return PT_ERR_INVALID_ARG;
+    if (c->ex_base && c->ew == w && c->eh == h && c->nl == num_layers) return PT_OK;
     free(c->ex_base); free(c->ex_out); free(c->ex_orient); free(c->ex_dt); free(c->ex_id);
-    c->ew = w; c->eh = h;
-    c->ex_id = (int32_t *)calloc((size_t)w * h, sizeof(int32_t));
-    c->ex_base = (f4 *)calloc((size_t)w * h, sizeof(f4));
-    c->ex_out = (f4 *)calloc((size_t)w * h, sizeof(f4));
-    c->ex_orient = (float *)calloc((size_t)w * h, sizeof(float));
-    c->ex_dt = (float *)calloc((size_t)w * h, sizeof(float));
+    c->ew = w; c->eh = h; c->nl = num_layers;
+    size_t n = (size_t)w * h * num_layers;
+    c->ex_id = (int32_t *)calloc(n, sizeof(int32_t));
+    c->ex_base = (f4 *)calloc(n, sizeof(f4));
+    c->ex_out = (f4 *)calloc(n, sizeof(f4));
+    c->ex_orient = (float *)calloc(n, sizeof(float));
+    c->ex_dt = (float *)calloc(n, sizeof(float));
+    return PT_OK;
+}
+
+/* TexSynth::initialize_cumulativeFields (texSynth_kernel.cu:2136-2140):
+ * zero-filled layered fields. */
+int ptref_initialize_cumulative_fields(ptref_ctx *c, int w, int h, int num_layers)
+{
+    if (!c || w < 1 || h < 1 || num_layers < 1 || num_layers > PT_MAX_STYLE_LAYERS) return PT_ERR_INVALID_ARG;
+    for (int k = 0; k < 2; k++) {
+        free(c->cumul[k]);
+        c->cumul[k] = (f4 *)calloc((size_t)w * h * num_layers, sizeof(f4));
+        if (!c->cumul[k]) return fail(c, PT_ERR_OUT_OF_MEMORY, "out of memory");
+    }
+    c->cw = w; c->ch = h; c->cl = num_layers;
     return PT_OK;
 }
 
@@ -782,6 +846,28 @@ int ptref_pyramid_create(ptref_ctx *c, ptref_pyramid **out, int elem, int w, int
     if (elem == PT_ELEM_I32) {
         for (int l = 1; l < lv; l++)
             downsample_vote((const int32_t *)p->data[l - 1], p->lw[l - 1], (int32_t *)p->data[l], p->lw[l], p->lh[l]);
+        *out = p;
+        return PT_OK;
+    }
+    if (scale_mode == PT_DOWNSAMPLE_RIBBON) {
+        /* initializeRibbonPyramid (imagePyramid.cpp:211-235): the top-left
+         * texel of each 2x2 block as RibbonP -- time step and layer through
+         * int, x and y halved -- packed back as Color4(time_step, y, x, layer) */
+        if (elem != PT_ELEM_F32X4) {
+            ptref_pyramid_destroy(p);
+            return fail(c, PT_ERR_INVALID_ARG, "ribbon pyramids are F32X4");
+        }
+        for (int l = 1; l < lv; l++) {
+            int lw = p->lw[l], lh = p->lh[l], pw = p->lw[l - 1];
+            const f4 *prev = (const f4 *)p->data[l - 1];
+            f4 *dst = (f4 *)p->data[l];
+            for (int y = 0; y < lh; y++)
+                for (int x = 0; x < lw; x++) {
+                    f4 q = prev[(y * 2) * pw + x * 2];
+                    f4 r = {(float)(int32_t)q.x, q.y * 0.5f, q.z * 0.5f, (float)(int32_t)q.w};
+                    dst[y * lw + x] = r;
+                }
+        }
         *out = p;
         return PT_OK;
     }
@@ -866,11 +952,8 @@ static float *realloc_tex(float *old, size_t n)
 int ptref_upload_image(ptref_ctx *c, int level, int layer, const ptref_pyramid *pyr, int style_layer)
 {
     if (!c || !pyr || level < 0 || level >= pyr->levels) return fail(c, PT_ERR_INVALID_ARG, "bad upload");
-    if (style_layer < 0 || style_layer > 3) return fail(c, PT_ERR_INVALID_ARG, "style layer must be in [0, 3]");
-    /* style layers above 0 are never sampled without keyframe interpolation
-     * (every offset keeps layer 0: synthesisFrame.cpp:106-110,
-     * texSynth_kernel.cu:925,939) */
-    if (style_layer > 0 && layer >= PT_LAYER_EXEMPLAR_BASE && layer <= PT_LAYER_EXEMPLAR_OBJECT_IDS) return PT_OK;
+    if (style_layer < 0 || style_layer >= PT_MAX_STYLE_LAYERS)
+        return fail(c, PT_ERR_INVALID_ARG, "style layer must be in [0, %d]", PT_MAX_STYLE_LAYERS - 1);
     int w = pyr->lw[level], h = pyr->lh[level];
     const float *src = pyr->data[level];
     int prev = (c->cur + 1) % 2;
@@ -919,19 +1002,35 @@ int ptref_upload_image(ptref_ctx *c, int level, int layer, const ptref_pyramid *
         c->idw = w; c->idh = h;
         return PT_OK;
     case PT_LAYER_EXEMPLAR_OBJECT_IDS:
-        if (pyr->elem != PT_ELEM_I32 || !c->ex_id || w > c->ew || h > c->eh) break;
-        copy_rect((float *)c->ex_id, c->ew, src, w, h, 1);
+        if (pyr->elem != PT_ELEM_I32 || !c->ex_id || w > c->ew || h > c->eh || style_layer >= c->nl) break;
+        copy_rect((float *)EXID(c, style_layer), c->ew, src, w, h, 1);
         return PT_OK;
+    case PT_LAYER_CUMUL_FIELD_B:
+    case PT_LAYER_CUMUL_FIELD_F: {
+        int k = layer == PT_LAYER_CUMUL_FIELD_B ? 0 : 1;
+        if (pyr->elem != 4 || !c->cumul[k] || w > c->cw || h > c->ch || style_layer >= c->cl) break;
+        copy_rect((float *)(c->cumul[k] + (size_t)style_layer * c->cw * c->ch), c->cw, src, w, h, 4);
+        return PT_OK;
+    }
+    case PT_LAYER_FRAME_TO_KEY_RIBBON_B:
+    case PT_LAYER_FRAME_TO_KEY_RIBBON_F: {
+        int k = layer == PT_LAYER_FRAME_TO_KEY_RIBBON_B ? 0 : 1;
+        if (pyr->elem != 4) break;
+        c->ftk[k] = (f4 *)realloc_tex((float *)c->ftk[k], (size_t)w * h * 4);
+        memcpy(c->ftk[k], src, sizeof(float) * 4 * w * h);
+        c->fw[k] = w; c->fh[k] = h;
+        return PT_OK;
+    }
     case PT_LAYER_EXEMPLAR_BASE:
     case PT_LAYER_EXEMPLAR_OUTPUT:
     case PT_LAYER_EXEMPLAR_ORIENTATION:
     case PT_LAYER_EXEMPLAR_DIST_TRANS: {
-        if (!c->ex_base || w > c->ew || h > c->eh) break;
+        if (!c->ex_base || w > c->ew || h > c->eh || style_layer >= c->nl) break;
         int ch = (layer == PT_LAYER_EXEMPLAR_BASE || layer == PT_LAYER_EXEMPLAR_OUTPUT) ? 4 : 1;
         if (pyr->elem != ch) break;
-        float *dst = layer == PT_LAYER_EXEMPLAR_BASE ? (float *)c->ex_base
-                     : layer == PT_LAYER_EXEMPLAR_OUTPUT ? (float *)c->ex_out
-                     : layer == PT_LAYER_EXEMPLAR_ORIENTATION ? c->ex_orient : c->ex_dt;
+        float *dst = layer == PT_LAYER_EXEMPLAR_BASE ? (float *)EXB(c, style_layer)
+                     : layer == PT_LAYER_EXEMPLAR_OUTPUT ? (float *)EXO(c, style_layer)
+                     : layer == PT_LAYER_EXEMPLAR_ORIENTATION ? EXOR(c, style_layer) : EXDT(c, style_layer);
         copy_rect(dst, c->ew, src, w, h, ch);
         return PT_OK;
     }
@@ -943,22 +1042,25 @@ int ptref_upload_image(ptref_ctx *c, int level, int layer, const ptref_pyramid *
 
 /* SynthesisFrame::createRandomPyramid (synthesisFrame.cpp:66-150), with
  * Philox4x32-10 draws in place of glibc rand(): key (seed, frame*level),
- * counter (x, y, trial, 0) for trial 0..49 and (x, y, 50, 0) for the
- * fractional parts. */
-int ptref_create_random_pyramid(ptref_ctx *c, int level, int frame, const ptref_pyramid *icol,
-                                const ptref_pyramid *iori, const ptref_pyramid *eout,
-                                const ptref_pyramid *eori)
+ * counter (x, y, trial, 0) for trial 0..49, (x, y, 50, 0) for the
+ * fractional parts and, with keys, (x, y, 51, 0) for the style
+ * (thisStyle = rand() % keys + first key, :107-110). */
+static int random_pyramid(ptref_ctx *c, int level, int frame, const ptref_pyramid *icol, const ptref_pyramid *iori,
+                          int ns, const ptref_pyramid *const *eouts, const ptref_pyramid *const *eoris, int keys)
 {
-    if (!c || !icol || !iori || !eout || !eori) return PT_ERR_INVALID_ARG;
-    if (level >= icol->levels || level >= iori->levels || level >= eout->levels || level >= eori->levels)
-        return fail(c, PT_ERR_INVALID_ARG, "level out of range");
+    if (!c || !icol || !iori || ns < 1 || !eouts || !eoris) return PT_ERR_INVALID_ARG;
+    if (level >= icol->levels || level >= iori->levels) return fail(c, PT_ERR_INVALID_ARG, "level out of range");
+    for (int k = 0; k < ns; k++) {
+        if (!eouts[k] || !eoris[k] || level >= eouts[k]->levels || level >= eoris[k]->levels)
+            return fail(c, PT_ERR_INVALID_ARG, "style %d: level out of range", k);
+        if (eouts[k]->lw[level] < 2 || eouts[k]->lh[level] < 2)
+            return fail(c, PT_ERR_INVALID_ARG, "exemplar level too small");
+    }
+    int nk = c->p.last_key_index - c->p.first_key_index + 1, k0 = c->p.first_key_index;
+    if (keys && (nk < 1 || k0 < 0 || k0 + nk > ns)) return fail(c, PT_ERR_INVALID_ARG, "keys outside the styles");
     int w = icol->lw[level], h = icol->lh[level];
-    int ew = eout->lw[level], eh = eout->lh[level];
-    if (ew < 2 || eh < 2) return fail(c, PT_ERR_INVALID_ARG, "exemplar level too small");
     const f4 *col = (const f4 *)icol->data[level];
     const float *ior = iori->data[level];
-    const f4 *eo = (const f4 *)eout->data[level];
-    const float *eor = eori->data[level];
     c->random = (f4 *)realloc_tex((float *)c->random, (size_t)w * h * 4);
     c->rw = w; c->rh = h;
     uint32_t key[2] = {c->p.seed, (uint32_t)(frame * level)};
@@ -968,6 +1070,15 @@ int ptref_create_random_pyramid(ptref_ctx *c, int level, int frame, const ptref_
     for (int y = 0; y < h; y++) {
         for (int x = 0; x < w; x++) {
             uint32_t ctr[4] = {(uint32_t)x, (uint32_t)y, 0u, 0u}, o[4];
+            int st = 0;
+            if (keys) {
+                ctr[2] = 51u;
+                pt_philox4x32_10(ctr, key, o);
+                st = pt_rand31(o[0]) % nk + k0;
+            }
+            int ew = eouts[st]->lw[level], eh = eouts[st]->lh[level];
+            const f4 *eo = (const f4 *)eouts[st]->data[level];
+            const float *eor = eoris[st]->data[level];
             int ox, oy, trial = 0;
             do {
                 ctr[2] = (uint32_t)trial;
@@ -981,11 +1092,25 @@ int ptref_create_random_pyramid(ptref_ctx *c, int level, int frame, const ptref_
             pt_philox4x32_10(ctr, key, o);
             float rx = pt_rand_unit(o[0]);
             float ry = pt_rand_unit(o[1]);
-            f4 v = {(float)ox + rx, (float)oy + ry, orientation, 0.0f};
+            f4 v = {(float)ox + rx, (float)oy + ry, orientation, (float)st};
             c->random[y * w + x] = v;
         }
     }
     return PT_OK;
+}
+
+int ptref_create_random_pyramid(ptref_ctx *c, int level, int frame, const ptref_pyramid *icol,
+                                const ptref_pyramid *iori, const ptref_pyramid *eout,
+                                const ptref_pyramid *eori)
+{
+    return random_pyramid(c, level, frame, icol, iori, 1, &eout, &eori, 0);
+}
+
+int ptref_create_random_pyramid_keys(ptref_ctx *c, int level, int frame, const ptref_pyramid *icol,
+                                     const ptref_pyramid *iori, int num_styles,
+                                     const ptref_pyramid *const *eouts, const ptref_pyramid *const *eoris)
+{
+    return random_pyramid(c, level, frame, icol, iori, num_styles, eouts, eoris, 1);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1027,7 +1152,7 @@ static void offsets_to_output(ptref_ctx *c, int level)
     for (int y = 0; y < lh; y++)
         for (int x = 0; x < lw; x++) {
             nnf n = c->off_base[y * c->W + x];
-            f4 v = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, n.x, n.y);
+            f4 v = tex4_lin(EXO(c, n.la), c->ew, c->ew, c->eh, n.x, n.y);
             if (c->lum_live) { /* toColor4(output, luminanceShift) (texSynth_kernel.cu:272) */
                 v.x = v.x + n.ly;
                 v.y = v.y + n.ly;
@@ -1061,8 +1186,9 @@ int ptref_randomize_offsets(ptref_ctx *c, int level)
     int lw = c->W >> level, lh = c->H >> level;
     for (int y = 0; y < lh; y++)
         for (int x = 0; x < lw; x++) {
+            /* toPatchXF(random): the style layer is the texel's w (texSynth_util.h:212-214) */
             f4 r = tex4_point(c->random, c->rw, c->rh, (float)x, (float)y);
-            nnf n = {r.x, r.y, r.z, 0.f, 1.f, 1.f, 0.f};
+            nnf n = {r.x, r.y, r.z, 0.f, 1.f, 1.f, 0.f, (int)r.w};
             c->off_work[y * c->W + x] = n;
         }
     return ptref_sync_buffers(c, level, 0);
@@ -1076,7 +1202,8 @@ static inline nnf pixel_wrap(const ptref_ctx *c, const nnf *buf, int x, int y)
     return buf[wy * c->W + wx];
 }
 
-/* calculatePropagateOffset (texSynth_kernel.cu:956-982) */
+/* calculatePropagateOffset (texSynth_kernel.cu:956-982): the neighbour's
+ * layer is kept */
 static nnf propagate_offset(const ptref_ctx *c, int tx, int ty, float x, float y)
 {
     nnf t = pixel_wrap(c, c->off_base, tx, ty);
@@ -1086,27 +1213,98 @@ static nnf propagate_offset(const ptref_ctx *c, int tx, int ty, float x, float y
     float ex, ey;
     new_coord(x - (float)tx, y - (float)ty, t.x, t.y, b0x, b0y, b1x, b1y, &ex, &ey);
     clamp_ex(&ex, &ey, c->ew, c->eh);
-    float ang = screen_to_ex_angle(c, x, y, ex, ey);
+    float ang = screen_to_ex_angle_l(c, x, y, ex, ey, t.la);
     f2 s2 = img_scale_at(c, x, y);
-    nnf r = {ex, ey, ang, 0.f, s2.x, s2.y, 0.f};
+    nnf r = {ex, ey, ang, 0.f, s2.x, s2.y, 0.f, t.la};
     return r;
 }
 
-/* propagate_kernel (texSynth_kernel.cu:1112-1219) + TexSynth::propagate (:1306-1357) */
+/* The keymask of a keyframe (texSynth_kernel.cu:1137-1142, 887-892): the
+ * exemplar output of style layer key_index at the pixel's own position
+ * (integer fetch: the texel itself, clamped); alpha > 0 marks a painted
+ * keyframe pixel. */
+static inline int keymask(const ptref_ctx *c, int key_index, int x, int y)
+{
+    if (key_index < 0) return 0;
+    f4 m = EXO(c, key_index)[clampi(y, 0, c->eh - 1) * c->ew + clampi(x, 0, c->ew - 1)];
+    return m.w > 0.f;
+}
+
+/* A cumulative key-to-key field texel at exemplar (ex, ey) of field k
+ * (0 = gCumulFieldB, 1 = gCumulFieldF), layer `layer`: bilinear, clamped in
+ * x, y and layer (unbound fields read 0). */
+static inline f4 cumul_at(const ptref_ctx *c, int k, float ex, float ey, int layer)
+{
+    if (!c->cumul[k]) {
+        f4 z = {0.f, 0.f, 0.f, 0.f};
+        return z;
+    }
+    int l = layer < 0 ? 0 : (layer >= c->cl ? c->cl - 1 : layer);
+    return tex4_lin(c->cumul[k] + (size_t)l * c->cw * c->ch, c->cw, c->cw, c->ch, ex, ey);
+}
+
+/* toRibbonP (texSynth_util.h:216-218): Color4(time_step, y, x, layer) */
+static inline ribbon to_ribbon(f4 v)
+{
+    ribbon r = {v.z, v.y, (int32_t)v.x, (int32_t)v.w};
+    return r;
+}
+
+/* The candidates of propagate_kernel_keys / scatter_kernel_keys that follow
+ * candidate (ex, ey) of layer `elayer` through the cumulative fields to the
+ * earlier keys (field B, layers elayer-1 down to firstKeyIndex) and to the
+ * later keys (field F, layers elayer+1 up to lastKeyIndex), each tested in
+ * turn (texSynth_kernel.cu:1061-1107, 792-834).  Propagation stops a walk
+ * on a texel whose time-step channel is below -EPSILON, scatter on an
+ * invalid ribbon (layer < 0). */
+static void key_walks(const ptref_ctx *c, int x, int y, float ex0, float ey0, int elayer, int level,
+                      float image_alpha, int scatter_rule, nnf *ow, float *best)
+{
+    for (int dir = 0; dir < 2; dir++) {
+        float ex = ex0, ey = ey0;
+        int j = dir == 0 ? elayer - 1 : elayer + 1;
+        for (; dir == 0 ? j >= c->p.first_key_index : j <= c->p.last_key_index; j += dir == 0 ? -1 : 1) {
+            f4 m = cumul_at(c, dir, ex, ey, dir == 0 ? j + 1 : j - 1);
+            if (scatter_rule) {
+                ribbon rb = to_ribbon(m);
+                if (rb.layer < 0) break;
+                ex += rb.x;
+                ey += rb.y;
+            } else {
+                if (m.x < -0.000001f) break;
+                ex += m.z;
+                ey += m.y;
+            }
+            float ang = screen_to_ex_angle_l(c, (float)x, (float)y, ex, ey, j);
+            nnf t = {ex, ey, ang, 0.f, 1.f, 1.f, 0.f, j};
+            float r = test_offset(c, x, y, t, level, image_alpha);
+            if (r < *best) { *ow = t; *best = r; }
+        }
+    }
+}
+
+/* propagate_kernel and propagate_kernel_keys (texSynth_kernel.cu:984-1219)
+ * + TexSynth::propagate (:1306-1357) */
 int ptref_propagate(ptref_ctx *c, int level, int jump, int key_index)
 {
     int rc = need_level(c, level);
     if (rc) return rc;
-    if (key_index >= 0) return fail(c, PT_ERR_UNSUPPORTED, "keyframes are not supported");
     int lw = c->W >> level, lh = c->H >> level;
+    const int keys = c->p.interpolate_key_frame != 0;
+    uint64_t walks = 0;
     ensure_threads(c);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) reduction(+ : walks)
     for (int y = 0; y < lh; y++)
         for (int x = 0; x < lw; x++) {
             int o = y * c->W + x;
             if (image_culled(c, x, y)) {
-                nnf z = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0.f};
+                nnf z = {0.f, 0.f, 0.f, 0.f, 1.f, 1.f, 0.f, 0};
                 c->off_work[o] = z;
+                continue;
+            }
+            if (keymask(c, key_index, x, y)) {
+                nnf k = {(float)x, (float)y, 0.f, 1.f, 1.f, 1.f, 0.f, key_index};
+                c->off_work[o] = k;
                 continue;
             }
             float best = c->res_base[o];
@@ -1121,24 +1319,32 @@ int ptref_propagate(ptref_ctx *c, int level, int jump, int key_index)
                 nnf cand = propagate_offset(c, tx, ty, (float)x, (float)y);
                 float r = test_offset(c, x, y, cand, level, ia);
                 if (r < best) { ow = cand; best = r; }
+                if (keys) {
+                    key_walks(c, x, y, cand.x, cand.y, cand.la, level, ia, 0, &ow, &best);
+                    walks++;
+                }
             }
-            /* The two advected candidates (:1184-1216) are built with
-             * hysteresis 0 by calculatePropagateOffset and so never tested. */
+            /* The two advected candidates of propagate_kernel (:1184-1216) are
+             * built with hysteresis 0 by calculatePropagateOffset and so never
+             * tested; propagate_kernel_keys has none. */
             c->off_work[o] = ow;
         }
+    (void)walks;
     c->pde += (uint64_t)lw * lh * 4;
     return ptref_sync_buffers(c, level, 0);
 }
 
-/* averageOffset (texSynth_kernel.cu:841-862) */
-static nnf average_offset(const ptref_ctx *c, int x, int y, int lw, int lh)
+/* averageOffset (texSynth_kernel.cu:841-862): the mean position of the
+ * in-bounds 8-neighbours of style layer `layer` */
+static nnf average_offset(const ptref_ctx *c, int x, int y, int lw, int lh, int layer)
 {
-    nnf a = {0.f, 0.f, 0.f, 1.f, 1.f, 1.f, 0.f};
+    nnf a = {0.f, 0.f, 0.f, 1.f, 1.f, 1.f, 0.f, layer};
     int sum = 0;
     for (int i = -1; i <= 1; i++)
         for (int j = -1; j <= 1; j++) {
             if (!in_bounds(x + i, y + j, lw, lh, 0) || (i == 0 && j == 0)) continue;
             nnf n = c->off_base[(y + j) * c->W + (x + i)];
+            if (n.la != layer) continue;
             a.x += n.x;
             a.y += n.y;
             sum++;
@@ -1150,15 +1356,21 @@ static nnf average_offset(const ptref_ctx *c, int x, int y, int lw, int lh)
 
 static inline float u24(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
 
-/* scatter_kernel (texSynth_kernel.cu:865-952) + TexSynth::scatter (:1360-1401) */
+/* scatter_kernel and scatter_kernel_keys (texSynth_kernel.cu:723-839,
+ * 865-952) + TexSynth::scatter (:1360-1401) */
 int ptref_scatter(ptref_ctx *c, int level, int key_index)
 {
     int rc = need_level(c, level);
     if (rc) return rc;
-    if (key_index >= 0) return fail(c, PT_ERR_UNSUPPORTED, "keyframes are not supported");
     int lw = c->W >> level, lh = c->H >> level;
     int ew = c->ew, eh = c->eh;
+    const int keys = c->p.interpolate_key_frame != 0;
     int samples = c->p.scatter_samples_count;
+    if (keys) {
+        int nk = c->p.last_key_index - c->p.first_key_index + 1;
+        if (nk < 1) return fail(c, PT_ERR_INVALID_ARG, "last_key_index below first_key_index");
+        samples = samples / nk; /* the samples are shared by the keys (:765) */
+    }
     uint32_t call = c->scatter_calls++;
     uint32_t key[2] = {c->p.seed ^ 0x5CA77E12u, (uint32_t)(c->p.frame_number * level)};
     int counter_rng = c->p.rng_mode == PT_RNG_COUNTER;
@@ -1169,6 +1381,10 @@ int ptref_scatter(ptref_ctx *c, int level, int key_index)
             int o = y * c->W + x;
             if (image_culled(c, x, y)) continue;
             nnf off = c->off_base[o];
+            if (keymask(c, key_index, x, y)) { /* keep the current offset */
+                c->off_work[o] = off;
+                continue;
+            }
             float cached = c->res_base[o];
             float best = (float)((double)cached + 10e-7);
             nnf ow = off;
@@ -1191,16 +1407,28 @@ int ptref_scatter(ptref_ctx *c, int level, int key_index)
                 float ex = ow.x + co * len;
                 float ey = ow.y + s * len;
                 clamp_ex(&ex, &ey, ew, eh);
-                float ang = screen_to_ex_angle(c, (float)x, (float)y, ex, ey);
-                nnf cand = {ex, ey, ang, 0.f, sc.x, sc.y, 0.f};
-                float res = test_offset(c, x, y, cand, level, ia);
-                if (res < best) { ow = cand; best = res; }
+                if (keys) {
+                    /* the sample stays in the winner's layer, scale (1, 1) */
+                    int el = ow.la;
+                    float ang = screen_to_ex_angle_l(c, (float)x, (float)y, ex, ey, el);
+                    nnf cand = {ex, ey, ang, 0.f, 1.f, 1.f, 0.f, el};
+                    float res = test_offset(c, x, y, cand, level, ia);
+                    if (res < best) { ow = cand; best = res; }
+                    key_walks(c, x, y, ex, ey, el, level, ia, 1, &ow, &best);
+                } else {
+                    float ang = screen_to_ex_angle(c, (float)x, (float)y, ex, ey);
+                    nnf cand = {ex, ey, ang, 0.f, sc.x, sc.y, 0.f, 0};
+                    float res = test_offset(c, x, y, cand, level, ia);
+                    if (res < best) { ow = cand; best = res; }
+                }
                 dist *= factor;
             }
-            nnf avg = average_offset(c, x, y, lw, lh);
-            avg.th = screen_to_ex_angle(c, (float)x, (float)y, avg.x, avg.y);
-            float res = test_offset(c, x, y, avg, level, ia);
-            if (res < best) { ow = avg; best = res; }
+            if (!keys) { /* the averaged-neighbour candidate of scatter_kernel (:938-947) */
+                nnf avg = average_offset(c, x, y, lw, lh, 0);
+                avg.th = screen_to_ex_angle(c, (float)x, (float)y, avg.x, avg.y);
+                float res = test_offset(c, x, y, avg, level, ia);
+                if (res < best) { ow = avg; best = res; }
+            }
             if (best < cached) c->off_work[o] = ow;
         }
     {
@@ -1208,7 +1436,7 @@ int ptref_scatter(ptref_ctx *c, int level, int key_index)
         float dist = (float)(ew > eh ? ew : eh);
         int n = 0;
         for (int i = 0; (i < samples) && (dist * 0.9f >= 10.f); i++) { n++; dist *= 0.9f; }
-        c->pde += (uint64_t)lw * lh * (uint64_t)(n + 1);
+        c->pde += (uint64_t)lw * lh * (uint64_t)(n + (keys ? 0 : 1));
     }
     return ptref_sync_buffers(c, level, 0);
 }
@@ -1257,9 +1485,9 @@ int ptref_upsample(ptref_ctx *c, int fine_level)
             }
 #undef D2
             float rw = c->res_base[lcy * c->W + lcx];
-            float ang = screen_to_ex_angle(c, (float)x, (float)y, ux, uy);
+            float ang = screen_to_ex_angle_l(c, (float)x, (float)y, ux, uy, o00.la);
             f2 sc = img_scale_at(c, (float)x, (float)y);
-            nnf n = {ux, uy, ang, 0.f, sc.x, sc.y, 0.f};
+            nnf n = {ux, uy, ang, 0.f, sc.x, sc.y, 0.f, o00.la};
             c->off_work[y * c->W + x] = n;
             c->res_work[y * c->W + x] = rw;
         }
@@ -1290,6 +1518,14 @@ static int translated_position(const ptref_ctx *c, int fwd, int x, int y, float 
     float dist = sqrtf((forward.x - back.x) * (forward.x - back.x) + (forward.y - back.y) * (forward.y - back.y));
     return in_bounds((int)ceilf(*tx), (int)ceilf(*ty), lw, lh, 0) &&
            in_bounds((int)floorf(*tx), (int)floorf(*ty), lw, lh, 0) && (dist < valid_dist);
+}
+
+/* A frame-to-key ribbon texel (k = 0: gFrameToKeyRibbonB, 1: ...F) at the
+ * integer pixel (x, y): the texel itself, clamped (one 0 texel, a valid
+ * layer-0 ribbon, until uploaded). */
+static inline ribbon ftk_at(const ptref_ctx *c, int k, int x, int y)
+{
+    return to_ribbon(c->ftk[k][clampi(y, 0, c->fh[k] - 1) * c->fw[k] + clampi(x, 0, c->fw[k] - 1)]);
 }
 
 /* advectKernel (texSynth_kernel.cu:1577-1693) + TexSynth::advectOffsets (:1695-1726) */
@@ -1333,6 +1569,7 @@ int ptref_advect_offsets(ptref_ctx *c, int level, int time_step, int forwards, i
                 out.x = ix;
                 out.y = iy;
                 out.th = nr.th;
+                out.la = nr.la;
                 out.hy = 1.f;
                 /* luminanceShift = nearest's + (color - prev_color).toLuminance(), .a = 0 (:1626-1631) */
                 if (c->p.lum_shift_enable) {
@@ -1340,11 +1577,26 @@ int ptref_advect_offsets(ptref_ctx *c, int level, int time_step, int forwards, i
                     out.ly = nr.ly + pt_luminance_delta(col.x, col.y, col.z, prev_color.x, prev_color.y, prev_color.z);
                 }
                 d->base[o] = prev_color;
-                d->out[o] = tex4_lin(c->ex_out, c->ew, c->ew, c->eh, out.x, out.y);
+                d->out[o] = tex4_lin(EXO(c, out.la), c->ew, c->ew, c->eh, out.x, out.y);
                 d->ts[o] = time_step;
             }
+            /* ADVECT_CHECK_FRAME_TO_KEY (:1643-1668): a long step, or a failed
+             * advection, may take the pixel's ribbon to a keyframe instead */
+            if ((flags & PT_ADVECT_CHECK_FRAME_TO_KEY) && (time_step > 1 || out.hy == 0.f)) {
+                ribbon fk = ftk_at(c, forwards ? 0 : 1, x, y);
+                if (fk.layer >= 0 && (abs(fk.ts) < time_step || out.hy == 0.f)) {
+                    out.x = fk.x + (float)x;
+                    out.y = fk.y + (float)y;
+                    out.la = fk.layer;
+                    out.hy = 1.f;
+                    out.ly = 0.f;
+                }
+                d->base[o] = tex4_lin(EXB(c, out.la), c->ew, c->ew, c->eh, out.x, out.y);
+                d->out[o] = tex4_lin(EXO(c, out.la), c->ew, c->ew, c->eh, out.x, out.y);
+                d->ts[o] = abs(fk.ts);
+            }
             if (out.hy == 0.f) {
-                nnf rn = {rnd.x, rnd.y, rnd.z, 0.f, 1.f, 1.f, 0.f};
+                nnf rn = {rnd.x, rnd.y, rnd.z, 0.f, 1.f, 1.f, 0.f, (int)rnd.w};
                 f4 z = {0, 0, 0, 0};
                 out = rn;
                 d->base[o] = z;
@@ -1354,7 +1606,7 @@ int ptref_advect_offsets(ptref_ctx *c, int level, int time_step, int forwards, i
             float jit = c->p.advection_jitter;
             out.x = out.x * (1 - jit) + frac.x * (jit);
             out.y = out.y * (1 - jit) + frac.y * (jit);
-            out.th = screen_to_ex_angle(c, (float)x, (float)y, out.x, out.y);
+            out.th = screen_to_ex_angle_l(c, (float)x, (float)y, out.x, out.y, out.la);
             d->off[o] = out;
         }
     size_t n = (size_t)c->W * c->H;
@@ -1370,8 +1622,9 @@ int ptref_advect_offsets(ptref_ctx *c, int level, int time_step, int forwards, i
 }
 
 /* mergeRandomlyKernel (texSynth_kernel.cu:1891-1936) + TexSynth::mergeRandomly (:2003-2024).
- * Without keyframes both ribbon textures read as zero, so both ribbons are
- * "valid" (layer 0 >= 0) and the ribbon clause is false. */
+ * The advected offset wins where its direction's frame-to-key ribbon is
+ * valid and the other one is not; unbound ribbon textures read as zero,
+ * "valid" (layer 0), which leaves that clause false. */
 int ptref_merge_randomly(ptref_ctx *c, int level, int forwards, float weight, int force_valid)
 {
     int rc = need_level(c, level);
@@ -1386,7 +1639,9 @@ int ptref_merge_randomly(ptref_ctx *c, int level, int forwards, float weight, in
             int av = a.hy > 0;
             f4 rnd = tex4_point(c->random, c->rw, c->rh, (float)x, (float)y);
             float r = forwards ? rnd.x - floorf(rnd.x) : rnd.y - floorf(rnd.y);
-            c->off_work[o] = ((av || force_valid) && r < weight) ? a : u;
+            ribbon ar = ftk_at(c, forwards ? 0 : 1, x, y), ur = ftk_at(c, forwards ? 1 : 0, x, y);
+            int rib = ar.layer >= 0 && !(ur.layer >= 0);
+            c->off_work[o] = (rib || ((av || force_valid) && r < weight)) ? a : u;
         }
     return ptref_sync_buffers(c, level, 0);
 }
@@ -1409,13 +1664,176 @@ int ptref_merge_using_residual(ptref_ctx *c, int level, int forwards)
     return ptref_sync_buffers(c, level, 0);
 }
 
+/* mergeAdvectedKernel (texSynth_kernel.cu:1938-1976) + TexSynth::
+ * mergeAdvectedBuffers (:2026-2042): both valid advected offsets -> one of
+ * them by the fractional random x, one valid -> it, none -> the fractional
+ * random texel as an offset (MERGE_FALLBACK_RANDOM) or the current base. */
+int ptref_merge_advected_buffers(ptref_ctx *c, int level, int fallback)
+{
+    int rc = need_level(c, level);
+    if (rc) return rc;
+    if (!c->random) return fail(c, PT_ERR_STATE, "no random texture");
+    int lw = c->W >> level, lh = c->H >> level;
+    for (int y = 0; y < lh; y++)
+        for (int x = 0; x < lw; x++) {
+            int o = y * c->W + x;
+            nnf f = c->adv[0].off[o], b = c->adv[1].off[o], r;
+            f4 rnd = tex4_point(c->random, c->rw, c->rh, (float)x, (float)y);
+            /* randomFloat4's out parameter: the fractional parts (:207-213) */
+            f4 fr = {rnd.x - floorf(rnd.x), rnd.y - floorf(rnd.y), rnd.z - floorf(rnd.z), rnd.w - floorf(rnd.w)};
+            if (f.hy > 0 && b.hy > 0) r = (fr.x < 0.5f) ? f : b;
+            else if (f.hy > 0) r = f;
+            else if (b.hy > 0) r = b;
+            else if (fallback == PT_MERGE_FALLBACK_RANDOM) {
+                nnf t = {fr.x, fr.y, fr.z, 0.f, 1.f, 1.f, 0.f, (int)fr.w};
+                r = t;
+            } else r = c->off_base[o];
+            c->off_work[o] = r;
+        }
+    return ptref_sync_buffers(c, level, 0);
+}
+
+/* copyFrameToKeyRibbonKernel (texSynth_kernel.cu:2061-2097) + TexSynth::
+ * offsetsFromFrameToKeyRibbon (:2111-2118): each pixel starts at the
+ * keyframe texel its ribbon reaches (hysteresis 1), or at its random
+ * texel where the ribbon is invalid. */
+int ptref_offsets_from_frame_to_key_ribbon(ptref_ctx *c, int level, int look_forwards)
+{
+    int rc = need_level(c, level);
+    if (rc) return rc;
+    if (!c->random) return fail(c, PT_ERR_STATE, "no random texture");
+    int lw = c->W >> level, lh = c->H >> level;
+    for (int y = 0; y < lh; y++)
+        for (int x = 0; x < lw; x++) {
+            ribbon rb = ftk_at(c, look_forwards ? 1 : 0, x, y);
+            nnf out;
+            if (rb.layer >= 0) {
+                nnf t = {rb.x + (float)x, rb.y + (float)y, 0.f, 1.f, 1.f, 1.f, 0.f, rb.layer};
+                t.th = screen_to_ex_angle_l(c, (float)x, (float)y, t.x, t.y, t.la);
+                out = t;
+            } else {
+                f4 rnd = tex4_point(c->random, c->rw, c->rh, (float)x, (float)y);
+                nnf t = {rnd.x, rnd.y, rnd.z, 0.f, 1.f, 1.f, 0.f, (int)rnd.w};
+                out = t;
+            }
+            c->off_work[y * c->W + x] = out;
+        }
+    return ptref_sync_buffers(c, level, 0);
+}
+
+/* accumulateAdvectionKernel (texSynth_kernel.cu:1728-1826) and
+ * TexSynth::accumulateRibbonField (:1828-1854): the ribbon field (base)
+ * advected one step of `time_step` frames along the image velocities into
+ * the working buffer, which is then copied to the base whole;
+ * cumulativeAdvectionFieldToOutput (:290-305) then writes the canvas texel
+ * each valid ribbon reaches into the forward advected output colours (the
+ * reference reads that texel unchecked; here it is clamped into the
+ * level-0-sized canvas). */
+int ptref_accumulate_ribbon_field(ptref_ctx *c, int level, int forwards, int time_step, int overwrite)
+{
+    if (!c || !c->off_base) return fail(c, PT_ERR_STATE, "not initialized");
+    if (level < 0 || (c->W >> level) < 1 || (c->H >> level) < 1) return fail(c, PT_ERR_INVALID_ARG, "bad level");
+    int lw = c->W >> level, lh = c->H >> level;
+    const ribbon *src = c->rib_base;
+    ribbon *dst = c->rib_work;
+    ensure_threads(c);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < lh; y++)
+        for (int x = 0; x < lw; x++) {
+            int o = y * c->W + x;
+            ribbon out = {0.f, 0.f, 0, -1};
+            f4 col = img_color_at(c, c->cur, x, y);
+            float tx, ty;
+            if (col.w == 0.f || !translated_position(c, forwards, x, y, &tx, &ty, lw, lh)) {
+                if (overwrite) dst[o] = out;
+                continue;
+            }
+            int ftx = (int)floorf(tx), fty = (int)floorf(ty), ctx = (int)ceilf(tx), cty = (int)ceilf(ty);
+            ribbon s00 = src[fty * c->W + ftx], s10 = src[fty * c->W + ctx];
+            ribbon s01 = src[cty * c->W + ftx], s11 = src[cty * c->W + ctx];
+            int v00 = s00.layer >= 0, v10 = s10.layer >= 0, v01 = s01.layer >= 0, v11 = s11.layer >= 0;
+            if (!(v00 || v10 || v01 || v11)) {
+                if (overwrite) dst[o] = out;
+                continue;
+            }
+            int all_valid = v00 && v10 && v01 && v11;
+            /* (s00.layer == s10.layer == s01.layer == s11.layer), evaluated
+             * left to right as C++ does: a bool compared with the next int */
+            int same = ((((s00.layer == s10.layer) ? 1 : 0) == s01.layer) ? 1 : 0) == s11.layer;
+            const float th = 5.f;
+            int diverges = fabsf(s00.x - s10.x) > th || fabsf(s00.x - s01.x) > th || fabsf(s00.x - s11.x) > th ||
+                           fabsf(s00.y - s10.y) > th || fabsf(s00.y - s01.y) > th || fabsf(s00.y - s11.y) > th;
+            if (!all_valid || !same || diverges) {
+                int rtx = (int)roundf(tx), rty = (int)roundf(ty);
+                if (ftx == rtx && fty == rty) out = s00;
+                else if (ftx + 1 == rtx && fty == rty) out = s10;
+                else if (ftx == rtx && fty + 1 == rty) out = s01;
+                else out = s11;
+            } else {
+                float wx = tx - (float)ftx, wy = ty - (float)fty;
+                float rx0 = s00.x * (1.f - wx) + s10.x * wx;
+                float ry0 = s00.y * (1.f - wx) + s10.y * wx;
+                float rx1 = s01.x * (1.f - wx) + s11.x * wx;
+                float ry1 = s01.y * (1.f - wx) + s11.y * wx;
+                out.x = rx0 * (1.f - wy) + rx1 * wy;
+                out.y = ry0 * (1.f - wy) + ry1 * wy;
+                out.ts = s00.ts;
+                out.layer = s00.layer;
+            }
+            out.x += (tx - (float)x);
+            out.y += (ty - (float)y);
+            out.ts += time_step;
+            ribbon cur = dst[o];
+            if (overwrite || !(cur.layer >= 0) || fabsf((float)cur.ts) >= fabsf((float)out.ts)) dst[o] = out;
+        }
+    memcpy(c->rib_base, c->rib_work, sizeof(ribbon) * (size_t)c->W * c->H);
+    for (int y = 0; y < lh; y++)
+        for (int x = 0; x < lw; x++) {
+            int o = y * c->W + x;
+            ribbon r = c->rib_base[o];
+            f4 v = {0.f, 0.f, 0.f, 0.f};
+            if (r.layer >= 0) {
+                int sx = clampi((int)((float)x + r.x), 0, c->W - 1), sy = clampi((int)((float)y + r.y), 0, c->H - 1);
+                v = c->canvas[sy * c->W + sx];
+            }
+            c->adv[0].out[o] = v;
+        }
+    return PT_OK;
+}
+
+int ptref_load_ribbon_field(ptref_ctx *c, int level, const pt_ribbonp *in)
+{
+    if (!c || !c->off_base || !in || level < 0) return PT_ERR_INVALID_ARG;
+    int lw = c->W >> level, lh = c->H >> level;
+    for (int y = 0; y < lh; y++)
+        for (int x = 0; x < lw; x++) {
+            const pt_ribbonp *p = &in[y * lw + x];
+            ribbon r = {p->x, p->y, p->time_step, p->layer};
+            c->rib_base[y * c->W + x] = r;
+        }
+    return PT_OK;
+}
+
+int ptref_get_ribbon_field(ptref_ctx *c, int level, pt_ribbonp *out)
+{
+    if (!c || !c->off_base || !out || level < 0) return PT_ERR_INVALID_ARG;
+    int lw = c->W >> level, lh = c->H >> level;
+    for (int y = 0; y < lh; y++)
+        for (int x = 0; x < lw; x++) {
+            ribbon r = c->rib_base[y * c->W + x];
+            pt_ribbonp *p = &out[y * lw + x];
+            p->x = r.x; p->y = r.y; p->time_step = r.ts; p->layer = r.layer;
+        }
+    return PT_OK;
+}
+
 /* ------------------------------------------------------------------ */
 /* host readback / load                                                */
 /* ------------------------------------------------------------------ */
 static pt_patchxf to_patchxf(nnf n)
 {
     pt_patchxf p;
-    p.x = n.x; p.y = n.y; p.theta = n.th; p.hysteresis = n.hy; p.layer = 0;
+    p.x = n.x; p.y = n.y; p.theta = n.th; p.hysteresis = n.hy; p.layer = n.la;
     p.scale_u = n.su; p.scale_v = n.sv; p.lum_y = n.ly; p.lum_a = 0.f;
     return p;
 }
@@ -1459,10 +1877,10 @@ int ptref_load_offsets(ptref_ctx *c, int level, const pt_patchxf *in)
     for (int y = 0; y < lh; y++)
         for (int x = 0; x < lw; x++) {
             const pt_patchxf *p = &in[y * lw + x];
-            if (p->layer != 0 || p->lum_a != 0.f)
-                return fail(c, PT_ERR_UNSUPPORTED, "offsets with a style layer or an alpha luminance shift");
+            if (p->lum_a != 0.f)
+                return fail(c, PT_ERR_UNSUPPORTED, "offsets with an alpha luminance shift");
             if (p->lum_y != 0.f) c->lum_live = 1;
-            nnf n = {p->x, p->y, p->theta, p->hysteresis, p->scale_u, p->scale_v, p->lum_y};
+            nnf n = {p->x, p->y, p->theta, p->hysteresis, p->scale_u, p->scale_v, p->lum_y, p->layer};
             c->off_base[y * c->W + x] = n;
         }
     return PT_OK;
@@ -1486,7 +1904,7 @@ int ptref_pde_count(ptref_ctx *c, uint64_t *out)
 
 float ptref_patch_distance(ptref_ctx *c, int x, int y, int level, const pt_patchxf *cand)
 {
-    nnf n = {cand->x, cand->y, cand->theta, cand->hysteresis, cand->scale_u, cand->scale_v, 0.f};
+    nnf n = {cand->x, cand->y, cand->theta, cand->hysteresis, cand->scale_u, cand->scale_v, 0.f, cand->layer};
     return calc_residual(c, x, y, level, n);
 }
 

@@ -46,6 +46,16 @@ int ptref_create_random_pyramid(ptref_ctx *ctx, int level, int frame_number,
                                 const ptref_pyramid *exemplar_output,
                                 const ptref_pyramid *exemplar_orientation);
 
+int ptref_create_random_pyramid_keys(ptref_ctx *ctx, int level, int frame_number, const ptref_pyramid *image_color,
+                                     const ptref_pyramid *image_orientation, int num_styles,
+                                     const ptref_pyramid *const *exemplar_outputs,
+                                     const ptref_pyramid *const *exemplar_orientations);
+int ptref_initialize_cumulative_fields(ptref_ctx *ctx, int w, int h, int num_layers);
+int ptref_offsets_from_frame_to_key_ribbon(ptref_ctx *ctx, int level, int look_forwards);
+int ptref_merge_advected_buffers(ptref_ctx *ctx, int level, int merge_fallback);
+int ptref_accumulate_ribbon_field(ptref_ctx *ctx, int level, int forwards, int time_step, int overwrite);
+int ptref_load_ribbon_field(ptref_ctx *ctx, int level, const pt_ribbonp *ribbons);
+int ptref_get_ribbon_field(ptref_ctx *ctx, int level, pt_ribbonp *ribbons);
 int ptref_randomize_offsets(ptref_ctx *ctx, int level);
 int ptref_propagate(ptref_ctx *ctx, int level, int jump, int key_index);
 int ptref_scatter(ptref_ctx *ctx, int level, int key_index);

@@ -25,6 +25,10 @@ PT_DONT_UPDATE_RESIDUAL = 0x10
 PT_ADVECT_IGNORE_RANDOM = 0x1
 PT_ADVECT_DO_NOT_CHECK_PREVIOUS = 0x10
 PT_ADVECT_UPDATE_RESIDUAL = 0x80
+PT_ADVECT_CHECK_FRAME_TO_KEY = 0x20
+PT_MERGE_FALLBACK_RANDOM = 0x20
+PT_MERGE_FALLBACK_LAST_PASS = 0x40
+PT_MAX_STYLE_LAYERS = 4
 PT_ELEM_F32, PT_ELEM_F32X2, PT_ELEM_F32X3, PT_ELEM_F32X4, PT_ELEM_I32 = 1, 2, 3, 4, 5
 PT_TUNE_EARLY_EXIT = 1
 PT_TUNE_EARLY_EXIT_MIN_PIXELS = 2
@@ -35,12 +39,13 @@ PT_TUNE_TAP_PARALLEL_EVALS = 6
 PT_TUNE_EXEMPLAR_WINDOW = 7
 PT_TUNE_TIME_TABLE_MAX_TAPS = 8
 PT_SERVER_FILL_TILES = 5
-PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT = 0, 1
+PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT, PT_DOWNSAMPLE_RIBBON = 0, 1, 2
 
 LAYER = dict(
     EXEMPLAR_BASE=0, EXEMPLAR_OUTPUT=1, EXEMPLAR_ORIENTATION=2, EXEMPLAR_DIST_TRANS=3, EXEMPLAR_OBJECT_IDS=4,
     INPUT_COLOR=5, INPUT_ORIENTATION=6, INPUT_DIST_TRANS=7, INPUT_SCALE=8, INPUT_ID=9,
     INPUT_VEL_B=11, INPUT_VEL_F=12, INPUT_VEL_B_PREVIOUS=13, INPUT_VEL_F_PREVIOUS=14,
+    CUMUL_FIELD_B=17, CUMUL_FIELD_F=18, FRAME_TO_KEY_RIBBON_B=19, FRAME_TO_KEY_RIBBON_F=20,
     RANDOM=21,
 )
 
@@ -101,6 +106,9 @@ PATCHXF_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("theta", "<f4"), ("hyster
                           ("layer", "<i4"), ("scale_u", "<f4"), ("scale_v", "<f4"),
                           ("lum_y", "<f4"), ("lum_a", "<f4")])
 assert PATCHXF_DTYPE.itemsize == C.sizeof(PatchXF) == 36
+
+# pt_ribbonp: a pixel's ribbon to a keyframe (RibbonP, types.h:116-126)
+RIBBON_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("time_step", "<i4"), ("layer", "<i4")])
 
 FP = C.POINTER(C.c_float)
 IP = C.POINTER(C.c_int32)
@@ -211,6 +219,13 @@ SIGNATURES = [
     ("load_residual", I, [VP, I, VP]),
     ("get_advected", I, [VP, I, I, VP, VP, VP, VP, VP]),
     ("pde_count", I, [VP, C.POINTER(C.c_uint64)]),
+    ("initialize_cumulative_fields", I, [VP, I, I, I]),
+    ("create_random_pyramid_keys", I, [VP, I, I, VP, VP, I, C.POINTER(VP), C.POINTER(VP)]),
+    ("offsets_from_frame_to_key_ribbon", I, [VP, I, I]),
+    ("merge_advected_buffers", I, [VP, I, I]),
+    ("accumulate_ribbon_field", I, [VP, I, I, I, I]),
+    ("load_ribbon_field", I, [VP, I, VP]),
+    ("get_ribbon_field", I, [VP, I, VP]),
     ("synthesize", I, [VP, C.POINTER(PtParams), I, I, I, I, I, I, C.POINTER(FrameGuides), I, I,
                        C.POINTER(StyleGuides), VP, VP, VP]),
     ("last_error", C.c_char_p, [VP]),

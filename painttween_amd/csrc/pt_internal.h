@@ -11,12 +11,13 @@
 #include "../../include/pt_api.h"
 
 /* Nearest-neighbour field as structure-of-arrays planes: {x, y, theta,
- * hysteresis} and {scaleU, scaleV}.  Style layer and luminance shift are
- * fixed at 0 on this path (PatchXF, types.h:98-114). */
+ * hysteresis}, {scaleU, scaleV}, the luminance shift and the style layer
+ * (PatchXF, types.h:98-114). */
 struct PtNnf {
     float4 *xyth;
     float2 *sc;
-    float *lum; /* luminance shift y (PatchXF::luminanceShift); read only while lum_live */
+    float *lum;  /* luminance shift y (PatchXF::luminanceShift); read only while lum_live */
+    int32_t *la; /* style layer (PatchXF::layer); read and written only while layered */
 };
 
 /* AdvectedBuffersDevice (advectedBuffers.h:37-52). */
@@ -28,6 +29,7 @@ struct PtAdv {
     float *res;
     int32_t *ts;
     float *lum;
+    int32_t *la;
 };
 
 #define PT_MAX_R 7
@@ -72,6 +74,22 @@ struct PtArgs {
     int tap_np_res;    /* tap-parallel refresh: pixels per workgroup (pt_tap_configure)   */
     int tap_np_prop;   /* tap-parallel propagate: pixels per workgroup (pt_tap_configure) */
     int ex_window;     /* XL kernel variants: exemplar windows in LDS */
+
+    /* style layers (keyframe interpolation): with `layered` set every
+     * offset carries its layer (PtNnf::la) and the per-pixel kernels' RT < 0
+     * variant samples the exemplar layer of each candidate.  The exemplar
+     * holds nl layers of ew x eh texels, layer l from texel l * ew * eh. */
+    int layered, nl;
+    int keys;          /* interpolateKeyFrame: the *_keys search kernels     */
+    int key_first, key_last;
+    int key_index;     /* propagate/scatter: keyframe layer whose painted texels are fixed (-1: none) */
+    /* cumulative key-to-key fields (layered, cl layers of cw x ch, bilinear,
+     * [0] = backward, [1] = forward) and the frame-to-key ribbons (fw x fh,
+     * point fetches); a context without them binds one zero texel */
+    const float4 *cumul[2];
+    int cw, ch, cl;
+    const float4 *ftk[2];
+    int fw[2], fh[2];
 
     uint8_t *changed;  /* per pixel: base offset changed by the last sync */
     /* time tables of the advected sets (k_time_table) or null: entry
@@ -169,6 +187,18 @@ hipError_t pt_launch_random_pyramid(const float4 *color, const float *iorient, i
                                     const float4 *eout, const float *eorient, int ew, int eh,
                                     int transp, int orient_en, uint32_t key0, uint32_t key1,
                                     float4 *out, hipStream_t s);
+hipError_t pt_launch_random_pyramid_keys(const float4 *color, const float *iorient, int w, int h, int nstyles,
+                                         const float4 *const *eouts, const float *const *eorients, const int *ews,
+                                         const int *ehs, int key_first, int num_keys, int transp, int orient_en,
+                                         uint32_t key0, uint32_t key1, float4 *out, hipStream_t s);
+/* keyframe interpolation: ribbons are pt_ribbonp, level-0 pitch W */
+hipError_t pt_launch_merge_advected(const PtArgs &a, int fallback, hipStream_t s);
+hipError_t pt_launch_offsets_from_ribbon(const PtArgs &a, int look_forwards, hipStream_t s);
+hipError_t pt_launch_accumulate_ribbon(const PtArgs &a, const pt_ribbonp *src, pt_ribbonp *dst, int forwards,
+                                       int time_step, int overwrite, hipStream_t s);
+hipError_t pt_launch_ribbon_output(const PtArgs &a, const pt_ribbonp *rib, hipStream_t s);
+hipError_t pt_launch_downsample_ribbon(const float *src, int pw, float *dst, int lw, int lh, hipStream_t s);
+hipError_t pt_launch_fill_ribbon(pt_ribbonp *dst, size_t n, hipStream_t s);
 hipError_t pt_launch_pack_nnf(const PtNnf &src, int W, int lw, int lh, void *dst_patchxf, hipStream_t s);
 hipError_t pt_launch_unpack_nnf(const void *src_patchxf, int lw, int lh, PtNnf dst, int W, hipStream_t s);
 

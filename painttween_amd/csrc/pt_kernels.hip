@@ -109,17 +109,24 @@ __device__ __forceinline__ void lum_set(const PtArgs &A, int o, bool keep)
     if (A.lum_live) A.work.lum[o] = keep ? A.base.lum[o] : 0.f;
 }
 
-__device__ __forceinline__ float4 tex_ex_out(const PtArgs &A, float x, float y)
+/* First texel of exemplar style layer l: a layered texture clamps the layer
+ * index into [0, nl - 1].  A literal 0 folds away. */
+__device__ __forceinline__ int lay(const PtArgs &A, int l)
+{
+    return (l <= 0 ? 0 : (l < A.nl ? l : A.nl - 1)) * (A.ew * A.eh);
+}
+
+__device__ __forceinline__ float4 tex_ex_out(const PtArgs &A, float x, float y, int l = 0)
 {
     Bilin q = bilin(A.ew, A.ew, A.eh, x, y);
-    const float4 *e = A.ex;
+    const float4 *e = A.ex + 2 * lay(A, l);
     return mix4(q, e[2 * q.i00 + 1], e[2 * q.i10 + 1], e[2 * q.i01 + 1], e[2 * q.i11 + 1]);
 }
 
-__device__ __forceinline__ float tex_ex_out_alpha(const PtArgs &A, float x, float y)
+__device__ __forceinline__ float tex_ex_out_alpha(const PtArgs &A, float x, float y, int l = 0)
 {
     Bilin q = bilin(A.ew, A.ew, A.eh, x, y);
-    const float4 *e = A.ex;
+    const float4 *e = A.ex + 2 * lay(A, l);
     return mix1(q, e[2 * q.i00 + 1].w, e[2 * q.i10 + 1].w, e[2 * q.i01 + 1].w, e[2 * q.i11 + 1].w);
 }
 
@@ -200,11 +207,12 @@ __device__ __forceinline__ void new_coord(float i, float j, float ex, float ey, 
     oy = ey + cy;
 }
 
-/* screenToExemplarAngle (texSynth_kernel.cu:425-436) */
-__device__ __forceinline__ float screen_to_ex_angle(const PtArgs &A, float x, float y, float ex, float ey)
+/* screenToExemplarAngle (texSynth_kernel.cu:425-436), style layer l */
+__device__ __forceinline__ float screen_to_ex_angle(const PtArgs &A, float x, float y, float ex, float ey, int l = 0)
 {
     if (A.orient_en) {
-        float theta = A.ex_orient[point_index(A.ew, A.eh, ex, ey)] - A.orient[point_index(A.ow, A.oh, x, y)];
+        float theta =
+            A.ex_orient[lay(A, l) + point_index(A.ew, A.eh, ex, ey)] - A.orient[point_index(A.ow, A.oh, x, y)];
         if ((double)theta > 1.57079632679489661923)
             return (float)((double)theta - 3.14159265358979323846);
         if ((double)theta < -1.57079632679489661923)
@@ -223,9 +231,9 @@ __device__ __forceinline__ bool image_culled(const PtArgs &A, int x, int y)
 /* isExemplarPixelCulled (texSynth_kernel.cu:200-205): its parameters are
  * int, so the candidate position is truncated toward zero and the texel at
  * that integer position read exactly (clamped). */
-__device__ __forceinline__ bool exemplar_culled(const PtArgs &A, float x, float y)
+__device__ __forceinline__ bool exemplar_culled(const PtArgs &A, float x, float y, int l = 0)
 {
-    float d = A.ex_dt[clampi((int)y, 0, A.eh - 1) * A.ew + clampi((int)x, 0, A.ew - 1)];
+    float d = A.ex_dt[lay(A, l) + clampi((int)y, 0, A.eh - 1) * A.ew + clampi((int)x, 0, A.ew - 1)];
     return (double)d > (double)A.cull * 1.25;
 }
 
@@ -283,16 +291,18 @@ __device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, int off)
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
-/* Buffer resource over the interleaved exemplar texture (32 B per texel). */
+/* Buffer resource over the interleaved exemplar texture (32 B per texel),
+ * every style layer. */
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t ex_rsrc(const PtArgs &A)
 {
-    return __builtin_amdgcn_make_buffer_rsrc((void *)A.ex, (short)0, A.ew * A.eh * 32, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)A.ex, (short)0, (A.nl > 1 ? A.nl : 1) * A.ew * A.eh * 32,
+                                             0x00020000);
 }
 
 /* Bilinear exemplar base and output at (x, y), clamp addressing; the same
- * weights and fused order as tex4_lin. */
+ * weights and fused order as tex4_lin.  lb: first texel of the style layer. */
 __device__ __forceinline__ void ex_pair(__amdgpu_buffer_rsrc_t er, int ew, int eh, float x, float y, float4 &b,
-                                        float4 &o)
+                                        float4 &o, int lb = 0)
 {
     float fx = floorf(x), fy = floorf(y);
     float a = x - fx, c = y - fy;
@@ -304,7 +314,7 @@ __device__ __forceinline__ void ex_pair(__amdgpu_buffer_rsrc_t er, int ew, int e
     q.w10 = a * (1.0f - c);
     q.w01 = (1.0f - a) * c;
     q.w11 = a * c;
-    int ra = ya * ew, rb = yb * ew;
+    int ra = lb + ya * ew, rb = lb + yb * ew;
     int o00 = (ra + xa) << 5, o10 = (ra + xb) << 5, o01 = (rb + xa) << 5, o11 = (rb + xb) << 5;
     b = mix4(q, ld4(er, o00), ld4(er, o10), ld4(er, o01), ld4(er, o11));
     o = mix4(q, ld4(er, o00 + 16), ld4(er, o10 + 16), ld4(er, o01 + 16), ld4(er, o11 + 16));
@@ -314,7 +324,7 @@ __device__ __forceinline__ void ex_pair(__amdgpu_buffer_rsrc_t er, int ew, int e
  * inside it (same texels, weights and order as the global fetch). */
 template <bool XL>
 __device__ __forceinline__ void ex_pair_t(const Tile &T, __amdgpu_buffer_rsrc_t er, int ew, int eh, float x, float y,
-                                          float4 &b, float4 &o)
+                                          float4 &b, float4 &o, int lb = 0)
 {
     if (XL && T.exl) {
         float fx = floorf(x), fy = floorf(y);
@@ -330,7 +340,7 @@ __device__ __forceinline__ void ex_pair_t(const Tile &T, __amdgpu_buffer_rsrc_t 
         b = mix4(q, e[i00], e[i00 + 2], e[i01], e[i01 + 2]);
         o = mix4(q, e[i00 + 1], e[i00 + 3], e[i01 + 1], e[i01 + 3]);
     } else {
-        ex_pair(er, ew, eh, x, y, b, o);
+        ex_pair(er, ew, eh, x, y, b, o, lb);
     }
 }
 
@@ -370,7 +380,7 @@ __device__ __forceinline__ void stage_window(const PtArgs &A, float ax, float ay
 template <bool EXT = false>
 __device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k, float4 prev, float pb0x, float pb0y,
                               float pb1x, float pb1y, float4 base, float4 tb, float4 canvas, float4 to, int oo, int x,
-                              int y, int i, int j)
+                              int y, int i, int j, int plb = 0)
 {
     const PtAdv &d = A.adv[k];
     float4 prev_base = d.base[oo];
@@ -385,7 +395,7 @@ __device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k,
     } else {
         float ptx, pty;
         new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, ptx, pty);
-        ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo);
+        ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo, plb);
     }
     if (EXT && A.lab) {
         prev_base = lab4(prev_base);
@@ -427,9 +437,12 @@ __device__ __forceinline__ float coherence_add(const PtArgs &A, float res, int x
  */
 template <int RT, bool TIME, bool XL = false>
 __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc, float4 pf,
-                                    float4 pb, bool cpv, bool cnv)
+                                    float4 pb, bool cpv, bool cnv, int tl = 0, int pfl = 0, int pbl = 0)
 {
     const int r = RT > 0 ? RT : A.r;
+    /* style layers of the candidate and of the advected offsets (RT < 0) */
+    const int tlb = RT < 0 ? lay(A, tl) : 0;
+    const int pflb = RT < 0 ? lay(A, pfl) : 0, pblb = RT < 0 ? lay(A, pbl) : 0;
     __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
     const int ew = A.ew, eh = A.eh;
     const float hw = (float)A.hw, hh = (float)A.hh;
@@ -442,12 +455,12 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
     if (TIME) {
         if (cpv) {
             float2 ps = A.adv[0].sc[o];
-            float ang = screen_to_ex_angle(A, (float)x, (float)y, pf.x, pf.y);
+            float ang = screen_to_ex_angle(A, (float)x, (float)y, pf.x, pf.y, RT < 0 ? pfl : 0);
             basis(ang, A.su * ps.x, A.sv * ps.y, fb0x, fb0y, fb1x, fb1y);
         }
         if (cnv) {
             float2 ps = A.adv[1].sc[o];
-            float ang = screen_to_ex_angle(A, (float)x, (float)y, pb.x, pb.y);
+            float ang = screen_to_ex_angle(A, (float)x, (float)y, pb.x, pb.y, RT < 0 ? pbl : 0);
             basis(ang, A.su * ps.x, A.sv * ps.y, bb0x, bb0y, bb1x, bb1y);
         }
         if (cpv)
@@ -479,7 +492,7 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
             if (ty >= hh) ty -= hh;
             else if (ty < 0) ty += hh;
             float4 tb, to;
-            ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to);
+            ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to, tlb);
             if (RT < 0 && A.lab) {
                 tb = lab4(tb);
                 to = lab4(to);
@@ -488,13 +501,15 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
             if (TIME && any_time_w) {
                 int oo = (y + j) * A.W + (x + i);
                 if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
-                    res = res + time_terms_b<(RT < 0)>(A, er, 0, pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo, x, y, i, j);
+                    res = res + time_terms_b<(RT < 0)>(A, er, 0, pf, fb0x, fb0y, fb1x, fb1y, base, tb, canvas, to, oo,
+                                                       x, y, i, j, pflb);
                 if (bwd_dir && cnv && A.adv[1].xyth[oo].w > 0)
-                    res = res + time_terms_b<(RT < 0)>(A, er, 1, pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo, x, y, i, j);
+                    res = res + time_terms_b<(RT < 0)>(A, er, 1, pb, bb0x, bb0y, bb1x, bb1y, base, tb, canvas, to, oo,
+                                                       x, y, i, j, pblb);
             }
             if (i == 0 && j == 0) {
                 if (A.dt_w > 0.f) {
-                    float de = tex1_lin(A.ex_dt, A.ew, A.ew, A.eh, tx, ty);
+                    float de = tex1_lin(A.ex_dt + tlb, A.ew, A.ew, A.eh, tx, ty);
                     de /= tsc.y;
                     float di = A.dt[A.cur][o];
                     float diff = fabsf(de - di);
@@ -505,7 +520,7 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
                  * fetches, the image ID at the pixel, the style's at the tap */
                 if (RT < 0 && A.obj_w > 0.f) {
                     int iid = A.img_id[point_index(A.idw, A.idh, (float)x, (float)y)];
-                    int sid = A.ex_id[point_index(A.ew, A.eh, tx, ty)];
+                    int sid = A.ex_id[tlb + point_index(A.ew, A.eh, tx, ty)];
                     if (iid != sid) res = res + A.obj_w;
                 }
             } else {
@@ -526,31 +541,39 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
  * offset is valid (the advected buffers were reset and not written since);
  * the per-pixel test then always takes the plain variant. */
 template <int RT, bool TK, bool XL = false>
-__device__ __forceinline__ float calc_residual(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc)
+__device__ __forceinline__ float calc_residual(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc,
+                                               int tl = 0)
 {
     if (TK) {
         int o = y * A.W + x;
         float4 pf = A.adv[0].xyth[o], pb = A.adv[1].xyth[o];
         bool cpv = pf.w > 0, cnv = pb.w > 0;
+        int pfl = 0, pbl = 0;
+        if (RT < 0 && A.layered) {
+            pfl = A.adv[0].la[o];
+            pbl = A.adv[1].la[o];
+        }
         if (cpv || cnv)
-            return calc_residual_impl<RT, true, XL>(A, T, x, y, t, tsc, pf, pb, cpv, cnv);
+            return calc_residual_impl<RT, true, XL>(A, T, x, y, t, tsc, pf, pb, cpv, cnv, tl, pfl, pbl);
     }
     float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    return calc_residual_impl<RT, false, XL>(A, T, x, y, t, tsc, z, z, false, false);
+    return calc_residual_impl<RT, false, XL>(A, T, x, y, t, tsc, z, z, false, false, tl);
 }
 
-/* test_offset (texSynth_kernel.cu:695-721) */
+/* test_offset (texSynth_kernel.cu:695-721); tl: the candidate's style
+ * layer (RT < 0 only) */
 template <int RT, bool TK>
 __device__ __forceinline__ float test_offset(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc,
-                                             float image_alpha)
+                                             float image_alpha, int tl = 0)
 {
-    if (!exemplar_culled(A, t.x, t.y)) {
+    if (RT >= 0) tl = 0;
+    if (!exemplar_culled(A, t.x, t.y, tl)) {
         if (!A.transp) {
-            float ea = tex_ex_out_alpha(A, t.x, t.y);
+            float ea = tex_ex_out_alpha(A, t.x, t.y, tl);
             if (ea < 0.99f && image_alpha > 0.99f)
                 return 1e31f;
         }
-        return calc_residual<RT, TK>(A, T, x, y, t, tsc);
+        return calc_residual<RT, TK>(A, T, x, y, t, tsc, tl);
     }
     return 1e31f;
 }
@@ -667,7 +690,7 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
         float4 t = A.base.xyth[o];
         float2 tsc = A.base.sc[o];
         if (XL) T.exl = fits(t, tsc);
-        A.res_base[o] = calc_residual<RT, TK, XL>(A, T, x, y, t, tsc);
+        A.res_base[o] = calc_residual<RT, TK, XL>(A, T, x, y, t, tsc, (RT < 0 && A.layered) ? A.base.la[o] : 0);
         return;
     }
     if (valid) {
@@ -699,7 +722,7 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
     float4 t = A.base.xyth[o];
     float2 tsc = A.base.sc[o];
     if (XL) T.exl = fits(t, tsc);
-    A.res_base[o] = calc_residual<RT, TK, XL>(A, T, x, y, t, tsc);
+    A.res_base[o] = calc_residual<RT, TK, XL>(A, T, x, y, t, tsc, (RT < 0 && A.layered) ? A.base.la[o] : 0);
 }
 
 template <int RT, bool TK, bool XL = false>
@@ -715,7 +738,7 @@ __device__ __forceinline__ void b_canvas(const PtArgs &A, int bx, int by)
     if (!pixel_of_rows(A, bx, by, x, y)) return;
     int o = y * A.W + x;
     float4 n = A.base.xyth[o];
-    A.canvas[o] = lum_shifted(A, tex_ex_out(A, n.x, n.y), o);
+    A.canvas[o] = lum_shifted(A, tex_ex_out(A, n.x, n.y, A.layered ? A.base.la[o] : 0), o);
     if (A.changed) A.changed[o] = 0;
 }
 
@@ -736,6 +759,7 @@ __device__ __forceinline__ void b_sync_canvas(const PtArgs &A, int bx, int by, i
     int y = by * PT_RY + threadIdx.y;
     int o = y * A.W + x;
     float4 n;
+    int nl = 0;
     bool have = false, ch = false;
     if (x < dw && y < dh) {
         n = A.work.xyth[o];
@@ -753,11 +777,19 @@ __device__ __forceinline__ void b_sync_canvas(const PtArgs &A, int bx, int by, i
             ch = ch || (A.changed && bits_differ(l, A.base.lum[o]));
             A.base.lum[o] = l;
         }
+        if (A.layered) {
+            nl = A.work.la[o];
+            ch = ch || (A.changed && nl != A.base.la[o]);
+            A.base.la[o] = nl;
+        }
         have = true;
     }
     if (x < A.lw && y < A.lh) {
-        if (!have) n = A.base.xyth[o];
-        A.canvas[o] = lum_shifted(A, tex_ex_out(A, n.x, n.y), o);
+        if (!have) {
+            n = A.base.xyth[o];
+            if (A.layered) nl = A.base.la[o];
+        }
+        A.canvas[o] = lum_shifted(A, tex_ex_out(A, n.x, n.y, nl), o);
         if (A.changed) A.changed[o] = ch ? 1 : 0;
     }
 }
@@ -777,23 +809,28 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A)
     A.work.xyth[o] = make_float4(r.x, r.y, r.z, 0.f);
     A.work.sc[o] = make_float2(1.f, 1.f);
     lum_set(A, o, false);
+    /* toPatchXF(random): the style layer is the texel's w (texSynth_util.h:212-214) */
+    if (A.layered) A.work.la[o] = (int)r.w;
 }
 
 /* calculatePropagateOffset (texSynth_kernel.cu:956-982); pixelWrap
- * (cudaImageBuffer.h:81-87) wraps modulo the level-0 dimensions. */
+ * (cudaImageBuffer.h:81-87) wraps modulo the level-0 dimensions.  With cl
+ * given, the candidate keeps the neighbour's style layer (*cl). */
 __device__ __forceinline__ void propagate_offset(const PtArgs &A, int tx, int ty, float x, float y, float4 &c,
-                                                 float2 &csc)
+                                                 float2 &csc, int *cl = nullptr)
 {
     int wx = (int)((float)tx - (float)A.W * floorf((float)tx / (float)A.W));
     int wy = (int)((float)ty - (float)A.H * floorf((float)ty / (float)A.H));
     float4 t = A.base.xyth[wy * A.W + wx];
+    int l = 0;
+    if (cl) *cl = l = A.base.la[wy * A.W + wx];
     float2 sc = scale_at(A, (float)tx, (float)ty);
     float b0x, b0y, b1x, b1y;
     basis(t.z, A.su * sc.x, A.sv * sc.y, b0x, b0y, b1x, b1y);
     float ex, ey;
     new_coord(x - (float)tx, y - (float)ty, t.x, t.y, b0x, b0y, b1x, b1y, ex, ey);
     clamp_ex(ex, ey, A.ew, A.eh);
-    float ang = screen_to_ex_angle(A, x, y, ex, ey);
+    float ang = screen_to_ex_angle(A, x, y, ex, ey, l);
     csc = scale_at(A, x, y);
     c = make_float4(ex, ey, ang, 0.f);
 }
@@ -817,7 +854,62 @@ __device__ __forceinline__ void propagate_offset_io(const PtArgs &A, int tx, int
     csc = csc_px;
 }
 
-/* propagate_kernel (texSynth_kernel.cu:1112-1219) */
+/* The keymask of keyframe layer k (texSynth_kernel.cu:1137-1142, 887-892):
+ * the exemplar output of that layer at the pixel's own integer position (the
+ * texel itself, clamped); alpha > 0 marks a painted keyframe pixel. */
+__device__ __forceinline__ bool keymask(const PtArgs &A, int k, int x, int y)
+{
+    if (k < 0) return false;
+    return A.ex[2 * (lay(A, k) + clampi(y, 0, A.eh - 1) * A.ew + clampi(x, 0, A.ew - 1)) + 1].w > 0.f;
+}
+
+/* A cumulative key-to-key field texel (k = 0 backward, 1 forward) at
+ * exemplar (ex, ey) of layer l: bilinear, clamped in x, y and layer. */
+__device__ __forceinline__ float4 cumul_at(const PtArgs &A, int k, float ex, float ey, int l)
+{
+    int c = l <= 0 ? 0 : (l < A.cl ? l : A.cl - 1);
+    return tex4_lin(A.cumul[k] + (size_t)c * A.cw * A.ch, A.cw, A.cw, A.ch, ex, ey);
+}
+
+/* The candidates of propagate_kernel_keys / scatter_kernel_keys
+ * (texSynth_kernel.cu:1061-1107, 792-834) that follow candidate (ex0, ey0)
+ * of layer el through the cumulative fields: to the earlier keys (field 0,
+ * layers el-1 down to the first key, each step read in the layer above) and
+ * to the later keys (field 1, layers el+1 up to the last key, read in the
+ * layer below), each tested in turn with scale (1, 1).  The field texel is
+ * Color4(time_step, y, x, layer): propagation stops a walk on a time step
+ * below -EPSILON, scatter on an invalid ribbon (layer < 0). */
+template <int RT, bool TK>
+__device__ void key_walks(const PtArgs &A, const Tile &T, int x, int y, float ex0, float ey0, int el, float ia,
+                          bool scatter_rule, float4 &ow, float2 &osc, int &ol, float &best, bool &moved)
+{
+#pragma unroll 1
+    for (int dir = 0; dir < 2; dir++) {
+        float ex = ex0, ey = ey0;
+        const int step = dir == 0 ? -1 : 1;
+#pragma unroll 1
+        for (int j = el + step; dir == 0 ? j >= A.key_first : j <= A.key_last; j += step) {
+            float4 m = cumul_at(A, dir, ex, ey, j - step);
+            if (scatter_rule ? ((int)m.w < 0) : (m.x < -0.000001f)) break;
+            ex += m.z;
+            ey += m.y;
+            float4 t = make_float4(ex, ey, screen_to_ex_angle(A, (float)x, (float)y, ex, ey, j), 0.f);
+            float2 ts = make_float2(1.f, 1.f);
+            float r = test_offset<RT, TK>(A, T, x, y, t, ts, ia, j);
+            if (r < best) {
+                ow = t;
+                osc = ts;
+                ol = j;
+                best = r;
+                moved = true;
+            }
+        }
+    }
+}
+
+/* propagate_kernel (texSynth_kernel.cu:1112-1219) and, in the RT < 0
+ * variant, propagate_kernel_keys (:984-1110): style layers, the keymask and
+ * the walks through the cumulative fields. */
 template <int RT, bool TK>
 __device__ __forceinline__ void b_propagate(const PtArgs &A, int bx, int by, int jump)
 {
@@ -827,16 +919,26 @@ __device__ __forceinline__ void b_propagate(const PtArgs &A, int bx, int by, int
     Tile T = stage_tile<RT>(A, bx, by, scol, scan);
     if (!valid) return;
     int o = y * A.W + x;
+    const bool layers = RT < 0 && A.layered;
     if (image_culled(A, x, y)) {
         A.work.xyth[o] = make_float4(0.f, 0.f, 0.f, 0.f);
         A.work.sc[o] = make_float2(1.f, 1.f);
         lum_set(A, o, false);
+        if (layers) A.work.la[o] = 0;
+        return;
+    }
+    if (layers && keymask(A, A.key_index, x, y)) {
+        A.work.xyth[o] = make_float4((float)x, (float)y, 0.f, 1.f);
+        A.work.sc[o] = make_float2(1.f, 1.f);
+        lum_set(A, o, false);
+        A.work.la[o] = A.key_index;
         return;
     }
     float best = A.res_base[o];
     float ia = A.color[A.cur][o].w;
     float4 ow = A.base.xyth[o];
     float2 osc = A.base.sc[o];
+    int ol = layers ? A.base.la[o] : 0;
     bool moved = false;
 #pragma unroll 1
     for (int i = 0; i < 4; i++) {
@@ -847,18 +949,23 @@ __device__ __forceinline__ void b_propagate(const PtArgs &A, int bx, int by, int
         else ty = y - jump;
         float4 c;
         float2 csc;
-        propagate_offset(A, tx, ty, (float)x, (float)y, c, csc);
-        float r = test_offset<RT, TK>(A, T, x, y, c, csc, ia);
+        int cl = 0;
+        if (layers) propagate_offset(A, tx, ty, (float)x, (float)y, c, csc, &cl);
+        else propagate_offset(A, tx, ty, (float)x, (float)y, c, csc);
+        float r = test_offset<RT, TK>(A, T, x, y, c, csc, ia, cl);
         if (r < best) {
             ow = c;
             osc = csc;
+            ol = cl;
             best = r;
             moved = true;
         }
+        if (RT < 0 && A.keys) key_walks<RT, TK>(A, T, x, y, c.x, c.y, cl, ia, false, ow, osc, ol, best, moved);
     }
     A.work.xyth[o] = ow;
     A.work.sc[o] = osc;
     lum_set(A, o, !moved);
+    if (layers) A.work.la[o] = ol;
 }
 
 template <int RT, bool TK>
@@ -869,9 +976,12 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
 
 __device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
 
-/* scatter_kernel (texSynth_kernel.cu:865-952) with averageOffset (:841-862).
- * As in the reference, a pixel is written only when a candidate beats its
- * cached residual; others keep the working buffer's current content. */
+/* scatter_kernel (texSynth_kernel.cu:865-952) with averageOffset (:841-862)
+ * and, in the RT < 0 variant, scatter_kernel_keys (:723-839): style layers,
+ * the keymask, samples in the winner's layer followed through the cumulative
+ * fields, no averaged candidate.  As in the reference, a pixel is written
+ * only when a candidate beats its cached residual; others keep the working
+ * buffer's current content. */
 template <int RT, bool TK>
 __device__ __forceinline__ void b_scatter(const PtArgs &A, int bx, int by)
 {
@@ -882,12 +992,23 @@ __device__ __forceinline__ void b_scatter(const PtArgs &A, int bx, int by)
     if (!valid) return;
     int o = y * A.W + x;
     if (image_culled(A, x, y)) return;
+    const bool layers = RT < 0 && A.layered;
+    const bool keys = RT < 0 && A.keys;
     float4 off = A.base.xyth[o];
     float2 offsc = A.base.sc[o];
+    if (layers && keymask(A, A.key_index, x, y)) { /* keep the current offset */
+        A.work.xyth[o] = off;
+        A.work.sc[o] = offsc;
+        lum_set(A, o, true);
+        A.work.la[o] = A.base.la[o];
+        return;
+    }
     float cached = A.res_base[o];
     float best = (float)((double)cached + 10e-7);
     float4 ow = off;
     float2 osc = offsc;
+    int ol = layers ? A.base.la[o] : 0;
+    bool moved = false;
     float ia = A.color[A.cur][o].w;
     float dist = (float)(A.ew > A.eh ? A.ew : A.eh);
     const float closest = 10.f, factor = 0.9f;
@@ -907,41 +1028,62 @@ __device__ __forceinline__ void b_scatter(const PtArgs &A, int bx, int by)
         float ex = ow.x + co * len;
         float ey = ow.y + s * len;
         clamp_ex(ex, ey, A.ew, A.eh);
-        float ang = screen_to_ex_angle(A, (float)x, (float)y, ex, ey);
-        float4 c = make_float4(ex, ey, ang, 0.f);
-        float res = test_offset<RT, TK>(A, T, x, y, c, sc, ia);
-        if (res < best) {
-            ow = c;
-            osc = sc;
-            best = res;
+        if (keys) {
+            /* the sample stays in the winner's layer, scale (1, 1) */
+            const int el = ol;
+            float4 c = make_float4(ex, ey, screen_to_ex_angle(A, (float)x, (float)y, ex, ey, el), 0.f);
+            float2 c1 = make_float2(1.f, 1.f);
+            float res = test_offset<RT, TK>(A, T, x, y, c, c1, ia, el);
+            if (res < best) {
+                ow = c;
+                osc = c1;
+                best = res;
+            }
+            key_walks<RT, TK>(A, T, x, y, ex, ey, el, ia, true, ow, osc, ol, best, moved);
+        } else {
+            float ang = screen_to_ex_angle(A, (float)x, (float)y, ex, ey);
+            float4 c = make_float4(ex, ey, ang, 0.f);
+            float res = test_offset<RT, TK>(A, T, x, y, c, sc, ia);
+            if (res < best) {
+                ow = c;
+                osc = sc;
+                ol = 0;
+                best = res;
+            }
         }
         dist *= factor;
     }
-    /* averageOffset */
-    float ax = 0.f, ay = 0.f;
-    int sum = 0;
-    for (int i = -1; i <= 1; i++)
-        for (int j = -1; j <= 1; j++) {
-            if (!in_bounds(x + i, y + j, A.lw, A.lh, 0) || (i == 0 && j == 0)) continue;
-            float4 n = A.base.xyth[(y + j) * A.W + (x + i)];
-            ax += n.x;
-            ay += n.y;
-            sum++;
+    if (!keys) {
+        /* averageOffset: the neighbours of layer 0 */
+        float ax = 0.f, ay = 0.f;
+        int sum = 0;
+        for (int i = -1; i <= 1; i++)
+            for (int j = -1; j <= 1; j++) {
+                if (!in_bounds(x + i, y + j, A.lw, A.lh, 0) || (i == 0 && j == 0)) continue;
+                if (layers && A.base.la[(y + j) * A.W + (x + i)] != 0) continue;
+                float4 n = A.base.xyth[(y + j) * A.W + (x + i)];
+                ax += n.x;
+                ay += n.y;
+                sum++;
+            }
+        ax /= (float)sum;
+        ay /= (float)sum;
+        float4 avg = make_float4(ax, ay, screen_to_ex_angle(A, (float)x, (float)y, ax, ay), 1.f);
+        float2 avgsc = make_float2(1.f, 1.f);
+        float res = test_offset<RT, TK>(A, T, x, y, avg, avgsc, ia);
+        if (res < best) {
+            ow = avg;
+            osc = avgsc;
+            ol = 0;
+            best = res;
         }
-    ax /= (float)sum;
-    ay /= (float)sum;
-    float4 avg = make_float4(ax, ay, screen_to_ex_angle(A, (float)x, (float)y, ax, ay), 1.f);
-    float2 avgsc = make_float2(1.f, 1.f);
-    float res = test_offset<RT, TK>(A, T, x, y, avg, avgsc, ia);
-    if (res < best) {
-        ow = avg;
-        osc = avgsc;
-        best = res;
     }
+    (void)moved;
     if (best < cached) {
         A.work.xyth[o] = ow;
         A.work.sc[o] = osc;
         lum_set(A, o, false);
+        if (layers) A.work.la[o] = ol;
     }
 }
 
@@ -2011,12 +2153,15 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A)
     }
 #undef D2
     float rw = A.res_base[lcy * A.W + lcx];
-    float ang = screen_to_ex_angle(A, (float)x, (float)y, ux, uy);
+    /* the style layer of the top-left coarse offset */
+    int l00 = A.layered ? A.base.la[lcy * A.W + lcx] : 0;
+    float ang = screen_to_ex_angle(A, (float)x, (float)y, ux, uy, l00);
     float2 sc = scale_at(A, (float)x, (float)y);
     int o = y * A.W + x;
     A.work.xyth[o] = make_float4(ux, uy, ang, 0.f);
     A.work.sc[o] = sc;
     lum_set(A, o, false);
+    if (A.layered) A.work.la[o] = l00;
     A.res_work[o] = rw;
 }
 
@@ -2044,7 +2189,14 @@ __device__ __forceinline__ bool translated_position(const PtArgs &A, int fwd, in
            in_bounds((int)floorf(tx), (int)floorf(ty), A.lw, A.lh, 0) && (dist < valid_dist);
 }
 
-/* advectKernel (texSynth_kernel.cu:1577-1693), without the keyframe branch */
+/* A frame-to-key ribbon texel (k = 0 backward, 1 forward) at the integer
+ * pixel (x, y): the texel itself, clamped.  Color4(time_step, y, x, layer). */
+__device__ __forceinline__ float4 ftk_at(const PtArgs &A, int k, int x, int y)
+{
+    return A.ftk[k][clampi(y, 0, A.fh[k] - 1) * A.fw[k] + clampi(x, 0, A.fw[k] - 1)];
+}
+
+/* advectKernel (texSynth_kernel.cu:1577-1693) */
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int time_step, int flags)
 {
     int x, y;
@@ -2057,6 +2209,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
         d.xyth[o] = make_float4(0, 0, 0, 0);
         d.sc[o] = make_float2(1.f, 1.f);
         if (A.lum_live) d.lum[o] = 0.f;
+        if (A.layered) d.la[o] = 0;
         return;
     }
     int prev = (A.cur + 1) % 2;
@@ -2065,6 +2218,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
                               rnd.w - floorf(rnd.w));
     float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
     float lum = 0.f;
+    int ol = 0;
     float tx, ty;
     /* isImagePixelCulledPreviousFrame(tx, ty, level, 8.0) takes int
      * coordinates (texSynth_kernel.cu:188-198): the position is truncated and
@@ -2073,6 +2227,7 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
         !(A.dt[prev][(int)ty * A.W + (int)tx] > ((float)A.cull - 8.0f))) {
         int rtx = (int)roundf(tx), rty = (int)roundf(ty);
         float4 nr = A.base.xyth[rty * A.W + rtx];
+        if (A.layered) ol = A.base.la[rty * A.W + rtx];
         float b0x, b0y, b1x, b1y;
         basis(nr.z, 1, 1, b0x, b0y, b1x, b1y);
         float ix, iy;
@@ -2089,11 +2244,30 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
                                : 0.f;
         }
         d.base[o] = prev_color;
-        d.out[o] = tex_ex_out(A, out.x, out.y);
+        d.out[o] = tex_ex_out(A, out.x, out.y, ol);
         d.ts[o] = time_step;
+    }
+    /* ADVECT_CHECK_FRAME_TO_KEY (:1643-1668): a long step, or a failed
+     * advection, may take the pixel's ribbon to a keyframe instead */
+    if ((flags & PT_ADVECT_CHECK_FRAME_TO_KEY) && (time_step > 1 || out.w == 0.f)) {
+        float4 fk = ftk_at(A, fwd ? 0 : 1, x, y);
+        int fts = (int)fk.x, fla = (int)fk.w;
+        if (fla >= 0 && (abs(fts) < time_step || out.w == 0.f)) {
+            out.x = fk.z + (float)x;
+            out.y = fk.y + (float)y;
+            ol = fla;
+            out.w = 1.f;
+            lum = 0.f;
+        }
+        float4 eb, eo;
+        ex_pair(ex_rsrc(A), A.ew, A.eh, out.x, out.y, eb, eo, lay(A, ol));
+        d.base[o] = eb;
+        d.out[o] = eo;
+        d.ts[o] = abs(fts);
     }
     if (out.w == 0.f) {
         out = make_float4(rnd.x, rnd.y, rnd.z, 0.f);
+        ol = (int)rnd.w;
         lum = 0.f;
         d.base[o] = make_float4(0, 0, 0, 0);
         d.out[o] = make_float4(0, 0, 0, 0);
@@ -2102,10 +2276,11 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_advect(PtArgs A, int fwd, int 
     float jit = A.jitter;
     out.x = out.x * (1 - jit) + frac.x * (jit);
     out.y = out.y * (1 - jit) + frac.y * (jit);
-    out.z = screen_to_ex_angle(A, (float)x, (float)y, out.x, out.y);
+    out.z = screen_to_ex_angle(A, (float)x, (float)y, out.x, out.y, ol);
     d.xyth[o] = out;
     d.sc[o] = make_float2(1.f, 1.f);
     if (A.lum_live) d.lum[o] = lum;
+    if (A.layered) d.la[o] = ol;
 }
 
 /* The candidate-independent exemplar samples of timeDerivativeTerms
@@ -2140,8 +2315,10 @@ __global__ __launch_bounds__(PT_RX *PT_RY) void k_time_table(PtArgs A, int k, fl
     }
 }
 
-/* mergeRandomlyKernel (texSynth_kernel.cu:1891-1936); the ribbon textures
- * are unbound without keyframes and read as zero. */
+/* mergeRandomlyKernel (texSynth_kernel.cu:1891-1936): the advected offset
+ * also wins where this direction's frame-to-key ribbon is valid and the
+ * other one is not (a context without ribbons binds zero texels, valid
+ * layer-0 ribbons, which leave that clause false). */
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int fwd, float weight, int force)
 {
     int x, y;
@@ -2151,14 +2328,17 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int f
     float4 a = d.xyth[o];
     float4 rnd = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
     float r = fwd ? rnd.x - floorf(rnd.x) : rnd.y - floorf(rnd.y);
-    if ((a.w > 0 || force) && r < weight) {
+    bool rib = (int)ftk_at(A, fwd ? 0 : 1, x, y).w >= 0 && !((int)ftk_at(A, fwd ? 1 : 0, x, y).w >= 0);
+    if (rib || ((a.w > 0 || force) && r < weight)) {
         A.work.xyth[o] = a;
         A.work.sc[o] = d.sc[o];
         if (A.lum_live) A.work.lum[o] = d.lum[o];
+        if (A.layered) A.work.la[o] = d.la[o];
     } else {
         A.work.xyth[o] = A.base.xyth[o];
         A.work.sc[o] = A.base.sc[o];
         lum_set(A, o, true);
+        if (A.layered) A.work.la[o] = A.base.la[o];
     }
 }
 
@@ -2175,13 +2355,153 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_residual(PtArgs A, int f
         A.work.xyth[o] = a;
         A.work.sc[o] = d.sc[o];
         if (A.lum_live) A.work.lum[o] = d.lum[o];
+        if (A.layered) A.work.la[o] = d.la[o];
     } else {
         float4 u = A.base.xyth[o];
         u.w = 0.f;
         A.work.xyth[o] = u;
         A.work.sc[o] = A.base.sc[o];
         lum_set(A, o, true);
+        if (A.layered) A.work.la[o] = A.base.la[o];
     }
+}
+
+/* mergeAdvectedKernel (texSynth_kernel.cu:1938-1976): both advected offsets
+ * valid -> one of them by the fractional random x, one valid -> it, none ->
+ * the fractional random texel as an offset (MERGE_FALLBACK_RANDOM) or the
+ * current base offset. */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_advected(PtArgs A, int fallback)
+{
+    int x, y;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
+    int o = y * A.W + x;
+    float4 f = A.adv[0].xyth[o], b = A.adv[1].xyth[o];
+    float4 rnd = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
+    /* randomFloat4's out parameter: the fractional parts (:207-213) */
+    float4 fr = make_float4(rnd.x - floorf(rnd.x), rnd.y - floorf(rnd.y), rnd.z - floorf(rnd.z),
+                            rnd.w - floorf(rnd.w));
+    int k = -1; /* advected set taken, -1: none */
+    if (f.w > 0 && b.w > 0) k = (fr.x < 0.5f) ? 0 : 1;
+    else if (f.w > 0) k = 0;
+    else if (b.w > 0) k = 1;
+    if (k >= 0) {
+        const PtAdv &d = A.adv[k];
+        A.work.xyth[o] = d.xyth[o];
+        A.work.sc[o] = d.sc[o];
+        if (A.lum_live) A.work.lum[o] = d.lum[o];
+        A.work.la[o] = d.la[o];
+    } else if (fallback == PT_MERGE_FALLBACK_RANDOM) {
+        A.work.xyth[o] = make_float4(fr.x, fr.y, fr.z, 0.f);
+        A.work.sc[o] = make_float2(1.f, 1.f);
+        lum_set(A, o, false);
+        A.work.la[o] = (int)fr.w;
+    } else {
+        A.work.xyth[o] = A.base.xyth[o];
+        A.work.sc[o] = A.base.sc[o];
+        lum_set(A, o, true);
+        A.work.la[o] = A.base.la[o];
+    }
+}
+
+/* copyFrameToKeyRibbonKernel (texSynth_kernel.cu:2061-2097): each pixel
+ * starts at the keyframe texel its ribbon reaches (hysteresis 1, scale
+ * (1, 1)), or at its random texel where the ribbon is invalid. */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_offsets_from_ribbon(PtArgs A, int look_forwards)
+{
+    int x, y;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
+    int o = y * A.W + x;
+    float4 rb = ftk_at(A, look_forwards ? 1 : 0, x, y);
+    int rl = (int)rb.w;
+    if (rl >= 0) {
+        float ex = rb.z + (float)x, ey = rb.y + (float)y;
+        A.work.xyth[o] = make_float4(ex, ey, screen_to_ex_angle(A, (float)x, (float)y, ex, ey, rl), 1.f);
+        A.work.la[o] = rl;
+    } else {
+        float4 r = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
+        A.work.xyth[o] = make_float4(r.x, r.y, r.z, 0.f);
+        A.work.la[o] = (int)r.w;
+    }
+    A.work.sc[o] = make_float2(1.f, 1.f);
+    lum_set(A, o, false);
+}
+
+__device__ __forceinline__ bool ribbon_valid(const pt_ribbonp &r) { return r.layer >= 0; }
+
+/* accumulateAdvectionKernel (texSynth_kernel.cu:1728-1826): the ribbon
+ * field `src` advected one step of time_step frames along the image
+ * velocities into `dst`.  Where the four source ribbons around the
+ * translated position are all valid, of one layer and within 5 px of each
+ * other the position is interpolated bilinearly, else the nearest one is
+ * taken; the step's displacement and time are then added.  Without
+ * `overwrite` a valid ribbon already in dst is replaced only by one of no
+ * longer time. */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_accumulate_ribbon(PtArgs A, const pt_ribbonp *src, pt_ribbonp *dst,
+                                                                   int fwd, int time_step, int overwrite)
+{
+    int x, y;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
+    int o = y * A.W + x;
+    pt_ribbonp out = {0.f, 0.f, 0, -1};
+    float tx, ty;
+    if (A.color[A.cur][o].w == 0.f || !translated_position(A, fwd, x, y, tx, ty)) {
+        if (overwrite) dst[o] = out;
+        return;
+    }
+    int ftx = (int)floorf(tx), fty = (int)floorf(ty), ctx = (int)ceilf(tx), cty = (int)ceilf(ty);
+    pt_ribbonp s00 = src[fty * A.W + ftx], s10 = src[fty * A.W + ctx];
+    pt_ribbonp s01 = src[cty * A.W + ftx], s11 = src[cty * A.W + ctx];
+    bool v00 = ribbon_valid(s00), v10 = ribbon_valid(s10), v01 = ribbon_valid(s01), v11 = ribbon_valid(s11);
+    if (!(v00 || v10 || v01 || v11)) {
+        if (overwrite) dst[o] = out;
+        return;
+    }
+    bool all_valid = v00 && v10 && v01 && v11;
+    /* the reference's chained comparison of the four layers, evaluated as
+     * C++ does: the bool of one comparison compared with the next layer */
+    bool same = ((int)((int)(s00.layer == s10.layer) == s01.layer)) == s11.layer;
+    const float th = 5.f;
+    bool diverges = fabsf(s00.x - s10.x) > th || fabsf(s00.x - s01.x) > th || fabsf(s00.x - s11.x) > th ||
+                    fabsf(s00.y - s10.y) > th || fabsf(s00.y - s01.y) > th || fabsf(s00.y - s11.y) > th;
+    if (!all_valid || !same || diverges) {
+        int rtx = (int)roundf(tx), rty = (int)roundf(ty);
+        if (ftx == rtx && fty == rty) out = s00;
+        else if (ftx + 1 == rtx && fty == rty) out = s10;
+        else if (ftx == rtx && fty + 1 == rty) out = s01;
+        else out = s11;
+    } else {
+        float wx = tx - (float)ftx, wy = ty - (float)fty;
+        float rx0 = s00.x * (1.f - wx) + s10.x * wx;
+        float ry0 = s00.y * (1.f - wx) + s10.y * wx;
+        float rx1 = s01.x * (1.f - wx) + s11.x * wx;
+        float ry1 = s01.y * (1.f - wx) + s11.y * wx;
+        out.x = rx0 * (1.f - wy) + rx1 * wy;
+        out.y = ry0 * (1.f - wy) + ry1 * wy;
+        out.time_step = s00.time_step;
+        out.layer = s00.layer;
+    }
+    out.x += (tx - (float)x);
+    out.y += (ty - (float)y);
+    out.time_step += time_step;
+    pt_ribbonp cur = dst[o];
+    if (overwrite || !ribbon_valid(cur) || fabsf((float)cur.time_step) >= fabsf((float)out.time_step)) dst[o] = out;
+}
+
+/* cumulativeAdvectionFieldToOutput (texSynth_kernel.cu:290-305): the
+ * canvas texel each valid ribbon reaches (clamped into the level-0-sized
+ * canvas) as the forward advected output colour, zero elsewhere. */
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_ribbon_output(PtArgs A, const pt_ribbonp *rib)
+{
+    int x, y;
+    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
+    int o = y * A.W + x;
+    pt_ribbonp r = rib[o];
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ribbon_valid(r)) {
+        int sx = clampi((int)((float)x + r.x), 0, A.W - 1), sy = clampi((int)((float)y + r.y), 0, A.H - 1);
+        v = A.canvas[sy * A.W + sx];
+    }
+    A.adv[0].out[o] = v;
 }
 
 /* downsampleAverage (downsample.cpp:41-160): one level, CH floats per
@@ -2233,6 +2553,19 @@ __global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_v(const float *src,
     ((V *)dst)[y * lw + x] = o;
 }
 
+/* initializeRibbonPyramid (imagePyramid.cpp:211-235): the top-left texel
+ * of each 2x2 block as a ribbon -- time step and layer through int, x and y
+ * halved -- packed back as (time_step, y, x, layer). */
+__global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_ribbon(const float4 *src, int pw, float4 *dst, int lw,
+                                                                   int lh)
+{
+    int x = blockIdx.x * PT_RX + threadIdx.x;
+    int y = blockIdx.y * PT_RY + threadIdx.y;
+    if (x >= lw || y >= lh) return;
+    float4 q = src[(y * 2) * pw + x * 2];
+    dst[y * lw + x] = make_float4((float)(int)q.x, q.y * 0.5f, q.z * 0.5f, (float)(int)q.w);
+}
+
 /* other element widths (the structure tensor, 3 floats) */
 __global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample(const float *src, int pw, float *dst, int lw, int lh,
                                                             int ch, int scale_mode)
@@ -2278,6 +2611,13 @@ __global__ void k_fill(float *dst, size_t n, float v)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = v;
+}
+
+/* RibbonP(): invalid (layer -1) */
+__global__ void k_fill_ribbon(pt_ribbonp *dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = pt_ribbonp{0.f, 0.f, 0, -1};
 }
 
 /* strucTensorToAngle (imageIO.h:136-139) per element */
@@ -2326,6 +2666,46 @@ __global__ void k_random_pyramid(const float4 *color, const float *iorient, int 
     out[y * w + x] = make_float4((float)ox + rx, (float)oy + ry, orientation, 0.0f);
 }
 
+/* The same with keyframes: the style of each pixel drawn first from
+ * counter (x, y, 51, 0), rand() % keys + first key (synthesisFrame.cpp:
+ * 107-110), its exemplar used for the trials and the orientation, the style
+ * stored in w. */
+struct PtStyleSet {
+    const float4 *eout[PT_MAX_STYLE_LAYERS];
+    const float *eori[PT_MAX_STYLE_LAYERS];
+    int ew[PT_MAX_STYLE_LAYERS], eh[PT_MAX_STYLE_LAYERS];
+};
+
+__global__ void k_random_pyramid_keys(const float4 *color, const float *iorient, int w, int h, PtStyleSet S,
+                                      int key_first, int num_keys, int transp, int orient_en, uint32_t key0,
+                                      uint32_t key1, float4 *out)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    uint32_t key[2] = {key0, key1};
+    uint32_t ctr[4] = {(uint32_t)x, (uint32_t)y, 51u, 0u}, o[4];
+    pt_philox4x32_10(ctr, key, o);
+    const int st = pt_rand31(o[0]) % num_keys + key_first;
+    const int ew = S.ew[st], eh = S.eh[st];
+    const float4 *eout = S.eout[st];
+    int ox, oy, trial = 0;
+    float ia = color[y * w + x].w;
+    do {
+        ctr[2] = (uint32_t)trial;
+        pt_philox4x32_10(ctr, key, o);
+        ox = pt_rand31(o[0]) % (ew - 1);
+        oy = pt_rand31(o[1]) % (eh - 1);
+        trial++;
+    } while (!transp && ia > 0.99f && eout[oy * ew + ox].w < 0.99f && trial < 50);
+    float orientation = orient_en ? iorient[y * w + x] - S.eori[st][oy * ew + ox] : 0.f;
+    ctr[2] = 50u;
+    pt_philox4x32_10(ctr, key, o);
+    float rx = pt_rand_unit(o[0]);
+    float ry = pt_rand_unit(o[1]);
+    out[y * w + x] = make_float4((float)ox + rx, (float)oy + ry, orientation, (float)st);
+}
+
 /* PatchXF (36 B, types.h:98-114) <-> SoA planes for the host boundary. */
 struct PatchXF36 {
     float x, y, theta, hysteresis;
@@ -2340,7 +2720,7 @@ __global__ void k_pack_nnf(PtNnf s, int W, int lw, int lh, PatchXF36 *d)
     if (x >= lw || y >= lh) return;
     float4 v = s.xyth[y * W + x];
     float2 c = s.sc[y * W + x];
-    PatchXF36 p = {v.x, v.y, v.z, v.w, 0, c.x, c.y, s.lum ? s.lum[y * W + x] : 0.f, 0.f};
+    PatchXF36 p = {v.x, v.y, v.z, v.w, s.la ? s.la[y * W + x] : 0, c.x, c.y, s.lum ? s.lum[y * W + x] : 0.f, 0.f};
     d[y * lw + x] = p;
 }
 
@@ -2353,6 +2733,7 @@ __global__ void k_unpack_nnf(const PatchXF36 *s, int lw, int lh, PtNnf d, int W)
     d.xyth[y * W + x] = make_float4(p.x, p.y, p.theta, p.hysteresis);
     d.sc[y * W + x] = make_float2(p.su, p.sv);
     if (d.lum) d.lum[y * W + x] = p.ly;
+    if (d.la) d.la[y * W + x] = p.layer;
 }
 
 /* ------------------------------------------------------------------ */
@@ -2753,9 +3134,10 @@ __global__ __launch_bounds__(PT_UP_THREADS) void km_scatter_up(PT_MULTI_PARAMS)
 /* ------------------------------------------------------------------ */
 /* launchers                                                           */
 /* ------------------------------------------------------------------ */
-/* the extended patch-distance terms (coherence, LAB, object IDs) run in the per-pixel
- * kernels' RT = -1 variant */
-static bool ext_terms(const PtArgs &a) { return a.coh_w > 0.f || a.lab || a.obj_w > 0.f; }
+/* the extended patch-distance terms (coherence, LAB, object IDs) and style
+ * layers (keyframe interpolation) run in the per-pixel kernels' RT = -1
+ * variant */
+static bool ext_terms(const PtArgs &a) { return a.coh_w > 0.f || a.lab || a.obj_w > 0.f || a.layered; }
 
 static dim3 grid_for(int w, int h) { return dim3((w + PT_BX - 1) / PT_BX, (h + PT_BY - 1) / PT_BY); }
 static const dim3 kBlock(PT_BX, PT_BY);
@@ -2904,8 +3286,9 @@ void pt_tap_configure(PtArgs &a, int evals)
 int pt_shape_valid(int op, int shape, const PtArgs &a)
 {
     if (shape < 0 || shape >= PT_SHAPE_COUNT) return 0;
-    /* the coherence term, the LAB colour space and the object-ID term are
-     * evaluated by the per-pixel in-order kernels only (their RT < 0 variant) */
+    /* the coherence term, the LAB colour space, the object-ID term and style
+     * layers are evaluated by the per-pixel in-order kernels only (their
+     * RT < 0 variant) */
     if (ext_terms(a) && shape != PT_SHAPE_PIXEL) return 0;
     if (shape == PT_SHAPE_TAP) return tap_pixels(op, a) > 0;
     if (op == PT_OP_RESIDUAL || op == PT_OP_SYNC_CANVAS) return shape == PT_SHAPE_PIXEL;
@@ -3138,6 +3521,67 @@ hipError_t pt_launch_merge_randomly(const PtArgs &a, int forwards, float weight,
 hipError_t pt_launch_merge_residual(const PtArgs &a, int forwards, hipStream_t s)
 {
     hipLaunchKernelGGL(k_merge_residual, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, forwards);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_merge_advected(const PtArgs &a, int fallback, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_merge_advected, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, fallback);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_offsets_from_ribbon(const PtArgs &a, int look_forwards, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_offsets_from_ribbon, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, look_forwards);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_accumulate_ribbon(const PtArgs &a, const pt_ribbonp *src, pt_ribbonp *dst, int forwards,
+                                       int time_step, int overwrite, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_accumulate_ribbon, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, src, dst, forwards, time_step,
+                       overwrite);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_ribbon_output(const PtArgs &a, const pt_ribbonp *rib, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ribbon_output, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, rib);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_downsample_ribbon(const float *src, int pw, float *dst, int lw, int lh, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_downsample_ribbon, grid_rows(lw, lh), kRowBlock, 0, s, (const float4 *)src, pw,
+                       (float4 *)dst, lw, lh);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_fill_ribbon(pt_ribbonp *dst, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_ribbon, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t pt_launch_random_pyramid_keys(const float4 *color, const float *iorient, int w, int h, int nstyles,
+                                         const float4 *const *eouts, const float *const *eorients, const int *ews,
+                                         const int *ehs, int key_first, int num_keys, int transp, int orient_en,
+                                         uint32_t key0, uint32_t key1, float4 *out, hipStream_t s)
+{
+    if (nstyles < 1 || nstyles > PT_MAX_STYLE_LAYERS || num_keys < 1 || key_first < 0 ||
+        key_first + num_keys > nstyles)
+        return hipErrorInvalidValue;
+    PtStyleSet S{};
+    for (int k = 0; k < nstyles; k++) {
+        S.eout[k] = eouts[k];
+        S.eori[k] = eorients[k];
+        S.ew[k] = ews[k];
+        S.eh[k] = ehs[k];
+    }
+    dim3 b(16, 16), g((w + 15) / 16, (h + 15) / 16);
+    hipLaunchKernelGGL(k_random_pyramid_keys, g, b, 0, s, color, iorient, w, h, S, key_first, num_keys, transp,
+                       orient_en, key0, key1, out);
     return hipGetLastError();
 }
 

@@ -50,6 +50,7 @@ struct DevAdv {
     float *res = nullptr;
     int32_t *ts = nullptr;
     float *lum = nullptr;
+    int32_t *la = nullptr;
 };
 
 /* `id` (object IDs) exists only while the object-ID term is on */
@@ -133,6 +134,13 @@ struct pt_ctx {
      * lum_shift_enable (or loaded nonzero shifts) until the working buffers
      * are reset; before that every shift is 0 */
     bool lum_live = false;
+    /* style layers: once live, every operator maintains the layer planes
+     * (PtNnf::la) and the searches run their layered variant.  Set by the
+     * first operator that runs with several exemplar layers, keyframe
+     * interpolation or a keyframe index, by keyframe fields and ribbons, and
+     * by offsets loaded with a layer; cleared with the working buffers (the
+     * planes are zero until then). */
+    bool layered_live = false;
     bool no_early_exit = false;            /* force the plain candidate kernels (tests) */
     /* Early rejection lengthens each lane's dependent chain; it pays only on
      * levels with enough waves to hide the latency. */
@@ -167,6 +175,17 @@ struct pt_ctx {
     int32_t *ex_id = nullptr;
     size_t ex_cap = 0;
     int ew = 0, eh = 0;
+    int nl = 1; /* exemplar style layers */
+    /* keyframe interpolation: cumulative key-to-key fields (cl layers of
+     * cw x ch; one zero texel until initialize_cumulative_fields), the
+     * frame-to-key ribbons (fw x fh; one zero texel until uploaded) and the
+     * ribbon field pair (W x H, allocated invalid on first use) */
+    float4 *cumul[2] = {nullptr, nullptr};
+    int cw = 1, ch = 1, cl = 1;
+    float4 *ftk[2] = {nullptr, nullptr};
+    size_t ftk_cap[2] = {0, 0};
+    int fw[2] = {1, 1}, fh[2] = {1, 1};
+    pt_ribbonp *rib_base = nullptr, *rib_work = nullptr;
     uint32_t scatter_calls = 0;
     uint64_t pde = 0;
     /* schedule results */
@@ -254,8 +273,10 @@ static int check_supported(pt_ctx *c, const pt_params *p)
         return set_err(c, PT_ERR_INVALID_ARG, "coherence needs ratio range > 1 and angle range > 0");
     if (p->colorspace != 0 && p->colorspace != 1)
         return set_err(c, PT_ERR_UNSUPPORTED, "colourspace must be RGB (0) or LAB (1)");
-    if (p->interpolate_key_frame)
-        return set_err(c, PT_ERR_UNSUPPORTED, "keyframe interpolation is not supported");
+    if (p->interpolate_key_frame &&
+        (p->first_key_index < 0 || p->last_key_index < p->first_key_index || p->last_key_index >= PT_MAX_STYLE_LAYERS))
+        return set_err(c, PT_ERR_INVALID_ARG, "keyframe interpolation needs 0 <= first_key_index <= last_key_index < %d",
+                       PT_MAX_STYLE_LAYERS);
     /* The offsets-histogram term (histogramTerm, texSynth_kernel.cu:486-494)
      * reads a histogram that stays all zero: its accumulation is commented out
      * (:258), the buffer is reset before it and the two blur passes of zeros
@@ -415,6 +436,10 @@ static void free_working(pt_ctx *c)
     dfree(c->work.sc);
     dfree(c->base.lum);
     dfree(c->work.lum);
+    dfree(c->base.la);
+    dfree(c->work.la);
+    dfree(c->rib_base);
+    dfree(c->rib_work);
     dfree(c->res_base);
     dfree(c->res_work);
     dfree(c->canvas);
@@ -427,6 +452,7 @@ static void free_working(pt_ctx *c)
         dfree(c->adv[k].res);
         dfree(c->adv[k].ts);
         dfree(c->adv[k].lum);
+        dfree(c->adv[k].la);
         dfree(c->color[k]);
         dfree(c->dt[k]);
         dfree(c->velb[k]);
@@ -456,6 +482,10 @@ extern "C" int pt_destroy(pt_ctx *c)
     dfree(c->ex_orient);
     dfree(c->ex_dt);
     dfree(c->ex_id);
+    for (int k = 0; k < 2; k++) {
+        dfree(c->cumul[k]);
+        dfree(c->ftk[k]);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
     return PT_OK;
@@ -703,6 +733,47 @@ extern "C" int pt_setup(pt_ctx *c, const pt_params *p)
     return PT_OK;
 }
 
+/* Frame-to-key ribbon k back to one zero texel (an unbound texture). */
+static int ftk_reset(pt_ctx *c, int k)
+{
+    if (c->ftk_cap[k] < 1) {
+        dfree(c->ftk[k]);
+        RC_CHECK(dalloc(c, &c->ftk[k], 1));
+        c->ftk_cap[k] = 1;
+    }
+    RC_CHECK(zero_fill(c, c->ftk[k], sizeof(float4)));
+    c->fw[k] = c->fh[k] = 1;
+    return PT_OK;
+}
+
+/* Cumulative fields default to one zero texel (unbound). */
+static int cumul_defaults(pt_ctx *c)
+{
+    for (int k = 0; k < 2; k++) {
+        if (c->cumul[k]) continue;
+        RC_CHECK(dalloc(c, &c->cumul[k], 1));
+        RC_CHECK(zero_fill(c, c->cumul[k], sizeof(float4)));
+        c->cw = c->ch = c->cl = 1;
+    }
+    return PT_OK;
+}
+
+/* The ribbon field pair, allocated on first use with every ribbon invalid
+ * (RibbonP()). */
+static int ribbons_ready(pt_ctx *c)
+{
+    if (c->rib_base) return PT_OK;
+    if (c->rec) return set_err(c, PT_ERR_STATE, "ribbon fields must exist before recording");
+    size_t n = (size_t)c->W * c->H;
+    RC_CHECK(dalloc(c, &c->rib_base, n));
+    RC_CHECK(dalloc(c, &c->rib_work, n));
+    pt_ribbonp *a = c->rib_base, *b = c->rib_work;
+    return issue(c, "ribbon fill", [=](hipStream_t s) {
+        hipError_t e = pt_launch_fill_ribbon(a, n, s);
+        return e != hipSuccess ? e : pt_launch_fill_ribbon(b, n, s);
+    });
+}
+
 extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
 {
     if ((int64_t)iw * ih > PT_MAX_IMAGE_PIXELS)
@@ -726,6 +797,10 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
     RC_CHECK(dalloc(c, &c->work.sc, n));
     RC_CHECK(dalloc(c, &c->base.lum, n));
     RC_CHECK(dalloc(c, &c->work.lum, n));
+    RC_CHECK(dalloc(c, &c->base.la, n));
+    RC_CHECK(dalloc(c, &c->work.la, n));
+    RC_CHECK(zero_fill(c, c->base.la, n * sizeof(int32_t)));
+    RC_CHECK(zero_fill(c, c->work.la, n * sizeof(int32_t)));
     RC_CHECK(dalloc(c, &c->res_base, n));
     RC_CHECK(dalloc(c, &c->res_work, n));
     RC_CHECK(dalloc(c, &c->canvas, n));
@@ -738,6 +813,8 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
         RC_CHECK(dalloc(c, &c->adv[k].res, n));
         RC_CHECK(dalloc(c, &c->adv[k].ts, n));
         RC_CHECK(dalloc(c, &c->adv[k].lum, n));
+        RC_CHECK(dalloc(c, &c->adv[k].la, n));
+        RC_CHECK(zero_fill(c, c->adv[k].la, n * sizeof(int32_t)));
         RC_CHECK(dalloc(c, &c->color[k], n));
         RC_CHECK(dalloc(c, &c->dt[k], n));
         RC_CHECK(dalloc(c, &c->velb[k], n));
@@ -750,6 +827,10 @@ extern "C" int pt_initialize(pt_ctx *c, int iw, int ih, int sw, int sh)
     RC_CHECK(dalloc(c, &c->img_id, n));
     RC_CHECK(zero_fill(c, c->img_id, sizeof(int32_t)));
     c->idw = c->idh = 1;
+    /* unbound frame-to-key ribbons read as one zero texel */
+    for (int k = 0; k < 2; k++) RC_CHECK(ftk_reset(c, k));
+    RC_CHECK(cumul_defaults(c));
+    c->layered_live = false;
     c->ow = c->oh = c->scw = c->sch = c->rw = c->rh = 0;
     c->off_diff = Rect();
     c->res_diff = Rect();
@@ -765,15 +846,17 @@ extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
 {
     /* the gathers address the interleaved 32-byte texels with 32-bit byte
      * offsets (ex_rsrc, ex_pair); checked before anything else */
-    if ((int64_t)w * h > PT_MAX_EXEMPLAR_TEXELS)
-        return set_err(c, PT_ERR_UNSUPPORTED, "exemplar larger than PT_MAX_EXEMPLAR_TEXELS");
+    if ((int64_t)w * h * (num_layers > 1 ? num_layers : 1) > PT_MAX_EXEMPLAR_TEXELS)
+        return set_err(c, PT_ERR_UNSUPPORTED, "exemplar layers larger than PT_MAX_EXEMPLAR_TEXELS");
     if (!c || w < 1 || h < 1) return set_err(c, PT_ERR_INVALID_ARG, "bad exemplar dimensions");
-    if (num_layers < 1 || num_layers > 4) return set_err(c, PT_ERR_INVALID_ARG, "num_layers must be in [1, 4]");
+    if (num_layers < 1 || num_layers > PT_MAX_STYLE_LAYERS)
+        return set_err(c, PT_ERR_INVALID_ARG, "num_layers must be in [1, %d]", PT_MAX_STYLE_LAYERS);
     HIP_CHECK(c, hipSetDevice(c->device));
     res_invalidate(c);
     tt_invalidate(c);
-    if (c->ex && c->ew == w && c->eh == h) return PT_OK;
-    size_t n = (size_t)w * h;
+    if (c->ex && c->ew == w && c->eh == h && c->nl == num_layers) return PT_OK;
+    /* style layer l occupies texels [l * w * h, (l + 1) * w * h) */
+    size_t n = (size_t)w * h * num_layers;
     if (n > c->ex_cap) {
         if (c->rec) return set_err(c, PT_ERR_STATE, "exemplar buffers must be sized before recording");
         HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -795,6 +878,29 @@ extern "C" int pt_initialize_exemplars(pt_ctx *c, int w, int h, int num_layers)
     }
     c->ew = w;
     c->eh = h;
+    c->nl = num_layers;
+    return sync_stream(c);
+}
+
+/* TexSynth::initialize_cumulativeFields (texSynth_kernel.cu:2136-2140):
+ * zero-filled layered fields of w x h texels. */
+extern "C" int pt_initialize_cumulative_fields(pt_ctx *c, int w, int h, int num_layers)
+{
+    if (!c || w < 1 || h < 1 || num_layers < 1 || num_layers > PT_MAX_STYLE_LAYERS)
+        return set_err(c, PT_ERR_INVALID_ARG, "bad cumulative field dimensions");
+    HIP_CHECK(c, hipSetDevice(c->device));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    size_t n = (size_t)w * h * num_layers;
+    for (int k = 0; k < 2; k++) {
+        dfree(c->cumul[k]);
+        RC_CHECK(dalloc(c, &c->cumul[k], n));
+        RC_CHECK(zero_fill(c, c->cumul[k], n * sizeof(float4)));
+    }
+    c->cw = w;
+    c->ch = h;
+    c->cl = num_layers;
+    c->layered_live = true;
+    res_invalidate(c);
     return sync_stream(c);
 }
 
@@ -810,6 +916,7 @@ static int reset_advected(pt_ctx *c)
         RC_CHECK(zero_fill(c, c->adv[k].res, n * sizeof(float)));
         RC_CHECK(zero_fill(c, c->adv[k].ts, n * sizeof(int32_t)));
         RC_CHECK(zero_fill(c, c->adv[k].lum, n * sizeof(float)));
+        RC_CHECK(zero_fill(c, c->adv[k].la, n * sizeof(int32_t)));
         c->adv_written[k] = false;
     }
     tt_invalidate(c);
@@ -846,6 +953,7 @@ static int pyramid_rebuild(pt_ctx *c, pt_pyramid *p, int scale_mode)
         int pr = prof_begin(c, PK_PYRAMID, (uint64_t)lw * lh, (uint64_t)lw * lh * ch * 4 * 5);
         RC_CHECK(issue(c, "downsample", [=](hipStream_t s) {
             if (vote) return pt_launch_downsample_vote((const int *)src, pw, (int *)dst, lw, lh, s);
+            if (scale_mode == PT_DOWNSAMPLE_RIBBON) return pt_launch_downsample_ribbon(src, pw, dst, lw, lh, s);
             return pt_launch_downsample(src, pw, dst, lw, lh, ch, scale_mode, s);
         }));
         prof_end(c, pr);
@@ -895,6 +1003,8 @@ extern "C" int pt_pyramid_create(pt_ctx *c, pt_pyramid **out, int elem, int w, i
                                  const void *host0, float defv)
 {
     if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
+    if (scale_mode == PT_DOWNSAMPLE_RIBBON && elem != PT_ELEM_F32X4)
+        return set_err(c, PT_ERR_INVALID_ARG, "ribbon pyramids are F32X4");
     HIP_CHECK(c, hipSetDevice(c->device));
     pt_pyramid *p = nullptr;
     RC_CHECK(pyramid_alloc(c, &p, elem, w, h));
@@ -975,14 +1085,11 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
 {
     if (!c || !p || level < 0 || level >= p->levels) return set_err(c, PT_ERR_INVALID_ARG, "bad upload");
     if (!c->base.xyth) return set_err(c, PT_ERR_STATE, "not initialized");
-    if (style_layer < 0 || style_layer > 3) return set_err(c, PT_ERR_INVALID_ARG, "style layer must be in [0, 3]");
-    /* Style layers above 0 are never sampled without keyframe interpolation:
-     * random initialisation, scatter and the average candidate all use style
-     * 0 (synthesisFrame.cpp:106-110, texSynth_kernel.cu:925,939) and every
-     * other offset inherits its layer, so their uploads are accepted and
-     * have nothing to bind. */
-    if (style_layer > 0 && layer >= PT_LAYER_EXEMPLAR_BASE && layer <= PT_LAYER_EXEMPLAR_OBJECT_IDS) return PT_OK;
+    if (style_layer < 0 || style_layer >= PT_MAX_STYLE_LAYERS)
+        return set_err(c, PT_ERR_INVALID_ARG, "style layer must be in [0, %d]", PT_MAX_STYLE_LAYERS - 1);
     int w = p->lw[level], h = p->lh[level];
+    /* first texel of the exemplar style layer */
+    const size_t exl = (size_t)style_layer * c->ew * c->eh;
     const float *src = p->data + p->off[level];
     int prev = (c->cur + 1) % 2;
     res_invalidate(c);
@@ -1042,8 +1149,8 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
         c->idh = h;
         return tight(c->img_id, 1);
     case PT_LAYER_EXEMPLAR_OBJECT_IDS: {
-        if (p->elem != PT_ELEM_I32 || !c->ex || w > c->ew || h > c->eh) break;
-        int32_t *dst = c->ex_id;
+        if (p->elem != PT_ELEM_I32 || !c->ex || w > c->ew || h > c->eh || style_layer >= c->nl) break;
+        int32_t *dst = c->ex_id + exl;
         int ew = c->ew;
         return issue(c, "exemplar ID upload", [=](hipStream_t s) {
             return hipMemcpy2DAsync(dst, (size_t)ew * sizeof(int32_t), src, (size_t)w * sizeof(int32_t),
@@ -1052,9 +1159,9 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
     }
     case PT_LAYER_EXEMPLAR_BASE:
     case PT_LAYER_EXEMPLAR_OUTPUT:
-        if (p->elem != 4 || !c->ex || w > c->ew || h > c->eh) break;
+        if (p->elem != 4 || !c->ex || w > c->ew || h > c->eh || style_layer >= c->nl) break;
     {
-        float4 *ex = c->ex;
+        float4 *ex = c->ex + 2 * exl;
         int ew = c->ew, slot = layer == PT_LAYER_EXEMPLAR_BASE ? 0 : 1;
         return issue(c, "exemplar upload", [=](hipStream_t s) {
             return pt_launch_interleave((const float4 *)src, w, h, ex, ew, slot, s);
@@ -1062,13 +1169,45 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
     }
     case PT_LAYER_EXEMPLAR_ORIENTATION:
     case PT_LAYER_EXEMPLAR_DIST_TRANS: {
-        if (p->elem != 1 || !c->ex || w > c->ew || h > c->eh) break;
-        float *dst = layer == PT_LAYER_EXEMPLAR_ORIENTATION ? c->ex_orient : c->ex_dt;
+        if (p->elem != 1 || !c->ex || w > c->ew || h > c->eh || style_layer >= c->nl) break;
+        float *dst = (layer == PT_LAYER_EXEMPLAR_ORIENTATION ? c->ex_orient : c->ex_dt) + exl;
         int ew = c->ew;
         return issue(c, "exemplar guide upload", [=](hipStream_t s) {
             return hipMemcpy2DAsync(dst, (size_t)ew * sizeof(float), src, (size_t)w * sizeof(float),
                                     (size_t)w * sizeof(float), h, hipMemcpyDeviceToDevice, s);
         });
+    }
+    case PT_LAYER_CUMUL_FIELD_B:
+    case PT_LAYER_CUMUL_FIELD_F: {
+        /* into style layer `style_layer` of the cumulative field */
+        int k = layer == PT_LAYER_CUMUL_FIELD_B ? 0 : 1;
+        if (p->elem != 4 || w > c->cw || h > c->ch || style_layer >= c->cl) break;
+        float4 *dst = c->cumul[k] + (size_t)style_layer * c->cw * c->ch;
+        int cw = c->cw;
+        c->layered_live = true;
+        return issue(c, "cumulative field upload", [=](hipStream_t s) {
+            return hipMemcpy2DAsync(dst, (size_t)cw * sizeof(float4), src, (size_t)w * sizeof(float4),
+                                    (size_t)w * sizeof(float4), h, hipMemcpyDeviceToDevice, s);
+        });
+    }
+    case PT_LAYER_FRAME_TO_KEY_RIBBON_B:
+    case PT_LAYER_FRAME_TO_KEY_RIBBON_F: {
+        /* a texture of the level's size */
+        int k = layer == PT_LAYER_FRAME_TO_KEY_RIBBON_B ? 0 : 1;
+        if (p->elem != 4) break;
+        size_t need = (size_t)w * h;
+        if (need > c->ftk_cap[k]) {
+            if (c->rec) return set_err(c, PT_ERR_STATE, "ribbon textures must be sized before recording");
+            HIP_CHECK(c, hipStreamSynchronize(c->stream));
+            dfree(c->ftk[k]);
+            c->ftk_cap[k] = 0;
+            RC_CHECK(dalloc(c, &c->ftk[k], need));
+            c->ftk_cap[k] = need;
+        }
+        c->fw[k] = w;
+        c->fh[k] = h;
+        c->layered_live = true;
+        return tight(c->ftk[k], 4);
     }
     default:
         return set_err(c, PT_ERR_UNSUPPORTED, "layer %d not supported", layer);
@@ -1125,6 +1264,54 @@ extern "C" int pt_create_random_pyramid(pt_ctx *c, int level, int frame, const p
     return sync_stream(c);
 }
 
+/* createRandomPyramid with keyframes (synthesisFrame.cpp:102-143): each
+ * pixel draws its style among the keys first_key_index..last_key_index and
+ * samples that style's exemplar; the texel's w holds the style. */
+extern "C" int pt_create_random_pyramid_keys(pt_ctx *c, int level, int frame, const pt_pyramid *icol,
+                                             const pt_pyramid *iori, int num_styles,
+                                             const pt_pyramid *const *eouts, const pt_pyramid *const *eoris)
+{
+    if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
+    if (!icol || !iori || !eouts || !eoris) return set_err(c, PT_ERR_INVALID_ARG, "NULL pyramid");
+    if (!c->base.xyth || !c->have_params) return set_err(c, PT_ERR_STATE, "not initialized");
+    if (num_styles < 1 || num_styles > PT_MAX_STYLE_LAYERS)
+        return set_err(c, PT_ERR_INVALID_ARG, "num_styles must be in [1, %d]", PT_MAX_STYLE_LAYERS);
+    if (icol->elem != 4 || iori->elem != 1) return set_err(c, PT_ERR_INVALID_ARG, "pyramid kinds");
+    if (level < 0 || level >= icol->levels || level >= iori->levels) return set_err(c, PT_ERR_INVALID_ARG, "level out of range");
+    int w = icol->lw[level], h = icol->lh[level];
+    if (iori->lw[level] != w || iori->lh[level] != h) return set_err(c, PT_ERR_INVALID_ARG, "pyramid sizes differ");
+    if (w > c->W || h > c->H) return set_err(c, PT_ERR_INVALID_ARG, "level larger than image");
+    int k0 = c->p.first_key_index, nk = c->p.last_key_index - c->p.first_key_index + 1;
+    if (nk < 1 || k0 < 0 || k0 + nk > num_styles) return set_err(c, PT_ERR_INVALID_ARG, "keys outside the styles");
+    const float4 *eo[PT_MAX_STYLE_LAYERS];
+    const float *er[PT_MAX_STYLE_LAYERS];
+    int ews[PT_MAX_STYLE_LAYERS], ehs[PT_MAX_STYLE_LAYERS];
+    for (int k = 0; k < num_styles; k++) {
+        const pt_pyramid *a = eouts[k], *b = eoris[k];
+        if (!a || !b || a->elem != 4 || b->elem != 1 || level >= a->levels || level >= b->levels)
+            return set_err(c, PT_ERR_INVALID_ARG, "style %d: pyramid kind or level", k);
+        if (a->lw[level] != b->lw[level] || a->lh[level] != b->lh[level])
+            return set_err(c, PT_ERR_INVALID_ARG, "style %d: pyramid sizes differ", k);
+        if (a->lw[level] < 2 || a->lh[level] < 2) return set_err(c, PT_ERR_INVALID_ARG, "exemplar level too small");
+        eo[k] = (const float4 *)(a->data + a->off[level]);
+        er[k] = b->data + b->off[level];
+        ews[k] = a->lw[level];
+        ehs[k] = a->lh[level];
+    }
+    HIP_CHECK(c, hipSetDevice(c->device));
+    const float4 *ic = (const float4 *)(icol->data + icol->off[level]);
+    const float *io = iori->data + iori->off[level];
+    int transp = c->p.transparency_ok, oe = c->p.orientation_enabled;
+    uint32_t seed = c->p.seed, k1 = (uint32_t)(frame * level);
+    float4 *out = c->random;
+    HIP_CHECK(c, pt_launch_random_pyramid_keys(ic, io, w, h, num_styles, eo, er, ews, ehs, k0, nk, transp, oe, seed,
+                                               k1, out, c->stream));
+    c->rw = w;
+    c->rh = h;
+    c->layered_live = true;
+    return sync_stream(c);
+}
+
 /* ------------------------------------------------------------------ */
 /* operators                                                           */
 /* ------------------------------------------------------------------ */
@@ -1138,6 +1325,7 @@ static int need_level(pt_ctx *c, int level)
     if (level < 0 || level > 30 || (c->W >> level) < 1 || (c->H >> level) < 1)
         return set_err(c, PT_ERR_INVALID_ARG, "bad level %d", level);
     if (c->ow == 0 || c->scw == 0) return set_err(c, PT_ERR_STATE, "orientation/scale not uploaded");
+    if (c->nl > 1 || c->p.interpolate_key_frame) c->layered_live = true;
     return PT_OK;
 }
 
@@ -1232,6 +1420,22 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.ex_dt = c->ex_dt;
     a.img_id = c->img_id;
     a.ex_id = c->ex_id;
+    a.layered = c->layered_live ? 1 : 0;
+    a.nl = c->nl;
+    a.keys = p.interpolate_key_frame ? 1 : 0;
+    a.key_first = p.first_key_index;
+    a.key_last = p.last_key_index;
+    a.key_index = -1;
+    for (int k = 0; k < 2; k++) {
+        a.adv[k].la = c->adv[k].la;
+        a.cumul[k] = c->cumul[k];
+        a.ftk[k] = c->ftk[k];
+        a.fw[k] = c->fw[k];
+        a.fh[k] = c->fh[k];
+    }
+    a.cw = c->cw;
+    a.ch = c->ch;
+    a.cl = c->cl;
     return a;
 }
 
@@ -1240,13 +1444,16 @@ static int copy_nnf_rect(pt_ctx *c, const PtNnf &dst, const PtNnf &src, Rect r)
 {
     if (r.empty()) return PT_OK;
     const size_t W = c->W;
-    const bool lum = c->lum_live;
+    const bool lum = c->lum_live, lay = c->layered_live;
     return issue(c, "offset copy", [=](hipStream_t s) {
         hipError_t e = hipMemcpy2DAsync(dst.xyth, W * sizeof(float4), src.xyth, W * sizeof(float4),
                                         (size_t)r.w * sizeof(float4), r.h, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess && lum)
             e = hipMemcpy2DAsync(dst.lum, W * sizeof(float), src.lum, W * sizeof(float), (size_t)r.w * sizeof(float),
                                  r.h, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && lay)
+            e = hipMemcpy2DAsync(dst.la, W * sizeof(int32_t), src.la, W * sizeof(int32_t),
+                                 (size_t)r.w * sizeof(int32_t), r.h, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
         return hipMemcpy2DAsync(dst.sc, W * sizeof(float2), src.sc, W * sizeof(float2), (size_t)r.w * sizeof(float2),
                                 r.h, hipMemcpyDeviceToDevice, s);
@@ -1277,7 +1484,7 @@ static int copy_base_to_working(pt_ctx *c)
 static int time_tables(pt_ctx *c, int level, PtArgs &a)
 {
     a.tt[0] = a.tt[1] = nullptr;
-    if (!a.time_active) return PT_OK;
+    if (!a.time_active || a.layered) return PT_OK; /* tables hold layer-0 samples */
     if (!(a.tdi > 0.f || a.tdo > 0.f || a.tcw > 0.f)) return PT_OK;
     const int64_t taps = (int64_t)(2 * a.r + 1) * (2 * a.r + 1);
     if (taps > c->tt_max_taps) return PT_OK;
@@ -1366,11 +1573,13 @@ static int randomize_offsets(pt_ctx *c, int level)
     return sync_buffers(c, level, 0);
 }
 
-static int propagate(pt_ctx *c, int level, int jump)
+static int propagate(pt_ctx *c, int level, int jump, int key_index = -1)
 {
+    if (key_index >= 0) c->layered_live = true;
     PtArgs a = make_args(c, level);
     RC_CHECK(time_tables(c, level, a));
     a.jump = jump;
+    a.key_index = key_index;
     int pr = prof_begin(c, PK_PROPAGATE, (uint64_t)a.lw * a.lh, PB_NNF * a.lw * a.lh);
     RC_CHECK(issue_op(c, PT_OP_PROPAGATE, a));
     prof_end(c, pr);
@@ -1390,17 +1599,22 @@ static int scatter_samples(int ew, int eh, int samples)
     return n;
 }
 
-static int scatter(pt_ctx *c, int level)
+static int scatter(pt_ctx *c, int level, int key_index = -1)
 {
+    if (key_index >= 0) c->layered_live = true;
     PtArgs a = make_args(c, level);
     RC_CHECK(time_tables(c, level, a));
     a.call = c->scatter_calls++;
+    a.key_index = key_index;
+    /* with keyframes the samples are shared by the keys
+     * (texSynth_kernel.cu:765) and there is no averaged candidate */
+    if (a.keys) a.samples /= (a.key_last - a.key_first + 1);
     a.nscat = scatter_samples(c->ew, c->eh, a.samples);
     int pr = prof_begin(c, PK_SCATTER, (uint64_t)a.lw * a.lh, PB_NNF * a.lw * a.lh);
     RC_CHECK(issue_op(c, PT_OP_SCATTER, a));
     prof_end(c, pr);
     c->off_diff.grow(a.lw, a.lh);
-    c->pde += (uint64_t)a.lw * a.lh * (uint64_t)(scatter_samples(c->ew, c->eh, a.samples) + 1);
+    c->pde += (uint64_t)a.lw * a.lh * (uint64_t)(scatter_samples(c->ew, c->eh, a.samples) + (a.keys ? 0 : 1));
     return sync_buffers(c, level, 0);
 }
 
@@ -1431,6 +1645,7 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
 {
     if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
     res_invalidate(c);
+    if (flags & PT_ADVECT_CHECK_FRAME_TO_KEY) c->layered_live = true;
     PtArgs a = make_args(c, level);
     int pr = prof_begin(c, PK_ADVECT, (uint64_t)a.lw * a.lh, PB_ADVECT * a.lw * a.lh);
     RC_CHECK(issue(c, "advect", [=](hipStream_t s) { return pt_launch_advect(a, forwards, time_step, flags, s); }));
@@ -1441,10 +1656,12 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
     size_t n = (size_t)c->W * c->H;
     /* dest.offsets -> base, whole buffer (texSynth_kernel.cu:1716) */
     PtNnf base = c->base;
-    const bool lum = c->lum_live;
+    const bool lum = c->lum_live, lay = c->layered_live;
     RC_CHECK(issue(c, "advected offset copy", [=](hipStream_t s) {
         hipError_t e = hipMemcpyAsync(base.xyth, d.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess && lum) e = hipMemcpyAsync(base.lum, d.lum, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && lay)
+            e = hipMemcpyAsync(base.la, d.la, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
         return hipMemcpyAsync(base.sc, d.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
     }));
@@ -1482,6 +1699,55 @@ static int merge_residual(pt_ctx *c, int level, int forwards)
     return sync_buffers(c, level, 0);
 }
 
+static int merge_advected(pt_ctx *c, int level, int fallback)
+{
+    if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
+    res_invalidate(c);
+    c->layered_live = true;
+    PtArgs a = make_args(c, level);
+    int pr = prof_begin(c, PK_MERGE, (uint64_t)a.lw * a.lh, PB_MERGE * a.lw * a.lh);
+    RC_CHECK(issue(c, "merge", [=](hipStream_t s) { return pt_launch_merge_advected(a, fallback, s); }));
+    prof_end(c, pr);
+    c->off_diff.grow(a.lw, a.lh);
+    return sync_buffers(c, level, 0);
+}
+
+static int offsets_from_ribbon(pt_ctx *c, int level, int look_forwards)
+{
+    if (c->rw == 0) return set_err(c, PT_ERR_STATE, "no random texture");
+    res_invalidate(c);
+    c->layered_live = true;
+    PtArgs a = make_args(c, level);
+    int pr = prof_begin(c, PK_RANDOM, (uint64_t)a.lw * a.lh, PB_RANDOM * a.lw * a.lh);
+    RC_CHECK(issue(c, "ribbon offsets", [=](hipStream_t s) { return pt_launch_offsets_from_ribbon(a, look_forwards, s); }));
+    prof_end(c, pr);
+    c->off_diff.grow(a.lw, a.lh);
+    return sync_buffers(c, level, 0);
+}
+
+/* TexSynth::accumulateRibbonField (texSynth_kernel.cu:1828-1854): one step
+ * into the working ribbon buffer, copied whole to the base, then the canvas
+ * texels the ribbons reach into the forward advected output colours. */
+static int accumulate_ribbon(pt_ctx *c, int level, int forwards, int time_step, int overwrite)
+{
+    RC_CHECK(ribbons_ready(c));
+    PtArgs a = make_args(c, level);
+    pt_ribbonp *rb = c->rib_base, *rw = c->rib_work;
+    size_t n = (size_t)c->W * c->H;
+    int pr = prof_begin(c, PK_ADVECT, (uint64_t)a.lw * a.lh, 80 * (uint64_t)a.lw * a.lh);
+    RC_CHECK(issue(c, "ribbon accumulation", [=](hipStream_t s) {
+        hipError_t e = pt_launch_accumulate_ribbon(a, rb, rw, forwards, time_step, overwrite, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(rb, rw, n * sizeof(pt_ribbonp), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = pt_launch_ribbon_output(a, rb, s);
+        return e;
+    }));
+    prof_end(c, pr);
+    c->adv_written[0] = true;
+    tt_invalidate(c);
+    res_invalidate(c);
+    return PT_OK;
+}
+
 #define PUBLIC_OP(call)                                                                             \
     do {                                                                                            \
         HIP_CHECK(c, hipSetDevice(c->device));                                                      \
@@ -1495,18 +1761,25 @@ extern "C" int pt_randomize_offsets(pt_ctx *c, int level)
     PUBLIC_OP(randomize_offsets(c, level));
 }
 
+static int check_key_index(pt_ctx *c, int key_index)
+{
+    if (key_index >= PT_MAX_STYLE_LAYERS)
+        return set_err(c, PT_ERR_INVALID_ARG, "key_index must be below %d", PT_MAX_STYLE_LAYERS);
+    return PT_OK;
+}
+
 extern "C" int pt_propagate(pt_ctx *c, int level, int jump, int key_index)
 {
     RC_CHECK(need_level(c, level));
-    if (key_index >= 0) return set_err(c, PT_ERR_UNSUPPORTED, "keyframes are not supported");
-    PUBLIC_OP(propagate(c, level, jump));
+    RC_CHECK(check_key_index(c, key_index));
+    PUBLIC_OP(propagate(c, level, jump, key_index));
 }
 
 extern "C" int pt_scatter(pt_ctx *c, int level, int key_index)
 {
     RC_CHECK(need_level(c, level));
-    if (key_index >= 0) return set_err(c, PT_ERR_UNSUPPORTED, "keyframes are not supported");
-    PUBLIC_OP(scatter(c, level));
+    RC_CHECK(check_key_index(c, key_index));
+    PUBLIC_OP(scatter(c, level, key_index));
 }
 
 extern "C" int pt_upsample(pt_ctx *c, int fine_level)
@@ -1545,6 +1818,52 @@ extern "C" int pt_merge_using_residual(pt_ctx *c, int level, int forwards)
     PUBLIC_OP(merge_residual(c, level, forwards));
 }
 
+extern "C" int pt_merge_advected_buffers(pt_ctx *c, int level, int merge_fallback)
+{
+    RC_CHECK(need_level(c, level));
+    PUBLIC_OP(merge_advected(c, level, merge_fallback));
+}
+
+extern "C" int pt_offsets_from_frame_to_key_ribbon(pt_ctx *c, int level, int look_forwards)
+{
+    RC_CHECK(need_level(c, level));
+    PUBLIC_OP(offsets_from_ribbon(c, level, look_forwards));
+}
+
+extern "C" int pt_accumulate_ribbon_field(pt_ctx *c, int level, int forwards, int time_step, int overwrite)
+{
+    if (!c) return set_err(nullptr, PT_ERR_INVALID_ARG, "NULL context");
+    if (!c->base.xyth) return set_err(c, PT_ERR_STATE, "not initialized");
+    if (level < 0 || level > 30 || (c->W >> level) < 1 || (c->H >> level) < 1)
+        return set_err(c, PT_ERR_INVALID_ARG, "bad level %d", level);
+    PUBLIC_OP(accumulate_ribbon(c, level, forwards, time_step, overwrite));
+}
+
+/* The ribbon field (base) of a level, lw x lh ribbons, to or from the host. */
+extern "C" int pt_load_ribbon_field(pt_ctx *c, int level, const pt_ribbonp *ribbons)
+{
+    if (!c || !c->base.xyth || !ribbons || level < 0 || level > 30 || (c->W >> level) < 1 || (c->H >> level) < 1)
+        return set_err(c, PT_ERR_INVALID_ARG, "bad load_ribbon_field");
+    HIP_CHECK(c, hipSetDevice(c->device));
+    RC_CHECK(ribbons_ready(c));
+    int lw = c->W >> level, lh = c->H >> level;
+    HIP_CHECK(c, hipMemcpy2DAsync(c->rib_base, (size_t)c->W * sizeof(pt_ribbonp), ribbons, (size_t)lw * sizeof(pt_ribbonp),
+                                  (size_t)lw * sizeof(pt_ribbonp), lh, hipMemcpyHostToDevice, c->stream));
+    return sync_stream(c);
+}
+
+extern "C" int pt_get_ribbon_field(pt_ctx *c, int level, pt_ribbonp *ribbons)
+{
+    if (!c || !c->base.xyth || !ribbons || level < 0 || level > 30 || (c->W >> level) < 1 || (c->H >> level) < 1)
+        return set_err(c, PT_ERR_INVALID_ARG, "bad get_ribbon_field");
+    HIP_CHECK(c, hipSetDevice(c->device));
+    RC_CHECK(ribbons_ready(c));
+    int lw = c->W >> level, lh = c->H >> level;
+    HIP_CHECK(c, hipMemcpy2DAsync(ribbons, (size_t)lw * sizeof(pt_ribbonp), c->rib_base, (size_t)c->W * sizeof(pt_ribbonp),
+                                  (size_t)lw * sizeof(pt_ribbonp), lh, hipMemcpyDeviceToHost, c->stream));
+    return sync_stream(c);
+}
+
 /* ------------------------------------------------------------------ */
 /* readback and load                                                   */
 /* ------------------------------------------------------------------ */
@@ -1576,6 +1895,7 @@ extern "C" int pt_get_output(pt_ctx *c, int level, pt_patchxf *offsets, float *c
     int lw = c->W >> level, lh = c->H >> level;
     PtNnf b = c->base;
     if (!c->lum_live) b.lum = nullptr;
+    if (!c->layered_live) b.la = nullptr;
     if (offsets) RC_CHECK(get_nnf(c, b, lw, lh, offsets));
     if (canvas) RC_CHECK(get_plane(c, c->canvas, lw, lh, (float4 *)canvas));
     if (residual) RC_CHECK(get_plane(c, c->res_base, lw, lh, residual));
@@ -1594,6 +1914,7 @@ extern "C" int pt_get_output_noalloc(pt_ctx *c, int level, pt_patchxf *offsets, 
     HIP_CHECK(c, hipSetDevice(c->device));
     PtNnf b = c->base;
     if (!c->lum_live) b.lum = nullptr;
+    if (!c->layered_live) b.la = nullptr;
     if (offsets) {
         /* every call on this GPU shares its array: one user at a time */
         std::lock_guard<std::mutex> lock(g_scratch_mu);
@@ -1618,7 +1939,7 @@ extern "C" int pt_get_advected(pt_ctx *c, int level, int which, pt_patchxf *offs
     HIP_CHECK(c, hipSetDevice(c->device));
     int lw = c->W >> level, lh = c->H >> level;
     DevAdv &d = c->adv[which];
-    PtNnf n{d.xyth, d.sc, c->lum_live ? d.lum : nullptr};
+    PtNnf n{d.xyth, d.sc, c->lum_live ? d.lum : nullptr, c->layered_live ? d.la : nullptr};
     if (offsets) RC_CHECK(get_nnf(c, n, lw, lh, offsets));
     if (base) RC_CHECK(get_plane(c, d.base, lw, lh, (float4 *)base));
     if (output) RC_CHECK(get_plane(c, d.out, lw, lh, (float4 *)output));
@@ -1634,15 +1955,18 @@ extern "C" int pt_load_offsets(pt_ctx *c, int level, const pt_patchxf *in)
     HIP_CHECK(c, hipSetDevice(c->device));
     res_invalidate(c);
     int lw = c->W >> level, lh = c->H >> level;
-    bool shifted = false;
+    bool shifted = false, layered = false;
     for (size_t i = 0; i < (size_t)lw * lh; i++) {
-        if (in[i].layer != 0 || in[i].lum_a != 0.f)
-            return set_err(c, PT_ERR_UNSUPPORTED, "offsets with a style layer or an alpha luminance shift");
+        if (in[i].lum_a != 0.f)
+            return set_err(c, PT_ERR_UNSUPPORTED, "offsets with an alpha luminance shift");
         shifted = shifted || in[i].lum_y != 0.f;
+        layered = layered || in[i].layer != 0;
     }
     if (shifted) c->lum_live = true;
+    if (layered) c->layered_live = true;
     PtNnf b = c->base;
     if (!c->lum_live) b.lum = nullptr;
+    if (!c->layered_live) b.la = nullptr;
     pt_patchxf *tmp = nullptr;
     RC_CHECK(dalloc(c, &tmp, (size_t)lw * lh));
     hipError_t e = hipMemcpyAsync(tmp, in, (size_t)lw * lh * sizeof(pt_patchxf), hipMemcpyHostToDevice, c->stream);
@@ -1849,10 +2173,11 @@ static int copy_base_to_advected_f(pt_ctx *c)
     DevAdv d = c->adv[0];
     float *rb = c->res_base;
     size_t n = (size_t)c->W * c->H;
-    const bool lum = c->lum_live;
+    const bool lum = c->lum_live, lay = c->layered_live;
     RC_CHECK(issue(c, "copy to advected", [=](hipStream_t st) {
         hipError_t e = hipMemcpyAsync(d.xyth, base.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess && lum) e = hipMemcpyAsync(d.lum, base.lum, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && lay) e = hipMemcpyAsync(d.la, base.la, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(d.sc, base.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(d.res, rb, n * sizeof(float), hipMemcpyDeviceToDevice, st);
         return e;
@@ -2145,6 +2470,8 @@ static int check_scheme(pt_ctx *c, const pt_params *params, int scheme)
         return set_err(c, PT_ERR_UNSUPPORTED, "TCTF with a backward first pass is not supported");
     if (params->max_level != 0 && scheme != PT_S_FMBM && scheme != PT_S_FMBM_RM)
         return set_err(c, PT_ERR_UNSUPPORTED, "max_level above 0 is implemented for FMBM and FMBM_RM only");
+    if (params->interpolate_key_frame)
+        return set_err(c, PT_ERR_UNSUPPORTED, "keyframe schedules run through the operator entry points only");
     return PT_OK;
 }
 
@@ -2386,7 +2713,7 @@ static int read_snapshot(pt_ctx *c, const Snapshot &s, pt_patchxf *out_off, floa
             RC_CHECK(dalloc(c, &c->pack_tmp, n));
             c->pack_cap = n;
         }
-        PtNnf nn{s.xyth, s.sc, c->lum_live ? s.lum : nullptr};
+        PtNnf nn{s.xyth, s.sc, c->lum_live ? s.lum : nullptr, nullptr};
         int w = s.w, h = s.h;
         pt_patchxf *tmp = c->pack_tmp, *dst = out_off;
         if (on_device) {
@@ -2440,6 +2767,9 @@ static int reset_working(pt_ctx *c)
     RC_CHECK(zero_fill(c, c->base.lum, n * sizeof(float)));
     RC_CHECK(zero_fill(c, c->work.lum, n * sizeof(float)));
     c->lum_live = false;
+    RC_CHECK(zero_fill(c, c->base.la, n * sizeof(int32_t)));
+    RC_CHECK(zero_fill(c, c->work.la, n * sizeof(int32_t)));
+    c->layered_live = false;
     RC_CHECK(zero_fill(c, c->res_base, n * sizeof(float)));
     RC_CHECK(zero_fill(c, c->res_work, n * sizeof(float)));
     RC_CHECK(zero_fill(c, c->canvas, n * sizeof(float4)));
@@ -2801,11 +3131,11 @@ size_t pt_sequence_bytes(int W, int H, int SW, int SH, int levels, int r)
     /* the forward time table: 32 B per tap and pixel (patches up to 81 taps) */
     const double taps = (double)(2 * r + 1) * (2 * r + 1);
     const double tt = taps <= 81 ? 32.0 * taps : 0.0;
-    /* working buffers 328 B/px (luminance-shift planes and the image-ID
-     * texture included); two frame pyramids 2 x 60 B/px x 4/3;
+    /* working buffers 344 B/px (luminance-shift and style-layer planes and
+     * the image-ID texture included); two frame pyramids 2 x 60 B/px x 4/3;
      * two snapshot sets 2 x 48 B/px x 4/3; exemplar textures 40 B/texel;
      * style pyramids 52 B/texel x 4/3; plus 10% slack */
-    double b = px * (328.0 + tt + 2 * 60.0 * 4.0 / 3.0 + 2 * 48.0 * 4.0 / 3.0) + ex * (40.0 + 52.0 * 4.0 / 3.0);
+    double b = px * (344.0 + tt + 2 * 60.0 * 4.0 / 3.0 + 2 * 48.0 * 4.0 / 3.0) + ex * (40.0 + 52.0 * 4.0 / 3.0);
     return (size_t)(b * 1.1) + ((size_t)8 << 20);
 }
 

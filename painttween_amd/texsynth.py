@@ -167,6 +167,36 @@ class Context:
         self.call("create_random_pyramid", level, frame, image_color.handle, image_orient.handle,
                   ex_output.handle, ex_orient.handle)
 
+    def create_random_pyramid_keys(self, level, frame, image_color, image_orient, ex_outputs, ex_orients):
+        """createRandomPyramid with keyframes: one exemplar output and
+        orientation pyramid per style."""
+        n = len(ex_outputs)
+        assert len(ex_orients) == n
+        outs = (C.c_void_p * n)(*[p.handle.value for p in ex_outputs])
+        oris = (C.c_void_p * n)(*[p.handle.value for p in ex_orients])
+        self.call("create_random_pyramid_keys", level, frame, image_color.handle, image_orient.handle, n, outs, oris)
+
+    def initialize_cumulative_fields(self, w, h, layers):
+        self.call("initialize_cumulative_fields", w, h, layers)
+
+    def offsets_from_frame_to_key_ribbon(self, level, look_forwards):
+        self.call("offsets_from_frame_to_key_ribbon", level, int(look_forwards))
+
+    def merge_advected_buffers(self, level, fallback=0):
+        self.call("merge_advected_buffers", level, fallback)
+
+    def accumulate_ribbon_field(self, level, forwards, time_step, overwrite):
+        self.call("accumulate_ribbon_field", level, int(forwards), time_step, int(overwrite))
+
+    def load_ribbon_field(self, level, ribbons):
+        ribbons = np.ascontiguousarray(ribbons, _ffi.RIBBON_DTYPE)
+        self.call("load_ribbon_field", level, _ffi.vptr(ribbons))
+
+    def get_ribbon_field(self, level, w, h):
+        out = np.zeros((h, w), _ffi.RIBBON_DTYPE)
+        self.call("get_ribbon_field", level, _ffi.vptr(out))
+        return out
+
     def randomize_offsets(self, level):
         self.call("randomize_offsets", level)
 

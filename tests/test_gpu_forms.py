@@ -325,9 +325,9 @@ def test_admission_waits_for_budget():
 
 def _seq_bytes(s):
     sh, sw = s["style"]["base"].shape[:2]
-    # pt_sequence_bytes estimate, mirrored: 328 B/px working buffers, 32 B per tap and pixel of the forward
+    # pt_sequence_bytes estimate, mirrored: 344 B/px working buffers, 32 B per tap and pixel of the forward
     # time table, 2*60*4/3 + 2*48*4/3 B/px pyramids and snapshots, 40 + 52*4/3 B/texel, +10%, +8 MiB
     px, ex = s["w"] * s["h"], sw * sh
     taps = (2 * s["kw"]["residual_window_size"] + 1) ** 2
     tt = 32 * taps if taps <= 81 else 0
-    return int((px * (328 + tt + 2 * 60 * 4 / 3 + 2 * 48 * 4 / 3) + ex * (40 + 52 * 4 / 3)) * 1.1) + (8 << 20)
+    return int((px * (344 + tt + 2 * 60 * 4 / 3 + 2 * 48 * 4 / 3) + ex * (40 + 52 * 4 / 3)) * 1.1) + (8 << 20)
