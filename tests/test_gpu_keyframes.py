@@ -103,7 +103,8 @@ def test_layered_exemplar_without_keys():
     off["y"] = rng.uniform(0, (h >> 1) - 1, off.shape)
     off["layer"] = rng.integers(0, 3, off.shape)
     off["scale_u"] = off["scale_v"] = 1.0
-    ops = [("styles", 1), ("frame", 1, 0, False), ("load_offsets", off), ("sync", 0), ("propagate", 2, -1),
+    ops = [("styles", 1), ("frame", 1, 0, False), ("load_offsets", off), ("sync", _ffi.PT_DONT_COPY_OFFSETS),
+           ("residual",), ("propagate", 2, -1),
            ("scatter", -1), ("propagate", 1, -1), ("styles", 0), ("frame", 0, 0, False), ("upsample",), ("sync", 0),
            ("scatter", -1), ("propagate", 4, -1)]
     ga, oa = _both(w, h, 3, 1, dict(interpolate_key_frame=0, first_key_index=-1, last_key_index=-1,
