@@ -440,6 +440,11 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * level into a table of 32 B per tap and pixel, which every candidate then
  * reads instead of gathering them again; 0 disables it.  Same results. */
 #define PT_TUNE_TIME_TABLE_MAX_TAPS 8
+/* PT_TUNE_XCD_TILES (default 1): the single-job NNF kernels (propagate,
+ * scatter, residual refresh) deal the image's tiles to the 8 XCDs in
+ * contiguous row-major ranges instead of round-robin, so neighbouring tiles
+ * share one XCD's L2.  Same results. */
+#define PT_TUNE_XCD_TILES 9
 int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --

@@ -38,6 +38,7 @@ PT_TUNE_TAP_PARALLEL_MAX_PIXELS = 5
 PT_TUNE_TAP_PARALLEL_EVALS = 6
 PT_TUNE_EXEMPLAR_WINDOW = 7
 PT_TUNE_TIME_TABLE_MAX_TAPS = 8
+PT_TUNE_XCD_TILES = 9
 PT_SERVER_FILL_TILES = 5
 PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT, PT_DOWNSAMPLE_RIBBON = 0, 1, 2
 

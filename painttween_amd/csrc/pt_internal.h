@@ -74,6 +74,7 @@ struct PtArgs {
     int tap_np_res;    /* tap-parallel refresh: pixels per workgroup (pt_tap_configure)   */
     int tap_np_prop;   /* tap-parallel propagate: pixels per workgroup (pt_tap_configure) */
     int ex_window;     /* XL kernel variants: exemplar windows in LDS */
+    int xcd_tiles;     /* single-job launches: contiguous tile ranges per XCD */
 
     /* style layers (keyframe interpolation): with `layered` set every
      * offset carries its layer (PtNnf::la) and the per-pixel kernels' RT < 0

@@ -581,6 +581,23 @@ __device__ __forceinline__ float test_offset(const PtArgs &A, const Tile &T, int
 /* ------------------------------------------------------------------ */
 /* kernels                                                             */
 /* ------------------------------------------------------------------ */
+/* The tile of this workgroup in a single-job launch.  Workgroups are dealt
+ * to the 8 XCDs round-robin; with xcd_tiles XCD x takes the contiguous
+ * row-major range of tiles starting at sum_{y<x} count_y, so neighbouring
+ * tiles (whose candidates gather neighbouring exemplar lines) share one L2. */
+__device__ __forceinline__ void xcd_tile(const PtArgs &A, int &bx, int &by)
+{
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    if (!A.xcd_tiles) return;
+    const int nx = gridDim.x, n = gridDim.x * gridDim.y;
+    int g = blockIdx.x + blockIdx.y * nx;
+    int q = n >> 3, rem = n & 7, x = g & 7, k = g >> 3;
+    g = x * q + (x < rem ? x : rem) + k;
+    bx = g % nx;
+    by = g / nx;
+}
+
 __device__ __forceinline__ bool pixel_of_rows(const PtArgs &A, int bx, int by, int &x, int &y)
 {
     x = bx * PT_RX + threadIdx.x;
@@ -728,7 +745,9 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
 template <int RT, bool TK, bool XL = false>
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_update_residual(PtArgs A)
 {
-    b_update_residual<RT, TK, XL>(A, blockIdx.x, blockIdx.y);
+    int bx, by;
+    xcd_tile(A, bx, by);
+    b_update_residual<RT, TK, XL>(A, bx, by);
 }
 
 /* offsetsToOutput (texSynth_kernel.cu:263-274) */
@@ -971,7 +990,9 @@ __device__ __forceinline__ void b_propagate(const PtArgs &A, int bx, int by, int
 template <int RT, bool TK>
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_propagate(PtArgs A, int jump)
 {
-    b_propagate<RT, TK>(A, blockIdx.x, blockIdx.y, jump);
+    int bx, by;
+    xcd_tile(A, bx, by);
+    b_propagate<RT, TK>(A, bx, by, jump);
 }
 
 __device__ __forceinline__ float u24(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
@@ -1090,7 +1111,9 @@ __device__ __forceinline__ void b_scatter(const PtArgs &A, int bx, int by)
 template <int RT, bool TK>
 __global__ __launch_bounds__(PT_BX *PT_BY) void k_scatter(PtArgs A)
 {
-    b_scatter<RT, TK>(A, blockIdx.x, blockIdx.y);
+    int bx, by;
+    xcd_tile(A, bx, by);
+    b_scatter<RT, TK>(A, bx, by);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1963,7 +1986,9 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
 template <int RT, bool EE, int TX, int TY, bool TM>
 __global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int jump)
 {
-    b_propagate_up<RT, EE, TX, TY, TM>(A, blockIdx.x, blockIdx.y, jump);
+    int bx, by;
+    xcd_tile(A, bx, by);
+    b_propagate_up<RT, EE, TX, TY, TM>(A, bx, by, jump);
 }
 
 /* scatter_kernel (texSynth_kernel.cu:865-952), unit-parallel in rounds.
@@ -2116,7 +2141,9 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
 template <int RT, bool EE, int TX, int TY, bool TM>
 __global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
 {
-    b_scatter_up<RT, EE, TX, TY, TM>(A, blockIdx.x, blockIdx.y, n);
+    int bx, by;
+    xcd_tile(A, bx, by);
+    b_scatter_up<RT, EE, TX, TY, TM>(A, bx, by, n);
 }
 
 /* upsampleBilinearKernel (texSynth_kernel.cu:1453-1510) with

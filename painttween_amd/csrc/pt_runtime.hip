@@ -155,6 +155,7 @@ struct pt_ctx {
     int64_t tap_max_pixels = 128 * 128; /* tap-parallel shapes up to this level size */
     int tap_evals = 1024;               /* tap-parallel: tap evaluations per workgroup */
     int64_t ex_window_min_pixels = 400000; /* XL refresh on levels of at least this size (0: never) */
+    bool xcd_tiles = true; /* contiguous tile ranges per XCD in single-job launches */
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -687,6 +688,9 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
         if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative tap count");
         c->tt_max_taps = value;
         tt_invalidate(c);
+        return PT_OK;
+    case PT_TUNE_XCD_TILES:
+        c->xcd_tiles = value != 0;
         return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
@@ -1393,6 +1397,7 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.tap_max_px = (int)std::min<int64_t>(c->tap_max_pixels, 1 << 30);
     pt_tap_configure(a, c->tap_evals);
     a.ex_window = (c->ex_window_min_pixels > 0 && (int64_t)a.lw * a.lh >= c->ex_window_min_pixels) ? 1 : 0;
+    a.xcd_tiles = c->xcd_tiles ? 1 : 0;
     a.changed = c->changed;
     a.base = c->base;
     a.work = c->work;
@@ -2800,6 +2805,8 @@ static int check_synthesis(pt_ctx *c, const pt_params *params, int W, int H, int
     int sty_lv = (int)fmin(floor(log2f((float)SW)), floor(log2f((float)SH)));
     if (params->levels < 1 || params->levels > img_lv || params->levels > sty_lv)
         return set_err(c, PT_ERR_INVALID_ARG, "levels exceeds the pyramid depth");
+    if (params->interpolate_key_frame)
+        return set_err(c, PT_ERR_UNSUPPORTED, "keyframe schedules run through the operator entry points only");
     return check_supported(c, params);
 }
 

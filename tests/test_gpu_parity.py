@@ -332,17 +332,20 @@ def test_object_ids_with_temporal_and_coherence_terms():
 
 
 def test_kernel_variants_agree_at_scale():
-    """Early rejection, unit-parallel and tap-parallel search on and off, on
-    a 256^2 still with guides: offsets, canvas and residual identical."""
+    """Early rejection, unit-parallel and tap-parallel search and the XCD
+    tile mapping on and off, on a 256^2 still with guides: offsets, canvas
+    and residual identical."""
     st = W.style_pair(256, 256, seed=51, guides=True)
     fr = W.still_frame(256, 256, seed=52, guides=True)
     outs = []
-    for ee, up, tap in ((1, 1, 128 * 128), (0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 1 << 30)):
+    for ee, up, tap, xcd in ((1, 1, 128 * 128, 1), (0, 0, 0, 1), (1, 0, 0, 0), (0, 1, 0, 1), (1, 1, 1 << 30, 1),
+                             (1, 1, 128 * 128, 0)):
         with Context.gpu(0) as g:
             g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT, ee)
             g.set_tuning(_ffi.PT_TUNE_UNIT_PARALLEL, up)
             g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)
             g.set_tuning(_ffi.PT_TUNE_TAP_PARALLEL_MAX_PIXELS, tap)
+            g.set_tuning(_ffi.PT_TUNE_XCD_TILES, xcd)
             p = g.default_params(residual_window_size=3, levels=4, dist_trans_weight=0.5)
             outs.append(g.synthesize(p, [fr], st, iterations=4))
     for other in outs[1:]:
