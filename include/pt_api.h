@@ -445,6 +445,15 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * contiguous row-major ranges instead of round-robin, so neighbouring tiles
  * share one XCD's L2.  Same results. */
 #define PT_TUNE_XCD_TILES 9
+/* PT_TUNE_PROPAGATE_WINDOW_JUMP (default 8) and
+ * PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS (default 200000): on levels of at
+ * least that many pixels, the unit-parallel propagate steps with a jump up
+ * to this stage a 24x24-texel exemplar window in LDS, anchored on each
+ * tile's current offsets, and candidates whose whole patch lies in it read
+ * it there (short jumps propose offsets near the current ones); 0 disables
+ * it.  Same results. */
+#define PT_TUNE_PROPAGATE_WINDOW_JUMP 10
+#define PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS 11
 int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --

@@ -75,6 +75,8 @@ struct PtArgs {
     int tap_np_prop;   /* tap-parallel propagate: pixels per workgroup (pt_tap_configure) */
     int ex_window;     /* XL kernel variants: exemplar windows in LDS */
     int xcd_tiles;     /* single-job launches: contiguous tile ranges per XCD */
+    int prop_window;      /* unit-parallel propagate: exemplar windows on this level ... */
+    int prop_window_jump; /* ... for jumps up to this */
 
     /* style layers (keyframe interpolation): with `layered` set every
      * offset carries its layer (PtNnf::la) and the per-pixel kernels' RT < 0

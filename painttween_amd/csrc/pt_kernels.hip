@@ -1170,7 +1170,7 @@ __device__ __forceinline__ float tap_weight_sum(const PtArgs &A, int r, int x, i
 
 /* One row (fixed i) of calculateResidual's tap loop, accumulated into
  * (acc, swa) with the reference's per-tap arithmetic. */
-template <int RT, bool TIME>
+template <int RT, bool TIME, bool XL = false>
 __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
                                           int i, const Cand &c, float4 pf, float4 pb, bool cpv, bool cnv,
                                           const float *tb4, float &acc, float &swa)
@@ -1195,7 +1195,7 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
         if (ty >= hh) ty -= hh;
         else if (ty < 0) ty += hh;
         float4 tb, to;
-        ex_pair(er, ew, eh, tx, ty, tb, to);
+        ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to);
         float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
         if (TIME) {
             bool any_time_w = A.tdi > 0.f || A.tdo > 0.f || A.tcw > 0.f;
@@ -1231,7 +1231,7 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
 
 /* row_accum split in two for a fixed radius: the tap residuals of row i
  * (no temporal terms) ... */
-template <int RT>
+template <int RT, bool XL = false>
 __device__ __forceinline__ void row_taps(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y, int i,
                                          const Cand &c, float (&rv)[2 * RT + 1])
 {
@@ -1255,7 +1255,7 @@ __device__ __forceinline__ void row_taps(const PtArgs &A, const Tile &T, __amdgp
         if (ty >= hh) ty -= hh;
         else if (ty < 0) ty += hh;
         float4 tb, to;
-        ex_pair(er, ew, eh, tx, ty, tb, to);
+        ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to);
         float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
         if (i == 0 && j == 0) {
             if (A.dt_w > 0.f) {
@@ -1641,7 +1641,7 @@ __device__ __forceinline__ void stage_tile_up(const PtArgs &A, int x0, int y0, f
 /* Evaluates one unit (candidate c with its basis) for pixel (x, y) at tile
  * position (lx, ly): the exact patch distance, or +inf when a partial sum
  * proves it cannot beat `best` (EE). */
-template <int RT, bool EE, int TX, bool TM>
+template <int RT, bool EE, int TX, bool TM, bool XL = false>
 __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
                                          Cand c, const TimeCand &tc, float best, float sw)
 {
@@ -1656,7 +1656,7 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
             int i = step == 0 ? 0 : (step == 1 ? -1 : 1);
             if (i > r) break;
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
+                row_accum<RT, TM, XL>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
             if (acc >= T0) return PT_INF;
         }
         acc = a0;
@@ -1664,14 +1664,14 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
 #pragma unroll 1
         for (int i = -r; i <= r; i++) {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
+                row_accum<RT, TM, XL>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
             if (acc >= T0) return PT_INF;
         }
     } else {
 #pragma unroll 1
         for (int i = -r; i <= r; i++) {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
+                row_accum<RT, TM, XL>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
         }
     }
     return (swa < 0.000001f) ? 1e31f : acc / swa;
@@ -1685,7 +1685,7 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
  * the unit needs no evaluation; it has then stored its value itself);
  * DONE(u, value) stores a decided unit's value (+inf when rejected).
  */
-template <int RT, int TX, bool KEEP, bool TM, class Setup, class Done>
+template <int RT, int TX, bool KEEP, bool TM, bool XL = false, class Setup, class Done>
 __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_buffer_rsrc_t er, int first_u,
                                             int U, int *s_next, const float *s_best, const float *s_sw, Setup setup,
                                             Done done)
@@ -1729,7 +1729,7 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
             if (x + i >= 0 && x + i < A.lw) {
                 if (ph == 0) {
                     float rv[NK];
-                    row_taps<RT>(A, T, er, x, y, i, c, rv);
+                    row_taps<RT, XL>(A, T, er, x, y, i, c, rv);
                     row_sum<RT>(A, y, i, rv, acc, swa);
 #pragma unroll
                     for (int j = 0; j < NK; j++) {
@@ -1743,15 +1743,15 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
                     for (int j = 0; j < NK; j++) rv[j] = i == 0 ? k0[j] : (i < 0 ? km[j] : kp[j]);
                     row_sum<RT>(A, y, i, rv, acc, swa);
                 } else {
-                    row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+                    row_accum<RT, false, XL>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
                 }
             }
         } else if constexpr (TM) {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, true>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
+                row_accum<RT, true, XL>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
         } else {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, false>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
+                row_accum<RT, false, XL>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
         }
         float val = 0.f;
         bool decided = false;
@@ -1848,11 +1848,23 @@ template <int P> __device__ __forceinline__ void up_time_load(const UpTime<P> &S
  * setup is a few LDS reads, so a wave does not pay the full setup whenever
  * one of its lanes starts a new unit.
  */
-template <int RT, bool EE, int TX, int TY, bool TM>
+/* Exemplar window side of the XL propagate: 24 texels (18 KB) hold the
+ * patches of a coherent 8x8 tile's candidates while keeping four workgroups
+ * per CU (measured on the 1024^2 still: 20 / 24 / 28 / 32 texels give
+ * 120.9 / 119.7 / 122.6 / 123.8 ms per frame). */
+#define PT_XW_PROP 24
+
+template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false>
 __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, int jump)
 {
     constexpr int P = TX * TY, U = 4 * P;
+    static_assert(!XL || RT > 0, "exemplar windows need a fixed radius");
     __shared__ UpTime<TM ? P : 1> s_tm;
+    /* XL: a window of the interleaved exemplar anchored on the current offset
+     * of the tile's middle pixel; units whose every tap footprint lies in it
+     * read their taps from LDS (same texels, weights and order) */
+    __shared__ float4 s_exw[XL ? PT_XW_PROP * PT_XW_PROP * 2 : 1];
+    __shared__ unsigned char s_exl[XL ? U : 1];
     __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
     __shared__ float s_best[P], s_sw[P], s_ia[P], s_io[P];
     __shared__ float2 s_sc[P];
@@ -1863,6 +1875,12 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
+    int wx = 0, wy = 0;
+    if constexpr (XL) {
+        int ax = min(x0 + TX / 2, A.lw - 1), ay = min(y0 + TY / 2, A.lh - 1);
+        float4 an = A.base.xyth[ay * A.W + ax];
+        stage_window(A, an.x, an.y, PT_XW_PROP, s_exw, wx, wy, threadIdx.x, PT_UP_THREADS);
+    }
     for (int p = threadIdx.x; p < P; p += PT_UP_THREADS) {
         int x = x0 + p % TX, y = y0 + p / TX;
         bool live = x < A.lw && y < A.lh && !image_culled(A, x, y);
@@ -1898,6 +1916,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         if (!candidate_testable(A, c, s_ia[p])) continue;
         float b0x, b0y, b1x, b1y;
         basis(c.z, A.su * csc.x, A.sv * csc.y, b0x, b0y, b1x, b1y);
+        if constexpr (XL) s_exl[u] = in_window(A, r, c.x, c.y, b0x, b0y, b1x, b1y, wx, wy, PT_XW_PROP) ? 1 : 0;
         s_cx[u] = c.x;
         s_cy[u] = c.y;
         s_b0x[u] = b0x;
@@ -1915,11 +1934,17 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     T.col = scol;
     T.can = scan;
     T.tw = TX + 2 * r;
+    T.exw = s_exw;
+    T.wx = wx;
+    T.wy = wy;
+    T.ww = PT_XW_PROP;
+    T.exl = false;
     auto setup = [&](int idx, Cand &c, TimeCand &tc, int &x, int &y, int &p) -> bool {
         int u = s_list[idx];
         p = u >> 2;
         x = x0 + p % TX;
         y = y0 + p / TX;
+        if constexpr (XL) T.exl = s_exl[u] != 0;
         c.t = make_float4(s_cx[u], s_cy[u], 0.f, 0.f);
         c.sc = make_float2(1.f, s_scy[u]);
         c.b0x = s_b0x[u];
@@ -1931,7 +1956,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     };
     auto done = [&](int idx, float v) { s_r[s_list[idx]] = v; };
     if (EE) {
-        up_queue_ee<RT, TX, (RT > 0) && !TM, TM>(A, T, er, threadIdx.x, n, &s_next, s_best, s_sw, setup, done);
+        up_queue_ee<RT, TX, (RT > 0) && !TM, TM, XL>(A, T, er, threadIdx.x, n, &s_next, s_best, s_sw, setup, done);
     } else {
         for (int idx = threadIdx.x; idx < n;) {
             Cand c;
@@ -1940,7 +1965,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
             setup(idx, c, tc, x, y, p);
             T.lx = p % TX;
             T.ly = p / TX;
-            done(idx, up_eval<RT, false, TX, TM>(A, T, er, x, y, c, tc, s_best[p], s_sw[p]));
+            done(idx, up_eval<RT, false, TX, TM, XL>(A, T, er, x, y, c, tc, s_best[p], s_sw[p]));
             idx = atomicAdd(&s_next, 1);
         }
     }
@@ -1983,12 +2008,12 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     }
 }
 
-template <int RT, bool EE, int TX, int TY, bool TM>
+template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false>
 __global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int jump)
 {
     int bx, by;
     xcd_tile(A, bx, by);
-    b_propagate_up<RT, EE, TX, TY, TM>(A, bx, by, jump);
+    b_propagate_up<RT, EE, TX, TY, TM, XL>(A, bx, by, jump);
 }
 
 /* scatter_kernel (texSynth_kernel.cu:865-952), unit-parallel in rounds.
@@ -3128,12 +3153,12 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void km_propagate_ee(PT_MULTI_PARAMS)
     b_propagate_ee<RT, TK>(A, t.bx, t.by, A.jump);
 }
 
-template <int RT, bool EE, int TX, int TY, bool TM>
+template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false>
 __global__ __launch_bounds__(PT_UP_THREADS) void km_propagate_up(PT_MULTI_PARAMS)
 {
     MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
     const PtArgs &A = PT_JOB(t.j);
-    b_propagate_up<RT, EE, TX, TY, TM>(A, t.bx, t.by, A.jump);
+    b_propagate_up<RT, EE, TX, TY, TM, XL>(A, t.bx, t.by, A.jump);
 }
 
 template <int RT, bool TK>
@@ -3362,11 +3387,21 @@ int pt_shape_tiles(int op, int shape, const PtArgs &a, int *tiles_x)
 
 static int rt_of(int r) { return (r == 2 || r == 3 || r == 4) ? r : 0; }
 
+/* Whether a propagate launch of this shape takes the exemplar-window
+ * variant: a short jump (a.jump) on a large level, a fixed radius, an
+ * exemplar that holds the window, an 8x8 unit-parallel shape. */
+static bool prop_xl(int shape, const PtArgs &a)
+{
+    return a.prop_window && a.jump <= a.prop_window_jump && a.ew >= PT_XW_PROP && a.eh >= PT_XW_PROP &&
+           (a.r == 2 || a.r == 3) && (shape == PT_SHAPE_UP_EE_B || shape == PT_SHAPE_UP_A) && !ext_terms(a);
+}
+
 int pt_shape_key(int op, int shape, const PtArgs &a)
 {
     int tk = (op == PT_OP_SYNC_CANVAS) ? 0 : (a.time_active ? 1 : 0);
     int rt = (op == PT_OP_SYNC_CANVAS) ? 0 : (ext_terms(a) ? 1 : rt_of(a.r));
-    return ((op * PT_SHAPE_COUNT + shape) * 8 + rt) * 2 + tk;
+    int xl = (op == PT_OP_PROPAGATE && prop_xl(shape, a)) ? 1 : 0;
+    return (((op * PT_SHAPE_COUNT + shape) * 8 + rt) * 2 + tk) * 2 + xl;
 }
 
 /* One kernel instantiation per (op, shape, RT, TK); `L` launches it either
@@ -3404,6 +3439,19 @@ hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
 #define L_PEE(R) hipLaunchKernelGGL((tk ? k_propagate_ee<R, true> : k_propagate_ee<R, false>), g, kBlock, 0, s, a, jump)
 #define L_UEB(R) hipLaunchKernelGGL((tk ? k_propagate_up<R, true, 8, 8, true> : k_propagate_up<R, true, 8, 8, false>), g, bu, 0, s, a, jump)
 #define L_UA(R) hipLaunchKernelGGL((tk ? k_propagate_up<R, false, 8, 8, true> : k_propagate_up<R, false, 8, 8, false>), g, bu, 0, s, a, jump)
+    /* exemplar windows for the short jumps on large levels, whose candidates
+     * lie near the tile's current offsets */
+    if (prop_xl(shape, a)) {
+        const bool ee = shape == PT_SHAPE_UP_EE_B;
+#define L_XL(R)                                                                                           \
+    hipLaunchKernelGGL((ee ? (tk ? k_propagate_up<R, true, 8, 8, true, true> : k_propagate_up<R, true, 8, 8, false, true>) \
+                           : (tk ? k_propagate_up<R, false, 8, 8, true, true> : k_propagate_up<R, false, 8, 8, false, true>)), \
+                       g, bu, 0, s, a, jump)
+        if (a.r == 2) L_XL(2);
+        else L_XL(3);
+#undef L_XL
+        return hipGetLastError();
+    }
     switch (shape) {
     case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
     case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
@@ -3473,6 +3521,17 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
 #define L_UA(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, false, 8, 8, true> : km_propagate_up<R, false, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_UEB(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, true, 8, 8, true> : km_propagate_up<R, true, 8, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
 #define L_UB(R) hipLaunchKernelGGL((tk ? km_propagate_up<R, false, 16, 8, true> : km_propagate_up<R, false, 16, 8, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+        if (prop_xl(shape, a)) {
+            const bool ee = shape == PT_SHAPE_UP_EE_B;
+#define L_XL(R)                                                                                           \
+    hipLaunchKernelGGL((ee ? (tk ? km_propagate_up<R, true, 8, 8, true, true> : km_propagate_up<R, true, 8, 8, false, true>) \
+                           : (tk ? km_propagate_up<R, false, 8, 8, true, true> : km_propagate_up<R, false, 8, 8, false, true>)), \
+                       g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap)
+            if (a.r == 2) L_XL(2);
+            else L_XL(3);
+#undef L_XL
+            break;
+        }
         switch (shape) {
         case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
         case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;

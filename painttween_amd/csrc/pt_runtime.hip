@@ -156,6 +156,8 @@ struct pt_ctx {
     int tap_evals = 1024;               /* tap-parallel: tap evaluations per workgroup */
     int64_t ex_window_min_pixels = 400000; /* XL refresh on levels of at least this size (0: never) */
     bool xcd_tiles = true; /* contiguous tile ranges per XCD in single-job launches */
+    int prop_window_jump = 8; /* XL propagate for jumps up to this (0: never) ... */
+    int64_t prop_window_min_pixels = 200000; /* ... on levels of at least this size (0: never) */
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -691,6 +693,14 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
         return PT_OK;
     case PT_TUNE_XCD_TILES:
         c->xcd_tiles = value != 0;
+        return PT_OK;
+    case PT_TUNE_PROPAGATE_WINDOW_JUMP:
+        if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative jump");
+        c->prop_window_jump = (int)std::min<int64_t>(value, 1 << 30);
+        return PT_OK;
+    case PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS:
+        if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative pixel count");
+        c->prop_window_min_pixels = value;
         return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
@@ -1398,6 +1408,8 @@ static PtArgs make_args(const pt_ctx *c, int level)
     pt_tap_configure(a, c->tap_evals);
     a.ex_window = (c->ex_window_min_pixels > 0 && (int64_t)a.lw * a.lh >= c->ex_window_min_pixels) ? 1 : 0;
     a.xcd_tiles = c->xcd_tiles ? 1 : 0;
+    a.prop_window_jump = c->prop_window_jump;
+    a.prop_window = (c->prop_window_min_pixels > 0 && (int64_t)a.lw * a.lh >= c->prop_window_min_pixels) ? 1 : 0;
     a.changed = c->changed;
     a.base = c->base;
     a.work = c->work;

@@ -204,19 +204,25 @@ def test_coherence_term_with_temporal_terms(tap):
     assert_same_states(ga, oa)
 
 
-@pytest.mark.parametrize("window", [0, 1])
+@pytest.mark.parametrize("window,ee", [(0, 0), (1, 0), (1, 1)])
 @pytest.mark.parametrize("r", [2, 3])
-def test_exemplar_window_refresh(r, window):
-    """The residual refresh reading patches from an LDS window of the
-    exemplar (PT_TUNE_EXEMPLAR_WINDOW), with and without temporal terms,
-    against the oracle; exemplars from 96 down to 24 texels cover windows
-    that clamp at the exemplar edges and levels too small for a window."""
+def test_exemplar_window_refresh_and_propagate(r, window, ee):
+    """The residual refresh and the short-jump propagate steps reading
+    patches from an LDS window of the exemplar (PT_TUNE_EXEMPLAR_WINDOW,
+    PT_TUNE_PROPAGATE_WINDOW_JUMP), with and without temporal terms and early
+    rejection, against the oracle; exemplars from 96 down to 24 texels cover
+    windows that clamp at the exemplar edges and levels too small for a
+    window."""
     st = W.style_pair(96, 80, seed=33, guides=True)
     frames = W.animation(72, 64, 2, seed=34)
     ov = dict(residual_window_size=r, levels=3, temporal_coherence_weight=0.5, orientation_enabled=1)
     g = Context.gpu(0)
     g.set_tuning(_ffi.PT_TUNE_EXEMPLAR_WINDOW, window)
+    g.set_tuning(_ffi.PT_TUNE_PROPAGATE_WINDOW_JUMP, 1 << 20)
+    g.set_tuning(_ffi.PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS, window)
     g.set_tuning(_ffi.PT_TUNE_TAP_PARALLEL_MAX_PIXELS, 0)
+    if ee:
+        g.set_tuning(_ffi.PT_TUNE_EARLY_EXIT_MIN_PIXELS, 0)
     o = OracleContext()
     try:
         ga = Session(g, st, frames, g.default_params(**ov)).run(_anim_ops(3, 2))
