@@ -326,7 +326,9 @@ template <bool XL>
 __device__ __forceinline__ void ex_pair_t(const Tile &T, __amdgpu_buffer_rsrc_t er, int ew, int eh, float x, float y,
                                           float4 &b, float4 &o, int lb = 0)
 {
-    if (XL && T.exl) {
+    if constexpr (XL) {
+        /* one set of weights and one mix for both sources, so that a wave
+         * whose lanes read from both pays the arithmetic once */
         float fx = floorf(x), fy = floorf(y);
         float a = x - fx, c = y - fy;
         Bilin q;
@@ -334,11 +336,36 @@ __device__ __forceinline__ void ex_pair_t(const Tile &T, __amdgpu_buffer_rsrc_t 
         q.w10 = a * (1.0f - c);
         q.w01 = (1.0f - a) * c;
         q.w11 = a * c;
-        int i00 = (((int)fy - T.wy) * T.ww + ((int)fx - T.wx)) * 2;
-        int i01 = i00 + 2 * T.ww;
-        const float4 *e = T.exw;
-        b = mix4(q, e[i00], e[i00 + 2], e[i01], e[i01 + 2]);
-        o = mix4(q, e[i00 + 1], e[i00 + 3], e[i01 + 1], e[i01 + 3]);
+        int x0 = (int)fx, y0 = (int)fy;
+        float4 b00, b10, b01, b11, o00, o10, o01, o11;
+        if (T.exl) {
+            int i00 = ((y0 - T.wy) * T.ww + (x0 - T.wx)) * 2;
+            int i01 = i00 + 2 * T.ww;
+            const float4 *e = T.exw;
+            b00 = e[i00];
+            o00 = e[i00 + 1];
+            b10 = e[i00 + 2];
+            o10 = e[i00 + 3];
+            b01 = e[i01];
+            o01 = e[i01 + 1];
+            b11 = e[i01 + 2];
+            o11 = e[i01 + 3];
+        } else {
+            int xa = clampi(x0, 0, ew - 1), xb = clampi(x0 + 1, 0, ew - 1);
+            int ya = clampi(y0, 0, eh - 1), yb = clampi(y0 + 1, 0, eh - 1);
+            int ra = lb + ya * ew, rb = lb + yb * ew;
+            int f00 = (ra + xa) << 5, f10 = (ra + xb) << 5, f01 = (rb + xa) << 5, f11 = (rb + xb) << 5;
+            b00 = ld4(er, f00);
+            b10 = ld4(er, f10);
+            b01 = ld4(er, f01);
+            b11 = ld4(er, f11);
+            o00 = ld4(er, f00 + 16);
+            o10 = ld4(er, f10 + 16);
+            o01 = ld4(er, f01 + 16);
+            o11 = ld4(er, f11 + 16);
+        }
+        b = mix4(q, b00, b10, b01, b11);
+        o = mix4(q, o00, o10, o01, o11);
     } else {
         ex_pair(er, ew, eh, x, y, b, o, lb);
     }
