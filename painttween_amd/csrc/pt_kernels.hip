@@ -280,6 +280,12 @@ struct Tile {
     const float4 *exw;
     int wx, wy, ww;
     bool exl;
+    /* advected set sdir (-1: none) of the tile's pixels and halo, staged in
+     * LDS with the tile's pitch: validity, previous-frame colour and
+     * exemplar output (read only by the TS kernel variants) */
+    const float4 *ab, *ao;
+    const unsigned char *av;
+    int sdir;
 };
 
 typedef float pt_f4v __attribute__((ext_vector_type(4)));
@@ -405,13 +411,10 @@ __device__ __forceinline__ void stage_window(const PtArgs &A, float ax, float ay
  * variant with the extended terms, where the LAB colour space converts the
  * previous frame's colours and exemplar samples (:454-459) */
 template <bool EXT = false>
-__device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k, float4 prev, float pb0x, float pb0y,
-                              float pb1x, float pb1y, float4 base, float4 tb, float4 canvas, float4 to, int oo, int x,
-                              int y, int i, int j, int plb = 0)
+__device__ float time_terms_v(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k, float4 prev, float pb0x, float pb0y,
+                              float pb1x, float pb1y, float4 base, float4 tb, float4 canvas, float4 to,
+                              float4 prev_base, float4 prev_out, int x, int y, int i, int j, int plb = 0)
 {
-    const PtAdv &d = A.adv[k];
-    float4 prev_base = d.base[oo];
-    float4 prev_out = d.out[oo];
     float4 peb, peo;
     if (A.tt[k]) {
         /* the exemplar samples at the advected offset's tap, gathered once
@@ -434,6 +437,18 @@ __device__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k,
     r = fmaf(ssd4(sub4(canvas, prev_out), sub4(to, peo)), A.tdo, r);
     r = fmaf(ssd4(prev_out, to), A.tcw, r);
     return r;
+}
+
+/* time_terms_v with the advected colours of tap pixel oo read from set k */
+template <bool EXT = false>
+__device__ __forceinline__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k, float4 prev,
+                                              float pb0x, float pb0y, float pb1x, float pb1y, float4 base, float4 tb,
+                                              float4 canvas, float4 to, int oo, int x, int y, int i, int j,
+                                              int plb = 0)
+{
+    const PtAdv &d = A.adv[k];
+    return time_terms_v<EXT>(A, er, k, prev, pb0x, pb0y, pb1x, pb1y, base, tb, canvas, to, d.base[oo], d.out[oo], x, y,
+                             i, j, plb);
 }
 
 /* The coherence term of tap (i, j) (texSynth_kernel.cu:642-646): against
@@ -1197,7 +1212,7 @@ __device__ __forceinline__ float tap_weight_sum(const PtArgs &A, int r, int x, i
 
 /* One row (fixed i) of calculateResidual's tap loop, accumulated into
  * (acc, swa) with the reference's per-tap arithmetic. */
-template <int RT, bool TIME, bool XL = false>
+template <int RT, bool TIME, bool XL = false, bool TS = false>
 __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
                                           int i, const Cand &c, float4 pf, float4 pb, bool cpv, bool cnv,
                                           const float *tb4, float &acc, float &swa)
@@ -1228,7 +1243,18 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
             bool any_time_w = A.tdi > 0.f || A.tdo > 0.f || A.tcw > 0.f;
             bool fwd_dir = A.direction == PT_FORWARD || A.direction == PT_BIDIRECTIONAL;
             bool bwd_dir = A.direction == PT_BACKWARD || A.direction == PT_BIDIRECTIONAL;
-            if (any_time_w) {
+            if (TS && T.sdir >= 0) {
+                /* one direction in use, its advected set staged in LDS */
+                if (any_time_w) {
+                    const int si = (T.ly + r + j) * T.tw + (T.lx + r + i);
+                    const bool k0 = T.sdir == 0;
+                    if ((k0 ? (fwd_dir && cpv) : (bwd_dir && cnv)) && T.av[si]) {
+                        const float *b4 = tb4 + (k0 ? 0 : 4);
+                        res = res + time_terms_v(A, er, T.sdir, k0 ? pf : pb, b4[0], b4[1], b4[2], b4[3], base, tb,
+                                                 canvas, to, T.ab[si], T.ao[si], x, y, i, j);
+                    }
+                }
+            } else if (any_time_w) {
                 int oo = (y + j) * A.W + (x + i);
                 if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
                     res = res + time_terms_b(A, er, 0, pf, tb4[0], tb4[1], tb4[2], tb4[3], base, tb, canvas,
@@ -1665,6 +1691,36 @@ __device__ __forceinline__ void stage_tile_up(const PtArgs &A, int x0, int y0, f
     }
 }
 
+/* The advected set of the launch's only temporal direction (forward or
+ * backward; -1 when both or neither are in use) over the tile and its r
+ * halo, coordinates clamped into the level as stage_tile_up's (taps outside
+ * the level are skipped, so clamped entries are never read): validity, and
+ * for valid entries the previous-frame colour and exemplar output. */
+template <int RT, int TX, int TY>
+__device__ __forceinline__ int stage_adv_up(const PtArgs &A, int x0, int y0, float4 *sab, float4 *sao,
+                                            unsigned char *sav)
+{
+    const bool fwd = A.direction == PT_FORWARD || A.direction == PT_BIDIRECTIONAL;
+    const bool bwd = A.direction == PT_BACKWARD || A.direction == PT_BIDIRECTIONAL;
+    if (fwd == bwd) return -1;
+    const int k = fwd ? 0 : 1;
+    const PtAdv &d = A.adv[k];
+    const int r = RT > 0 ? RT : A.r;
+    const int tw = TX + 2 * r, th = TY + 2 * r;
+    for (int e = threadIdx.x; e < tw * th; e += PT_UP_THREADS) {
+        int tx = clampi(x0 - r + e % tw, 0, A.lw - 1);
+        int ty = clampi(y0 - r + e / tw, 0, A.lh - 1);
+        int o = ty * A.W + tx;
+        bool v = d.xyth[o].w > 0;
+        sav[e] = v ? 1 : 0;
+        if (v) {
+            sab[e] = d.base[o];
+            sao[e] = d.out[o];
+        }
+    }
+    return k;
+}
+
 /* Evaluates one unit (candidate c with its basis) for pixel (x, y) at tile
  * position (lx, ly): the exact patch distance, or +inf when a partial sum
  * proves it cannot beat `best` (EE). */
@@ -1683,7 +1739,7 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
             int i = step == 0 ? 0 : (step == 1 ? -1 : 1);
             if (i > r) break;
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, TM, XL>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
+                row_accum<RT, TM, XL, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
             if (acc >= T0) return PT_INF;
         }
         acc = a0;
@@ -1691,14 +1747,14 @@ __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgp
 #pragma unroll 1
         for (int i = -r; i <= r; i++) {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, TM, XL>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
+                row_accum<RT, TM, XL, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
             if (acc >= T0) return PT_INF;
         }
     } else {
 #pragma unroll 1
         for (int i = -r; i <= r; i++) {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, TM, XL>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
+                row_accum<RT, TM, XL, TM>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
         }
     }
     return (swa < 0.000001f) ? 1e31f : acc / swa;
@@ -1775,7 +1831,7 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
             }
         } else if constexpr (TM) {
             if (x + i >= 0 && x + i < A.lw)
-                row_accum<RT, true, XL>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
+                row_accum<RT, true, XL, true>(A, T, er, x, y, i, c, tc.pf, tc.pb, tc.cpv, tc.cnv, tc.tb4, acc, swa);
         } else {
             if (x + i >= 0 && x + i < A.lw)
                 row_accum<RT, false, XL>(A, T, er, x, y, i, c, z4, z4, false, false, tb4, acc, swa);
@@ -1892,6 +1948,9 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
      * read their taps from LDS (same texels, weights and order) */
     __shared__ float4 s_exw[XL ? PT_XW_PROP * PT_XW_PROP * 2 : 1];
     __shared__ unsigned char s_exl[XL ? U : 1];
+    /* TM: the advected set of the launch's direction over the tile */
+    __shared__ float4 s_ab[TM ? UpTile<RT, TX, TY>::N : 1], s_ao[TM ? UpTile<RT, TX, TY>::N : 1];
+    __shared__ unsigned char s_av[TM ? UpTile<RT, TX, TY>::N : 1];
     __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
     __shared__ float s_best[P], s_sw[P], s_ia[P], s_io[P];
     __shared__ float2 s_sc[P];
@@ -1902,6 +1961,8 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
+    int sdir = -1;
+    if constexpr (TM) sdir = stage_adv_up<RT, TX, TY>(A, x0, y0, s_ab, s_ao, s_av);
     int wx = 0, wy = 0;
     if constexpr (XL) {
         int ax = min(x0 + TX / 2, A.lw - 1), ay = min(y0 + TY / 2, A.lh - 1);
@@ -1966,6 +2027,10 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     T.wy = wy;
     T.ww = PT_XW_PROP;
     T.exl = false;
+    T.ab = s_ab;
+    T.ao = s_ao;
+    T.av = s_av;
+    T.sdir = sdir;
     auto setup = [&](int idx, Cand &c, TimeCand &tc, int &x, int &y, int &p) -> bool {
         int u = s_list[idx];
         p = u >> 2;
@@ -2061,11 +2126,15 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     __shared__ float s_dist[S];
     __shared__ float s_r[P * S];
     __shared__ int s_next, s_more;
+    __shared__ float4 s_ab[TM ? UpTile<RT, TX, TY>::N : 1], s_ao[TM ? UpTile<RT, TX, TY>::N : 1];
+    __shared__ unsigned char s_av[TM ? UpTile<RT, TX, TY>::N : 1];
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     const int SU = n + 1; /* units per pixel */
     const bool pmaj = up_pixel_major<TM>(A);
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
+    int sdir = -1;
+    if constexpr (TM) sdir = stage_adv_up<RT, TX, TY>(A, x0, y0, s_ab, s_ao, s_av);
     if (threadIdx.x == 0) {
         float d = (float)(A.ew > A.eh ? A.ew : A.eh);
         for (int k = 0; k < n; k++) {
@@ -2094,6 +2163,11 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     T.col = scol;
     T.can = scan;
     T.tw = TX + 2 * r;
+    T.exl = false;
+    T.ab = s_ab;
+    T.ao = s_ao;
+    T.av = s_av;
+    T.sdir = sdir;
     const float2 one = make_float2(1.f, 1.f);
     bool first = true;
     while (true) {
