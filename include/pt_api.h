@@ -615,6 +615,27 @@ int pt_batch_tune(pt_batch *b, int rounds);
 /* default 2048: a job step of at least this many workgroup tiles fills the
  * GPU alone and gets a launch of its own instead of sharing one */
 #define PT_BATCH_ISOLATE_TILES 4
+/*
+ * Persistent form (default 0): pt_batch_run makes ONE kernel launch that
+ * runs every job to completion.  Each job's recorded plan becomes a device
+ * program (one step per kernel its plain path launches, each step's
+ * workgroup tiles claimed from a per-job counter); a job's steps run in
+ * order, different jobs overlap freely, and the grid holds only as many
+ * workgroups as fit on the GPU at once.  Outputs are bit-identical to the
+ * plain path.  Jobs whose plans need a kernel without a device-program form
+ * (temporal terms, extended patch-distance terms, style layers, keyframes,
+ * patch radii other than 2 and 3) make pt_batch_run return
+ * PT_ERR_UNSUPPORTED naming the step.  pt_batch_tune then picks the tiles
+ * per claim.  A launch that sees no tile complete for the timeout gives up
+ * and pt_batch_run returns PT_ERR_HIP (its outputs are then undefined).
+ */
+#define PT_BATCH_PERSISTENT 5
+#define PT_BATCH_PERSIST_CHUNK 6      /* default 8: most tiles one claim takes */
+#define PT_BATCH_PERSIST_TIMEOUT_MS 7 /* default 2000 */
+/* value 1: the next persistent launch starts with its give-up flag set, so
+ * every workgroup stops at its first look for work and pt_batch_run returns
+ * PT_ERR_HIP (exercises the give-up path; later runs are unaffected) */
+#define PT_BATCH_PERSIST_ABORT 8
 int pt_batch_set(pt_batch *b, int key, int value);
 
 typedef struct pt_batch_stats {
@@ -623,6 +644,12 @@ typedef struct pt_batch_stats {
     double tune_ms;      /* total time spent in pt_batch_tune */
     int32_t shape[2][3]; /* chosen shape per (propagate, scatter) x size class; -1 = default */
     int32_t xcd_remap, isolate_tiles;
+    int32_t persistent;           /* PT_BATCH_PERSISTENT */
+    int32_t persistent_grid;      /* workgroups of the persistent launch */
+    int64_t persistent_tasks;     /* steps of all jobs' device programs */
+    int64_t persistent_tiles;     /* workgroup tiles of all steps */
+    int32_t persistent_max_chunk; /* tiles per claim */
+    int32_t reserved;
 } pt_batch_stats;
 int pt_batch_get_stats(pt_batch *b, pt_batch_stats *out);
 

@@ -139,7 +139,9 @@ class BatchStats(C.Structure):
     _fields_ = [("jobs", C.c_int64), ("ops", C.c_int64), ("launches", C.c_int64),
                 ("multi_launches", C.c_int64), ("ticks", C.c_int64), ("last_run_ms", C.c_double),
                 ("tune_ms", C.c_double), ("shape", (C.c_int32 * 3) * 2), ("xcd_remap", C.c_int32),
-                ("isolate_tiles", C.c_int32)]
+                ("isolate_tiles", C.c_int32), ("persistent", C.c_int32), ("persistent_grid", C.c_int32),
+                ("persistent_tasks", C.c_int64), ("persistent_tiles", C.c_int64),
+                ("persistent_max_chunk", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SequenceDesc(C.Structure):
@@ -160,6 +162,7 @@ STRUCTS = {0: PtParams, 1: PatchXF, 2: FrameGuides, 3: StyleGuides, 4: PtJob, 5:
            6: SequenceDesc, 7: ServerStats}
 
 PT_BATCH_USE_GRAPH, PT_BATCH_XCD_REMAP, PT_BATCH_SHAPE, PT_BATCH_ISOLATE_TILES = 1, 2, 3, 4
+PT_BATCH_PERSISTENT, PT_BATCH_PERSIST_CHUNK, PT_BATCH_PERSIST_TIMEOUT_MS, PT_BATCH_PERSIST_ABORT = 5, 6, 7, 8
 PT_OP_PROPAGATE, PT_OP_SCATTER = 0, 1
 SHAPE_NAMES = ["pixel", "pixel_ee", "unit_ee_a", "unit_a", "unit_ee_b", "unit_b", "tap"]
 

@@ -40,9 +40,27 @@ def _run(cmd):
 
 
 def build_hip():
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Compiles the translation units in parallel, then links them."""
+    inc = ["-I", CSRC, "-I", os.path.join(ROOT, "include")]
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs, procs = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        procs.append(subprocess.Popen([HIPCC] + cflags + inc + ["-c", os.path.join(CSRC, src), "-o", obj],
+                                      cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    failed = []
+    for src, p in zip(HIP_SOURCES, procs):
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(src)
+    if failed:
+        raise RuntimeError("build step failed: " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    for obj in objs:
+        os.remove(obj)
     os.replace(tmp, LIB)
     return LIB
 

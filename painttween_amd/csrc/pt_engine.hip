@@ -236,6 +236,20 @@ struct pt_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     double last_ms = 0.0, tune_ms = 0.0;
+    /* persistent form: every job's program in one launch */
+    int persistent = 0;
+    int pers_abort = 0; /* the next persistent launch starts with its give-up flag set */
+    bool pers_valid = false;
+    int pers_grid = 0, pers_max_chunk = 8;
+    double pers_timeout_ms = 2000.0;
+    int64_t pers_tasks = 0, pers_tiles = 0;
+    std::vector<int> pers_order; /* jobs by priority */
+    PtTask *pd_tasks = nullptr;
+    int *pd_first = nullptr;
+    PtArgs *pd_args = nullptr;
+    unsigned long long *pd_claim = nullptr;
+    int *pd_done = nullptr;
+    int *pd_ctl = nullptr;
 };
 
 static int bfail(pt_batch *b, int code, const char *fmt, ...)
@@ -262,6 +276,161 @@ static void drop_graph(pt_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->gexec = nullptr;
     b->graph = nullptr;
+}
+
+static void free_persistent(pt_batch *b)
+{
+    (void)hipFree(b->pd_tasks);
+    (void)hipFree(b->pd_first);
+    (void)hipFree(b->pd_args);
+    (void)hipFree(b->pd_claim);
+    (void)hipFree(b->pd_done);
+    (void)hipFree(b->pd_ctl);
+    b->pd_tasks = nullptr;
+    b->pd_first = nullptr;
+    b->pd_args = nullptr;
+    b->pd_claim = nullptr;
+    b->pd_done = nullptr;
+    b->pd_ctl = nullptr;
+    b->pers_valid = false;
+}
+
+static const char *op_name(int op)
+{
+    static const char *const n[] = {"propagate", "scatter", "residual refresh", "sync"};
+    return (op >= 0 && op < 4) ? n[op] : "operator";
+}
+
+/*
+ * The persistent form's device program: each job's recorded plan becomes a
+ * list of tasks, one per kernel the plain path would launch (operators with
+ * a multi-job form directly, closures through a task sink that captures
+ * their launches and copies).  Jobs are ordered by their tile count, most
+ * first, since the claim scan serves lower indices first: the longest chains
+ * of dependent steps start first and the short jobs fill the gaps.
+ */
+static int build_persistent(pt_batch *b)
+{
+    free_persistent(b);
+    const int n = (int)b->lanes.size();
+    std::vector<std::vector<PtTask>> jt(n);
+    std::vector<std::vector<PtArgs>> ja(n);
+    std::vector<int64_t> work(n, 0);
+    for (int j = 0; j < n; j++) {
+        int64_t cum = 0;
+        for (const RecOp &o : b->lanes[j].ops) {
+            std::vector<PtTask> got;
+            std::vector<PtArgs> args;
+            if (o.op >= 0) {
+                PtTask t;
+                const char *why = nullptr;
+                int shape = b->pol.pick(o.op, o.a);
+                int rc = pt_task_op(o.op, shape, o.a, 0, &t, &why);
+                if (rc != PT_OK && shape != pt_default_shape(o.op, o.a))
+                    rc = pt_task_op(o.op, pt_default_shape(o.op, o.a), o.a, 0, &t, &why);
+                if (rc != PT_OK)
+                    return bfail(b, PT_ERR_UNSUPPORTED, "persistent form: job %d, %s: %s", j, op_name(o.op), why);
+                if (t.ntiles > 0) {
+                    got.push_back(t);
+                    args.push_back(o.a);
+                }
+            } else {
+                PtTaskSink sink;
+                pt_sink_begin(&sink);
+                hipError_t e = o.fn(nullptr);
+                pt_sink_end();
+                if (sink.unsupported || e != hipSuccess)
+                    return bfail(b, PT_ERR_UNSUPPORTED, "persistent form: job %d, step '%s' has no device task (%s)", j,
+                                 o.what ? o.what : "?", sink.unsupported ? sink.unsupported : hipGetErrorString(e));
+                got = sink.tasks;
+                args = sink.args;
+            }
+            for (PtTask &t : got) {
+                if (t.arg >= 0) {
+                    ja[j].push_back(args[t.arg]);
+                    t.arg = (int)ja[j].size() - 1; /* rebased below */
+                }
+                cum += t.ntiles;
+                if (cum > (1ll << 30)) return bfail(b, PT_ERR_UNSUPPORTED, "persistent form: job %d has too many tiles", j);
+                t.end = (int)cum;
+                jt[j].push_back(t);
+            }
+        }
+        work[j] = cum;
+        if (jt[j].empty()) return bfail(b, PT_ERR_STATE, "persistent form: job %d has no work", j);
+    }
+    std::vector<int> order(n);
+    for (int j = 0; j < n; j++) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    std::vector<PtTask> tasks;
+    std::vector<PtArgs> args;
+    std::vector<int> first;
+    for (int j : order) {
+        first.push_back((int)tasks.size());
+        const int base = (int)args.size();
+        for (PtTask t : jt[j]) {
+            if (t.arg >= 0) t.arg += base;
+            tasks.push_back(t);
+        }
+        args.insert(args.end(), ja[j].begin(), ja[j].end());
+    }
+    first.push_back((int)tasks.size());
+    if (args.empty()) args.push_back(PtArgs{});
+    B_HIP(b, hipMalloc((void **)&b->pd_tasks, tasks.size() * sizeof(PtTask)));
+    B_HIP(b, hipMalloc((void **)&b->pd_first, first.size() * sizeof(int)));
+    B_HIP(b, hipMalloc((void **)&b->pd_args, args.size() * sizeof(PtArgs)));
+    B_HIP(b, hipMalloc((void **)&b->pd_claim, (size_t)n * sizeof(unsigned long long)));
+    B_HIP(b, hipMalloc((void **)&b->pd_done, (size_t)n * sizeof(int)));
+    B_HIP(b, hipMalloc((void **)&b->pd_ctl, 4 * sizeof(int)));
+    B_HIP(b, hipMemcpy(b->pd_tasks, tasks.data(), tasks.size() * sizeof(PtTask), hipMemcpyHostToDevice));
+    B_HIP(b, hipMemcpy(b->pd_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
+    B_HIP(b, hipMemcpy(b->pd_args, args.data(), args.size() * sizeof(PtArgs), hipMemcpyHostToDevice));
+    b->pers_grid = pt_persistent_grid(b->device);
+    if (b->pers_grid < 1) return bfail(b, PT_ERR_HIP, "persistent form: no occupancy for the persistent kernel");
+    b->pers_tasks = (int64_t)tasks.size();
+    b->pers_tiles = 0;
+    for (int64_t w : work) b->pers_tiles += w;
+    b->pers_order = order;
+    b->pers_valid = true;
+    return PT_OK;
+}
+
+/* One persistent launch over every job; the device-side state is cleared
+ * first, and the launch's outcome is read back. */
+static int run_persistent(pt_batch *b)
+{
+    if (!b->pers_valid) {
+        int rc = build_persistent(b);
+        if (rc != PT_OK) return rc;
+    }
+    const int n = (int)b->lanes.size();
+    B_HIP(b, hipMemsetAsync(b->pd_claim, 0, (size_t)n * sizeof(unsigned long long), b->stream));
+    B_HIP(b, hipMemsetAsync(b->pd_done, 0, (size_t)n * sizeof(int), b->stream));
+    const int ctl0[4] = {0, 0, b->pers_abort ? 1 : 0, 0};
+    b->pers_abort = 0;
+    B_HIP(b, hipMemcpy(b->pd_ctl, ctl0, sizeof(ctl0), hipMemcpyHostToDevice));
+    int khz = 0;
+    B_HIP(b, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+    PtPersist P;
+    P.tasks = b->pd_tasks;
+    P.first = b->pd_first;
+    P.args = b->pd_args;
+    P.claim = b->pd_claim;
+    P.done = b->pd_done;
+    P.ctl = b->pd_ctl;
+    P.njobs = n;
+    /* about two claims per workgroup for a step that fills the device */
+    P.chunk_div = 2 * b->pers_grid;
+    P.max_chunk = b->pers_max_chunk;
+    P.timeout_ticks = (long long)((double)(khz > 0 ? khz : 100000) * b->pers_timeout_ms);
+    B_HIP(b, pt_launch_persistent(P, b->pers_grid, b->stream));
+    B_HIP(b, hipStreamSynchronize(b->stream));
+    int ctl[4] = {0, 0, 0, 0};
+    B_HIP(b, hipMemcpy(ctl, b->pd_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+    if (ctl[2] || ctl[0] != n)
+        return bfail(b, PT_ERR_HIP, "persistent launch gave up: %d of %d jobs finished, no progress for %.0f ms", ctl[0],
+                     n, b->pers_timeout_ms);
+    return PT_OK;
 }
 
 static int build_program(pt_batch *b)
@@ -362,6 +531,7 @@ extern "C" int pt_batch_destroy(pt_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
+    free_persistent(b);
     if (b->tables_dev) (void)hipFree(b->tables_dev);
     for (auto *c : b->ctxs) pt_destroy(c);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -376,6 +546,12 @@ extern "C" int pt_batch_run(pt_batch *b)
     if (!b) return bfail(nullptr, PT_ERR_INVALID_ARG, "NULL batch");
     B_HIP(b, hipSetDevice(b->device));
     double t0 = now_ms();
+    if (b->persistent) {
+        int rc = run_persistent(b);
+        if (rc != PT_OK) return rc;
+        b->last_ms = now_ms() - t0;
+        return PT_OK;
+    }
     if (!b->prog_valid) {
         int rc = build_program(b);
         if (rc != PT_OK) return rc;
@@ -409,6 +585,29 @@ extern "C" int pt_batch_tune(pt_batch *b, int rounds)
     if (!b || rounds < 1) return bfail(b, PT_ERR_INVALID_ARG, "bad tune arguments");
     B_HIP(b, hipSetDevice(b->device));
     double t0 = now_ms();
+    if (b->persistent) {
+        /* the persistent form: tiles per claim, on whole-batch time */
+        const int cand[] = {1, 2, 4, 8, 16};
+        double best = 0.0;
+        int pick = b->pers_max_chunk;
+        for (int c : cand) {
+            b->pers_max_chunk = c;
+            double tt = 0.0;
+            for (int r = 0; r < rounds; r++) {
+                double t1 = now_ms();
+                int rc = run_persistent(b);
+                if (rc != PT_OK) return rc;
+                tt += now_ms() - t1;
+            }
+            if (c == cand[0] || tt < best) {
+                best = tt;
+                pick = c;
+            }
+        }
+        b->pers_max_chunk = pick;
+        b->tune_ms += now_ms() - t0;
+        return PT_OK;
+    }
     ShapePolicy keep = b->pol;
     /* time[op][class][shape], launches counted to check coverage */
     double tm[2][3][PT_SHAPE_COUNT] = {};
@@ -501,9 +700,20 @@ extern "C" int pt_batch_set(pt_batch *b, int key, int value)
         b->pol.shape[op][cls] = shape == 15 ? -1 : shape;
         break;
     }
+    case PT_BATCH_PERSISTENT: b->persistent = value != 0; return PT_OK;
+    case PT_BATCH_PERSIST_CHUNK:
+        if (value < 1 || value > 1024) return bfail(b, PT_ERR_INVALID_ARG, "tiles per claim must be 1..1024");
+        b->pers_max_chunk = value;
+        return PT_OK;
+    case PT_BATCH_PERSIST_TIMEOUT_MS:
+        if (value < 1) return bfail(b, PT_ERR_INVALID_ARG, "timeout must be >= 1 ms");
+        b->pers_timeout_ms = (double)value;
+        return PT_OK;
+    case PT_BATCH_PERSIST_ABORT: b->pers_abort = value != 0; return PT_OK;
     default: return bfail(b, PT_ERR_INVALID_ARG, "unknown batch key %d", key);
     }
     b->prog_valid = false;
+    b->pers_valid = false;
     drop_graph(b);
     return PT_OK;
 }
@@ -528,6 +738,15 @@ extern "C" int pt_batch_get_stats(pt_batch *b, pt_batch_stats *o)
         for (int c = 0; c < 3; c++) o->shape[op][c] = b->pol.shape[op][c];
     o->xcd_remap = b->xcd;
     o->isolate_tiles = b->pol.isolate_tiles;
+    o->persistent = b->persistent;
+    o->persistent_max_chunk = b->pers_max_chunk;
+    if (b->persistent && !b->pers_valid) {
+        int rc = build_persistent(b);
+        if (rc != PT_OK) return rc;
+    }
+    o->persistent_grid = b->pers_valid ? b->pers_grid : 0;
+    o->persistent_tasks = b->pers_valid ? b->pers_tasks : 0;
+    o->persistent_tiles = b->pers_valid ? b->pers_tiles : 0;
     return PT_OK;
 }
 

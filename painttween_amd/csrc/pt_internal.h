@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/pt_api.h"
 
 /* Nearest-neighbour field as structure-of-arrays planes: {x, y, theta,
@@ -204,5 +206,84 @@ hipError_t pt_launch_downsample_ribbon(const float *src, int pw, float *dst, int
 hipError_t pt_launch_fill_ribbon(pt_ribbonp *dst, size_t n, hipStream_t s);
 hipError_t pt_launch_pack_nnf(const PtNnf &src, int W, int lw, int lh, void *dst_patchxf, hipStream_t s);
 hipError_t pt_launch_unpack_nnf(const void *src_patchxf, int lw, int lh, PtNnf dst, int W, hipStream_t s);
+
+/* Device-to-device copies of recorded stream work go through these (same
+ * arguments as hipMemcpyAsync / hipMemcpy2DAsync), so that a task sink can
+ * capture them. */
+hipError_t pt_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
+hipError_t pt_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                     hipMemcpyKind kind, hipStream_t s);
+
+/* ------------------------------------------------------------------ */
+/* persistent batch launch                                             */
+/* ------------------------------------------------------------------ */
+/* One step of a job's device program: every workgroup tile of one kernel
+ * the plain path would launch.  The first four kinds are the PtOp values. */
+enum PtTaskKind {
+    PT_TASK_PROPAGATE = 0,
+    PT_TASK_SCATTER = 1,
+    PT_TASK_RESIDUAL = 2,
+    PT_TASK_SYNC = 3,
+    PT_TASK_COPY_RANDOM = 4,
+    PT_TASK_UPSAMPLE = 5,
+    PT_TASK_COPY = 6,
+    PT_TASK_FILL = 7,
+    PT_TASK_DOWNSAMPLE = 8,
+    PT_TASK_TENSOR_ANGLE = 9,
+    PT_TASK_INTERLEAVE = 10,
+    PT_TASK_RANDOM_PYRAMID = 11,
+    PT_TASK_PACK_NNF = 12,
+    PT_TASK_COUNT = 13
+};
+
+struct PtTask {
+    int kind;    /* PtTaskKind */
+    int variant; /* NNF kinds: shape << 4 | exemplar window << 3 | radius; copy: bytes per element;
+                    downsample: channels */
+    int ntiles;  /* workgroup tiles (> 0) */
+    int tiles_x; /* tiles per row of a 2-D tiling */
+    int end;     /* the job's tiles through this step (cumulative) */
+    int arg;     /* index of the step's PtArgs in the argument table, or -1 */
+    int i[8];    /* integer operands */
+    uint32_t u[2];
+    float f;
+    int pad;
+    const void *p[4]; /* source operands */
+    void *q;          /* destination operand */
+    int64_t n[3];     /* counts and pitches */
+};
+
+/* Task capture: while a sink is active on the calling thread, the
+ * launchers of pt_kernels.hip and pt_copy / pt_copy2d append tasks (and the
+ * PtArgs they take) instead of issuing work; a launcher without a task form
+ * records its name in `unsupported` and fails. */
+struct PtTaskSink {
+    std::vector<PtTask> tasks;
+    std::vector<PtArgs> args; /* args[k]: the PtArgs of a task with arg == k */
+    const char *unsupported = nullptr;
+};
+void pt_sink_begin(PtTaskSink *s);
+void pt_sink_end();
+
+/* The task of one multi-job operator step (op, shape) of job `a`, its
+ * arguments at `arg`; PT_ERR_UNSUPPORTED when the persistent kernel has no
+ * body for it (the message names what). */
+int pt_task_op(int op, int shape, const PtArgs &a, int arg, PtTask *t, const char **why);
+
+/* Device state of one persistent launch. */
+struct PtPersist {
+    const PtTask *tasks;      /* every job's program, concatenated        */
+    const int *first;         /* njobs + 1: job j's steps are tasks[first[j] .. first[j+1]) */
+    const PtArgs *args;
+    unsigned long long *claim; /* per job: step << 32 | next unclaimed tile */
+    int *done;                /* per job: tiles completed, all steps     */
+    int *ctl;                 /* [0] jobs finished, [1] claims completed, [2] gave up */
+    int njobs;
+    int chunk_div, max_chunk; /* tiles per claim: clamp(ntiles / chunk_div, 1, max_chunk) */
+    long long timeout_ticks;  /* wall-clock ticks without a completed claim before giving up */
+};
+/* Workgroups of the persistent kernel that fit on the device at once. */
+int pt_persistent_grid(int device);
+hipError_t pt_launch_persistent(const PtPersist &p, int grid, hipStream_t s);
 
 #endif

@@ -13,6 +13,8 @@
 #include "pt_internal.h"
 #include "pt_math.h"
 
+#include <string.h>
+
 #define PT_BX 16
 #define PT_BY 16
 /* Bandwidth-bound per-pixel kernels (sync/canvas, upsample, advection,
@@ -20,6 +22,27 @@
  * pixels of one row, so its float4 / float2 accesses are whole cache lines. */
 #define PT_RX 64
 #define PT_RY 4
+/* Every workgroup is 256 lanes in one dimension; a kernel that works on a
+ * BX-wide 2-D block of pixels takes lane (id % BX, id / BX), the layout a
+ * BX x (256 / BX) block would have (x fastest), so each wave covers the same
+ * pixels.  One-dimensional workgroups let the persistent batch kernel run
+ * every operator's tile body. */
+#define PT_LX(BX) ((int)threadIdx.x % (BX))
+#define PT_LY(BX) ((int)threadIdx.x / (BX))
+
+/* The LDS of a tile body is one struct: a kernel of its own gets it as a
+ * static __shared__ variable; the persistent batch kernel, which runs many
+ * tile bodies one after another in one workgroup, passes one arena that all
+ * of them share (template flag AR). */
+#define PT_LDS(TYPE, NAME)                                                                                \
+    TYPE *NAME##_p;                                                                                       \
+    if constexpr (AR) {                                                                                   \
+        NAME##_p = reinterpret_cast<TYPE *>(arena);                                                       \
+    } else {                                                                                              \
+        __shared__ TYPE NAME##_s;                                                                         \
+        NAME##_p = &NAME##_s;                                                                             \
+    }                                                                                                     \
+    TYPE &NAME = *NAME##_p;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -642,15 +665,15 @@ __device__ __forceinline__ void xcd_tile(const PtArgs &A, int &bx, int &by)
 
 __device__ __forceinline__ bool pixel_of_rows(const PtArgs &A, int bx, int by, int &x, int &y)
 {
-    x = bx * PT_RX + threadIdx.x;
-    y = by * PT_RY + threadIdx.y;
+    x = bx * PT_RX + PT_LX(PT_RX);
+    y = by * PT_RY + PT_LY(PT_RX);
     return x < A.lw && y < A.lh;
 }
 
 __device__ __forceinline__ bool pixel_of(const PtArgs &A, int bx, int by, int &x, int &y)
 {
-    x = bx * PT_BX + threadIdx.x;
-    y = by * PT_BY + threadIdx.y;
+    x = bx * PT_BX + PT_LX(PT_BX);
+    y = by * PT_BY + PT_LY(PT_BX);
     return x < A.lw && y < A.lh;
 }
 
@@ -669,7 +692,7 @@ __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, floa
     const int tw = PT_BX + 2 * r, th = PT_BY + 2 * r;
     const int x0 = bx * PT_BX - r, y0 = by * PT_BY - r;
     const float4 *color = A.color[A.cur];
-    for (int k = threadIdx.y * PT_BX + threadIdx.x; k < tw * th; k += PT_BX * PT_BY) {
+    for (int k = threadIdx.x; k < tw * th; k += PT_BX * PT_BY) {
         int tx = clampi(x0 + k % tw, 0, A.lw - 1);
         int ty = clampi(y0 + k / tw, 0, A.lh - 1);
         scol[k] = color[ty * A.W + tx];
@@ -680,8 +703,8 @@ __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, floa
     T.col = scol;
     T.can = scan;
     T.tw = tw;
-    T.lx = threadIdx.x;
-    T.ly = threadIdx.y;
+    T.lx = PT_LX(PT_BX);
+    T.ly = PT_LY(PT_BX);
     return T;
 }
 
@@ -699,25 +722,36 @@ __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, floa
  */
 #define PT_XW_RES 32 /* exemplar window side of the XL residual refresh */
 
-template <int RT, bool TK, bool XL>
-__device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int by)
+template <int RT, bool XL> struct ResLds {
+    float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
+    float4 sexw[XL ? PT_XW_RES * PT_XW_RES * 2 : 1];
+    unsigned char sflag[TileSize<RT>::N];
+    unsigned short slist[PT_BX * PT_BY];
+    int scount;
+};
+
+template <int RT, bool TK, bool XL, bool AR = false>
+__device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int by, void *arena = nullptr)
 {
-    __shared__ float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];
-    __shared__ float4 sexw[XL ? PT_XW_RES * PT_XW_RES * 2 : 1];
-    __shared__ unsigned char sflag[TileSize<RT>::N];
-    __shared__ unsigned short slist[PT_BX * PT_BY];
-    __shared__ int scount;
+    using Lds = ResLds<RT, XL>;
+    PT_LDS(Lds, L);
+    auto &scol = L.scol;
+    auto &scan = L.scan;
+    auto &sexw = L.sexw;
+    auto &sflag = L.sflag;
+    auto &slist = L.slist;
+    int &scount = L.scount;
     const int r = RT > 0 ? RT : A.r;
     const bool inc = A.res_incremental != 0;
     if (inc) {
         const int tw = PT_BX + 2 * r, th = PT_BY + 2 * r;
         const int x0 = bx * PT_BX - r, y0 = by * PT_BY - r;
-        for (int k = threadIdx.y * PT_BX + threadIdx.x; k < tw * th; k += PT_BX * PT_BY) {
+        for (int k = threadIdx.x; k < tw * th; k += PT_BX * PT_BY) {
             int tx = clampi(x0 + k % tw, 0, A.lw - 1);
             int ty = clampi(y0 + k / tw, 0, A.lh - 1);
             sflag[k] = A.changed[ty * A.W + tx];
         }
-        if (threadIdx.x == 0 && threadIdx.y == 0) scount = 0;
+        if (threadIdx.x == 0) scount = 0;
     }
     int x, y;
     bool valid = pixel_of(A, bx, by, x, y);
@@ -725,7 +759,7 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
     /* XL: the window is anchored on the offset of the tile's middle pixel
      * (clamped into the level) and read by the pixels whose patch lies in it */
     auto window = [&]() {
-        const int tid = threadIdx.y * PT_BX + threadIdx.x;
+        const int tid = threadIdx.x;
         int ax = min(bx * PT_BX + PT_BX / 2, A.lw - 1), ay = min(by * PT_BY + PT_BY / 2, A.lh - 1);
         float4 an = A.base.xyth[ay * A.W + ax];
         stage_window(A, an.x, an.y, PT_XW_RES, sexw, T.wx, T.wy, tid, PT_BX * PT_BY);
@@ -757,16 +791,16 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
         bool need = false;
         for (int j = 0; j <= 2 * r && !need; j++)
             for (int i = 0; i <= 2 * r; i++)
-                if (sflag[(threadIdx.y + j) * T.tw + threadIdx.x + i]) {
+                if (sflag[(PT_LY(PT_BX) + j) * T.tw + PT_LX(PT_BX) + i]) {
                     need = true;
                     break;
                 }
-        if (need) slist[atomicAdd(&scount, 1)] = (unsigned short)(threadIdx.y * PT_BX + threadIdx.x);
+        if (need) slist[atomicAdd(&scount, 1)] = (unsigned short)threadIdx.x;
     }
     __syncthreads();
     const int n = scount;
     if (XL && n > 0) window();
-    const int lane = threadIdx.y * PT_BX + threadIdx.x;
+    const int lane = threadIdx.x;
     if (lane >= n) return;
     const int p = slist[lane];
     x = bx * PT_BX + p % PT_BX;
@@ -816,8 +850,8 @@ __device__ __forceinline__ bool bits_differ(float a, float b) { return __float_a
 
 __device__ __forceinline__ void b_sync_canvas(const PtArgs &A, int bx, int by, int dw, int dh)
 {
-    int x = bx * PT_RX + threadIdx.x;
-    int y = by * PT_RY + threadIdx.y;
+    int x = bx * PT_RX + PT_LX(PT_RX);
+    int y = by * PT_RY + PT_LY(PT_RX);
     int o = y * A.W + x;
     float4 n;
     int nl = 0;
@@ -861,10 +895,10 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_sync_canvas(PtArgs A, int dw, 
 }
 
 /* copyRandomKernel (texSynth_kernel.cu:2044-2059) */
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A)
+__device__ __forceinline__ void b_copy_random(const PtArgs &A, int bx, int by)
 {
     int x, y;
-    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
+    if (!pixel_of_rows(A, bx, by, x, y)) return;
     int o = y * A.W + x;
     float4 r = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
     A.work.xyth[o] = make_float4(r.x, r.y, r.z, 0.f);
@@ -873,6 +907,8 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A)
     /* toPatchXF(random): the style layer is the texel's w (texSynth_util.h:212-214) */
     if (A.layered) A.work.la[o] = (int)r.w;
 }
+
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_copy_random(PtArgs A) { b_copy_random(A, blockIdx.x, blockIdx.y); }
 
 /* calculatePropagateOffset (texSynth_kernel.cu:956-982); pixelWrap
  * (cudaImageBuffer.h:81-87) wraps modulo the level-0 dimensions.  With cl
@@ -1937,27 +1973,57 @@ template <int P> __device__ __forceinline__ void up_time_load(const UpTime<P> &S
  * 120.9 / 119.7 / 122.6 / 123.8 ms per frame). */
 #define PT_XW_PROP 24
 
-template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false>
-__device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, int jump)
-{
-    constexpr int P = TX * TY, U = 4 * P;
-    static_assert(!XL || RT > 0, "exemplar windows need a fixed radius");
-    __shared__ UpTime<TM ? P : 1> s_tm;
+template <int RT, int TX, int TY, bool TM, bool XL> struct PropUpLds {
+    static constexpr int P = TX * TY, U = 4 * P, N = UpTile<RT, TX, TY>::N;
+    UpTime<TM ? P : 1> s_tm;
     /* XL: a window of the interleaved exemplar anchored on the current offset
      * of the tile's middle pixel; units whose every tap footprint lies in it
      * read their taps from LDS (same texels, weights and order) */
-    __shared__ float4 s_exw[XL ? PT_XW_PROP * PT_XW_PROP * 2 : 1];
-    __shared__ unsigned char s_exl[XL ? U : 1];
+    float4 s_exw[XL ? PT_XW_PROP * PT_XW_PROP * 2 : 1];
+    unsigned char s_exl[XL ? U : 1];
     /* TM: the advected set of the launch's direction over the tile */
-    __shared__ float4 s_ab[TM ? UpTile<RT, TX, TY>::N : 1], s_ao[TM ? UpTile<RT, TX, TY>::N : 1];
-    __shared__ unsigned char s_av[TM ? UpTile<RT, TX, TY>::N : 1];
-    __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
-    __shared__ float s_best[P], s_sw[P], s_ia[P], s_io[P];
-    __shared__ float2 s_sc[P];
-    __shared__ float s_r[U];
-    __shared__ float s_cx[U], s_cy[U], s_b0x[U], s_b0y[U], s_b1x[U], s_b1y[U], s_scy[U];
-    __shared__ unsigned short s_list[U];
-    __shared__ int s_n, s_next;
+    float4 s_ab[TM ? N : 1], s_ao[TM ? N : 1];
+    unsigned char s_av[TM ? N : 1];
+    float4 scol[N], scan[N];
+    float s_best[P], s_sw[P], s_ia[P], s_io[P];
+    float2 s_sc[P];
+    float s_r[U];
+    float s_cx[U], s_cy[U], s_b0x[U], s_b0y[U], s_b1x[U], s_b1y[U], s_scy[U];
+    unsigned short s_list[U];
+    int s_n, s_next;
+};
+
+template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false, bool AR = false>
+__device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, int jump, void *arena = nullptr)
+{
+    constexpr int P = TX * TY, U = 4 * P;
+    static_assert(!XL || RT > 0, "exemplar windows need a fixed radius");
+    using Lds = PropUpLds<RT, TX, TY, TM, XL>;
+    PT_LDS(Lds, L);
+    auto &s_tm = L.s_tm;
+    auto &s_exw = L.s_exw;
+    auto &s_exl = L.s_exl;
+    auto &s_ab = L.s_ab;
+    auto &s_ao = L.s_ao;
+    auto &s_av = L.s_av;
+    auto &scol = L.scol;
+    auto &scan = L.scan;
+    auto &s_best = L.s_best;
+    auto &s_sw = L.s_sw;
+    auto &s_ia = L.s_ia;
+    auto &s_io = L.s_io;
+    auto &s_sc = L.s_sc;
+    auto &s_r = L.s_r;
+    auto &s_cx = L.s_cx;
+    auto &s_cy = L.s_cy;
+    auto &s_b0x = L.s_b0x;
+    auto &s_b0y = L.s_b0y;
+    auto &s_b1x = L.s_b1x;
+    auto &s_b1y = L.s_b1y;
+    auto &s_scy = L.s_scy;
+    auto &s_list = L.s_list;
+    int &s_n = L.s_n;
+    int &s_next = L.s_next;
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
@@ -2111,23 +2177,48 @@ __global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int ju
 /* scatter_kernel (texSynth_kernel.cu:865-952), unit-parallel in rounds.
  * n = number of random samples (the reference's loop bound, identical for
  * every pixel); unit k == n is the averaged-neighbour candidate. */
-template <int RT, bool EE, int TX, int TY, bool TM>
-__device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, int n)
+template <int RT, int TX, int TY, bool TM> struct ScatUpLds {
+    static constexpr int P = TX * TY, S = PT_UP_MAX_SAMPLES + 1, N = UpTile<RT, TX, TY>::N;
+    UpTime<TM ? P : 1> s_tm;
+    float4 scol[N], scan[N];
+    float4 s_ow[P];
+    float2 s_osc[P];
+    float s_best[P], s_sw[P], s_ia[P], s_cached[P], s_iori[P];
+    float2 s_sc[P];
+    int s_start[P];
+    float s_dist[S];
+    float s_r[P * S];
+    int s_next, s_more;
+    float4 s_ab[TM ? N : 1], s_ao[TM ? N : 1];
+    unsigned char s_av[TM ? N : 1];
+};
+
+template <int RT, bool EE, int TX, int TY, bool TM, bool AR = false>
+__device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, int n, void *arena = nullptr)
 {
     constexpr int P = TX * TY;
     constexpr int S = PT_UP_MAX_SAMPLES + 1;
-    __shared__ UpTime<TM ? P : 1> s_tm;
-    __shared__ float4 scol[UpTile<RT, TX, TY>::N], scan[UpTile<RT, TX, TY>::N];
-    __shared__ float4 s_ow[P];
-    __shared__ float2 s_osc[P];
-    __shared__ float s_best[P], s_sw[P], s_ia[P], s_cached[P], s_iori[P];
-    __shared__ float2 s_sc[P];
-    __shared__ int s_start[P];
-    __shared__ float s_dist[S];
-    __shared__ float s_r[P * S];
-    __shared__ int s_next, s_more;
-    __shared__ float4 s_ab[TM ? UpTile<RT, TX, TY>::N : 1], s_ao[TM ? UpTile<RT, TX, TY>::N : 1];
-    __shared__ unsigned char s_av[TM ? UpTile<RT, TX, TY>::N : 1];
+    using Lds = ScatUpLds<RT, TX, TY, TM>;
+    PT_LDS(Lds, L);
+    auto &s_tm = L.s_tm;
+    auto &scol = L.scol;
+    auto &scan = L.scan;
+    auto &s_ow = L.s_ow;
+    auto &s_osc = L.s_osc;
+    auto &s_best = L.s_best;
+    auto &s_sw = L.s_sw;
+    auto &s_ia = L.s_ia;
+    auto &s_cached = L.s_cached;
+    auto &s_iori = L.s_iori;
+    auto &s_sc = L.s_sc;
+    auto &s_start = L.s_start;
+    auto &s_dist = L.s_dist;
+    auto &s_r = L.s_r;
+    int &s_next = L.s_next;
+    int &s_more = L.s_more;
+    auto &s_ab = L.s_ab;
+    auto &s_ao = L.s_ao;
+    auto &s_av = L.s_av;
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     const int SU = n + 1; /* units per pixel */
@@ -2274,10 +2365,10 @@ __global__ __launch_bounds__(PT_UP_THREADS) void k_scatter_up(PtArgs A, int n)
 
 /* upsampleBilinearKernel (texSynth_kernel.cu:1453-1510) with
  * offsetUpsampleNearest (:1403-1417). */
-__global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A)
+__device__ __forceinline__ void b_upsample(const PtArgs &A, int bx, int by)
 {
     int x, y;
-    if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
+    if (!pixel_of_rows(A, bx, by, x, y)) return;
     if (!in_bounds(x, y, A.lw, A.lh, 1)) return;
     float cxf = (float)((double)x - 0.5) / 2.0f;
     float cyf = (float)((double)y - 0.5) / 2.0f;
@@ -2317,6 +2408,8 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A)
     if (A.layered) A.work.la[o] = l00;
     A.res_work[o] = rw;
 }
+
+__global__ __launch_bounds__(PT_BX *PT_BY) void k_upsample(PtArgs A) { b_upsample(A, blockIdx.x, blockIdx.y); }
 
 /* calculateTranslatedPosition (texSynth_kernel.cu:1541-1575) */
 __device__ __forceinline__ bool translated_position(const PtArgs &A, int fwd, int x, int y, float &tx, float &ty)
@@ -2680,12 +2773,12 @@ __device__ __forceinline__ void ds_set(float4 &v, int k, float f)
 }
 
 template <int CH>
-__global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_v(const float *src, int pw, float *dst, int lw, int lh,
-                                                              int scale_mode)
+__device__ __forceinline__ void b_downsample_v(const float *src, int pw, float *dst, int lw, int lh, int scale_mode,
+                                               int bx, int by)
 {
     typedef typename DsVec<CH>::T V;
-    int x = blockIdx.x * PT_RX + threadIdx.x;
-    int y = blockIdx.y * PT_RY + threadIdx.y;
+    int x = bx * PT_RX + PT_LX(PT_RX);
+    int y = by * PT_RY + PT_LY(PT_RX);
     if (x >= lw || y >= lh) return;
     const V *s = (const V *)src;
     V a = s[(y * 2) * pw + x * 2], b = s[(y * 2) * pw + x * 2 + 1];
@@ -2706,25 +2799,32 @@ __global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_v(const float *src,
     ((V *)dst)[y * lw + x] = o;
 }
 
+template <int CH>
+__global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_v(const float *src, int pw, float *dst, int lw, int lh,
+                                                              int scale_mode)
+{
+    b_downsample_v<CH>(src, pw, dst, lw, lh, scale_mode, blockIdx.x, blockIdx.y);
+}
+
 /* initializeRibbonPyramid (imagePyramid.cpp:211-235): the top-left texel
  * of each 2x2 block as a ribbon -- time step and layer through int, x and y
  * halved -- packed back as (time_step, y, x, layer). */
 __global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_ribbon(const float4 *src, int pw, float4 *dst, int lw,
                                                                    int lh)
 {
-    int x = blockIdx.x * PT_RX + threadIdx.x;
-    int y = blockIdx.y * PT_RY + threadIdx.y;
+    int x = blockIdx.x * PT_RX + PT_LX(PT_RX);
+    int y = blockIdx.y * PT_RY + PT_LY(PT_RX);
     if (x >= lw || y >= lh) return;
     float4 q = src[(y * 2) * pw + x * 2];
     dst[y * lw + x] = make_float4((float)(int)q.x, q.y * 0.5f, q.z * 0.5f, (float)(int)q.w);
 }
 
 /* other element widths (the structure tensor, 3 floats) */
-__global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample(const float *src, int pw, float *dst, int lw, int lh,
-                                                            int ch, int scale_mode)
+__device__ __forceinline__ void b_downsample(const float *src, int pw, float *dst, int lw, int lh, int ch,
+                                             int scale_mode, int bx, int by)
 {
-    int x = blockIdx.x * PT_RX + threadIdx.x;
-    int y = blockIdx.y * PT_RY + threadIdx.y;
+    int x = bx * PT_RX + PT_LX(PT_RX);
+    int y = by * PT_RY + PT_LY(PT_RX);
     if (x >= lw || y >= lh) return;
     for (int k = 0; k < ch; k++) {
         float s = 0.f;
@@ -2739,12 +2839,18 @@ __global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample(const float *src, i
     }
 }
 
+__global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample(const float *src, int pw, float *dst, int lw, int lh,
+                                                            int ch, int scale_mode)
+{
+    b_downsample(src, pw, dst, lw, lh, ch, scale_mode, blockIdx.x, blockIdx.y);
+}
+
 /* downsampleVote (downsample.cpp:162-210) for object-ID pyramids: the most
  * frequent of the four IDs, a tie going to the first in scan order. */
 __global__ __launch_bounds__(PT_RX *PT_RY) void k_downsample_vote(const int *src, int pw, int *dst, int lw, int lh)
 {
-    int x = blockIdx.x * PT_RX + threadIdx.x;
-    int y = blockIdx.y * PT_RY + threadIdx.y;
+    int x = blockIdx.x * PT_RX + PT_LX(PT_RX);
+    int y = blockIdx.y * PT_RY + PT_LY(PT_RX);
     if (x >= lw || y >= lh) return;
     int v[4] = {src[(y * 2) * pw + x * 2], src[(y * 2) * pw + x * 2 + 1], src[(y * 2 + 1) * pw + x * 2],
                 src[(y * 2 + 1) * pw + x * 2 + 1]};
@@ -2782,23 +2888,30 @@ __global__ void k_tensor_angle(const float *t, float *a, size_t n)
 
 /* Writes a w x h level into slot `slot` (0 base, 1 output) of the
  * interleaved exemplar texture of width ew. */
-__global__ void k_interleave(const float4 *src, int w, int h, float4 *ex, int ew, int slot)
+__device__ __forceinline__ void b_interleave(const float4 *src, int w, int h, float4 *ex, int ew, int slot, int bx,
+                                             int by)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x = bx * 16 + PT_LX(16);
+    int y = by * 16 + PT_LY(16);
     if (x >= w || y >= h) return;
     ex[2 * (y * ew + x) + slot] = src[y * w + x];
+}
+
+__global__ void k_interleave(const float4 *src, int w, int h, float4 *ex, int ew, int slot)
+{
+    b_interleave(src, w, h, ex, ew, slot, blockIdx.x, blockIdx.y);
 }
 
 /* createRandomPyramid (synthesisFrame.cpp:102-143) for one level, Philox
  * draws: key (seed, frame*level), counter (x, y, trial, 0), fractional parts
  * from counter (x, y, 50, 0). */
-__global__ void k_random_pyramid(const float4 *color, const float *iorient, int w, int h, const float4 *eout,
-                                 const float *eorient, int ew, int eh, int transp, int orient_en, uint32_t key0,
-                                 uint32_t key1, float4 *out)
+__device__ __forceinline__ void b_random_pyramid(const float4 *color, const float *iorient, int w, int h,
+                                                 const float4 *eout, const float *eorient, int ew, int eh, int transp,
+                                                 int orient_en, uint32_t key0, uint32_t key1, float4 *out, int bx,
+                                                 int by)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x = bx * 16 + PT_LX(16);
+    int y = by * 16 + PT_LY(16);
     if (x >= w || y >= h) return;
     uint32_t key[2] = {key0, key1};
     uint32_t ctr[4] = {(uint32_t)x, (uint32_t)y, 0u, 0u}, o[4];
@@ -2819,6 +2932,14 @@ __global__ void k_random_pyramid(const float4 *color, const float *iorient, int 
     out[y * w + x] = make_float4((float)ox + rx, (float)oy + ry, orientation, 0.0f);
 }
 
+__global__ void k_random_pyramid(const float4 *color, const float *iorient, int w, int h, const float4 *eout,
+                                 const float *eorient, int ew, int eh, int transp, int orient_en, uint32_t key0,
+                                 uint32_t key1, float4 *out)
+{
+    b_random_pyramid(color, iorient, w, h, eout, eorient, ew, eh, transp, orient_en, key0, key1, out, blockIdx.x,
+                     blockIdx.y);
+}
+
 /* The same with keyframes: the style of each pixel drawn first from
  * counter (x, y, 51, 0), rand() % keys + first key (synthesisFrame.cpp:
  * 107-110), its exemplar used for the trials and the orientation, the style
@@ -2833,8 +2954,8 @@ __global__ void k_random_pyramid_keys(const float4 *color, const float *iorient,
                                       int key_first, int num_keys, int transp, int orient_en, uint32_t key0,
                                       uint32_t key1, float4 *out)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x = blockIdx.x * 16 + PT_LX(16);
+    int y = blockIdx.y * 16 + PT_LY(16);
     if (x >= w || y >= h) return;
     uint32_t key[2] = {key0, key1};
     uint32_t ctr[4] = {(uint32_t)x, (uint32_t)y, 51u, 0u}, o[4];
@@ -2866,10 +2987,10 @@ struct PatchXF36 {
     float su, sv, ly, la;
 };
 
-__global__ void k_pack_nnf(PtNnf s, int W, int lw, int lh, PatchXF36 *d)
+__device__ __forceinline__ void b_pack_nnf(const PtNnf &s, int W, int lw, int lh, PatchXF36 *d, int bx, int by)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x = bx * 16 + PT_LX(16);
+    int y = by * 16 + PT_LY(16);
     if (x >= lw || y >= lh) return;
     float4 v = s.xyth[y * W + x];
     float2 c = s.sc[y * W + x];
@@ -2877,10 +2998,12 @@ __global__ void k_pack_nnf(PtNnf s, int W, int lw, int lh, PatchXF36 *d)
     d[y * lw + x] = p;
 }
 
+__global__ void k_pack_nnf(PtNnf s, int W, int lw, int lh, PatchXF36 *d) { b_pack_nnf(s, W, lw, lh, d, blockIdx.x, blockIdx.y); }
+
 __global__ void k_unpack_nnf(const PatchXF36 *s, int lw, int lh, PtNnf d, int W)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x = blockIdx.x * 16 + PT_LX(16);
+    int y = blockIdx.y * 16 + PT_LY(16);
     if (x >= lw || y >= lh) return;
     PatchXF36 p = s[y * lw + x];
     d.xyth[y * W + x] = make_float4(p.x, p.y, p.theta, p.hysteresis);
@@ -3011,12 +3134,19 @@ __device__ __forceinline__ void tap_phase(const PtArgs &A, const TapUnits<NUM, T
 
 /* updateResidualCache_kernel, tap-parallel: workgroup g refreshes pixels
  * g*NP .. g*NP+NP-1 of the level in row-major order. */
-template <int RT, bool TK>
-__device__ __forceinline__ void b_update_residual_tp(const PtArgs &A, int g)
+template <int RT, bool TK> struct ResTapLds {
+    TapUnits<TapCfg<RT>::RES_MAX, TK> su;
+    float sv[TapCfg<RT>::RES_MAX * TapCfg<RT>::T];
+};
+
+template <int RT, bool TK, bool AR = false>
+__device__ __forceinline__ void b_update_residual_tp(const PtArgs &A, int g, void *arena = nullptr)
 {
-    constexpr int T = TapCfg<RT>::T, NPM = TapCfg<RT>::RES_MAX;
-    __shared__ TapUnits<NPM, TK> su;
-    __shared__ float sv[NPM * T];
+    constexpr int T = TapCfg<RT>::T;
+    using Lds = ResTapLds<RT, TK>;
+    PT_LDS(Lds, L);
+    auto &su = L.su;
+    auto &sv = L.sv;
     const int tid = threadIdx.x, NP = A.tap_np_res;
     if (tid < NP) {
         int q = g * NP + tid;
@@ -3068,14 +3198,23 @@ __device__ __forceinline__ void b_update_residual_tp(const PtArgs &A, int g)
 
 /* propagate_kernel, tap-parallel: workgroup g searches pixels g*NP ..
  * g*NP+NP-1; unit u = 4*pixel + candidate. */
-template <int RT, bool TK>
-__device__ __forceinline__ void b_propagate_tp(const PtArgs &A, int g, int jump)
+template <int RT, bool TK> struct PropTapLds {
+    static constexpr int NUM = 4 * TapCfg<RT>::PROP_MAX;
+    TapUnits<NUM, TK> su;
+    float sv[NUM * TapCfg<RT>::T];
+    float sr[NUM];
+};
+
+template <int RT, bool TK, bool AR = false>
+__device__ __forceinline__ void b_propagate_tp(const PtArgs &A, int g, int jump, void *arena = nullptr)
 {
-    constexpr int T = TapCfg<RT>::T, NPM = TapCfg<RT>::PROP_MAX, NUM = 4 * NPM;
+    constexpr int T = TapCfg<RT>::T, NPM = TapCfg<RT>::PROP_MAX;
     static_assert(NPM >= 1, "tap-parallel propagate needs 4 (2r+1)^2 <= PT_TAP_EVALS");
-    __shared__ TapUnits<NUM, TK> su;
-    __shared__ float sv[NUM * T];
-    __shared__ float sr[NUM];
+    using Lds = PropTapLds<RT, TK>;
+    PT_LDS(Lds, L);
+    auto &su = L.su;
+    auto &sv = L.sv;
+    auto &sr = L.sr;
     const int tid = threadIdx.x, NP = A.tap_np_prop, NU = 4 * NP;
     if (tid < NU) {
         int k = tid >> 2, cand = tid & 3;
@@ -3287,15 +3426,68 @@ __global__ __launch_bounds__(PT_UP_THREADS) void km_scatter_up(PT_MULTI_PARAMS)
 /* ------------------------------------------------------------------ */
 /* launchers                                                           */
 /* ------------------------------------------------------------------ */
+/* Task capture (pt_sink_begin): while a sink is active on the calling
+ * thread, a launcher appends the task form of its kernel instead of
+ * launching it. */
+static thread_local PtTaskSink *g_sink = nullptr;
+
+/* elements per workgroup tile of the copy and fill tasks (4 per lane) */
+#define PT_TASK_ELEMS 1024
+
+void pt_sink_begin(PtTaskSink *s) { g_sink = s; }
+void pt_sink_end() { g_sink = nullptr; }
+
+static PtTask task_new(int kind, int64_t ntiles, int tiles_x)
+{
+    PtTask t;
+    memset(&t, 0, sizeof(t));
+    t.kind = kind;
+    t.ntiles = (int)ntiles;
+    t.tiles_x = tiles_x > 0 ? tiles_x : 1;
+    t.arg = -1;
+    return t;
+}
+
+/* tasks without tiles are dropped: they do no work */
+static hipError_t sink_push(const PtTask &t, const PtArgs *a = nullptr)
+{
+    if (t.ntiles <= 0) return hipSuccess;
+    PtTask u = t;
+    if (a) {
+        u.arg = (int)g_sink->args.size();
+        g_sink->args.push_back(*a);
+    }
+    g_sink->tasks.push_back(u);
+    return hipSuccess;
+}
+
+static hipError_t sink_refuse(const char *what)
+{
+    if (!g_sink->unsupported) g_sink->unsupported = what;
+    return hipErrorNotSupported;
+}
+
+/* a 2-D tiling of w x h pixels in tx x ty tiles */
+static PtTask task_2d(int kind, int w, int h, int tx, int ty)
+{
+    int nx = w > 0 ? (w + tx - 1) / tx : 0, ny = h > 0 ? (h + ty - 1) / ty : 0;
+    return task_new(kind, (int64_t)nx * ny, nx);
+}
+
+#define PT_SINK_REFUSE(name)                                                                              \
+    do {                                                                                                  \
+        if (g_sink) return sink_refuse(name);                                                             \
+    } while (0)
+
 /* the extended patch-distance terms (coherence, LAB, object IDs) and style
  * layers (keyframe interpolation) run in the per-pixel kernels' RT = -1
  * variant */
 static bool ext_terms(const PtArgs &a) { return a.coh_w > 0.f || a.lab || a.obj_w > 0.f || a.layered; }
 
 static dim3 grid_for(int w, int h) { return dim3((w + PT_BX - 1) / PT_BX, (h + PT_BY - 1) / PT_BY); }
-static const dim3 kBlock(PT_BX, PT_BY);
+static const dim3 kBlock(PT_BX * PT_BY);
 static dim3 grid_rows(int w, int h) { return dim3((w + PT_RX - 1) / PT_RX, (h + PT_RY - 1) / PT_RY); }
-static const dim3 kRowBlock(PT_RX, PT_RY);
+static const dim3 kRowBlock(PT_RX * PT_RY);
 
 #define PT_DISPATCH_R(KERNEL, A, ...)                                                                     \
     do {                                                                                                  \
@@ -3317,8 +3509,17 @@ static const dim3 kRowBlock(PT_RX, PT_RY);
         }                                                                                                 \
     } while (0)
 
+static hipError_t sink_op(int op, const PtArgs &a)
+{
+    PtTask t;
+    const char *why = nullptr;
+    if (pt_task_op(op, pt_default_shape(op, a), a, 0, &t, &why) != PT_OK) return sink_refuse(why);
+    return sink_push(t, &a);
+}
+
 hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s)
 {
+    if (g_sink) return sink_op(PT_OP_RESIDUAL, a);
     if (pt_default_shape(PT_OP_RESIDUAL, a) == PT_SHAPE_TAP) {
         dim3 g(pt_shape_tiles(PT_OP_RESIDUAL, PT_SHAPE_TAP, a, nullptr)), bu(PT_UP_THREADS);
         bool tk = a.time_active != 0;
@@ -3351,12 +3552,23 @@ hipError_t pt_launch_update_residual(const PtArgs &a, hipStream_t s)
 
 hipError_t pt_launch_canvas(const PtArgs &a, hipStream_t s)
 {
+    if (g_sink) {
+        PtArgs c = a;
+        c.dw = c.dh = 0;
+        return sink_op(PT_OP_SYNC_CANVAS, c);
+    }
     hipLaunchKernelGGL(k_canvas, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_sync_canvas(const PtArgs &a, int dw, int dh, hipStream_t s)
 {
+    if (g_sink) {
+        PtArgs c = a;
+        c.dw = dw;
+        c.dh = dh;
+        return sink_op(PT_OP_SYNC_CANVAS, c);
+    }
     int gw = dw > a.lw ? dw : a.lw, gh = dh > a.lh ? dh : a.lh;
     hipLaunchKernelGGL(k_sync_canvas, grid_rows(gw, gh), kRowBlock, 0, s, a, dw, dh);
     return hipGetLastError();
@@ -3364,6 +3576,7 @@ hipError_t pt_launch_sync_canvas(const PtArgs &a, int dw, int dh, hipStream_t s)
 
 hipError_t pt_launch_copy_random(const PtArgs &a, hipStream_t s)
 {
+    if (g_sink) return sink_push(task_2d(PT_TASK_COPY_RANDOM, a.lw, a.lh, PT_RX, PT_RY), &a);
     hipLaunchKernelGGL(k_copy_random, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a);
     return hipGetLastError();
 }
@@ -3525,6 +3738,7 @@ hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
 {
     PtArgs a = a0;
     a.jump = jump;
+    if (g_sink) return sink_op(PT_OP_PROPAGATE, a);
     int shape = pt_default_shape(PT_OP_PROPAGATE, a);
     bool tk = a.time_active != 0;
     if (shape == PT_SHAPE_TAP) {
@@ -3570,6 +3784,7 @@ hipError_t pt_launch_scatter(const PtArgs &a0, hipStream_t s)
 {
     PtArgs a = a0;
     a.nscat = scatter_count(a);
+    if (g_sink) return sink_op(PT_OP_SCATTER, a);
     const int n = a.nscat;
     int shape = pt_default_shape(PT_OP_SCATTER, a);
     int tx, ty;
@@ -3595,6 +3810,7 @@ hipError_t pt_launch_scatter(const PtArgs &a0, hipStream_t s)
 
 hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable &t, int ntiles, hipStream_t s)
 {
+    PT_SINK_REFUSE("multi-job launch");
     if (ntiles <= 0) return hipSuccess;
     if (!pt_shape_valid(op, shape, a)) return hipErrorInvalidValue;
     dim3 g(ntiles), bu(PT_UP_THREADS);
@@ -3683,42 +3899,49 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
 
 hipError_t pt_launch_upsample(const PtArgs &a, hipStream_t s)
 {
+    if (g_sink) return sink_push(task_2d(PT_TASK_UPSAMPLE, a.lw, a.lh, PT_RX, PT_RY), &a);
     hipLaunchKernelGGL(k_upsample, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_advect(const PtArgs &a, int forwards, int time_step, int flags, hipStream_t s)
 {
+    PT_SINK_REFUSE("advection");
     hipLaunchKernelGGL(k_advect, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, forwards, time_step, flags);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_time_table(const PtArgs &a, int k, float4 *tt, hipStream_t s)
 {
+    PT_SINK_REFUSE("time table");
     hipLaunchKernelGGL(k_time_table, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, k, tt);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_merge_randomly(const PtArgs &a, int forwards, float weight, int force, hipStream_t s)
 {
+    PT_SINK_REFUSE("random merge");
     hipLaunchKernelGGL(k_merge_randomly, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, forwards, weight, force);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_merge_residual(const PtArgs &a, int forwards, hipStream_t s)
 {
+    PT_SINK_REFUSE("residual merge");
     hipLaunchKernelGGL(k_merge_residual, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, forwards);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_merge_advected(const PtArgs &a, int fallback, hipStream_t s)
 {
+    PT_SINK_REFUSE("advected merge");
     hipLaunchKernelGGL(k_merge_advected, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, fallback);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_offsets_from_ribbon(const PtArgs &a, int look_forwards, hipStream_t s)
 {
+    PT_SINK_REFUSE("ribbon offsets");
     hipLaunchKernelGGL(k_offsets_from_ribbon, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, look_forwards);
     return hipGetLastError();
 }
@@ -3726,6 +3949,7 @@ hipError_t pt_launch_offsets_from_ribbon(const PtArgs &a, int look_forwards, hip
 hipError_t pt_launch_accumulate_ribbon(const PtArgs &a, const pt_ribbonp *src, pt_ribbonp *dst, int forwards,
                                        int time_step, int overwrite, hipStream_t s)
 {
+    PT_SINK_REFUSE("ribbon accumulation");
     hipLaunchKernelGGL(k_accumulate_ribbon, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, src, dst, forwards, time_step,
                        overwrite);
     return hipGetLastError();
@@ -3733,12 +3957,14 @@ hipError_t pt_launch_accumulate_ribbon(const PtArgs &a, const pt_ribbonp *src, p
 
 hipError_t pt_launch_ribbon_output(const PtArgs &a, const pt_ribbonp *rib, hipStream_t s)
 {
+    PT_SINK_REFUSE("ribbon output");
     hipLaunchKernelGGL(k_ribbon_output, grid_rows(a.lw, a.lh), kRowBlock, 0, s, a, rib);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_downsample_ribbon(const float *src, int pw, float *dst, int lw, int lh, hipStream_t s)
 {
+    PT_SINK_REFUSE("ribbon pyramid");
     hipLaunchKernelGGL(k_downsample_ribbon, grid_rows(lw, lh), kRowBlock, 0, s, (const float4 *)src, pw,
                        (float4 *)dst, lw, lh);
     return hipGetLastError();
@@ -3746,6 +3972,7 @@ hipError_t pt_launch_downsample_ribbon(const float *src, int pw, float *dst, int
 
 hipError_t pt_launch_fill_ribbon(pt_ribbonp *dst, size_t n, hipStream_t s)
 {
+    PT_SINK_REFUSE("ribbon fill");
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_ribbon, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, n);
     return hipGetLastError();
@@ -3756,6 +3983,7 @@ hipError_t pt_launch_random_pyramid_keys(const float4 *color, const float *iorie
                                          const int *ehs, int key_first, int num_keys, int transp, int orient_en,
                                          uint32_t key0, uint32_t key1, float4 *out, hipStream_t s)
 {
+    PT_SINK_REFUSE("keyed random pyramid");
     if (nstyles < 1 || nstyles > PT_MAX_STYLE_LAYERS || num_keys < 1 || key_first < 0 ||
         key_first + num_keys > nstyles)
         return hipErrorInvalidValue;
@@ -3766,7 +3994,7 @@ hipError_t pt_launch_random_pyramid_keys(const float4 *color, const float *iorie
         S.ew[k] = ews[k];
         S.eh[k] = ehs[k];
     }
-    dim3 b(16, 16), g((w + 15) / 16, (h + 15) / 16);
+    dim3 b(256), g((w + 15) / 16, (h + 15) / 16);
     hipLaunchKernelGGL(k_random_pyramid_keys, g, b, 0, s, color, iorient, w, h, S, key_first, num_keys, transp,
                        orient_en, key0, key1, out);
     return hipGetLastError();
@@ -3775,6 +4003,19 @@ hipError_t pt_launch_random_pyramid_keys(const float4 *color, const float *iorie
 hipError_t pt_launch_downsample(const float *src, int pw, float *dst, int lw, int lh, int ch, int scale_mode,
                                 hipStream_t s)
 {
+    if (g_sink) {
+        if (ch < 1 || ch > 4 || (scale_mode != PT_DOWNSAMPLE_SCALE_CONSTANT && scale_mode != PT_DOWNSAMPLE_SCALE_WITH_LEVELS))
+            return sink_refuse("downsample mode");
+        PtTask t = task_2d(PT_TASK_DOWNSAMPLE, lw, lh, PT_RX, PT_RY);
+        t.variant = ch;
+        t.p[0] = src;
+        t.q = dst;
+        t.i[0] = pw;
+        t.i[1] = lw;
+        t.i[2] = lh;
+        t.i[3] = scale_mode;
+        return sink_push(t);
+    }
     dim3 b = kRowBlock, g = grid_rows(lw, lh);
     switch (ch) {
     case 1: hipLaunchKernelGGL(k_downsample_v<1>, g, b, 0, s, src, pw, dst, lw, lh, scale_mode); break;
@@ -3787,6 +4028,7 @@ hipError_t pt_launch_downsample(const float *src, int pw, float *dst, int lw, in
 
 hipError_t pt_launch_downsample_vote(const int *src, int pw, int *dst, int lw, int lh, hipStream_t s)
 {
+    PT_SINK_REFUSE("object-ID pyramid");
     hipLaunchKernelGGL(k_downsample_vote, grid_rows(lw, lh), kRowBlock, 0, s, src, pw, dst, lw, lh);
     return hipGetLastError();
 }
@@ -3794,6 +4036,13 @@ hipError_t pt_launch_downsample_vote(const int *src, int pw, int *dst, int lw, i
 hipError_t pt_launch_fill(float *dst, size_t n, float v, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
+    if (g_sink) {
+        PtTask t = task_new(PT_TASK_FILL, (int64_t)((n + PT_TASK_ELEMS - 1) / PT_TASK_ELEMS), 0);
+        t.q = dst;
+        t.n[0] = (int64_t)n;
+        t.f = v;
+        return sink_push(t);
+    }
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, n, v);
     return hipGetLastError();
 }
@@ -3801,13 +4050,30 @@ hipError_t pt_launch_fill(float *dst, size_t n, float v, hipStream_t s)
 hipError_t pt_launch_tensor_angle(const float *t, float *a, size_t n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
+    if (g_sink) {
+        PtTask k = task_new(PT_TASK_TENSOR_ANGLE, (int64_t)((n + 255) / 256), 0);
+        k.p[0] = t;
+        k.q = a;
+        k.n[0] = (int64_t)n;
+        return sink_push(k);
+    }
     hipLaunchKernelGGL(k_tensor_angle, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, a, n);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_interleave(const float4 *src, int w, int h, float4 *ex, int ew, int slot, hipStream_t s)
 {
-    dim3 b(16, 16), g((w + 15) / 16, (h + 15) / 16);
+    if (g_sink) {
+        PtTask t = task_2d(PT_TASK_INTERLEAVE, w, h, 16, 16);
+        t.p[0] = src;
+        t.q = ex;
+        t.i[0] = w;
+        t.i[1] = h;
+        t.i[2] = ew;
+        t.i[3] = slot;
+        return sink_push(t);
+    }
+    dim3 b(256), g((w + 15) / 16, (h + 15) / 16);
     hipLaunchKernelGGL(k_interleave, g, b, 0, s, src, w, h, ex, ew, slot);
     return hipGetLastError();
 }
@@ -3816,7 +4082,24 @@ hipError_t pt_launch_random_pyramid(const float4 *color, const float *iorient, i
                                     const float *eorient, int ew, int eh, int transp, int orient_en,
                                     uint32_t key0, uint32_t key1, float4 *out, hipStream_t s)
 {
-    dim3 b(16, 16), g((w + 15) / 16, (h + 15) / 16);
+    if (g_sink) {
+        PtTask t = task_2d(PT_TASK_RANDOM_PYRAMID, w, h, 16, 16);
+        t.p[0] = color;
+        t.p[1] = iorient;
+        t.p[2] = eout;
+        t.p[3] = eorient;
+        t.q = out;
+        t.i[0] = w;
+        t.i[1] = h;
+        t.i[2] = ew;
+        t.i[3] = eh;
+        t.i[4] = transp;
+        t.i[5] = orient_en;
+        t.u[0] = key0;
+        t.u[1] = key1;
+        return sink_push(t);
+    }
+    dim3 b(256), g((w + 15) / 16, (h + 15) / 16);
     hipLaunchKernelGGL(k_random_pyramid, g, b, 0, s, color, iorient, w, h, eout, eorient, ew, eh, transp,
                        orient_en, key0, key1, out);
     return hipGetLastError();
@@ -3824,14 +4107,425 @@ hipError_t pt_launch_random_pyramid(const float4 *color, const float *iorient, i
 
 hipError_t pt_launch_pack_nnf(const PtNnf &src, int W, int lw, int lh, void *dst, hipStream_t s)
 {
-    dim3 b(16, 16), g((lw + 15) / 16, (lh + 15) / 16);
+    if (g_sink) {
+        PtTask t = task_2d(PT_TASK_PACK_NNF, lw, lh, 16, 16);
+        t.p[0] = src.xyth;
+        t.p[1] = src.sc;
+        t.p[2] = src.lum;
+        t.p[3] = src.la;
+        t.q = dst;
+        t.i[0] = W;
+        t.i[1] = lw;
+        t.i[2] = lh;
+        return sink_push(t);
+    }
+    dim3 b(256), g((lw + 15) / 16, (lh + 15) / 16);
     hipLaunchKernelGGL(k_pack_nnf, g, b, 0, s, src, W, lw, lh, (PatchXF36 *)dst);
     return hipGetLastError();
 }
 
 hipError_t pt_launch_unpack_nnf(const void *src, int lw, int lh, PtNnf dst, int W, hipStream_t s)
 {
-    dim3 b(16, 16), g((lw + 15) / 16, (lh + 15) / 16);
+    PT_SINK_REFUSE("offset unpack");
+    dim3 b(256), g((lw + 15) / 16, (lh + 15) / 16);
     hipLaunchKernelGGL(k_unpack_nnf, g, b, 0, s, (const PatchXF36 *)src, lw, lh, dst, W);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ */
+/* copies of recorded stream work                                      */
+/* ------------------------------------------------------------------ */
+hipError_t pt_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                     hipMemcpyKind kind, hipStream_t s)
+{
+    if (!g_sink) return hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, kind, s);
+    if (width == 0 || height == 0) return hipSuccess;
+    if (kind != hipMemcpyDeviceToDevice) return sink_refuse("host copy");
+    /* the widest element (16, 8 or 4 bytes) that divides every address,
+     * pitch and row width */
+    uint64_t m = (uint64_t)(uintptr_t)dst | (uint64_t)(uintptr_t)src | (uint64_t)width |
+                 (height > 1 ? ((uint64_t)dpitch | (uint64_t)spitch) : 0);
+    int e = (m % 16 == 0) ? 16 : ((m % 8 == 0) ? 8 : ((m % 4 == 0) ? 4 : 0));
+    if (e == 0) return sink_refuse("unaligned copy");
+    int64_t per_row = (int64_t)(width / e);
+    int64_t tx = (per_row + PT_TASK_ELEMS - 1) / PT_TASK_ELEMS;
+    if (tx * (int64_t)height > (1ll << 30) || per_row > (1ll << 30)) return sink_refuse("oversized copy");
+    PtTask t = task_new(PT_TASK_COPY, tx * (int64_t)height, (int)tx);
+    t.variant = e;
+    t.q = dst;
+    t.p[0] = src;
+    t.n[0] = (int64_t)dpitch;
+    t.n[1] = (int64_t)spitch;
+    t.i[0] = (int)per_row;
+    t.i[1] = (int)height;
+    return sink_push(t);
+}
+
+hipError_t pt_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+    if (!g_sink) return hipMemcpyAsync(dst, src, bytes, kind, s);
+    return pt_copy2d(dst, bytes, src, bytes, bytes, 1, kind, s);
+}
+
+/* ------------------------------------------------------------------ */
+/* persistent batch kernel                                             */
+/* ------------------------------------------------------------------ */
+/*
+ * One launch runs every step of every job of a batch.  Each job's program
+ * is its plain path's sequence of kernels (PtTask: kind, operands, tiles);
+ * the steps of one job run in order, each one only after every tile of the
+ * previous one has completed, while steps of different jobs overlap freely.
+ *
+ * Per job one 64-bit word holds (current step << 32 | next unclaimed tile).
+ * A workgroup's first wave reads the words of 64 jobs at once, takes the
+ * first job (lowest index = highest priority) whose current step has
+ * unclaimed tiles, and claims a run of them with one atomic add; the add
+ * returns the step it claimed in, so a claim that races with the opening of
+ * the next step is a valid claim of that step.  The workgroup runs the
+ * tiles with the same tile bodies as the kernels of the plain path, then
+ * adds their number to the job's completed-tile count; the add that reaches
+ * the step's cumulative end opens the next step.  Release fences before
+ * that add and acquire fences after a claim order every tile's writes
+ * before every tile of the next step (agent scope, across XCDs).
+ *
+ * A workgroup never waits for a particular tile: when no job has claimable
+ * tiles it sleeps and looks again, and it exits when every job has finished.
+ * A workgroup that sees no completed claim anywhere for timeout_ticks of the
+ * constant wall clock sets the give-up flag; every workgroup then exits, and
+ * the host reports the failure.
+ */
+typedef const __attribute__((address_space(4))) PtTask ConstTask;
+typedef const __attribute__((address_space(4))) int ConstInt;
+
+template <class T> constexpr size_t pt_lds_max() { return sizeof(T); }
+template <class T, class U, class... R> constexpr size_t pt_lds_max()
+{
+    return sizeof(T) > pt_lds_max<U, R...>() ? sizeof(T) : pt_lds_max<U, R...>();
+}
+/* the arena holds the LDS of every tile body the kernel runs */
+constexpr size_t kPersistLds =
+    pt_lds_max<PropUpLds<2, 8, 8, false, true>, PropUpLds<3, 8, 8, false, true>, PropUpLds<2, 8, 8, false, false>,
+               PropUpLds<3, 8, 8, false, false>, ScatUpLds<2, 16, 8, false>, ScatUpLds<3, 16, 8, false>,
+               ScatUpLds<2, 4, 4, false>, ScatUpLds<3, 4, 4, false>, ResLds<2, false>, ResLds<3, false>,
+               ResTapLds<2, false>, ResTapLds<3, false>, PropTapLds<2, false>, PropTapLds<3, false>>();
+
+/* Each tile body of the NNF operators runs as a function of its own, not
+ * inlined into the claim loop: inlined, the loop keeps the lane-index
+ * arithmetic and constants of every body live across all of them (measured
+ * 151-256 VGPRs against 101-113 for the same bodies in their own kernels).
+ * Called, each body has its own register allocation under the kernel's
+ * occupancy target (4 waves per SIMD, what the LDS arena allows anyway), at
+ * the cost of saving callee-saved registers once per tile.  Arguments are
+ * made wave-uniform inside, so argument-table loads stay scalar and the
+ * arena stays in LDS. */
+typedef __attribute__((address_space(3))) char *LdsArena;
+
+__device__ __forceinline__ ConstArgs *uniform_args(ConstArgs *p)
+{
+    uint64_t v = (uint64_t)(uintptr_t)p;
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (ConstArgs *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ LdsArena uniform_arena(LdsArena p)
+{
+    return (LdsArena)(uintptr_t)__builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)p);
+}
+
+#define PT_PERSIST_BODY(NAME, TPL, CALL)                                                                  \
+    TPL __device__ __attribute__((noinline)) void NAME(ConstArgs *a_, LdsArena ar_, int bx, int by, int tile) \
+    {                                                                                                     \
+        const PtArgs &A = (const PtArgs &)*uniform_args(a_);                                              \
+        void *arena = (void *)uniform_arena(ar_);                                                         \
+        bx = __builtin_amdgcn_readfirstlane(bx);                                                          \
+        by = __builtin_amdgcn_readfirstlane(by);                                                          \
+        tile = __builtin_amdgcn_readfirstlane(tile);                                                      \
+        CALL;                                                                                             \
+    }
+
+#define PT_COMMA ,
+PT_PERSIST_BODY(pk_prop_up, template <int RT PT_COMMA bool EE PT_COMMA bool XL>,
+                (b_propagate_up<RT, EE, 8, 8, false, XL, true>(A, bx, by, A.jump, arena)))
+PT_PERSIST_BODY(pk_prop_tp, template <int RT>, (b_propagate_tp<RT, false, true>(A, tile, A.jump, arena)))
+PT_PERSIST_BODY(pk_scat_ee, template <int RT>, (b_scatter_up<RT, true, 16, 8, false, true>(A, bx, by, A.nscat, arena)))
+PT_PERSIST_BODY(pk_scat_a, template <int RT>, (b_scatter_up<RT, false, 4, 4, false, true>(A, bx, by, A.nscat, arena)))
+PT_PERSIST_BODY(pk_res, template <int RT>, (b_update_residual<RT, false, false, true>(A, bx, by, arena)))
+PT_PERSIST_BODY(pk_res_tp, template <int RT>, (b_update_residual_tp<RT, false, true>(A, tile, arena)))
+#undef PT_COMMA
+#undef PT_PERSIST_BODY
+
+template <int RT>
+__device__ __forceinline__ void persist_nnf(const PtTask &t, ConstArgs *A, int bx, int by, int tile, LdsArena arena)
+{
+    const int shape = t.variant >> 4;
+    const bool xl = (t.variant & 8) != 0;
+    if (t.kind == PT_TASK_PROPAGATE) {
+        if (shape == PT_SHAPE_TAP) pk_prop_tp<RT>(A, arena, bx, by, tile);
+        else if (shape == PT_SHAPE_UP_EE_B) {
+            if (xl) pk_prop_up<RT, true, true>(A, arena, bx, by, tile);
+            else pk_prop_up<RT, true, false>(A, arena, bx, by, tile);
+        } else {
+            if (xl) pk_prop_up<RT, false, true>(A, arena, bx, by, tile);
+            else pk_prop_up<RT, false, false>(A, arena, bx, by, tile);
+        }
+    } else if (t.kind == PT_TASK_SCATTER) {
+        if (shape == PT_SHAPE_UP_EE_A) pk_scat_ee<RT>(A, arena, bx, by, tile);
+        else pk_scat_a<RT>(A, arena, bx, by, tile);
+    } else {
+        if (shape == PT_SHAPE_TAP) pk_res_tp<RT>(A, arena, bx, by, tile);
+        else pk_res<RT>(A, arena, bx, by, tile);
+    }
+}
+
+template <class V>
+__device__ __forceinline__ void persist_copy(const PtTask &t, int row, int seg)
+{
+    const V *src = (const V *)((const char *)t.p[0] + (int64_t)row * t.n[1]);
+    V *dst = (V *)((char *)t.q + (int64_t)row * t.n[0]);
+    const int e0 = seg * PT_TASK_ELEMS + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < PT_TASK_ELEMS / 256; k++) {
+        int e = e0 + k * 256;
+        if (e < t.i[0]) dst[e] = src[e];
+    }
+}
+
+__device__ __forceinline__ void persist_tile(const PtPersist &P, const PtTask &t, int tile, void *arena)
+{
+    const int bx = tile % t.tiles_x, by = tile / t.tiles_x;
+    switch (t.kind) {
+    case PT_TASK_PROPAGATE:
+    case PT_TASK_SCATTER:
+    case PT_TASK_RESIDUAL: {
+        ConstArgs *A = (ConstArgs *)P.args + t.arg;
+        if ((t.variant & 7) == 2) persist_nnf<2>(t, A, bx, by, tile, (LdsArena)arena);
+        else persist_nnf<3>(t, A, bx, by, tile, (LdsArena)arena);
+        break;
+    }
+    case PT_TASK_SYNC: {
+        const PtArgs &A = (const PtArgs &)((ConstArgs *)P.args)[t.arg];
+        b_sync_canvas(A, bx, by, A.dw, A.dh);
+        break;
+    }
+    case PT_TASK_COPY_RANDOM: b_copy_random((const PtArgs &)((ConstArgs *)P.args)[t.arg], bx, by); break;
+    case PT_TASK_UPSAMPLE: b_upsample((const PtArgs &)((ConstArgs *)P.args)[t.arg], bx, by); break;
+    case PT_TASK_COPY:
+        if (t.variant == 16) persist_copy<float4>(t, by, bx);
+        else if (t.variant == 8) persist_copy<float2>(t, by, bx);
+        else persist_copy<float>(t, by, bx);
+        break;
+    case PT_TASK_FILL: {
+        float *dst = (float *)t.q;
+#pragma unroll
+        for (int k = 0; k < PT_TASK_ELEMS / 256; k++) {
+            int64_t e = (int64_t)tile * PT_TASK_ELEMS + k * 256 + threadIdx.x;
+            if (e < t.n[0]) dst[e] = t.f;
+        }
+        break;
+    }
+    case PT_TASK_DOWNSAMPLE: {
+        const float *src = (const float *)t.p[0];
+        float *dst = (float *)t.q;
+        if (t.variant == 1) b_downsample_v<1>(src, t.i[0], dst, t.i[1], t.i[2], t.i[3], bx, by);
+        else if (t.variant == 2) b_downsample_v<2>(src, t.i[0], dst, t.i[1], t.i[2], t.i[3], bx, by);
+        else if (t.variant == 4) b_downsample_v<4>(src, t.i[0], dst, t.i[1], t.i[2], t.i[3], bx, by);
+        else b_downsample(src, t.i[0], dst, t.i[1], t.i[2], t.variant, t.i[3], bx, by);
+        break;
+    }
+    case PT_TASK_TENSOR_ANGLE: {
+        int64_t e = (int64_t)tile * 256 + threadIdx.x;
+        const float *tn = (const float *)t.p[0];
+        if (e < t.n[0]) ((float *)t.q)[e] = pt_tensor_to_angle(tn[3 * e], tn[3 * e + 1], tn[3 * e + 2]);
+        break;
+    }
+    case PT_TASK_INTERLEAVE:
+        b_interleave((const float4 *)t.p[0], t.i[0], t.i[1], (float4 *)t.q, t.i[2], t.i[3], bx, by);
+        break;
+    case PT_TASK_RANDOM_PYRAMID:
+        b_random_pyramid((const float4 *)t.p[0], (const float *)t.p[1], t.i[0], t.i[1], (const float4 *)t.p[2],
+                         (const float *)t.p[3], t.i[2], t.i[3], t.i[4], t.i[5], t.u[0], t.u[1], (float4 *)t.q, bx,
+                         by);
+        break;
+    case PT_TASK_PACK_NNF: {
+        PtNnf n;
+        n.xyth = (float4 *)t.p[0];
+        n.sc = (float2 *)t.p[1];
+        n.lum = (float *)t.p[2];
+        n.la = (int32_t *)t.p[3];
+        b_pack_nnf(n, t.i[0], t.i[1], t.i[2], (PatchXF36 *)t.q, bx, by);
+        break;
+    }
+    default: break;
+    }
+}
+
+/* 4 waves per SIMD: four workgroups of 256 lanes per CU, as many as the LDS
+ * arena allows */
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_persistent(PtPersist P)
+{
+    __shared__ __attribute__((aligned(16))) char arena[kPersistLds];
+    __shared__ int s_mode, s_job, s_step, s_t0, s_cnt;
+    ConstTask *tasks = (ConstTask *)P.tasks;
+    ConstInt *first = (ConstInt *)P.first;
+    int seen = -1;          /* wave 0, lane 0: completed claims at the last look */
+    long long quiet_since = 0;
+    for (;;) {
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            int job = -1, step = 0, t0 = 0, cnt = 0;
+            /* the give-up flag (set by a workgroup that timed out, or by the
+             * host before the launch) stops every workgroup at its next look */
+            const int gave =
+                __builtin_amdgcn_readfirstlane(__hip_atomic_load(P.ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            for (int base = 0; base < P.njobs && job < 0 && !gave; base += 64) {
+                const int j = base + lane;
+                unsigned long long w = 0;
+                bool open = false;
+                if (j < P.njobs) {
+                    w = __hip_atomic_load(P.claim + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const int s = (int)(w >> 32), ns = first[j + 1] - first[j];
+                    open = s < ns && (unsigned)w < (unsigned)tasks[first[j] + s].ntiles;
+                }
+                unsigned long long m = __ballot(open);
+                while (m != 0ull && job < 0) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    int gs = -1, gt = 0, gc = 0;
+                    if (lane == l) {
+                        const int f0 = first[j], ns = first[j + 1] - f0;
+                        int c = tasks[f0 + (int)(w >> 32)].ntiles / P.chunk_div;
+                        c = c < 1 ? 1 : (c > P.max_chunk ? P.max_chunk : c);
+                        unsigned long long old = __hip_atomic_fetch_add(P.claim + j, (unsigned long long)c,
+                                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const int os = (int)(old >> 32);
+                        const unsigned oc = (unsigned)old;
+                        if (os < ns) {
+                            const unsigned nt = (unsigned)tasks[f0 + os].ntiles;
+                            if (oc < nt) {
+                                gs = os;
+                                gt = (int)oc;
+                                gc = (int)min((unsigned)c, nt - oc);
+                            }
+                        }
+                    }
+                    gs = __shfl(gs, l);
+                    if (gs >= 0) {
+                        job = base + l;
+                        step = gs;
+                        t0 = __shfl(gt, l);
+                        cnt = __shfl(gc, l);
+                    }
+                }
+            }
+            if (lane == 0) {
+                int mode = 0;
+                if (job < 0) {
+                    const int fin = __hip_atomic_load(P.ctl + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    mode = (fin >= P.njobs || gave) ? 2 : 1;
+                    if (mode == 1) {
+                        const int prog = __hip_atomic_load(P.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const long long now = wall_clock64();
+                        if (prog != seen) {
+                            seen = prog;
+                            quiet_since = now;
+                        } else if (now - quiet_since > P.timeout_ticks) {
+                            __hip_atomic_store(P.ctl + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            mode = 2;
+                        }
+                    }
+                }
+                s_mode = mode;
+                s_job = job;
+                s_step = step;
+                s_t0 = t0;
+                s_cnt = cnt;
+            }
+        }
+        __syncthreads();
+        const int mode = s_mode, job = s_job, step = s_step, t0 = s_t0, cnt = s_cnt;
+        __syncthreads();
+        if (mode == 2) break;
+        if (mode == 1) {
+            __builtin_amdgcn_s_sleep(32);
+            continue;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const PtTask &t = (const PtTask &)tasks[first[job] + step];
+        for (int k = 0; k < cnt; k++) {
+            persist_tile(P, t, t0 + k, arena);
+            __syncthreads();
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int old = __hip_atomic_fetch_add(P.done + job, cnt, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (old + cnt == t.end) {
+                /* the step is complete: open the next one */
+                __hip_atomic_store(P.claim + job, (unsigned long long)(step + 1) << 32, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (step + 1 == first[job + 1] - first[job])
+                    __hip_atomic_fetch_add(P.ctl + 0, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __hip_atomic_fetch_add(P.ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+int pt_task_op(int op, int shape, const PtArgs &a, int arg, PtTask *t, const char **why)
+{
+    *why = nullptr;
+    if (op < 0 || op >= PT_OP_COUNT) {
+        *why = "unknown operator";
+        return PT_ERR_UNSUPPORTED;
+    }
+    if (op != PT_OP_SYNC_CANVAS) {
+        if (a.time_active) *why = "temporal terms";
+        else if (ext_terms(a)) *why = "extended patch-distance terms or style layers";
+        else if (a.keys) *why = "keyframe search";
+        else if (a.r != 2 && a.r != 3) *why = "patch radius other than 2 or 3";
+        if (*why) return PT_ERR_UNSUPPORTED;
+    }
+    if (!pt_shape_valid(op, shape, a)) {
+        *why = "invalid kernel shape";
+        return PT_ERR_UNSUPPORTED;
+    }
+    int v = 0;
+    switch (op) {
+    case PT_OP_PROPAGATE:
+        if (shape != PT_SHAPE_TAP && shape != PT_SHAPE_UP_EE_B && shape != PT_SHAPE_UP_A) *why = "propagate shape";
+        v = (shape << 4) | (prop_xl(shape, a) ? 8 : 0) | a.r;
+        break;
+    case PT_OP_SCATTER:
+        if (shape != PT_SHAPE_UP_EE_A && shape != PT_SHAPE_UP_A) *why = "scatter shape";
+        v = (shape << 4) | a.r;
+        break;
+    case PT_OP_RESIDUAL:
+        /* the refresh runs without the exemplar window (identical results) */
+        if (shape != PT_SHAPE_TAP && shape != PT_SHAPE_PIXEL) *why = "refresh shape";
+        v = (shape << 4) | a.r;
+        break;
+    default: break;
+    }
+    if (*why) return PT_ERR_UNSUPPORTED;
+    int tx = 1;
+    int n = pt_shape_tiles(op, shape, a, &tx);
+    *t = task_new(op, n, tx);
+    t->variant = v;
+    t->arg = arg;
+    return PT_OK;
+}
+
+int pt_persistent_grid(int device)
+{
+    int nb = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_persistent, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    return nb * cus;
+}
+
+hipError_t pt_launch_persistent(const PtPersist &p, int grid, hipStream_t s)
+{
+    if (grid < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_persistent, dim3(grid), dim3(256), 0, s, p);
     return hipGetLastError();
 }

@@ -23,6 +23,7 @@ struct RecOp {
     int op = -1; /* PT_OP_* with a multi-job form, or -1 for fn */
     PtArgs a{};
     std::function<hipError_t(hipStream_t)> fn;
+    const char *what = nullptr; /* what fn issues (error messages) */
 };
 
 /* Launches one recorded operator as a single-job kernel with the default

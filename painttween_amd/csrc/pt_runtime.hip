@@ -516,6 +516,7 @@ static int issue(pt_ctx *c, const char *what, std::function<hipError_t(hipStream
         RecOp o;
         o.op = -1;
         o.fn = std::move(fn);
+        o.what = what;
         c->rec->push_back(std::move(o));
         return PT_OK;
     }
@@ -1114,13 +1115,13 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
     auto rect = [&](void *dst, int ch) -> int {
         if (w > c->W || h > c->H) return set_err(c, PT_ERR_INVALID_ARG, "level larger than image");
         return issue(c, "image upload", [=](hipStream_t s) {
-            return hipMemcpy2DAsync(dst, (size_t)W * ch * sizeof(float), src, (size_t)w * ch * sizeof(float),
+            return pt_copy2d(dst, (size_t)W * ch * sizeof(float), src, (size_t)w * ch * sizeof(float),
                                     (size_t)w * ch * sizeof(float), h, hipMemcpyDeviceToDevice, s);
         });
     };
     auto tight = [&](void *dst, int ch) -> int {
         return issue(c, "texture upload", [=](hipStream_t s) {
-            return hipMemcpyAsync(dst, src, (size_t)w * h * ch * sizeof(float), hipMemcpyDeviceToDevice, s);
+            return pt_copy(dst, src, (size_t)w * h * ch * sizeof(float), hipMemcpyDeviceToDevice, s);
         });
     };
     switch (layer) {
@@ -1167,7 +1168,7 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
         int32_t *dst = c->ex_id + exl;
         int ew = c->ew;
         return issue(c, "exemplar ID upload", [=](hipStream_t s) {
-            return hipMemcpy2DAsync(dst, (size_t)ew * sizeof(int32_t), src, (size_t)w * sizeof(int32_t),
+            return pt_copy2d(dst, (size_t)ew * sizeof(int32_t), src, (size_t)w * sizeof(int32_t),
                                     (size_t)w * sizeof(int32_t), h, hipMemcpyDeviceToDevice, s);
         });
     }
@@ -1187,7 +1188,7 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
         float *dst = (layer == PT_LAYER_EXEMPLAR_ORIENTATION ? c->ex_orient : c->ex_dt) + exl;
         int ew = c->ew;
         return issue(c, "exemplar guide upload", [=](hipStream_t s) {
-            return hipMemcpy2DAsync(dst, (size_t)ew * sizeof(float), src, (size_t)w * sizeof(float),
+            return pt_copy2d(dst, (size_t)ew * sizeof(float), src, (size_t)w * sizeof(float),
                                     (size_t)w * sizeof(float), h, hipMemcpyDeviceToDevice, s);
         });
     }
@@ -1200,7 +1201,7 @@ static int upload_image(pt_ctx *c, int level, int layer, const pt_pyramid *p, in
         int cw = c->cw;
         c->layered_live = true;
         return issue(c, "cumulative field upload", [=](hipStream_t s) {
-            return hipMemcpy2DAsync(dst, (size_t)cw * sizeof(float4), src, (size_t)w * sizeof(float4),
+            return pt_copy2d(dst, (size_t)cw * sizeof(float4), src, (size_t)w * sizeof(float4),
                                     (size_t)w * sizeof(float4), h, hipMemcpyDeviceToDevice, s);
         });
     }
@@ -1463,16 +1464,16 @@ static int copy_nnf_rect(pt_ctx *c, const PtNnf &dst, const PtNnf &src, Rect r)
     const size_t W = c->W;
     const bool lum = c->lum_live, lay = c->layered_live;
     return issue(c, "offset copy", [=](hipStream_t s) {
-        hipError_t e = hipMemcpy2DAsync(dst.xyth, W * sizeof(float4), src.xyth, W * sizeof(float4),
+        hipError_t e = pt_copy2d(dst.xyth, W * sizeof(float4), src.xyth, W * sizeof(float4),
                                         (size_t)r.w * sizeof(float4), r.h, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess && lum)
-            e = hipMemcpy2DAsync(dst.lum, W * sizeof(float), src.lum, W * sizeof(float), (size_t)r.w * sizeof(float),
+            e = pt_copy2d(dst.lum, W * sizeof(float), src.lum, W * sizeof(float), (size_t)r.w * sizeof(float),
                                  r.h, hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess && lay)
-            e = hipMemcpy2DAsync(dst.la, W * sizeof(int32_t), src.la, W * sizeof(int32_t),
+            e = pt_copy2d(dst.la, W * sizeof(int32_t), src.la, W * sizeof(int32_t),
                                  (size_t)r.w * sizeof(int32_t), r.h, hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
-        return hipMemcpy2DAsync(dst.sc, W * sizeof(float2), src.sc, W * sizeof(float2), (size_t)r.w * sizeof(float2),
+        return pt_copy2d(dst.sc, W * sizeof(float2), src.sc, W * sizeof(float2), (size_t)r.w * sizeof(float2),
                                 r.h, hipMemcpyDeviceToDevice, s);
     });
 }
@@ -1650,7 +1651,7 @@ static int upsample(pt_ctx *c, int fine_level)
         size_t W = c->W;
         Rect r = c->res_diff;
         RC_CHECK(issue(c, "residual copy", [=](hipStream_t s) {
-            return hipMemcpy2DAsync(rb, W * sizeof(float), rw, W * sizeof(float), (size_t)r.w * sizeof(float), r.h,
+            return pt_copy2d(rb, W * sizeof(float), rw, W * sizeof(float), (size_t)r.w * sizeof(float), r.h,
                                     hipMemcpyDeviceToDevice, s);
         }));
     }
@@ -1675,12 +1676,12 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
     PtNnf base = c->base;
     const bool lum = c->lum_live, lay = c->layered_live;
     RC_CHECK(issue(c, "advected offset copy", [=](hipStream_t s) {
-        hipError_t e = hipMemcpyAsync(base.xyth, d.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, s);
-        if (e == hipSuccess && lum) e = hipMemcpyAsync(base.lum, d.lum, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+        hipError_t e = pt_copy(base.xyth, d.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && lum) e = pt_copy(base.lum, d.lum, n * sizeof(float), hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess && lay)
-            e = hipMemcpyAsync(base.la, d.la, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+            e = pt_copy(base.la, d.la, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
         if (e != hipSuccess) return e;
-        return hipMemcpyAsync(base.sc, d.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
+        return pt_copy(base.sc, d.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, s);
     }));
     c->off_diff.grow(c->W, c->H);
     RC_CHECK(sync_buffers(c, level, PT_DONT_COPY_OFFSETS | PT_DONT_UPDATE_RESIDUAL));
@@ -1688,7 +1689,7 @@ static int advect(pt_ctx *c, int level, int time_step, int forwards, int flags)
     /* residual -> dest.residual, whole buffer (:1724) */
     float *rb = c->res_base;
     RC_CHECK(issue(c, "advected residual copy", [=](hipStream_t s) {
-        return hipMemcpyAsync(d.res, rb, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+        return pt_copy(d.res, rb, n * sizeof(float), hipMemcpyDeviceToDevice, s);
     }));
     return PT_OK;
 }
@@ -1754,7 +1755,7 @@ static int accumulate_ribbon(pt_ctx *c, int level, int forwards, int time_step, 
     int pr = prof_begin(c, PK_ADVECT, (uint64_t)a.lw * a.lh, 80 * (uint64_t)a.lw * a.lh);
     RC_CHECK(issue(c, "ribbon accumulation", [=](hipStream_t s) {
         hipError_t e = pt_launch_accumulate_ribbon(a, rb, rw, forwards, time_step, overwrite, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(rb, rw, n * sizeof(pt_ribbonp), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = pt_copy(rb, rw, n * sizeof(pt_ribbonp), hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess) e = pt_launch_ribbon_output(a, rb, s);
         return e;
     }));
@@ -2061,19 +2062,19 @@ static int snap_save(pt_ctx *c, int level, Snapshot &sn)
     const bool lum = c->lum_live;
     (void)level;
     return issue(c, "snapshot save", [=](hipStream_t st) {
-        hipError_t e = hipMemcpy2DAsync(d.xyth, (size_t)d.w * sizeof(float4), base.xyth, W * sizeof(float4),
+        hipError_t e = pt_copy2d(d.xyth, (size_t)d.w * sizeof(float4), base.xyth, W * sizeof(float4),
                                         (size_t)d.w * sizeof(float4), d.h, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess && lum)
-            e = hipMemcpy2DAsync(d.lum, (size_t)d.w * sizeof(float), base.lum, W * sizeof(float),
+            e = pt_copy2d(d.lum, (size_t)d.w * sizeof(float), base.lum, W * sizeof(float),
                                  (size_t)d.w * sizeof(float), d.h, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess)
-            e = hipMemcpy2DAsync(d.sc, (size_t)d.w * sizeof(float2), base.sc, W * sizeof(float2),
+            e = pt_copy2d(d.sc, (size_t)d.w * sizeof(float2), base.sc, W * sizeof(float2),
                                  (size_t)d.w * sizeof(float2), d.h, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess)
-            e = hipMemcpy2DAsync(d.res, (size_t)d.w * sizeof(float), rb, W * sizeof(float), (size_t)d.w * sizeof(float),
+            e = pt_copy2d(d.res, (size_t)d.w * sizeof(float), rb, W * sizeof(float), (size_t)d.w * sizeof(float),
                                  d.h, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess)
-            e = hipMemcpy2DAsync(d.canvas, (size_t)d.w * sizeof(float4), cv, W * sizeof(float4),
+            e = pt_copy2d(d.canvas, (size_t)d.w * sizeof(float4), cv, W * sizeof(float4),
                                  (size_t)d.w * sizeof(float4), d.h, hipMemcpyDeviceToDevice, st);
         return e;
     });
@@ -2088,13 +2089,13 @@ static int snap_load_offsets(pt_ctx *c, const Snapshot &sn)
     size_t W = c->W;
     const bool lum = c->lum_live;
     RC_CHECK(issue(c, "snapshot offset load", [=](hipStream_t st) {
-        hipError_t e = hipMemcpy2DAsync(base.xyth, W * sizeof(float4), d.xyth, (size_t)d.w * sizeof(float4),
+        hipError_t e = pt_copy2d(base.xyth, W * sizeof(float4), d.xyth, (size_t)d.w * sizeof(float4),
                                         (size_t)d.w * sizeof(float4), d.h, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess && lum)
-            e = hipMemcpy2DAsync(base.lum, W * sizeof(float), d.lum, (size_t)d.w * sizeof(float),
+            e = pt_copy2d(base.lum, W * sizeof(float), d.lum, (size_t)d.w * sizeof(float),
                                  (size_t)d.w * sizeof(float), d.h, hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) return e;
-        return hipMemcpy2DAsync(base.sc, W * sizeof(float2), d.sc, (size_t)d.w * sizeof(float2),
+        return pt_copy2d(base.sc, W * sizeof(float2), d.sc, (size_t)d.w * sizeof(float2),
                                 (size_t)d.w * sizeof(float2), d.h, hipMemcpyDeviceToDevice, st);
     }));
     c->off_diff.grow(sn.w, sn.h);
@@ -2109,7 +2110,7 @@ static int snap_load_residual(pt_ctx *c, const Snapshot &sn)
     float *rb = c->res_base;
     size_t W = c->W;
     RC_CHECK(issue(c, "snapshot residual load", [=](hipStream_t st) {
-        return hipMemcpy2DAsync(rb, W * sizeof(float), d.res, (size_t)d.w * sizeof(float), (size_t)d.w * sizeof(float),
+        return pt_copy2d(rb, W * sizeof(float), d.res, (size_t)d.w * sizeof(float), (size_t)d.w * sizeof(float),
                                 d.h, hipMemcpyDeviceToDevice, st);
     }));
     c->res_diff.grow(sn.w, sn.h);
@@ -2192,11 +2193,11 @@ static int copy_base_to_advected_f(pt_ctx *c)
     size_t n = (size_t)c->W * c->H;
     const bool lum = c->lum_live, lay = c->layered_live;
     RC_CHECK(issue(c, "copy to advected", [=](hipStream_t st) {
-        hipError_t e = hipMemcpyAsync(d.xyth, base.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess && lum) e = hipMemcpyAsync(d.lum, base.lum, n * sizeof(float), hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess && lay) e = hipMemcpyAsync(d.la, base.la, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(d.sc, base.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(d.res, rb, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+        hipError_t e = pt_copy(d.xyth, base.xyth, n * sizeof(float4), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && lum) e = pt_copy(d.lum, base.lum, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && lay) e = pt_copy(d.la, base.la, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = pt_copy(d.sc, base.sc, n * sizeof(float2), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = pt_copy(d.res, rb, n * sizeof(float), hipMemcpyDeviceToDevice, st);
         return e;
     }));
     c->adv_written[0] = true;
@@ -2639,7 +2640,7 @@ static int pyramid_fill(pt_ctx *c, pt_pyramid *p, const float *src, int on_devic
         hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         if (c->rec && !on_device) return set_err(c, PT_ERR_STATE, "recorded plans take device inputs only");
         RC_CHECK(issue(c, "pyramid level-0 copy",
-                       [=](hipStream_t s) { return hipMemcpyAsync(dst, src, n0 * sizeof(float), kind, s); }));
+                       [=](hipStream_t s) { return pt_copy(dst, src, n0 * sizeof(float), kind, s); }));
     } else {
         float v = fill_bits(p->elem, defv);
         RC_CHECK(issue(c, "pyramid fill", [=](hipStream_t s) { return pt_launch_fill(dst, n0, v, s); }));
@@ -2740,7 +2741,7 @@ static int read_snapshot(pt_ctx *c, const Snapshot &s, pt_patchxf *out_off, floa
             RC_CHECK(issue(c, "offset pack", [=](hipStream_t st) {
                 hipError_t e = pt_launch_pack_nnf(nn, w, w, h, tmp, st);
                 if (e != hipSuccess) return e;
-                return hipMemcpyAsync(dst, tmp, n * sizeof(pt_patchxf), kind, st);
+                return pt_copy(dst, tmp, n * sizeof(pt_patchxf), kind, st);
             }));
         }
     }
@@ -2748,13 +2749,13 @@ static int read_snapshot(pt_ctx *c, const Snapshot &s, pt_patchxf *out_off, floa
         float *dst = out_canvas;
         const float4 *src = s.canvas;
         RC_CHECK(issue(c, "canvas readback",
-                       [=](hipStream_t st) { return hipMemcpyAsync(dst, src, n * sizeof(float4), kind, st); }));
+                       [=](hipStream_t st) { return pt_copy(dst, src, n * sizeof(float4), kind, st); }));
     }
     if (out_res) {
         float *dst = out_res;
         const float *src = s.res;
         RC_CHECK(issue(c, "residual readback",
-                       [=](hipStream_t st) { return hipMemcpyAsync(dst, src, n * sizeof(float), kind, st); }));
+                       [=](hipStream_t st) { return pt_copy(dst, src, n * sizeof(float), kind, st); }));
     }
     return PT_OK;
 }

@@ -139,7 +139,9 @@ class Batch:
             shapes[name] = [(_ffi.SHAPE_NAMES[s] if s >= 0 else "default") for s in st.shape[op]]
         return dict(jobs=st.jobs, ops=st.ops, launches=st.launches, multi_launches=st.multi_launches,
                     ticks=st.ticks, last_run_ms=st.last_run_ms, tune_ms=st.tune_ms, shapes=shapes,
-                    xcd_remap=st.xcd_remap, isolate_tiles=st.isolate_tiles)
+                    xcd_remap=st.xcd_remap, isolate_tiles=st.isolate_tiles, persistent=st.persistent,
+                    persistent_grid=st.persistent_grid, persistent_tasks=st.persistent_tasks,
+                    persistent_tiles=st.persistent_tiles, persistent_max_chunk=st.persistent_max_chunk)
 
     def result(self, j):
         return self.outputs[j].numpy()

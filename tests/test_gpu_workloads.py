@@ -118,6 +118,19 @@ def test_batched_workload_64_jobs_equal_plain_after_tuning():
         for i in range(64):
             _same(b.result(i), alone[i], "job %d (%dx%d, r=%d, %d levels)" % (
                 i, specs[i]["w"], specs[i]["h"], specs[i]["r"], specs[i]["levels"]))
+        # the persistent form: the whole batch in one launch, claim size tuned
+        for o in b.outputs:
+            o.off.zero_()
+            o.canvas.zero_()
+            o.res.zero_()
+        b.set(_ffi.PT_BATCH_PERSISTENT, 1)
+        b.tune(1)
+        b.run()
+        st = b.stats()
+        assert st["persistent"] == 1 and st["persistent_tasks"] > 0
+        for i in range(64):
+            _same(b.result(i), alone[i], "persistent job %d (%dx%d, r=%d, %d levels)" % (
+                i, specs[i]["w"], specs[i]["h"], specs[i]["r"], specs[i]["levels"]))
     finally:
         b.close()
 
