@@ -275,6 +275,32 @@ def run_batched(ctx, jobs_in, steps, cpu_threads=0):
            "bit_exact_vs_plain": bool(exact), "launches": st["launches"], "multi_launches": st["multi_launches"],
            "ticks": st["ticks"], "recorded_ops": st["ops"], "tuned_shapes": st["shapes"], "tune_s": tune_s,
            "hip_graph": True}
+    # the persistent form: the whole batch in one kernel launch
+    for o in b.outputs:
+        o.off.zero_()
+        o.canvas.zero_()
+        o.res.zero_()
+    b.set(_ffi.PT_BATCH_PERSISTENT, 1)
+    t0 = time.perf_counter()
+    b.tune(1)
+    ptune_s = time.perf_counter() - t0
+    b.run()
+    ptimes = []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        b.run()
+        ptimes.append(time.perf_counter() - t0)
+        print("persistent run %d: %.1f ms" % (k, ptimes[-1] * 1e3), file=sys.stderr, flush=True)
+    pexact = all(torch.equal(b.outputs[i].off, plain_out[i].off) and
+                 torch.equal(b.outputs[i].canvas, plain_out[i].canvas) and
+                 torch.equal(b.outputs[i].res, plain_out[i].res) for i in range(len(jobs)))
+    pst = b.stats()
+    pers_s = float(np.median(ptimes))
+    out["persistent"] = {"ms_per_batch": pers_s * 1e3, "jobs_per_s": len(jobs) / pers_s, "runs": steps,
+                         "speedup_vs_plain": plain_s / pers_s, "vs_graph": batch_s / pers_s,
+                         "bit_exact_vs_plain": bool(pexact), "launches": 1, "grid": pst["persistent_grid"],
+                         "tasks": pst["persistent_tasks"], "tiles": pst["persistent_tiles"],
+                         "tiles_per_claim": pst["persistent_max_chunk"], "tune_s": ptune_s}
     b.close()
     for c in ctxs:
         c.close()

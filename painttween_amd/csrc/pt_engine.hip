@@ -244,12 +244,14 @@ struct pt_batch {
     double pers_timeout_ms = 2000.0;
     int64_t pers_tasks = 0, pers_tiles = 0;
     std::vector<int> pers_order; /* jobs by priority */
+    std::vector<unsigned long long> pers_claim0; /* claim words at the start of a launch */
     PtTask *pd_tasks = nullptr;
     int *pd_first = nullptr;
     PtArgs *pd_args = nullptr;
     unsigned long long *pd_claim = nullptr;
     int *pd_done = nullptr;
     int *pd_ctl = nullptr;
+    int pers_xcds = 0; /* XCDs the persistent grid spans */
 };
 
 static int bfail(pt_batch *b, int code, const char *fmt, ...)
@@ -301,6 +303,19 @@ static const char *op_name(int op)
     return (op >= 0 && op < 4) ? n[op] : "operator";
 }
 
+/* Tiles per claim of a task: a job runs on one XCD, so a step's tiles are
+ * shared by that XCD's workgroups: about two claims each for the
+ * patch-distance operators (at most max_chunk tiles), one each for the
+ * light kinds (copies, fills, syncs, pyramid and set-up kernels), whose
+ * tiles take a few microseconds against several for a claim. */
+static int task_chunk(const PtTask &t, int wg_per_xcd, int max_chunk)
+{
+    const bool heavy = t.kind == PT_TASK_PROPAGATE || t.kind == PT_TASK_SCATTER || t.kind == PT_TASK_RESIDUAL;
+    int c = heavy ? t.ntiles / (2 * wg_per_xcd) : (t.ntiles + wg_per_xcd - 1) / wg_per_xcd;
+    const int cap = heavy ? max_chunk : 64;
+    return c < 1 ? 1 : (c > cap ? cap : c);
+}
+
 /*
  * The persistent form's device program: each job's recorded plan becomes a
  * list of tasks, one per kernel the plain path would launch (operators with
@@ -316,6 +331,7 @@ static int build_persistent(pt_batch *b)
     std::vector<std::vector<PtTask>> jt(n);
     std::vector<std::vector<PtArgs>> ja(n);
     std::vector<int64_t> work(n, 0);
+    std::vector<double> est(n, 0.0); /* estimated work: patch taps of the NNF steps */
     for (int j = 0; j < n; j++) {
         int64_t cum = 0;
         for (const RecOp &o : b->lanes[j].ops) {
@@ -346,35 +362,76 @@ static int build_persistent(pt_batch *b)
                 args = sink.args;
             }
             for (PtTask &t : got) {
+                if (t.ntiles > PT_PERSIST_MAX_CLAIMS)
+                    return bfail(b, PT_ERR_UNSUPPORTED, "persistent form: job %d has a step of %d tiles", j, t.ntiles);
                 if (t.arg >= 0) {
                     ja[j].push_back(args[t.arg]);
                     t.arg = (int)ja[j].size() - 1; /* rebased below */
                 }
                 cum += t.ntiles;
                 if (cum > (1ll << 30)) return bfail(b, PT_ERR_UNSUPPORTED, "persistent form: job %d has too many tiles", j);
+                if (t.kind <= PT_TASK_RESIDUAL) {
+                    const PtArgs &a = ja[j][t.arg];
+                    const double taps = (double)a.lw * a.lh * (2 * a.r + 1) * (2 * a.r + 1);
+                    est[j] += taps * (t.kind == PT_TASK_PROPAGATE ? 2.0 : (t.kind == PT_TASK_SCATTER ? 0.5 * (a.nscat + 1) : 0.5));
+                } else {
+                    est[j] += 10.0 * t.ntiles;
+                }
                 t.end = (int)cum;
                 jt[j].push_back(t);
             }
         }
         work[j] = cum;
         if (jt[j].empty()) return bfail(b, PT_ERR_STATE, "persistent form: job %d has no work", j);
+        if (jt[j].size() >= (size_t)PT_PERSIST_MAX_STEPS)
+            return bfail(b, PT_ERR_UNSUPPORTED, "persistent form: job %d has too many steps", j);
     }
     std::vector<int> order(n);
     for (int j = 0; j < n; j++) order[j] = j;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return est[x] > est[y]; });
+    b->pers_grid = pt_persistent_grid(b->device);
+    if (b->pers_grid < 1) return bfail(b, PT_ERR_HIP, "persistent form: no occupancy for the persistent kernel");
+    /* the XCDs that run the persistent grid's workgroups */
+    unsigned mask = 0, *dmask = nullptr;
+    B_HIP(b, hipMalloc((void **)&dmask, sizeof(unsigned)));
+    hipError_t pe = hipMemsetAsync(dmask, 0, sizeof(unsigned), b->stream);
+    if (pe == hipSuccess) pe = pt_launch_xcd_probe(b->pers_grid, dmask, b->stream);
+    if (pe == hipSuccess) pe = hipStreamSynchronize(b->stream);
+    if (pe == hipSuccess) pe = hipMemcpy(&mask, dmask, sizeof(unsigned), hipMemcpyDeviceToHost);
+    (void)hipFree(dmask);
+    if (pe != hipSuccess) return bfail(b, PT_ERR_HIP, "persistent form: XCD probe failed: %s", hipGetErrorString(pe));
+    std::vector<int> xcds;
+    for (int x = 0; x < 16; x++)
+        if (mask & (1u << x)) xcds.push_back(x);
+    if (xcds.empty()) return bfail(b, PT_ERR_HIP, "persistent form: XCD probe found no XCD");
+    b->pers_xcds = (int)xcds.size();
+    const int wg_per_xcd = b->pers_grid / b->pers_xcds > 0 ? b->pers_grid / b->pers_xcds : 1;
+    /* longest job first onto the least loaded XCD */
+    std::vector<double> load(xcds.size(), 0.0);
+    std::vector<int> home_of(n, 0);
+    for (int j : order) {
+        size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[k] += est[j];
+        home_of[j] = xcds[k];
+    }
     std::vector<PtTask> tasks;
     std::vector<PtArgs> args;
     std::vector<int> first;
+    b->pers_claim0.clear();
     for (int j : order) {
         first.push_back((int)tasks.size());
         const int base = (int)args.size();
         for (PtTask t : jt[j]) {
             if (t.arg >= 0) t.arg += base;
+            t.chunk = task_chunk(t, wg_per_xcd, b->pers_max_chunk);
             tasks.push_back(t);
         }
+        const PtTask &t0 = tasks[first.back()];
+        b->pers_claim0.push_back(pt_claim_word(0, (t0.ntiles + t0.chunk - 1) / t0.chunk));
         args.insert(args.end(), ja[j].begin(), ja[j].end());
     }
     first.push_back((int)tasks.size());
+    for (int j : order) first.push_back(home_of[j]);
     if (args.empty()) args.push_back(PtArgs{});
     B_HIP(b, hipMalloc((void **)&b->pd_tasks, tasks.size() * sizeof(PtTask)));
     B_HIP(b, hipMalloc((void **)&b->pd_first, first.size() * sizeof(int)));
@@ -385,8 +442,6 @@ static int build_persistent(pt_batch *b)
     B_HIP(b, hipMemcpy(b->pd_tasks, tasks.data(), tasks.size() * sizeof(PtTask), hipMemcpyHostToDevice));
     B_HIP(b, hipMemcpy(b->pd_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
     B_HIP(b, hipMemcpy(b->pd_args, args.data(), args.size() * sizeof(PtArgs), hipMemcpyHostToDevice));
-    b->pers_grid = pt_persistent_grid(b->device);
-    if (b->pers_grid < 1) return bfail(b, PT_ERR_HIP, "persistent form: no occupancy for the persistent kernel");
     b->pers_tasks = (int64_t)tasks.size();
     b->pers_tiles = 0;
     for (int64_t w : work) b->pers_tiles += w;
@@ -404,7 +459,8 @@ static int run_persistent(pt_batch *b)
         if (rc != PT_OK) return rc;
     }
     const int n = (int)b->lanes.size();
-    B_HIP(b, hipMemsetAsync(b->pd_claim, 0, (size_t)n * sizeof(unsigned long long), b->stream));
+    B_HIP(b, hipMemcpy(b->pd_claim, b->pers_claim0.data(), b->pers_claim0.size() * sizeof(unsigned long long),
+                       hipMemcpyHostToDevice));
     B_HIP(b, hipMemsetAsync(b->pd_done, 0, (size_t)n * sizeof(int), b->stream));
     const int ctl0[4] = {0, 0, b->pers_abort ? 1 : 0, 0};
     b->pers_abort = 0;
@@ -419,9 +475,6 @@ static int run_persistent(pt_batch *b)
     P.done = b->pd_done;
     P.ctl = b->pd_ctl;
     P.njobs = n;
-    /* about two claims per workgroup for a step that fills the device */
-    P.chunk_div = 2 * b->pers_grid;
-    P.max_chunk = b->pers_max_chunk;
     P.timeout_ticks = (long long)((double)(khz > 0 ? khz : 100000) * b->pers_timeout_ms);
     B_HIP(b, pt_launch_persistent(P, b->pers_grid, b->stream));
     B_HIP(b, hipStreamSynchronize(b->stream));
@@ -592,6 +645,7 @@ extern "C" int pt_batch_tune(pt_batch *b, int rounds)
         int pick = b->pers_max_chunk;
         for (int c : cand) {
             b->pers_max_chunk = c;
+            b->pers_valid = false;
             double tt = 0.0;
             for (int r = 0; r < rounds; r++) {
                 double t1 = now_ms();
@@ -605,6 +659,7 @@ extern "C" int pt_batch_tune(pt_batch *b, int rounds)
             }
         }
         b->pers_max_chunk = pick;
+        b->pers_valid = false;
         b->tune_ms += now_ms() - t0;
         return PT_OK;
     }
@@ -704,6 +759,7 @@ extern "C" int pt_batch_set(pt_batch *b, int key, int value)
     case PT_BATCH_PERSIST_CHUNK:
         if (value < 1 || value > 1024) return bfail(b, PT_ERR_INVALID_ARG, "tiles per claim must be 1..1024");
         b->pers_max_chunk = value;
+        b->pers_valid = false;
         return PT_OK;
     case PT_BATCH_PERSIST_TIMEOUT_MS:
         if (value < 1) return bfail(b, PT_ERR_INVALID_ARG, "timeout must be >= 1 ms");

@@ -247,7 +247,7 @@ struct PtTask {
     int i[8];    /* integer operands */
     uint32_t u[2];
     float f;
-    int pad;
+    int chunk;        /* tiles per claim in the persistent launch */
     const void *p[4]; /* source operands */
     void *q;          /* destination operand */
     int64_t n[3];     /* counts and pitches */
@@ -270,20 +270,35 @@ void pt_sink_end();
  * body for it (the message names what). */
 int pt_task_op(int op, int shape, const PtArgs &a, int arg, PtTask *t, const char **why);
 
+/* Claim word of a job: step << 44 | claims (runs of PtTask::chunk tiles)
+ * in the step << 22 | claims taken. */
+__host__ __device__ inline unsigned long long pt_claim_word(int step, int nclaims)
+{
+    return ((unsigned long long)step << 44) | ((unsigned long long)nclaims << 22);
+}
+__host__ __device__ inline int pt_cw_step(unsigned long long w) { return (int)(w >> 44); }
+__host__ __device__ inline int pt_cw_claims(unsigned long long w) { return (int)((w >> 22) & 0x3FFFFFull); }
+__host__ __device__ inline int pt_cw_taken(unsigned long long w) { return (int)(w & 0x3FFFFFull); }
+#define PT_PERSIST_MAX_CLAIMS (1 << 21) /* claims of one step (the word's field holds 22 bits) */
+#define PT_PERSIST_MAX_STEPS (1 << 19)  /* steps of one job */
+
 /* Device state of one persistent launch. */
 struct PtPersist {
-    const PtTask *tasks;      /* every job's program, concatenated        */
-    const int *first;         /* njobs + 1: job j's steps are tasks[first[j] .. first[j+1]) */
+    const PtTask *tasks;       /* every job's program, concatenated        */
+    const int *first;          /* njobs + 1: job j's steps are tasks[first[j] .. first[j+1]);
+                                  then njobs more: job j's XCD (first[njobs + 1 + j]) */
     const PtArgs *args;
-    unsigned long long *claim; /* per job: step << 32 | next unclaimed tile */
-    int *done;                /* per job: tiles completed, all steps     */
-    int *ctl;                 /* [0] jobs finished, [1] claims completed, [2] gave up */
+    unsigned long long *claim; /* per job: claim word (pt_claim_word)     */
+    int *done;                 /* per job: tiles completed, all steps     */
+    int *ctl;                  /* [0] jobs finished, [1] steps completed, [2] gave up */
     int njobs;
-    int chunk_div, max_chunk; /* tiles per claim: clamp(ntiles / chunk_div, 1, max_chunk) */
-    long long timeout_ticks;  /* wall-clock ticks without a completed claim before giving up */
+    long long timeout_ticks;   /* wall-clock ticks without a completed step before giving up */
 };
 /* Workgroups of the persistent kernel that fit on the device at once. */
 int pt_persistent_grid(int device);
 hipError_t pt_launch_persistent(const PtPersist &p, int grid, hipStream_t s);
+/* ORs 1 << (XCD id) into *mask (device memory) for every XCD that runs one
+ * of `grid` workgroups. */
+hipError_t pt_launch_xcd_probe(int grid, unsigned *mask, hipStream_t s);
 
 #endif

@@ -30,14 +30,26 @@
 #define PT_LX(BX) ((int)threadIdx.x % (BX))
 #define PT_LY(BX) ((int)threadIdx.x / (BX))
 
-/* The LDS of a tile body is one struct: a kernel of its own gets it as a
- * static __shared__ variable; the persistent batch kernel, which runs many
- * tile bodies one after another in one workgroup, passes one arena that all
- * of them share (template flag AR). */
-#define PT_LDS(TYPE, NAME)                                                                                \
+/* The LDS of a tile body: in a kernel of its own, static __shared__ arrays;
+ * in the persistent batch kernel, which runs many tile bodies one after
+ * another in one workgroup, the members of the body's LDS struct (Lds) in
+ * one arena that all of them share (template flag AR).  The kernels of their
+ * own keep separate arrays: with one struct the compiler kept the exemplar's
+ * buffer resource in VGPRs in the windowed propagate and re-read it into
+ * SGPRs around every gather (15% slower). */
+#define PT_LDS_ARRAY(TYPE, NAME, N)                                                                       \
+    TYPE(*NAME##_p)[N];                                                                                   \
+    if constexpr (AR) {                                                                                   \
+        NAME##_p = &reinterpret_cast<Lds *>(arena)->NAME;                                                 \
+    } else {                                                                                              \
+        __shared__ TYPE NAME##_s[N];                                                                      \
+        NAME##_p = &NAME##_s;                                                                             \
+    }                                                                                                     \
+    TYPE(&NAME)[N] = *NAME##_p;
+#define PT_LDS_VAR(TYPE, NAME)                                                                            \
     TYPE *NAME##_p;                                                                                       \
     if constexpr (AR) {                                                                                   \
-        NAME##_p = reinterpret_cast<TYPE *>(arena);                                                       \
+        NAME##_p = &reinterpret_cast<Lds *>(arena)->NAME;                                                 \
     } else {                                                                                              \
         __shared__ TYPE NAME##_s;                                                                         \
         NAME##_p = &NAME##_s;                                                                             \
@@ -734,13 +746,12 @@ template <int RT, bool TK, bool XL, bool AR = false>
 __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int by, void *arena = nullptr)
 {
     using Lds = ResLds<RT, XL>;
-    PT_LDS(Lds, L);
-    auto &scol = L.scol;
-    auto &scan = L.scan;
-    auto &sexw = L.sexw;
-    auto &sflag = L.sflag;
-    auto &slist = L.slist;
-    int &scount = L.scount;
+    PT_LDS_ARRAY(float4, scol, TileSize<RT>::N)
+    PT_LDS_ARRAY(float4, scan, TileSize<RT>::N)
+    PT_LDS_ARRAY(float4, sexw, (XL ? PT_XW_RES * PT_XW_RES * 2 : 1))
+    PT_LDS_ARRAY(unsigned char, sflag, TileSize<RT>::N)
+    PT_LDS_ARRAY(unsigned short, slist, PT_BX * PT_BY)
+    PT_LDS_VAR(int, scount)
     const int r = RT > 0 ? RT : A.r;
     const bool inc = A.res_incremental != 0;
     if (inc) {
@@ -1999,31 +2010,31 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     constexpr int P = TX * TY, U = 4 * P;
     static_assert(!XL || RT > 0, "exemplar windows need a fixed radius");
     using Lds = PropUpLds<RT, TX, TY, TM, XL>;
-    PT_LDS(Lds, L);
-    auto &s_tm = L.s_tm;
-    auto &s_exw = L.s_exw;
-    auto &s_exl = L.s_exl;
-    auto &s_ab = L.s_ab;
-    auto &s_ao = L.s_ao;
-    auto &s_av = L.s_av;
-    auto &scol = L.scol;
-    auto &scan = L.scan;
-    auto &s_best = L.s_best;
-    auto &s_sw = L.s_sw;
-    auto &s_ia = L.s_ia;
-    auto &s_io = L.s_io;
-    auto &s_sc = L.s_sc;
-    auto &s_r = L.s_r;
-    auto &s_cx = L.s_cx;
-    auto &s_cy = L.s_cy;
-    auto &s_b0x = L.s_b0x;
-    auto &s_b0y = L.s_b0y;
-    auto &s_b1x = L.s_b1x;
-    auto &s_b1y = L.s_b1y;
-    auto &s_scy = L.s_scy;
-    auto &s_list = L.s_list;
-    int &s_n = L.s_n;
-    int &s_next = L.s_next;
+    constexpr int N = Lds::N;
+    PT_LDS_VAR(UpTime<TM ? P : 1>, s_tm)
+    PT_LDS_ARRAY(float4, s_exw, (XL ? PT_XW_PROP * PT_XW_PROP * 2 : 1))
+    PT_LDS_ARRAY(unsigned char, s_exl, (XL ? U : 1))
+    PT_LDS_ARRAY(float4, s_ab, (TM ? N : 1))
+    PT_LDS_ARRAY(float4, s_ao, (TM ? N : 1))
+    PT_LDS_ARRAY(unsigned char, s_av, (TM ? N : 1))
+    PT_LDS_ARRAY(float4, scol, N)
+    PT_LDS_ARRAY(float4, scan, N)
+    PT_LDS_ARRAY(float, s_best, P)
+    PT_LDS_ARRAY(float, s_sw, P)
+    PT_LDS_ARRAY(float, s_ia, P)
+    PT_LDS_ARRAY(float, s_io, P)
+    PT_LDS_ARRAY(float2, s_sc, P)
+    PT_LDS_ARRAY(float, s_r, U)
+    PT_LDS_ARRAY(float, s_cx, U)
+    PT_LDS_ARRAY(float, s_cy, U)
+    PT_LDS_ARRAY(float, s_b0x, U)
+    PT_LDS_ARRAY(float, s_b0y, U)
+    PT_LDS_ARRAY(float, s_b1x, U)
+    PT_LDS_ARRAY(float, s_b1y, U)
+    PT_LDS_ARRAY(float, s_scy, U)
+    PT_LDS_ARRAY(unsigned short, s_list, U)
+    PT_LDS_VAR(int, s_n)
+    PT_LDS_VAR(int, s_next)
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
@@ -2199,26 +2210,26 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     constexpr int P = TX * TY;
     constexpr int S = PT_UP_MAX_SAMPLES + 1;
     using Lds = ScatUpLds<RT, TX, TY, TM>;
-    PT_LDS(Lds, L);
-    auto &s_tm = L.s_tm;
-    auto &scol = L.scol;
-    auto &scan = L.scan;
-    auto &s_ow = L.s_ow;
-    auto &s_osc = L.s_osc;
-    auto &s_best = L.s_best;
-    auto &s_sw = L.s_sw;
-    auto &s_ia = L.s_ia;
-    auto &s_cached = L.s_cached;
-    auto &s_iori = L.s_iori;
-    auto &s_sc = L.s_sc;
-    auto &s_start = L.s_start;
-    auto &s_dist = L.s_dist;
-    auto &s_r = L.s_r;
-    int &s_next = L.s_next;
-    int &s_more = L.s_more;
-    auto &s_ab = L.s_ab;
-    auto &s_ao = L.s_ao;
-    auto &s_av = L.s_av;
+    constexpr int N = Lds::N;
+    PT_LDS_VAR(UpTime<TM ? P : 1>, s_tm)
+    PT_LDS_ARRAY(float4, scol, N)
+    PT_LDS_ARRAY(float4, scan, N)
+    PT_LDS_ARRAY(float4, s_ow, P)
+    PT_LDS_ARRAY(float2, s_osc, P)
+    PT_LDS_ARRAY(float, s_best, P)
+    PT_LDS_ARRAY(float, s_sw, P)
+    PT_LDS_ARRAY(float, s_ia, P)
+    PT_LDS_ARRAY(float, s_cached, P)
+    PT_LDS_ARRAY(float, s_iori, P)
+    PT_LDS_ARRAY(float2, s_sc, P)
+    PT_LDS_ARRAY(int, s_start, P)
+    PT_LDS_ARRAY(float, s_dist, S)
+    PT_LDS_ARRAY(float, s_r, P * S)
+    PT_LDS_VAR(int, s_next)
+    PT_LDS_VAR(int, s_more)
+    PT_LDS_ARRAY(float4, s_ab, (TM ? N : 1))
+    PT_LDS_ARRAY(float4, s_ao, (TM ? N : 1))
+    PT_LDS_ARRAY(unsigned char, s_av, (TM ? N : 1))
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     const int SU = n + 1; /* units per pixel */
@@ -3144,9 +3155,9 @@ __device__ __forceinline__ void b_update_residual_tp(const PtArgs &A, int g, voi
 {
     constexpr int T = TapCfg<RT>::T;
     using Lds = ResTapLds<RT, TK>;
-    PT_LDS(Lds, L);
-    auto &su = L.su;
-    auto &sv = L.sv;
+    using Units = TapUnits<TapCfg<RT>::RES_MAX, TK>;
+    PT_LDS_VAR(Units, su)
+    PT_LDS_ARRAY(float, sv, TapCfg<RT>::RES_MAX * T)
     const int tid = threadIdx.x, NP = A.tap_np_res;
     if (tid < NP) {
         int q = g * NP + tid;
@@ -3211,10 +3222,10 @@ __device__ __forceinline__ void b_propagate_tp(const PtArgs &A, int g, int jump,
     constexpr int T = TapCfg<RT>::T, NPM = TapCfg<RT>::PROP_MAX;
     static_assert(NPM >= 1, "tap-parallel propagate needs 4 (2r+1)^2 <= PT_TAP_EVALS");
     using Lds = PropTapLds<RT, TK>;
-    PT_LDS(Lds, L);
-    auto &su = L.su;
-    auto &sv = L.sv;
-    auto &sr = L.sr;
+    using Units = TapUnits<Lds::NUM, TK>;
+    PT_LDS_VAR(Units, su)
+    PT_LDS_ARRAY(float, sv, Lds::NUM * T)
+    PT_LDS_ARRAY(float, sr, Lds::NUM)
     const int tid = threadIdx.x, NP = A.tap_np_prop, NU = 4 * NP;
     if (tid < NU) {
         int k = tid >> 2, cand = tid & 3;
@@ -4176,21 +4187,32 @@ hipError_t pt_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind,
  * the steps of one job run in order, each one only after every tile of the
  * previous one has completed, while steps of different jobs overlap freely.
  *
- * Per job one 64-bit word holds (current step << 32 | next unclaimed tile).
- * A workgroup's first wave reads the words of 64 jobs at once, takes the
- * first job (lowest index = highest priority) whose current step has
- * unclaimed tiles, and claims a run of them with one atomic add; the add
- * returns the step it claimed in, so a claim that races with the opening of
- * the next step is a valid claim of that step.  The workgroup runs the
- * tiles with the same tile bodies as the kernels of the plain path, then
- * adds their number to the job's completed-tile count; the add that reaches
- * the step's cumulative end opens the next step.  Release fences before
- * that add and acquire fences after a claim order every tile's writes
- * before every tile of the next step (agent scope, across XCDs).
+ * A job runs on one XCD from its first claim to its end.  Its steps then
+ * only pass data through that XCD's L2: a tile's writes are complete for the
+ * next step once they have reached the L2 (the vector L1 writes through),
+ * and a claim invalidates the claiming CU's L1.  Passing data between XCDs
+ * would need an agent-scope release, an L2 write-back, at every completed
+ * claim, which measured slower than the work itself (the batch took 1.7x as
+ * long with it).  A claim still takes an agent-scope acquire, which
+ * invalidates the CU's vector L1.  Workgroups read the XCD they run on from
+ * the hardware.
+ *
+ * The host places the jobs on the XCDs by their estimated work (longest
+ * first onto the least loaded XCD).  Per job one 64-bit claim word holds
+ * (step, claims of the step, claims taken); a claim is a run of
+ * PtTask::chunk tiles.  A workgroup's first wave reads the claim words of 64
+ * jobs at once and takes the first job (lowest index = highest priority,
+ * the host orders jobs by work) that runs on its XCD and has an open claim,
+ * with one atomic add; the add returns the step and claim count it claimed
+ * in, so a claim that races with the opening of the next step is a valid
+ * claim of that step.  The workgroup runs the tiles with the same tile bodies as the kernels of
+ * the plain path, then adds their number to the job's completed-tile count;
+ * the add that reaches the step's cumulative end writes the next step's
+ * claim word.
  *
  * A workgroup never waits for a particular tile: when no job has claimable
  * tiles it sleeps and looks again, and it exits when every job has finished.
- * A workgroup that sees no completed claim anywhere for timeout_ticks of the
+ * A workgroup that sees no step complete anywhere for timeout_ticks of the
  * constant wall clock sets the give-up flag; every workgroup then exits, and
  * the host reports the failure.
  */
@@ -4364,15 +4386,17 @@ __device__ __forceinline__ void persist_tile(const PtPersist &P, const PtTask &t
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_persistent(PtPersist P)
 {
     __shared__ __attribute__((aligned(16))) char arena[kPersistLds];
-    __shared__ int s_mode, s_job, s_step, s_t0, s_cnt;
+    __shared__ int s_mode, s_job, s_step, s_claim;
     ConstTask *tasks = (ConstTask *)P.tasks;
     ConstInt *first = (ConstInt *)P.first;
-    int seen = -1;          /* wave 0, lane 0: completed claims at the last look */
+    /* the XCD this workgroup runs on (HW_REG_XCC_ID, bits 3:0) */
+    const int xcd = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);
+    int seen = -1; /* wave 0, lane 0: completed steps at the last look */
     long long quiet_since = 0;
     for (;;) {
         if (threadIdx.x < 64) {
             const int lane = threadIdx.x;
-            int job = -1, step = 0, t0 = 0, cnt = 0;
+            int job = -1, step = 0, claim = 0;
             /* the give-up flag (set by a workgroup that timed out, or by the
              * host before the launch) stops every workgroup at its next look */
             const int gave =
@@ -4380,42 +4404,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             for (int base = 0; base < P.njobs && job < 0 && !gave; base += 64) {
                 const int j = base + lane;
                 unsigned long long w = 0;
-                bool open = false;
-                if (j < P.njobs) {
+                bool ok = false;
+                if (j < P.njobs && first[P.njobs + 1 + j] == xcd) {
                     w = __hip_atomic_load(P.claim + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const int s = (int)(w >> 32), ns = first[j + 1] - first[j];
-                    open = s < ns && (unsigned)w < (unsigned)tasks[first[j] + s].ntiles;
+                    ok = pt_cw_taken(w) < pt_cw_claims(w);
                 }
-                unsigned long long m = __ballot(open);
+                unsigned long long m = __ballot(ok);
                 while (m != 0ull && job < 0) {
                     const int l = __builtin_ctzll(m);
                     m &= m - 1ull;
-                    int gs = -1, gt = 0, gc = 0;
+                    int gs = -1, gk = 0;
                     if (lane == l) {
-                        const int f0 = first[j], ns = first[j + 1] - f0;
-                        int c = tasks[f0 + (int)(w >> 32)].ntiles / P.chunk_div;
-                        c = c < 1 ? 1 : (c > P.max_chunk ? P.max_chunk : c);
-                        unsigned long long old = __hip_atomic_fetch_add(P.claim + j, (unsigned long long)c,
-                                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        const int os = (int)(old >> 32);
-                        const unsigned oc = (unsigned)old;
-                        if (os < ns) {
-                            const unsigned nt = (unsigned)tasks[f0 + os].ntiles;
-                            if (oc < nt) {
-                                gs = os;
-                                gt = (int)oc;
-                                gc = (int)min((unsigned)c, nt - oc);
-                            }
+                        const unsigned long long old =
+                            __hip_atomic_fetch_add(P.claim + j, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (pt_cw_taken(old) < pt_cw_claims(old)) {
+                            gs = pt_cw_step(old);
+                            gk = pt_cw_taken(old);
                         }
                     }
                     gs = __shfl(gs, l);
                     if (gs >= 0) {
                         job = base + l;
                         step = gs;
-                        t0 = __shfl(gt, l);
-                        cnt = __shfl(gc, l);
+                        claim = __shfl(gk, l);
                     }
                 }
+            }
+            if (job >= 0) {
+                /* this CU's L1 may hold lines the job's earlier steps rewrote */
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             if (lane == 0) {
                 int mode = 0;
@@ -4437,36 +4454,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 s_mode = mode;
                 s_job = job;
                 s_step = step;
-                s_t0 = t0;
-                s_cnt = cnt;
+                s_claim = claim;
             }
         }
         __syncthreads();
-        const int mode = s_mode, job = s_job, step = s_step, t0 = s_t0, cnt = s_cnt;
+        const int mode = s_mode, job = s_job, step = s_step, claim = s_claim;
         __syncthreads();
         if (mode == 2) break;
         if (mode == 1) {
             __builtin_amdgcn_s_sleep(32);
             continue;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const PtTask &t = (const PtTask &)tasks[first[job] + step];
-        for (int k = 0; k < cnt; k++) {
-            persist_tile(P, t, t0 + k, arena);
+        const int t0 = claim * t.chunk, t1 = min(t0 + t.chunk, t.ntiles);
+        for (int k = t0; k < t1; k++) {
+            persist_tile(P, t, k, arena);
             __syncthreads();
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        /* every wave's writes have reached the XCD's L2 before the count below */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (threadIdx.x == 0) {
-            const int old = __hip_atomic_fetch_add(P.done + job, cnt, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            const int cnt = t1 - t0;
+            const int old = __hip_atomic_fetch_add(P.done + job, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (old + cnt == t.end) {
                 /* the step is complete: open the next one */
-                __hip_atomic_store(P.claim + job, (unsigned long long)(step + 1) << 32, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                if (step + 1 == first[job + 1] - first[job])
-                    __hip_atomic_fetch_add(P.ctl + 0, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                const int ns = first[job + 1] - first[job];
+                unsigned long long w = pt_claim_word(ns, 0);
+                if (step + 1 < ns) {
+                    const PtTask &u = (const PtTask &)tasks[first[job] + step + 1];
+                    w = pt_claim_word(step + 1, (u.ntiles + u.chunk - 1) / u.chunk);
+                }
+                __hip_atomic_store(P.claim + job, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (step + 1 == ns) __hip_atomic_fetch_add(P.ctl + 0, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(P.ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            __hip_atomic_fetch_add(P.ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -4513,6 +4536,19 @@ int pt_task_op(int op, int shape, const PtArgs &a, int arg, PtTask *t, const cha
     t->variant = v;
     t->arg = arg;
     return PT_OK;
+}
+
+/* Each workgroup records the XCD it runs on. */
+__global__ void k_xcd_probe(unsigned *mask)
+{
+    if (threadIdx.x == 0) atomicOr(mask, 1u << (__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15));
+}
+
+hipError_t pt_launch_xcd_probe(int grid, unsigned *mask, hipStream_t s)
+{
+    if (grid < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_xcd_probe, dim3(grid), dim3(64), 0, s, mask);
+    return hipGetLastError();
 }
 
 int pt_persistent_grid(int device)
