@@ -250,7 +250,11 @@ struct pt_batch {
     PtArgs *pd_args = nullptr;
     unsigned long long *pd_claim = nullptr;
     int *pd_done = nullptr;
+    unsigned long long *pd_cost = nullptr;
     int *pd_ctl = nullptr;
+    /* measured work per job (shader clocks, by job index), used for the
+     * placement when set (pt_batch_tune) */
+    std::vector<double> pers_measured;
     int pers_xcds = 0; /* XCDs the persistent grid spans */
 };
 
@@ -287,12 +291,14 @@ static void free_persistent(pt_batch *b)
     (void)hipFree(b->pd_args);
     (void)hipFree(b->pd_claim);
     (void)hipFree(b->pd_done);
+    (void)hipFree(b->pd_cost);
     (void)hipFree(b->pd_ctl);
     b->pd_tasks = nullptr;
     b->pd_first = nullptr;
     b->pd_args = nullptr;
     b->pd_claim = nullptr;
     b->pd_done = nullptr;
+    b->pd_cost = nullptr;
     b->pd_ctl = nullptr;
     b->pers_valid = false;
 }
@@ -386,6 +392,7 @@ static int build_persistent(pt_batch *b)
         if (jt[j].size() >= (size_t)PT_PERSIST_MAX_STEPS)
             return bfail(b, PT_ERR_UNSUPPORTED, "persistent form: job %d has too many steps", j);
     }
+    if ((int)b->pers_measured.size() == n) est = b->pers_measured;
     std::vector<int> order(n);
     for (int j = 0; j < n; j++) order[j] = j;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return est[x] > est[y]; });
@@ -438,6 +445,7 @@ static int build_persistent(pt_batch *b)
     B_HIP(b, hipMalloc((void **)&b->pd_args, args.size() * sizeof(PtArgs)));
     B_HIP(b, hipMalloc((void **)&b->pd_claim, (size_t)n * sizeof(unsigned long long)));
     B_HIP(b, hipMalloc((void **)&b->pd_done, (size_t)n * sizeof(int)));
+    B_HIP(b, hipMalloc((void **)&b->pd_cost, (size_t)n * sizeof(unsigned long long)));
     B_HIP(b, hipMalloc((void **)&b->pd_ctl, 4 * sizeof(int)));
     B_HIP(b, hipMemcpy(b->pd_tasks, tasks.data(), tasks.size() * sizeof(PtTask), hipMemcpyHostToDevice));
     B_HIP(b, hipMemcpy(b->pd_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -462,6 +470,7 @@ static int run_persistent(pt_batch *b)
     B_HIP(b, hipMemcpy(b->pd_claim, b->pers_claim0.data(), b->pers_claim0.size() * sizeof(unsigned long long),
                        hipMemcpyHostToDevice));
     B_HIP(b, hipMemsetAsync(b->pd_done, 0, (size_t)n * sizeof(int), b->stream));
+    B_HIP(b, hipMemsetAsync(b->pd_cost, 0, (size_t)n * sizeof(unsigned long long), b->stream));
     const int ctl0[4] = {0, 0, b->pers_abort ? 1 : 0, 0};
     b->pers_abort = 0;
     B_HIP(b, hipMemcpy(b->pd_ctl, ctl0, sizeof(ctl0), hipMemcpyHostToDevice));
@@ -473,6 +482,7 @@ static int run_persistent(pt_batch *b)
     P.args = b->pd_args;
     P.claim = b->pd_claim;
     P.done = b->pd_done;
+    P.cost = b->pd_cost;
     P.ctl = b->pd_ctl;
     P.njobs = n;
     P.timeout_ticks = (long long)((double)(khz > 0 ? khz : 100000) * b->pers_timeout_ms);
@@ -639,7 +649,23 @@ extern "C" int pt_batch_tune(pt_batch *b, int rounds)
     B_HIP(b, hipSetDevice(b->device));
     double t0 = now_ms();
     if (b->persistent) {
-        /* the persistent form: tiles per claim, on whole-batch time */
+        /* the persistent form: first the placement by measured work (one
+         * launch with the estimated placement, the shader clocks of each
+         * job's claims read back), then the tiles per claim, on whole-batch
+         * time */
+        {
+            b->pers_measured.clear();
+            b->pers_valid = false;
+            int rc = run_persistent(b);
+            if (rc != PT_OK) return rc;
+            const int n = (int)b->lanes.size();
+            std::vector<unsigned long long> cost(n);
+            B_HIP(b, hipMemcpy(cost.data(), b->pd_cost, (size_t)n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            /* pd_cost is in the launch's job order */
+            b->pers_measured.assign(n, 0.0);
+            for (int k = 0; k < n; k++) b->pers_measured[b->pers_order[k]] = (double)cost[k];
+            b->pers_valid = false;
+        }
         const int cand[] = {1, 2, 4, 8, 16};
         double best = 0.0;
         int pick = b->pers_max_chunk;

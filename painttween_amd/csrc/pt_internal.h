@@ -290,6 +290,7 @@ struct PtPersist {
     const PtArgs *args;
     unsigned long long *claim; /* per job: claim word (pt_claim_word)     */
     int *done;                 /* per job: tiles completed, all steps     */
+    unsigned long long *cost;  /* per job: shader clocks its claims took (placement feedback) */
     int *ctl;                  /* [0] jobs finished, [1] steps completed, [2] gave up */
     int njobs;
     long long timeout_ticks;   /* wall-clock ticks without a completed step before giving up */

@@ -4197,8 +4197,9 @@ hipError_t pt_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind,
  * invalidates the CU's vector L1.  Workgroups read the XCD they run on from
  * the hardware.
  *
- * The host places the jobs on the XCDs by their estimated work (longest
- * first onto the least loaded XCD).  Per job one 64-bit claim word holds
+ * The host places the jobs on the XCDs by their work (longest first onto
+ * the least loaded XCD): estimated from the patch taps of their steps, or
+ * the shader clocks their claims took in an earlier launch (P.cost).  Per job one 64-bit claim word holds
  * (step, claims of the step, claims taken); a claim is a run of
  * PtTask::chunk tiles.  A workgroup's first wave reads the claim words of 64
  * jobs at once and takes the first job (lowest index = highest priority,
@@ -4228,7 +4229,9 @@ template <class T, class U, class... R> constexpr size_t pt_lds_max()
 constexpr size_t kPersistLds =
     pt_lds_max<PropUpLds<2, 8, 8, false, true>, PropUpLds<3, 8, 8, false, true>, PropUpLds<2, 8, 8, false, false>,
                PropUpLds<3, 8, 8, false, false>, ScatUpLds<2, 16, 8, false>, ScatUpLds<3, 16, 8, false>,
-               ScatUpLds<2, 4, 4, false>, ScatUpLds<3, 4, 4, false>, ResLds<2, false>, ResLds<3, false>,
+               ScatUpLds<2, 4, 4, false>, ScatUpLds<3, 4, 4, false>, PropUpLds<2, 16, 8, false, false>,
+               PropUpLds<3, 16, 8, false, false>, ScatUpLds<2, 8, 8, false>, ScatUpLds<3, 8, 8, false>,
+               ResLds<2, false>, ResLds<3, false>,
                ResTapLds<2, false>, ResTapLds<3, false>, PropTapLds<2, false>, PropTapLds<3, false>>();
 
 /* Each tile body of the NNF operators runs as a function of its own, not
@@ -4268,6 +4271,9 @@ __device__ __forceinline__ LdsArena uniform_arena(LdsArena p)
 #define PT_COMMA ,
 PT_PERSIST_BODY(pk_prop_up, template <int RT PT_COMMA bool EE PT_COMMA bool XL>,
                 (b_propagate_up<RT, EE, 8, 8, false, XL, true>(A, bx, by, A.jump, arena)))
+PT_PERSIST_BODY(pk_prop_ee_a, template <int RT>,
+                (b_propagate_up<RT, true, 16, 8, false, false, true>(A, bx, by, A.jump, arena)))
+PT_PERSIST_BODY(pk_scat_ee_b, template <int RT>, (b_scatter_up<RT, true, 8, 8, false, true>(A, bx, by, A.nscat, arena)))
 PT_PERSIST_BODY(pk_prop_tp, template <int RT>, (b_propagate_tp<RT, false, true>(A, tile, A.jump, arena)))
 PT_PERSIST_BODY(pk_scat_ee, template <int RT>, (b_scatter_up<RT, true, 16, 8, false, true>(A, bx, by, A.nscat, arena)))
 PT_PERSIST_BODY(pk_scat_a, template <int RT>, (b_scatter_up<RT, false, 4, 4, false, true>(A, bx, by, A.nscat, arena)))
@@ -4283,6 +4289,7 @@ __device__ __forceinline__ void persist_nnf(const PtTask &t, ConstArgs *A, int b
     const bool xl = (t.variant & 8) != 0;
     if (t.kind == PT_TASK_PROPAGATE) {
         if (shape == PT_SHAPE_TAP) pk_prop_tp<RT>(A, arena, bx, by, tile);
+        else if (shape == PT_SHAPE_UP_EE_A) pk_prop_ee_a<RT>(A, arena, bx, by, tile);
         else if (shape == PT_SHAPE_UP_EE_B) {
             if (xl) pk_prop_up<RT, true, true>(A, arena, bx, by, tile);
             else pk_prop_up<RT, true, false>(A, arena, bx, by, tile);
@@ -4292,6 +4299,7 @@ __device__ __forceinline__ void persist_nnf(const PtTask &t, ConstArgs *A, int b
         }
     } else if (t.kind == PT_TASK_SCATTER) {
         if (shape == PT_SHAPE_UP_EE_A) pk_scat_ee<RT>(A, arena, bx, by, tile);
+        else if (shape == PT_SHAPE_UP_EE_B) pk_scat_ee_b<RT>(A, arena, bx, by, tile);
         else pk_scat_a<RT>(A, arena, bx, by, tile);
     } else {
         if (shape == PT_SHAPE_TAP) pk_res_tp<RT>(A, arena, bx, by, tile);
@@ -4467,16 +4475,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         }
         const PtTask &t = (const PtTask &)tasks[first[job] + step];
         const int t0 = claim * t.chunk, t1 = min(t0 + t.chunk, t.ntiles);
+        const long long c0 = clock64();
         for (int k = t0; k < t1; k++) {
             persist_tile(P, t, k, arena);
             __syncthreads();
         }
+        const long long c1 = clock64();
         /* every wave's writes have reached the XCD's L2 before the count below */
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (threadIdx.x == 0) {
             const int cnt = t1 - t0;
+            __hip_atomic_fetch_add(P.cost + job, (unsigned long long)(c1 - c0), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             const int old = __hip_atomic_fetch_add(P.done + job, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (old + cnt == t.end) {
                 /* the step is complete: open the next one */
@@ -4515,11 +4527,12 @@ int pt_task_op(int op, int shape, const PtArgs &a, int arg, PtTask *t, const cha
     int v = 0;
     switch (op) {
     case PT_OP_PROPAGATE:
-        if (shape != PT_SHAPE_TAP && shape != PT_SHAPE_UP_EE_B && shape != PT_SHAPE_UP_A) *why = "propagate shape";
+        if (shape != PT_SHAPE_TAP && shape != PT_SHAPE_UP_EE_A && shape != PT_SHAPE_UP_EE_B && shape != PT_SHAPE_UP_A)
+            *why = "propagate shape";
         v = (shape << 4) | (prop_xl(shape, a) ? 8 : 0) | a.r;
         break;
     case PT_OP_SCATTER:
-        if (shape != PT_SHAPE_UP_EE_A && shape != PT_SHAPE_UP_A) *why = "scatter shape";
+        if (shape != PT_SHAPE_UP_EE_A && shape != PT_SHAPE_UP_EE_B && shape != PT_SHAPE_UP_A) *why = "scatter shape";
         v = (shape << 4) | a.r;
         break;
     case PT_OP_RESIDUAL:
