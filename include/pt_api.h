@@ -619,9 +619,9 @@ int pt_batch_tune(pt_batch *b, int rounds);
  * Persistent form (default 0): pt_batch_run makes ONE kernel launch that
  * runs every job to completion.  Each job's recorded plan becomes a device
  * program (one step per kernel its plain path launches, each step's
- * workgroup tiles claimed from a per-job counter); a job's steps run in
- * order, different jobs overlap freely, and the grid holds only as many
- * workgroups as fit on the GPU at once.  Outputs are bit-identical to the
+ * workgroup tiles claimed from per-job counters); a job's steps run in
+ * order on the XCDs it is placed on, different jobs overlap freely, and the
+ * grid holds only as many workgroups as fit on the GPU at once.  Outputs are bit-identical to the
  * plain path.  Jobs whose plans need a kernel without a device-program form
  * (temporal terms, extended patch-distance terms, style layers, keyframes,
  * patch radii other than 2 and 3) make pt_batch_run return
@@ -649,7 +649,7 @@ typedef struct pt_batch_stats {
     int64_t persistent_tasks;     /* steps of all jobs' device programs */
     int64_t persistent_tiles;     /* workgroup tiles of all steps */
     int32_t persistent_max_chunk; /* tiles per claim */
-    int32_t reserved;
+    int32_t persistent_spanned;   /* jobs placed on more than one XCD */
 } pt_batch_stats;
 int pt_batch_get_stats(pt_batch *b, pt_batch_stats *out);
 

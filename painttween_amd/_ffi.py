@@ -141,7 +141,7 @@ class BatchStats(C.Structure):
                 ("tune_ms", C.c_double), ("shape", (C.c_int32 * 3) * 2), ("xcd_remap", C.c_int32),
                 ("isolate_tiles", C.c_int32), ("persistent", C.c_int32), ("persistent_grid", C.c_int32),
                 ("persistent_tasks", C.c_int64), ("persistent_tiles", C.c_int64),
-                ("persistent_max_chunk", C.c_int32), ("reserved", C.c_int32)]
+                ("persistent_max_chunk", C.c_int32), ("persistent_spanned", C.c_int32)]
 
 
 class SequenceDesc(C.Structure):

@@ -25,6 +25,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <deque>
 #include <map>
@@ -250,12 +251,14 @@ struct pt_batch {
     PtArgs *pd_args = nullptr;
     unsigned long long *pd_claim = nullptr;
     int *pd_done = nullptr;
+    int *pd_parts = nullptr;
     unsigned long long *pd_cost = nullptr;
     int *pd_ctl = nullptr;
     /* measured work per job (shader clocks, by job index), used for the
      * placement when set (pt_batch_tune) */
     std::vector<double> pers_measured;
-    int pers_xcds = 0; /* XCDs the persistent grid spans */
+    int pers_xcds = 0;  /* XCDs the persistent grid spans */
+    int pers_spans = 0; /* jobs placed on more than one XCD */
 };
 
 static int bfail(pt_batch *b, int code, const char *fmt, ...)
@@ -291,6 +294,7 @@ static void free_persistent(pt_batch *b)
     (void)hipFree(b->pd_args);
     (void)hipFree(b->pd_claim);
     (void)hipFree(b->pd_done);
+    (void)hipFree(b->pd_parts);
     (void)hipFree(b->pd_cost);
     (void)hipFree(b->pd_ctl);
     b->pd_tasks = nullptr;
@@ -298,6 +302,7 @@ static void free_persistent(pt_batch *b)
     b->pd_args = nullptr;
     b->pd_claim = nullptr;
     b->pd_done = nullptr;
+    b->pd_parts = nullptr;
     b->pd_cost = nullptr;
     b->pd_ctl = nullptr;
     b->pers_valid = false;
@@ -309,15 +314,15 @@ static const char *op_name(int op)
     return (op >= 0 && op < 4) ? n[op] : "operator";
 }
 
-/* Tiles per claim of a task: a job runs on one XCD, so a step's tiles are
- * shared by that XCD's workgroups: about two claims each for the
+/* Tiles per claim of a task whose bands hold `band` tiles each: a band is
+ * shared by its XCD's workgroups, about two claims each for the
  * patch-distance operators (at most max_chunk tiles), one each for the
  * light kinds (copies, fills, syncs, pyramid and set-up kernels), whose
  * tiles take a few microseconds against several for a claim. */
-static int task_chunk(const PtTask &t, int wg_per_xcd, int max_chunk)
+static int task_chunk(const PtTask &t, int band, int wg_per_xcd, int max_chunk)
 {
     const bool heavy = t.kind == PT_TASK_PROPAGATE || t.kind == PT_TASK_SCATTER || t.kind == PT_TASK_RESIDUAL;
-    int c = heavy ? t.ntiles / (2 * wg_per_xcd) : (t.ntiles + wg_per_xcd - 1) / wg_per_xcd;
+    int c = heavy ? band / (2 * wg_per_xcd) : (band + wg_per_xcd - 1) / wg_per_xcd;
     const int cap = heavy ? max_chunk : 64;
     return c < 1 ? 1 : (c > cap ? cap : c);
 }
@@ -383,7 +388,6 @@ static int build_persistent(pt_batch *b)
                 } else {
                     est[j] += 10.0 * t.ntiles;
                 }
-                t.end = (int)cum;
                 jt[j].push_back(t);
             }
         }
@@ -413,13 +417,25 @@ static int build_persistent(pt_batch *b)
     if (xcds.empty()) return bfail(b, PT_ERR_HIP, "persistent form: XCD probe found no XCD");
     b->pers_xcds = (int)xcds.size();
     const int wg_per_xcd = b->pers_grid / b->pers_xcds > 0 ? b->pers_grid / b->pers_xcds : 1;
-    /* longest job first onto the least loaded XCD */
-    std::vector<double> load(xcds.size(), 0.0);
-    std::vector<int> home_of(n, 0);
+    if (b->pers_xcds > 8) return bfail(b, PT_ERR_UNSUPPORTED, "persistent form: more than 8 XCDs");
+    /* longest job first onto the least loaded XCDs; a job with more than
+     * one XCD's share of the batch gets ceil(work / share) XCDs */
+    const int nx = (int)xcds.size();
+    double total = 0.0;
+    for (double e : est) total += e;
+    const double share = total / nx;
+    std::vector<double> load(nx, 0.0);
+    std::vector<int> xmask_of(n, 0);
     for (int j : order) {
-        size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[k] += est[j];
-        home_of[j] = xcds[k];
+        int k = share > 0.0 ? (int)std::ceil(est[j] / share - 1e-9) : 1;
+        k = k < 1 ? 1 : (k > nx ? nx : k);
+        std::vector<int> idx(nx);
+        for (int x = 0; x < nx; x++) idx[x] = x;
+        std::stable_sort(idx.begin(), idx.end(), [&](int u, int v) { return load[u] < load[v]; });
+        for (int q = 0; q < k; q++) {
+            load[idx[q]] += est[j] / k;
+            xmask_of[j] |= 1 << xcds[idx[q]];
+        }
     }
     std::vector<PtTask> tasks;
     std::vector<PtArgs> args;
@@ -428,23 +444,40 @@ static int build_persistent(pt_batch *b)
     for (int j : order) {
         first.push_back((int)tasks.size());
         const int base = (int)args.size();
+        const int nb = __builtin_popcount((unsigned)xmask_of[j]);
+        int64_t band_cum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int parts = 0;
         for (PtTask t : jt[j]) {
             if (t.arg >= 0) t.arg += base;
-            t.chunk = task_chunk(t, wg_per_xcd, b->pers_max_chunk);
+            /* contiguous bands of the tiles, one per XCD of the job */
+            for (int q = 0; q <= 8; q++) t.band[q] = q <= nb ? (int)((int64_t)t.ntiles * q / nb) : t.ntiles;
+            for (int q = 0; q < 8; q++) {
+                const int len = q < nb ? t.band[q + 1] - t.band[q] : 0;
+                band_cum[q] += len;
+                t.band_end[q] = (int)band_cum[q];
+                parts += len > 0;
+            }
+            t.parts_end = parts;
+            t.chunk = task_chunk(t, (t.ntiles + nb - 1) / nb, wg_per_xcd, b->pers_max_chunk);
             tasks.push_back(t);
         }
         const PtTask &t0 = tasks[first.back()];
-        b->pers_claim0.push_back(pt_claim_word(0, (t0.ntiles + t0.chunk - 1) / t0.chunk));
+        for (int q = 0; q < 8; q++)
+            b->pers_claim0.push_back(
+                q < nb ? pt_claim_word(0, (t0.band[q + 1] - t0.band[q] + t0.chunk - 1) / t0.chunk) : 0ull);
         args.insert(args.end(), ja[j].begin(), ja[j].end());
     }
     first.push_back((int)tasks.size());
-    for (int j : order) first.push_back(home_of[j]);
+    for (int j : order) first.push_back(xmask_of[j]);
+    b->pers_spans = 0;
+    for (int j = 0; j < n; j++) b->pers_spans += __builtin_popcount((unsigned)xmask_of[j]) > 1;
     if (args.empty()) args.push_back(PtArgs{});
     B_HIP(b, hipMalloc((void **)&b->pd_tasks, tasks.size() * sizeof(PtTask)));
     B_HIP(b, hipMalloc((void **)&b->pd_first, first.size() * sizeof(int)));
     B_HIP(b, hipMalloc((void **)&b->pd_args, args.size() * sizeof(PtArgs)));
-    B_HIP(b, hipMalloc((void **)&b->pd_claim, (size_t)n * sizeof(unsigned long long)));
-    B_HIP(b, hipMalloc((void **)&b->pd_done, (size_t)n * sizeof(int)));
+    B_HIP(b, hipMalloc((void **)&b->pd_claim, (size_t)n * 8 * sizeof(unsigned long long)));
+    B_HIP(b, hipMalloc((void **)&b->pd_done, (size_t)n * 8 * sizeof(int)));
+    B_HIP(b, hipMalloc((void **)&b->pd_parts, (size_t)n * sizeof(int)));
     B_HIP(b, hipMalloc((void **)&b->pd_cost, (size_t)n * sizeof(unsigned long long)));
     B_HIP(b, hipMalloc((void **)&b->pd_ctl, 4 * sizeof(int)));
     B_HIP(b, hipMemcpy(b->pd_tasks, tasks.data(), tasks.size() * sizeof(PtTask), hipMemcpyHostToDevice));
@@ -469,7 +502,8 @@ static int run_persistent(pt_batch *b)
     const int n = (int)b->lanes.size();
     B_HIP(b, hipMemcpy(b->pd_claim, b->pers_claim0.data(), b->pers_claim0.size() * sizeof(unsigned long long),
                        hipMemcpyHostToDevice));
-    B_HIP(b, hipMemsetAsync(b->pd_done, 0, (size_t)n * sizeof(int), b->stream));
+    B_HIP(b, hipMemsetAsync(b->pd_done, 0, (size_t)n * 8 * sizeof(int), b->stream));
+    B_HIP(b, hipMemsetAsync(b->pd_parts, 0, (size_t)n * sizeof(int), b->stream));
     B_HIP(b, hipMemsetAsync(b->pd_cost, 0, (size_t)n * sizeof(unsigned long long), b->stream));
     const int ctl0[4] = {0, 0, b->pers_abort ? 1 : 0, 0};
     b->pers_abort = 0;
@@ -482,6 +516,7 @@ static int run_persistent(pt_batch *b)
     P.args = b->pd_args;
     P.claim = b->pd_claim;
     P.done = b->pd_done;
+    P.parts = b->pd_parts;
     P.cost = b->pd_cost;
     P.ctl = b->pd_ctl;
     P.njobs = n;
@@ -829,6 +864,7 @@ extern "C" int pt_batch_get_stats(pt_batch *b, pt_batch_stats *o)
     o->persistent_grid = b->pers_valid ? b->pers_grid : 0;
     o->persistent_tasks = b->pers_valid ? b->pers_tasks : 0;
     o->persistent_tiles = b->pers_valid ? b->pers_tiles : 0;
+    o->persistent_spanned = b->pers_valid ? b->pers_spans : 0;
     return PT_OK;
 }
 

@@ -242,7 +242,6 @@ struct PtTask {
                     downsample: channels */
     int ntiles;  /* workgroup tiles (> 0) */
     int tiles_x; /* tiles per row of a 2-D tiling */
-    int end;     /* the job's tiles through this step (cumulative) */
     int arg;     /* index of the step's PtArgs in the argument table, or -1 */
     int i[8];    /* integer operands */
     uint32_t u[2];
@@ -251,6 +250,13 @@ struct PtTask {
     const void *p[4]; /* source operands */
     void *q;          /* destination operand */
     int64_t n[3];     /* counts and pitches */
+    /* persistent launch: the tiles are split into one contiguous band per
+     * XCD the job runs on (band b: tiles band[b] .. band[b + 1]); per band
+     * the job's tiles through this step, and the job's nonempty bands
+     * through this step (cumulative counts) */
+    int band[9];
+    int band_end[8];
+    int parts_end;
 };
 
 /* Task capture: while a sink is active on the calling thread, the
@@ -286,10 +292,11 @@ __host__ __device__ inline int pt_cw_taken(unsigned long long w) { return (int)(
 struct PtPersist {
     const PtTask *tasks;       /* every job's program, concatenated        */
     const int *first;          /* njobs + 1: job j's steps are tasks[first[j] .. first[j+1]);
-                                  then njobs more: job j's XCD (first[njobs + 1 + j]) */
+                                  then njobs more: job j's XCDs (bit x of first[njobs + 1 + j]) */
     const PtArgs *args;
-    unsigned long long *claim; /* per job: claim word (pt_claim_word)     */
-    int *done;                 /* per job: tiles completed, all steps     */
+    unsigned long long *claim; /* 8 per job: claim word of each band (pt_claim_word) */
+    int *done;                 /* 8 per job: tiles of each band completed, all steps */
+    int *parts;                /* per job: bands completed, all steps      */
     unsigned long long *cost;  /* per job: shader clocks its claims took (placement feedback) */
     int *ctl;                  /* [0] jobs finished, [1] steps completed, [2] gave up */
     int njobs;

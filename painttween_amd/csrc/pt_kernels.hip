@@ -4187,29 +4187,33 @@ hipError_t pt_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind,
  * the steps of one job run in order, each one only after every tile of the
  * previous one has completed, while steps of different jobs overlap freely.
  *
- * A job runs on one XCD from its first claim to its end.  Its steps then
- * only pass data through that XCD's L2: a tile's writes are complete for the
- * next step once they have reached the L2 (the vector L1 writes through),
- * and a claim invalidates the claiming CU's L1.  Passing data between XCDs
- * would need an agent-scope release, an L2 write-back, at every completed
- * claim, which measured slower than the work itself (the batch took 1.7x as
- * long with it).  A claim still takes an agent-scope acquire, which
- * invalidates the CU's vector L1.  Workgroups read the XCD they run on from
- * the hardware.
+ * A job runs on a fixed set of XCDs; each step's tiles are split into one
+ * contiguous band per XCD (row bands of the image, the same for every step),
+ * so most of a step's inputs were written by the same XCD in the previous
+ * step.  Within an XCD a tile's writes are complete for the next step once
+ * they have reached its L2 (the vector L1 writes through): a claim completes
+ * after a workgroup-scope release and s_waitcnt 0, and starts with an
+ * agent-scope acquire (L1 invalidate).  When a job spans several XCDs, the
+ * workgroup that completes an XCD's band of a step also writes that XCD's
+ * L2 back (agent-scope release) before counting the band done, once per band
+ * and step: a write-back at every claim measured slower than the work itself
+ * (the batch took 1.7x as long).  The host places the jobs by their work
+ * (longest first onto the least loaded XCDs; a job with more than an XCD's
+ * share of the batch gets several), estimated from the patch taps of their
+ * steps or measured in an earlier launch (P.cost); workgroups read the XCD
+ * they run on from the hardware.
  *
- * The host places the jobs on the XCDs by their work (longest first onto
- * the least loaded XCD): estimated from the patch taps of their steps, or
- * the shader clocks their claims took in an earlier launch (P.cost).  Per job one 64-bit claim word holds
- * (step, claims of the step, claims taken); a claim is a run of
- * PtTask::chunk tiles.  A workgroup's first wave reads the claim words of 64
- * jobs at once and takes the first job (lowest index = highest priority,
- * the host orders jobs by work) that runs on its XCD and has an open claim,
- * with one atomic add; the add returns the step and claim count it claimed
- * in, so a claim that races with the opening of the next step is a valid
- * claim of that step.  The workgroup runs the tiles with the same tile bodies as the kernels of
- * the plain path, then adds their number to the job's completed-tile count;
- * the add that reaches the step's cumulative end writes the next step's
- * claim word.
+ * Per job and band one 64-bit claim word holds (step, claims of the band in
+ * the step, claims taken); a claim is a run of PtTask::chunk tiles.  A
+ * workgroup's first wave reads the words of 64 jobs for its own XCD at once
+ * and takes the first job (lowest index = highest priority, the host orders
+ * jobs by work) with an open claim, with one atomic add; the add returns the
+ * step and claim count it claimed in, so a claim that races with the opening
+ * of the next step is a valid claim of that step.  The workgroup runs the
+ * tiles with the same tile bodies as the kernels of the plain path, then adds
+ * their number to the band's completed-tile count; the add that reaches the
+ * band's cumulative end counts the band done, and the count that reaches the
+ * step's cumulative number of bands writes the next step's claim words.
  *
  * A workgroup never waits for a particular tile: when no job has claimable
  * tiles it sleeps and looks again, and it exits when every job has finished.
@@ -4394,7 +4398,7 @@ __device__ __forceinline__ void persist_tile(const PtPersist &P, const PtTask &t
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_persistent(PtPersist P)
 {
     __shared__ __attribute__((aligned(16))) char arena[kPersistLds];
-    __shared__ int s_mode, s_job, s_step, s_claim;
+    __shared__ int s_mode, s_job, s_step, s_claim, s_band;
     ConstTask *tasks = (ConstTask *)P.tasks;
     ConstInt *first = (ConstInt *)P.first;
     /* the XCD this workgroup runs on (HW_REG_XCC_ID, bits 3:0) */
@@ -4404,7 +4408,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     for (;;) {
         if (threadIdx.x < 64) {
             const int lane = threadIdx.x;
-            int job = -1, step = 0, claim = 0;
+            int job = -1, step = 0, claim = 0, band = 0;
             /* the give-up flag (set by a workgroup that timed out, or by the
              * host before the launch) stops every workgroup at its next look */
             const int gave =
@@ -4412,10 +4416,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             for (int base = 0; base < P.njobs && job < 0 && !gave; base += 64) {
                 const int j = base + lane;
                 unsigned long long w = 0;
+                int b = 0;
                 bool ok = false;
-                if (j < P.njobs && first[P.njobs + 1 + j] == xcd) {
-                    w = __hip_atomic_load(P.claim + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = pt_cw_taken(w) < pt_cw_claims(w);
+                if (j < P.njobs) {
+                    const int xm = first[P.njobs + 1 + j];
+                    if ((xm >> xcd) & 1) {
+                        b = __builtin_popcount(xm & ((1 << xcd) - 1)); /* this XCD's band */
+                        w = __hip_atomic_load(P.claim + (size_t)j * 8 + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = pt_cw_taken(w) < pt_cw_claims(w);
+                    }
                 }
                 unsigned long long m = __ballot(ok);
                 while (m != 0ull && job < 0) {
@@ -4423,8 +4432,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                     m &= m - 1ull;
                     int gs = -1, gk = 0;
                     if (lane == l) {
-                        const unsigned long long old =
-                            __hip_atomic_fetch_add(P.claim + j, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const unsigned long long old = __hip_atomic_fetch_add(
+                            P.claim + (size_t)j * 8 + b, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         if (pt_cw_taken(old) < pt_cw_claims(old)) {
                             gs = pt_cw_step(old);
                             gk = pt_cw_taken(old);
@@ -4435,6 +4444,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                         job = base + l;
                         step = gs;
                         claim = __shfl(gk, l);
+                        band = __shfl(b, l);
                     }
                 }
             }
@@ -4463,10 +4473,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 s_job = job;
                 s_step = step;
                 s_claim = claim;
+                s_band = band;
             }
         }
         __syncthreads();
-        const int mode = s_mode, job = s_job, step = s_step, claim = s_claim;
+        const int mode = s_mode, job = s_job, step = s_step, claim = s_claim, band = s_band;
         __syncthreads();
         if (mode == 2) break;
         if (mode == 1) {
@@ -4474,7 +4485,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             continue;
         }
         const PtTask &t = (const PtTask &)tasks[first[job] + step];
-        const int t0 = claim * t.chunk, t1 = min(t0 + t.chunk, t.ntiles);
+        const int t0 = t.band[band] + claim * t.chunk, t1 = min(t0 + t.chunk, t.band[band + 1]);
         const long long c0 = clock64();
         for (int k = t0; k < t1; k++) {
             persist_tile(P, t, k, arena);
@@ -4489,18 +4500,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             const int cnt = t1 - t0;
             __hip_atomic_fetch_add(P.cost + job, (unsigned long long)(c1 - c0), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-            const int old = __hip_atomic_fetch_add(P.done + job, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (old + cnt == t.end) {
-                /* the step is complete: open the next one */
-                const int ns = first[job + 1] - first[job];
-                unsigned long long w = pt_claim_word(ns, 0);
-                if (step + 1 < ns) {
-                    const PtTask &u = (const PtTask &)tasks[first[job] + step + 1];
-                    w = pt_claim_word(step + 1, (u.ntiles + u.chunk - 1) / u.chunk);
+            const int old =
+                __hip_atomic_fetch_add(P.done + (size_t)job * 8 + band, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old + cnt == t.band_end[band]) {
+                /* this XCD's band of the step is complete */
+                const int xm = first[P.njobs + 1 + job];
+                const int nb = __builtin_popcount(xm);
+                if (nb > 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); /* write this XCD's L2 back */
+                const int parts = __hip_atomic_fetch_add(P.parts + job, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (parts + 1 == t.parts_end) {
+                    /* the step is complete: open the next one on every band */
+                    if (nb > 1) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                    const int ns = first[job + 1] - first[job];
+                    for (int bb = 0; bb < nb; bb++) {
+                        unsigned long long w = pt_claim_word(ns, 0);
+                        if (step + 1 < ns) {
+                            const PtTask &u = (const PtTask &)tasks[first[job] + step + 1];
+                            w = pt_claim_word(step + 1, (u.band[bb + 1] - u.band[bb] + u.chunk - 1) / u.chunk);
+                        }
+                        __hip_atomic_store(P.claim + (size_t)job * 8 + bb, w, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    if (step + 1 == ns) __hip_atomic_fetch_add(P.ctl + 0, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_add(P.ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                __hip_atomic_store(P.claim + job, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (step + 1 == ns) __hip_atomic_fetch_add(P.ctl + 0, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(P.ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }

@@ -141,7 +141,8 @@ class Batch:
                     ticks=st.ticks, last_run_ms=st.last_run_ms, tune_ms=st.tune_ms, shapes=shapes,
                     xcd_remap=st.xcd_remap, isolate_tiles=st.isolate_tiles, persistent=st.persistent,
                     persistent_grid=st.persistent_grid, persistent_tasks=st.persistent_tasks,
-                    persistent_tiles=st.persistent_tiles, persistent_max_chunk=st.persistent_max_chunk)
+                    persistent_tiles=st.persistent_tiles, persistent_max_chunk=st.persistent_max_chunk,
+                    persistent_spanned=st.persistent_spanned)
 
     def result(self, j):
         return self.outputs[j].numpy()

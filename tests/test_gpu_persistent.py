@@ -47,8 +47,10 @@ def test_persistent_batch_equals_each_job_alone():
         assert st["persistent"] == 1 and st["persistent_tasks"] > 0 and st["persistent_tiles"] > 0, st
         # one workgroup per slot the device holds at once: at most 8 per CU on 256 CUs
         assert 0 < st["persistent_grid"] <= 8 * 256, st
-        print("persistent grid %d, %d tasks, %d tiles" % (st["persistent_grid"], st["persistent_tasks"],
-                                                            st["persistent_tiles"]), flush=True)
+        print("persistent grid %d, %d tasks, %d tiles, %d jobs on several XCDs" % (
+            st["persistent_grid"], st["persistent_tasks"], st["persistent_tiles"], st["persistent_spanned"]), flush=True)
+        # the 128x96 job holds about a third of the work: it is spread over several XCDs
+        assert st["persistent_spanned"] >= 1, st
         for chunk in (8, 1, 16):
             b.set(_ffi.PT_BATCH_PERSIST_CHUNK, chunk)
             for rep in range(2):
@@ -56,6 +58,23 @@ def test_persistent_batch_equals_each_job_alone():
                 b.run()
                 for i in range(len(jobs)):
                     _same(b.result(i), alone[i], "job %d, %d tiles per claim, run %d" % (i, chunk, rep))
+    finally:
+        b.close()
+
+
+def test_persistent_single_job_spans_every_xcd():
+    """One job alone holds the whole batch's work: its steps are split into
+    one band per XCD, each band's writes written back once per step."""
+    jobs = _batch_jobs()[2:3]
+    alone = _alone(jobs)
+    b = Batch(jobs)
+    try:
+        b.set(_ffi.PT_BATCH_PERSISTENT, 1)
+        assert b.stats()["persistent_spanned"] == 1
+        for rep in range(2):
+            _clear(b)
+            b.run()
+            _same(b.result(0), alone[0], "single job, run %d" % rep)
     finally:
         b.close()
 
