@@ -10,7 +10,10 @@ Beside it, on the same GPU in the same run:
   batched -- 64 uneven single-frame jobs (128^2..1024^2, aspect 0.5..2,
              5x5/7x7 patches, 3-5 levels, 6 iterations) planned as one
              multi-job launch sequence, shapes autotuned, replayed as a HIP
-             graph; compared per job, bit for bit, with the plain path;
+             graph, then run as one persistent launch; compared per job,
+             bit for bit, with the plain path;
+  extended_terms -- a 512^2 still with the default patch distance and with
+             coherence, LAB colour space and object IDs on;
   still256 -- the CPU-runnable still (256^2, 5x5 patches, 3 levels, 4
              iterations, no orientation): GPU frame time, the oracle on all
              host cores (median of 5) and on 1 core, offsets compared;
@@ -210,6 +213,51 @@ def run_still256(ctx, inputs, cpu_runs, cpu_threads):
                     "cpu_frames_per_s": 1.0 / cpu[cpu_threads], "cpu_1thread_frames_per_s": 1.0 / cpu[1],
                     "cpu_runs": cpu_runs, "gpu_vs_cpu": cpu[cpu_threads] / (gpu_ms * 1e-3),
                     "offsets_identical": bool(ref[0].tobytes() == gpu_off.tobytes())})
+    return res
+
+
+def run_ext_terms(ctx, steps=5):
+    """A 512^2 still (exemplar 512^2, 7x7 patches, 4 levels, 4 iterations,
+    orientation and distance-transform guides) with the default patch
+    distance, then with the coherence term, the LAB colour space and the
+    object-ID term on (the per-pixel kernels that evaluate them); GPU time per
+    frame of each (median of `steps`), inputs resident on the device."""
+    import torch
+    from painttween_amd import _ffi
+    from painttween_amd import workloads as W
+    from painttween_amd.forms import DeviceImages, DeviceOutputs
+    style = dict(W.style_pair(512, 512, seed=6, guides=True), object_ids=W.object_ids(6, 30, 512, 512))
+    frame = dict(W.still_frame(512, 512, seed=7, guides=True), object_id=W.object_ids(7, 31, 512, 512))
+    base = dict(residual_window_size=3, levels=4, dist_trans_weight=0.5, orientation_enabled=1)
+    ext = dict(base, coherence_weight=0.3, colorspace=1, style_obj_id_weight=0.6)
+    img = DeviceImages(0)
+    fg = (_ffi.FrameGuides * 1)()
+    fg[0] = img.frame(frame)
+    sg = img.style(style)
+    out = DeviceOutputs(512, 512, 1)
+    o, c, r = out.pointers()
+    res = {"image": [512, 512], "exemplar": [512, 512], "patch": 49, "levels": 4, "iterations": 4,
+           "terms": "coherence 0.3, LAB colour space, object IDs 0.6"}
+    for name, kw in (("default", base), ("extended", ext)):
+        params = ctx.default_params(**kw)
+
+        def run():
+            ctx._check(ctx.lib.pt_synthesize_device(ctx.handle, C.byref(params), _ffi.PT_S_FMBM_RM, 4, 1, 512, 512, 1,
+                                                    fg, 512, 512, C.byref(sg), o, c, r))
+
+        run()
+        torch.cuda.synchronize()
+        p0 = ctx.pde_count()
+        ts = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            run()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ms = float(np.median(ts)) * 1e3
+        pdes = (ctx.pde_count() - p0) // steps
+        res[name] = {"ms_per_frame": ms, "pde_per_frame": int(pdes), "pde_per_s": pdes / (ms * 1e-3)}
+    res["extended_vs_default"] = res["extended"]["ms_per_frame"] / res["default"]["ms_per_frame"]
     return res
 
 
@@ -589,6 +637,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-anim", action="store_true")
     ap.add_argument("--no-still256", action="store_true")
+    ap.add_argument("--no-ext", action="store_true", help="skip the extended patch-distance terms leg")
     ap.add_argument("--anim-levels", type=int, default=4)
     ap.add_argument("--anim-iterations", type=int, default=3)
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -751,6 +800,11 @@ def main():
     if still256_in is not None:
         line["still256"] = run_still256(ctx, still256_in, 0 if args.no_cpu else 5, min(os.cpu_count() or 1, 16))
         print("still256: %.2f ms on the GPU" % line["still256"]["gpu_ms"], file=sys.stderr, flush=True)
+    if not args.no_ext:
+        line["extended_terms"] = run_ext_terms(ctx)
+        print("extended terms: %.1f ms per 512^2 frame against %.1f ms with the default terms" % (
+            line["extended_terms"]["extended"]["ms_per_frame"], line["extended_terms"]["default"]["ms_per_frame"]),
+              file=sys.stderr, flush=True)
     if anim_in is not None:
         line["animation"] = run_animation(ctx, anim_in, args.anim_levels, args.anim_iterations,
                                           0 if args.no_cpu else min(os.cpu_count() or 1, 16))

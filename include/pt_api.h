@@ -454,6 +454,14 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * it.  Same results. */
 #define PT_TUNE_PROPAGATE_WINDOW_JUMP 10
 #define PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS 11
+/* PT_TUNE_TEMPORAL_UNIT_ORDER: with temporal terms and time tables, the
+ * order in which a unit-parallel workgroup enumerates its (pixel,
+ * candidate) units: 0 candidate-major (neighbouring lanes evaluate the same
+ * candidate of neighbouring pixels: coherent exemplar gathers), 1
+ * pixel-major in the propagate steps with an exemplar window (the lanes of
+ * one pixel read the same time-table entries in the same instruction), 2
+ * pixel-major in every propagate and scatter step.  Same results. */
+#define PT_TUNE_TEMPORAL_UNIT_ORDER 12
 int pt_set_tuning(pt_ctx *ctx, int key, int value);
 
 /* ---- timing (no reference counterpart; the reference has no GPU timers) --

@@ -41,6 +41,7 @@ PT_TUNE_TIME_TABLE_MAX_TAPS = 8
 PT_TUNE_XCD_TILES = 9
 PT_TUNE_PROPAGATE_WINDOW_JUMP = 10
 PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS = 11
+PT_TUNE_TEMPORAL_UNIT_ORDER = 12
 PT_SERVER_FILL_TILES = 5
 PT_DOWNSAMPLE_SCALE_WITH_LEVELS, PT_DOWNSAMPLE_SCALE_CONSTANT, PT_DOWNSAMPLE_RIBBON = 0, 1, 2
 

@@ -79,6 +79,7 @@ struct PtArgs {
     int xcd_tiles;     /* single-job launches: contiguous tile ranges per XCD */
     int prop_window;      /* unit-parallel propagate: exemplar windows on this level ... */
     int prop_window_jump; /* ... for jumps up to this */
+    int time_unit_order;  /* PT_TUNE_TEMPORAL_UNIT_ORDER */
 
     /* style layers (keyframe interpolation): with `layered` set every
      * offset carries its layer (PtNnf::la) and the per-pixel kernels' RT < 0

@@ -2064,7 +2064,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     __syncthreads();
     /* phase 1: candidate set-up in lockstep, in the order of up_pixel_major
      * (the list, and the first units each lane evaluates, follow it) */
-    const bool pmaj = up_pixel_major<TM>(A);
+    const bool pmaj = up_pixel_major<TM>(A) || (TM && (A.time_unit_order == 2 || (XL && A.time_unit_order == 1)));
     for (int v = threadIdx.x; v < U; v += PT_UP_THREADS) {
         int p = pmaj ? v >> 2 : v % P, k = pmaj ? v & 3 : v / P, u = 4 * p + k;
         int x = x0 + p % TX, y = y0 + p / TX;
@@ -2233,7 +2233,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     const int x0 = bx * TX, y0 = by * TY;
     const int r = RT > 0 ? RT : A.r;
     const int SU = n + 1; /* units per pixel */
-    const bool pmaj = up_pixel_major<TM>(A);
+    const bool pmaj = up_pixel_major<TM>(A) || (TM && A.time_unit_order == 2);
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
     int sdir = -1;
     if constexpr (TM) sdir = stage_adv_up<RT, TX, TY>(A, x0, y0, s_ab, s_ao, s_av);

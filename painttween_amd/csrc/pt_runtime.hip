@@ -158,6 +158,7 @@ struct pt_ctx {
     bool xcd_tiles = true; /* contiguous tile ranges per XCD in single-job launches */
     int prop_window_jump = 8; /* XL propagate for jumps up to this (0: never) ... */
     int64_t prop_window_min_pixels = 200000; /* ... on levels of at least this size (0: never) */
+    int time_unit_order = 0; /* PT_TUNE_TEMPORAL_UNIT_ORDER */
     Rect off_diff, res_diff;
     /* image textures */
     float4 *color[2] = {nullptr, nullptr};
@@ -702,6 +703,10 @@ extern "C" int pt_set_tuning(pt_ctx *c, int key, int value)
     case PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS:
         if (value < 0) return set_err(c, PT_ERR_INVALID_ARG, "negative pixel count");
         c->prop_window_min_pixels = value;
+        return PT_OK;
+    case PT_TUNE_TEMPORAL_UNIT_ORDER:
+        if (value < 0 || value > 2) return set_err(c, PT_ERR_INVALID_ARG, "unit order must be 0, 1 or 2");
+        c->time_unit_order = value;
         return PT_OK;
     default:
         return set_err(c, PT_ERR_INVALID_ARG, "unknown tuning key %d", key);
@@ -1411,6 +1416,7 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.xcd_tiles = c->xcd_tiles ? 1 : 0;
     a.prop_window_jump = c->prop_window_jump;
     a.prop_window = (c->prop_window_min_pixels > 0 && (int64_t)a.lw * a.lh >= c->prop_window_min_pixels) ? 1 : 0;
+    a.time_unit_order = c->time_unit_order;
     a.changed = c->changed;
     a.base = c->base;
     a.work = c->work;
