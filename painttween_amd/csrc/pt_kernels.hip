@@ -197,11 +197,12 @@ __device__ __forceinline__ float ssd_alpha(float4 a, float4 b, float alpha)
     return s;
 }
 
-/* RGBtoLAB of a colour (texSynth_util.h:220-249), alpha unchanged */
+/* RGBtoLAB of a colour (texSynth_util.h:220-249), alpha unchanged; the
+ * fast powers give the plain conversion's floats bit for bit */
 __device__ __forceinline__ float4 lab4(float4 c)
 {
     float l, a, b;
-    pt_rgb_to_lab(c.x, c.y, c.z, &l, &a, &b);
+    pt_rgb_to_lab_fast(c.x, c.y, c.z, &l, &a, &b);
     return make_float4(l, a, b, c.w);
 }
 
@@ -556,12 +557,9 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
         for (int j = -r; j <= r; j++) {
             if (y + j < 0 || y + j >= A.lh)
                 continue;
+            /* staged converted to LAB when the colour space is on */
             float4 base = tcol[j * T.tw];
             float4 canvas = tcan[j * T.tw];
-            if (RT < 0 && A.lab) {
-                base = lab4(base);
-                canvas = lab4(canvas);
-            }
             float tx = t.x + (ibx + (float)j * b1x);
             float ty = t.y + (iby + (float)j * b1y);
             if (tx >= hw) tx -= hw;
@@ -696,7 +694,9 @@ template <int RT> struct TileSize {
 
 /* Stages the block's target colour and canvas (with an r halo, coordinates
  * clamped into the level; clamped entries are never read because taps
- * outside the level are skipped) and returns the lane's view. */
+ * outside the level are skipped) and returns the lane's view.  With the LAB
+ * colour space (RT < 0) both are staged converted: every tap reading the
+ * entry converts that same colour (texSynth_kernel.cu:575-578). */
 template <int RT>
 __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, float4 *scol, float4 *scan)
 {
@@ -704,11 +704,17 @@ __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, floa
     const int tw = PT_BX + 2 * r, th = PT_BY + 2 * r;
     const int x0 = bx * PT_BX - r, y0 = by * PT_BY - r;
     const float4 *color = A.color[A.cur];
+    const bool lab = RT < 0 && A.lab;
     for (int k = threadIdx.x; k < tw * th; k += PT_BX * PT_BY) {
         int tx = clampi(x0 + k % tw, 0, A.lw - 1);
         int ty = clampi(y0 + k / tw, 0, A.lh - 1);
-        scol[k] = color[ty * A.W + tx];
-        scan[k] = A.canvas[ty * A.W + tx];
+        float4 c = color[ty * A.W + tx], v = A.canvas[ty * A.W + tx];
+        if (lab) {
+            c = lab4(c);
+            v = lab4(v);
+        }
+        scol[k] = c;
+        scan[k] = v;
     }
     __syncthreads();
     Tile T;

@@ -44,6 +44,9 @@
  *      (:166-195), keeping the reference's mix of double and float
  *      arithmetic; powf(c, 3) and powf(c, 2) of its constants are the
  *      correctly rounded float values.
+ *  - pt_rgb_to_lab_fast: the same function, bit for bit, with its two powers
+ *      evaluated by inverse-root Newton iterations (below); the kernels use
+ *      it, the CPU oracle the plain form.
  *  - pt_philox4x32_10:
  *      Philox4x32-10 counter-based generator (J. K. Salmon, M. A. Moraes,
  *      R. O. Dror, D. E. Shaw, "Parallel Random Numbers: As Easy as 1, 2, 3",
@@ -399,6 +402,121 @@ PT_HD void pt_rgb_to_lab(float r, float g, float b, float *lo, float *ao, float 
     *lo = (116.f * fy - 16) / 100.f;
     *ao = (500.f * (pt_f_lab(x / ref_x) - fy) + 100) / 200.f;
     *bo = (200.f * (fy - pt_f_lab(z / ref_z)) + 100) / 200.f;
+}
+
+/*
+ * Fast forms of the two powers of the colour conversion, equal bit for bit
+ * to pt_srgb_to_linear and pt_f_lab.  Inside a fast range each forms the
+ * power in double from an inverse root w: an exponent-bits estimate (the
+ * float's bit pattern read as a scaled base-2 logarithm), three Newton steps
+ * in float, one in double, which leaves the power within about 2^-41
+ * relative of its exact value.  The result is rounded to float only when
+ * every value within 2^-38 relative of it rounds to the same float, which
+ * then is also the rounding of the plain form's double-series value (within
+ * 2^-48 of the exact power); otherwise, and outside the fast range, the plain
+ * form is evaluated.  The identity is checked over every float input of the
+ * fast ranges by tests/lab_fast_check.c.
+ */
+#define PT_ROOT5_MAGIC 0x4c2c47e6u /* 6/5 * 2^23 * (127 - 0.0450465) */
+#define PT_ROOT3_MAGIC 0x54a2fa8eu /* 4/3 * 2^23 * (127 - 0.0450465) */
+#define PT_SRGB_K 0x1.c242be51c25dfp-1 /* (1 + (double)0.055f)^(-12/5) */
+#define PT_LAB_DP 0x1.5555555555555p-27 /* (double)0.33333334f - 1/3 */
+#define PT_FAST_TOL 0x1p-38
+
+PT_HD float pt_bits_to_float(uint32_t u)
+{
+    union {
+        uint32_t u;
+        float f;
+    } b;
+    b.u = u;
+    return b.f;
+}
+
+PT_HD uint32_t pt_float_to_bits(float f)
+{
+    union {
+        float f;
+        uint32_t u;
+    } b;
+    b.f = f;
+    return b.u;
+}
+
+/* (v / (1 + (double)0.055f))^2.4 for v = c + 0.055f in (0.0955, 2.06]:
+ * (v w)^3 K with w = v^(-1/5); the exponent 12/5 differs from (double)2.4
+ * by 9e-17. */
+PT_HD double pt_srgb_pow_est(float v)
+{
+    float w = pt_bits_to_float(PT_ROOT5_MAGIC - pt_float_to_bits(v) / 5u);
+    for (int k = 0; k < 3; k++) {
+        float w2 = w * w;
+        float q = v * (w2 * w2 * w);
+        w = w * fmaf(-0.2f, q, 1.2f);
+    }
+    double V = (double)v, W = (double)w;
+    double W2 = W * W;
+    W = W * fma(-0.2, V * (W2 * W2 * W), 1.2);
+    double p = V * W;
+    return p * p * p * PT_SRGB_K;
+}
+
+/* t^p for p = (double)0.33333334f and t in (PT_LAB_T0, 4]: t w^2 with
+ * w = t^(-1/3), times t^(p - 1/3) = 1 + (p - 1/3) log t (the next term is
+ * below 2e-15). */
+PT_HD double pt_f_lab_pow_est(float t)
+{
+    float w = pt_bits_to_float(PT_ROOT3_MAGIC - pt_float_to_bits(t) / 3u);
+    for (int k = 0; k < 3; k++) {
+        float q = t * (w * w * w);
+        w = w * fmaf(-0.333333343f, q, 1.33333337f);
+    }
+    double T = (double)t, W = (double)w;
+    W = W * fma(-1.0 / 3.0, T * (W * W * W), 4.0 / 3.0);
+    return (T * (W * W)) * fma(PT_LAB_DP, (double)pt_logf(t), 1.0);
+}
+
+/* 1 when every value within PT_FAST_TOL relative of r > 0 rounds to the
+ * same float, stored in *f */
+PT_HD int pt_round_decided(double r, float *f)
+{
+    float lo = (float)(r * (1.0 - PT_FAST_TOL)), hi = (float)(r * (1.0 + PT_FAST_TOL));
+    *f = lo;
+    return lo == hi;
+}
+
+PT_HD float pt_srgb_to_linear_fast(float c)
+{
+    if ((double)c > 0.04045 && c <= 2.0f) {
+        float f;
+        if (pt_round_decided(pt_srgb_pow_est(c + 0.055f), &f)) return f;
+    }
+    return pt_srgb_to_linear(c);
+}
+
+PT_HD float pt_f_lab_fast(float t)
+{
+    if (t > PT_LAB_T0 && t <= 4.0f) {
+        float f;
+        if (pt_round_decided(pt_f_lab_pow_est(t), &f)) return f;
+    }
+    return pt_f_lab(t);
+}
+
+/* pt_rgb_to_lab with the fast powers: the same operations otherwise */
+PT_HD void pt_rgb_to_lab_fast(float r, float g, float b, float *lo, float *ao, float *bo)
+{
+    const float ref_x = 95.0469970703125f, ref_y = 100.0f, ref_z = 108.88300323486328f;
+    float rl = 100 * pt_srgb_to_linear_fast(r);
+    float gl = 100 * pt_srgb_to_linear_fast(g);
+    float bl = 100 * pt_srgb_to_linear_fast(b);
+    float x = (float)(0.4124 * (double)rl + 0.3576 * (double)gl + 0.1805 * (double)bl);
+    float y = (float)(0.2126 * (double)rl + 0.7152 * (double)gl + 0.0722 * (double)bl);
+    float z = (float)(0.0193 * (double)rl + 0.1192 * (double)gl + 0.9505 * (double)bl);
+    float fy = pt_f_lab_fast(y / ref_y);
+    *lo = (116.f * fy - 16) / 100.f;
+    *ao = (500.f * (pt_f_lab_fast(x / ref_x) - fy) + 100) / 200.f;
+    *bo = (200.f * (fy - pt_f_lab_fast(z / ref_z)) + 100) / 200.f;
 }
 
 /* Philox4x32-10. ctr[4], key[2] in; out[4]. */

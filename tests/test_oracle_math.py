@@ -172,3 +172,30 @@ def test_rgb_to_lab_against_float64_and_exact_values():
         lib.ptref_rgb_to_lab(*map(float, rgb), out)
         assert abs(out[0] * 100 - lab[0]) < 0.05 and abs(out[1] * 200 - 100 - lab[1]) < 0.1 \
             and abs(out[2] * 200 - 100 - lab[2]) < 0.1, (rgb, list(out))
+
+
+def test_fast_lab_powers_equal_plain_forms_on_every_float():
+    """The kernels' fast colour-conversion powers (inverse-root Newton steps,
+    rounded only when the rounding is decided) return the plain forms' floats
+    bit for bit for every float input around their fast ranges; the checker
+    also bounds the fast double estimate's distance from the plain form's
+    double value well inside the rounding-decision tolerance (2^-38)."""
+    import os
+    import subprocess
+    import tempfile
+    from painttween_amd import build as B
+    flags = [f for f in B.ORACLE_FLAGS if f not in ("-shared", "-fPIC")]
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "lab_fast_check")
+        subprocess.run(["gcc"] + flags + ["-I", B.CSRC, "-o", exe, os.path.join(os.path.dirname(__file__),
+                                                                                "lab_fast_check.c"), "-lm"],
+                       check=True)
+        out = subprocess.run([exe], check=True, stdout=subprocess.PIPE, text=True).stdout
+    lines = {ln.split()[0]: ln.split() for ln in out.strip().splitlines()}
+    for name, min_inputs in (("srgb", 50_000_000), ("f_lab", 75_000_000)):
+        f = lines[name]
+        vals = dict(zip(f[1::2], f[2::2]))
+        assert int(vals["inputs"]) >= min_inputs, out
+        assert int(vals["mismatches"]) == 0, out
+        assert float(vals["worst_rel"]) < 2.0 ** -40, out
+        assert int(vals["undecided"]) < int(vals["inputs"]) // 1000, out
