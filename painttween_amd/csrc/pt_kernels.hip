@@ -500,10 +500,20 @@ __device__ __forceinline__ float coherence_add(const PtArgs &A, float res, int x
     return res;
 }
 
+/* Radius codes of the kernel templates: RT = 2, 3, 4 a patch radius fixed
+ * at compile time, 0 a run-time radius; RT < 0 the variants with the
+ * extended patch-distance terms (coherence, LAB colour space, object IDs):
+ * -1 at a run-time radius (the per-pixel kernels, style layers included),
+ * -2 and -3 at radius 2 and 3 (the unit-parallel kernels, without style
+ * layers).  The default kernels (RT >= 0) leave those terms out and keep
+ * their registers. */
+template <int RT> struct Rt {
+    static constexpr int R = RT > 0 ? RT : (RT < -1 ? -RT : 0);
+    static constexpr bool EXT = RT < 0;
+};
+#define PT_RAD(RT, r) (Rt<RT>::R > 0 ? Rt<RT>::R : (r))
+
 /*
- * RT < 0 selects the variant of the per-pixel kernels with the extended
- * terms (coherence, LAB colour space, object IDs) at a run-time radius; the default
- * kernels (RT >= 0) leave them out and keep their registers.
  *
  * calculateResidual (texSynth_kernel.cu:537-659): one patch distance of
  * candidate (t, tsc) at pixel (x, y).  Taps run i over x outer, j over y
@@ -517,7 +527,7 @@ template <int RT, bool TIME, bool XL = false>
 __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y, float4 t, float2 tsc, float4 pf,
                                     float4 pb, bool cpv, bool cnv, int tl = 0, int pfl = 0, int pbl = 0)
 {
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     /* style layers of the candidate and of the advected offsets (RT < 0) */
     const int tlb = RT < 0 ? lay(A, tl) : 0;
     const int pflb = RT < 0 ? lay(A, pfl) : 0, pblb = RT < 0 ? lay(A, pbl) : 0;
@@ -688,7 +698,7 @@ __device__ __forceinline__ bool pixel_of(const PtArgs &A, int bx, int by, int &x
 }
 
 template <int RT> struct TileSize {
-    static constexpr int R = RT > 0 ? RT : PT_MAX_R;
+    static constexpr int R = PT_RAD(RT, PT_MAX_R);
     static constexpr int N = (PT_BX + 2 * R) * (PT_BY + 2 * R);
 };
 
@@ -700,7 +710,7 @@ template <int RT> struct TileSize {
 template <int RT>
 __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, float4 *scol, float4 *scan)
 {
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     const int tw = PT_BX + 2 * r, th = PT_BY + 2 * r;
     const int x0 = bx * PT_BX - r, y0 = by * PT_BY - r;
     const float4 *color = A.color[A.cur];
@@ -758,7 +768,7 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
     PT_LDS_ARRAY(unsigned char, sflag, TileSize<RT>::N)
     PT_LDS_ARRAY(unsigned short, slist, PT_BX * PT_BY)
     PT_LDS_VAR(int, scount)
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     const bool inc = A.res_incremental != 0;
     if (inc) {
         const int tw = PT_BX + 2 * r, th = PT_BY + 2 * r;
@@ -1264,13 +1274,15 @@ __device__ __forceinline__ float tap_weight_sum(const PtArgs &A, int r, int x, i
 }
 
 /* One row (fixed i) of calculateResidual's tap loop, accumulated into
- * (acc, swa) with the reference's per-tap arithmetic. */
+ * (acc, swa) with the reference's per-tap arithmetic; RT < 0 adds the
+ * extended terms in calc_residual_impl's order (the tile's target and
+ * canvas are staged converted to LAB when that colour space is on). */
 template <int RT, bool TIME, bool XL = false, bool TS = false>
 __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
                                           int i, const Cand &c, float4 pf, float4 pb, bool cpv, bool cnv,
                                           const float *tb4, float &acc, float &swa)
 {
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     const int ew = A.ew, eh = A.eh;
     const float hw = (float)A.hw, hh = (float)A.hh;
     const float gi = A.gauss[i + r];
@@ -1291,6 +1303,10 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
         else if (ty < 0) ty += hh;
         float4 tb, to;
         ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to);
+        if (Rt<RT>::EXT && A.lab) {
+            tb = lab4(tb);
+            to = lab4(to);
+        }
         float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
         if (TIME) {
             bool any_time_w = A.tdi > 0.f || A.tdo > 0.f || A.tcw > 0.f;
@@ -1303,18 +1319,19 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
                     const bool k0 = T.sdir == 0;
                     if ((k0 ? (fwd_dir && cpv) : (bwd_dir && cnv)) && T.av[si]) {
                         const float *b4 = tb4 + (k0 ? 0 : 4);
-                        res = res + time_terms_v(A, er, T.sdir, k0 ? pf : pb, b4[0], b4[1], b4[2], b4[3], base, tb,
-                                                 canvas, to, T.ab[si], T.ao[si], x, y, i, j);
+                        res = res + time_terms_v<Rt<RT>::EXT>(A, er, T.sdir, k0 ? pf : pb, b4[0], b4[1], b4[2],
+                                                              b4[3], base, tb, canvas, to, T.ab[si], T.ao[si], x, y,
+                                                              i, j);
                     }
                 }
             } else if (any_time_w) {
                 int oo = (y + j) * A.W + (x + i);
                 if (fwd_dir && cpv && A.adv[0].xyth[oo].w > 0)
-                    res = res + time_terms_b(A, er, 0, pf, tb4[0], tb4[1], tb4[2], tb4[3], base, tb, canvas,
-                                             to, oo, x, y, i, j);
+                    res = res + time_terms_b<Rt<RT>::EXT>(A, er, 0, pf, tb4[0], tb4[1], tb4[2], tb4[3], base, tb,
+                                                          canvas, to, oo, x, y, i, j);
                 if (bwd_dir && cnv && A.adv[1].xyth[oo].w > 0)
-                    res = res + time_terms_b(A, er, 1, pb, tb4[4], tb4[5], tb4[6], tb4[7], base, tb, canvas,
-                                             to, oo, x, y, i, j);
+                    res = res + time_terms_b<Rt<RT>::EXT>(A, er, 1, pb, tb4[4], tb4[5], tb4[6], tb4[7], base, tb,
+                                                          canvas, to, oo, x, y, i, j);
             }
         }
         if (i == 0 && j == 0) {
@@ -1326,8 +1343,14 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
                 diff = fminf(fmaxf(diff, 0.0f), 10.0f);
                 res = res + diff * A.dt_w;
             }
+            if (Rt<RT>::EXT && A.obj_w > 0.f) {
+                int iid = A.img_id[point_index(A.idw, A.idh, (float)x, (float)y)];
+                int sid = A.ex_id[point_index(A.ew, A.eh, tx, ty)];
+                if (iid != sid) res = res + A.obj_w;
+            }
         } else {
             res = fmaf(ssd_alpha(canvas, to, A.alpha_w), A.canvas_w, res);
+            if (Rt<RT>::EXT) res = coherence_add(A, res, x, y, i, j, c.t.x, c.t.y, tx, ty);
         }
         float wgt = gi * A.gauss[j + r];
         acc = fmaf(res, wgt, acc);
@@ -1444,7 +1467,7 @@ template <int RT, bool TIME, class Next, class Accept>
 __device__ __forceinline__ void ee_search(const PtArgs &A, const Tile &T, int x, int y, float &best, Next next,
                                           Accept accept)
 {
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
     float4 pf, pb;
     bool cpv, cnv;
@@ -1726,21 +1749,29 @@ __device__ __forceinline__ float4 average_candidate(const PtArgs &A, int x, int 
 #define PT_INF __builtin_inff()
 
 template <int RT, int TX, int TY> struct UpTile {
-    static constexpr int R = RT > 0 ? RT : PT_MAX_R;
+    static constexpr int R = PT_RAD(RT, PT_MAX_R);
     static constexpr int N = (TX + 2 * R) * (TY + 2 * R);
 };
 
+/* stage_tile's staging for the unit kernels (LAB-converted in the
+ * extended variants when that colour space is on) */
 template <int RT, int TX, int TY>
 __device__ __forceinline__ void stage_tile_up(const PtArgs &A, int x0, int y0, float4 *scol, float4 *scan)
 {
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     const int tw = TX + 2 * r, th = TY + 2 * r;
     const float4 *color = A.color[A.cur];
+    const bool lab = Rt<RT>::EXT && A.lab;
     for (int k = threadIdx.x; k < tw * th; k += PT_UP_THREADS) {
         int tx = clampi(x0 - r + k % tw, 0, A.lw - 1);
         int ty = clampi(y0 - r + k / tw, 0, A.lh - 1);
-        scol[k] = color[ty * A.W + tx];
-        scan[k] = A.canvas[ty * A.W + tx];
+        float4 c = color[ty * A.W + tx], v = A.canvas[ty * A.W + tx];
+        if (lab) {
+            c = lab4(c);
+            v = lab4(v);
+        }
+        scol[k] = c;
+        scan[k] = v;
     }
 }
 
@@ -1758,7 +1789,7 @@ __device__ __forceinline__ int stage_adv_up(const PtArgs &A, int x0, int y0, flo
     if (fwd == bwd) return -1;
     const int k = fwd ? 0 : 1;
     const PtAdv &d = A.adv[k];
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     const int tw = TX + 2 * r, th = TY + 2 * r;
     for (int e = threadIdx.x; e < tw * th; e += PT_UP_THREADS) {
         int tx = clampi(x0 - r + e % tw, 0, A.lw - 1);
@@ -1781,7 +1812,7 @@ template <int RT, bool EE, int TX, bool TM, bool XL = false>
 __device__ __forceinline__ float up_eval(const PtArgs &A, const Tile &T, __amdgpu_buffer_rsrc_t er, int x, int y,
                                          Cand c, const TimeCand &tc, float best, float sw)
 {
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     const float a0 = TM ? tc.acc0 : 0.f;
     float acc = a0, swa = 0.f;
     if (EE) {
@@ -1827,7 +1858,7 @@ __device__ __forceinline__ void up_queue_ee(const PtArgs &A, Tile &T, __amdgpu_b
                                             Done done)
 {
     static_assert(!(KEEP && TM), "the replayed rows carry no temporal terms");
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float tb4[8];
     Cand c;
@@ -2042,7 +2073,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     PT_LDS_VAR(int, s_n)
     PT_LDS_VAR(int, s_next)
     const int x0 = bx * TX, y0 = by * TY;
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
     int sdir = -1;
     if constexpr (TM) sdir = stage_adv_up<RT, TX, TY>(A, x0, y0, s_ab, s_ao, s_av);
@@ -2237,7 +2268,7 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
     PT_LDS_ARRAY(float4, s_ao, (TM ? N : 1))
     PT_LDS_ARRAY(unsigned char, s_av, (TM ? N : 1))
     const int x0 = bx * TX, y0 = by * TY;
-    const int r = RT > 0 ? RT : A.r;
+    const int r = PT_RAD(RT, A.r);
     const int SU = n + 1; /* units per pixel */
     const bool pmaj = up_pixel_major<TM>(A) || (TM && A.time_unit_order == 2);
     stage_tile_up<RT, TX, TY>(A, x0, y0, scol, scan);
@@ -3501,6 +3532,10 @@ static PtTask task_2d(int kind, int w, int h, int tx, int ty)
  * variant */
 static bool ext_terms(const PtArgs &a) { return a.coh_w > 0.f || a.lab || a.obj_w > 0.f || a.layered; }
 
+/* ... and, at radius 2 or 3 without style layers, the default unit-parallel
+ * propagate and scatter shapes' RT = -2, -3 variants */
+static bool ext_up(const PtArgs &a) { return ext_terms(a) && !a.layered && !a.keys && (a.r == 2 || a.r == 3); }
+
 static dim3 grid_for(int w, int h) { return dim3((w + PT_BX - 1) / PT_BX, (h + PT_BY - 1) / PT_BY); }
 static const dim3 kBlock(PT_BX * PT_BY);
 static dim3 grid_rows(int w, int h) { return dim3((w + PT_RX - 1) / PT_RX, (h + PT_RY - 1) / PT_RY); }
@@ -3670,9 +3705,20 @@ int pt_shape_valid(int op, int shape, const PtArgs &a)
 {
     if (shape < 0 || shape >= PT_SHAPE_COUNT) return 0;
     /* the coherence term, the LAB colour space, the object-ID term and style
-     * layers are evaluated by the per-pixel in-order kernels only (their
-     * RT < 0 variant) */
-    if (ext_terms(a) && shape != PT_SHAPE_PIXEL) return 0;
+     * layers are evaluated by the per-pixel in-order kernels (their RT = -1
+     * variant) and, where ext_up holds, by the unit-parallel shapes that
+     * pt_default_shape picks for propagate (8x8, with and without early
+     * rejection) and scatter (16x8 with, 4x4 without) */
+    if (ext_terms(a) && shape != PT_SHAPE_PIXEL) {
+        if (!ext_up(a)) return 0;
+        if (op == PT_OP_PROPAGATE) {
+            if (shape != PT_SHAPE_UP_EE_B && shape != PT_SHAPE_UP_A) return 0;
+        } else if (op == PT_OP_SCATTER) {
+            if (shape != PT_SHAPE_UP_EE_A && shape != PT_SHAPE_UP_A) return 0;
+        } else {
+            return 0;
+        }
+    }
     if (shape == PT_SHAPE_TAP) return tap_pixels(op, a) > 0;
     if (op == PT_OP_RESIDUAL || op == PT_OP_SYNC_CANVAS) return shape == PT_SHAPE_PIXEL;
     if (op != PT_OP_PROPAGATE && op != PT_OP_SCATTER) return 0;
@@ -3730,7 +3776,7 @@ static bool prop_xl(int shape, const PtArgs &a)
 int pt_shape_key(int op, int shape, const PtArgs &a)
 {
     int tk = (op == PT_OP_SYNC_CANVAS) ? 0 : (a.time_active ? 1 : 0);
-    int rt = (op == PT_OP_SYNC_CANVAS) ? 0 : (ext_terms(a) ? 1 : rt_of(a.r));
+    int rt = (op == PT_OP_SYNC_CANVAS) ? 0 : (ext_terms(a) ? (shape_is_up(shape) ? 4 + a.r : 1) : rt_of(a.r));
     int xl = (op == PT_OP_PROPAGATE && prop_xl(shape, a)) ? 1 : 0;
     return (((op * PT_SHAPE_COUNT + shape) * 8 + rt) * 2 + tk) * 2 + xl;
 }
@@ -3747,6 +3793,14 @@ int pt_shape_key(int op, int shape, const PtArgs &a)
 #define PT_RTX_SWITCH(a, F)                                                                               \
     if (ext_terms(a)) {                                                                                   \
         F(-1);                                                                                            \
+    } else {                                                                                              \
+        PT_RT_SWITCH((a).r, F)                                                                            \
+    }
+/* the unit-parallel shapes that carry the extended terms (ext_up) */
+#define PT_RTU_SWITCH(a, F)                                                                               \
+    if (ext_terms(a)) {                                                                                   \
+        if ((a).r == 2) F(-2);                                                                            \
+        else F(-3);                                                                                       \
     } else {                                                                                              \
         PT_RT_SWITCH((a).r, F)                                                                            \
     }
@@ -3787,8 +3841,8 @@ hipError_t pt_launch_propagate(const PtArgs &a0, int jump, hipStream_t s)
     switch (shape) {
     case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
     case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
-    case PT_SHAPE_UP_EE_B: PT_RT_SWITCH(a.r, L_UEB); break;
-    default: PT_RT_SWITCH(a.r, L_UA); break;
+    case PT_SHAPE_UP_EE_B: PT_RTU_SWITCH(a, L_UEB); break;
+    default: PT_RTU_SWITCH(a, L_UA); break;
     }
 #undef L_PIX
 #undef L_PEE
@@ -3815,8 +3869,8 @@ hipError_t pt_launch_scatter(const PtArgs &a0, hipStream_t s)
     switch (shape) {
     case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
     case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
-    case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
-    default: PT_RT_SWITCH(a.r, L_UA); break;
+    case PT_SHAPE_UP_EE_A: PT_RTU_SWITCH(a, L_UEA); break;
+    default: PT_RTU_SWITCH(a, L_UA); break;
     }
 #undef L_PIX
 #undef L_PEE
@@ -3870,8 +3924,8 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
         case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
         case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
         case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
-        case PT_SHAPE_UP_A: PT_RT_SWITCH(a.r, L_UA); break;
-        case PT_SHAPE_UP_EE_B: PT_RT_SWITCH(a.r, L_UEB); break;
+        case PT_SHAPE_UP_A: PT_RTU_SWITCH(a, L_UA); break;
+        case PT_SHAPE_UP_EE_B: PT_RTU_SWITCH(a, L_UEB); break;
         case PT_SHAPE_TAP:
             if (a.r == 2) hipLaunchKernelGGL((tk ? km_propagate_tp<2, true> : km_propagate_tp<2, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap);
             else hipLaunchKernelGGL((tk ? km_propagate_tp<3, true> : km_propagate_tp<3, false>), g, bu, 0, s, (ConstArgs *)t.args, t.prefix, t.tiles_x, t.n, t.xcd_remap);
@@ -3896,8 +3950,8 @@ hipError_t pt_launch_multi(int op, int shape, const PtArgs &a, const PtJobTable 
         switch (shape) {
         case PT_SHAPE_PIXEL: PT_RTX_SWITCH(a, L_PIX); break;
         case PT_SHAPE_PIXEL_EE: PT_RT_SWITCH(a.r, L_PEE); break;
-        case PT_SHAPE_UP_EE_A: PT_RT_SWITCH(a.r, L_UEA); break;
-        case PT_SHAPE_UP_A: PT_RT_SWITCH(a.r, L_UA); break;
+        case PT_SHAPE_UP_EE_A: PT_RTU_SWITCH(a, L_UEA); break;
+        case PT_SHAPE_UP_A: PT_RTU_SWITCH(a, L_UA); break;
         case PT_SHAPE_UP_EE_B: PT_RT_SWITCH(a.r, L_UEB); break;
         default: PT_RT_SWITCH(a.r, L_UB); break;
         }

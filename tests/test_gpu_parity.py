@@ -185,6 +185,21 @@ def test_lab_colour_space_operator_sequence(early_exit, unit_parallel, tap):
     assert_same_states(ga, oa)
 
 
+@pytest.mark.parametrize("early_exit,unit_parallel", [(1, 1), (0, 1), (0, 0)])
+@pytest.mark.parametrize("r", [2, 3])
+def test_extended_terms_in_unit_kernels(r, early_exit, unit_parallel):
+    """Coherence, LAB colour space and object IDs together, evaluated by the
+    unit-parallel propagate and scatter (their extended variants at a fixed
+    radius, with and without early rejection) and by the per-pixel kernels,
+    against the oracle (texSynth_kernel.cu:497-504,575-606,642-646)."""
+    st, frs = _with_ids(W.style_pair(56, 48, seed=61, guides=True),
+                        [W.still_frame(60, 44, seed=62, guides=True)], 63)
+    ov = dict(residual_window_size=r, levels=3, coherence_weight=0.3, colorspace=1, style_obj_id_weight=0.6,
+              orientation_enabled=1, dist_trans_weight=0.5)
+    ga, oa = _both(st, frs, ov, _still_ops(3), early_exit, unit_parallel)
+    assert_same_states(ga, oa)
+
+
 def test_lab_with_coherence_and_temporal_terms():
     st = W.style_pair(48, 48, seed=21, guides=True)
     frames = W.animation(40, 32, 3, seed=22)
