@@ -54,6 +54,8 @@ struct PtArgs {
     int cull, transp, orient_en, direction;
     float tdi, tdo, tcw, hyst_w, max_dist, jitter;
     float coh_w, coh_ratio, coh_angle; /* coherence term: weight (0 = off), ratio and angle ranges */
+    float coh_inv_ratio, coh_z;         /* its constants: 1 / ratio range, log(ratio range)^2 */
+    double coh_lim;                     /* and angle range * pi */
     int lab;           /* colour distances in the LAB space (colorspace 1) */
     float obj_w;       /* object-ID term weight (0 = off)              */
     int lum_live;      /* luminance-shift planes maintained (some shift may be nonzero) */

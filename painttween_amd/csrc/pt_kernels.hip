@@ -199,7 +199,7 @@ __device__ __forceinline__ float ssd_alpha(float4 a, float4 b, float alpha)
 
 /* RGBtoLAB of a colour (texSynth_util.h:220-249), alpha unchanged; the
  * fast powers give the plain conversion's floats bit for bit */
-__device__ __forceinline__ float4 lab4(float4 c)
+__device__ __noinline__ float4 lab4(float4 c)
 {
     float l, a, b;
     pt_rgb_to_lab_fast(c.x, c.y, c.z, &l, &a, &b);
@@ -445,8 +445,11 @@ __device__ __forceinline__ void stage_window(const PtArgs &A, float ax, float ay
 
 /* timeDerivativeTerms (texSynth_kernel.cu:438-469); EXT: the kernel
  * variant with the extended terms, where the LAB colour space converts the
- * previous frame's colours and exemplar samples (:454-459) */
-template <bool EXT = false>
+ * previous frame's colours and exemplar samples (:454-459).  Time-table
+ * entries are stored converted when that colour space is on (only the
+ * extended variants run then); PREV_LAB: the caller's previous-frame colours
+ * are converted already. */
+template <bool EXT = false, bool PREV_LAB = false>
 __device__ float time_terms_v(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k, float4 prev, float pb0x, float pb0y,
                               float pb1x, float pb1y, float4 base, float4 tb, float4 canvas, float4 to,
                               float4 prev_base, float4 prev_out, int x, int y, int i, int j, int plb = 0)
@@ -462,12 +465,14 @@ __device__ float time_terms_v(const PtArgs &A, __amdgpu_buffer_rsrc_t er, int k,
         float ptx, pty;
         new_coord((float)i, (float)j, prev.x, prev.y, pb0x, pb0y, pb1x, pb1y, ptx, pty);
         ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo, plb);
+        if (EXT && A.lab) {
+            peb = lab4(peb);
+            peo = lab4(peo);
+        }
     }
-    if (EXT && A.lab) {
+    if (EXT && !PREV_LAB && A.lab) {
         prev_base = lab4(prev_base);
         prev_out = lab4(prev_out);
-        peb = lab4(peb);
-        peo = lab4(peo);
     }
     float r = ssd4(sub4(base, prev_base), sub4(tb, peb)) * A.tdi;
     r = fmaf(ssd4(sub4(canvas, prev_out), sub4(to, peo)), A.tdo, r);
@@ -487,6 +492,14 @@ __device__ __forceinline__ float time_terms_b(const PtArgs &A, __amdgpu_buffer_r
                              i, j, plb);
 }
 
+/* pt_coherence_term_k as a called function: the tap loops that evaluate it
+ * keep their registers for the gathers */
+__device__ __noinline__ float coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j,
+                                             int level, float inv_ratio, float ratio, float z, double lim)
+{
+    return pt_coherence_term_k(cx, cy, tcx, tcy, tx, ty, i, j, level, inv_ratio, ratio, z, lim);
+}
+
 /* The coherence term of tap (i, j) (texSynth_kernel.cu:642-646): against
  * the current offset of pixel (x+i, y+j); (tcx, tcy) is the candidate's
  * centre and (tx, ty) the wrapped tap position. */
@@ -495,7 +508,9 @@ __device__ __forceinline__ float coherence_add(const PtArgs &A, float res, int x
 {
     if (A.coh_w > 0.f) {
         float4 n = A.base.xyth[(y + j) * A.W + (x + i)];
-        res = res + pt_coherence_term(n.x, n.y, tcx, tcy, tx, ty, i, j, A.level, A.coh_ratio, A.coh_angle) * A.coh_w;
+        res = res + coherence_term(n.x, n.y, tcx, tcy, tx, ty, i, j, A.level, A.coh_inv_ratio, A.coh_ratio, A.coh_z,
+                                   A.coh_lim) *
+                        A.coh_w;
     }
     return res;
 }
@@ -1319,9 +1334,9 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
                     const bool k0 = T.sdir == 0;
                     if ((k0 ? (fwd_dir && cpv) : (bwd_dir && cnv)) && T.av[si]) {
                         const float *b4 = tb4 + (k0 ? 0 : 4);
-                        res = res + time_terms_v<Rt<RT>::EXT>(A, er, T.sdir, k0 ? pf : pb, b4[0], b4[1], b4[2],
-                                                              b4[3], base, tb, canvas, to, T.ab[si], T.ao[si], x, y,
-                                                              i, j);
+                        res = res + time_terms_v<Rt<RT>::EXT, true>(A, er, T.sdir, k0 ? pf : pb, b4[0], b4[1],
+                                                                    b4[2], b4[3], base, tb, canvas, to, T.ab[si],
+                                                                    T.ao[si], x, y, i, j);
                     }
                 }
             } else if (any_time_w) {
@@ -1791,6 +1806,7 @@ __device__ __forceinline__ int stage_adv_up(const PtArgs &A, int x0, int y0, flo
     const PtAdv &d = A.adv[k];
     const int r = PT_RAD(RT, A.r);
     const int tw = TX + 2 * r, th = TY + 2 * r;
+    const bool lab = Rt<RT>::EXT && A.lab; /* staged converted (time_terms_v's PREV_LAB) */
     for (int e = threadIdx.x; e < tw * th; e += PT_UP_THREADS) {
         int tx = clampi(x0 - r + e % tw, 0, A.lw - 1);
         int ty = clampi(y0 - r + e / tw, 0, A.lh - 1);
@@ -1798,8 +1814,13 @@ __device__ __forceinline__ int stage_adv_up(const PtArgs &A, int x0, int y0, flo
         bool v = d.xyth[o].w > 0;
         sav[e] = v ? 1 : 0;
         if (v) {
-            sab[e] = d.base[o];
-            sao[e] = d.out[o];
+            float4 b = d.base[o], u = d.out[o];
+            if (lab) {
+                b = lab4(b);
+                u = lab4(u);
+            }
+            sab[e] = b;
+            sao[e] = u;
         }
     }
     return k;
@@ -2045,7 +2066,7 @@ template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false, bool AR = f
 __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, int jump, void *arena = nullptr)
 {
     constexpr int P = TX * TY, U = 4 * P;
-    static_assert(!XL || RT > 0, "exemplar windows need a fixed radius");
+    static_assert(!XL || Rt<RT>::R > 0, "exemplar windows need a fixed radius");
     using Lds = PropUpLds<RT, TX, TY, TM, XL>;
     constexpr int N = Lds::N;
     PT_LDS_VAR(UpTime<TM ? P : 1>, s_tm)
@@ -2602,6 +2623,10 @@ __global__ __launch_bounds__(PT_RX *PT_RY) void k_time_table(PtArgs A, int k, fl
             new_coord((float)i, (float)j, prev.x, prev.y, b0x, b0y, b1x, b1y, ptx, pty);
             float4 peb, peo;
             ex_pair(er, A.ew, A.eh, ptx, pty, peb, peo);
+            if (A.lab) { /* read by the extended variants only */
+                peb = lab4(peb);
+                peo = lab4(peo);
+            }
             float4 *e = tt + time_table_index(A, x, y, i, j);
             e[0] = peb;
             e[1] = peo;

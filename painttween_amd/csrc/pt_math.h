@@ -260,17 +260,18 @@ PT_HD float pt_acosf(float x)
  * min/division in double (M_PI is a double) and stores the result in a
  * float; the same is done here.  Both sides include this one definition.
  */
-PT_HD float pt_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
-                              float ratio_range, float angle_range)
+/* pt_coherence_term with its launch constants formed by the caller:
+ * inv_ratio = 1 / ratio_range, z = log(ratio_range)^2 (pt_logf, squared in
+ * float) and lim = angle_range * pi in double */
+PT_HD float pt_coherence_term_k(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
+                                float inv_ratio, float ratio_range, float z, double lim)
 {
     float dx = cx - tcx, dy = cy - tcy;
     float lvl = (float)(1 << level);
     float dist_ex = (dx * dx + dy * dy) * lvl;
     float dist_im = (float)(i * i + j * j) * lvl;
-    float ratio = fmaxf(1.0f / ratio_range, fminf(dist_ex / dist_im, ratio_range));
+    float ratio = fmaxf(inv_ratio, fminf(dist_ex / dist_im, ratio_range));
     float lr = pt_logf(ratio);
-    float lz = pt_logf(ratio_range);
-    float z = lz * lz;
     float e_ratio = (lr * lr) / z;
     float ux = tx - tcx, uy = ty - tcy;
     float lt = sqrtf(ux * ux + uy * uy);
@@ -281,9 +282,16 @@ PT_HD float pt_coherence_term(float cx, float cy, float tcx, float tcy, float tx
     float vy = (lc > 1e-4f) ? dy * (1.0f / lc) : 0.0f;
     float d = ux * vx + uy * vy;
     float angle = fmaxf(-1.0f, fminf(pt_acosf(d), 1.0f));
-    double lim = (double)angle_range * 3.14159265358979323846;
     float e_angle = (float)(fmin((double)angle, lim) / lim);
     return 1.0f - (1.0f - e_ratio) * (1.0f - e_angle);
+}
+
+PT_HD float pt_coherence_term(float cx, float cy, float tcx, float tcy, float tx, float ty, int i, int j, int level,
+                              float ratio_range, float angle_range)
+{
+    float lz = pt_logf(ratio_range);
+    return pt_coherence_term_k(cx, cy, tcx, tcy, tx, ty, i, j, level, 1.0f / ratio_range, ratio_range, lz * lz,
+                               (double)angle_range * 3.14159265358979323846);
 }
 
 /* ln 2 = PT_LN2_HI_D + PT_LN2_LO_D, the head with 32 significant bits */

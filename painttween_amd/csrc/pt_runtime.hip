@@ -14,6 +14,7 @@
  * whole-buffer copy.
  */
 #include "pt_internal.h"
+#include "pt_math.h"
 #include "pt_plan.h"
 
 #include <math.h>
@@ -129,6 +130,7 @@ struct pt_ctx {
     size_t tt_cap[2] = {0, 0}; /* float4 elements */
     bool tt_valid[2] = {false, false};
     int tt_level[2] = {-1, -1}, tt_r[2] = {-1, -1};
+    int tt_lab[2] = {0, 0}; /* entries stored in LAB (the colour space of the launch that built them) */
     int64_t tt_max_taps = 81; /* tables for patches of at most this many taps (0: never) */
     /* luminance-shift planes are maintained from the first parameters with
      * lum_shift_enable (or loaded nonzero shifts) until the working buffers
@@ -1388,6 +1390,12 @@ static PtArgs make_args(const pt_ctx *c, int level)
     a.coh_w = p.coherence_weight > 0.f ? p.coherence_weight : 0.f;
     a.coh_ratio = p.coherence_ratio_range;
     a.coh_angle = p.coherence_angle_range;
+    {
+        float lz = pt_logf(a.coh_ratio);
+        a.coh_inv_ratio = 1.0f / a.coh_ratio;
+        a.coh_z = lz * lz;
+        a.coh_lim = (double)a.coh_angle * 3.14159265358979323846;
+    }
     a.lab = p.colorspace == 1 ? 1 : 0;
     a.obj_w = p.style_obj_id_weight > 0.f ? p.style_obj_id_weight : 0.f;
     a.lum_live = c->lum_live ? 1 : 0;
@@ -1531,7 +1539,7 @@ static int time_tables(pt_ctx *c, int level, PtArgs &a)
             RC_CHECK(dalloc(c, &c->tt[k], cap));
             c->tt_cap[k] = cap;
         }
-        if (!c->tt_valid[k] || c->tt_level[k] != level || c->tt_r[k] != a.r) {
+        if (!c->tt_valid[k] || c->tt_level[k] != level || c->tt_r[k] != a.r || c->tt_lab[k] != a.lab) {
             PtArgs b = a;
             float4 *t = c->tt[k];
             const int dk = k;
@@ -1539,6 +1547,7 @@ static int time_tables(pt_ctx *c, int level, PtArgs &a)
             c->tt_valid[k] = true;
             c->tt_level[k] = level;
             c->tt_r[k] = a.r;
+            c->tt_lab[k] = a.lab;
         }
         a.tt[k] = c->tt[k];
     }

@@ -200,13 +200,29 @@ def test_extended_terms_in_unit_kernels(r, early_exit, unit_parallel):
     assert_same_states(ga, oa)
 
 
-def test_lab_with_coherence_and_temporal_terms():
+@pytest.mark.parametrize("hyst,tt", [(0.3, None), (0.0, None), (0.3, 0), (0.0, 0)])
+def test_lab_with_coherence_and_temporal_terms(hyst, tt):
+    """LAB and coherence with the temporal terms: time tables stored in LAB
+    (default) or samples gathered per candidate (tt=0); previous-frame colours
+    staged converted by the unit kernels; hysteresis 0 lets the early
+    rejection shapes run."""
     st = W.style_pair(48, 48, seed=21, guides=True)
     frames = W.animation(40, 32, 3, seed=22)
-    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=0.3,
+    ov = dict(residual_window_size=2, levels=2, temporal_coherence_weight=0.5, hysteresis_weight=hyst,
               coherence_weight=0.3, colorspace=1)
-    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, "merge_random"))
+    ga, oa = _both(st, frames, ov, _anim_ops(2, 3, "merge_random"), time_table=tt)
     assert_same_states(ga, oa)
+
+
+def test_extended_terms_bidirectional_passes():
+    """FBMO fixup passes (both neighbours advected: bidirectional temporal
+    terms, both time tables) with LAB, coherence and object IDs."""
+    st, frames = _with_ids(W.style_pair(48, 48, seed=71, guides=True), W.animation(40, 32, 3, seed=72), 73)
+    kw = dict(residual_window_size=3, levels=2, temporal_coherence_weight=0.5, coherence_weight=0.3, colorspace=1,
+              style_obj_id_weight=0.6)
+    with Context.gpu(0) as g, OracleContext() as o:
+        _cmp_synth(g.synthesize(g.default_params(**kw), frames, st, scheme=_ffi.PT_S_FBMO, iterations=2),
+                   o.synthesize(o.default_params(**kw), frames, st, scheme=_ffi.PT_S_FBMO, iterations=2))
 
 
 @pytest.mark.parametrize("tap", [0, 1024])
