@@ -12,8 +12,10 @@ Beside it, on the same GPU in the same run:
              multi-job launch sequence, shapes autotuned, replayed as a HIP
              graph, then run as one persistent launch; compared per job,
              bit for bit, with the plain path;
-  extended_terms -- a 512^2 still with the default patch distance and with
-             coherence, LAB colour space and object IDs on;
+  extended_terms -- a 512^2 still with the default patch distance, with
+             object IDs, coherence or the LAB colour space alone and with
+             all three; a temporal sequence with and without LAB and
+             coherence;
   still256 -- the CPU-runnable still (256^2, 5x5 patches, 3 levels, 4
              iterations, no orientation): GPU frame time, the oracle on all
              host cores (median of 5) and on 1 core, offsets compared;
@@ -219,9 +221,12 @@ def run_still256(ctx, inputs, cpu_runs, cpu_threads):
 def run_ext_terms(ctx, steps=5):
     """A 512^2 still (exemplar 512^2, 7x7 patches, 4 levels, 4 iterations,
     orientation and distance-transform guides) with the default patch
-    distance, then with the coherence term, the LAB colour space and the
-    object-ID term on (the per-pixel kernels that evaluate them); GPU time per
-    frame of each (median of `steps`), inputs resident on the device."""
+    distance, with each extended term alone (object IDs 0.6, coherence 0.3,
+    the LAB colour space) and with all three; GPU time per frame of each
+    (median of `steps`), inputs resident on the device.  Then a temporal
+    sequence (6 frames 256x192, exemplar 256^2, 3 levels, 3 iterations,
+    temporal coherence 0.5) with and without LAB and coherence, timed through
+    Context.synthesize (host arrays in and out, uploads included)."""
     import torch
     from painttween_amd import _ffi
     from painttween_amd import workloads as W
@@ -229,7 +234,9 @@ def run_ext_terms(ctx, steps=5):
     style = dict(W.style_pair(512, 512, seed=6, guides=True), object_ids=W.object_ids(6, 30, 512, 512))
     frame = dict(W.still_frame(512, 512, seed=7, guides=True), object_id=W.object_ids(7, 31, 512, 512))
     base = dict(residual_window_size=3, levels=4, dist_trans_weight=0.5, orientation_enabled=1)
-    ext = dict(base, coherence_weight=0.3, colorspace=1, style_obj_id_weight=0.6)
+    cases = (("default", base), ("object_ids", dict(base, style_obj_id_weight=0.6)),
+             ("coherence", dict(base, coherence_weight=0.3)), ("lab", dict(base, colorspace=1)),
+             ("extended", dict(base, coherence_weight=0.3, colorspace=1, style_obj_id_weight=0.6)))
     img = DeviceImages(0)
     fg = (_ffi.FrameGuides * 1)()
     fg[0] = img.frame(frame)
@@ -237,8 +244,8 @@ def run_ext_terms(ctx, steps=5):
     out = DeviceOutputs(512, 512, 1)
     o, c, r = out.pointers()
     res = {"image": [512, 512], "exemplar": [512, 512], "patch": 49, "levels": 4, "iterations": 4,
-           "terms": "coherence 0.3, LAB colour space, object IDs 0.6"}
-    for name, kw in (("default", base), ("extended", ext)):
+           "terms": "extended: coherence 0.3, LAB colour space, object IDs 0.6"}
+    for name, kw in cases:
         params = ctx.default_params(**kw)
 
         def run():
@@ -258,6 +265,19 @@ def run_ext_terms(ctx, steps=5):
         pdes = (ctx.pde_count() - p0) // steps
         res[name] = {"ms_per_frame": ms, "pde_per_frame": int(pdes), "pde_per_s": pdes / (ms * 1e-3)}
     res["extended_vs_default"] = res["extended"]["ms_per_frame"] / res["default"]["ms_per_frame"]
+    frames = W.animation(256, 192, 6, seed=3)
+    st = W.style_pair(256, 256, seed=4, guides=True)
+    tkw = dict(residual_window_size=3, levels=3, temporal_coherence_weight=0.5)
+    tres = {"frames": 6, "image": [256, 192], "levels": 3, "iterations": 3, "timing": "Context.synthesize, host I/O"}
+    for name, kw in (("default", tkw), ("lab_coherence", dict(tkw, coherence_weight=0.3, colorspace=1))):
+        params = ctx.default_params(**kw)
+        ctx.synthesize(params, frames, st, iterations=3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.synthesize(params, frames, st, iterations=3)
+        torch.cuda.synchronize()
+        tres[name + "_ms"] = (time.perf_counter() - t0) * 1e3
+    res["temporal"] = tres
     return res
 
 
@@ -802,9 +822,12 @@ def main():
         print("still256: %.2f ms on the GPU" % line["still256"]["gpu_ms"], file=sys.stderr, flush=True)
     if not args.no_ext:
         line["extended_terms"] = run_ext_terms(ctx)
-        print("extended terms: %.1f ms per 512^2 frame against %.1f ms with the default terms" % (
-            line["extended_terms"]["extended"]["ms_per_frame"], line["extended_terms"]["default"]["ms_per_frame"]),
-              file=sys.stderr, flush=True)
+        et = line["extended_terms"]
+        print("extended terms (ms per 512^2 frame): default %.1f, object IDs %.1f, coherence %.1f, LAB %.1f, all %.1f;"
+              " temporal 6 frames %.0f ms, with LAB and coherence %.0f ms" % (
+                  et["default"]["ms_per_frame"], et["object_ids"]["ms_per_frame"], et["coherence"]["ms_per_frame"],
+                  et["lab"]["ms_per_frame"], et["extended"]["ms_per_frame"], et["temporal"]["default_ms"],
+                  et["temporal"]["lab_coherence_ms"]), file=sys.stderr, flush=True)
     if anim_in is not None:
         line["animation"] = run_animation(ctx, anim_in, args.anim_levels, args.anim_iterations,
                                           0 if args.no_cpu else min(os.cpu_count() or 1, 16))
