@@ -2059,7 +2059,7 @@ template <int RT, int TX, int TY, bool TM, bool XL> struct PropUpLds {
     float s_r[U];
     float s_cx[U], s_cy[U], s_b0x[U], s_b0y[U], s_b1x[U], s_b1y[U], s_scy[U];
     unsigned short s_list[U];
-    int s_n, s_next;
+    int s_n, s_nb, s_next;
 };
 
 template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false, bool AR = false>
@@ -2092,6 +2092,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     PT_LDS_ARRAY(float, s_scy, U)
     PT_LDS_ARRAY(unsigned short, s_list, U)
     PT_LDS_VAR(int, s_n)
+    PT_LDS_VAR(int, s_nb)
     PT_LDS_VAR(int, s_next)
     const int x0 = bx * TX, y0 = by * TY;
     const int r = PT_RAD(RT, A.r);
@@ -2117,6 +2118,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     }
     if (threadIdx.x == 0) {
         s_n = 0;
+        s_nb = 0;
         s_next = PT_UP_THREADS;
     }
     __syncthreads();
@@ -2147,11 +2149,22 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         s_b1x[u] = b1x;
         s_b1y[u] = b1y;
         s_scy[u] = csc.y;
-        s_list[atomicAdd(&s_n, 1)] = (unsigned short)u;
+        if constexpr (XL) {
+            /* units inside the window from the front of the list, the others
+             * from its end, so that a wave mostly takes units of one kind
+             * and issues one kind of gather per tap */
+            if (s_exl[u]) s_list[atomicAdd(&s_n, 1)] = (unsigned short)u;
+            else s_list[U - 1 - atomicAdd(&s_nb, 1)] = (unsigned short)u;
+        } else {
+            s_list[atomicAdd(&s_n, 1)] = (unsigned short)u;
+        }
     }
     __syncthreads();
-    /* phase 2: evaluation of the listed units */
-    const int n = s_n;
+    /* phase 2: evaluation of the listed units (list position of unit index
+     * idx: the window units, then the others from the end backwards) */
+    const int na = s_n;
+    const int n = XL ? na + s_nb : na;
+    auto at = [&](int idx) { return (!XL || idx < na) ? idx : U - 1 - (idx - na); };
     __amdgpu_buffer_rsrc_t er = ex_rsrc(A);
     Tile T;
     T.col = scol;
@@ -2167,7 +2180,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     T.av = s_av;
     T.sdir = sdir;
     auto setup = [&](int idx, Cand &c, TimeCand &tc, int &x, int &y, int &p) -> bool {
-        int u = s_list[idx];
+        int u = s_list[at(idx)];
         p = u >> 2;
         x = x0 + p % TX;
         y = y0 + p / TX;
@@ -2181,7 +2194,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         if constexpr (TM) up_time_load<P>(s_tm, p, tc);
         return true;
     };
-    auto done = [&](int idx, float v) { s_r[s_list[idx]] = v; };
+    auto done = [&](int idx, float v) { s_r[s_list[at(idx)]] = v; };
     if (EE) {
         up_queue_ee<RT, TX, (RT > 0) && !TM, TM, XL>(A, T, er, threadIdx.x, n, &s_next, s_best, s_sw, setup, done);
     } else {
