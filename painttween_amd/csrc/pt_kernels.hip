@@ -770,7 +770,7 @@ template <int RT, bool XL> struct ResLds {
     float4 sexw[XL ? PT_XW_RES * PT_XW_RES * 2 : 1];
     unsigned char sflag[TileSize<RT>::N];
     unsigned short slist[PT_BX * PT_BY];
-    int scount;
+    int scount, scount_b;
 };
 
 template <int RT, bool TK, bool XL, bool AR = false>
@@ -783,6 +783,7 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
     PT_LDS_ARRAY(unsigned char, sflag, TileSize<RT>::N)
     PT_LDS_ARRAY(unsigned short, slist, PT_BX * PT_BY)
     PT_LDS_VAR(int, scount)
+    PT_LDS_VAR(int, scount_b)
     const int r = PT_RAD(RT, A.r);
     const bool inc = A.res_incremental != 0;
     if (inc) {
@@ -793,7 +794,10 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
             int ty = clampi(y0 + k / tw, 0, A.lh - 1);
             sflag[k] = A.changed[ty * A.W + tx];
         }
-        if (threadIdx.x == 0) scount = 0;
+        if (threadIdx.x == 0) {
+            scount = 0;
+            scount_b = 0;
+        }
     }
     int x, y;
     bool valid = pixel_of(A, bx, by, x, y);
@@ -828,6 +832,14 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
         A.res_base[o] = calc_residual<RT, TK, XL>(A, T, x, y, t, tsc, (RT < 0 && A.layered) ? A.base.la[o] : 0);
         return;
     }
+    if constexpr (XL) {
+        /* the window's origin (stage_window's), for the fit test below */
+        int ax = min(bx * PT_BX + PT_BX / 2, A.lw - 1), ay = min(by * PT_BY + PT_BY / 2, A.lh - 1);
+        float4 an = A.base.xyth[ay * A.W + ax];
+        T.wx = clampi((int)floorf(an.x) - PT_XW_RES / 2, 0, A.ew - PT_XW_RES);
+        T.wy = clampi((int)floorf(an.y) - PT_XW_RES / 2, 0, A.eh - PT_XW_RES);
+        T.ww = PT_XW_RES;
+    }
     if (valid) {
         /* clamped halo entries repeat edge flags: conservative */
         bool need = false;
@@ -837,14 +849,25 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
                     need = true;
                     break;
                 }
-        if (need) slist[atomicAdd(&scount, 1)] = (unsigned short)threadIdx.x;
+        if (need) {
+            /* XL: pixels whose patch lies in the window first, the others
+             * from the end, so that a wave mostly issues one kind of gather */
+            bool front = true;
+            if constexpr (XL) {
+                int o = y * A.W + x;
+                front = !image_culled(A, x, y) && fits(A.base.xyth[o], A.base.sc[o]);
+            }
+            if (front) slist[atomicAdd(&scount, 1)] = (unsigned short)threadIdx.x;
+            else slist[PT_BX * PT_BY - 1 - atomicAdd(&scount_b, 1)] = (unsigned short)threadIdx.x;
+        }
     }
     __syncthreads();
-    const int n = scount;
+    const int na = scount;
+    const int n = XL ? na + scount_b : na;
     if (XL && n > 0) window();
     const int lane = threadIdx.x;
     if (lane >= n) return;
-    const int p = slist[lane];
+    const int p = slist[(!XL || lane < na) ? lane : PT_BX * PT_BY - 1 - (lane - na)];
     x = bx * PT_BX + p % PT_BX;
     y = by * PT_BY + p / PT_BX;
     T.lx = p % PT_BX;
@@ -856,7 +879,7 @@ __device__ __forceinline__ void b_update_residual(const PtArgs &A, int bx, int b
     }
     float4 t = A.base.xyth[o];
     float2 tsc = A.base.sc[o];
-    if (XL) T.exl = fits(t, tsc);
+    if (XL) T.exl = lane < na; /* the pixels listed in front fit the window */
     A.res_base[o] = calc_residual<RT, TK, XL>(A, T, x, y, t, tsc, (RT < 0 && A.layered) ? A.base.la[o] : 0);
 }
 
