@@ -21,13 +21,15 @@ Beside it, on the same GPU in the same run:
              host cores (median of 5) and on 1 core, offsets compared;
   served  -- 32 streamed sequences (256..1920 x 256..1080, exemplars
              512..1024, 7x7 patches, 4 levels, 3 iterations, temporal
-             coherence 0.5) under Poisson frame arrivals, continuous
-             batching in the server loop, p50/p99 latency and frames/s
-             against the plain per-frame path.
+             coherence 0.5) under Poisson frame arrivals, the server loop
+             with 2 frames in flight, p50/p99 latency and frames/s against
+             the plain per-frame path (a FIFO over service times measured
+             before and after the rates).
 Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-levels L]
                   [--no-batch] [--no-serve] [--serve-rates 4,20,100,400]
+                  [--serve-in-flight 2]
 """
 import argparse
 import ctypes as C
@@ -527,9 +529,15 @@ def served_alone_1024(params, iterations, img, frames=4):
             "served_mean_ms": float(np.mean(lat)) if lat else None, "plain_mean_ms": float(np.mean(plain))}
 
 
-def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0, cpu_pde_rate=None):
+def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0, cpu_pde_rate=None, in_flight=2):
     """The serving loop under Poisson arrivals against the plain per-frame
-    path (one context per sequence, frames one at a time in arrival order)."""
+    path (one context per sequence, frames one at a time in arrival order).
+    The plain path is a FIFO model over each frame's measured service time:
+    the mean of a timed round before the rates and one after them, so that a
+    drift of the GPU's speed over the leg weighs on both sides alike.  The
+    server keeps `in_flight` frames in flight (measured on this mix: 1 and 2
+    give the FIFO's latencies, 32 a higher p50; frames of 256^2 and more fill
+    most of the GPU alone)."""
     import torch
     from painttween_amd import Context
     from painttween_amd.forms import DeviceImages, DeviceOutputs, Server, next_frame
@@ -548,29 +556,28 @@ def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0, cpu_pde_rate=None):
     # plain per-frame path
     pctx = [Context.gpu(0) for _ in range(N)]
     first_round = {}
-    service = []
-    pde0 = None
-    for rnd in range(2):          # round 0 warms allocations, round 1 is timed
-        if rnd == 1:
-            pde0 = sum(c.pde_count() for c in pctx)
+
+    def plain_round(keep=None):
+        times = []
         for (k, f) in order:
             d = dev[k]
             out = DeviceOutputs(d["w"], d["h"], 1)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             next_frame(pctx[k], params, iterations, d["frames"][f], d["style"], d["sw"], d["sh"], out)
-            dt = time.perf_counter() - t0
-            if rnd == 0:
-                first_round[(k, f)] = out
-            else:
-                service.append(dt)
+            times.append(time.perf_counter() - t0)
+            if keep is not None:
+                keep[(k, f)] = out
+        return times
+
+    plain_round(first_round)      # round 0 warms allocations (and is the bit-exactness reference)
+    pde0 = sum(c.pde_count() for c in pctx)
+    service_before = plain_round()
     served_pdes = sum(c.pde_count() for c in pctx) - pde0
-    print("served: plain per-frame path %.1f ms for %d frames" % (sum(service) * 1e3, len(service)),
+    print("served: plain per-frame path %.1f ms for %d frames" % (sum(service_before) * 1e3, len(service_before)),
           file=sys.stderr, flush=True)
-    for c in pctx:
-        c.close()
     # server: one warm round (compared with the plain round 0), then the rates
-    srv = Server(int(budget_gb * (1 << 30)), N)
+    srv = Server(int(budget_gb * (1 << 30)), min(N, in_flight))
     qs = [srv.open_device(params, iterations, d["w"], d["h"], d["style"], d["sw"], d["sh"]) for d in dev]
     tickets = {}
     for (k, f) in order:
@@ -612,6 +619,19 @@ def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0, cpu_pde_rate=None):
                 time.sleep(max(0.0, min(0.002, arrivals[i] - (time.perf_counter() - t0))))
         span = last - t0
         srv.outputs.clear()
+        per_rate.append((lam, arrivals, span, lat))
+        print("served at %g fps offered: %.2f fps, p50 %.0f ms, p99 %.0f ms" % (
+            lam, len(order) / span, np.percentile(lat, 50), np.percentile(lat, 99)), file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    service_after = plain_round()
+    for c in pctx:
+        c.close()
+    service = [(a + b) / 2 for a, b in zip(service_before, service_after)]
+    print("served: plain per-frame path %.1f ms for %d frames after the rates (model: mean %.1f ms)" % (
+        sum(service_after) * 1e3, len(service_after), sum(service) * 1e3), file=sys.stderr, flush=True)
+    rows = []
+    for lam, arrivals, span, lat in per_rate:
         # plain FIFO single server with the measured service times
         c_prev, plat = 0.0, []
         for a, s_ in zip(arrivals, service):
@@ -623,16 +643,17 @@ def run_served(ctx, seqs_in, frames, rates, budget_gb=200.0, cpu_pde_rate=None):
              "plain_fps": len(order) / pspan, "plain_p50_ms": float(np.percentile(plat, 50)),
              "plain_p99_ms": float(np.percentile(plat, 99))}
         r["served_vs_plain_fps"] = r["served_fps"] / r["plain_fps"]
-        per_rate.append(r)
-        print("served at %g fps offered: %.2f fps, p50 %.0f ms, p99 %.0f ms (plain %.2f fps, p99 %.0f ms)"
-              % (lam, r["served_fps"], r["served_p50_ms"], r["served_p99_ms"], r["plain_fps"], r["plain_p99_ms"]),
-              file=sys.stderr, flush=True)
-    st = srv.stats()
-    srv.close()
+        rows.append(r)
+        print("served at %g fps offered: %.2f fps, p50 %.0f ms, p99 %.0f ms (plain %.2f fps, p50 %.0f ms, "
+              "p99 %.0f ms)" % (lam, r["served_fps"], r["served_p50_ms"], r["served_p99_ms"], r["plain_fps"],
+                                r["plain_p50_ms"], r["plain_p99_ms"]), file=sys.stderr, flush=True)
+    per_rate = rows
     alone = served_alone_1024(params, iterations, img)
     px = sum(d["w"] * d["h"] for d in dev)
     return {"sequences": N, "frames_per_sequence_per_round": frames, "pixels_per_round": px * frames,
-            "plain_fps": len(service) / sum(service), "bit_exact_vs_plain": bool(exact), "rates": per_rate,
+            "plain_fps": len(service) / sum(service), "plain_ms_before": sum(service_before) * 1e3,
+            "plain_ms_after": sum(service_after) * 1e3, "in_flight": min(N, in_flight),
+            "bit_exact_vs_plain": bool(exact), "rates": per_rate,
             "budget_gb": budget_gb, "resident_bytes": st["resident_bytes"], "launches": st["launches"],
             "multi_launches": st["multi_launches"], "ticks": st["ticks"],
             "schedule": "per-frame FMBM_RM first pass, 7x7 patches, 4 levels, 3 iterations, temporal coherence 0.5",
@@ -665,6 +686,7 @@ def main():
     ap.add_argument("--serve-seqs", type=int, default=32)
     ap.add_argument("--serve-frames", type=int, default=3)
     ap.add_argument("--serve-rates", default="4,20,100,400")
+    ap.add_argument("--serve-in-flight", type=int, default=2, help="frames the server keeps in flight")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 256 MiB five times (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against known bytes) and exit")
@@ -843,7 +865,8 @@ def main():
     if serve_in is not None:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         anim_cpu = (line.get("animation") or {}).get("cpu_baseline")
-        line["served"] = run_served(ctx, serve_in, args.serve_frames, rates, cpu_pde_rate=anim_cpu)
+        line["served"] = run_served(ctx, serve_in, args.serve_frames, rates, cpu_pde_rate=anim_cpu,
+                                    in_flight=args.serve_in_flight)
     ctx.close()
     print(json.dumps(line), flush=True)
 
