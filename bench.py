@@ -502,8 +502,9 @@ def served_alone_1024(params, iterations, img, frames=4):
     fg = [img.frame(f) for f in fr]
     plain = []
     with Context.gpu(0) as c:
+        # round 1 continues the sequence of round 0, as the server's resident
+        # sequence does: both time the same four temporal frames
         for rnd in range(2):
-            c.lib.pt_sequence_reset(c.handle)
             for f in fg:
                 out = DeviceOutputs(1024, 1024, 1)
                 torch.cuda.synchronize()

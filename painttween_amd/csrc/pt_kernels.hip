@@ -1726,7 +1726,8 @@ __device__ __forceinline__ float image_orient_at(const PtArgs &A, int x, int y)
 
 /* Scatter sample k of pixel (x, y) placed from winner `ow`
  * (texSynth_kernel.cu:909-925); dist is the loop's dist at iteration k. */
-__device__ __forceinline__ float4 scatter_sample(const PtArgs &A, int x, int y, int k, float dist, float4 ow, float io)
+__device__ __forceinline__ void scatter_pos(const PtArgs &A, int x, int y, int k, float dist, float4 ow, float &ex,
+                                            float &ey)
 {
     float rr = 0.0f, theta = 0.0f;
     if (A.rng_mode == PT_RNG_COUNTER) {
@@ -1739,9 +1740,15 @@ __device__ __forceinline__ float4 scatter_sample(const PtArgs &A, int x, int y, 
     float sn, co;
     pt_sincosf(theta, &sn, &co);
     float len = rr * (dmax - dmin) + dmin;
-    float ex = ow.x + co * len;
-    float ey = ow.y + sn * len;
+    ex = ow.x + co * len;
+    ey = ow.y + sn * len;
     clamp_ex(ex, ey, A.ew, A.eh);
+}
+
+__device__ __forceinline__ float4 scatter_sample(const PtArgs &A, int x, int y, int k, float dist, float4 ow, float io)
+{
+    float ex, ey;
+    scatter_pos(A, x, y, k, dist, ow, ex, ey);
     return make_float4(ex, ey, ex_angle_io(A, io, ex, ey), 0.f);
 }
 
@@ -2382,6 +2389,20 @@ __device__ __forceinline__ void b_scatter_up(const PtArgs &A, int bx, int by, in
             if (k < n) {
                 c.t = scatter_sample(A, x, y, k, s_dist[k], s_ow[p], s_iori[p]);
                 c.sc = s_sc[p];
+                if (A.rng_mode == PT_RNG_REFERENCE && k > st) {
+                    /* the truncated reference draw places every sample on the
+                     * +x ray from the winner: samples past the exemplar edge
+                     * all clamp to its last column.  A sample equal to the
+                     * previous one of this round (same winner, same scale)
+                     * has that one's residual bit for bit, so it cannot be the
+                     * first to beat the best: it is not evaluated. */
+                    float px, py;
+                    scatter_pos(A, x, y, k - 1, s_dist[k - 1], s_ow[p], px, py);
+                    if (px == c.t.x && py == c.t.y) {
+                        s_r[p * S + k] = PT_INF;
+                        return false;
+                    }
+                }
             } else {
                 c.t = average_candidate(A, x, y, s_iori[p]);
                 c.sc = one;
