@@ -2701,18 +2701,21 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void k_merge_randomly(PtArgs A, int f
     if (!pixel_of_rows(A, blockIdx.x, blockIdx.y, x, y)) return;
     int o = y * A.W + x;
     const PtAdv &d = A.adv[fwd ? 0 : 1];
-    float4 a = d.xyth[o];
+    /* both sources' offsets are read before the choice (one round of
+     * loads: the launch is latency-bound, every pixel's loads are in bounds) */
+    float4 a = d.xyth[o], b = A.base.xyth[o];
+    float2 asc = d.sc[o], bsc = A.base.sc[o];
     float4 rnd = A.random[point_index(A.rw, A.rh, (float)x, (float)y)];
     float r = fwd ? rnd.x - floorf(rnd.x) : rnd.y - floorf(rnd.y);
     bool rib = (int)ftk_at(A, fwd ? 0 : 1, x, y).w >= 0 && !((int)ftk_at(A, fwd ? 1 : 0, x, y).w >= 0);
     if (rib || ((a.w > 0 || force) && r < weight)) {
         A.work.xyth[o] = a;
-        A.work.sc[o] = d.sc[o];
+        A.work.sc[o] = asc;
         if (A.lum_live) A.work.lum[o] = d.lum[o];
         if (A.layered) A.work.la[o] = d.la[o];
     } else {
-        A.work.xyth[o] = A.base.xyth[o];
-        A.work.sc[o] = A.base.sc[o];
+        A.work.xyth[o] = b;
+        A.work.sc[o] = bsc;
         lum_set(A, o, true);
         if (A.layered) A.work.la[o] = A.base.la[o];
     }
