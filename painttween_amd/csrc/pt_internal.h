@@ -65,6 +65,7 @@ struct PtArgs {
     uint32_t key0, key1, call;
     int time_active;   /* some advected offset may be valid */
     int ee_ok;         /* all energy terms non-negative: exact early rejection is exact */
+    int res_exact;     /* propagate: res_base holds the exact residual of the base offsets */
     int early_exit;    /* default policy: use early rejection on this launch */
     int unit_parallel; /* default policy: candidate units over the lanes of a workgroup */
     /* per-launch operands, so that one multi-job launch can serve jobs whose

@@ -2086,6 +2086,8 @@ template <int RT, int TX, int TY, bool TM, bool XL> struct PropUpLds {
     float4 scol[N], scan[N];
     float s_best[P], s_sw[P], s_ia[P], s_io[P];
     float2 s_sc[P];
+    float4 s_own[P]; /* the pixel's current offset and scale (the cached residual's) */
+    float2 s_osc[P];
     float s_r[U];
     float s_cx[U], s_cy[U], s_b0x[U], s_b0y[U], s_b1x[U], s_b1y[U], s_scy[U];
     unsigned short s_list[U];
@@ -2112,6 +2114,8 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     PT_LDS_ARRAY(float, s_ia, P)
     PT_LDS_ARRAY(float, s_io, P)
     PT_LDS_ARRAY(float2, s_sc, P)
+    PT_LDS_ARRAY(float4, s_own, P)
+    PT_LDS_ARRAY(float2, s_osc, P)
     PT_LDS_ARRAY(float, s_r, U)
     PT_LDS_ARRAY(float, s_cx, U)
     PT_LDS_ARRAY(float, s_cy, U)
@@ -2144,6 +2148,8 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         s_ia[p] = live ? A.color[A.cur][o].w : 0.f;
         s_io[p] = live ? image_orient_at(A, x, y) : 0.f;
         s_sc[p] = live ? scale_at(A, (float)x, (float)y) : make_float2(1.f, 1.f);
+        s_own[p] = live ? A.base.xyth[o] : make_float4(0.f, 0.f, 0.f, 0.f);
+        s_osc[p] = live ? A.base.sc[o] : make_float2(0.f, 0.f);
         if constexpr (TM) up_time_init<P>(A, s_tm, p, x, y, live);
     }
     if (threadIdx.x == 0) {
@@ -2168,6 +2174,14 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         float4 c;
         float2 csc;
         propagate_offset_io(A, tx, ty, (float)x, (float)y, s_io[p], s_sc[p], c, csc);
+        /* while the residual cache is exact (A.res_exact), a candidate equal
+         * to the pixel's current offset (position, angle and scale) has the
+         * cached residual bit for bit, the best it must beat strictly: it is
+         * not evaluated */
+        const float4 own = s_own[p];
+        const float2 osc = s_osc[p];
+        if (A.res_exact && c.x == own.x && c.y == own.y && c.z == own.z && csc.x == osc.x && csc.y == osc.y)
+            continue;
         if (!candidate_testable(A, c, s_ia[p])) continue;
         float b0x, b0y, b1x, b1y;
         basis(c.z, A.su * csc.x, A.sv * csc.y, b0x, b0y, b1x, b1y);

@@ -1613,6 +1613,9 @@ static int propagate(pt_ctx *c, int level, int jump, int key_index = -1)
     RC_CHECK(time_tables(c, level, a));
     a.jump = jump;
     a.key_index = key_index;
+    /* the cache is known to equal a recomputation for the current offsets
+     * (the kernels then skip candidates equal to a pixel's offset) */
+    a.res_exact = c->res_level == level ? 1 : 0;
     int pr = prof_begin(c, PK_PROPAGATE, (uint64_t)a.lw * a.lh, PB_NNF * a.lw * a.lh);
     RC_CHECK(issue_op(c, PT_OP_PROPAGATE, a));
     prof_end(c, pr);
