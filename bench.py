@@ -833,7 +833,10 @@ def main():
                      "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": "profiles/traffic_still1024.md (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
                                        "calibrated by a plain copy)",
-                     "avg_launch_ms": prop_avg_ms, "algorithmic_bytes_per_launch": prop_bytes},
+                     "avg_launch_ms": prop_avg_ms, "algorithmic_bytes_per_launch": prop_bytes,
+                     "binding_unit": "not HBM: the windowed short-jump launches are VALU-issue bound (83% at "
+                                     "level 0), the long jumps bound by the gathers' address path (TA busy 69%); "
+                                     "profiles/counters_propagate_still1024.md"},
         "valu": {"achieved": valu_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": valu_tflops / FP32_PEAK_TFLOPS, "flop_per_tap": FLOP_PER_TAP},
         "kernels": kstats,
