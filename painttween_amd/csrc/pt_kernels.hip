@@ -299,6 +299,17 @@ __device__ __forceinline__ int time_table_index(const PtArgs &A, int x, int y, i
     return (t * (A.lw * A.lh) + y * A.lw + x) * 2;
 }
 
+/* The toroidal wrap of a tap coordinate (texSynth_kernel.cu:590-599): once
+ * down when t >= h, else once up when t < 0.  Written as selects: when
+ * t >= h, t - h is >= 0 exactly (rounding keeps its sign), so the second test
+ * fires only when the first did not, as in the reference's if / else if; a
+ * NaN fails both.  Branch-free, so a wave does not split per tap. */
+__device__ __forceinline__ float wrap_tap(float t, float h)
+{
+    float a = t >= h ? t - h : t;
+    return a < 0.f ? a + h : a;
+}
+
 /* ------------------------------------------------------------------ */
 /* patch distance                                                      */
 /* ------------------------------------------------------------------ */
@@ -587,10 +598,8 @@ __device__ float calc_residual_impl(const PtArgs &A, const Tile &T, int x, int y
             float4 canvas = tcan[j * T.tw];
             float tx = t.x + (ibx + (float)j * b1x);
             float ty = t.y + (iby + (float)j * b1y);
-            if (tx >= hw) tx -= hw;
-            else if (tx < 0) tx += hw;
-            if (ty >= hh) ty -= hh;
-            else if (ty < 0) ty += hh;
+            tx = wrap_tap(tx, hw);
+            ty = wrap_tap(ty, hh);
             float4 tb, to;
             ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to, tlb);
             if (RT < 0 && A.lab) {
@@ -1335,10 +1344,8 @@ __device__ __forceinline__ void row_accum(const PtArgs &A, const Tile &T, __amdg
         float4 canvas = tcan[j * T.tw];
         float tx = c.t.x + (ibx + (float)j * c.b1x);
         float ty = c.t.y + (iby + (float)j * c.b1y);
-        if (tx >= hw) tx -= hw;
-        else if (tx < 0) tx += hw;
-        if (ty >= hh) ty -= hh;
-        else if (ty < 0) ty += hh;
+        tx = wrap_tap(tx, hw);
+        ty = wrap_tap(ty, hh);
         float4 tb, to;
         ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to);
         if (Rt<RT>::EXT && A.lab) {
@@ -1417,10 +1424,8 @@ __device__ __forceinline__ void row_taps(const PtArgs &A, const Tile &T, __amdgp
         float4 canvas = tcan[j * T.tw];
         float tx = c.t.x + (ibx + (float)j * c.b1x);
         float ty = c.t.y + (iby + (float)j * c.b1y);
-        if (tx >= hw) tx -= hw;
-        else if (tx < 0) tx += hw;
-        if (ty >= hh) ty -= hh;
-        else if (ty < 0) ty += hh;
+        tx = wrap_tap(tx, hw);
+        ty = wrap_tap(ty, hh);
         float4 tb, to;
         ex_pair_t<XL>(T, er, ew, eh, tx, ty, tb, to);
         float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
@@ -3181,10 +3186,8 @@ __device__ __forceinline__ float tap_residual(const PtArgs &A, __amdgpu_buffer_r
     const float ibx = (float)i * c.b0x, iby = (float)i * c.b0y;
     float tx = c.t.x + (ibx + (float)j * c.b1x);
     float ty = c.t.y + (iby + (float)j * c.b1y);
-    if (tx >= hw) tx -= hw;
-    else if (tx < 0) tx += hw;
-    if (ty >= hh) ty -= hh;
-    else if (ty < 0) ty += hh;
+    tx = wrap_tap(tx, hw);
+    ty = wrap_tap(ty, hh);
     float4 tb, to;
     ex_pair(er, A.ew, A.eh, tx, ty, tb, to);
     float res = ssd_alpha(base, tb, A.alpha_w) * A.in_w;
