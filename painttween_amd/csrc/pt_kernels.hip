@@ -2076,6 +2076,12 @@ template <int P> __device__ __forceinline__ void up_time_load(const UpTime<P> &S
  * per CU (measured on the 1024^2 still: 20 / 24 / 28 / 32 texels give
  * 120.9 / 119.7 / 122.6 / 123.8 ms per frame). */
 #define PT_XW_PROP 24
+/* the temporal variants' window: 20 texels, so that their LDS (the staged
+ * advected set on top) leaves room for four workgroups per CU */
+#define PT_XW_PROP_TM 20
+template <bool TM> struct PropXw {
+    static constexpr int W = TM ? PT_XW_PROP_TM : PT_XW_PROP;
+};
 
 template <int RT, int TX, int TY, bool TM, bool XL> struct PropUpLds {
     static constexpr int P = TX * TY, U = 4 * P, N = UpTile<RT, TX, TY>::N;
@@ -2083,7 +2089,7 @@ template <int RT, int TX, int TY, bool TM, bool XL> struct PropUpLds {
     /* XL: a window of the interleaved exemplar anchored on the current offset
      * of the tile's middle pixel; units whose every tap footprint lies in it
      * read their taps from LDS (same texels, weights and order) */
-    float4 s_exw[XL ? PT_XW_PROP * PT_XW_PROP * 2 : 1];
+    float4 s_exw[XL ? PropXw<TM>::W * PropXw<TM>::W * 2 : 1];
     unsigned char s_exl[XL ? U : 1];
     /* TM: the advected set of the launch's direction over the tile */
     float4 s_ab[TM ? N : 1], s_ao[TM ? N : 1];
@@ -2091,8 +2097,6 @@ template <int RT, int TX, int TY, bool TM, bool XL> struct PropUpLds {
     float4 scol[N], scan[N];
     float s_best[P], s_sw[P], s_ia[P], s_io[P];
     float2 s_sc[P];
-    float4 s_own[P]; /* the pixel's current offset and scale (the cached residual's) */
-    float2 s_osc[P];
     float s_r[U];
     float s_cx[U], s_cy[U], s_b0x[U], s_b0y[U], s_b1x[U], s_b1y[U], s_scy[U];
     unsigned short s_list[U];
@@ -2107,7 +2111,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     using Lds = PropUpLds<RT, TX, TY, TM, XL>;
     constexpr int N = Lds::N;
     PT_LDS_VAR(UpTime<TM ? P : 1>, s_tm)
-    PT_LDS_ARRAY(float4, s_exw, (XL ? PT_XW_PROP * PT_XW_PROP * 2 : 1))
+    PT_LDS_ARRAY(float4, s_exw, (XL ? PropXw<TM>::W * PropXw<TM>::W * 2 : 1))
     PT_LDS_ARRAY(unsigned char, s_exl, (XL ? U : 1))
     PT_LDS_ARRAY(float4, s_ab, (TM ? N : 1))
     PT_LDS_ARRAY(float4, s_ao, (TM ? N : 1))
@@ -2119,8 +2123,6 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     PT_LDS_ARRAY(float, s_ia, P)
     PT_LDS_ARRAY(float, s_io, P)
     PT_LDS_ARRAY(float2, s_sc, P)
-    PT_LDS_ARRAY(float4, s_own, P)
-    PT_LDS_ARRAY(float2, s_osc, P)
     PT_LDS_ARRAY(float, s_r, U)
     PT_LDS_ARRAY(float, s_cx, U)
     PT_LDS_ARRAY(float, s_cy, U)
@@ -2142,7 +2144,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     if constexpr (XL) {
         int ax = min(x0 + TX / 2, A.lw - 1), ay = min(y0 + TY / 2, A.lh - 1);
         float4 an = A.base.xyth[ay * A.W + ax];
-        stage_window(A, an.x, an.y, PT_XW_PROP, s_exw, wx, wy, threadIdx.x, PT_UP_THREADS);
+        stage_window(A, an.x, an.y, PropXw<TM>::W, s_exw, wx, wy, threadIdx.x, PT_UP_THREADS);
     }
     for (int p = threadIdx.x; p < P; p += PT_UP_THREADS) {
         int x = x0 + p % TX, y = y0 + p / TX;
@@ -2153,8 +2155,6 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
         s_ia[p] = live ? A.color[A.cur][o].w : 0.f;
         s_io[p] = live ? image_orient_at(A, x, y) : 0.f;
         s_sc[p] = live ? scale_at(A, (float)x, (float)y) : make_float2(1.f, 1.f);
-        s_own[p] = live ? A.base.xyth[o] : make_float4(0.f, 0.f, 0.f, 0.f);
-        s_osc[p] = live ? A.base.sc[o] : make_float2(0.f, 0.f);
         if constexpr (TM) up_time_init<P>(A, s_tm, p, x, y, live);
     }
     if (threadIdx.x == 0) {
@@ -2183,14 +2183,16 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
          * to the pixel's current offset (position, angle and scale) has the
          * cached residual bit for bit, the best it must beat strictly: it is
          * not evaluated */
-        const float4 own = s_own[p];
-        const float2 osc = s_osc[p];
-        if (A.res_exact && c.x == own.x && c.y == own.y && c.z == own.z && csc.x == osc.x && csc.y == osc.y)
-            continue;
+        if (A.res_exact) {
+            const int o = y * A.W + x;
+            const float4 own = A.base.xyth[o];
+            const float2 osc = A.base.sc[o];
+            if (c.x == own.x && c.y == own.y && c.z == own.z && csc.x == osc.x && csc.y == osc.y) continue;
+        }
         if (!candidate_testable(A, c, s_ia[p])) continue;
         float b0x, b0y, b1x, b1y;
         basis(c.z, A.su * csc.x, A.sv * csc.y, b0x, b0y, b1x, b1y);
-        if constexpr (XL) s_exl[u] = in_window(A, r, c.x, c.y, b0x, b0y, b1x, b1y, wx, wy, PT_XW_PROP) ? 1 : 0;
+        if constexpr (XL) s_exl[u] = in_window(A, r, c.x, c.y, b0x, b0y, b1x, b1y, wx, wy, PropXw<TM>::W) ? 1 : 0;
         s_cx[u] = c.x;
         s_cy[u] = c.y;
         s_b0x[u] = b0x;
@@ -2222,7 +2224,7 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
     T.exw = s_exw;
     T.wx = wx;
     T.wy = wy;
-    T.ww = PT_XW_PROP;
+    T.ww = PropXw<TM>::W;
     T.exl = false;
     T.ab = s_ab;
     T.ao = s_ao;
@@ -2298,7 +2300,8 @@ __device__ __forceinline__ void b_propagate_up(const PtArgs &A, int bx, int by, 
 }
 
 template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false>
-__global__ __launch_bounds__(PT_UP_THREADS) void k_propagate_up(PtArgs A, int jump)
+__global__ __launch_bounds__(PT_UP_THREADS) __attribute__((amdgpu_waves_per_eu(TM ? 4 : 1))) void
+k_propagate_up(PtArgs A, int jump)
 {
     int bx, by;
     xcd_tile(A, bx, by);
