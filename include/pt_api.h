@@ -448,10 +448,10 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
 /* PT_TUNE_PROPAGATE_WINDOW_JUMP (default 8) and
  * PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS (default 200000): on levels of at
  * least that many pixels, the unit-parallel propagate steps with a jump up
- * to this stage a 24x24-texel exemplar window in LDS, anchored on each
- * tile's current offsets, and candidates whose whole patch lies in it read
- * it there (short jumps propose offsets near the current ones); 0 disables
- * it.  Same results. */
+ * to this stage a 24x24-texel exemplar window in LDS (20x20 with temporal
+ * terms), anchored on each tile's current offsets, and candidates whose
+ * whole patch lies in it read it there (short jumps propose offsets near the
+ * current ones); 0 disables it.  Same results. */
 #define PT_TUNE_PROPAGATE_WINDOW_JUMP 10
 #define PT_TUNE_PROPAGATE_WINDOW_MIN_PIXELS 11
 /* PT_TUNE_TEMPORAL_UNIT_ORDER: with temporal terms and time tables, the
