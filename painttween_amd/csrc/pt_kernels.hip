@@ -3547,7 +3547,8 @@ __global__ __launch_bounds__(PT_BX *PT_BY) void km_propagate_ee(PT_MULTI_PARAMS)
 }
 
 template <int RT, bool EE, int TX, int TY, bool TM, bool XL = false>
-__global__ __launch_bounds__(PT_UP_THREADS) void km_propagate_up(PT_MULTI_PARAMS)
+__global__ __launch_bounds__(PT_UP_THREADS) __attribute__((amdgpu_waves_per_eu(TM ? 4 : 1))) void
+km_propagate_up(PT_MULTI_PARAMS)
 {
     MultiTile t = multi_tile(prefix, tiles_x, njobs, xcd);
     const PtArgs &A = PT_JOB(t.j);
