@@ -427,7 +427,7 @@ int pt_pde_count(pt_ctx *ctx, uint64_t *out);
  * in the refresh and / (4 (2r+1)^2) in propagate (at least one).  Same
  * results. */
 #define PT_TUNE_TAP_PARALLEL_EVALS 6
-/* PT_TUNE_EXEMPLAR_WINDOW (default 400000): on levels of at least this
+/* PT_TUNE_EXEMPLAR_WINDOW (default 200000): on levels of at least this
  * many pixels the per-pixel residual refresh stages an exemplar window around
  * each tile's offsets in LDS and reads the taps of patches inside it from
  * there (smaller levels measured slower with it); 0 reads every tap from

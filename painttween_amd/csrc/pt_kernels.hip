@@ -772,7 +772,7 @@ __device__ __forceinline__ Tile stage_tile(const PtArgs &A, int bx, int by, floa
  * the pixel is skipped.  Pixels that need the refresh are compacted onto the
  * first lanes of the workgroup so that whole waves skip.
  */
-#define PT_XW_RES 32 /* exemplar window side of the XL residual refresh */
+#define PT_XW_RES 24 /* exemplar window side of the XL residual refresh */
 
 template <int RT, bool XL> struct ResLds {
     float4 scol[TileSize<RT>::N], scan[TileSize<RT>::N];

@@ -156,7 +156,7 @@ struct pt_ctx {
     bool incremental_residual = true;
     int64_t tap_max_pixels = 128 * 128; /* tap-parallel shapes up to this level size */
     int tap_evals = 1024;               /* tap-parallel: tap evaluations per workgroup */
-    int64_t ex_window_min_pixels = 400000; /* XL refresh on levels of at least this size (0: never) */
+    int64_t ex_window_min_pixels = 200000; /* XL refresh on levels of at least this size (0: never) */
     bool xcd_tiles = true; /* contiguous tile ranges per XCD in single-job launches */
     int prop_window_jump = 8; /* XL propagate for jumps up to this (0: never) ... */
     int64_t prop_window_min_pixels = 200000; /* ... on levels of at least this size (0: never) */
